@@ -1,0 +1,175 @@
+/*
+ * libwoodem_hip -- MI355X (gfx950) implementation of the Woo DEM timestep.
+ *
+ * C ABI: plain pointers and sizes, no C++ or torch types.  Every entry point
+ * returns WDM_OK (0) or a negative WDM_E_* code; wdm_last_error() gives the
+ * message.  No exception or abort crosses the boundary.  Calls on one context
+ * are not re-entrant; separate contexts may be used from separate threads.
+ *
+ * Each entry point names the reference interface it replaces.  Paths are
+ * relative to the Woo source tree (Azeko2xo/woodem).
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes (the reference throws std::runtime_error / logic_error, see
+        pkg/dem/IdealElPl.cpp:76-80, InsertionSortCollider.cpp:945-949, core/Cell.cpp:96) ---- */
+#define WDM_OK 0
+#define WDM_E_ARG -1          /* bad argument / unknown scene */
+#define WDM_E_HIP -2          /* HIP runtime failure */
+#define WDM_E_OOM -3          /* device allocation failed or HBM budget exceeded */
+#define WDM_E_NAN -4          /* "NaN force in contact" (IdealElPl.cpp:76-80) */
+#define WDM_E_OVERSIZE -5     /* "Particle larger than half of the cell size" (InsertionSortCollider.cpp:945-949) */
+#define WDM_E_DEGENERATE -6   /* "Cell is degenerate (zero volume)" (core/Cell.cpp:96) */
+#define WDM_E_UNSUPPORTED -7  /* input outside the implemented path (clumps, aspherical free nodes, sheared cells, adhesive Hertz) */
+#define WDM_E_STATE -8        /* call order violated (e.g. step before upload) */
+
+/* shape codes: Sphere (pkg/dem/Sphere.hpp), Wall (pkg/dem/Wall.hpp), Facet (pkg/dem/Facet.hpp) */
+#define WDM_SHAPE_NONE 0
+#define WDM_SHAPE_SPHERE 1
+#define WDM_SHAPE_WALL 2
+#define WDM_SHAPE_FACET 3
+
+/* contact laws: Cp2_FrictMat_FrictPhys + Law2_L6Geom_FrictPhys_IdealElPl (pkg/dem/FrictMat.cpp:13-44,
+   pkg/dem/IdealElPl.cpp:12-122); Cp2_FrictMat_HertzPhys + Law2_L6Geom_HertzPhys_DMT (pkg/dem/Hertz.cpp:9-182) */
+#define WDM_LAW_IDEALELPL 0
+#define WDM_LAW_HERTZ_DMT 1
+
+/* arithmetic of the contact kernel: FP64 throughout, or L6Geom/Cp2/Law2 in FP32 on FP64 state */
+#define WDM_PREC_FP64 0
+#define WDM_PREC_MIXED 1
+
+/* DemData.flags bits (pkg/dem/Particle.hpp:177-182) */
+#define WDM_DOF_X 1u
+#define WDM_DOF_Y 2u
+#define WDM_DOF_Z 4u
+#define WDM_DOF_RX 8u
+#define WDM_DOF_RY 16u
+#define WDM_DOF_RZ 32u
+#define WDM_CLUMP_CLUMPED 64u
+#define WDM_CLUMP_CLUMP 128u
+#define WDM_ENERGY_SKIP 256u
+#define WDM_GRAVITY_SKIP 512u
+#define WDM_DAMPING_SKIP 2048u
+
+/* energy slots of EnergyTracker (core/EnergyTracker.hpp:61-79), fixed order */
+#define WDM_E_KINETIC 0
+#define WDM_E_ELAST 1
+#define WDM_E_PLAST 2
+#define WDM_E_GRAV 3
+#define WDM_E_NONVISCDAMP 4
+#define WDM_E_VISCN 5
+#define WDM_E_VISCT 6
+#define WDM_E_BROKEN 7
+#define WDM_E_COUNT 8
+
+/* Engine attributes of the timestep path; layout: doubles first, then 32-bit ints.
+   Sources: Leapfrog.damping (pkg/dem/Leapfrog.hpp:50), DemField.gravity/loneMask
+   (pkg/dem/Particle.hpp:316-319), InsertionSortCollider.verletDist
+   (pkg/dem/InsertionSortCollider.hpp:262), Bo1_Sphere_Aabb/Cg2_Sphere_Sphere_L6Geom
+   distFactor (pkg/dem/Sphere.hpp), Cp2_FrictMat_HertzPhys.{poisson,en,gamma,alpha}
+   (pkg/dem/Hertz.hpp:27-32), Law2_L6Geom_FrictPhys_IdealElPl.{noSlip,noBreak,noFrict}
+   (pkg/dem/IdealElPl.hpp:10-26), Law2_L6Geom_HertzPhys_DMT.noAttraction (Hertz.hpp:52),
+   Scene.trackEnergy, Cg2_Any_Any_L6Geom__Base.trsfRenorm (pkg/dem/L6Geom.hpp:36),
+   ContactLoop.dist00 (pkg/dem/ContactLoop.hpp:118). */
+typedef struct wdm_params {
+    double dt;
+    double damping;
+    double gravity[3];
+    double verletDist;  /* < 0: |verletDist| x minimum sphere radius */
+    double distFactor;
+    double poisson;
+    double en;
+    double gamma;       /* must be 0 (adhesion is outside the implemented path) */
+    double alpha;       /* must be 0 */
+    int32_t law;        /* WDM_LAW_* */
+    int32_t noSlip, noBreak, noFrict;
+    int32_t noAttraction;
+    int32_t trackEnergy;
+    int32_t trsfRenorm;
+    int32_t dist00;
+    uint32_t loneMask;
+    int32_t precision;  /* WDM_PREC_* */
+} wdm_params;
+
+typedef struct wdm_ctx wdm_ctx;
+
+/* Context: owns device buffers and one HIP stream on `device`.  hbm_budget_bytes
+   bounds what scenes may allocate (0 = no bound). */
+int wdm_create(wdm_ctx** out, int device, size_t hbm_budget_bytes);
+void wdm_destroy(wdm_ctx* ctx);
+const char* wdm_last_error(wdm_ctx* ctx);
+
+/* A scene slot: Scene + DemField (core/Scene.hpp:27, pkg/dem/Particle.hpp:278).
+   Uninodal particles (Sphere, Wall) must use node index == particle index; Facet
+   particles reference three nodes with index >= n_par. */
+int wdm_scene_alloc(wdm_ctx* ctx, int32_t n_par, int32_t n_node, int32_t n_mat, int32_t* scene_id);
+int wdm_scene_free(wdm_ctx* ctx, int32_t scene);
+/* bytes of HBM a scene of this size needs with `contacts_per_particle` potential
+   contacts (used by admission control) */
+size_t wdm_scene_footprint(int32_t n_par, int32_t n_node, double contacts_per_particle);
+
+/* Node + DemData arrays (core/Field.hpp:93-99, pkg/dem/Particle.hpp:249-262); SoA,
+   caller-owned host arrays copied synchronously.  force3/torque3 may be NULL (zero). */
+int wdm_upload_nodes(wdm_ctx* ctx, int32_t scene, int32_t n, const double* pos3, const double* ori4,
+                     const double* vel3, const double* angVel3, const double* mass, const double* inertia3,
+                     const uint32_t* flags, const double* force3, const double* torque3);
+/* Particle + Shape (pkg/dem/Particle.hpp:26-130): shape code, nodes (3 per particle, unused = -1),
+   Sphere.radius or Facet.halfThick, Wall.axis/sense, Particle.mask, material index */
+int wdm_upload_particles(wdm_ctx* ctx, int32_t scene, int32_t n, const int32_t* shape, const int32_t* nodes3,
+                         const double* radius, const int32_t* axis, const int32_t* sense,
+                         const uint32_t* mask, const int32_t* mat);
+/* FrictMat table (pkg/dem/FrictMat.hpp:18-28) */
+int wdm_upload_materials(wdm_ctx* ctx, int32_t scene, int32_t n, const double* density, const double* young,
+                         const double* ktDivKn, const double* tanPhi);
+/* Cell (core/Cell.hpp:116-160): hSize, gradV, nextGradV row-major 3x3 */
+int wdm_set_cell(wdm_ctx* ctx, int32_t scene, int32_t periodic, const double* hSize9, const double* gradV9,
+                 const double* nextGradV9, int32_t homoDeform);
+int wdm_set_params(wdm_ctx* ctx, int32_t scene, const wdm_params* params);
+
+/* Engines.  One call = one Engine::run() of the reference:
+   Leapfrog::run (pkg/dem/Leapfrog.cpp:103-255),
+   InsertionSortCollider::run (pkg/dem/InsertionSortCollider.cpp:445-646),
+   ContactLoop::run (pkg/dem/ContactLoop.cpp:93-259).
+   wdm_step_begin/end are the prologue/epilogue of Scene::doOneStep
+   (core/Scene.cpp:371-389); wdm_step runs whole steps for one or more scenes. */
+int wdm_step_begin(wdm_ctx* ctx, int32_t scene);
+int wdm_leapfrog_run(wdm_ctx* ctx, int32_t scene);
+int wdm_collider_run(wdm_ctx* ctx, int32_t scene);
+int wdm_contact_loop_run(wdm_ctx* ctx, int32_t scene);
+int wdm_step_end(wdm_ctx* ctx, int32_t scene);
+int wdm_step(wdm_ctx* ctx, const int32_t* scenes, int32_t n_scenes, int32_t n_steps);
+
+/* Readback (synchronous). */
+int wdm_download_nodes(wdm_ctx* ctx, int32_t scene, double* pos3, double* ori4, double* vel3, double* angVel3,
+                       double* force3, double* torque3);
+/* number of contacts currently held (real + potential) */
+int wdm_num_contacts(wdm_ctx* ctx, int32_t scene, int64_t* n);
+/* contacts sorted by (min id, max id); ids in functor order (Wall/Facet first, else lower id);
+   any output pointer may be NULL */
+int wdm_get_contacts(wdm_ctx* ctx, int32_t scene, int64_t cap, int32_t* idA, int32_t* idB, int32_t* cellDist3,
+                     uint8_t* isReal, double* forceLocal3, double* contPt3, double* trsf9, double* uN, int64_t* n_out);
+/* EnergyTracker values and whether each slot exists */
+int wdm_get_energy(wdm_ctx* ctx, int32_t scene, double* values, int32_t* created);
+/* Scene.step, InsertionSortCollider.nFullRuns, resolved verletDist */
+int wdm_get_stats(wdm_ctx* ctx, int32_t scene, int64_t* step, int64_t* n_full_runs, double* verlet_dist);
+int wdm_get_cell(wdm_ctx* ctx, int32_t scene, double* hSize9, double* gradV9);
+int wdm_sync(wdm_ctx* ctx);
+
+/* Per-kernel timing of the last wdm_step call, measured with HIP events on the
+   context's stream: fills up to cap (name, total ms, launches) entries. */
+int wdm_kernel_times(wdm_ctx* ctx, int32_t cap, char* names_64, double* ms, int64_t* launches, int32_t* n_out);
+int wdm_enable_timing(wdm_ctx* ctx, int32_t on);
+
+/* Scene generation helper (no counterpart in the timestep path): pushes overlapping
+   spheres apart with `iters` Jacobi relaxation sweeps on the device, deterministic. */
+int wdm_relax_overlaps(wdm_ctx* ctx, int32_t scene, int32_t iters);
+
+#ifdef __cplusplus
+}
+#endif

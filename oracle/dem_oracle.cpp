@@ -1,0 +1,1189 @@
+/*
+ * CPU oracle for the Woo DEM timestep -- TEST INFRASTRUCTURE ONLY (see dem_oracle.h).
+ *
+ * Restatement of the reference's CPU arithmetic, operation by operation.  Built
+ * with -ffp-contract=off so that no a*b+c is fused.  Conventions that the
+ * reference leaves to its Eigen build (and that this restatement fixes):
+ *   - 3-term sums (dot products, matrix-vector rows) are evaluated left to right,
+ *     (a0*b0 + a1*b1) + a2*b2;
+ *   - pow(x,2) is x*x, pow(sqrt(x),3) is (s*s)*s, pow(sqrt(x),5) is (((s*s)*s)*s)*s;
+ *   - the contact-to-global rotation uses the L6Geom.trsf matrix directly
+ *     (the reference goes through Quaternionr(trsf), equal to a few ulp);
+ *   - contacts are kept in canonical order: Wall/Facet first (functor order,
+ *     pkg/dem/Wall.hpp:54-55, Facet.hpp:79-80), otherwise the lower particle id
+ *     first (the reference's order comes from insertion-sort history,
+ *     pkg/dem/InsertionSortCollider.cpp:87,190);
+ *   - forces are applied deterministically (ContactLoop.deterministic,
+ *     pkg/dem/ContactLoop.cpp:250-257), contacts visited in (min id, max id) order,
+ *     so each particle sums its contacts in ascending order of the other id.
+ *   - energies are summed sequentially in node order / contact order.
+ * The "parity unpinned" items are listed in the module docstring of
+ * tests/oracle_lib.py and in DESIGN.md.
+ */
+#include "dem_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+const double INF = std::numeric_limits<double>::infinity();
+const double PI = 3.14159265358979323846;
+
+/* ---------- small fixed-size algebra (left-to-right evaluation) ---------- */
+struct V3 { double x[3]; double& operator[](int i) { return x[i]; } const double& operator[](int i) const { return x[i]; } };
+struct M3 { double m[3][3]; };
+struct Qt { double w, x, y, z; };
+
+inline V3 v3(double a, double b, double c) { V3 r; r[0] = a; r[1] = b; r[2] = c; return r; }
+inline V3 vzero() { return v3(0, 0, 0); }
+inline V3 vadd(const V3& a, const V3& b) { return v3(a[0] + b[0], a[1] + b[1], a[2] + b[2]); }
+inline V3 vsub(const V3& a, const V3& b) { return v3(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+inline V3 vmul(const V3& a, double s) { return v3(a[0] * s, a[1] * s, a[2] * s); }
+inline V3 vdiv(const V3& a, double s) { return v3(a[0] / s, a[1] / s, a[2] / s); }
+inline double vdot(const V3& a, const V3& b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+inline double vsqn(const V3& a) { return vdot(a, a); }
+inline double vnorm(const V3& a) { return std::sqrt(vsqn(a)); }
+inline V3 vcross(const V3& a, const V3& b) {
+    return v3(a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]);
+}
+inline bool viszero(const V3& a) { return a[0] == 0 && a[1] == 0 && a[2] == 0; }
+/* Eigen normalize(): divide by sqrt(squaredNorm) when positive */
+inline V3 vnormalized(const V3& a) { double z = vsqn(a); if (z > 0) return vdiv(a, std::sqrt(z)); return a; }
+inline V3 vabs(const V3& a) { return v3(std::fabs(a[0]), std::fabs(a[1]), std::fabs(a[2])); }
+
+inline M3 mzero() { M3 r; for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) r.m[i][j] = 0; return r; }
+inline M3 meye() { M3 r = mzero(); r.m[0][0] = r.m[1][1] = r.m[2][2] = 1; return r; }
+inline V3 mcol(const M3& a, int j) { return v3(a.m[0][j], a.m[1][j], a.m[2][j]); }
+inline void msetcol(M3& a, int j, const V3& c) { a.m[0][j] = c[0]; a.m[1][j] = c[1]; a.m[2][j] = c[2]; }
+inline V3 mvmul(const M3& a, const V3& v) {
+    return v3((a.m[0][0] * v[0] + a.m[0][1] * v[1]) + a.m[0][2] * v[2],
+              (a.m[1][0] * v[0] + a.m[1][1] * v[1]) + a.m[1][2] * v[2],
+              (a.m[2][0] * v[0] + a.m[2][1] * v[1]) + a.m[2][2] * v[2]);
+}
+/* transpose(a) * v == column dots */
+inline V3 mtvmul(const M3& a, const V3& v) { return v3(vdot(mcol(a, 0), v), vdot(mcol(a, 1), v), vdot(mcol(a, 2), v)); }
+inline M3 mmul(const M3& a, const M3& b) {
+    M3 r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r.m[i][j] = (a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j]) + a.m[i][2] * b.m[2][j];
+    return r;
+}
+inline M3 madd(const M3& a, const M3& b) { M3 r; for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) r.m[i][j] = a.m[i][j] + b.m[i][j]; return r; }
+inline M3 msub(const M3& a, const M3& b) { M3 r; for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) r.m[i][j] = a.m[i][j] - b.m[i][j]; return r; }
+inline M3 mscale(const M3& a, double s) { M3 r; for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) r.m[i][j] = a.m[i][j] * s; return r; }
+inline M3 mdivs(const M3& a, double s) { M3 r; for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) r.m[i][j] = a.m[i][j] / s; return r; }
+inline M3 mtrans(const M3& a) { M3 r; for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) r.m[i][j] = a.m[j][i]; return r; }
+inline bool mequal(const M3& a, const M3& b) { for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) if (a.m[i][j] != b.m[i][j]) return false; return true; }
+inline double mcof(const M3& a, int i, int j) {
+    int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+    return a.m[i1][j1] * a.m[i2][j2] - a.m[i1][j2] * a.m[i2][j1];
+}
+/* 3x3 inverse by cofactors times 1/det (Eigen 3 compute_inverse_size3 algorithm) */
+inline M3 minverse(const M3& a) {
+    double c0 = mcof(a, 0, 0), c1 = mcof(a, 1, 0), c2 = mcof(a, 2, 0);
+    double det = (c0 * a.m[0][0] + c1 * a.m[1][0]) + c2 * a.m[2][0];
+    double inv = 1.0 / det;
+    M3 r;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r.m[i][j] = mcof(a, j, i) * inv;
+    return r;
+}
+inline double mdet(const M3& a) {
+    double c0 = mcof(a, 0, 0), c1 = mcof(a, 1, 0), c2 = mcof(a, 2, 0);
+    return (c0 * a.m[0][0] + c1 * a.m[1][0]) + c2 * a.m[2][0];
+}
+/* lib/base/Math.hpp:222-228 */
+inline V3 leviCivita(const M3& m) { return v3(m.m[1][2] - m.m[2][1], m.m[2][0] - m.m[0][2], m.m[0][1] - m.m[1][0]); }
+
+inline Qt qmul(const Qt& a, const Qt& b) {
+    Qt r;
+    r.w = ((a.w * b.w - a.x * b.x) - a.y * b.y) - a.z * b.z;
+    r.x = ((a.w * b.x + a.x * b.w) + a.y * b.z) - a.z * b.y;
+    r.y = ((a.w * b.y + a.y * b.w) + a.z * b.x) - a.x * b.z;
+    r.z = ((a.w * b.z + a.z * b.w) + a.x * b.y) - a.y * b.x;
+    return r;
+}
+inline Qt qnormalized(const Qt& q) {
+    double n = std::sqrt(((q.w * q.w + q.x * q.x) + q.y * q.y) + q.z * q.z);
+    Qt r; r.w = q.w / n; r.x = q.x / n; r.y = q.y / n; r.z = q.z / n; return r;
+}
+
+/* Mathr::Sign, lib/base/Math.hpp:162 */
+inline double Sign(double f) { if (f < 0) return -1; if (f > 0) return 1; return 0; }
+
+/* ---------- data model ---------- */
+/* DemData flags, pkg/dem/Particle.hpp:177-182 */
+enum : uint32_t {
+    DOF_X = 1, DOF_Y = 2, DOF_Z = 4, DOF_RX = 8, DOF_RY = 16, DOF_RZ = 32,
+    CLUMP_CLUMPED = 64, CLUMP_CLUMP = 128, ENERGY_SKIP = 256, GRAVITY_SKIP = 512, TRACER_SKIP = 1024, DAMPING_SKIP = 2048,
+    DOF_ALL = 63, DOF_RXRYRZ = 56
+};
+inline uint32_t axisDOF(int axis, bool rot) { return 1u << (axis + (rot ? 3 : 0)); }
+
+enum { HOMO_NONE = 0, HOMO_POS = 1, HOMO_VEL = 2, HOMO_VEL_2ND = 3, HOMO_GRADV2 = 4 };
+
+struct Mat { double density, young, ktDivKn, tanPhi; };
+
+/* Contact + L6Geom + FrictPhys/HertzPhys, pkg/dem/Contact.hpp:47-115, L6Geom.hpp:10-26, Hertz.hpp:5-21 */
+struct Contact {
+    int idA, idB;
+    int cd[3];
+    long stepCreated;
+    double minDist00Sq;
+    bool geom, phys;
+    // L6Geom
+    M3 trsf; V3 contPt; double uN; double lens[2]; double contA; V3 vel, angVel;
+    // FrictPhys
+    double kn, kt, tanPhi; V3 force, torque;
+    // HertzPhys
+    double K, R, kt0, alpha_sqrtMK, gamma, alpha;
+    bool isReal() const { return geom && phys; }
+};
+
+/* InsertionSortCollider::Bounds, pkg/dem/InsertionSortCollider.hpp:120-141 */
+struct Bounds {
+    double coord; int id; int period; unsigned hasBB : 1; unsigned isMin : 1; unsigned isInf : 1;
+    bool operator<(const Bounds& b) const { if (id == b.id && coord == b.coord) return isMin; return coord < b.coord; }
+    bool operator>(const Bounds& b) const { if (id == b.id && coord == b.coord) return !isMin; return coord > b.coord; }
+};
+/* VecBounds, pkg/dem/InsertionSortCollider.hpp:145-168 */
+struct VecBounds {
+    int axis = -1; std::vector<Bounds> vec; double cellDim = 0; long size = 0; long loIdx = 0;
+    Bounds& operator[](long i) { return vec[i]; }
+    long norm(long i) const { if (i < 0) i += size; return i % size; }
+};
+
+struct Aabb { V3 mn, mx; V3 lastPos[3]; int nNodes = 0; double maxD2 = 0; bool has = false; };
+
+struct Oracle {
+    std::string err;
+    orc_params P{};
+    // nodes
+    int nn = 0;
+    std::vector<V3> pos, vel, angVel, force, torque, inertia;
+    std::vector<Qt> ori;
+    std::vector<double> mass;
+    std::vector<uint32_t> flags;
+    // particles
+    int np = 0;
+    std::vector<int> shape, nodeIx, axis, sense, matId;  // nodeIx: 3 per particle
+    std::vector<double> radius;
+    std::vector<uint32_t> mask;
+    std::vector<Mat> mats;
+    // cell
+    bool periodic = false;
+    M3 hSize = meye(), gradV = mzero(), nextGradV = mzero(), pprevHsize = meye(), trsf = meye();
+    V3 cellSize = v3(1, 1, 1), spinVec = vzero();
+    int homoDeform = HOMO_GRADV2;
+    // scene
+    long step = 0;
+    // energies
+    double E[ORC_E_N] = {0};
+    int Ecreated[ORC_E_N] = {0};
+    // collider
+    bool contactsDirty = true;
+    bool colPeriodic = false;
+    double verletDistActual = NAN;
+    bool verletResolved = false;
+    long nFullRuns = 0;
+    VecBounds BB[3];
+    std::vector<double> minima, maxima;
+    std::vector<Aabb> aabb;
+    // contacts: key (min<<32|max) -> contact
+    std::map<uint64_t, Contact> con;
+    std::vector<uint64_t> pending;
+    std::vector<Contact> makeLater;
+    std::vector<uint64_t> removeLater;
+    // deterministic force application: per-particle list of (other id, contact*) sorted by other id
+    bool adjDirty = true;
+    std::vector<Contact*> lin;
+    std::vector<long> adjOff;
+    std::vector<Contact*> adjC;
+
+    int nodeOf(int p, int k = 0) const { return nodeIx[3 * p + k]; }
+    int numNodes(int p) const { return shape[p] == ORC_SHAPE_FACET ? 3 : (shape[p] == ORC_SHAPE_NONE ? 0 : 1); }
+    static uint64_t key(int a, int b) { int lo = a < b ? a : b, hi = a < b ? b : a; return ((uint64_t)(uint32_t)lo << 32) | (uint32_t)hi; }
+
+    /* ---------- energy tracker, core/EnergyTracker.cpp:10-17 ---------- */
+    void eAdd(int slot, double val, bool zeroDontCreate) {
+        if (!Ecreated[slot]) { if (val == 0. && zeroDontCreate) return; Ecreated[slot] = 1; }
+        if (std::isnan(val)) return;
+        E[slot] += val;
+    }
+
+    /* ---------- Cell, core/Cell.cpp:28-120 ---------- */
+    void cellIntegrateAndUpdate(double dt) {
+        M3 trsfInc = mscale(gradV, dt);
+        trsf = madd(trsf, mmul(trsfInc, trsf));
+        M3 prevHsize = hSize;
+        if (homoDeform == HOMO_GRADV2) {
+            M3 gh = mdivs(mscale(gradV, dt), 2.);
+            hSize = mmul(mmul(minverse(msub(meye(), gh)), madd(meye(), gh)), hSize);
+        } else hSize = madd(hSize, mmul(trsfInc, hSize));
+        pprevHsize = mscale(madd(prevHsize, hSize), .5);
+        if (mdet(hSize) == 0) throw std::runtime_error("Cell is degenerate (zero volume).");
+        for (int i = 0; i < 3; i++) cellSize[i] = vnorm(mcol(hSize, i));
+        if (hSize.m[0][1] != 0 || hSize.m[0][2] != 0 || hSize.m[1][0] != 0 || hSize.m[1][2] != 0 || hSize.m[2][0] != 0 || hSize.m[2][1] != 0)
+            throw std::runtime_error("sheared periodic cells are outside the restated path");
+    }
+    void cellSetNextGradV() {  // core/Cell.cpp:53-58
+        gradV = nextGradV;
+        M3 W = mscale(msub(gradV, mtrans(gradV)), .5);
+        spinVec = vmul(leviCivita(W), .5);
+    }
+    V3 intrShiftPos(const int cd[3]) const { return mvmul(hSize, v3((double)cd[0], (double)cd[1], (double)cd[2])); }
+    V3 intrShiftVel(const int cd[3]) const {  // core/Cell.cpp:28-34
+        V3 c = v3((double)cd[0], (double)cd[1], (double)cd[2]);
+        switch (homoDeform) {
+            case HOMO_VEL: case HOMO_VEL_2ND: return mvmul(mmul(gradV, hSize), c);
+            case HOMO_GRADV2: return mvmul(mmul(gradV, pprevHsize), c);
+            default: return vzero();
+        }
+    }
+    V3 pprevFluctVel(const V3& p, const V3& v, double dt) const {  // core/Cell.cpp:36-50
+        switch (homoDeform) {
+            case HOMO_NONE: case HOMO_POS: return v;
+            case HOMO_VEL: case HOMO_VEL_2ND: return vsub(v, mvmul(gradV, p));
+            case HOMO_GRADV2: return vsub(v, mvmul(gradV, vsub(p, vmul(v, dt / 2))));
+        }
+        return v;
+    }
+    V3 pprevFluctAngVel(const V3& w) const { return homoDeform == HOMO_GRADV2 ? vsub(w, spinVec) : w; }
+
+    /* ---------- Leapfrog::run, pkg/dem/Leapfrog.cpp:103-255 ---------- */
+    void leapfrog() {
+        const double dt = P.dt;
+        const double damping = P.damping;
+        int hd = periodic ? homoDeform : -1;
+        M3 ImLL4hInv = mzero(), IpLL4h = mzero(), LmL = mzero(); V3 deltaSpinVec = vzero();
+        M3 dGradV = msub(nextGradV, gradV), midGradV = mscale(madd(gradV, nextGradV), .5);
+        if (hd == HOMO_GRADV2) {
+            const M3& pprevL = gradV; const M3& nnextL = nextGradV;
+            ImLL4hInv = minverse(msub(meye(), mdivs(mscale(madd(nnextL, pprevL), dt), 4.)));
+            IpLL4h = madd(meye(), mdivs(mscale(madd(nnextL, pprevL), dt), 4.));
+            LmL = msub(nnextL, pprevL);
+            deltaSpinVec = vadd(vmul(leviCivita(mscale(msub(pprevL, mtrans(pprevL)), .5)), -.5),
+                                vmul(leviCivita(mscale(msub(nnextL, mtrans(nnextL)), .5)), .5));
+        }
+        const bool reallyTrackEnergy = P.trackEnergy && (!periodic || step > 0 || mequal(gradV, meye()));
+        V3 g = v3(P.gravity[0], P.gravity[1], P.gravity[2]);
+        bool hasGravity = !viszero(g);
+        std::vector<double> eDamp(nn, 0.), eGrav(nn, 0.), eKin(nn, 0.);
+        std::vector<char> hDamp(nn, 0), hGrav(nn, 0), hKin(nn, 0);
+        #pragma omp parallel for schedule(guided)
+        for (int i = 0; i < nn; i++) {
+            uint32_t fl = flags[i];
+            if (fl & CLUMP_CLUMPED) continue;
+            if (fl & CLUMP_CLUMP) continue;  // clump masters are outside the restated path (rejected at upload)
+            bool damp = (damping != 0. && !(fl & DAMPING_SKIP));
+            V3 f = force[i], t = torque[i];
+            bool blockedNone = (fl & DOF_ALL) == 0, blockedAll = (fl & DOF_ALL) == DOF_ALL;
+            if (reallyTrackEnergy) {
+                if (damp && !(fl & ENERGY_SKIP)) {  // :52-63
+                    double d = vdot(vabs(vel[i]), vabs(f)) * damping * dt + vdot(vabs(angVel[i]), vabs(t)) * damping * dt;
+                    eDamp[i] = d; hDamp[i] = 1;
+                }
+                if (hasGravity && !(fl & GRAVITY_SKIP)) {  // :66-73
+                    double gr = 0;
+                    if (blockedNone) gr = -vdot(g, vel[i]) * mass[i] * dt;
+                    else { for (int ax = 0; ax < 3; ax++) if (!(fl & axisDOF(ax, false))) gr -= g[ax] * vel[i][ax] * mass[i] * dt; }
+                    eGrav[i] = gr; hGrav[i] = 1;
+                }
+            }
+            V3 pprevFV = vzero(), pprevFAV = vzero();
+            V3 linAccel = vzero(), angAccel = vzero();
+            if (!blockedAll) {
+                // computeAccel, :36-41
+                if (blockedNone) linAccel = vdiv(f, mass[i]);
+                else { linAccel = vzero(); for (int k = 0; k < 3; k++) if (!(fl & axisDOF(k, false))) linAccel[k] += f[k] / mass[i]; }
+                if (periodic) { pprevFV = pprevFluctVel(pos[i], vel[i], dt); pprevFAV = pprevFluctAngVel(angVel[i]); }
+                else { pprevFV = vel[i]; pprevFAV = angVel[i]; }
+                if (damp) for (int k = 0; k < 3; k++) linAccel[k] *= 1 - damping * Sign(f[k] * (pprevFV[k] + 0.5 * dt * linAccel[k]));  // :32-34
+                if (hd == HOMO_GRADV2) vel[i] = mvmul(ImLL4hInv, vadd(vadd(mvmul(LmL, pos[i]), mvmul(IpLL4h, vel[i])), vmul(linAccel, dt)));
+                else vel[i] = vadd(vel[i], vmul(linAccel, dt));
+                if (!viszero(inertia[i])) {
+                    bool aspherical = !(inertia[i][0] == inertia[i][1] && inertia[i][1] == inertia[i][2]);
+                    if (aspherical && (fl & DOF_RXRYRZ) != DOF_RXRYRZ) { /* rejected at upload */ }
+                    // computeAngAccel, :42-47
+                    if (blockedNone) angAccel = v3(t[0] / inertia[i][0], t[1] / inertia[i][1], t[2] / inertia[i][2]);
+                    else { angAccel = vzero(); for (int k = 0; k < 3; k++) if (!(fl & axisDOF(k, true))) angAccel[k] += t[k] / inertia[i][k]; }
+                    if (damp) for (int k = 0; k < 3; k++) angAccel[k] *= 1 - damping * Sign(t[k] * (pprevFAV[k] + 0.5 * dt * angAccel[k]));
+                    angVel[i] = vadd(angVel[i], vmul(angAccel, dt));
+                    if (hd == HOMO_GRADV2) angVel[i] = vsub(angVel[i], deltaSpinVec);
+                }
+            } else {
+                if (hd == HOMO_GRADV2) vel[i] = mvmul(ImLL4hInv, vadd(mvmul(LmL, pos[i]), mvmul(IpLL4h, vel[i])));
+            }
+            if (hd >= 0 && hd != HOMO_GRADV2) {  // applyPeriodicCorrections, :258-272
+                if (hd == HOMO_VEL || hd == HOMO_VEL_2ND) {
+                    if (hd == HOMO_VEL_2ND) vel[i] = vadd(vel[i], vmul(mvmul(midGradV, vsub(vel[i], vmul(linAccel, dt / 2.))), dt));
+                    vel[i] = vadd(vel[i], mvmul(dGradV, pos[i]));
+                } else if (hd == HOMO_POS) pos[i] = vadd(pos[i], vmul(mvmul(nextGradV, pos[i]), dt));
+            }
+            if (reallyTrackEnergy && !(fl & ENERGY_SKIP)) {  // doKineticEnergy, :75-97
+                V3 cfv = vadd(pprevFV, vmul(linAccel, .5 * dt)), cfav = vadd(pprevFAV, vmul(angAccel, .5 * dt));
+                double Etrans = .5 * mass[i] * vsqn(cfv);
+                double Erot = 0.5 * vdot(cfav, v3(inertia[i][0] * cfav[0], inertia[i][1] * cfav[1], inertia[i][2] * cfav[2]));
+                double mxI = std::max(std::max(inertia[i][0], inertia[i][1]), inertia[i][2]);
+                if (std::isnan(Erot) || std::isinf(mxI)) Erot = 0;
+                eKin[i] = Etrans + Erot; hKin[i] = 1;
+            }
+            // leapfrogTranslate, :275-278
+            pos[i] = vadd(pos[i], vmul(vel[i], dt));
+            // leapfrogSphericalRotate, :280-288
+            V3 ax = angVel[i];
+            if (!viszero(ax)) {
+                double angle = vnorm(ax); ax = vdiv(ax, angle);
+                double ha = 0.5 * (angle * dt);
+                double s = std::sin(ha);
+                Qt q; q.w = std::cos(ha); q.x = s * ax[0]; q.y = s * ax[1]; q.z = s * ax[2];
+                ori[i] = qmul(q, ori[i]);
+            }
+            ori[i] = qnormalized(ori[i]);
+            // reset, :240-245
+            force[i] = (hasGravity && !(fl & GRAVITY_SKIP)) ? vmul(g, mass[i]) : vzero();
+            torque[i] = vzero();
+        }
+        if (reallyTrackEnergy) {
+            for (int i = 0; i < nn; i++) {
+                if (hDamp[i]) eAdd(ORC_E_NONVISCDAMP, eDamp[i], true);
+                if (hGrav[i]) eAdd(ORC_E_GRAV, eGrav[i], false);
+                if (hKin[i]) eAdd(ORC_E_KINETIC, eKin[i], false);
+            }
+        }
+    }
+
+    /* ---------- bounds: Bo1_Sphere_Aabb (Sphere.cpp:50-79), Bo1_Wall_Aabb (Wall.cpp:24-33), Bo1_Facet_Aabb (Facet.cpp:177-200) ---------- */
+    void computeBound(int p) {
+        Aabb& a = aabb[p];
+        a.has = true;
+        if (shape[p] == ORC_SHAPE_SPHERE) {
+            double hs = (P.distFactor > 0 ? P.distFactor : 1.) * radius[p];
+            const V3& x = pos[nodeOf(p)];
+            // non-sheared periodic cell: unshearPt(pos)==pos, shearAlignedExtents(h)==h
+            a.mn = v3(x[0] - hs, x[1] - hs, x[2] - hs);
+            a.mx = v3(x[0] + hs, x[1] + hs, x[2] + hs);
+        } else if (shape[p] == ORC_SHAPE_WALL) {
+            a.mn = v3(-INF, -INF, -INF); a.mx = v3(INF, INF, INF);
+            a.mn[axis[p]] = a.mx[axis[p]] = pos[nodeOf(p)][axis[p]];
+        } else if (shape[p] == ORC_SHAPE_FACET) {
+            double ht = radius[p];
+            const V3& x0 = pos[nodeOf(p, 0)];
+            a.mn = v3(x0[0] - ht, x0[1] - ht, x0[2] - ht); a.mx = v3(x0[0] + ht, x0[1] + ht, x0[2] + ht);
+            for (int k = 1; k < 3; k++) {
+                const V3& x = pos[nodeOf(p, k)];
+                for (int d = 0; d < 3; d++) { a.mn[d] = std::min(a.mn[d], x[d] - ht); a.mx[d] = std::max(a.mx[d], x[d] + ht); }
+            }
+        } else a.has = false;
+    }
+
+    /* InsertionSortCollider::updateBboxes_doFullRun, InsertionSortCollider.cpp:281-417 */
+    bool updateBboxes_doFullRun() {
+        if (!verletResolved) {  // :290-298
+            double vd = P.verletDist;
+            if (vd < 0) {
+                double minR = INF;
+                for (int p = 0; p < np; p++) if (shape[p] == ORC_SHAPE_SPHERE) minR = std::min(radius[p], minR);
+                vd = std::isinf(minR) ? 0 : std::fabs(vd) * minR;
+            }
+            verletDistActual = vd; verletResolved = true;
+        }
+        const double vd = verletDistActual;
+        bool recompute = false;
+        if (vd == 0) recompute = true;
+        else {
+            for (int p = 0; p < np; p++) {
+                if (shape[p] == ORC_SHAPE_NONE) continue;
+                const Aabb& a = aabb[p];
+                if (!a.has) { recompute = true; break; }
+                double d2 = 0;
+                for (int k = 0; k < a.nNodes; k++) d2 = std::max(d2, vsqn(vsub(a.lastPos[k], pos[nodeOf(p, k)])));
+                if (d2 > a.maxD2) { recompute = true; break; }  // moveDueToRot2==0: all bounds here set maxRot=-1
+            }
+        }
+        if (!recompute) return false;
+        for (int p = 0; p < np; p++) {
+            if (shape[p] == ORC_SHAPE_NONE) continue;
+            computeBound(p);
+            Aabb& a = aabb[p];
+            a.nNodes = numNodes(p);
+            for (int k = 0; k < a.nNodes; k++) a.lastPos[k] = pos[nodeOf(p, k)];
+            a.maxD2 = vd * vd;
+            if (vd > 0) {
+                for (int d = 0; d < 3; d++) { a.mx[d] += vd; a.mn[d] -= vd; }
+            }
+        }
+        return true;
+    }
+
+    /* Collider::mayCollide, pkg/dem/Collision.cpp:19-41 (clumps are outside the restated path) */
+    bool mayCollide(int a, int b) const {
+        if (a == b || shape[a] == ORC_SHAPE_NONE || shape[b] == ORC_SHAPE_NONE) return false;
+        if (!(mask[a] & mask[b])) return false;
+        if ((mask[a] & mask[b] & P.loneMask) != 0) return false;
+        for (int i = 0; i < numNodes(a); i++) for (int j = 0; j < numNodes(b); j++) if (nodeOf(a, i) == nodeOf(b, j)) return false;
+        return true;
+    }
+
+    /* InsertionSortCollider::spatialOverlap, :52-58 */
+    bool spatialOverlap(int id1, int id2) const {
+        return (minima[3 * id1 + 0] <= maxima[3 * id2 + 0]) && (maxima[3 * id1 + 0] >= minima[3 * id2 + 0]) &&
+               (minima[3 * id1 + 1] <= maxima[3 * id2 + 1]) && (maxima[3 * id1 + 1] >= minima[3 * id2 + 1]) &&
+               (minima[3 * id1 + 2] <= maxima[3 * id2 + 2]) && (maxima[3 * id1 + 2] >= minima[3 * id2 + 2]);
+    }
+    /* cellWrap / cellWrapRel, :650-660 */
+    static double cellWrap(double x, double x0, double x1, int& period) {
+        double xNorm = (x - x0) / (x1 - x0);
+        period = (int)std::floor(xNorm);
+        return x0 + (xNorm - period) * (x1 - x0);
+    }
+    static double cellWrapRel(double x, double x0, double x1) {
+        double xNorm = (x - x0) / (x1 - x0);
+        return (xNorm - std::floor(xNorm)) * (x1 - x0);
+    }
+    /* spatialOverlapPeri, :903-957 */
+    bool spatialOverlapPeri(int id1, int id2, int periods[3]) const {
+        for (int axis = 0; axis < 3; axis++) {
+            double dim = cellSize[axis];
+            double m1 = minima[3 * id1 + axis], m2 = minima[3 * id2 + axis];
+            if (std::isinf(m1) || std::isinf(m2)) { periods[axis] = 0; continue; }
+            double wMn = (cellWrapRel(m1, m2, m2 + dim) < cellWrapRel(m2, m1, m1 + dim)) ? m2 : m1;
+            int pmn1, pmx1, pmn2, pmx2;
+            double mn1 = cellWrap(minima[3 * id1 + axis], wMn, wMn + dim, pmn1), mx1 = cellWrap(maxima[3 * id1 + axis], wMn, wMn + dim, pmx1);
+            double mn2 = cellWrap(minima[3 * id2 + axis], wMn, wMn + dim, pmn2), mx2 = cellWrap(maxima[3 * id2 + axis], wMn, wMn + dim, pmx2);
+            if ((pmn1 != pmx1) || (pmn2 != pmx2)) throw std::runtime_error("Particle larger than half of the cell size encountered.");
+            periods[axis] = (int)(pmn1 - pmn2);
+            if (!(mn1 <= mx2 && mx1 >= mn2)) return false;
+        }
+        return true;
+    }
+    bool boundsOverlap(int a, int b) const { int per[3]; return periodic ? spatialOverlapPeri(a, b, per) : spatialOverlap(a, b); }
+
+    Contact* find(int a, int b) { auto it = con.find(key(a, b)); return it == con.end() ? nullptr : &it->second; }
+
+    /* makeContactLater, InsertionSortCollider.hpp:106-113, followed by the functor-order
+       swap of ContactLoop.cpp:145-152 / Contact::swapOrder (Contact.cpp:12-16), done here
+       once at creation so that contacts are stored in canonical order. */
+    void makeContactLater(int id1, int id2, const int cd[3]) {
+        Contact c{};
+        c.idA = id1; c.idB = id2;
+        for (int k = 0; k < 3; k++) c.cd[k] = cd[k];
+        c.stepCreated = step; c.minDist00Sq = -1; c.geom = c.phys = false;
+        bool swap;
+        int sa = shape[id1], sb = shape[id2];
+        if (sa == sb) swap = id1 > id2;
+        else swap = (sb == ORC_SHAPE_WALL || sb == ORC_SHAPE_FACET) && sa == ORC_SHAPE_SPHERE;
+        if (swap) { std::swap(c.idA, c.idB); for (int k = 0; k < 3; k++) c.cd[k] = -c.cd[k]; }
+        makeLater.push_back(c);
+    }
+    /* handleBoundInversion, :63-98 */
+    void handleBoundInversion(int id1, int id2, bool separating) {
+        bool overlap = separating ? false : spatialOverlap(id1, id2);
+        Contact* C = find(id1, id2);
+        bool hasCon = C != nullptr;
+        if (!overlap && !hasCon) return;
+        if (overlap && hasCon) return;
+        if (overlap) { if (!mayCollide(id1, id2)) return; int z[3] = {0, 0, 0}; makeContactLater(id1, id2, z); return; }
+        if (!C->isReal()) removeLater.push_back(key(id1, id2));
+    }
+    /* handleBoundInversionPeri, :836-874 */
+    void handleBoundInversionPeri(int id1, int id2, bool separating) {
+        int periods[3] = {0, 0, 0};
+        bool overlap = separating ? false : spatialOverlapPeri(id1, id2, periods);
+        Contact* C = find(id1, id2);
+        bool hasCon = C != nullptr;
+        if (!overlap && !hasCon) return;
+        if (overlap && hasCon) return;
+        if (overlap) { if (!mayCollide(id1, id2)) return; makeContactLater(id1, id2, periods); return; }
+        if (!C->isReal()) removeLater.push_back(key(id1, id2));
+    }
+    /* insertionSort_part (serial), :162-196 */
+    void insertionSort(VecBounds& v, bool doCollide) {
+        for (long i = 1; i < v.size; i++) {
+            if (!(v[i - 1] > v[i])) continue;
+            const Bounds viInit = v[i]; long j = i - 1;
+            while (j >= 0 && v[j] > viInit) {
+                v[j + 1] = v[j];
+                if (viInit.isMin != v[j].isMin && doCollide && viInit.hasBB && v[j].hasBB && viInit.id != v[j].id)
+                    handleBoundInversion(viInit.id, v[j].id, !viInit.isMin);
+                j--;
+            }
+            v[j + 1] = viInit;
+        }
+    }
+    /* insertionSortPeri_part (serial, partial=false), :718-791 */
+    void insertionSortPeri(VecBounds& v, bool doCollide) {
+        long& loIdx = v.loIdx;
+        for (long _i = 0; _i < v.size; _i++) {
+            const long i = v.norm(_i);
+            const long i_1 = v.norm(i - 1);
+            if (i == loIdx && v[i].coord < 0) { v[i].period -= 1; v[i].coord += v.cellDim; loIdx = v.norm(loIdx + 1); }
+            const double iCmpCoord = v[i].coord + (i == loIdx ? v.cellDim : 0);
+            if (v[i_1].coord <= iCmpCoord) continue;
+            long j = i_1; Bounds vi = v[i];
+            const bool viHasBB = vi.hasBB, viIsMin = vi.isMin, viIsInf = vi.isInf;
+            while (v[j].coord > vi.coord + (v.norm(j + 1) == loIdx ? v.cellDim : 0)) {
+                long j1 = v.norm(j + 1);
+                Bounds& vNew = v[j1];
+                vNew = v[j];
+                if (j == loIdx && vi.coord < 0) { vi.period -= 1; vi.coord += v.cellDim; loIdx = v.norm(loIdx + 1); }
+                else if (j1 == loIdx) { vNew.period += 1; vNew.coord -= v.cellDim; loIdx = v.norm(loIdx - 1); }
+                if (viIsMin != v[j].isMin && doCollide && viHasBB && v[j].hasBB) {
+                    if (vi.id != vNew.id) handleBoundInversionPeri(vi.id, vNew.id, (!viIsMin && !viIsInf && !v[j].isInf));
+                }
+                j = v.norm(j - 1);
+            }
+            v[v.norm(j + 1)] = vi;
+        }
+    }
+    /* ContactContainer::removePending with shouldBeRemoved, ContactContainer.hpp:78-93, InsertionSortCollider.hpp:209-214 */
+    void removePending() {
+        for (uint64_t k : pending) {
+            auto it = con.find(k);
+            if (it == con.end()) continue;
+            if (!boundsOverlap(it->second.idA, it->second.idB)) { con.erase(it); adjDirty = true; }
+        }
+        pending.clear();
+    }
+    /* makeRemoveContactLater_process, :20-48 */
+    void makeRemoveLaterProcess() {
+        for (uint64_t k : removeLater) { if (con.erase(k)) adjDirty = true; }
+        removeLater.clear();
+        for (const Contact& c : makeLater) {
+            uint64_t k = key(c.idA, c.idB);
+            if (con.count(k)) continue;
+            con.emplace(k, c); adjDirty = true;
+        }
+        makeLater.clear();
+    }
+
+    /* InsertionSortCollider::run, :445-646 */
+    void collider() {
+        bool fullRun = false;
+        if (contactsDirty) { fullRun = true; contactsDirty = false; }
+        if ((long)BB[0].size != 2L * np) fullRun = true;
+        if (periodic != colPeriodic) { for (int i = 0; i < 3; i++) { BB[i].vec.clear(); BB[i].size = 0; } colPeriodic = periodic; fullRun = true; }
+        if ((long)aabb.size() != np) aabb.assign(np, Aabb());
+        bool runBboxes = updateBboxes_doFullRun();
+        if (runBboxes) fullRun = true;
+        if (!fullRun) { removePending(); return; }
+        nFullRuns++;
+        long nPar = np;
+        bool doInitSort = false;
+        if (BB[0].size != 2 * nPar) {
+            if (2 * nPar < BB[0].size) { for (int i = 0; i < 3; i++) { BB[i].vec.clear(); BB[i].size = 0; } }
+            else if (2 * nPar - BB[0].size > 200 || BB[0].size == 0) doInitSort = true;
+            for (int i = 0; i < 3; i++) {
+                BB[i].axis = i;
+                for (long id = (long)BB[i].vec.size() / 2; id < nPar; id++) {
+                    Bounds b0{}; b0.coord = 0; b0.id = (int)id; b0.period = 0; b0.isMin = 1;
+                    Bounds b1{}; b1.coord = 0; b1.id = (int)id; b1.period = 0; b1.isMin = 0;
+                    BB[i].vec.push_back(b0); BB[i].vec.push_back(b1);
+                }
+                BB[i].size = (long)BB[i].vec.size();
+            }
+        }
+        minima.assign(3 * nPar, 0.); maxima.assign(3 * nPar, 0.);
+        if (periodic) for (int i = 0; i < 3; i++) BB[i].cellDim = cellSize[i];
+        // copy bounds, :516-548
+        for (long i = 0; i < 2 * nPar; i++) {
+            for (int j = 0; j < 3; j++) {
+                VecBounds& BBj = BB[j];
+                Bounds& b = BBj[i];
+                const int id = b.id;
+                const Aabb& a = aabb[id];
+                b.hasBB = a.has;
+                b.coord = (a.has ? (b.isMin ? a.mn[j] : a.mx[j]) : pos[nodeOf(id)][j]) - (periodic ? BBj.cellDim * b.period : 0.);
+                b.isInf = false;
+                if (periodic && std::isinf(b.coord)) { b.period = (b.coord < 0 ? 0 : 1); b.coord = 0.; b.isInf = true; }
+                if (doInitSort && periodic) { if (!b.isInf) { int per; b.coord = cellWrap(b.coord, 0, BBj.cellDim, per); b.period = per; } }
+            }
+        }
+        // copy minima/maxima, :552-560
+        for (int id = 0; id < nPar; id++) {
+            const Aabb& a = aabb[id];
+            if (a.has) { for (int d = 0; d < 3; d++) { minima[3 * id + d] = a.mn[d]; maxima[3 * id + d] = a.mx[d]; } }
+            else if (shape[id] != ORC_SHAPE_NONE) { for (int d = 0; d < 3; d++) minima[3 * id + d] = maxima[3 * id + d] = pos[nodeOf(id)][d]; }
+        }
+        removePending();
+        if (!doInitSort) {
+            if (!periodic) for (int i = 0; i < 3; i++) insertionSort(BB[i], true);
+            else for (int i = 0; i < 3; i++) insertionSortPeri(BB[i], true);
+        } else {
+            for (int i = 0; i < 3; i++) { BB[i].loIdx = 0; std::sort(BB[i].vec.begin(), BB[i].vec.end()); }
+            VecBounds& V = BB[0];  // sortAxis=0
+            if (!periodic) {
+                for (long i = 0; i < 2 * nPar; i++) {
+                    if (!(V[i].isMin && V[i].hasBB)) continue;
+                    const int iid = V[i].id;
+                    for (long j = i + 1; j < 2 * nPar && V[j].id != iid; j++) {
+                        if (!V[j].isMin) continue;
+                        handleBoundInversion(iid, V[j].id, false);
+                    }
+                }
+            } else {
+                for (long i = 0; i < 2 * nPar; i++) {
+                    if (!(V[i].isMin && V[i].hasBB)) continue;
+                    const int iid = V[i].id;
+                    long cnt = 0;
+                    for (long j = V.norm(i + 1); V[j].id != iid; j = V.norm(j + 1)) {
+                        if (!V[j].isMin) continue;
+                        handleBoundInversionPeri(iid, V[j].id, false);
+                        if (cnt++ > 2 * nPar) throw std::logic_error("loop in the initial sort");
+                    }
+                }
+            }
+        }
+        makeRemoveLaterProcess();
+    }
+
+    /* ---------- L6Geom, pkg/dem/L6Geom.cpp:27-217 ---------- */
+    static void setInitialLocalCoords(Contact& C, const V3& locX) {  // :27-38
+        V3 locY = vcross(locX, std::fabs(locX[1]) < std::fabs(locX[2]) ? v3(0, 1, 0) : v3(0, 0, 1));
+        locY = vsub(locY, vmul(locX, vdot(locY, locX)));
+        locY = vnormalized(locY);
+        V3 locZ = vcross(locX, locY);
+        msetcol(C.trsf, 0, locX); msetcol(C.trsf, 1, locY); msetcol(C.trsf, 2, locZ);
+    }
+    void handleSpheresLikeContact(Contact& C, const V3& pos1, const V3& vel1, const V3& angVel1, const V3& pos2, const V3& vel2,
+                                  const V3& angVel2, const V3& normal, const V3& contPt, double uN, double r1, double r2) {
+        if (!C.geom) {  // :53-72
+            C.geom = true;
+            C.trsf = meye();
+            setInitialLocalCoords(C, normal);
+            C.uN = uN;
+            C.lens[0] = std::fabs(r1) + uN / 2; C.lens[1] = std::fabs(r2) + uN / 2;
+            if (C.lens[0] < 0) C.lens[0] = C.lens[1];
+            if (C.lens[1] < 0) C.lens[1] = C.lens[0];
+            double rr = (r1 > 0 && r2 > 0) ? std::min(r1, r2) : (r1 > 0 ? r1 : r2);
+            C.contA = PI * (rr * rr);
+            C.contPt = contPt;
+            C.vel = vzero(); C.angVel = vzero();
+            return;
+        }
+        const double dt = P.dt;
+        const V3 currNormal = normal; const V3 prevNormal = mcol(C.trsf, 0);
+        const V3 prevContPt = C.contPt;
+        V3 shiftVel2 = periodic ? intrShiftVel(C.cd) : vzero();
+        V3 normRotVec = vcross(prevNormal, currNormal);
+        V3 midNormal = vmul(vadd(prevNormal, currNormal), .5);
+        if (vnorm(midNormal) == 0) midNormal = currNormal;
+        else midNormal = vnormalized(midNormal);
+        V3 normTwistVec = vmul(vmul(vmul(midNormal, dt), .5), vdot(midNormal, vadd(angVel1, angVel2)));
+        const M3 prevTrsf = C.trsf;
+        M3 midTrsf;
+        V3 rt = vadd(normRotVec, normTwistVec);
+        msetcol(midTrsf, 0, midNormal);
+        msetcol(midTrsf, 1, vsub(mcol(prevTrsf, 1), vdiv(vcross(mcol(prevTrsf, 1), rt), 2.)));
+        msetcol(midTrsf, 2, vcross(mcol(midTrsf, 0), mcol(midTrsf, 1)));
+        M3 currTrsf;
+        msetcol(currTrsf, 0, currNormal);
+        msetcol(currTrsf, 1, vsub(mcol(prevTrsf, 1), vcross(mcol(midTrsf, 1), rt)));
+        msetcol(currTrsf, 2, vcross(mcol(currTrsf, 0), mcol(currTrsf, 1)));
+        if (P.trsfRenorm > 0 && (step % P.trsfRenorm) == 0) {
+            V3 c0 = vnormalized(mcol(currTrsf, 0));
+            V3 c1 = mcol(currTrsf, 1);
+            c1 = vsub(c1, vmul(c0, vdot(c1, c0)));
+            c1 = vnormalized(c1);
+            msetcol(currTrsf, 0, c0); msetcol(currTrsf, 1, c1); msetcol(currTrsf, 2, vcross(c0, c1));
+        }
+        V3 midContPt = vmul(vadd(prevContPt, contPt), .5);
+        V3 midPos1 = vsub(pos1, vmul(vel1, dt / 2.));
+        V3 midPos2 = vsub(pos2, vmul(vadd(vel2, shiftVel2), dt / 2.));
+        V3 c1x = vsub(midContPt, midPos1);
+        V3 c2x = vsub(midContPt, midPos2);
+        V3 relVel = vsub(vadd(vadd(vel2, shiftVel2), vcross(angVel2, c2x)), vadd(vel1, vcross(angVel1, c1x)));
+        C.trsf = currTrsf;
+        C.vel = mtvmul(midTrsf, relVel);
+        C.angVel = mtvmul(midTrsf, vsub(angVel2, angVel1));
+        C.uN = uN;
+        C.contPt = contPt;
+    }
+    /* Cg2_Sphere_Sphere_L6Geom::go, pkg/dem/Sphere.cpp:88-109 */
+    bool cg2SphereSphere(Contact& C, const V3& shift2) {
+        int a = C.idA, b = C.idB;
+        double r1 = radius[a], r2 = radius[b];
+        int na = nodeOf(a), nb = nodeOf(b);
+        V3 p2 = vadd(pos[nb], shift2);
+        V3 relPos = vsub(p2, pos[na]);
+        double df = std::fabs(P.distFactor) * (r1 + r2);
+        double unDistSq = vsqn(relPos) - df * df;
+        if (unDistSq > 0 && !C.isReal()) return false;
+        double dist = vnorm(relPos);
+        double uN = dist - (r1 + r2);
+        V3 normal = vdiv(relPos, dist);
+        V3 contPt = vadd(pos[na], vmul(normal, r1 + 0.5 * uN));
+        handleSpheresLikeContact(C, pos[na], vel[na], angVel[na], p2, vel[nb], angVel[nb], normal, contPt, uN, r1, r2);
+        return true;
+    }
+    /* Cg2_Wall_Sphere_L6Geom::go, pkg/dem/Wall.cpp:45-79 */
+    bool cg2WallSphere(Contact& C, const V3& shift2) {
+        int w = C.idA, s = C.idB;
+        double rad = radius[s]; int ax = axis[w]; int sn = sense[w];
+        const V3& wallPos = pos[nodeOf(w)];
+        V3 spherePos = vadd(pos[nodeOf(s)], shift2);
+        double dist = spherePos[ax] - wallPos[ax];
+        if (!C.isReal() && std::fabs(dist) > rad) return false;
+        V3 contPt = spherePos; contPt[ax] = wallPos[ax];
+        V3 normal = vzero();
+        if (sn == 0) {
+            if (!C.geom) normal[ax] = dist > 0 ? 1. : -1.;
+            else normal[ax] = C.trsf.m[ax][0];
+        } else normal[ax] = (sn == 1 ? 1. : -1);
+        double uN = normal[ax] * dist - rad;
+        handleSpheresLikeContact(C, wallPos, vel[nodeOf(w)], angVel[nodeOf(w)], spherePos, vel[nodeOf(s)], angVel[nodeOf(s)], normal, contPt, uN, -rad, rad);
+        return true;
+    }
+    /* CompUtils::closestSegmentPt, lib/base/CompUtils.cpp:57-63 */
+    static V3 closestSegmentPt(const V3& Pt, const V3& A, const V3& B) {
+        V3 BA = vsub(B, A);
+        if (vsqn(BA) == 0.) return A;
+        double u = (vdot(Pt, BA) - vdot(A, BA)) / vsqn(BA);
+        return vadd(A, vmul(BA, std::min(1., std::max(0., u))));
+    }
+    /* CompUtils::triangleBarycentrics, lib/base/CompUtils.cpp:92-110 (2x2 inverse as
+       adjugate times 1/det, Eigen 3's compute_inverse for 2x2) */
+    static V3 triangleBarycentrics(const V3& x, const V3& A, const V3& B, const V3& Cc) {
+        V3 vB = vsub(B, A), vC = vsub(Cc, A), vX = vsub(x, A);
+        double m00 = vdot(vB, vB), m01 = vdot(vC, vB), m10 = vdot(vB, vC), m11 = vdot(vC, vC);
+        double invdet = 1. / (m00 * m11 - m10 * m01);
+        double i00 = m11 * invdet, i01 = -m01 * invdet, i10 = -m10 * invdet, i11 = m00 * invdet;
+        double r0 = vdot(vX, vB), r1 = vdot(vX, vC);
+        double u = i00 * r0 + i01 * r1, v = i10 * r0 + i11 * r1;
+        return v3(1 - u - v, u, v);
+    }
+    /* Cg2_Facet_Sphere_L6Geom::go, pkg/dem/Facet.cpp:235-288; getNormal :71-74; getOuterVectors :110-115;
+       interpolatePtLinAngVel :158-174 (fakeVel==0) */
+    bool cg2FacetSphere(Contact& C, const V3& shift2) {
+        int f = C.idA, s = C.idB;
+        const V3& n0 = pos[nodeOf(f, 0)]; const V3& n1 = pos[nodeOf(f, 1)]; const V3& n2 = pos[nodeOf(f, 2)];
+        double halfThick = radius[f], rad = radius[s];
+        V3 sC = vadd(pos[nodeOf(s)], shift2);
+        V3 fNormal = vnormalized(vcross(vsub(n1, n0), vsub(n2, n0)));
+        double planeDist = vdot(vsub(sC, n0), fNormal);
+        if (std::fabs(planeDist) > (rad + halfThick) && !C.isReal()) return false;
+        V3 fC = vsub(sC, vmul(fNormal, planeDist));
+        V3 nn = vcross(vsub(n1, n0), vsub(n2, n0));
+        V3 outVec[3] = {vcross(vsub(n1, n0), nn), vcross(vsub(n2, n1), nn), vcross(vsub(n0, n2), nn)};
+        const V3* nd[3] = {&n0, &n1, &n2};
+        double ll[3];
+        for (int i = 0; i < 3; i++) ll[i] = vdot(outVec[i], vsub(fC, *nd[i]));
+        int w = (ll[0] > 0 ? 1 : 0) + (ll[1] > 0 ? 2 : 0) + (ll[2] > 0 ? 4 : 0);
+        V3 contPt;
+        switch (w) {
+            case 0: contPt = fC; break;
+            case 1: contPt = closestSegmentPt(fC, n0, n1); break;
+            case 2: contPt = closestSegmentPt(fC, n1, n2); break;
+            case 4: contPt = closestSegmentPt(fC, n2, n0); break;
+            case 3: contPt = n1; break;
+            case 5: contPt = n0; break;
+            case 6: contPt = n2; break;
+            default: throw std::logic_error("Impossible sphere-facet intersection");
+        }
+        V3 normal = vsub(sC, contPt);
+        double rs = rad + halfThick;
+        if (vsqn(normal) > rs * rs && !C.isReal()) return false;
+        double dist = vnorm(normal);
+        normal = vdiv(normal, dist);
+        if (halfThick > 0) contPt = vadd(contPt, vmul(normal, halfThick));
+        double uN = dist - rad - halfThick;
+        V3 bc = triangleBarycentrics(contPt, n0, n1, n2);
+        const V3& v0 = vel[nodeOf(f, 0)]; const V3& v1 = vel[nodeOf(f, 1)]; const V3& v2 = vel[nodeOf(f, 2)];
+        V3 linVel = vadd(vadd(vmul(v0, bc[0]), vmul(v1, bc[1])), vmul(v2, bc[2]));
+        V3 av = vadd(vadd(vcross(vsub(n0, contPt), v0), vcross(vsub(n1, contPt), v1)), vcross(vsub(n2, contPt), v2));
+        handleSpheresLikeContact(C, contPt, linVel, av, sC, vel[nodeOf(s)], angVel[nodeOf(s)], normal, contPt, uN, std::max(halfThick, rad), rad);
+        return true;
+    }
+
+    /* Cp2_FrictMat_FrictPhys::updateFrictPhys, pkg/dem/FrictMat.cpp:21-44 */
+    void cp2Frict(Contact& C) {
+        const Mat& m1 = mats[matId[C.idA]]; const Mat& m2 = mats[matId[C.idB]];
+        double l0 = C.lens[0], l1 = C.lens[1], A = C.contA;
+        C.kn = 1 / (1 / (m1.young * A / l0) + 1 / (m2.young * A / l1));
+        C.kt = .5 * (m1.ktDivKn + m2.ktDivKn) * C.kn;
+        C.tanPhi = std::min(m1.tanPhi, m2.tanPhi);
+        C.force = vzero(); C.torque = vzero();
+        C.phys = true;
+    }
+    /* Cp2_FrictMat_HertzPhys::go, pkg/dem/Hertz.cpp:9-50 */
+    void cp2Hertz(Contact& C) {
+        const Mat& m1 = mats[matId[C.idA]]; const Mat& m2 = mats[matId[C.idB]];
+        double r1 = C.lens[0], r2 = C.lens[1];
+        double E1 = m1.young, E2 = m2.young, nu1 = P.poisson, nu2 = P.poisson;
+        C.K = (4 / 3.) * 1. / ((1 - nu1 * nu1) / E1 + (1 - nu2 * nu2) / E2);
+        C.R = 1. / (1. / r1 + 1. / r2);
+        double kn0 = C.K * std::sqrt(C.R);
+        C.gamma = P.gamma; C.alpha = P.alpha;
+        double G1 = E1 / (2 * (1 + nu1)), G2 = E2 / (2 * (1 + nu2));
+        double G = .5 * (G1 + G2);
+        double nu = .5 * (nu1 + nu2);
+        C.kt0 = 2 * std::sqrt(4 * C.R) * G / (2 - nu);
+        C.tanPhi = std::min(m1.tanPhi, m2.tanPhi);
+        double en = P.en;
+        if (en > 0 && en < 1.) {
+            double ma = mass[nodeOf(C.idA)], mb = mass[nodeOf(C.idB)];
+            double mbar = (ma <= 0 && mb > 0) ? mb : ((ma > 0 && mb <= 0) ? ma : (ma * mb) / (ma + mb));
+            double le = std::log(en);
+            double viscAlpha = -std::sqrt(5) * le / (std::sqrt(le * le + PI * PI));
+            C.alpha_sqrtMK = std::max(0., viscAlpha * std::sqrt(mbar * kn0));
+        } else C.alpha_sqrtMK = 0.0;
+        C.kn = 0; C.kt = 0;
+        C.force = vzero(); C.torque = vzero();
+        C.phys = true;
+    }
+    /* Law2_L6Geom_FrictPhys_IdealElPl::go, pkg/dem/IdealElPl.cpp:12-122 (iniEqlb=false, relRollStiff=0) */
+    bool lawIdealElPl(Contact& C, double* e) {
+        double uN = C.uN;
+        const double dt = P.dt;
+        if (uN > 0 && !P.noBreak) {
+            if (P.noSlip && P.trackEnergy) {
+                double Fn = C.kn * uN;
+                double Ft0 = C.force[1] + dt * C.kt * C.vel[1], Ft1 = C.force[2] + dt * C.kt * C.vel[2];
+                e[ORC_E_BROKEN] += .5 * (Fn * Fn / C.kn + (Ft0 * Ft0 + Ft1 * Ft1) / C.kt);
+                e[ORC_E_N + ORC_E_BROKEN] = 1;
+            }
+            return false;
+        }
+        C.torque = vzero();
+        C.force[0] = C.kn * uN;
+        if (P.noFrict || C.tanPhi == 0.) { C.force[1] = 0; C.force[2] = 0; }
+        else {
+            C.force[1] += dt * C.kt * C.vel[1];
+            C.force[2] += dt * C.kt * C.vel[2];
+            double maxFt = std::fabs(C.force[0]) * C.tanPhi;
+            double ftsq = C.force[1] * C.force[1] + C.force[2] * C.force[2];
+            if (ftsq > maxFt * maxFt && !P.noSlip) {
+                double FtNorm = std::sqrt(ftsq);
+                double ratio = maxFt / FtNorm;
+                if (FtNorm == 0.) ratio = 0.;
+                if (P.trackEnergy) {
+                    double dissip = (C.kt != 0. ? (maxFt) * (FtNorm - maxFt) / C.kt : 0.);
+                    if (dissip != 0.) { e[ORC_E_PLAST] += dissip; e[ORC_E_N + ORC_E_PLAST] = 1; }
+                }
+                C.force[1] *= ratio; C.force[2] *= ratio;
+            }
+            if (std::isnan(C.force[0]) || std::isnan(C.force[1]) || std::isnan(C.force[2]))
+                throw std::runtime_error("NaN force in contact");
+        }
+        if (P.trackEnergy) {
+            double ftsq = C.force[1] * C.force[1] + C.force[2] * C.force[2];
+            double elast = 0.5 * ((C.force[0] * C.force[0]) / C.kn + (C.kt != 0. ? ftsq / C.kt : 0.));
+            if (std::isnan(elast)) elast = 0.;
+            e[ORC_E_ELAST] += elast; e[ORC_E_N + ORC_E_ELAST] = 1;
+        }
+        return true;
+    }
+    /* Law2_L6Geom_HertzPhys_DMT::go, pkg/dem/Hertz.cpp:58-182 (alpha==0 branch) */
+    bool lawHertz(Contact& C, double* e) {
+        const double dt = P.dt;
+        C.torque = vzero();
+        const double velN = C.vel[0];
+        double kn0 = C.K * std::sqrt(C.R);
+        C.kn = (3 / 2.) * kn0 * std::sqrt(-C.uN);
+        if (C.uN > 0) return false;
+        double s = std::sqrt(-C.uN);
+        double Fne = -kn0 * ((s * s) * s);
+        double Fna = 4 * PI * C.R * C.gamma;
+        double eta = (C.alpha_sqrtMK > 0 ? C.alpha_sqrtMK * std::sqrt(std::sqrt(std::max(-C.uN, 0.))) : 0.);
+        double Fnv = eta * velN;
+        if (C.alpha == 0. && P.noAttraction && Fne + Fnv > 0) Fnv = -Fne;
+        double Fn = Fne + Fna + Fnv;
+        C.force[0] = Fn;
+        if (P.trackEnergy) { double v = Fnv * velN * dt; if (v != 0. || e[ORC_E_N + ORC_E_VISCN]) { e[ORC_E_VISCN] += v; e[ORC_E_N + ORC_E_VISCN] = 1; } }
+        C.kt = C.kt0 * std::sqrt(C.uN < 0 ? -C.uN : 0);
+        double Ft0 = dt * C.kt * C.vel[1], Ft1 = dt * C.kt * C.vel[2];
+        double maxFt = std::max(0., Fn) * C.tanPhi;
+        double ftsq = Ft0 * Ft0 + Ft1 * Ft1;
+        if (ftsq > maxFt * maxFt) {
+            double FtNorm = std::sqrt(ftsq);
+            double ratio = maxFt / FtNorm;
+            if (P.trackEnergy) { double v = (.5 * (FtNorm - maxFt) + maxFt) * (FtNorm - maxFt) / C.kt; if (v != 0.) { e[ORC_E_PLAST] += v; e[ORC_E_N + ORC_E_PLAST] = 1; } }
+            Ft0 *= ratio; Ft1 *= ratio;
+        } else {
+            Ft0 += eta * C.vel[1]; Ft1 += eta * C.vel[2];
+            if (P.trackEnergy) { double v = eta * (C.vel[1] * C.vel[1] + C.vel[2] * C.vel[2]) * dt; if (v != 0.) { e[ORC_E_VISCT] += v; e[ORC_E_N + ORC_E_VISCT] = 1; } }
+        }
+        C.force[1] = Ft0; C.force[2] = Ft1;
+        if (P.trackEnergy && C.uN < 0) {
+            double s5 = (((s * s) * s) * s) * s;
+            double el = kn0 * (2 / 5.) * s5 + 0.5 * (Ft0 * Ft0 + Ft1 * Ft1) / C.kt;
+            if (!std::isnan(el)) e[ORC_E_ELAST] += el;
+            e[ORC_E_N + ORC_E_ELAST] = 1;
+        }
+        return true;
+    }
+
+    void rebuildAdjacency() {
+        lin.clear(); lin.reserve(con.size());
+        for (auto& kv : con) lin.push_back(&kv.second);
+        adjOff.assign(np + 1, 0);
+        for (Contact* c : lin) { adjOff[c->idA + 1]++; adjOff[c->idB + 1]++; }
+        for (int p = 0; p < np; p++) adjOff[p + 1] += adjOff[p];
+        adjC.assign(adjOff[np], nullptr);
+        std::vector<long> fill(adjOff.begin(), adjOff.end() - 1);
+        // contacts in (min,max) order: each particle receives them in ascending order of the other id
+        for (Contact* c : lin) {
+            int lo = std::min(c->idA, c->idB), hi = std::max(c->idA, c->idB);
+            adjC[fill[hi]++] = c;
+            (void)lo;
+        }
+        for (Contact* c : lin) {
+            int lo = std::min(c->idA, c->idB);
+            adjC[fill[lo]++] = c;
+        }
+        // per particle: entries where p is max come first (other<p, ascending), then where p is min (other>p, ascending)
+        adjDirty = false;
+    }
+
+    /* ContactLoop::run, pkg/dem/ContactLoop.cpp:93-259, with deterministic force application */
+    void contactLoop() {
+        pending.clear();  // removeAllPending: the collider already processed them
+        if (adjDirty) rebuildAdjacency();
+        const long nc = (long)lin.size();
+        std::vector<double> ebuf((size_t)nc * 2 * ORC_E_N, 0.);
+        std::vector<char> broke(nc, 0);
+        std::string perr;
+        #pragma omp parallel for schedule(guided)
+        for (long i = 0; i < nc; i++) {
+            try {
+                Contact& C = *lin[i];
+                double* e = &ebuf[(size_t)i * 2 * ORC_E_N];
+                int sa = shape[C.idA], sb = shape[C.idB];
+                bool fresh = (C.stepCreated == step);
+                if (!C.isReal() && fresh) {
+                    if (sa == ORC_SHAPE_SPHERE && sb == ORC_SHAPE_SPHERE) {
+                        double x = std::fabs(P.distFactor * (radius[C.idA] + radius[C.idB]));
+                        C.minDist00Sq = x * x;
+                    } else C.minDist00Sq = -1;
+                }
+                V3 shift2 = periodic ? intrShiftPos(C.cd) : vzero();
+                if (P.dist00 && !C.isReal() && !fresh && C.minDist00Sq > 0 &&
+                    vsqn(vsub(pos[nodeOf(C.idA)], vadd(pos[nodeOf(C.idB)], shift2))) > C.minDist00Sq) continue;
+                bool geomCreated;
+                if (sa == ORC_SHAPE_SPHERE && sb == ORC_SHAPE_SPHERE) geomCreated = cg2SphereSphere(C, shift2);
+                else if (sa == ORC_SHAPE_WALL && sb == ORC_SHAPE_SPHERE) geomCreated = cg2WallSphere(C, shift2);
+                else if (sa == ORC_SHAPE_FACET && sb == ORC_SHAPE_SPHERE) geomCreated = cg2FacetSphere(C, shift2);
+                else geomCreated = false;
+                if (!geomCreated) continue;
+                if (!C.phys) C.stepCreated = step;
+                if (!C.phys) { if (P.law == ORC_LAW_HERTZ_DMT) cp2Hertz(C); else cp2Frict(C); }
+                bool keep = (P.law == ORC_LAW_HERTZ_DMT) ? lawHertz(C, e) : lawIdealElPl(C, e);
+                if (!keep) { C.geom = C.phys = false; broke[i] = 1; }  // requestRemoval: Contact::reset, ContactContainer.cpp:164-171
+            } catch (std::exception& ex) {
+                #pragma omp critical
+                perr = ex.what();
+            }
+        }
+        if (!perr.empty()) throw std::runtime_error(perr);
+        for (long i = 0; i < nc; i++) if (broke[i]) pending.push_back(key(lin[i]->idA, lin[i]->idB));
+        if (P.trackEnergy) {
+            for (long i = 0; i < nc; i++) {
+                const double* e = &ebuf[(size_t)i * 2 * ORC_E_N];
+                for (int k = 0; k < ORC_E_N; k++) if (e[ORC_E_N + k]) eAdd(k, e[k], k != ORC_E_ELAST && k != ORC_E_BROKEN);
+            }
+        }
+        // deterministic force application (applyForceUninodal, ContactLoop.cpp:261-267; Contact::getForceTorqueBranch, Contact.cpp:24-34)
+        #pragma omp parallel for schedule(guided)
+        for (int p = 0; p < np; p++) {
+            if (numNodes(p) != 1) continue;
+            int nd = nodeOf(p);
+            V3 F0 = force[nd], T0 = torque[nd];
+            for (long k = adjOff[p]; k < adjOff[p + 1]; k++) {
+                const Contact& C = *adjC[k];
+                if (!C.isReal()) continue;
+                bool isPA = (C.idA == p);
+                double sign = isPA ? 1. : -1.;
+                V3 F = vmul(mvmul(C.trsf, C.force), sign);
+                V3 xc = vsub(C.contPt, (!isPA && periodic) ? vadd(pos[nd], intrShiftPos(C.cd)) : pos[nd]);
+                F0 = vadd(F0, F);
+                T0 = vadd(T0, vcross(xc, F));
+            }
+            force[nd] = F0; torque[nd] = T0;
+        }
+    }
+
+    /* Scene::doOneStep, core/Scene.cpp:319-424 */
+    void doOneStep() {
+        if (periodic) cellIntegrateAndUpdate(P.dt);
+        if (P.trackEnergy) { E[ORC_E_KINETIC] = 0; E[ORC_E_ELAST] = 0; }  // resetResettables
+        leapfrog();
+        collider();
+        contactLoop();
+        if (periodic) cellSetNextGradV();
+        step++;
+    }
+};
+
+Oracle* O(void* h) { return static_cast<Oracle*>(h); }
+
+}  // namespace
+
+extern "C" {
+
+void* orc_create(void) {
+    Oracle* o = new Oracle();
+    o->P.dt = 1e-5; o->P.damping = 0.2; o->P.verletDist = -.05; o->P.distFactor = 1; o->P.poisson = .2; o->P.en = NAN;
+    o->P.law = ORC_LAW_IDEALELPL; o->P.noAttraction = 1; o->P.trsfRenorm = 100; o->P.dist00 = 1; o->P.loneMask = 2;
+    return o;
+}
+void orc_destroy(void* h) { delete O(h); }
+const char* orc_last_error(void* h) { return O(h)->err.c_str(); }
+
+int orc_set_nodes(void* h, int32_t n, const double* pos3, const double* ori4, const double* vel3, const double* angVel3,
+                  const double* mass, const double* inertia3, const uint32_t* flags, const double* force3, const double* torque3) {
+    Oracle* o = O(h);
+    o->nn = n;
+    o->pos.resize(n); o->vel.resize(n); o->angVel.resize(n); o->force.resize(n); o->torque.resize(n); o->inertia.resize(n);
+    o->ori.resize(n); o->mass.resize(n); o->flags.resize(n);
+    for (int i = 0; i < n; i++) {
+        for (int k = 0; k < 3; k++) {
+            o->pos[i][k] = pos3[3 * i + k]; o->vel[i][k] = vel3[3 * i + k]; o->angVel[i][k] = angVel3[3 * i + k];
+            o->inertia[i][k] = inertia3[3 * i + k];
+            o->force[i][k] = force3 ? force3[3 * i + k] : 0.; o->torque[i][k] = torque3 ? torque3[3 * i + k] : 0.;
+        }
+        o->ori[i].w = ori4[4 * i + 0]; o->ori[i].x = ori4[4 * i + 1]; o->ori[i].y = ori4[4 * i + 2]; o->ori[i].z = ori4[4 * i + 3];
+        o->mass[i] = mass[i]; o->flags[i] = flags[i];
+        if (flags[i] & (CLUMP_CLUMP | CLUMP_CLUMPED)) { o->err = "clumps are outside the restated path"; return -1; }
+        bool asph = !(o->inertia[i][0] == o->inertia[i][1] && o->inertia[i][1] == o->inertia[i][2]);
+        if (asph && (flags[i] & DOF_RXRYRZ) != DOF_RXRYRZ) { o->err = "aspherical integrator is outside the restated path"; return -1; }
+    }
+    return 0;
+}
+int orc_set_particles(void* h, int32_t n, const int32_t* shape, const int32_t* nodes3, const double* radius, const int32_t* axis,
+                      const int32_t* sense, const uint32_t* mask, const int32_t* mat) {
+    Oracle* o = O(h);
+    o->np = n;
+    o->shape.assign(shape, shape + n); o->nodeIx.assign(nodes3, nodes3 + 3 * n); o->radius.assign(radius, radius + n);
+    o->axis.assign(axis, axis + n); o->sense.assign(sense, sense + n); o->mask.assign(mask, mask + n); o->matId.assign(mat, mat + n);
+    o->contactsDirty = true; o->con.clear(); o->adjDirty = true; o->aabb.assign(n, Aabb());
+    return 0;
+}
+int orc_set_materials(void* h, int32_t n, const double* density, const double* young, const double* ktDivKn, const double* tanPhi) {
+    Oracle* o = O(h);
+    o->mats.resize(n);
+    for (int i = 0; i < n; i++) o->mats[i] = Mat{density[i], young[i], ktDivKn[i], tanPhi[i]};
+    return 0;
+}
+int orc_set_cell(void* h, int32_t periodic, const double* hSize9, const double* gradV9, const double* nextGradV9, int32_t homoDeform) {
+    Oracle* o = O(h);
+    try {
+        o->periodic = periodic != 0;
+        for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) {
+            o->hSize.m[i][j] = hSize9[3 * i + j]; o->gradV.m[i][j] = gradV9[3 * i + j]; o->nextGradV.m[i][j] = nextGradV9[3 * i + j];
+        }
+        o->homoDeform = homoDeform;
+        o->trsf = meye(); o->pprevHsize = o->hSize;
+        M3 W = mscale(msub(o->gradV, mtrans(o->gradV)), .5);
+        o->spinVec = vmul(leviCivita(W), .5);
+        if (o->periodic) o->cellIntegrateAndUpdate(0);  // Cell::postLoad
+    } catch (std::exception& e) { o->err = e.what(); return -1; }
+    return 0;
+}
+int orc_set_params(void* h, const orc_params* p) {
+    Oracle* o = O(h);
+    o->P = *p;
+    if (p->law == ORC_LAW_HERTZ_DMT && (p->gamma != 0 || p->alpha != 0)) { o->err = "adhesive Hertz (gamma/alpha != 0) is outside the restated path"; return -1; }
+    o->verletResolved = false;
+#ifdef _OPENMP
+    if (p->nThreads > 0) omp_set_num_threads(p->nThreads);
+#endif
+    return 0;
+}
+int orc_step(void* h, int32_t n) {
+    Oracle* o = O(h);
+    try { for (int i = 0; i < n; i++) o->doOneStep(); }
+    catch (std::exception& e) { o->err = e.what(); return -1; }
+    return 0;
+}
+int orc_get_nodes(void* h, double* pos3, double* ori4, double* vel3, double* angVel3, double* force3, double* torque3) {
+    Oracle* o = O(h);
+    for (int i = 0; i < o->nn; i++) {
+        for (int k = 0; k < 3; k++) {
+            if (pos3) pos3[3 * i + k] = o->pos[i][k];
+            if (vel3) vel3[3 * i + k] = o->vel[i][k];
+            if (angVel3) angVel3[3 * i + k] = o->angVel[i][k];
+            if (force3) force3[3 * i + k] = o->force[i][k];
+            if (torque3) torque3[3 * i + k] = o->torque[i][k];
+        }
+        if (ori4) { ori4[4 * i] = o->ori[i].w; ori4[4 * i + 1] = o->ori[i].x; ori4[4 * i + 2] = o->ori[i].y; ori4[4 * i + 3] = o->ori[i].z; }
+    }
+    return 0;
+}
+int64_t orc_num_contacts(void* h) { return (int64_t)O(h)->con.size(); }
+int orc_get_contacts(void* h, int64_t cap, int32_t* idA, int32_t* idB, int32_t* cellDist3, uint8_t* isReal, double* forceLocal3,
+                     double* contPt3, double* trsf9, double* uN, int64_t* nOut) {
+    Oracle* o = O(h);
+    int64_t i = 0;
+    for (auto& kv : o->con) {
+        if (i >= cap) break;
+        const Contact& c = kv.second;
+        if (idA) idA[i] = c.idA;
+        if (idB) idB[i] = c.idB;
+        for (int k = 0; k < 3; k++) {
+            if (cellDist3) cellDist3[3 * i + k] = c.cd[k];
+            if (forceLocal3) forceLocal3[3 * i + k] = c.isReal() ? c.force[k] : 0.;
+            if (contPt3) contPt3[3 * i + k] = c.contPt[k];
+        }
+        if (trsf9) for (int a = 0; a < 3; a++) for (int b = 0; b < 3; b++) trsf9[9 * i + 3 * a + b] = c.trsf.m[a][b];
+        if (uN) uN[i] = c.isReal() ? c.uN : NAN;
+        if (isReal) isReal[i] = c.isReal() ? 1 : 0;
+        i++;
+    }
+    if (nOut) *nOut = i;
+    return 0;
+}
+int orc_get_energy(void* h, double* out, int32_t* created) {
+    Oracle* o = O(h);
+    for (int k = 0; k < ORC_E_N; k++) { if (out) out[k] = o->E[k]; if (created) created[k] = o->Ecreated[k]; }
+    return 0;
+}
+int orc_get_stats(void* h, int64_t* step, int64_t* nFullRuns, double* verletDist) {
+    Oracle* o = O(h);
+    if (step) *step = o->step;
+    if (nFullRuns) *nFullRuns = o->nFullRuns;
+    if (verletDist) *verletDist = o->verletDistActual;
+    return 0;
+}
+int orc_get_cell(void* h, double* hSize9, double* gradV9) {
+    Oracle* o = O(h);
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) { if (hSize9) hSize9[3 * i + j] = o->hSize.m[i][j]; if (gradV9) gradV9[3 * i + j] = o->gradV.m[i][j]; }
+    return 0;
+}
+int64_t orc_bruteforce_pairs(void* h, int64_t cap, int32_t* idMin, int32_t* idMax) {
+    Oracle* o = O(h);
+    int64_t n = 0;
+    if ((long)o->minima.size() != 3L * o->np) return -1;
+    for (int a = 0; a < o->np; a++)
+        for (int b = a + 1; b < o->np; b++) {
+            if (!o->aabb[a].has || !o->aabb[b].has) continue;
+            bool ov;
+            try { ov = o->boundsOverlap(a, b); } catch (...) { return -2; }
+            if (!ov || !o->mayCollide(a, b)) continue;
+            if (n < cap) { idMin[n] = a; idMax[n] = b; }
+            n++;
+        }
+    return n;
+}
+double orc_pwave_dt(void* h) {
+    Oracle* o = O(h);
+    double dt = INF;
+    for (int p = 0; p < o->np; p++) {
+        if (o->shape[p] != ORC_SHAPE_SPHERE) continue;
+        const Mat& m = o->mats[o->matId[p]];
+        dt = std::min(dt, o->radius[p] / (1. * std::sqrt(m.young / m.density)));
+    }
+    return dt;
+}
+
+}  // extern "C"

@@ -1,0 +1,193 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle_dem.so) -- TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+
+Parity-pinning status of the oracle's functions (details in DESIGN.md):
+  pinned by reference known answers / closed forms: pWaveDt, Sphere mass/inertia,
+    bounds, contact set (vs brute force), sphere/wall/facet L6Geom creation,
+    FrictPhys, IdealElPl slip cap and dissipation, Hertz collision time,
+    restitution, ballistic leapfrog, damping, GRADV2 cell, incident velocity
+    under PBC, energy balance;
+  parity unpinned (no reference vector exists; checked only by invariants):
+    multi-step L6Geom trsf evolution with renormalisation, IdealElPl Ft history
+    over many steps, summation order of forces and energies.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle_dem.so")
+
+ENERGY_NAMES = ["kinetic", "elast", "plast", "grav", "nonviscDamp", "viscN", "viscT", "broken"]
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("damping", C.c_double), ("gravity", C.c_double * 3), ("verletDist", C.c_double),
+        ("distFactor", C.c_double), ("poisson", C.c_double), ("en", C.c_double), ("gamma", C.c_double),
+        ("alpha", C.c_double),
+        ("law", C.c_int32), ("noSlip", C.c_int32), ("noBreak", C.c_int32), ("noFrict", C.c_int32),
+        ("noAttraction", C.c_int32), ("trackEnergy", C.c_int32), ("trsfRenorm", C.c_int32), ("dist00", C.c_int32),
+        ("loneMask", C.c_uint32), ("nThreads", C.c_int32),
+    ]
+
+
+_L = None
+
+
+def build_oracle():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _L
+    if _L is not None:
+        return _L
+    if not os.path.exists(ORACLE_SO):
+        build_oracle()
+    L = C.CDLL(ORACLE_SO)
+    P, I32, I64, D = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    pd, pi, pu, pl = C.POINTER(D), C.POINTER(I32), C.POINTER(C.c_uint32), C.POINTER(I64)
+    sig = {
+        "orc_create": (P, []), "orc_destroy": (None, [P]), "orc_last_error": (C.c_char_p, [P]),
+        "orc_set_nodes": (C.c_int, [P, I32, pd, pd, pd, pd, pd, pd, pu, pd, pd]),
+        "orc_set_particles": (C.c_int, [P, I32, pi, pi, pd, pi, pi, pu, pi]),
+        "orc_set_materials": (C.c_int, [P, I32, pd, pd, pd, pd]),
+        "orc_set_cell": (C.c_int, [P, I32, pd, pd, pd, I32]),
+        "orc_set_params": (C.c_int, [P, C.POINTER(OrcParams)]),
+        "orc_step": (C.c_int, [P, I32]),
+        "orc_get_nodes": (C.c_int, [P, pd, pd, pd, pd, pd, pd]),
+        "orc_num_contacts": (I64, [P]),
+        "orc_get_contacts": (C.c_int, [P, I64, pi, pi, pi, C.POINTER(C.c_uint8), pd, pd, pd, pd, pl]),
+        "orc_get_energy": (C.c_int, [P, pd, pi]),
+        "orc_get_stats": (C.c_int, [P, pl, pl, pd]),
+        "orc_get_cell": (C.c_int, [P, pd, pd]),
+        "orc_bruteforce_pairs": (I64, [P, I64, pi, pi]),
+        "orc_pwave_dt": (D, [P]),
+    }
+    for n, (r, a) in sig.items():
+        f = getattr(L, n)
+        f.restype, f.argtypes = r, a
+    _L = L
+    return L
+
+
+def _p(a, t):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+def orc_params_from(wp, nThreads=0):
+    """Copy a woodem_amd WdmParams (or any object with the same fields) into OrcParams."""
+    p = OrcParams()
+    for name, _ in OrcParams._fields_:
+        if name == "nThreads":
+            continue
+        if name == "gravity":
+            for k in range(3):
+                p.gravity[k] = wp.gravity[k]
+        else:
+            setattr(p, name, getattr(wp, name))
+    p.nThreads = nThreads
+    return p
+
+
+class OracleRun:
+    """Same interface as woodem_amd.runner.HipRun, on the CPU oracle."""
+
+    def __init__(self, arrays, params, cell, nThreads=0):
+        L = lib()
+        self.L = L
+        self.h = L.orc_create()
+        a = arrays.normalized()
+        self.a = a
+        D, I32, U32 = C.c_double, C.c_int32, C.c_uint32
+        self._chk(L.orc_set_nodes(self.h, a.n_node, _p(a.pos, D), _p(a.ori, D), _p(a.vel, D), _p(a.angVel, D), _p(a.mass, D),
+                                  _p(a.inertia, D), _p(a.flags, U32), _p(a.force, D), _p(a.torque, D)))
+        self._chk(L.orc_set_particles(self.h, a.n_par, _p(a.shape, I32), _p(a.nodes, I32), _p(a.radius, D), _p(a.axis, I32),
+                                      _p(a.sense, I32), _p(a.mask, U32), _p(a.mat, I32)))
+        self._chk(L.orc_set_materials(self.h, a.n_mat, _p(a.density, D), _p(a.young, D), _p(a.ktDivKn, D), _p(a.tanPhi, D)))
+        h9 = np.ascontiguousarray(cell["hSize"], np.float64).reshape(9)
+        g9 = np.ascontiguousarray(cell["gradV"], np.float64).reshape(9)
+        n9 = np.ascontiguousarray(cell["nextGradV"], np.float64).reshape(9)
+        self._chk(L.orc_set_cell(self.h, int(cell["periodic"]), _p(h9, D), _p(g9, D), _p(n9, D), int(cell["homoDeform"])))
+        op = params if isinstance(params, OrcParams) else orc_params_from(params, nThreads)
+        self._chk(L.orc_set_params(self.h, C.byref(op)))
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError("oracle: " + self.L.orc_last_error(self.h).decode())
+
+    def step(self, n=1):
+        self._chk(self.L.orc_step(self.h, int(n)))
+
+    def nodes(self):
+        n = self.a.n_node
+        out = {k: np.empty((n, 3)) for k in ("pos", "vel", "angVel", "force", "torque")}
+        out["ori"] = np.empty((n, 4))
+        D = C.c_double
+        self._chk(self.L.orc_get_nodes(self.h, _p(out["pos"], D), _p(out["ori"], D), _p(out["vel"], D), _p(out["angVel"], D),
+                                       _p(out["force"], D), _p(out["torque"], D)))
+        return out
+
+    def contacts(self):
+        cap = int(self.L.orc_num_contacts(self.h))
+        D, I32 = C.c_double, C.c_int32
+        idA, idB = np.empty(cap, np.int32), np.empty(cap, np.int32)
+        cd = np.empty((cap, 3), np.int32)
+        real = np.empty(cap, np.uint8)
+        fl, cp, tr, un = np.empty((cap, 3)), np.empty((cap, 3)), np.empty((cap, 9)), np.empty(cap)
+        nout = C.c_int64()
+        self._chk(self.L.orc_get_contacts(self.h, cap, _p(idA, I32), _p(idB, I32), _p(cd, I32),
+                                          real.ctypes.data_as(C.POINTER(C.c_uint8)), _p(fl, D), _p(cp, D), _p(tr, D), _p(un, D),
+                                          C.byref(nout)))
+        return dict(idA=idA, idB=idB, cellDist=cd, isReal=real.astype(bool), forceLocal=fl, contPt=cp,
+                    trsf=tr.reshape(-1, 3, 3), uN=un)
+
+    def energy(self):
+        v = np.zeros(8)
+        c = np.zeros(8, np.int32)
+        self._chk(self.L.orc_get_energy(self.h, _p(v, C.c_double), _p(c, C.c_int32)))
+        return v, c.astype(bool)
+
+    def stats(self):
+        s, f, vd = C.c_int64(), C.c_int64(), C.c_double()
+        self._chk(self.L.orc_get_stats(self.h, C.byref(s), C.byref(f), C.byref(vd)))
+        return dict(step=s.value, nFullRuns=f.value, verletDist=vd.value)
+
+    def cell(self):
+        h, g = np.zeros(9), np.zeros(9)
+        self._chk(self.L.orc_get_cell(self.h, _p(h, C.c_double), _p(g, C.c_double)))
+        return h.reshape(3, 3), g.reshape(3, 3)
+
+    def bruteforce_pairs(self):
+        n = int(self.L.orc_bruteforce_pairs(self.h, 0, None, None))
+        if n < 0:
+            raise RuntimeError("bruteforce failed")
+        a, b = np.empty(n, np.int32), np.empty(n, np.int32)
+        self.L.orc_bruteforce_pairs(self.h, n, _p(a, C.c_int32), _p(b, C.c_int32))
+        return a, b
+
+    def pwave_dt(self):
+        return float(self.L.orc_pwave_dt(self.h))
+
+    def close(self):
+        if self.h:
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pair_keys(c):
+    """Canonical (min,max) pair keys of a contacts dict, as sorted int64."""
+    lo = np.minimum(c["idA"], c["idB"]).astype(np.int64)
+    hi = np.maximum(c["idA"], c["idB"]).astype(np.int64)
+    return np.sort((lo << 32) | hi)

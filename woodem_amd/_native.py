@@ -1,0 +1,232 @@
+"""ctypes binding of libwoodem_hip.so (the C ABI declared in include/woodem_hip.h).
+
+The library is the only compute path: if it is missing, importing this module
+raises, and there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwoodem_hip.so")
+
+WDM_OK = 0
+ERRORS = {-1: "WDM_E_ARG", -2: "WDM_E_HIP", -3: "WDM_E_OOM", -4: "WDM_E_NAN", -5: "WDM_E_OVERSIZE",
+          -6: "WDM_E_DEGENERATE", -7: "WDM_E_UNSUPPORTED", -8: "WDM_E_STATE"}
+
+SHAPE_NONE, SHAPE_SPHERE, SHAPE_WALL, SHAPE_FACET = 0, 1, 2, 3
+LAW_IDEALELPL, LAW_HERTZ_DMT = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+ENERGY_NAMES = ["kinetic", "elast", "plast", "grav", "nonviscDamp", "viscN", "viscT", "broken"]
+
+
+class WdmParams(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("damping", C.c_double), ("gravity", C.c_double * 3), ("verletDist", C.c_double),
+        ("distFactor", C.c_double), ("poisson", C.c_double), ("en", C.c_double), ("gamma", C.c_double),
+        ("alpha", C.c_double),
+        ("law", C.c_int32), ("noSlip", C.c_int32), ("noBreak", C.c_int32), ("noFrict", C.c_int32),
+        ("noAttraction", C.c_int32), ("trackEnergy", C.c_int32), ("trsfRenorm", C.c_int32), ("dist00", C.c_int32),
+        ("loneMask", C.c_uint32), ("precision", C.c_int32),
+    ]
+
+
+class WdmError(RuntimeError):
+    """Error returned by the C ABI; `code` is the WDM_E_* value."""
+
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libwoodem_hip.so is not built (run __graft_entry__.build() or python woodem_amd/build.py)")
+    L = C.CDLL(LIB_PATH)
+    P, I32, I64, D, U8, U32 = C.c_void_p, C.c_int32, C.c_int64, C.c_double, C.c_uint8, C.c_uint32
+    pd, pi, pu, pl = C.POINTER(D), C.POINTER(I32), C.POINTER(U32), C.POINTER(I64)
+    sig = {
+        "wdm_create": (C.c_int, [C.POINTER(P), C.c_int, C.c_size_t]),
+        "wdm_destroy": (None, [P]),
+        "wdm_last_error": (C.c_char_p, [P]),
+        "wdm_scene_alloc": (C.c_int, [P, I32, I32, I32, pi]),
+        "wdm_scene_free": (C.c_int, [P, I32]),
+        "wdm_scene_footprint": (C.c_size_t, [I32, I32, D]),
+        "wdm_upload_nodes": (C.c_int, [P, I32, I32, pd, pd, pd, pd, pd, pd, pu, pd, pd]),
+        "wdm_upload_particles": (C.c_int, [P, I32, I32, pi, pi, pd, pi, pi, pu, pi]),
+        "wdm_upload_materials": (C.c_int, [P, I32, I32, pd, pd, pd, pd]),
+        "wdm_set_cell": (C.c_int, [P, I32, I32, pd, pd, pd, I32]),
+        "wdm_set_params": (C.c_int, [P, I32, C.POINTER(WdmParams)]),
+        "wdm_step_begin": (C.c_int, [P, I32]),
+        "wdm_leapfrog_run": (C.c_int, [P, I32]),
+        "wdm_collider_run": (C.c_int, [P, I32]),
+        "wdm_contact_loop_run": (C.c_int, [P, I32]),
+        "wdm_step_end": (C.c_int, [P, I32]),
+        "wdm_step": (C.c_int, [P, pi, I32, I32]),
+        "wdm_download_nodes": (C.c_int, [P, I32, pd, pd, pd, pd, pd, pd]),
+        "wdm_num_contacts": (C.c_int, [P, I32, pl]),
+        "wdm_get_contacts": (C.c_int, [P, I32, I64, pi, pi, pi, C.POINTER(U8), pd, pd, pd, pd, pl]),
+        "wdm_get_energy": (C.c_int, [P, I32, pd, pi]),
+        "wdm_get_stats": (C.c_int, [P, I32, pl, pl, pd]),
+        "wdm_get_cell": (C.c_int, [P, I32, pd, pd]),
+        "wdm_sync": (C.c_int, [P]),
+        "wdm_kernel_times": (C.c_int, [P, I32, C.c_char_p, pd, pl, pi]),
+        "wdm_enable_timing": (C.c_int, [P, I32]),
+        "wdm_relax_overlaps": (C.c_int, [P, I32, I32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def ptr(a, ctype):
+    """Pointer to a contiguous numpy array (None passes NULL)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+class Context:
+    """One wdm_ctx: a HIP stream on one device plus its scenes."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        L = lib()
+        h = C.c_void_p()
+        rc = L.wdm_create(C.byref(h), device, hbm_budget)
+        if rc != WDM_OK:
+            raise WdmError(rc, "wdm_create failed (no HIP device?)")
+        self.h = h
+        self.L = L
+
+    def check(self, rc):
+        if rc != WDM_OK:
+            raise WdmError(rc, self.L.wdm_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.L.wdm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- thin wrappers ----
+    def scene_alloc(self, n_par, n_node, n_mat):
+        sid = C.c_int32()
+        self.check(self.L.wdm_scene_alloc(self.h, n_par, n_node, n_mat, C.byref(sid)))
+        return sid.value
+
+    def scene_free(self, sid):
+        self.check(self.L.wdm_scene_free(self.h, sid))
+
+    def upload(self, sid, arrs):
+        """Upload a DemArrays (woodem_amd.soa) into scene `sid`."""
+        a = arrs
+        L, h = self.L, self.h
+        D, I32, U32 = C.c_double, C.c_int32, C.c_uint32
+        self.check(L.wdm_upload_nodes(h, sid, a.n_node, ptr(a.pos, D), ptr(a.ori, D), ptr(a.vel, D), ptr(a.angVel, D),
+                                      ptr(a.mass, D), ptr(a.inertia, D), ptr(a.flags, U32), ptr(a.force, D), ptr(a.torque, D)))
+        self.check(L.wdm_upload_particles(h, sid, a.n_par, ptr(a.shape, I32), ptr(a.nodes, I32), ptr(a.radius, D),
+                                          ptr(a.axis, I32), ptr(a.sense, I32), ptr(a.mask, U32), ptr(a.mat, I32)))
+        self.check(L.wdm_upload_materials(h, sid, a.n_mat, ptr(a.density, D), ptr(a.young, D), ptr(a.ktDivKn, D),
+                                          ptr(a.tanPhi, D)))
+
+    def set_cell(self, sid, periodic, hSize, gradV, nextGradV, homoDeform):
+        D = C.c_double
+        h9 = np.ascontiguousarray(hSize, dtype=np.float64).reshape(9)
+        g9 = np.ascontiguousarray(gradV, dtype=np.float64).reshape(9)
+        n9 = np.ascontiguousarray(nextGradV, dtype=np.float64).reshape(9)
+        self.check(self.L.wdm_set_cell(self.h, sid, int(periodic), ptr(h9, D), ptr(g9, D), ptr(n9, D), int(homoDeform)))
+
+    def set_params(self, sid, p):
+        self.check(self.L.wdm_set_params(self.h, sid, C.byref(p)))
+
+    def step(self, sids, n):
+        arr = np.ascontiguousarray(sids, dtype=np.int32)
+        self.check(self.L.wdm_step(self.h, ptr(arr, C.c_int32), len(arr), int(n)))
+
+    def download_nodes(self, sid, n_node):
+        D = C.c_double
+        out = {k: np.empty((n_node, 3)) for k in ("pos", "vel", "angVel", "force", "torque")}
+        out["ori"] = np.empty((n_node, 4))
+        self.check(self.L.wdm_download_nodes(self.h, sid, ptr(out["pos"], D), ptr(out["ori"], D), ptr(out["vel"], D),
+                                             ptr(out["angVel"], D), ptr(out["force"], D), ptr(out["torque"], D)))
+        return out
+
+    def contacts(self, sid):
+        n = C.c_int64()
+        self.check(self.L.wdm_num_contacts(self.h, sid, C.byref(n)))
+        cap = n.value
+        D, I32 = C.c_double, C.c_int32
+        idA = np.empty(cap, np.int32)
+        idB = np.empty(cap, np.int32)
+        cd = np.empty((cap, 3), np.int32)
+        real = np.empty(cap, np.uint8)
+        fl = np.empty((cap, 3))
+        cp = np.empty((cap, 3))
+        tr = np.empty((cap, 9))
+        un = np.empty(cap)
+        nout = C.c_int64()
+        self.check(self.L.wdm_get_contacts(self.h, sid, cap, ptr(idA, I32), ptr(idB, I32), ptr(cd, I32),
+                                           real.ctypes.data_as(C.POINTER(C.c_uint8)), ptr(fl, D), ptr(cp, D), ptr(tr, D),
+                                           ptr(un, D), C.byref(nout)))
+        k = nout.value
+        return dict(idA=idA[:k], idB=idB[:k], cellDist=cd[:k], isReal=real[:k].astype(bool), forceLocal=fl[:k],
+                    contPt=cp[:k], trsf=tr[:k].reshape(-1, 3, 3), uN=un[:k])
+
+    def energy(self, sid):
+        v = np.zeros(8)
+        cr = np.zeros(8, np.int32)
+        self.check(self.L.wdm_get_energy(self.h, sid, ptr(v, C.c_double), ptr(cr, C.c_int32)))
+        return v, cr.astype(bool)
+
+    def stats(self, sid):
+        s, f, vd = C.c_int64(), C.c_int64(), C.c_double()
+        self.check(self.L.wdm_get_stats(self.h, sid, C.byref(s), C.byref(f), C.byref(vd)))
+        return dict(step=s.value, nFullRuns=f.value, verletDist=vd.value)
+
+    def cell(self, sid):
+        h = np.zeros(9)
+        g = np.zeros(9)
+        self.check(self.L.wdm_get_cell(self.h, sid, ptr(h, C.c_double), ptr(g, C.c_double)))
+        return h.reshape(3, 3), g.reshape(3, 3)
+
+    def sync(self):
+        self.check(self.L.wdm_sync(self.h))
+
+    def enable_timing(self, on=True):
+        self.check(self.L.wdm_enable_timing(self.h, int(on)))
+
+    def kernel_times(self):
+        cap = 64
+        names = C.create_string_buffer(64 * cap)
+        ms = np.zeros(cap)
+        la = np.zeros(cap, np.int64)
+        n = C.c_int32()
+        self.check(self.L.wdm_kernel_times(self.h, cap, names, ptr(ms, C.c_double), ptr(la, C.c_int64), C.byref(n)))
+        out = {}
+        raw = names.raw
+        for i in range(n.value):
+            nm = raw[64 * i:64 * i + 64].split(b"\0", 1)[0].decode()
+            out[nm] = (float(ms[i]), int(la[i]))
+        return out
+
+    def relax(self, sid, iters):
+        self.check(self.L.wdm_relax_overlaps(self.h, sid, int(iters)))
+
+    @staticmethod
+    def footprint(n_par, n_node, contacts_per_particle):
+        return lib().wdm_scene_footprint(n_par, n_node, contacts_per_particle)

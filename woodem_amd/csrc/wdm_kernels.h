@@ -1,0 +1,1057 @@
+// HIP kernels of the DEM timestep (gfx950).  Included by wdm_lib.hip only.
+//
+// Layout in HBM (one scene):
+//   node arrays, N entries, 32-byte records:  posr = (x, y, z, radius|halfThick),
+//     velm = (vx, vy, vz, mass), angI = (wx, wy, wz, inertia), ori = (w, x, y, z),
+//     force/torque = (x, y, z, 0), lastPos = (x, y, z, 0); flags u32.
+//   particle arrays, P entries: ptype i32 (shape | axis<<4 | (sense+1)<<6), mask u32,
+//     mat i32, facet nodes int4; stored bounds bmin/bmax double4.
+//   contacts, C entries sorted by key = (min id << 32 | max id): ids int2 (functor
+//     order), info u32, packed cellDist i32, state T[NSTATE][cap] component-major,
+//     gather record gfc[2c] = (global force, 0), gfc[2c+1] = (contact point, uN).
+//   CSR: off[P+1], entries u32 (contact index | isB << 31), per particle sorted by
+//     the other particle's id.
+#pragma once
+#include "wdm_math.h"
+
+#define WDM_BLOCK 256
+
+// contact info bits
+#define CI_TYPE_MASK 3u
+#define CI_SS 0u
+#define CI_WS 1u
+#define CI_FS 2u
+#define CI_NONE 3u
+#define CI_REAL 4u
+#define CI_PENDING 8u
+#define CI_OVERLAP 16u
+#define CI_DEAD 32u
+
+// contact state components
+#define S_T 0      // trsf, 9 entries row-major
+#define S_F 9      // local force, 3 entries
+#define S_P 12     // phys, 5 entries
+#define NSTATE 17
+
+// device error bits
+#define ERR_NAN 1u
+#define ERR_OVERSIZE 2u
+#define ERR_CELLDIST 4u
+
+// node flags
+#define F_DOF_ALL 63u
+#define F_DOF_RXRYRZ 56u
+#define F_CLUMPED 64u
+#define F_CLUMP 128u
+#define F_ENERGY_SKIP 256u
+#define F_GRAVITY_SKIP 512u
+#define F_DAMPING_SKIP 2048u
+
+// energy slots (same order as the ABI)
+#define E_KIN 0
+#define E_ELAST 1
+#define E_PLAST 2
+#define E_GRAV 3
+#define E_DAMP 4
+#define E_VISCN 5
+#define E_VISCT 6
+#define E_BROKEN 7
+#define E_N 8
+
+__device__ __forceinline__ int ptShape(int32_t t) { return t & 15; }
+__device__ __forceinline__ int ptAxis(int32_t t) { return (t >> 4) & 3; }
+__device__ __forceinline__ int ptSense(int32_t t) { return ((t >> 6) & 3) - 1; }
+
+WDM_HD int32_t packCd(int a, int b, int c) { return (a & 1023) | ((b & 1023) << 10) | ((c & 1023) << 20); }
+WDM_HD int unpack10(int32_t v, int sh) { int x = (v >> sh) & 1023; return x >= 512 ? x - 1024 : x; }
+
+struct NodeView {
+    double4 *posr, *velm, *angI, *ori, *force, *torque, *lastPos;
+    uint32_t* flags;
+    uint8_t* checked;  // node belongs to a particle with a bound (Verlet check)
+    int nn;
+};
+struct PartView {
+    int32_t* ptype;
+    uint32_t* mask;
+    int32_t* mat;
+    int4* fnodes;
+    double4 *bmin, *bmax;
+    int np;
+};
+template <typename T>
+struct ConView {
+    uint64_t* key;
+    int2* ids;
+    uint32_t* info;
+    int32_t* cd;
+    T* st;
+    double4* gfc;
+    int64_t cap;
+};
+
+// arguments shared by the step kernels
+struct StepArgs {
+    double dt, damping;
+    double g[3];
+    int hasGravity;
+    int periodic, homoDeform;  // homoDeform: -1 aperiodic
+    int reallyTrackEnergy, trackEnergy;
+    // Leapfrog (pkg/dem/Leapfrog.cpp:113-126)
+    M3 ImLL4hInv, IpLL4h, LmL, dGradV, midGradV, nextGradV, gradV;
+    double deltaSpinVec[3], spinVec[3];
+    // cell
+    M3 hSize, shiftVelM;  // shiftVelM: gradV*pprevHsize (GRADV2) or gradV*hSize (HOMO_VEL*), else 0
+    double cellSize[3];
+    // collider
+    double verletD2;
+    int checkVerlet;
+    // contact
+    double distFactor;
+    int law, noSlip, noBreak, noFrict, noAttraction, trsfRenorm;
+    int64_t step;
+    uint32_t loneMask;
+    double viscAlpha;  // -sqrt(5) log(en) / sqrt(log(en)^2 + pi^2), or 0 when en is outside (0,1)
+    double poisson, gamma;
+    int hertzDamped;
+    double4* mat;  // material table (density, young, ktDivKn, tanPhi)
+};
+
+// ---------------- block reductions (fixed tree: deterministic) ----------------
+// bits: (1 << energy slot) for "slot created" (EnergyTracker::add, core/EnergyTracker.cpp:10-17),
+// VERLET_BIT for the Verlet trigger.
+#define VERLET_BIT 0x80000000u
+template <int NV>
+__device__ void blockSumWrite(double (&v)[NV], uint32_t orbits, double* partial, uint32_t* orOut, uint32_t* creOut) {
+    __shared__ double sh[NV][WDM_BLOCK];
+    __shared__ uint32_t shb[WDM_BLOCK];
+    int t = threadIdx.x;
+    for (int k = 0; k < NV; k++) sh[k][t] = v[k];
+    shb[t] = orbits;
+    __syncthreads();
+    for (int s = WDM_BLOCK / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            for (int k = 0; k < NV; k++) sh[k][t] = sh[k][t] + sh[k][t + s];
+            shb[t] |= shb[t + s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        if (partial) for (int k = 0; k < NV; k++) partial[(size_t)blockIdx.x * NV + k] = sh[k][0];
+        uint32_t b = shb[0];
+        if ((b & VERLET_BIT) && orOut) atomicOr(orOut, VERLET_BIT);
+        if ((b & ~VERLET_BIT) && creOut) atomicOr(creOut, b & ~VERLET_BIT);
+    }
+}
+
+// ---------------- Leapfrog::run, pkg/dem/Leapfrog.cpp:103-255 ----------------
+// Per node: accelerations with blocked DOFs, 2nd-order non-viscous damping, GRADV2
+// velocity map, spherical rotation, energy terms, and the Verlet displacement check
+// of InsertionSortCollider::updateBboxes_doFullRun (:340-350) on the new position.
+// Partial energies per block: [nonviscDamp, grav, kinetic].
+__global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(StepArgs a, NodeView n, double* epart, uint32_t* orOut, uint32_t* creOut) {
+    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+    double ev[3] = {0., 0., 0.};
+    uint32_t bits = 0;
+    if (i < n.nn) {
+        uint32_t fl = n.flags[i];
+        double4 pr = n.posr[i];
+        D3 pos = xyz(pr);
+        if (!(fl & (F_CLUMPED | F_CLUMP))) {
+            const double dt = a.dt, damping = a.damping;
+            double4 vm = n.velm[i], aw = n.angI[i];
+            D3 vel = xyz(vm), angVel = xyz(aw);
+            const double mass = vm.w, inertia = aw.w;
+            D3 f = xyz(n.force[i]), t = xyz(n.torque[i]);
+            D3 g = mk3(a.g[0], a.g[1], a.g[2]);
+            bool damp = (damping != 0. && !(fl & F_DAMPING_SKIP));
+            bool blockedNone = (fl & F_DOF_ALL) == 0, blockedAll = (fl & F_DOF_ALL) == F_DOF_ALL;
+            if (a.reallyTrackEnergy) {
+                if (damp && !(fl & F_ENERGY_SKIP)) {
+                    ev[0] = dot3(abs3(vel), abs3(f)) * damping * dt + dot3(abs3(angVel), abs3(t)) * damping * dt;
+                    if (ev[0] != 0.) bits |= 1u << E_DAMP;  // ZeroDontCreate
+                }
+                if (a.hasGravity && !(fl & F_GRAVITY_SKIP)) {
+                    double gr = 0;
+                    if (blockedNone) gr = -dot3(g, vel) * mass * dt;
+                    else {
+                        if (!(fl & 1u)) gr -= g.x * vel.x * mass * dt;
+                        if (!(fl & 2u)) gr -= g.y * vel.y * mass * dt;
+                        if (!(fl & 4u)) gr -= g.z * vel.z * mass * dt;
+                    }
+                    ev[1] = gr;
+                    bits |= 1u << E_GRAV;
+                }
+            }
+            D3 pfv = mk3(0., 0., 0.), pfav = mk3(0., 0., 0.);
+            D3 la = mk3(0., 0., 0.), aa = mk3(0., 0., 0.);
+            const bool gradv2 = a.homoDeform == 4;
+            if (!blockedAll) {
+                if (blockedNone) la = f / mass;
+                else {
+                    if (!(fl & 1u)) la.x += f.x / mass;
+                    if (!(fl & 2u)) la.y += f.y / mass;
+                    if (!(fl & 4u)) la.z += f.z / mass;
+                }
+                if (a.periodic) {
+                    switch (a.homoDeform) {  // Cell::pprevFluctVel, core/Cell.cpp:36-50
+                        case 2: case 3: pfv = vel - mv3(a.gradV, pos); break;
+                        case 4: pfv = vel - mv3(a.gradV, pos - vel * (dt / 2)); break;
+                        default: pfv = vel;
+                    }
+                    pfav = gradv2 ? angVel - mk3(a.spinVec[0], a.spinVec[1], a.spinVec[2]) : angVel;
+                } else {
+                    pfv = vel;
+                    pfav = angVel;
+                }
+                if (damp) {
+                    la.x *= 1 - damping * sgn(f.x * (pfv.x + 0.5 * dt * la.x));
+                    la.y *= 1 - damping * sgn(f.y * (pfv.y + 0.5 * dt * la.y));
+                    la.z *= 1 - damping * sgn(f.z * (pfv.z + 0.5 * dt * la.z));
+                }
+                if (gradv2) vel = mv3(a.ImLL4hInv, (mv3(a.LmL, pos) + mv3(a.IpLL4h, vel)) + la * dt);
+                else vel = vel + la * dt;
+                if (inertia != 0.) {
+                    if (blockedNone) aa = t / inertia;
+                    else {
+                        if (!(fl & 8u)) aa.x += t.x / inertia;
+                        if (!(fl & 16u)) aa.y += t.y / inertia;
+                        if (!(fl & 32u)) aa.z += t.z / inertia;
+                    }
+                    if (damp) {
+                        aa.x *= 1 - damping * sgn(t.x * (pfav.x + 0.5 * dt * aa.x));
+                        aa.y *= 1 - damping * sgn(t.y * (pfav.y + 0.5 * dt * aa.y));
+                        aa.z *= 1 - damping * sgn(t.z * (pfav.z + 0.5 * dt * aa.z));
+                    }
+                    angVel = angVel + aa * dt;
+                    if (gradv2) angVel = angVel - mk3(a.deltaSpinVec[0], a.deltaSpinVec[1], a.deltaSpinVec[2]);
+                }
+            } else {
+                if (gradv2) vel = mv3(a.ImLL4hInv, mv3(a.LmL, pos) + mv3(a.IpLL4h, vel));
+            }
+            if (a.homoDeform >= 0 && !gradv2) {  // applyPeriodicCorrections, :258-272
+                if (a.homoDeform == 2 || a.homoDeform == 3) {
+                    if (a.homoDeform == 3) vel = vel + mv3(a.midGradV, vel - la * (dt / 2.)) * dt;
+                    vel = vel + mv3(a.dGradV, pos);
+                } else if (a.homoDeform == 1) pos = pos + mv3(a.nextGradV, pos) * dt;
+            }
+            if (a.reallyTrackEnergy && !(fl & F_ENERGY_SKIP)) {  // doKineticEnergy, :75-97
+                D3 cfv = pfv + la * (.5 * dt), cfav = pfav + aa * (.5 * dt);
+                double Etrans = .5 * mass * sqn3(cfv);
+                double Erot = 0.5 * dot3(cfav, mk3(inertia * cfav.x, inertia * cfav.y, inertia * cfav.z));
+                if (isnan(Erot) || isinf(inertia)) Erot = 0;
+                ev[2] = Etrans + Erot;
+                bits |= 1u << E_KIN;
+            }
+            pos = pos + vel * dt;  // leapfrogTranslate, :275-278
+            double4 q = n.ori[i];  // leapfrogSphericalRotate, :280-288
+            if (!iszero3(angVel)) {
+                double angle = wsqrt(sqn3(angVel));
+                D3 ax = angVel / angle;
+                double ha = 0.5 * (angle * dt);
+                double s = sin(ha), cw = cos(ha);
+                double qx = s * ax.x, qy = s * ax.y, qz = s * ax.z;
+                double4 r;
+                r.x = ((cw * q.x - qx * q.y) - qy * q.z) - qz * q.w;
+                r.y = ((cw * q.y + qx * q.x) + qy * q.w) - qz * q.z;
+                r.z = ((cw * q.z + qy * q.x) + qz * q.y) - qx * q.w;
+                r.w = ((cw * q.w + qz * q.x) + qx * q.z) - qy * q.y;
+                q = r;
+            }
+            double qn = wsqrt(((q.x * q.x + q.y * q.y) + q.z * q.z) + q.w * q.w);
+            q.x = q.x / qn; q.y = q.y / qn; q.z = q.z / qn; q.w = q.w / qn;
+            n.ori[i] = q;
+            n.posr[i] = make_double4(pos.x, pos.y, pos.z, pr.w);
+            n.velm[i] = make_double4(vel.x, vel.y, vel.z, mass);
+            n.angI[i] = make_double4(angVel.x, angVel.y, angVel.z, inertia);
+        }
+        if (a.checkVerlet && n.checked[i]) {
+            D3 d = xyz(n.lastPos[i]) - pos;
+            if (sqn3(d) > a.verletD2) bits |= VERLET_BIT;
+        }
+    }
+    blockSumWrite<3>(ev, bits, a.reallyTrackEnergy ? epart : nullptr, orOut, creOut);
+}
+
+// sum per-block partials sequentially (block order) into the energy accumulators
+__global__ void k_energy_finalize(const double* part, int nblocks, int nv, const int* slots, double* E) {
+    int k = threadIdx.x;
+    if (k >= nv) return;
+    double s = 0;
+    for (int b = 0; b < nblocks; b++) s += part[(size_t)b * nv + k];
+    E[slots[k]] += s;
+}
+
+// ---------------- bounds: Bo1_Sphere_Aabb (pkg/dem/Sphere.cpp:50-79), Bo1_Wall_Aabb
+// (pkg/dem/Wall.cpp:24-33), Bo1_Facet_Aabb (pkg/dem/Facet.cpp:177-200), enlarged by
+// verletDist and with node positions saved (InsertionSortCollider.cpp:376-414) ----------------
+__global__ void __launch_bounds__(WDM_BLOCK) k_bounds(NodeView n, PartView p, double distFactor, double vd) {
+    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+    if (i >= p.np) return;
+    int32_t pt = p.ptype[i];
+    int sh = ptShape(pt);
+    const double inf = __builtin_inf();
+    D3 mn, mx;
+    if (sh == 1) {
+        double4 pr = n.posr[i];
+        double hs = (distFactor > 0 ? distFactor : 1.) * pr.w;
+        mn = mk3(pr.x - hs, pr.y - hs, pr.z - hs);
+        mx = mk3(pr.x + hs, pr.y + hs, pr.z + hs);
+        n.lastPos[i] = make_double4(pr.x, pr.y, pr.z, 0.);
+    } else if (sh == 2) {
+        double4 pr = n.posr[i];
+        int ax = ptAxis(pt);
+        mn = mk3(-inf, -inf, -inf);
+        mx = mk3(inf, inf, inf);
+        double c = ax == 0 ? pr.x : (ax == 1 ? pr.y : pr.z);
+        setcomp3(mn, ax, c);
+        setcomp3(mx, ax, c);
+        n.lastPos[i] = make_double4(pr.x, pr.y, pr.z, 0.);
+    } else if (sh == 3) {
+        int4 fn = p.fnodes[i];
+        double ht = n.posr[i].w;
+        double4 x0 = n.posr[fn.x], x1 = n.posr[fn.y], x2 = n.posr[fn.z];
+        mn = mk3(x0.x - ht, x0.y - ht, x0.z - ht);
+        mx = mk3(x0.x + ht, x0.y + ht, x0.z + ht);
+        double4 xs[2] = {x1, x2};
+        for (int k = 0; k < 2; k++) {
+            mn.x = fmin(mn.x, xs[k].x - ht); mx.x = fmax(mx.x, xs[k].x + ht);
+            mn.y = fmin(mn.y, xs[k].y - ht); mx.y = fmax(mx.y, xs[k].y + ht);
+            mn.z = fmin(mn.z, xs[k].z - ht); mx.z = fmax(mx.z, xs[k].z + ht);
+        }
+        n.lastPos[fn.x] = make_double4(x0.x, x0.y, x0.z, 0.);
+        n.lastPos[fn.y] = make_double4(x1.x, x1.y, x1.z, 0.);
+        n.lastPos[fn.z] = make_double4(x2.x, x2.y, x2.z, 0.);
+    } else {
+        return;
+    }
+    if (vd > 0) {
+        mx.x += vd; mx.y += vd; mx.z += vd;
+        mn.x -= vd; mn.y -= vd; mn.z -= vd;
+    }
+    p.bmin[i] = make_double4(mn.x, mn.y, mn.z, 0.);
+    p.bmax[i] = make_double4(mx.x, mx.y, mx.z, 0.);
+}
+
+// extent reduction over binned (finite-bound) particles: min corner, max corner, max extent
+__global__ void __launch_bounds__(WDM_BLOCK) k_extent(PartView p, const int32_t* binned, int nb, double* part) {
+    __shared__ double sh[7][WDM_BLOCK];
+    int t = threadIdx.x;
+    int i = blockIdx.x * WDM_BLOCK + t;
+    const double inf = __builtin_inf();
+    double v[7] = {inf, inf, inf, -inf, -inf, -inf, 0.};
+    if (i < nb) {
+        int id = binned[i];
+        double4 a = p.bmin[id], b = p.bmax[id];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = b.x; v[4] = b.y; v[5] = b.z;
+        v[6] = fmax(fmax(b.x - a.x, b.y - a.y), b.z - a.z);
+    }
+    for (int k = 0; k < 7; k++) sh[k][t] = v[k];
+    __syncthreads();
+    for (int s = WDM_BLOCK / 2; s > 0; s >>= 1) {
+        if (t < s) {
+            for (int k = 0; k < 3; k++) sh[k][t] = fmin(sh[k][t], sh[k][t + s]);
+            for (int k = 3; k < 7; k++) sh[k][t] = fmax(sh[k][t], sh[k][t + s]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) for (int k = 0; k < 7; k++) part[(size_t)blockIdx.x * 7 + k] = sh[k][0];
+}
+
+struct GridArgs {
+    double origin[3];
+    double w[3];   // cell width per axis
+    int n[3];      // cells per axis
+    int periodic;
+    double L[3];   // periodic cell size
+};
+
+__device__ __forceinline__ int cellCoord(const GridArgs& g, int d, double x) {
+    int c;
+    if (g.periodic) {
+        double u = x / g.L[d];
+        double f = u - floor(u);
+        c = (int)(f * g.n[d]);
+    } else {
+        c = (int)floor((x - g.origin[d]) / g.w[d]);
+    }
+    return c < 0 ? 0 : (c >= g.n[d] ? g.n[d] - 1 : c);
+}
+
+__global__ void __launch_bounds__(WDM_BLOCK) k_cell_keys(PartView p, GridArgs g, const int32_t* binned, int nb, uint32_t* keys, int32_t* vals) {
+    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+    if (i >= nb) return;
+    int id = binned[i];
+    double4 a = p.bmin[id];
+    int cx = cellCoord(g, 0, a.x), cy = cellCoord(g, 1, a.y), cz = cellCoord(g, 2, a.z);
+    keys[i] = (uint32_t)((cz * g.n[1] + cy) * g.n[0] + cx);
+    vals[i] = id;
+}
+
+__global__ void k_fill_i32(int32_t* a, int64_t n, int32_t v) {
+    int64_t i = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void __launch_bounds__(WDM_BLOCK) k_cell_ranges(const uint32_t* keys, int nb, int32_t* cstart, int32_t* cend) {
+    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+    if (i >= nb) return;
+    uint32_t k = keys[i];
+    if (i == 0 || keys[i - 1] != k) cstart[k] = i;
+    if (i == nb - 1 || keys[i + 1] != k) cend[k] = i + 1;
+}
+
+// InsertionSortCollider::cellWrap / cellWrapRel (InsertionSortCollider.cpp:650-660)
+__device__ __forceinline__ double cellWrapP(double x, double x0, double x1, int& period) {
+    double xNorm = (x - x0) / (x1 - x0);
+    period = (int)floor(xNorm);
+    return x0 + (xNorm - period) * (x1 - x0);
+}
+__device__ __forceinline__ double cellWrapRel(double x, double x0, double x1) {
+    double xNorm = (x - x0) / (x1 - x0);
+    return (xNorm - floor(xNorm)) * (x1 - x0);
+}
+// spatialOverlap (:52-58) and spatialOverlapPeri (:903-957) on the stored bounds
+__device__ __forceinline__ bool overlapAperiodic(const double4& a0, const double4& a1, const double4& b0, const double4& b1) {
+    return (a0.x <= b1.x) && (a1.x >= b0.x) && (a0.y <= b1.y) && (a1.y >= b0.y) && (a0.z <= b1.z) && (a1.z >= b0.z);
+}
+__device__ __forceinline__ bool overlapPeri(const double4& a0, const double4& a1, const double4& b0, const double4& b1,
+                                            const double* L, int periods[3], uint32_t* err) {
+    const double am[3] = {a0.x, a0.y, a0.z}, aM[3] = {a1.x, a1.y, a1.z};
+    const double bm[3] = {b0.x, b0.y, b0.z}, bM[3] = {b1.x, b1.y, b1.z};
+    for (int ax = 0; ax < 3; ax++) {
+        double dim = L[ax];
+        double m1 = am[ax], m2 = bm[ax];
+        if (isinf(m1) || isinf(m2)) { periods[ax] = 0; continue; }
+        double wMn = (cellWrapRel(m1, m2, m2 + dim) < cellWrapRel(m2, m1, m1 + dim)) ? m2 : m1;
+        int pmn1, pmx1, pmn2, pmx2;
+        double mn1 = cellWrapP(am[ax], wMn, wMn + dim, pmn1), mx1 = cellWrapP(aM[ax], wMn, wMn + dim, pmx1);
+        double mn2 = cellWrapP(bm[ax], wMn, wMn + dim, pmn2), mx2 = cellWrapP(bM[ax], wMn, wMn + dim, pmx2);
+        if ((pmn1 != pmx1) || (pmn2 != pmx2)) { atomicOr(err, ERR_OVERSIZE); return false; }
+        periods[ax] = pmn1 - pmn2;
+        if (!(mn1 <= mx2 && mx1 >= mn2)) return false;
+    }
+    return true;
+}
+
+// functor order: Wall/Facet first, otherwise the lower id
+__device__ __forceinline__ void canonical(int i, int j, int si, int sj, int& A, int& B, int& type) {
+    if (si == 1 && sj == 1) { A = i < j ? i : j; B = i < j ? j : i; type = CI_SS; return; }
+    if ((si == 2 || si == 3) && sj == 1) { A = i; B = j; type = si == 2 ? CI_WS : CI_FS; return; }
+    if ((sj == 2 || sj == 3) && si == 1) { A = j; B = i; type = sj == 2 ? CI_WS : CI_FS; return; }
+    A = i < j ? i : j; B = i < j ? j : i; type = CI_NONE;
+}
+
+// Collider::mayCollide, pkg/dem/Collision.cpp:19-41 (no clumps on this path)
+__device__ __forceinline__ bool mayCollideD(const PartView& p, int a, int b, int sa, int sb, uint32_t loneMask) {
+    if (a == b || sa == 0 || sb == 0) return false;
+    uint32_t ma = p.mask[a], mb = p.mask[b];
+    if (!(ma & mb)) return false;
+    if ((ma & mb & loneMask) != 0) return false;
+    if (sa == 3 || sb == 3) {
+        int na[3], nb[3], ka, kb;
+        if (sa == 3) { int4 f = p.fnodes[a]; na[0] = f.x; na[1] = f.y; na[2] = f.z; ka = 3; } else { na[0] = a; ka = 1; }
+        if (sb == 3) { int4 f = p.fnodes[b]; nb[0] = f.x; nb[1] = f.y; nb[2] = f.z; kb = 3; } else { nb[0] = b; kb = 1; }
+        for (int x = 0; x < ka; x++) for (int y = 0; y < kb; y++) if (na[x] == nb[y]) return false;
+    }
+    return true;
+}
+
+// Candidate pairs: every binned particle (sorted by cell) scans the 27 neighbouring
+// cells for partners with a larger id and tests every unbinned particle (walls).
+// The exact test is the collider's own closed-interval 3-axis overlap of the stored
+// bounds plus mayCollide, so the set equals the reference's; the grid only prunes.
+// mode 0 counts per thread, mode 1 writes keys and packed periods at offs[i].
+template <int MODE>
+__global__ void __launch_bounds__(WDM_BLOCK) k_pairs(PartView p, GridArgs g, const uint32_t* ckeys, const int32_t* cvals, int nb,
+                                                    const int32_t* cstart, const int32_t* cend, const int32_t* unb, int nunb,
+                                                    uint32_t loneMask, int32_t* counts, const int64_t* offs, uint64_t* okeys,
+                                                    int32_t* operiods, uint32_t* err) {
+    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+    if (i >= nb) return;
+    int id = cvals[i];
+    uint32_t ck = ckeys[i];
+    int cx = ck % g.n[0], cy = (ck / g.n[0]) % g.n[1], cz = ck / (g.n[0] * g.n[1]);
+    double4 a0 = p.bmin[id], a1 = p.bmax[id];
+    int sa = ptShape(p.ptype[id]);
+    int cnt = 0;
+    int64_t o = MODE ? offs[i] : 0;
+    int rx = g.n[0] >= 3 ? 1 : 0, ry = g.n[1] >= 3 ? 1 : 0, rz = g.n[2] >= 3 ? 1 : 0;
+    for (int dz = -rz; dz <= rz; dz++) {
+        int z = cz + dz;
+        if (g.periodic) z = (z + g.n[2]) % g.n[2];
+        else if (z < 0 || z >= g.n[2]) continue;
+        for (int dy = -ry; dy <= ry; dy++) {
+            int y = cy + dy;
+            if (g.periodic) y = (y + g.n[1]) % g.n[1];
+            else if (y < 0 || y >= g.n[1]) continue;
+            for (int dx = -rx; dx <= rx; dx++) {
+                int x = cx + dx;
+                if (g.periodic) x = (x + g.n[0]) % g.n[0];
+                else if (x < 0 || x >= g.n[0]) continue;
+                uint32_t cell = (uint32_t)((z * g.n[1] + y) * g.n[0] + x);
+                int s = cstart[cell];
+                if (s < 0) continue;
+                int e = cend[cell];
+                for (int k = s; k < e; k++) {
+                    int j = cvals[k];
+                    if (j <= id) continue;
+                    int sb = ptShape(p.ptype[j]);
+                    int A, B, ty;
+                    canonical(id, j, sa, sb, A, B, ty);
+                    double4 A0 = p.bmin[A], A1 = p.bmax[A], B0 = p.bmin[B], B1 = p.bmax[B];
+                    int per[3] = {0, 0, 0};
+                    bool ov = g.periodic ? overlapPeri(A0, A1, B0, B1, g.L, per, err) : overlapAperiodic(A0, A1, B0, B1);
+                    if (!ov || !mayCollideD(p, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), loneMask)) continue;
+                    if (MODE) {
+                        okeys[o + cnt] = ((uint64_t)(uint32_t)id << 32) | (uint32_t)j;
+                        operiods[o + cnt] = packCd(per[0], per[1], per[2]);
+                    }
+                    cnt++;
+                }
+            }
+        }
+    }
+    for (int u = 0; u < nunb; u++) {
+        int j = unb[u];
+        int sb = ptShape(p.ptype[j]);
+        int A, B, ty;
+        canonical(id, j, sa, sb, A, B, ty);
+        double4 A0 = p.bmin[A], A1 = p.bmax[A], B0 = p.bmin[B], B1 = p.bmax[B];
+        int per[3] = {0, 0, 0};
+        bool ov = g.periodic ? overlapPeri(A0, A1, B0, B1, g.L, per, err) : overlapAperiodic(A0, A1, B0, B1);
+        if (!ov || !mayCollideD(p, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), loneMask)) continue;
+        if (MODE) {
+            int lo = id < j ? id : j, hi = id < j ? j : id;
+            okeys[o + cnt] = ((uint64_t)(uint32_t)lo << 32) | (uint32_t)hi;
+            operiods[o + cnt] = packCd(per[0], per[1], per[2]);
+        }
+        cnt++;
+    }
+    if (!MODE) counts[i] = cnt;
+}
+
+__device__ __forceinline__ int64_t lowerBound64(const uint64_t* a, int64_t n, uint64_t k) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (a[mid] < k) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// old contacts that stay only because they are real (their bounds no longer overlap)
+__global__ void __launch_bounds__(WDM_BLOCK) k_flag_old(const uint64_t* okey, const uint32_t* oinfo, int64_t nold, const uint64_t* ckey,
+                                                       int64_t ncand, int32_t* flag) {
+    int64_t j = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
+    if (j >= nold) return;
+    uint32_t inf = oinfo[j];
+    int f = 0;
+    if ((inf & CI_REAL) && !(inf & CI_DEAD)) {
+        uint64_t k = okey[j];
+        int64_t lb = lowerBound64(ckey, ncand, k);
+        f = (lb < ncand && ckey[lb] == k) ? 0 : 1;
+    }
+    flag[j] = f;
+}
+
+template <typename T>
+__device__ __forceinline__ void copyContact(const ConView<T>& src, int64_t s, const ConView<T>& dst, int64_t d) {
+    dst.ids[d] = src.ids[s];
+    dst.cd[d] = src.cd[s];
+    for (int k = 0; k < NSTATE; k++) dst.st[(size_t)k * dst.cap + d] = src.st[(size_t)k * src.cap + s];
+    dst.gfc[2 * d] = src.gfc[2 * s];
+    dst.gfc[2 * d + 1] = src.gfc[2 * s + 1];
+}
+
+// merged list = candidates (bounds overlap) U old real contacts (InsertionSortCollider keeps
+// real contacts, :92-96); an old contact that is not dead keeps its state, ids order and cellDist
+template <typename T>
+__global__ void __launch_bounds__(WDM_BLOCK) k_emit_cand(PartView p, ConView<T> oldc, int64_t nold, const int64_t* ooRank,
+                                                        const uint64_t* ckey, const int32_t* cper, int64_t ncand, ConView<T> nc) {
+    int64_t i = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
+    if (i >= ncand) return;
+    uint64_t k = ckey[i];
+    int64_t lb = lowerBound64(oldc.key, nold, k);
+    int64_t d = i + ooRank[lb];
+    nc.key[d] = k;
+    if (lb < nold && oldc.key[lb] == k && !(oldc.info[lb] & CI_DEAD)) {
+        copyContact(oldc, lb, nc, d);
+        nc.info[d] = (oldc.info[lb] & ~(CI_PENDING | CI_DEAD)) | CI_OVERLAP;
+    } else {
+        int lo = (int)(k >> 32), hi = (int)(k & 0xffffffffu);
+        int A, B, ty;
+        canonical(lo, hi, ptShape(p.ptype[lo]), ptShape(p.ptype[hi]), A, B, ty);
+        nc.ids[d] = make_int2(A, B);
+        int32_t pc = cper[i];  // periods computed in canonical order
+        nc.cd[d] = pc;
+        nc.info[d] = (uint32_t)ty | CI_OVERLAP;
+        for (int s = 0; s < NSTATE; s++) nc.st[(size_t)s * nc.cap + d] = (T)0;
+        nc.gfc[2 * d] = make_double4(0., 0., 0., 0.);
+        nc.gfc[2 * d + 1] = make_double4(0., 0., 0., 0.);
+    }
+}
+template <typename T>
+__global__ void __launch_bounds__(WDM_BLOCK) k_emit_oor(ConView<T> oldc, int64_t nold, const int32_t* flag, const int64_t* ooRank,
+                                                       const uint64_t* ckey, int64_t ncand, ConView<T> nc) {
+    int64_t j = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
+    if (j >= nold || !flag[j]) return;
+    uint64_t k = oldc.key[j];
+    int64_t d = lowerBound64(ckey, ncand, k) + ooRank[j];
+    nc.key[d] = k;
+    copyContact(oldc, j, nc, d);
+    nc.info[d] = oldc.info[j] & ~(CI_PENDING | CI_DEAD | CI_OVERLAP);
+}
+
+// CSR entries: key (particle << 32 | other), value contact index | isB << 31
+__global__ void __launch_bounds__(WDM_BLOCK) k_csr_entries(const int2* ids, int64_t nc, uint64_t* ek, uint32_t* ev) {
+    int64_t c = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
+    if (c >= nc) return;
+    int2 id = ids[c];
+    ek[2 * c] = ((uint64_t)(uint32_t)id.x << 32) | (uint32_t)id.y;
+    ev[2 * c] = (uint32_t)c;
+    ek[2 * c + 1] = ((uint64_t)(uint32_t)id.y << 32) | (uint32_t)id.x;
+    ev[2 * c + 1] = (uint32_t)c | 0x80000000u;
+}
+__global__ void __launch_bounds__(WDM_BLOCK) k_csr_offsets(const uint64_t* ek, int64_t ne, int np, int64_t* off) {
+    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+    if (i > np) return;
+    off[i] = lowerBound64(ek, ne, (uint64_t)(uint32_t)i << 32);
+}
+
+// ---------------- ContactLoop::run, pkg/dem/ContactLoop.cpp:93-259 ----------------
+// One thread per contact: pending removal decided by the stored-bounds overlap
+// (ContactContainer::removePending), geometry (Cg2_*_L6Geom + handleSpheresLikeContact,
+// pkg/dem/L6Geom.cpp:51-217), physics on creation (Cp2_*), law (Law2_*), global force.
+
+template <typename T>
+struct Geo {
+    V3T<T> normal;
+    D3 contPt;
+    T uN, r1, r2;
+    D3 pos1, pos2, vel1, vel2, angVel1, angVel2;
+};
+
+// L6Geom::setInitialLocalCoords, pkg/dem/L6Geom.cpp:27-38
+template <typename T>
+__device__ __forceinline__ void initialLocalCoords(const V3T<T>& locX, T* tr) {
+    V3T<T> e = (fabs(locX.y) < fabs(locX.z)) ? mk3<T>(0, 1, 0) : mk3<T>(0, 0, 1);
+    V3T<T> locY = cross3(locX, e);
+    locY = locY - locX * dot3(locY, locX);
+    locY = normalized3(locY);
+    V3T<T> locZ = cross3(locX, locY);
+    tr[0] = locX.x; tr[1] = locY.x; tr[2] = locZ.x;
+    tr[3] = locX.y; tr[4] = locY.y; tr[5] = locZ.y;
+    tr[6] = locX.z; tr[7] = locY.z; tr[8] = locZ.z;
+}
+template <typename T> __device__ __forceinline__ V3T<T> trCol(const T* tr, int j) { return mk3<T>(tr[j], tr[3 + j], tr[6 + j]); }
+template <typename T> __device__ __forceinline__ void trSetCol(T* tr, int j, const V3T<T>& c) { tr[j] = c.x; tr[3 + j] = c.y; tr[6 + j] = c.z; }
+
+template <typename T, int LAW>
+__global__ void __launch_bounds__(WDM_BLOCK) k_contacts(StepArgs a, NodeView n, PartView p, ConView<T> cv, int64_t nc, double* epart,
+                                                       uint32_t* creOut, uint32_t* err) {
+    int64_t c = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
+    double ev[5] = {0., 0., 0., 0., 0.};  // elast, plast, viscN, viscT, broken
+    uint32_t bits = 0;
+    if (c < nc) do {
+        uint32_t info = cv.info[c];
+        if (info & CI_DEAD) break;
+        uint32_t ty = info & CI_TYPE_MASK;
+        if (info & CI_PENDING) {  // ContactContainer::removePending (ContactContainer.hpp:78-93)
+            info &= ~CI_PENDING;
+            if (!(info & CI_OVERLAP)) info |= CI_DEAD;
+            cv.info[c] = info;
+            if (info & CI_DEAD) break;
+        }
+        if (ty == CI_NONE) break;
+        const bool real = (info & CI_REAL) != 0;
+        const int2 id = cv.ids[c];
+        const double dt = a.dt;
+        double4 pa = n.posr[id.x], pb = n.posr[id.y];
+        D3 shift2 = mk3(0., 0., 0.);
+        int32_t cdp = 0;
+        if (a.periodic) {
+            cdp = cv.cd[c];
+            shift2 = mv3(a.hSize, mk3((double)unpack10(cdp, 0), (double)unpack10(cdp, 10), (double)unpack10(cdp, 20)));
+        }
+        T tr[9];
+        if (real) for (int k = 0; k < 9; k++) tr[k] = cv.st[(size_t)(S_T + k) * cv.cap + c];
+        Geo<T> G;
+        if (ty == CI_SS) {  // Cg2_Sphere_Sphere_L6Geom::go, pkg/dem/Sphere.cpp:88-109
+            D3 p2 = xyz(pb) + shift2;
+            D3 relPos = p2 - xyz(pa);
+            double r1 = pa.w, r2 = pb.w;
+            double df = fabs(a.distFactor) * (r1 + r2);
+            double sq = sqn3(relPos);
+            if (sq - df * df > 0 && !real) break;
+            double dist = wsqrt(sq);
+            double uN = dist - (r1 + r2);
+            D3 normal = relPos / dist;
+            G.normal = cvt3<T>(normal);
+            G.contPt = xyz(pa) + normal * (r1 + 0.5 * uN);
+            G.uN = (T)uN; G.r1 = (T)r1; G.r2 = (T)r2;
+            G.pos1 = xyz(pa); G.pos2 = p2;
+            double4 va = n.velm[id.x], vb = n.velm[id.y], wa = n.angI[id.x], wb = n.angI[id.y];
+            G.vel1 = xyz(va); G.vel2 = xyz(vb); G.angVel1 = xyz(wa); G.angVel2 = xyz(wb);
+        } else if (ty == CI_WS) {  // Cg2_Wall_Sphere_L6Geom::go, pkg/dem/Wall.cpp:45-79
+            int32_t wt = p.ptype[id.x];
+            int ax = ptAxis(wt), sn = ptSense(wt);
+            double rad = pb.w;
+            D3 wallPos = xyz(pa);
+            D3 spherePos = xyz(pb) + shift2;
+            double dist = comp3(spherePos, ax) - comp3(wallPos, ax);
+            if (!real && fabs(dist) > rad) break;
+            D3 contPt = spherePos;
+            setcomp3(contPt, ax, comp3(wallPos, ax));
+            D3 normal = mk3(0., 0., 0.);
+            double nax;
+            if (sn == 0) nax = real ? (double)tr[3 * ax] : (dist > 0 ? 1. : -1.);
+            else nax = (sn == 1 ? 1. : -1);
+            setcomp3(normal, ax, nax);
+            double uN = nax * dist - rad;
+            G.normal = cvt3<T>(normal);
+            G.contPt = contPt;
+            G.uN = (T)uN; G.r1 = (T)(-rad); G.r2 = (T)rad;
+            G.pos1 = wallPos; G.pos2 = spherePos;
+            double4 va = n.velm[id.x], vb = n.velm[id.y], wa = n.angI[id.x], wb = n.angI[id.y];
+            G.vel1 = xyz(va); G.vel2 = xyz(vb); G.angVel1 = xyz(wa); G.angVel2 = xyz(wb);
+        } else {  // Cg2_Facet_Sphere_L6Geom::go, pkg/dem/Facet.cpp:235-288
+            int4 fn = p.fnodes[id.x];
+            double halfThick = pa.w, rad = pb.w;
+            D3 n0 = xyz(n.posr[fn.x]), n1 = xyz(n.posr[fn.y]), n2 = xyz(n.posr[fn.z]);
+            D3 sC = xyz(pb) + shift2;
+            D3 nn = cross3(n1 - n0, n2 - n0);
+            D3 fNormal = normalized3(nn);
+            double planeDist = dot3(sC - n0, fNormal);
+            if (fabs(planeDist) > (rad + halfThick) && !real) break;
+            D3 fC = sC - fNormal * planeDist;
+            D3 o0 = cross3(n1 - n0, nn), o1 = cross3(n2 - n1, nn), o2 = cross3(n0 - n2, nn);
+            double l0 = dot3(o0, fC - n0), l1 = dot3(o1, fC - n1), l2 = dot3(o2, fC - n2);
+            int w = (l0 > 0 ? 1 : 0) + (l1 > 0 ? 2 : 0) + (l2 > 0 ? 4 : 0);
+            D3 contPt;
+            auto seg = [](const D3& P, const D3& A, const D3& B) -> D3 {  // CompUtils::closestSegmentPt, CompUtils.cpp:57-63
+                D3 BA = B - A;
+                if (sqn3(BA) == 0.) return A;
+                double u = (dot3(P, BA) - dot3(A, BA)) / sqn3(BA);
+                return A + BA * fmin(1., fmax(0., u));
+            };
+            if (w == 0) contPt = fC;
+            else if (w == 1) contPt = seg(fC, n0, n1);
+            else if (w == 2) contPt = seg(fC, n1, n2);
+            else if (w == 4) contPt = seg(fC, n2, n0);
+            else if (w == 3) contPt = n1;
+            else if (w == 5) contPt = n0;
+            else if (w == 6) contPt = n2;
+            else { atomicOr(err, ERR_NAN); break; }
+            D3 normal = sC - contPt;
+            double rs = rad + halfThick;
+            if (sqn3(normal) > rs * rs && !real) break;
+            double dist = wsqrt(sqn3(normal));
+            normal = normal / dist;
+            if (halfThick > 0) contPt = contPt + normal * halfThick;
+            double uN = dist - rad - halfThick;
+            // Facet::interpolatePtLinAngVel (Facet.cpp:158-174) with CompUtils::triangleBarycentrics (CompUtils.cpp:92-110)
+            D3 vB = n1 - n0, vC = n2 - n0, vX = contPt - n0;
+            double m00 = dot3(vB, vB), m01 = dot3(vC, vB), m10 = dot3(vB, vC), m11 = dot3(vC, vC);
+            double invdet = 1. / (m00 * m11 - m10 * m01);
+            double i00 = m11 * invdet, i01 = -m01 * invdet, i10 = -m10 * invdet, i11 = m00 * invdet;
+            double rr0 = dot3(vX, vB), rr1 = dot3(vX, vC);
+            double u = i00 * rr0 + i01 * rr1, v = i10 * rr0 + i11 * rr1;
+            double b0 = 1 - u - v, b1 = u, b2 = v;
+            D3 v0 = xyz(n.velm[fn.x]), v1 = xyz(n.velm[fn.y]), v2 = xyz(n.velm[fn.z]);
+            D3 linVel = (v0 * b0 + v1 * b1) + v2 * b2;
+            D3 av = (cross3(n0 - contPt, v0) + cross3(n1 - contPt, v1)) + cross3(n2 - contPt, v2);
+            G.normal = cvt3<T>(normal);
+            G.contPt = contPt;
+            G.uN = (T)uN; G.r1 = (T)fmax(halfThick, rad); G.r2 = (T)rad;
+            G.pos1 = contPt; G.pos2 = sC;
+            G.vel1 = linVel; G.angVel1 = av;
+            double4 vb = n.velm[id.y], wb = n.angI[id.y];
+            G.vel2 = xyz(vb); G.angVel2 = xyz(wb);
+        }
+        // handleSpheresLikeContact, pkg/dem/L6Geom.cpp:51-217
+        V3T<T> gvel, gangVel;
+        T lens0 = 0, lens1 = 0, contA = 0;
+        if (!real) {
+            initialLocalCoords(G.normal, tr);
+            lens0 = fabs(G.r1) + G.uN / 2;
+            lens1 = fabs(G.r2) + G.uN / 2;
+            if (lens0 < 0) lens0 = lens1;
+            if (lens1 < 0) lens1 = lens0;
+            T rr = (G.r1 > 0 && G.r2 > 0) ? fmin(G.r1, G.r2) : (G.r1 > 0 ? G.r1 : G.r2);
+            contA = (T)3.14159265358979323846 * (rr * rr);
+            gvel = mk3<T>(0, 0, 0);
+            gangVel = mk3<T>(0, 0, 0);
+        } else {
+            const V3T<T> currNormal = G.normal, prevNormal = trCol(tr, 0);
+            const double4 cprev = cv.gfc[2 * c + 1];
+            const D3 prevContPt = xyz(cprev);
+            D3 shiftVel2 = mk3(0., 0., 0.);
+            if (a.periodic) shiftVel2 = mv3(a.shiftVelM, mk3((double)unpack10(cdp, 0), (double)unpack10(cdp, 10), (double)unpack10(cdp, 20)));
+            V3T<T> normRotVec = cross3(prevNormal, currNormal);
+            V3T<T> midNormal = (prevNormal + currNormal) * (T).5;
+            if (wsqrt(sqn3(midNormal)) == 0) midNormal = currNormal;
+            else midNormal = normalized3(midNormal);
+            V3T<T> normTwistVec = ((midNormal * (T)dt) * (T).5) * dot3(midNormal, cvt3<T>(G.angVel1 + G.angVel2));
+            V3T<T> rt = normRotVec + normTwistVec;
+            V3T<T> pc1 = trCol(tr, 1);
+            V3T<T> mid1 = pc1 - cross3(pc1, rt) / (T)2.;
+            V3T<T> mid2 = cross3(midNormal, mid1);
+            V3T<T> cur1 = pc1 - cross3(mid1, rt);
+            V3T<T> cur0 = currNormal;
+            if (a.trsfRenorm > 0 && (a.step % a.trsfRenorm) == 0) {
+                cur0 = normalized3(cur0);
+                cur1 = cur1 - cur0 * dot3(cur1, cur0);
+                cur1 = normalized3(cur1);
+            }
+            V3T<T> cur2 = cross3(cur0, cur1);
+            D3 midContPt = (prevContPt + G.contPt) * .5;
+            D3 midPos1 = G.pos1 - G.vel1 * (dt / 2.);
+            D3 midPos2 = G.pos2 - (G.vel2 + shiftVel2) * (dt / 2.);
+            D3 c1x = midContPt - midPos1, c2x = midContPt - midPos2;
+            V3T<T> relVel;
+            if constexpr (sizeof(T) == 8) {
+                relVel = ((G.vel2 + shiftVel2) + cross3(G.angVel2, c2x)) - (G.vel1 + cross3(G.angVel1, c1x));
+            } else {
+                // mixed precision: translational difference in FP64, rotational terms in FP32
+                V3T<T> dv = cvt3<T>((G.vel2 + shiftVel2) - G.vel1);
+                relVel = dv + (cross3(cvt3<T>(G.angVel2), cvt3<T>(c2x)) - cross3(cvt3<T>(G.angVel1), cvt3<T>(c1x)));
+            }
+            trSetCol(tr, 0, cur0);
+            trSetCol(tr, 1, cur1);
+            trSetCol(tr, 2, cur2);
+            // midTrsf^T * v: columns (midNormal, mid1, mid2) dotted with v
+            gvel = mk3<T>(dot3(midNormal, relVel), dot3(mid1, relVel), dot3(mid2, relVel));
+            V3T<T> dw = cvt3<T>(G.angVel2 - G.angVel1);
+            gangVel = mk3<T>(dot3(midNormal, dw), dot3(mid1, dw), dot3(mid2, dw));
+        }
+        // physics (on creation) and law
+        T F[3];
+        T ph[5];
+        bool keep = true;
+        const T uN = G.uN;
+        if (!real) {
+            double4 m1 = a.mat[p.mat[id.x]], m2 = a.mat[p.mat[id.y]];
+            if (LAW == 0) {  // Cp2_FrictMat_FrictPhys::updateFrictPhys, pkg/dem/FrictMat.cpp:21-44
+                T l0 = lens0, l1 = lens1, A = contA;
+                T kn = (T)1 / ((T)1 / ((T)m1.y * A / l0) + (T)1 / ((T)m2.y * A / l1));
+                ph[0] = kn;
+                ph[1] = (T).5 * ((T)m1.z + (T)m2.z) * kn;
+                ph[2] = fmin((T)m1.w, (T)m2.w);
+                ph[3] = 0; ph[4] = 0;
+            } else {  // Cp2_FrictMat_HertzPhys::go, pkg/dem/Hertz.cpp:9-50
+                T r1 = lens0, r2 = lens1;
+                T E1 = (T)m1.y, E2 = (T)m2.y, nu1 = (T)a.poisson, nu2 = (T)a.poisson;
+                T K = (T)(4 / 3.) * (T)1. / (((T)1 - nu1 * nu1) / E1 + ((T)1 - nu2 * nu2) / E2);
+                T R = (T)1. / ((T)1. / r1 + (T)1. / r2);
+                T kn0 = K * wsqrt(R);
+                T G1 = E1 / ((T)2 * ((T)1 + nu1)), G2 = E2 / ((T)2 * ((T)1 + nu2));
+                T Gm = (T).5 * (G1 + G2);
+                T nu = (T).5 * (nu1 + nu2);
+                ph[0] = K; ph[1] = R;
+                ph[2] = fmin((T)m1.w, (T)m2.w);
+                ph[3] = (T)2 * wsqrt((T)4 * R) * Gm / ((T)2 - nu);
+                T asq = 0;
+                if (a.hertzDamped) {
+                    double ma = n.velm[id.x].w, mb = n.velm[id.y].w;
+                    double mbar = (ma <= 0 && mb > 0) ? mb : ((ma > 0 && mb <= 0) ? ma : (ma * mb) / (ma + mb));
+                    asq = fmax((T)0., (T)a.viscAlpha * wsqrt((T)mbar * kn0));
+                }
+                ph[4] = asq;
+            }
+            F[0] = F[1] = F[2] = 0;
+        } else {
+            for (int k = 0; k < 5; k++) ph[k] = cv.st[(size_t)(S_P + k) * cv.cap + c];
+            for (int k = 0; k < 3; k++) F[k] = cv.st[(size_t)(S_F + k) * cv.cap + c];
+        }
+        if (LAW == 0) {  // Law2_L6Geom_FrictPhys_IdealElPl::go, pkg/dem/IdealElPl.cpp:12-122
+            const T kn = ph[0], kt = ph[1], tanPhi = ph[2];
+            if (uN > 0 && !a.noBreak) {
+                if (a.noSlip && a.trackEnergy) {
+                    T Fn = kn * uN;
+                    T Ft0 = F[1] + (T)dt * kt * gvel.y, Ft1 = F[2] + (T)dt * kt * gvel.z;
+                    ev[4] = (double)((T).5 * (Fn * Fn / kn + (Ft0 * Ft0 + Ft1 * Ft1) / kt));
+                    bits |= 1u << E_BROKEN;
+                }
+                keep = false;
+            } else {
+                F[0] = kn * uN;
+                if (a.noFrict || tanPhi == 0.) { F[1] = 0; F[2] = 0; }
+                else {
+                    F[1] += (T)dt * kt * gvel.y;
+                    F[2] += (T)dt * kt * gvel.z;
+                    T maxFt = fabs(F[0]) * tanPhi;
+                    T ftsq = F[1] * F[1] + F[2] * F[2];
+                    if (ftsq > maxFt * maxFt && !a.noSlip) {
+                        T FtNorm = wsqrt(ftsq);
+                        T ratio = maxFt / FtNorm;
+                        if (FtNorm == (T)0.) ratio = 0.;
+                        if (a.trackEnergy) {
+                            T dissip = (kt != (T)0. ? (maxFt) * (FtNorm - maxFt) / kt : (T)0.);
+                            if (dissip != (T)0.) { ev[1] = (double)dissip; bits |= 1u << E_PLAST; }
+                        }
+                        F[1] *= ratio; F[2] *= ratio;
+                    }
+                    if (isnan(F[0]) || isnan(F[1]) || isnan(F[2])) atomicOr(err, ERR_NAN);
+                }
+                if (a.trackEnergy) {
+                    T ftsq = F[1] * F[1] + F[2] * F[2];
+                    T el = (T)0.5 * ((F[0] * F[0]) / kn + (kt != (T)0. ? ftsq / kt : (T)0.));
+                    if (isnan(el)) el = 0.;
+                    ev[0] = (double)el;
+                    bits |= 1u << E_ELAST;
+                }
+            }
+        } else {  // Law2_L6Geom_HertzPhys_DMT::go, pkg/dem/Hertz.cpp:58-182, alpha==0 branch
+            const T K = ph[0], R = ph[1], tanPhi = ph[2], kt0 = ph[3], asq = ph[4];
+            const T velN = gvel.x;
+            T kn0 = K * wsqrt(R);
+            if (uN > 0) keep = false;
+            else {
+                T s = wsqrt(-uN);
+                T Fne = -kn0 * ((s * s) * s);
+                T Fna = (T)(4 * 3.14159265358979323846) * R * (T)a.gamma;
+                T eta = (asq > 0 ? asq * wsqrt(wsqrt(fmax(-uN, (T)0.))) : (T)0.);
+                T Fnv = eta * velN;
+                if (a.noAttraction && Fne + Fnv > 0) Fnv = -Fne;
+                T Fn = Fne + Fna + Fnv;
+                F[0] = Fn;
+                if (a.trackEnergy) { T v = Fnv * velN * (T)dt; if (v != (T)0.) { ev[2] = (double)v; bits |= 1u << E_VISCN; } }
+                T kt = kt0 * wsqrt(uN < 0 ? -uN : (T)0);
+                T Ft0 = (T)dt * kt * gvel.y, Ft1 = (T)dt * kt * gvel.z;
+                T maxFt = fmax((T)0., Fn) * tanPhi;
+                T ftsq = Ft0 * Ft0 + Ft1 * Ft1;
+                if (ftsq > maxFt * maxFt) {
+                    T FtNorm = wsqrt(ftsq);
+                    T ratio = maxFt / FtNorm;
+                    if (a.trackEnergy) {
+                        T v = ((T).5 * (FtNorm - maxFt) + maxFt) * (FtNorm - maxFt) / kt;
+                        if (v != (T)0.) { ev[1] = (double)v; bits |= 1u << E_PLAST; }
+                    }
+                    Ft0 *= ratio; Ft1 *= ratio;
+                } else {
+                    Ft0 += eta * gvel.y; Ft1 += eta * gvel.z;
+                    if (a.trackEnergy) { T v = eta * (gvel.y * gvel.y + gvel.z * gvel.z) * (T)dt; if (v != (T)0.) { ev[3] = (double)v; bits |= 1u << E_VISCT; } }
+                }
+                F[1] = Ft0; F[2] = Ft1;
+                if (a.trackEnergy && uN < 0) {
+                    T s5 = (((s * s) * s) * s) * s;
+                    T el = kn0 * (T)(2 / 5.) * s5 + (T)0.5 * (Ft0 * Ft0 + Ft1 * Ft1) / kt;
+                    if (!isnan(el)) ev[0] = (double)el;
+                    bits |= 1u << E_ELAST;
+                }
+            }
+        }
+        if (!keep) {  // ContactContainer::requestRemoval -> Contact::reset (ContactContainer.cpp:164-171)
+            cv.info[c] = (info & ~CI_REAL) | CI_PENDING;
+            cv.gfc[2 * c] = make_double4(0., 0., 0., 0.);
+            break;
+        }
+        // store geometry/physics state
+        for (int k = 0; k < 9; k++) cv.st[(size_t)(S_T + k) * cv.cap + c] = tr[k];
+        for (int k = 0; k < 3; k++) cv.st[(size_t)(S_F + k) * cv.cap + c] = F[k];
+        if (!real) {
+            for (int k = 0; k < 5; k++) cv.st[(size_t)(S_P + k) * cv.cap + c] = ph[k];
+            cv.info[c] = info | CI_REAL;
+        }
+        // global force trsf*F (Contact::getForceTorqueBranch, Contact.cpp:24-34)
+        V3T<T> fl = mk3<T>(F[0], F[1], F[2]);
+        V3T<T> fg = mk3<T>(dot3(mk3<T>(tr[0], tr[1], tr[2]), fl), dot3(mk3<T>(tr[3], tr[4], tr[5]), fl), dot3(mk3<T>(tr[6], tr[7], tr[8]), fl));
+        cv.gfc[2 * c] = make_double4((double)fg.x, (double)fg.y, (double)fg.z, 0.);
+        cv.gfc[2 * c + 1] = make_double4(G.contPt.x, G.contPt.y, G.contPt.z, (double)uN);
+    } while (0);
+    if (a.trackEnergy) blockSumWrite<5>(ev, bits, epart, nullptr, creOut);
+}
+
+// ---------------- force application (ContactLoop::applyForceUninodal, ContactLoop.cpp:261-267)
+// plus the force reset of Leapfrog (Leapfrog.cpp:240-245): one thread per node; a
+// uninodal particle's node sums its contacts in ascending order of the other id ----------------
+__global__ void __launch_bounds__(WDM_BLOCK) k_gather(StepArgs a, NodeView n, PartView p, const int64_t* off, const uint32_t* ent,
+                                                     const double4* gfc, const int32_t* cd) {
+    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+    if (i >= n.nn) return;
+    uint32_t fl = n.flags[i];
+    double m = n.velm[i].w;
+    D3 F = (a.hasGravity && !(fl & F_GRAVITY_SKIP)) ? mk3(a.g[0] * m, a.g[1] * m, a.g[2] * m) : mk3(0., 0., 0.);
+    D3 T = mk3(0., 0., 0.);
+    if (i < p.np) {
+        int sh = ptShape(p.ptype[i]);
+        if (sh == 1 || sh == 2) {
+            D3 pos = xyz(n.posr[i]);
+            int64_t e0 = off[i], e1 = off[i + 1];
+            for (int64_t e = e0; e < e1; e++) {
+                uint32_t v = ent[e];
+                uint32_t c = v & 0x7fffffffu;
+                bool isB = (v >> 31) != 0;
+                double4 fg = gfc[2 * (size_t)c], cp = gfc[2 * (size_t)c + 1];
+                double sign = isB ? -1. : 1.;
+                D3 Fs = mk3(fg.x * sign, fg.y * sign, fg.z * sign);
+                D3 ref = pos;
+                if (isB && a.periodic) {
+                    int32_t q = cd[c];
+                    ref = pos + mv3(a.hSize, mk3((double)unpack10(q, 0), (double)unpack10(q, 10), (double)unpack10(q, 20)));
+                }
+                D3 xc = xyz(cp) - ref;
+                F = F + Fs;
+                T = T + cross3(xc, Fs);
+            }
+        }
+    }
+    n.force[i] = make_double4(F.x, F.y, F.z, 0.);
+    n.torque[i] = make_double4(T.x, T.y, T.z, 0.);
+}
+
+// ---------------- scene generation: Jacobi overlap relaxation ----------------
+// Each sphere moves by half of every overlap with its neighbours (found through the
+// same cell grid); all displacements are computed from the old positions, so the
+// result does not depend on thread order.
+__global__ void __launch_bounds__(WDM_BLOCK) k_relax(NodeView n, PartView p, GridArgs g, const uint32_t* ckeys, const int32_t* cvals, int nb,
+                                                    const int32_t* cstart, const int32_t* cend, double4* disp) {
+    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+    if (i >= nb) return;
+    int id = cvals[i];
+    uint32_t ck = ckeys[i];
+    int cx = ck % g.n[0], cy = (ck / g.n[0]) % g.n[1], cz = ck / (g.n[0] * g.n[1]);
+    double4 pi = n.posr[id];
+    D3 d = mk3(0., 0., 0.);
+    int rx = g.n[0] >= 3 ? 1 : 0, ry = g.n[1] >= 3 ? 1 : 0, rz = g.n[2] >= 3 ? 1 : 0;
+    for (int dz = -rz; dz <= rz; dz++) {
+        int z = cz + dz;
+        if (g.periodic) z = (z + g.n[2]) % g.n[2]; else if (z < 0 || z >= g.n[2]) continue;
+        for (int dy = -ry; dy <= ry; dy++) {
+            int y = cy + dy;
+            if (g.periodic) y = (y + g.n[1]) % g.n[1]; else if (y < 0 || y >= g.n[1]) continue;
+            for (int dx = -rx; dx <= rx; dx++) {
+                int x = cx + dx;
+                if (g.periodic) x = (x + g.n[0]) % g.n[0]; else if (x < 0 || x >= g.n[0]) continue;
+                uint32_t cell = (uint32_t)((z * g.n[1] + y) * g.n[0] + x);
+                int s = cstart[cell];
+                if (s < 0) continue;
+                int e = cend[cell];
+                for (int k = s; k < e; k++) {
+                    int j = cvals[k];
+                    if (j == id) continue;
+                    double4 pj = n.posr[j];
+                    D3 r = xyz(pi) - xyz(pj);
+                    if (g.periodic) {
+                        r.x -= g.L[0] * rint(r.x / g.L[0]);
+                        r.y -= g.L[1] * rint(r.y / g.L[1]);
+                        r.z -= g.L[2] * rint(r.z / g.L[2]);
+                    }
+                    double dist = sqrt(sqn3(r));
+                    double ov = (pi.w + pj.w) - dist;
+                    if (ov > 0 && dist > 0) d = d + r * (0.5 * ov / dist);
+                }
+            }
+        }
+    }
+    disp[id] = make_double4(d.x, d.y, d.z, 0.);
+}
+__global__ void k_relax_apply(NodeView n, const int32_t* binned, int nb, const double4* disp) {
+    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+    if (i >= nb) return;
+    int id = binned[i];
+    double4 pr = n.posr[id];
+    double4 d = disp[id];
+    n.posr[id] = make_double4(pr.x + d.x, pr.y + d.y, pr.z + d.z, pr.w);
+}

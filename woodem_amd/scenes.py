@@ -1,0 +1,102 @@
+"""Seeded generators for the benchmark scenes (as DemArrays + engine settings).
+
+box_lattice: 10,000 monodisperse spheres in a 6-wall box under gravity
+  (25x20x20 lattice, 11 mm spacing, +-0.4 mm jitter, r=5 mm, rho=3000,
+  FrictMat E=1e7, ktDivKn=.2, tanPhi=.5, IdealElPl, dt=0.3 p-wave, damping .2).
+  Placement (not fixed by the workload text): lattice centred in x and y, first
+  layer 6 mm above the floor (1 mm gap); box (0,0,0)-(0.3,0.25,0.25).
+periodic_hertz: N spheres, lognormal radii (median 1 mm, sigma .3, clipped to
+  [0.5, 3] mm) in a periodic cube at 0.60 solid fraction, Hertz E=7e10, nu=.3,
+  en=.7, tanPhi=.4, Gaussian velocities (std 0.05 m/s), no gravity, cell strain
+  rate 1e-3/s (isotropic).  Not fixed by the workload text and chosen here:
+  density 2650 kg/m3, dt = 0.3 r_min / sqrt(E/rho); the overlaps of the uniform
+  placement are relaxed on the device (wdm_relax_overlaps); particle ids are
+  then renumbered along a Morton curve so that neighbours have nearby ids.
+"""
+import math
+
+import numpy as np
+
+from . import soa as _soa
+
+
+def box_lattice(nx=25, ny=20, nz=20, seed=1, spacing=11e-3, jitter=0.4e-3, radius=5e-3, density=3000., young=1e7,
+                ktDivKn=.2, tanPhi=.5, box=((0., 0., 0.), (.3, .25, .25)), gap=1e-3):
+    rng = np.random.default_rng(seed)
+    mn, mx = np.asarray(box[0]), np.asarray(box[1])
+    span = np.array([(nx - 1) * spacing, (ny - 1) * spacing, (nz - 1) * spacing])
+    origin = np.array([mn[0] + .5 * (mx[0] - mn[0] - span[0]), mn[1] + .5 * (mx[1] - mn[1] - span[1]), mn[2] + radius + gap])
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    pos = origin + spacing * np.stack([ii.ravel(), jj.ravel(), kk.ravel()], 1).astype(np.float64)
+    pos = pos + rng.uniform(-jitter, jitter, size=pos.shape)
+    a = _soa.spheres_arrays(pos, radius, density=density, young=young, ktDivKn=ktDivKn, tanPhi=tanPhi)
+    walls = []
+    for sense, coord in ((1, mn), (-1, mx)):
+        for ax in (0, 1, 2):
+            walls.append((float(coord[ax]), ax, sense))
+    a = _soa.append_walls(a, walls)
+    dt = 0.3 * radius / math.sqrt(young / density)
+    a.meta.update(dict(kind="box_lattice", dt=dt, gravity=(0., 0., -9.81), damping=.2, law="IdealElPl", periodic=False,
+                       n_spheres=nx * ny * nz, seed=seed))
+    return a
+
+
+def lognormal_radii(n, rng, median=1e-3, sigma=.3, lo=.5e-3, hi=3e-3):
+    r = np.exp(math.log(median) + sigma * rng.standard_normal(n))
+    return np.clip(r, lo, hi)
+
+
+def morton_order(pos, L):
+    """Permutation sorting points along a 3-D Morton (Z-order) curve of 2^10 cells per axis."""
+    q = np.floor((np.mod(pos, L) / L) * 1024).astype(np.int64)
+    q = np.clip(q, 0, 1023)
+
+    def spread(v):
+        v = v & 0x3FF
+        v = (v | (v << 16)) & 0x30000FF
+        v = (v | (v << 8)) & 0x300F00F
+        v = (v | (v << 4)) & 0x30C30C3
+        v = (v | (v << 2)) & 0x9249249
+        return v
+
+    code = spread(q[:, 0]) | (spread(q[:, 1]) << 1) | (spread(q[:, 2]) << 2)
+    return np.argsort(code, kind="stable")
+
+
+def periodic_hertz(n=1_000_000, seed=2, phi=.60, density=2650., young=7e10, poisson=.3, en=.7, tanPhi=.4, velStd=.05,
+                   strainRate=1e-3):
+    """Initial state before relaxation (positions uniform, overlapping)."""
+    rng = np.random.default_rng(seed)
+    r = lognormal_radii(n, rng)
+    vol = float(np.sum((4. / 3.) * math.pi * r ** 3))
+    L = (vol / phi) ** (1. / 3.)
+    pos = rng.uniform(0, L, size=(n, 3))
+    vel = rng.normal(0, velStd, size=(n, 3))
+    a = _soa.spheres_arrays(pos, r, density=density, young=young, ktDivKn=.2, tanPhi=tanPhi, vel=vel)
+    dt = 0.3 * float(r.min()) / math.sqrt(young / density)
+    a.meta.update(dict(kind="periodic_hertz", dt=dt, gravity=(0., 0., 0.), damping=0., law="Hertz", poisson=poisson, en=en,
+                       periodic=True, L=L, strainRate=strainRate, n_spheres=n, seed=seed, phi=phi, density=density))
+    return a
+
+
+def relax_and_order(a, ctx, iters=500):
+    """Relax overlaps on the device, then renumber along a Morton curve."""
+    sid = ctx.scene_alloc(a.n_par, a.n_node, a.n_mat)
+    try:
+        ctx.upload(sid, a)
+        L = a.meta["L"]
+        ctx.set_cell(sid, True, np.diag([L, L, L]), np.zeros((3, 3)), np.zeros((3, 3)), 4)
+        ctx.relax(sid, iters)
+        st = ctx.download_nodes(sid, a.n_node)
+    finally:
+        ctx.scene_free(sid)
+    b = a.copy()
+    b.pos = np.mod(st["pos"], a.meta["L"])
+    b = _soa.permute_particles(b, morton_order(b.pos, a.meta["L"]))
+    b.meta["relaxIters"] = iters
+    return b
+
+
+def hertz_params_for(a):
+    """Engine attributes of the periodic Hertz workload, as a dict for woodem_amd.bind."""
+    return dict(dt=a.meta["dt"], damping=0., gravity=(0., 0., 0.), law=1, poisson=a.meta["poisson"], en=a.meta["en"])
