@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-steps/s of the Woo DEM timestep on one MI355X.
+
+Workload (default): 1,000,000 spheres in a periodic cell at 0.60 solid fraction,
+lognormal radii (median 1 mm, sigma 0.3, clipped to [0.5, 3] mm), Hertz contact
+(E=7e10, nu=.3, en=.7, tanPhi=.4), Gaussian velocities (std 0.05 m/s), no
+gravity, isotropic cell strain rate 1e-3/s, FP64.  Overlaps of the uniform
+placement are relaxed with 500 device sweeps before timing.  A step is one
+Scene::doOneStep: Leapfrog, InsertionSortCollider (Verlet-triggered), ContactLoop.
+
+One JSON line on stdout.  --gpus N with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# algorithmic HBM bytes per unit of the contact kernel (FP64 / mixed): real contact,
+# potential (non-real) contact
+BYTES_REAL = {"fp64": 496, "mixed": 324}
+BYTES_POT = {"fp64": 64, "mixed": 60}
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(ctx, n_sample, seconds):
+    """Oracle (C++ restatement, OpenMP) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as OL
+    from woodem_amd import runner, scenes
+    a = scenes.periodic_hertz(n=n_sample)
+    b = scenes.relax_and_order(a, ctx, iters=500)
+    p = runner.params_from_meta(b)
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = min(threads, 16)
+    o = OL.OracleRun(b, p, runner.cell_from_meta(b), nThreads=threads)
+    o.step(10)  # warm-up (includes the initial sort)
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        o.step(5)
+        k += 5
+        el = time.perf_counter() - t0
+        if el > seconds:
+            break
+    return dict(value=n_sample * k / el, unit="particle-steps/s", cores=threads, kind="port",
+                sample="periodic Hertz, %d spheres (same PSD, phi=0.60, relaxed), %d steps after 10 warm-up steps, %.1f s" % (
+                    n_sample, k, el))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--relax", type=int, default=500)
+    ap.add_argument("--latency-steps", type=int, default=200)
+    ap.add_argument("--timing-steps", type=int, default=50)
+    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs the timestep on one GPU only; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from woodem_amd import runner, scenes
+    from woodem_amd._native import PREC_FP64, PREC_MIXED, Context
+
+    ctx = Context(0)
+    t0 = time.time()
+    a = scenes.periodic_hertz(n=args.n)
+    b = scenes.relax_and_order(a, ctx, iters=args.relax)
+    log("scene: %d spheres, L=%.4f m, dt=%.3e s, relaxed in %.1f s" % (b.n_par, b.meta["L"], b.meta["dt"], time.time() - t0))
+    prec = PREC_MIXED if args.precision == "mixed" else PREC_FP64
+    p = runner.params_from_meta(b, precision=prec)
+    run = runner.HipRun(b, p, runner.cell_from_meta(b), ctx=ctx)
+    run.step(args.warmup)
+    ctx.sync()
+    log("warm-up done: %s" % run.stats())
+
+    st0 = run.stats()
+    t1 = time.perf_counter()
+    run.step(args.steps)
+    ctx.sync()
+    el = time.perf_counter() - t1
+    st1 = run.stats()
+    value = b.n_par * args.steps / el
+    log("timed %d steps in %.3f s: %.4g particle-steps/s, full collider runs %d" % (
+        args.steps, el, value, st1["nFullRuns"] - st0["nFullRuns"]))
+
+    # per-step latency (each call synchronises)
+    lat = []
+    for _ in range(args.latency_steps):
+        tt = time.perf_counter()
+        run.step(1)
+        lat.append(time.perf_counter() - tt)
+    lat = np.array(lat) * 1e3
+
+    # kernel timing with HIP events on the library's stream
+    ctx.enable_timing(True)
+    run.step(args.timing_steps)
+    kt = ctx.kernel_times()
+    ctx.enable_timing(False)
+    c = run.contacts()
+    n_con = len(c["idA"])
+    n_real = int(c["isReal"].sum())
+    ms, launches = kt.get("contacts", (float("nan"), 0))
+    avg_ms = ms / max(launches, 1)
+    alg_bytes = BYTES_REAL[args.precision] * n_real + BYTES_POT[args.precision] * (n_con - n_real)
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    kernels = {k: {"avg_ms": v[0] / max(v[1], 1), "launches": v[1]} for k, v in kt.items()}
+    log("kernels: %s" % json.dumps(kernels))
+    log("contacts: %d potential, %d real" % (n_con, n_real))
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(ctx, args.cpu_sample, args.cpu_seconds)
+            log("cpu baseline: %s" % json.dumps(cpu))
+        except Exception as e:  # reported, not fatal for the GPU figure
+            log("cpu baseline failed: %r" % (e,))
+
+    out = {
+        "metric": "particle-steps/s on 1 MI355X (FP64 vs mixed, vs host OpenMP); p50/p99 step latency",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64-state/f32-contact",
+        "data": "synthetic (seeded generator, relaxed on device)",
+        "config": {"workload": "periodic Hertz, %d spheres, phi=0.60, lognormal r (median 1 mm, sigma .3), 1000-step run" % b.n_par,
+                   "n_particles": b.n_par, "precision": args.precision, "dt": b.meta["dt"],
+                   "contacts_potential": n_con, "contacts_real": n_real,
+                   "full_collider_runs_in_timed_region": st1["nFullRuns"] - st0["nFullRuns"]},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)), "samples": len(lat)},
+        "roofline": {"bound": "hbm", "kernel": "k_contacts", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_ms},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    run.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,133 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle on the same inputs.
+
+Bars: the contact set (pairs and cellDist) is bit-exact; one-step forces and
+torques within 1e-12 relative (FP64); trajectories after 200 steps within 1e-9
+relative of the state scale; energies within 1e-9 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+from woodem_amd import runner, scenes, soa
+from woodem_amd._native import LAW_HERTZ_DMT, WdmError
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    scale = max(float(np.abs(b).max()), 1e-300)
+    return float(np.abs(a - b).max()) / scale
+
+
+def same_contacts(cg, co):
+    kg, ko = OL.pair_keys(cg), OL.pair_keys(co)
+    if len(kg) != len(ko) or not np.array_equal(kg, ko):
+        return False
+    # cellDist and functor order per pair
+    og = np.lexsort((np.maximum(cg["idA"], cg["idB"]), np.minimum(cg["idA"], cg["idB"])))
+    oo = np.lexsort((np.maximum(co["idA"], co["idB"]), np.minimum(co["idA"], co["idB"])))
+    return (np.array_equal(cg["idA"][og], co["idA"][oo]) and np.array_equal(cg["cellDist"][og], co["cellDist"][oo])
+            and np.array_equal(cg["isReal"][og], co["isReal"][oo]))
+
+
+def compare(g, o, tol, label):
+    ng, no = g.nodes(), o.nodes()
+    assert same_contacts(g.contacts(), o.contacts()), label + ": contact set differs"
+    for q in ("pos", "vel", "angVel", "force", "torque"):
+        e = rel_err(ng[q], no[q])
+        assert e <= tol, "%s: %s rel err %g > %g" % (label, q, e, tol)
+
+
+def test_box_lattice_parity(hip_ctx):
+    a = scenes.box_lattice(nx=10, ny=8, nz=6)
+    p = runner.params_from_meta(a, trackEnergy=True)
+    cell = runner.cell_from_meta(a)
+    g = runner.HipRun(a, p, cell, ctx=hip_ctx)
+    o = OL.OracleRun(a, p, cell)
+    g.step(1)
+    o.step(1)
+    compare(g, o, 1e-12, "step 1")
+    for n, tol in ((600, 1e-12), (600, 1e-12), (800, 1e-9)):
+        g.step(n)
+        o.step(n)
+        compare(g, o, tol, "step %d" % g.stats()["step"])
+    assert g.stats()["nFullRuns"] == o.stats()["nFullRuns"]
+    assert o.contacts()["isReal"].sum() > 30
+    vg, cg = g.energy()
+    vo, co = o.energy()
+    assert np.array_equal(cg, co)
+    assert np.allclose(vg, vo, rtol=1e-9, atol=1e-12 * np.abs(vo).max())
+    g.close()
+
+
+def test_one_step_forces_dense_box(hip_ctx):
+    """Force/torque after one contact loop on a state with many real contacts (1e-12 relative)."""
+    a = scenes.box_lattice(nx=8, ny=6, nz=6)
+    p = runner.params_from_meta(a)
+    cell = runner.cell_from_meta(a)
+    o = OL.OracleRun(a, p, cell)
+    o.step(2500)  # settle on the CPU, then hand the same state to both
+    st = o.nodes()
+    b = a.copy()
+    b.pos, b.vel, b.angVel, b.ori = st["pos"], st["vel"], st["angVel"], st["ori"]
+    g = runner.HipRun(b, p, cell, ctx=hip_ctx)
+    o2 = OL.OracleRun(b, p, cell)
+    g.step(1)
+    o2.step(1)
+    compare(g, o2, 1e-12, "one step")
+    assert o2.contacts()["isReal"].sum() > 50
+    g.close()
+
+
+def test_periodic_hertz_parity(hip_ctx):
+    a = scenes.periodic_hertz(n=3000)
+    b = scenes.relax_and_order(a, hip_ctx, iters=200)
+    p = runner.params_from_meta(b, trackEnergy=True)
+    cell = runner.cell_from_meta(b)
+    g = runner.HipRun(b, p, cell, ctx=hip_ctx)
+    o = OL.OracleRun(b, p, cell)
+    g.step(1)
+    o.step(1)
+    compare(g, o, 1e-12, "step 1")
+    g.step(199)
+    o.step(199)
+    compare(g, o, 1e-9, "step 200")
+    assert np.allclose(g.cell()[0], o.cell()[0], rtol=0, atol=0)
+    vg, cg = g.energy()
+    vo, co = o.energy()
+    assert np.array_equal(cg, co)
+    assert np.allclose(vg, vo, rtol=1e-9, atol=1e-12 * np.abs(vo).max())
+    assert o.contacts()["isReal"].sum() > 1000
+    g.close()
+
+
+def test_periodic_contacts_cross_boundary(hip_ctx):
+    """Contacts across the periodic boundary get the same cellDist as the oracle's."""
+    rng = np.random.default_rng(11)
+    n, L = 600, .02
+    r = rng.uniform(.8e-3, 1.4e-3, n)
+    pos = rng.uniform(-L, 2 * L, (n, 3))  # positions outside the cell, as Woo allows
+    a = soa.spheres_arrays(pos, r, density=2650, young=7e7, vel=rng.normal(0, .3, (n, 3)))
+    cell = dict(periodic=True, hSize=np.diag([L, L, L]), gradV=np.zeros((3, 3)), nextGradV=1e-2 * np.eye(3), homoDeform=4)
+    p = runner.make_params(dt=2e-6, law=LAW_HERTZ_DMT, poisson=.3, en=.7)
+    g = runner.HipRun(a, p, cell, ctx=hip_ctx)
+    o = OL.OracleRun(a, p, cell)
+    for k in (1, 150, 150):
+        g.step(k)
+        o.step(k)
+        compare(g, o, 1e-9, "step %d" % g.stats()["step"])
+    assert np.abs(o.contacts()["cellDist"]).max() >= 1
+    g.close()
+
+
+def test_nan_force_raises(hip_ctx):
+    """IdealElPl "NaN force in contact" (IdealElPl.cpp:76-80) becomes WDM_E_NAN (forced from the
+    host with a NaN Young's modulus, so that kn and the contact force are NaN)."""
+    a = scenes.box_lattice(nx=4, ny=4, nz=3)
+    a.young[:] = np.nan
+    p = runner.params_from_meta(a)
+    g = runner.HipRun(a, p, runner.cell_from_meta(a), ctx=hip_ctx)
+    with pytest.raises(WdmError) as e:
+        g.step(3000)
+    assert e.value.code == -4
+    g.close()
