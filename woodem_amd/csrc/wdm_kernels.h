@@ -363,7 +363,8 @@ struct GridArgs {
     double w[3];   // cell width per axis
     int n[3];      // cells per axis
     int periodic;
-    double L[3];   // periodic cell size
+    double L[3];     // periodic cell size
+    double invL[3];  // 1/L (prefilter only)
 };
 
 __device__ __forceinline__ int cellCoord(const GridArgs& g, int d, double x) {
@@ -457,11 +458,30 @@ __device__ __forceinline__ bool mayCollideD(const PartView& p, int a, int b, int
     return true;
 }
 
-// Candidate pairs: every binned particle (sorted by cell) scans the 27 neighbouring
-// cells for partners with a larger id and tests every unbinned particle (walls).
-// The exact test is the collider's own closed-interval 3-axis overlap of the stored
-// bounds plus mayCollide, so the set equals the reference's; the grid only prunes.
-// mode 0 counts per thread, mode 1 writes keys and packed periods at offs[i].
+// Candidate pairs.  Thread i takes the i-th binned particle in cell order and visits
+// its own cell (later entries only) and the 13 "forward" neighbour cells, so every
+// unordered pair of binned particles is visited once; it also tests every unbinned
+// particle (walls).  A conservative minimum-image test on bound centres (periodic
+// cells) rejects most candidates cheaply; the decision itself is the collider's own
+// closed-interval 3-axis overlap of the stored bounds plus mayCollide, so the set
+// equals the reference's and the grid only prunes.
+// MODE 0 counts per thread, MODE 1 writes keys and packed periods at offs[i].
+__device__ __forceinline__ bool periPrefilter(const double4& a0, const double4& a1, const double4& b0, const double4& b1, const GridArgs& g) {
+    // centres and half extents; reject when the minimum-image gap exceeds the half
+    // extents by more than 1e-9 L on some axis (rounding here is ~1e-16 L)
+    const double ca[3] = {.5 * (a0.x + a1.x), .5 * (a0.y + a1.y), .5 * (a0.z + a1.z)};
+    const double cb[3] = {.5 * (b0.x + b1.x), .5 * (b0.y + b1.y), .5 * (b0.z + b1.z)};
+    const double ha[3] = {.5 * (a1.x - a0.x), .5 * (a1.y - a0.y), .5 * (a1.z - a0.z)};
+    const double hb[3] = {.5 * (b1.x - b0.x), .5 * (b1.y - b0.y), .5 * (b1.z - b0.z)};
+    for (int d = 0; d < 3; d++) {
+        double L = g.L[d];
+        double dx = cb[d] - ca[d];
+        dx = dx - L * rint(dx * g.invL[d]);
+        if (fabs(dx) > ha[d] + hb[d] + 1e-9 * L) return false;
+    }
+    return true;
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(WDM_BLOCK) k_pairs(PartView p, GridArgs g, const uint32_t* ckeys, const int32_t* cvals, int nb,
                                                     const int32_t* cstart, const int32_t* cend, const int32_t* unb, int nunb,
@@ -477,34 +497,34 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pairs(PartView p, GridArgs g, con
     int cnt = 0;
     int64_t o = MODE ? offs[i] : 0;
     int rx = g.n[0] >= 3 ? 1 : 0, ry = g.n[1] >= 3 ? 1 : 0, rz = g.n[2] >= 3 ? 1 : 0;
-    for (int dz = -rz; dz <= rz; dz++) {
-        int z = cz + dz;
-        if (g.periodic) z = (z + g.n[2]) % g.n[2];
-        else if (z < 0 || z >= g.n[2]) continue;
-        for (int dy = -ry; dy <= ry; dy++) {
-            int y = cy + dy;
-            if (g.periodic) y = (y + g.n[1]) % g.n[1];
-            else if (y < 0 || y >= g.n[1]) continue;
-            for (int dx = -rx; dx <= rx; dx++) {
-                int x = cx + dx;
-                if (g.periodic) x = (x + g.n[0]) % g.n[0];
-                else if (x < 0 || x >= g.n[0]) continue;
+    for (int dz = 0; dz <= rz; dz++) {
+        for (int dy = (dz == 0 ? 0 : -ry); dy <= ry; dy++) {
+            for (int dx = ((dz == 0 && dy == 0) ? 0 : -rx); dx <= rx; dx++) {
+                int z = cz + dz, y = cy + dy, x = cx + dx;
+                if (g.periodic) {
+                    z = (z + g.n[2]) % g.n[2]; y = (y + g.n[1]) % g.n[1]; x = (x + g.n[0]) % g.n[0];
+                } else if (z < 0 || z >= g.n[2] || y < 0 || y >= g.n[1] || x < 0 || x >= g.n[0]) continue;
                 uint32_t cell = (uint32_t)((z * g.n[1] + y) * g.n[0] + x);
                 int s = cstart[cell];
                 if (s < 0) continue;
                 int e = cend[cell];
+                if (dx == 0 && dy == 0 && dz == 0) s = i + 1;  // own cell: later entries only
                 for (int k = s; k < e; k++) {
                     int j = cvals[k];
-                    if (j <= id) continue;
+                    double4 b0 = p.bmin[j], b1 = p.bmax[j];
+                    if (g.periodic) { if (!periPrefilter(a0, a1, b0, b1, g)) continue; }
+                    else if (!overlapAperiodic(a0, a1, b0, b1)) continue;
                     int sb = ptShape(p.ptype[j]);
                     int A, B, ty;
                     canonical(id, j, sa, sb, A, B, ty);
-                    double4 A0 = p.bmin[A], A1 = p.bmax[A], B0 = p.bmin[B], B1 = p.bmax[B];
+                    const bool aIsId = (A == id);
+                    const double4 &A0 = aIsId ? a0 : b0, &A1 = aIsId ? a1 : b1, &B0 = aIsId ? b0 : a0, &B1 = aIsId ? b1 : a1;
                     int per[3] = {0, 0, 0};
-                    bool ov = g.periodic ? overlapPeri(A0, A1, B0, B1, g.L, per, err) : overlapAperiodic(A0, A1, B0, B1);
+                    bool ov = g.periodic ? overlapPeri(A0, A1, B0, B1, g.L, per, err) : true;
                     if (!ov || !mayCollideD(p, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), loneMask)) continue;
                     if (MODE) {
-                        okeys[o + cnt] = ((uint64_t)(uint32_t)id << 32) | (uint32_t)j;
+                        int lo = id < j ? id : j, hi = id < j ? j : id;
+                        okeys[o + cnt] = ((uint64_t)(uint32_t)lo << 32) | (uint32_t)hi;
                         operiods[o + cnt] = packCd(per[0], per[1], per[2]);
                     }
                     cnt++;
@@ -649,10 +669,11 @@ template <typename T> __device__ __forceinline__ void trSetCol(T* tr, int j, con
 
 template <typename T, int LAW>
 __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(StepArgs a, NodeView n, PartView p, ConView<T> cv, int64_t nc, double* epart,
-                                                       uint32_t* creOut, uint32_t* err) {
+                                                       uint32_t* creOut, uint32_t* err, uint64_t* realMask) {
     int64_t c = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
     double ev[5] = {0., 0., 0., 0., 0.};  // elast, plast, viscN, viscT, broken
     uint32_t bits = 0;
+    bool realOut = false;  // contact is real after this step's law
     if (c < nc) do {
         uint32_t info = cv.info[c];
         if (info & CI_DEAD) break;
@@ -959,7 +980,12 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(StepArgs a, NodeView n, 
         V3T<T> fg = mk3<T>(dot3(mk3<T>(tr[0], tr[1], tr[2]), fl), dot3(mk3<T>(tr[3], tr[4], tr[5]), fl), dot3(mk3<T>(tr[6], tr[7], tr[8]), fl));
         cv.gfc[2 * c] = make_double4((double)fg.x, (double)fg.y, (double)fg.z, 0.);
         cv.gfc[2 * c + 1] = make_double4(G.contPt.x, G.contPt.y, G.contPt.z, (double)uN);
+        realOut = true;
     } while (0);
+    // one bit per contact, one 64-bit word per wavefront: the gather skips non-real contacts
+    uint64_t bal = __ballot(realOut);
+    int64_t w0 = blockIdx.x * (int64_t)WDM_BLOCK + (threadIdx.x & ~63);
+    if ((threadIdx.x & 63) == 0 && w0 < nc) realMask[w0 >> 6] = bal;
     if (a.trackEnergy) blockSumWrite<5>(ev, bits, epart, nullptr, creOut);
 }
 
@@ -967,7 +993,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(StepArgs a, NodeView n, 
 // plus the force reset of Leapfrog (Leapfrog.cpp:240-245): one thread per node; a
 // uninodal particle's node sums its contacts in ascending order of the other id ----------------
 __global__ void __launch_bounds__(WDM_BLOCK) k_gather(StepArgs a, NodeView n, PartView p, const int64_t* off, const uint32_t* ent,
-                                                     const double4* gfc, const int32_t* cd) {
+                                                     const double4* gfc, const int32_t* cd, const uint64_t* realMask) {
     int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
     if (i >= n.nn) return;
     uint32_t fl = n.flags[i];
@@ -982,6 +1008,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_gather(StepArgs a, NodeView n, Pa
             for (int64_t e = e0; e < e1; e++) {
                 uint32_t v = ent[e];
                 uint32_t c = v & 0x7fffffffu;
+                if (!((realMask[c >> 6] >> (c & 63)) & 1ull)) continue;  // non-real: no force (ContactLoop.cpp:188)
                 bool isB = (v >> 31) != 0;
                 double4 fg = gfc[2 * (size_t)c], cp = gfc[2 * (size_t)c + 1];
                 double sign = isB ? -1. : 1.;

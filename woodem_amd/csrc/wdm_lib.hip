@@ -195,6 +195,7 @@ struct Scene {
     DBuf<uint64_t> entK, entK2;
     DBuf<uint32_t> entV, entV2;
     DBuf<int64_t> csrOff;
+    DBuf<uint64_t> realMask;
     // energy, flags
     DBuf<double> E, epartL, epartC;
     DBuf<int32_t> eslots;
@@ -224,7 +225,7 @@ struct Scene {
         release(ctx, ckeys); release(ctx, ckeys2); release(ctx, cvals); release(ctx, cvals2); release(ctx, cstart); release(ctx, cend);
         release(ctx, counts); release(ctx, offs); release(ctx, ooFlag64); release(ctx, ooRank); release(ctx, ooFlag); release(ctx, cnt32);
         release(ctx, candK); release(ctx, candK2); release(ctx, candP); release(ctx, candP2); release(ctx, extPart); release(ctx, temp);
-        release(ctx, entK); release(ctx, entK2); release(ctx, entV); release(ctx, entV2); release(ctx, csrOff);
+        release(ctx, entK); release(ctx, entK2); release(ctx, entV); release(ctx, entV2); release(ctx, csrOff); release(ctx, realMask);
         release(ctx, E); release(ctx, epartL); release(ctx, epartC); release(ctx, eslots); release(ctx, orL); release(ctx, ecre); release(ctx, err);
         release(ctx, disp);
         if (hostWords) { (void)hipHostFree(hostWords); hostWords = nullptr; }
@@ -403,7 +404,7 @@ struct InsertionSortCollider {
         nbinned = nb;
         std::memset(&g, 0, sizeof(g));
         g.periodic = S.periodic;
-        for (int d = 0; d < 3; d++) { g.n[d] = 1; g.w[d] = 1; g.L[d] = S.size[d]; }
+        for (int d = 0; d < 3; d++) { g.n[d] = 1; g.w[d] = 1; g.L[d] = S.size[d]; g.invL[d] = 1. / S.size[d]; }
         if (nb == 0) return;
         int nbk = (int)nblk(nb);
         ensure(c, S.extPart, (size_t)nbk * 7);
@@ -576,19 +577,21 @@ struct ContactLoop {
         int64_t nc = S.nc;
         unsigned nbk = nblk(std::max<int64_t>(nc, 1));
         ensure(c, S.epartC, (size_t)nbk * 5 + 5);
+        ensure(c, S.realMask, (size_t)nbk * (WDM_BLOCK / 64) + 1);
+        if (nc == 0) HIPCHK(hipMemsetAsync(S.realMask.p, 0, sizeof(uint64_t), c->stream));
         if (nc > 0) {
             KTimer t(c, "contacts");
             if (S.P.law == WDM_LAW_HERTZ_DMT)
-                hipLaunchKernelGGL((k_contacts<T, 1>), dim3(nbk), dim3(WDM_BLOCK), 0, c->stream, a, S.nview(), S.pview(), b.view(), nc, S.epartC.p, S.ecre.p, S.err.p);
+                hipLaunchKernelGGL((k_contacts<T, 1>), dim3(nbk), dim3(WDM_BLOCK), 0, c->stream, a, S.nview(), S.pview(), b.view(), nc, S.epartC.p, S.ecre.p, S.err.p, S.realMask.p);
             else
-                hipLaunchKernelGGL((k_contacts<T, 0>), dim3(nbk), dim3(WDM_BLOCK), 0, c->stream, a, S.nview(), S.pview(), b.view(), nc, S.epartC.p, S.ecre.p, S.err.p);
+                hipLaunchKernelGGL((k_contacts<T, 0>), dim3(nbk), dim3(WDM_BLOCK), 0, c->stream, a, S.nview(), S.pview(), b.view(), nc, S.epartC.p, S.ecre.p, S.err.p, S.realMask.p);
             checkLaunch();
         }
         if (a.trackEnergy && nc > 0) finalizeEnergy(S, S.epartC, (int)nbk, 5, 3);
         {
             KTimer t(c, "gather");
             hipLaunchKernelGGL(k_gather, dim3(nblk(S.nn)), dim3(WDM_BLOCK), 0, c->stream, a, S.nview(), S.pview(), (const int64_t*)S.csrOff.p,
-                               (const uint32_t*)S.entV2.p, (const double4*)b.gfc.p, (const int32_t*)b.cd.p);
+                               (const uint32_t*)S.entV2.p, (const double4*)b.gfc.p, (const int32_t*)b.cd.p, (const uint64_t*)S.realMask.p);
             checkLaunch();
         }
     }
