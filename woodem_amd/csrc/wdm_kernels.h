@@ -7,7 +7,8 @@
 //   particle arrays, P entries: ptype i32 (shape | axis<<4 | (sense+1)<<6), mask u32,
 //     mat i32, facet nodes int4; stored bounds bmin/bmax double4.
 //   contacts, C entries sorted by key = (min id << 32 | max id): ids int2 (functor
-//     order), info u32, packed cellDist i32, state T[NSTATE][cap] component-major,
+//     order), info u32, packed cellDist i32, state row i32; state T[NSTATE][cap]
+//     component-major, rows of real contacts dense in key order after each full run,
 //     gather record gfc[2c] = (global force, 0), gfc[2c+1] = (contact point, uN).
 //   CSR: off[P+1], entries u32 (contact index | isB << 31), per particle sorted by
 //     the other particle's id.
@@ -26,6 +27,7 @@
 #define CI_PENDING 8u
 #define CI_OVERLAP 16u
 #define CI_DEAD 32u
+#define CI_DORMANT 64u  // cannot touch before the next full run (gap > 2 verletDist at this run)
 
 // contact state components
 #define S_T 0      // trsf, 9 entries row-major
@@ -37,6 +39,7 @@
 #define ERR_NAN 1u
 #define ERR_OVERSIZE 2u
 #define ERR_CELLDIST 4u
+#define ERR_CAPACITY 8u
 
 // node flags
 #define F_DOF_ALL 63u
@@ -84,6 +87,8 @@ struct ConView {
     uint64_t* key;
     int2* ids;
     uint32_t* info;
+    int32_t* slot;     // row of the contact's geometry/physics state, -1 when it has none
+    int32_t* counter;  // next free state row (rows are handed out densely at full runs)
     int32_t* cd;
     T* st;
     double4* gfc;
@@ -102,6 +107,7 @@ struct StepArgs {
     double deltaSpinVec[3], spinVec[3];
     // cell
     M3 hSize, shiftVelM;  // shiftVelM: gradV*pprevHsize (GRADV2) or gradV*hSize (HOMO_VEL*), else 0
+    M3 gatherHSize;       // hSize when the gathered contact forces were computed (ContactLoop of the previous step)
     double cellSize[3];
     // collider
     double verletD2;
@@ -144,12 +150,63 @@ __device__ void blockSumWrite(double (&v)[NV], uint32_t orbits, double* partial,
     }
 }
 
+// ---------------- force application (ContactLoop::applyForceUninodal, ContactLoop.cpp:261-267)
+// on top of the force reset of Leapfrog (Leapfrog.cpp:240-245).  A uninodal particle's
+// node sums its real contacts in ascending order of the other particle's id (CSR
+// order); F = trsf*Flocal and the contact point come from the contact's gather
+// record, the periodic shift of the B side from the cellDist packed in its w slot.
+struct GatherView {
+    const int64_t* off;
+    const uint32_t* ent;
+    const double4* gfc;
+    const uint64_t* realMask;
+};
+__device__ __forceinline__ void gatherForces(const StepArgs& a, const NodeView& n, const PartView& p, const GatherView& gv, int i,
+                                             uint32_t fl, double m, const D3& pos, D3& F, D3& T) {
+    F = (a.hasGravity && !(fl & F_GRAVITY_SKIP)) ? mk3(a.g[0] * m, a.g[1] * m, a.g[2] * m) : mk3(0., 0., 0.);
+    T = mk3(0., 0., 0.);
+    if (i >= p.np) return;
+    int sh = ptShape(p.ptype[i]);
+    if (sh != 1 && sh != 2) return;
+    int64_t e0 = gv.off[i], e1 = gv.off[i + 1];
+    for (int64_t e = e0; e < e1; e++) {
+        uint32_t v = gv.ent[e];
+        uint32_t c = v & 0x7fffffffu;
+        if (!((gv.realMask[c >> 6] >> (c & 63)) & 1ull)) continue;  // non-real: no force (ContactLoop.cpp:188)
+        bool isB = (v >> 31) != 0;
+        double4 fg = gv.gfc[2 * (size_t)c], cp = gv.gfc[2 * (size_t)c + 1];
+        double sign = isB ? -1. : 1.;
+        D3 Fs = mk3(fg.x * sign, fg.y * sign, fg.z * sign);
+        D3 ref = pos;
+        if (isB && a.periodic) {
+            int32_t q = (int32_t)fg.w;
+            ref = pos + mv3(a.gatherHSize, mk3((double)unpack10(q, 0), (double)unpack10(q, 10), (double)unpack10(q, 20)));
+        }
+        D3 xc = xyz(cp) - ref;
+        F = F + Fs;
+        T = T + cross3(xc, Fs);
+    }
+}
+
+__global__ void __launch_bounds__(WDM_BLOCK) k_gather(StepArgs a, NodeView n, PartView p, GatherView gv) {
+    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+    if (i >= n.nn) return;
+    D3 F, T;
+    gatherForces(a, n, p, gv, i, n.flags[i], n.velm[i].w, xyz(n.posr[i]), F, T);
+    n.force[i] = make_double4(F.x, F.y, F.z, 0.);
+    n.torque[i] = make_double4(T.x, T.y, T.z, 0.);
+}
+
 // ---------------- Leapfrog::run, pkg/dem/Leapfrog.cpp:103-255 ----------------
 // Per node: accelerations with blocked DOFs, 2nd-order non-viscous damping, GRADV2
 // velocity map, spherical rotation, energy terms, and the Verlet displacement check
 // of InsertionSortCollider::updateBboxes_doFullRun (:340-350) on the new position.
-// Partial energies per block: [nonviscDamp, grav, kinetic].
-__global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(StepArgs a, NodeView n, double* epart, uint32_t* orOut, uint32_t* creOut) {
+// Partial energies per block: [nonviscDamp, grav, kinetic].  With GATHER the node's
+// force and torque are summed from the contacts of the previous ContactLoop here
+// instead of being read from the force/torque arrays.
+template <bool GATHER>
+__global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(StepArgs a, NodeView n, PartView p, GatherView gv, double* epart, uint32_t* orOut,
+                                                       uint32_t* creOut) {
     int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
     double ev[3] = {0., 0., 0.};
     uint32_t bits = 0;
@@ -162,7 +219,9 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(StepArgs a, NodeView n, 
             double4 vm = n.velm[i], aw = n.angI[i];
             D3 vel = xyz(vm), angVel = xyz(aw);
             const double mass = vm.w, inertia = aw.w;
-            D3 f = xyz(n.force[i]), t = xyz(n.torque[i]);
+            D3 f, t;
+            if (GATHER) gatherForces(a, n, p, gv, i, fl, mass, pos, f, t);
+            else { f = xyz(n.force[i]); t = xyz(n.torque[i]); }
             D3 g = mk3(a.g[0], a.g[1], a.g[2]);
             bool damp = (damping != 0. && !(fl & F_DAMPING_SKIP));
             bool blockedNone = (fl & F_DOF_ALL) == 0, blockedAll = (fl & F_DOF_ALL) == F_DOF_ALL;
@@ -579,16 +638,37 @@ template <typename T>
 __device__ __forceinline__ void copyContact(const ConView<T>& src, int64_t s, const ConView<T>& dst, int64_t d) {
     dst.ids[d] = src.ids[s];
     dst.cd[d] = src.cd[s];
-    for (int k = 0; k < NSTATE; k++) dst.st[(size_t)k * dst.cap + d] = src.st[(size_t)k * src.cap + s];
     dst.gfc[2 * d] = src.gfc[2 * s];
     dst.gfc[2 * d + 1] = src.gfc[2 * s + 1];
 }
 
 // merged list = candidates (bounds overlap) U old real contacts (InsertionSortCollider keeps
 // real contacts, :92-96); an old contact that is not dead keeps its state, ids order and cellDist
+// Dormancy: between full runs every node moves by at most verletDist from its position at
+// the run (InsertionSortCollider.cpp:340-350), so a sphere-sphere or wall-sphere pair in
+// the same periodic image whose surface gap exceeds 2 verletDist now stays apart until the
+// next full run; the contact kernel skips it without changing any result.
+__device__ __forceinline__ bool dormantPair(const NodeView& n, const PartView& p, int A, int B, uint32_t ty, int32_t cdp, double vd,
+                                            double df) {
+    if (cdp != 0) return false;
+    double4 pa = n.posr[A], pb = n.posr[B];
+    if (ty == CI_SS) {
+        D3 r = xyz(pb) - xyz(pa);
+        double R = fabs(df) * (pa.w + pb.w);
+        return wsqrt(sqn3(r)) - R > 2 * vd * (1 + 1e-6) + 1e-12 * R;
+    }
+    if (ty == CI_WS) {
+        int ax = ptAxis(p.ptype[A]);
+        double dist = comp3(xyz(pb), ax) - comp3(xyz(pa), ax);
+        return fabs(dist) - pb.w > 2 * vd * (1 + 1e-6) + 1e-12 * pb.w;
+    }
+    return false;
+}
+
 template <typename T>
-__global__ void __launch_bounds__(WDM_BLOCK) k_emit_cand(PartView p, ConView<T> oldc, int64_t nold, const int64_t* ooRank,
-                                                        const uint64_t* ckey, const int32_t* cper, int64_t ncand, ConView<T> nc) {
+__global__ void __launch_bounds__(WDM_BLOCK) k_emit_cand(NodeView n, PartView p, ConView<T> oldc, int64_t nold, const int64_t* ooRank,
+                                                        const uint64_t* ckey, const int32_t* cper, int64_t ncand, ConView<T> nc,
+                                                        int32_t* oslot, int64_t* rflag, double vd, double df) {
     int64_t i = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
     if (i >= ncand) return;
     uint64_t k = ckey[i];
@@ -597,7 +677,16 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_emit_cand(PartView p, ConView<T> 
     nc.key[d] = k;
     if (lb < nold && oldc.key[lb] == k && !(oldc.info[lb] & CI_DEAD)) {
         copyContact(oldc, lb, nc, d);
-        nc.info[d] = (oldc.info[lb] & ~(CI_PENDING | CI_DEAD)) | CI_OVERLAP;
+        uint32_t inf = oldc.info[lb];
+        bool r = (inf & CI_REAL) != 0;
+        uint32_t ni = (inf & ~(CI_PENDING | CI_DEAD | CI_DORMANT)) | CI_OVERLAP;
+        if (!r) {
+            int2 id = oldc.ids[lb];
+            if (dormantPair(n, p, id.x, id.y, ni & CI_TYPE_MASK, oldc.cd[lb], vd, df)) ni |= CI_DORMANT;
+        }
+        nc.info[d] = ni;
+        oslot[d] = r ? oldc.slot[lb] : -1;
+        rflag[d] = r ? 1 : 0;
     } else {
         int lo = (int)(k >> 32), hi = (int)(k & 0xffffffffu);
         int A, B, ty;
@@ -605,22 +694,40 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_emit_cand(PartView p, ConView<T> 
         nc.ids[d] = make_int2(A, B);
         int32_t pc = cper[i];  // periods computed in canonical order
         nc.cd[d] = pc;
-        nc.info[d] = (uint32_t)ty | CI_OVERLAP;
+        nc.info[d] = (uint32_t)ty | CI_OVERLAP | (dormantPair(n, p, A, B, (uint32_t)ty, pc, vd, df) ? CI_DORMANT : 0u);
         for (int s = 0; s < NSTATE; s++) nc.st[(size_t)s * nc.cap + d] = (T)0;
         nc.gfc[2 * d] = make_double4(0., 0., 0., 0.);
         nc.gfc[2 * d + 1] = make_double4(0., 0., 0., 0.);
+        oslot[d] = -1;
+        rflag[d] = 0;
     }
 }
 template <typename T>
 __global__ void __launch_bounds__(WDM_BLOCK) k_emit_oor(ConView<T> oldc, int64_t nold, const int32_t* flag, const int64_t* ooRank,
-                                                       const uint64_t* ckey, int64_t ncand, ConView<T> nc) {
+                                                       const uint64_t* ckey, int64_t ncand, ConView<T> nc, int32_t* oslot,
+                                                       int64_t* rflag) {
     int64_t j = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
     if (j >= nold || !flag[j]) return;
     uint64_t k = oldc.key[j];
     int64_t d = lowerBound64(ckey, ncand, k) + ooRank[j];
     nc.key[d] = k;
     copyContact(oldc, j, nc, d);
-    nc.info[d] = oldc.info[j] & ~(CI_PENDING | CI_DEAD | CI_OVERLAP);
+    nc.info[d] = oldc.info[j] & ~(CI_PENDING | CI_DEAD | CI_OVERLAP | CI_DORMANT);
+    oslot[d] = oldc.slot[j];
+    rflag[d] = 1;
+}
+
+// dense state rows for the real contacts of the merged list, in key order
+template <typename T>
+__global__ void __launch_bounds__(WDM_BLOCK) k_copy_state(ConView<T> oldc, ConView<T> nc, int64_t nnew, const int32_t* oslot,
+                                                         const int64_t* rscan) {
+    int64_t d = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
+    if (d >= nnew) return;
+    int32_t os = oslot[d];
+    if (os < 0) { nc.slot[d] = -1; return; }
+    int64_t ns = rscan[d];
+    nc.slot[d] = (int32_t)ns;
+    for (int k = 0; k < NSTATE; k++) nc.st[(size_t)k * nc.cap + ns] = oldc.st[(size_t)k * oldc.cap + os];
 }
 
 // CSR entries: key (particle << 32 | other), value contact index | isB << 31
@@ -676,7 +783,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(StepArgs a, NodeView n, 
     bool realOut = false;  // contact is real after this step's law
     if (c < nc) do {
         uint32_t info = cv.info[c];
-        if (info & CI_DEAD) break;
+        if (info & (CI_DEAD | CI_DORMANT)) break;
         uint32_t ty = info & CI_TYPE_MASK;
         if (info & CI_PENDING) {  // ContactContainer::removePending (ContactContainer.hpp:78-93)
             info &= ~CI_PENDING;
@@ -696,7 +803,8 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(StepArgs a, NodeView n, 
             shift2 = mv3(a.hSize, mk3((double)unpack10(cdp, 0), (double)unpack10(cdp, 10), (double)unpack10(cdp, 20)));
         }
         T tr[9];
-        if (real) for (int k = 0; k < 9; k++) tr[k] = cv.st[(size_t)(S_T + k) * cv.cap + c];
+        int32_t row = real ? cv.slot[c] : -1;
+        if (real) for (int k = 0; k < 9; k++) tr[k] = cv.st[(size_t)(S_T + k) * cv.cap + row];
         Geo<T> G;
         if (ty == CI_SS) {  // Cg2_Sphere_Sphere_L6Geom::go, pkg/dem/Sphere.cpp:88-109
             D3 p2 = xyz(pb) + shift2;
@@ -882,8 +990,8 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(StepArgs a, NodeView n, 
             }
             F[0] = F[1] = F[2] = 0;
         } else {
-            for (int k = 0; k < 5; k++) ph[k] = cv.st[(size_t)(S_P + k) * cv.cap + c];
-            for (int k = 0; k < 3; k++) F[k] = cv.st[(size_t)(S_F + k) * cv.cap + c];
+            for (int k = 0; k < 5; k++) ph[k] = cv.st[(size_t)(S_P + k) * cv.cap + row];
+            for (int k = 0; k < 3; k++) F[k] = cv.st[(size_t)(S_F + k) * cv.cap + row];
         }
         if (LAW == 0) {  // Law2_L6Geom_FrictPhys_IdealElPl::go, pkg/dem/IdealElPl.cpp:12-122
             const T kn = ph[0], kt = ph[1], tanPhi = ph[2];
@@ -969,16 +1077,24 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(StepArgs a, NodeView n, 
             break;
         }
         // store geometry/physics state
-        for (int k = 0; k < 9; k++) cv.st[(size_t)(S_T + k) * cv.cap + c] = tr[k];
-        for (int k = 0; k < 3; k++) cv.st[(size_t)(S_F + k) * cv.cap + c] = F[k];
+        if (!real) {  // a contact becoming real keeps its earlier row or takes the next free one
+            row = cv.slot[c];
+            if (row < 0) {
+                row = atomicAdd(cv.counter, 1);
+                if (row >= cv.cap) { atomicOr(err, ERR_CAPACITY); break; }
+                cv.slot[c] = row;
+            }
+        }
+        for (int k = 0; k < 9; k++) cv.st[(size_t)(S_T + k) * cv.cap + row] = tr[k];
+        for (int k = 0; k < 3; k++) cv.st[(size_t)(S_F + k) * cv.cap + row] = F[k];
         if (!real) {
-            for (int k = 0; k < 5; k++) cv.st[(size_t)(S_P + k) * cv.cap + c] = ph[k];
+            for (int k = 0; k < 5; k++) cv.st[(size_t)(S_P + k) * cv.cap + row] = ph[k];
             cv.info[c] = info | CI_REAL;
         }
         // global force trsf*F (Contact::getForceTorqueBranch, Contact.cpp:24-34)
         V3T<T> fl = mk3<T>(F[0], F[1], F[2]);
         V3T<T> fg = mk3<T>(dot3(mk3<T>(tr[0], tr[1], tr[2]), fl), dot3(mk3<T>(tr[3], tr[4], tr[5]), fl), dot3(mk3<T>(tr[6], tr[7], tr[8]), fl));
-        cv.gfc[2 * c] = make_double4((double)fg.x, (double)fg.y, (double)fg.z, 0.);
+        cv.gfc[2 * c] = make_double4((double)fg.x, (double)fg.y, (double)fg.z, (double)cdp);
         cv.gfc[2 * c + 1] = make_double4(G.contPt.x, G.contPt.y, G.contPt.z, (double)uN);
         realOut = true;
     } while (0);
@@ -987,45 +1103,6 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(StepArgs a, NodeView n, 
     int64_t w0 = blockIdx.x * (int64_t)WDM_BLOCK + (threadIdx.x & ~63);
     if ((threadIdx.x & 63) == 0 && w0 < nc) realMask[w0 >> 6] = bal;
     if (a.trackEnergy) blockSumWrite<5>(ev, bits, epart, nullptr, creOut);
-}
-
-// ---------------- force application (ContactLoop::applyForceUninodal, ContactLoop.cpp:261-267)
-// plus the force reset of Leapfrog (Leapfrog.cpp:240-245): one thread per node; a
-// uninodal particle's node sums its contacts in ascending order of the other id ----------------
-__global__ void __launch_bounds__(WDM_BLOCK) k_gather(StepArgs a, NodeView n, PartView p, const int64_t* off, const uint32_t* ent,
-                                                     const double4* gfc, const int32_t* cd, const uint64_t* realMask) {
-    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
-    if (i >= n.nn) return;
-    uint32_t fl = n.flags[i];
-    double m = n.velm[i].w;
-    D3 F = (a.hasGravity && !(fl & F_GRAVITY_SKIP)) ? mk3(a.g[0] * m, a.g[1] * m, a.g[2] * m) : mk3(0., 0., 0.);
-    D3 T = mk3(0., 0., 0.);
-    if (i < p.np) {
-        int sh = ptShape(p.ptype[i]);
-        if (sh == 1 || sh == 2) {
-            D3 pos = xyz(n.posr[i]);
-            int64_t e0 = off[i], e1 = off[i + 1];
-            for (int64_t e = e0; e < e1; e++) {
-                uint32_t v = ent[e];
-                uint32_t c = v & 0x7fffffffu;
-                if (!((realMask[c >> 6] >> (c & 63)) & 1ull)) continue;  // non-real: no force (ContactLoop.cpp:188)
-                bool isB = (v >> 31) != 0;
-                double4 fg = gfc[2 * (size_t)c], cp = gfc[2 * (size_t)c + 1];
-                double sign = isB ? -1. : 1.;
-                D3 Fs = mk3(fg.x * sign, fg.y * sign, fg.z * sign);
-                D3 ref = pos;
-                if (isB && a.periodic) {
-                    int32_t q = cd[c];
-                    ref = pos + mv3(a.hSize, mk3((double)unpack10(q, 0), (double)unpack10(q, 10), (double)unpack10(q, 20)));
-                }
-                D3 xc = xyz(cp) - ref;
-                F = F + Fs;
-                T = T + cross3(xc, Fs);
-            }
-        }
-    }
-    n.force[i] = make_double4(F.x, F.y, F.z, 0.);
-    n.torque[i] = make_double4(T.x, T.y, T.z, 0.);
 }
 
 // ---------------- scene generation: Jacobi overlap relaxation ----------------

@@ -131,7 +131,7 @@ struct ConBuf {
     DBuf<uint64_t> key;
     DBuf<int2> ids;
     DBuf<uint32_t> info;
-    DBuf<int32_t> cd;
+    DBuf<int32_t> cd, slot, counter;
     DBuf<T> st;
     DBuf<double4> gfc;
     int64_t cap = 0;
@@ -139,12 +139,20 @@ struct ConBuf {
         if (n <= cap && key.p) return;
         int64_t nc = std::max<int64_t>(n + n / 3, 1024);
         release(c, key); release(c, ids); release(c, info); release(c, cd); release(c, st); release(c, gfc);
+        release(c, slot); release(c, counter);
         ensure(c, key, nc); ensure(c, ids, nc); ensure(c, info, nc); ensure(c, cd, nc);
+        ensure(c, slot, nc); ensure(c, counter, 1);
         ensure(c, st, (size_t)NSTATE * nc); ensure(c, gfc, 2 * (size_t)nc);
         cap = nc;
     }
-    ConView<T> view() const { ConView<T> v; v.key = key.p; v.ids = ids.p; v.info = info.p; v.cd = cd.p; v.st = st.p; v.gfc = gfc.p; v.cap = cap; return v; }
-    void free(Ctx* c) { release(c, key); release(c, ids); release(c, info); release(c, cd); release(c, st); release(c, gfc); cap = 0; }
+    ConView<T> view() const {
+        ConView<T> v; v.key = key.p; v.ids = ids.p; v.info = info.p; v.cd = cd.p; v.slot = slot.p; v.counter = counter.p;
+        v.st = st.p; v.gfc = gfc.p; v.cap = cap; return v;
+    }
+    void free(Ctx* c) {
+        release(c, key); release(c, ids); release(c, info); release(c, cd); release(c, st); release(c, gfc);
+        release(c, slot); release(c, counter); cap = 0;
+    }
 };
 
 // ---------------- Scene (core/Scene.hpp:27) with its DemField (pkg/dem/Particle.hpp:278) ----------------
@@ -168,6 +176,8 @@ struct Scene {
     bool verletResolved = false;
     bool needFull = true;
     bool lastLeapfrogVerletChecked = false;
+    bool forcesFromContacts = false;  // node forces are the sums of the last ContactLoop (not materialised)
+    HM3 hSizeCL = heye();             // cell hSize at the last ContactLoop (branch vectors of the B side)
     // nodes
     DBuf<double4> posr, velm, angI, ori, force, torque, lastPos;
     DBuf<uint32_t> flags;
@@ -186,7 +196,8 @@ struct Scene {
     DBuf<uint32_t> ckeys, ckeys2;
     DBuf<int32_t> cvals, cvals2, cstart, cend;
     DBuf<int64_t> counts, offs, ooFlag64, ooRank;
-    DBuf<int32_t> ooFlag, cnt32;
+    DBuf<int32_t> ooFlag, cnt32, oslot;
+    DBuf<int64_t> rflag, rscan;
     DBuf<uint64_t> candK, candK2;
     DBuf<int32_t> candP, candP2;
     DBuf<double> extPart;
@@ -213,6 +224,12 @@ struct Scene {
         NodeView v; v.posr = posr.p; v.velm = velm.p; v.angI = angI.p; v.ori = ori.p; v.force = force.p; v.torque = torque.p;
         v.lastPos = lastPos.p; v.flags = flags.p; v.checked = checked.p; v.nn = nn; return v;
     }
+    GatherView gview() {
+        GatherView g;
+        g.off = csrOff.p; g.ent = entV2.p; g.realMask = realMask.p;
+        g.gfc = (P.precision == WDM_PREC_MIXED) ? cb32[cur].gfc.p : cb64[cur].gfc.p;
+        return g;
+    }
     PartView pview() {
         PartView v; v.ptype = ptype.p; v.mask = pmask.p; v.mat = pmat.p; v.fnodes = fnodes.p; v.bmin = bmin.p; v.bmax = bmax.p; v.np = np; return v;
     }
@@ -223,7 +240,7 @@ struct Scene {
         release(ctx, unb); release(ctx, pmask); release(ctx, fnodes); release(ctx, bmin); release(ctx, bmax); release(ctx, mat);
         for (int i = 0; i < 2; i++) { cb64[i].free(ctx); cb32[i].free(ctx); }
         release(ctx, ckeys); release(ctx, ckeys2); release(ctx, cvals); release(ctx, cvals2); release(ctx, cstart); release(ctx, cend);
-        release(ctx, counts); release(ctx, offs); release(ctx, ooFlag64); release(ctx, ooRank); release(ctx, ooFlag); release(ctx, cnt32);
+        release(ctx, counts); release(ctx, offs); release(ctx, ooFlag64); release(ctx, ooRank); release(ctx, ooFlag); release(ctx, cnt32); release(ctx, oslot); release(ctx, rflag); release(ctx, rscan);
         release(ctx, candK); release(ctx, candK2); release(ctx, candP); release(ctx, candP2); release(ctx, extPart); release(ctx, temp);
         release(ctx, entK); release(ctx, entK2); release(ctx, entV); release(ctx, entV2); release(ctx, csrOff); release(ctx, realMask);
         release(ctx, E); release(ctx, epartL); release(ctx, epartC); release(ctx, eslots); release(ctx, orL); release(ctx, ecre); release(ctx, err);
@@ -278,6 +295,7 @@ int64_t readI64(Scene& S, const int64_t* p) {
     return v;
 }
 
+__global__ void k_i64_to_i32_one(const int64_t* a, int32_t* b) { *b = (int32_t)*a; }
 __global__ void k_i32_to_i64(const int32_t* a, int64_t* b, int64_t n) {
     int64_t i = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
     if (i < n) b[i] = a[i];
@@ -338,6 +356,7 @@ StepArgs makeArgs(Scene& S) {
     a.nextGradV = toM3(nnextL); a.gradV = toM3(pprevL);
     for (int k = 0; k < 3; k++) { a.deltaSpinVec[k] = dsv[k]; a.spinVec[k] = S.spinVec[k]; a.cellSize[k] = S.size[k]; }
     a.hSize = toM3(S.hSize);
+    a.gatherHSize = toM3(S.hSizeCL);
     if (S.periodic && S.homoDeform == 4) a.shiftVelM = toM3(hmul(S.gradV, S.pprevHsize));
     else if (S.periodic && (S.homoDeform == 2 || S.homoDeform == 3)) a.shiftVelM = toM3(hmul(S.gradV, S.hSize));
     else a.shiftVelM = toM3(hzero());
@@ -389,7 +408,10 @@ struct Leapfrog {
         HIPCHK(hipMemsetAsync(S.orL.p, 0, sizeof(uint32_t), c->stream));
         {
             KTimer t(c, "leapfrog");
-            hipLaunchKernelGGL(k_leapfrog, dim3(nb), dim3(WDM_BLOCK), 0, c->stream, a, S.nview(), S.epartL.p, S.orL.p, S.ecre.p);
+            if (S.forcesFromContacts)
+                hipLaunchKernelGGL(k_leapfrog<true>, dim3(nb), dim3(WDM_BLOCK), 0, c->stream, a, S.nview(), S.pview(), S.gview(), S.epartL.p, S.orL.p, S.ecre.p);
+            else
+                hipLaunchKernelGGL(k_leapfrog<false>, dim3(nb), dim3(WDM_BLOCK), 0, c->stream, a, S.nview(), S.pview(), S.gview(), S.epartL.p, S.orL.p, S.ecre.p);
             checkLaunch();
         }
         if (a.reallyTrackEnergy) finalizeEnergy(S, S.epartL, nb, 3, 0);
@@ -511,14 +533,25 @@ struct InsertionSortCollider {
         int64_t nnew = ncand + noor;
         newb.reserve(c, std::max<int64_t>(nnew, 1));
         if (oldb.cap == 0) oldb.reserve(c, 1);
+        ensure(c, S.oslot, std::max<int64_t>(nnew, 1)); ensure(c, S.rflag, std::max<int64_t>(nnew, 1));
+        ensure(c, S.rscan, (size_t)nnew + 1);
         {
             KTimer t(c, "merge");
             if (ncand > 0)
-                hipLaunchKernelGGL(k_emit_cand<T>, dim3(nblk(ncand)), dim3(WDM_BLOCK), 0, st, S.pview(), oldb.view(), nold, (const int64_t*)S.ooRank.p,
-                                   (const uint64_t*)S.candK2.p, (const int32_t*)S.candP2.p, ncand, newb.view());
+                hipLaunchKernelGGL(k_emit_cand<T>, dim3(nblk(ncand)), dim3(WDM_BLOCK), 0, st, S.nview(), S.pview(), oldb.view(), nold,
+                                   (const int64_t*)S.ooRank.p, (const uint64_t*)S.candK2.p, (const int32_t*)S.candP2.p, ncand, newb.view(),
+                                   S.oslot.p, S.rflag.p, S.verletDist, S.P.distFactor);
             if (noor > 0)
                 hipLaunchKernelGGL(k_emit_oor<T>, dim3(nblk(nold)), dim3(WDM_BLOCK), 0, st, oldb.view(), nold, (const int32_t*)S.ooFlag.p,
-                                   (const int64_t*)S.ooRank.p, (const uint64_t*)S.candK2.p, ncand, newb.view());
+                                   (const int64_t*)S.ooRank.p, (const uint64_t*)S.candK2.p, ncand, newb.view(), S.oslot.p, S.rflag.p);
+            checkLaunch();
+            scanInclusiveShift(S, S.rflag.p, S.rscan.p, nnew);
+            if (nnew > 0)
+                hipLaunchKernelGGL(k_copy_state<T>, dim3(nblk(nnew)), dim3(WDM_BLOCK), 0, st, oldb.view(), newb.view(), nnew,
+                                   (const int32_t*)S.oslot.p, (const int64_t*)S.rscan.p);
+            checkLaunch();
+            // next free row = number of real contacts carried over
+            hipLaunchKernelGGL(k_i64_to_i32_one, dim3(1), dim3(1), 0, st, (const int64_t*)S.rscan.p + nnew, newb.counter.p);
             checkLaunch();
         }
         S.cur = 1 - S.cur;
@@ -562,6 +595,7 @@ struct InsertionSortCollider {
     static void throwDeviceError(uint32_t e) {
         if (e & ERR_NAN) throw WdmError(WDM_E_NAN, "NaN force in contact");
         if (e & ERR_OVERSIZE) throw WdmError(WDM_E_OVERSIZE, "Particle larger than half of the cell size encountered.");
+        if (e & ERR_CAPACITY) throw WdmError(WDM_E_STATE, "contact state rows exhausted");
         throw WdmError(WDM_E_HIP, "device error");
     }
 };
@@ -588,12 +622,8 @@ struct ContactLoop {
             checkLaunch();
         }
         if (a.trackEnergy && nc > 0) finalizeEnergy(S, S.epartC, (int)nbk, 5, 3);
-        {
-            KTimer t(c, "gather");
-            hipLaunchKernelGGL(k_gather, dim3(nblk(S.nn)), dim3(WDM_BLOCK), 0, c->stream, a, S.nview(), S.pview(), (const int64_t*)S.csrOff.p,
-                               (const uint32_t*)S.entV2.p, (const double4*)b.gfc.p, (const int32_t*)b.cd.p, (const uint64_t*)S.realMask.p);
-            checkLaunch();
-        }
+        S.forcesFromContacts = true;  // applied by the next Leapfrog (or materialised on download)
+        S.hSizeCL = S.hSize;
     }
     static void run(Scene& S) {
         if (S.needFull) throw WdmError(WDM_E_STATE, "ContactLoop before the collider's first run");
@@ -627,6 +657,14 @@ void syncAndCheck(Scene& S) {
     if (S.hostWords[1]) InsertionSortCollider::throwDeviceError(S.hostWords[1]);
 }
 
+// DemData.force/torque after ContactLoop: gravity reset plus the contact sums
+void materialiseForces(Scene& S) {
+    StepArgs a = makeArgs(S);
+    KTimer t(S.ctx, "gather");
+    hipLaunchKernelGGL(k_gather, dim3(nblk(S.nn)), dim3(WDM_BLOCK), 0, S.ctx->stream, a, S.nview(), S.pview(), S.gview());
+    checkLaunch();
+}
+
 Scene& getScene(Ctx* c, int32_t id) {
     auto it = c->scenes.find(id);
     if (it == c->scenes.end()) throw WdmError(WDM_E_ARG, "unknown scene id " + std::to_string(id));
@@ -657,13 +695,14 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
     int64_t n = S.nc;
     std::vector<int2> ids(n);
     std::vector<uint32_t> info(n);
-    std::vector<int32_t> cd(n);
+    std::vector<int32_t> cd(n), slot(n);
     std::vector<T> st((size_t)NSTATE * std::max<int64_t>(b.cap, 1));
     std::vector<double4> gfc(2 * (size_t)n);
     if (n > 0) {
         HIPCHK(hipMemcpyAsync(ids.data(), b.ids.p, n * sizeof(int2), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(info.data(), b.info.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(cd.data(), b.cd.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(slot.data(), b.slot.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(st.data(), b.st.p, (size_t)NSTATE * b.cap * sizeof(T), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(gfc.data(), b.gfc.p, 2 * n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
     }
@@ -676,12 +715,13 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
         if (idB) idB[k] = ids[i].y;
         if (cellDist3) { cellDist3[3 * k] = unpack10(cd[i], 0); cellDist3[3 * k + 1] = unpack10(cd[i], 10); cellDist3[3 * k + 2] = unpack10(cd[i], 20); }
         if (isReal) isReal[k] = real ? 1 : 0;
+        int64_t row = slot[i];
         for (int d = 0; d < 3; d++) {
-            if (forceLocal3) forceLocal3[3 * k + d] = real ? (double)st[(size_t)(S_F + d) * b.cap + i] : 0.;
+            if (forceLocal3) forceLocal3[3 * k + d] = real ? (double)st[(size_t)(S_F + d) * b.cap + row] : 0.;
         }
         if (contPt3) { contPt3[3 * k] = gfc[2 * i + 1].x; contPt3[3 * k + 1] = gfc[2 * i + 1].y; contPt3[3 * k + 2] = gfc[2 * i + 1].z; }
         if (uN) uN[k] = real ? gfc[2 * i + 1].w : NAN;
-        if (trsf9) for (int q = 0; q < 9; q++) trsf9[9 * k + q] = (double)st[(size_t)(S_T + q) * b.cap + i];
+        if (trsf9) for (int q = 0; q < 9; q++) trsf9[9 * k + q] = (real && row >= 0) ? (double)st[(size_t)(S_T + q) * b.cap + row] : NAN;
         k++;
     }
     if (nOut) *nOut = k;
@@ -843,6 +883,7 @@ int wdm_upload_nodes(wdm_ctx* c, int32_t sid, int32_t n, const double* pos3, con
         HIPCHK(hipStreamSynchronize(st));
         S.nodesSet = true;
         S.needFull = true;
+        S.forcesFromContacts = false;
     });
 }
 
@@ -981,6 +1022,7 @@ int wdm_download_nodes(wdm_ctx* c, int32_t sid, double* pos3, double* ori4, doub
     return guard(c, [&] {
         Scene& S = getScene(c, sid);
         int n = S.nn;
+        if (S.forcesFromContacts && (force3 || torque3)) materialiseForces(S);
         std::vector<double4> a(n);
         auto get = [&](DBuf<double4>& b, double* out, int comps, bool quat) {
             if (!out) return;
