@@ -71,6 +71,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB (5 times), for scaling FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
@@ -137,6 +138,30 @@ def main():
     log("kernels: %s" % json.dumps(kernels))
     log("contacts: %d potential, %d real" % (n_con, n_real))
 
+    mixed = None
+    if not args.no_mixed and args.precision == "fp64":
+        run.close()
+        pm = runner.params_from_meta(b, precision=PREC_MIXED)
+        rm = runner.HipRun(b, pm, runner.cell_from_meta(b), ctx=ctx)
+        rm.step(args.warmup)
+        ctx.sync()
+        tm = time.perf_counter()
+        rm.step(args.steps)
+        ctx.sync()
+        elm = time.perf_counter() - tm
+        ctx.enable_timing(True)
+        rm.step(args.timing_steps)
+        ktm = ctx.kernel_times()
+        ctx.enable_timing(False)
+        cm = rm.contacts()
+        msm, lam = ktm.get("contacts", (float("nan"), 0))
+        mixed = {"value": b.n_par * args.steps / elm, "ms_per_step": elm / args.steps * 1e3,
+                 "speedup_vs_fp64": (b.n_par * args.steps / elm) / value, "contacts_avg_ms": msm / max(lam, 1),
+                 "contacts_real": int(cm["isReal"].sum())}
+        log("mixed precision: %s" % json.dumps(mixed))
+        rm.close()
+        run = None
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -167,10 +192,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_ms},
         "kernels": kernels,
+        "mixed_precision": mixed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    run.close()
+    if run is not None:
+        run.close()
 
 
 if __name__ == "__main__":

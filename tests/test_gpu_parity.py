@@ -131,3 +131,38 @@ def test_nan_force_raises(hip_ctx):
         g.step(3000)
     assert e.value.code == -4
     g.close()
+
+
+def test_facet_floor_parity(hip_ctx):
+    """Cg2_Facet_Sphere_L6Geom on the device (Facet.cpp:235-288): spheres landing on a tilted mesh of
+    fixed facets sharing vertex nodes."""
+    b = scenes.facet_floor()
+    p = runner.params_from_meta(b, trackEnergy=True)
+    cell = runner.cell_from_meta(b)
+    g = runner.HipRun(b, p, cell, ctx=hip_ctx)
+    o = OL.OracleRun(b, p, cell)
+    facet_real = 0
+    for k in (1, 2499, 500, 500, 500):
+        g.step(k)
+        o.step(k)
+        compare(g, o, 1e-9, "step %d" % g.stats()["step"])
+        c = o.contacts()
+        facet_real = max(facet_real, int((c["isReal"] & (b.shape[c["idA"]] == 3)).sum()))
+    assert facet_real > 0
+    g.close()
+
+
+def test_two_cell_grid_axis(hip_ctx):
+    """Regression: an aperiodic grid with exactly two cells along an axis must still pair the two
+    layers (diagonal neighbours across the cell boundary)."""
+    a = scenes.box_lattice(nx=6, ny=6, nz=2, box=((0., 0., 0.), (.1, .1, .05)))
+    p = runner.params_from_meta(a)
+    cell = runner.cell_from_meta(a)
+    g = runner.HipRun(a, p, cell, ctx=hip_ctx)
+    o = OL.OracleRun(a, p, cell)
+    g.step(1)
+    o.step(1)
+    compare(g, o, 0., "step 1")
+    lo, hi = o.bruteforce_pairs()
+    assert len(OL.pair_keys(g.contacts())) == len(lo)
+    g.close()

@@ -555,7 +555,8 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pairs(PartView p, GridArgs g, con
     int sa = ptShape(p.ptype[id]);
     int cnt = 0;
     int64_t o = MODE ? offs[i] : 0;
-    int rx = g.n[0] >= 3 ? 1 : 0, ry = g.n[1] >= 3 ? 1 : 0, rz = g.n[2] >= 3 ? 1 : 0;
+    // periodic axes with fewer than 3 cells hold a single cell (no neighbours); aperiodic axes always scan +-1
+    int rx = (!g.periodic || g.n[0] >= 3) ? 1 : 0, ry = (!g.periodic || g.n[1] >= 3) ? 1 : 0, rz = (!g.periodic || g.n[2] >= 3) ? 1 : 0;
     for (int dz = 0; dz <= rz; dz++) {
         for (int dy = (dz == 0 ? 0 : -ry); dy <= ry; dy++) {
             for (int dx = ((dz == 0 && dy == 0) ? 0 : -rx); dx <= rx; dx++) {
@@ -1118,7 +1119,8 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_relax(NodeView n, PartView p, Gri
     int cx = ck % g.n[0], cy = (ck / g.n[0]) % g.n[1], cz = ck / (g.n[0] * g.n[1]);
     double4 pi = n.posr[id];
     D3 d = mk3(0., 0., 0.);
-    int rx = g.n[0] >= 3 ? 1 : 0, ry = g.n[1] >= 3 ? 1 : 0, rz = g.n[2] >= 3 ? 1 : 0;
+    // periodic axes with fewer than 3 cells hold a single cell (no neighbours); aperiodic axes always scan +-1
+    int rx = (!g.periodic || g.n[0] >= 3) ? 1 : 0, ry = (!g.periodic || g.n[1] >= 3) ? 1 : 0, rz = (!g.periodic || g.n[2] >= 3) ? 1 : 0;
     for (int dz = -rz; dz <= rz; dz++) {
         int z = cz + dz;
         if (g.periodic) z = (z + g.n[2]) % g.n[2]; else if (z < 0 || z >= g.n[2]) continue;

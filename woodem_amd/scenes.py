@@ -100,3 +100,49 @@ def relax_and_order(a, ctx, iters=500):
 def hertz_params_for(a):
     """Engine attributes of the periodic Hertz workload, as a dict for woodem_amd.bind."""
     return dict(dt=a.meta["dt"], damping=0., gravity=(0., 0., 0.), law=1, poisson=a.meta["poisson"], en=a.meta["en"])
+
+
+def facet_floor(n=6, tilt=.2, cells=6, size=.1, radius=4e-3, spacing=9e-3, seed=3, density=2000., young=1e7, tanPhi=.5,
+                halfThick=0.):
+    """Spheres (n x n x 2 lattice, jittered) falling on a tilted square floor made of
+    2*cells^2 fixed facets sharing vertex nodes (utils.facet, py/utils.py:242-268)."""
+    rng = np.random.default_rng(seed)
+    ii, jj, kk = np.meshgrid(np.arange(n), np.arange(n), np.arange(2), indexing="ij")
+    pos = np.stack([ii.ravel(), jj.ravel(), kk.ravel()], 1).astype(np.float64) * spacing
+    pos += np.array([.5 * (size - (n - 1) * spacing), .5 * (size - (n - 1) * spacing), .022])
+    pos += rng.uniform(-.5e-3, .5e-3, pos.shape)
+    a = _soa.spheres_arrays(pos, radius, density=density, young=young, tanPhi=tanPhi)
+    ns = a.n_par
+    # vertex grid, tilted about the x axis
+    g = np.linspace(0, size, cells + 1)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    Z = Y * math.tan(tilt)
+    verts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    vid = lambda i, j: i * (cells + 1) + j
+    tris = []
+    for i in range(cells):
+        for j in range(cells):
+            tris.append((vid(i, j), vid(i + 1, j), vid(i + 1, j + 1)))
+            tris.append((vid(i, j), vid(i + 1, j + 1), vid(i, j + 1)))
+    nf = len(tris)
+    npar = ns + nf
+    nv = len(verts)
+    nn = npar + nv
+    ori = np.zeros((nn, 4))
+    ori[:, 0] = 1
+    pos_n = np.concatenate([a.pos, np.zeros((nf, 3)), verts])
+    flags = np.concatenate([a.flags, np.full(nf + nv, _soa.DOF_ALL, np.uint32)])
+    nodes = np.concatenate([a.nodes, np.array(tris, np.int32) + npar])
+    b = _soa.DemArrays(
+        pos=pos_n, ori=ori, vel=np.zeros((nn, 3)), angVel=np.zeros((nn, 3)),
+        mass=np.concatenate([a.mass, np.zeros(nf + nv)]), inertia=np.concatenate([a.inertia, np.zeros((nf + nv, 3))]),
+        flags=flags, force=np.zeros((nn, 3)), torque=np.zeros((nn, 3)),
+        shape=np.concatenate([a.shape, np.full(nf, 3, np.int32)]), nodes=nodes,
+        radius=np.concatenate([a.radius, np.full(nf, halfThick)]), axis=np.zeros(npar, np.int32),
+        sense=np.zeros(npar, np.int32), mask=np.concatenate([a.mask, np.full(nf, _soa.DEFAULT_BOUNDARY_MASK, np.uint32)]),
+        mat=np.zeros(npar, np.int32), density=a.density, young=a.young, ktDivKn=a.ktDivKn, tanPhi=a.tanPhi).normalized()
+    b.validate()
+    dt = 0.3 * radius / math.sqrt(young / density)
+    b.meta.update(dict(kind="facet_floor", dt=dt, gravity=(0., 0., -9.81), damping=.2, law="IdealElPl", periodic=False,
+                       n_spheres=ns, n_facets=nf, seed=seed))
+    return b
