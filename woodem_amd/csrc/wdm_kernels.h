@@ -12,10 +12,25 @@
 //     gather record gfc[2c] = (global force, 0), gfc[2c+1] = (contact point, uN).
 //   CSR: off[P+1], entries u32 (contact index | isB << 31), per particle sorted by
 //     the other particle's id.
+// Batches: every kernel takes a table of per-scene descriptors (views of the arrays
+// above) and a block prefix table; block b of a launch works on scene t with
+// bprefix[t] <= b < bprefix[t+1], so many small scenes share one launch and each scene
+// sees the same block layout (and the same reduction order) as when it runs alone.
 #pragma once
 #include "wdm_math.h"
 
 #define WDM_BLOCK 256
+
+// In the device pass, the array pointers held in the descriptor structs below are
+// declared in the global address space, so the compiler emits global (not flat) memory
+// instructions for them; the host pass sees plain pointers of the same size and layout.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GP __attribute__((address_space(1)))
+#else
+#define GP
+#endif
+template <typename T>
+__host__ __device__ inline GP T* gpc(T* p) { return (GP T*)p; }
 
 // contact info bits
 #define CI_TYPE_MASK 3u
@@ -69,49 +84,60 @@ WDM_HD int32_t packCd(int a, int b, int c) { return (a & 1023) | ((b & 1023) << 
 WDM_HD int unpack10(int32_t v, int sh) { int x = (v >> sh) & 1023; return x >= 512 ? x - 1024 : x; }
 
 struct NodeView {
-    double4 *posr, *velm, *angI, *ori, *force, *torque, *lastPos;
-    uint32_t* flags;
-    uint8_t* checked;  // node belongs to a particle with a bound (Verlet check)
+    GP double4 *posr, *velm, *angI, *ori, *force, *torque, *lastPos;
+    GP uint32_t* flags;
+    GP uint8_t* checked;  // node belongs to a particle with a bound (Verlet check)
     int nn;
 };
 struct PartView {
-    int32_t* ptype;
-    uint32_t* mask;
-    int32_t* mat;
-    int4* fnodes;
-    double4 *bmin, *bmax;
+    GP int32_t* ptype;
+    GP uint32_t* mask;
+    GP int32_t* mat;
+    GP int4* fnodes;
+    GP double4 *bmin, *bmax;
     int np;
 };
 template <typename T>
 struct ConView {
-    uint64_t* key;
-    int2* ids;
-    uint32_t* info;
-    int32_t* slot;     // row of the contact's geometry/physics state, -1 when it has none
+    GP uint64_t* key;
+    GP int2* ids;
+    GP uint32_t* info;
+    GP int32_t* slot;  // row of the contact's geometry/physics state, -1 when it has none
     int32_t* counter;  // next free state row (rows are handed out densely at full runs)
-    int32_t* cd;
-    T* st;
-    double4* gfc;
+    GP int32_t* cd;
+    GP T* st;
+    GP double4* gfc;
     int64_t cap;
 };
 
-// arguments shared by the step kernels
+__device__ __forceinline__ M3 ldM3(const GP M3& x) {
+    M3 r;
+    for (int k = 0; k < 9; k++) r.m[k] = x.m[k];
+    return r;
+}
+
+// cell quantities that change every step (periodic scenes; zero/identity otherwise)
+struct CellArgs {
+    // Leapfrog (pkg/dem/Leapfrog.cpp:113-126)
+    M3 ImLL4hInv, IpLL4h, LmL, dGradV, midGradV, nextGradV, gradV;
+    double deltaSpinVec[3], spinVec[3];
+    M3 hSize, shiftVelM;  // shiftVelM: gradV*pprevHsize (GRADV2) or gradV*hSize (HOMO_VEL*), else 0
+    M3 gatherHSize;       // hSize when the gathered contact forces were computed (ContactLoop of the previous step)
+    double cellSize[3];
+};
+
+// per-scene engine attributes and per-step flags
 struct StepArgs {
     double dt, damping;
     double g[3];
     int hasGravity;
     int periodic, homoDeform;  // homoDeform: -1 aperiodic
     int reallyTrackEnergy, trackEnergy;
-    // Leapfrog (pkg/dem/Leapfrog.cpp:113-126)
-    M3 ImLL4hInv, IpLL4h, LmL, dGradV, midGradV, nextGradV, gradV;
-    double deltaSpinVec[3], spinVec[3];
-    // cell
-    M3 hSize, shiftVelM;  // shiftVelM: gradV*pprevHsize (GRADV2) or gradV*hSize (HOMO_VEL*), else 0
-    M3 gatherHSize;       // hSize when the gathered contact forces were computed (ContactLoop of the previous step)
-    double cellSize[3];
     // collider
-    double verletD2;
+    double verletDist, verletD2;
     int checkVerlet;
+    uint32_t stamp;  // nonzero tag of this step: the Verlet trigger word holds it when a node moved too far
+    int gatherOn;  // node forces come from the contacts of the previous ContactLoop
     // contact
     double distFactor;
     int law, noSlip, noBreak, noFrict, noAttraction, trsfRenorm;
@@ -120,15 +146,84 @@ struct StepArgs {
     double viscAlpha;  // -sqrt(5) log(en) / sqrt(log(en)^2 + pi^2), or 0 when en is outside (0,1)
     double poisson, gamma;
     int hertzDamped;
-    double4* mat;  // material table (density, young, ktDivKn, tanPhi)
+    GP double4* mat;  // material table (density, young, ktDivKn, tanPhi)
+    const GP CellArgs* cell;
 };
+
+struct GridArgs {
+    double origin[3];
+    double w[3];   // cell width per axis
+    int n[3];      // cells per axis
+    int periodic;
+    double L[3];     // periodic cell size
+    double invL[3];  // 1/L (prefilter only)
+};
+
+
+// force gather views (CSR contact -> particle, realness mask, gather records)
+struct GatherView {
+    const GP int64_t* off;
+    const GP uint32_t* ent;
+    const GP double4* gfc;
+    const GP uint64_t* realMask;
+};
+
+// One scene of a batch.  Kernels read their views from a table of these, so one launch
+// covers many scenes; blocks are mapped to scenes by a prefix table (mapBlock).
+struct SceneDesc {
+    StepArgs a;
+    NodeView n;
+    PartView p;
+    GatherView gv;      // contacts of the last ContactLoop (forces gathered by the next Leapfrog)
+    const GP int32_t* binned;  // particles with finite bounds (spheres, facets)
+    const GP int32_t* unb;     // particles with infinite bounds (walls)
+    int nbinned, nunb;
+    GP double* E;       // energy accumulators [E_N]
+    uint32_t* words;    // [0] Verlet trigger, [1] created energy slots, [2] device errors
+};
+template <typename T>
+struct ConDesc {
+    ConView<T> cv;      // current contact list
+    ConView<T> cvNew;   // merge target of a collider full run
+    int64_t nc;
+    GP uint64_t* realMask;  // one bit per contact, written by the contact kernel
+    int32_t desc;       // index in the SceneDesc table
+};
+// one scene of a batched collider full run: its grid and its segments in the batch-wide
+// scratch arrays (bound entries in cell order, candidate pairs, old and new contacts)
+struct FRScene {
+    GridArgs g;
+    int64_t cellBase, binStart, candStart, nCand, oldStart, nOld, newStart, nNew;
+    GP int64_t* csrOff;
+    GP uint32_t* entV;
+    int32_t desc, pad;
+};
+
+// blocks of one launch spread over several scenes: scene t owns blocks
+// [bprefix[t], bprefix[t+1]) and its elements are numbered from 0 inside them
+struct Map {
+    const int64_t* bprefix;
+    int n;
+};
+// Every thread searches (the same uniform loads, no barrier); the result is uniform, so
+// the descriptor fields that follow are read with scalar loads.
+__device__ __forceinline__ int mapBlock(const Map& m, int64_t& lblk) {
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = m.n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (m.bprefix[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    lblk = b - m.bprefix[lo];
+    return lo;
+}
 
 // ---------------- block reductions (fixed tree: deterministic) ----------------
 // bits: (1 << energy slot) for "slot created" (EnergyTracker::add, core/EnergyTracker.cpp:10-17),
-// VERLET_BIT for the Verlet trigger.
+// VERLET_BIT for the Verlet trigger (the trigger word then receives the step's stamp).
 #define VERLET_BIT 0x80000000u
 template <int NV>
-__device__ void blockSumWrite(double (&v)[NV], uint32_t orbits, double* partial, uint32_t* orOut, uint32_t* creOut) {
+__device__ void blockSumWrite(double (&v)[NV], uint32_t orbits, double* partial, uint32_t* trigOut, uint32_t stamp, uint32_t* creOut) {
     __shared__ double sh[NV][WDM_BLOCK];
     __shared__ uint32_t shb[WDM_BLOCK];
     int t = threadIdx.x;
@@ -145,7 +240,7 @@ __device__ void blockSumWrite(double (&v)[NV], uint32_t orbits, double* partial,
     if (t == 0) {
         if (partial) for (int k = 0; k < NV; k++) partial[(size_t)blockIdx.x * NV + k] = sh[k][0];
         uint32_t b = shb[0];
-        if ((b & VERLET_BIT) && orOut) atomicOr(orOut, VERLET_BIT);
+        if ((b & VERLET_BIT) && trigOut) atomicMax(trigOut, stamp);
         if ((b & ~VERLET_BIT) && creOut) atomicOr(creOut, b & ~VERLET_BIT);
     }
 }
@@ -155,12 +250,6 @@ __device__ void blockSumWrite(double (&v)[NV], uint32_t orbits, double* partial,
 // node sums its real contacts in ascending order of the other particle's id (CSR
 // order); F = trsf*Flocal and the contact point come from the contact's gather
 // record, the periodic shift of the B side from the cellDist packed in its w slot.
-struct GatherView {
-    const int64_t* off;
-    const uint32_t* ent;
-    const double4* gfc;
-    const uint64_t* realMask;
-};
 __device__ __forceinline__ void gatherForces(const StepArgs& a, const NodeView& n, const PartView& p, const GatherView& gv, int i,
                                              uint32_t fl, double m, const D3& pos, D3& F, D3& T) {
     F = (a.hasGravity && !(fl & F_GRAVITY_SKIP)) ? mk3(a.g[0] * m, a.g[1] * m, a.g[2] * m) : mk3(0., 0., 0.);
@@ -180,7 +269,7 @@ __device__ __forceinline__ void gatherForces(const StepArgs& a, const NodeView& 
         D3 ref = pos;
         if (isB && a.periodic) {
             int32_t q = (int32_t)fg.w;
-            ref = pos + mv3(a.gatherHSize, mk3((double)unpack10(q, 0), (double)unpack10(q, 10), (double)unpack10(q, 20)));
+            ref = pos + mv3(ldM3(a.cell->gatherHSize), mk3((double)unpack10(q, 0), (double)unpack10(q, 10), (double)unpack10(q, 20)));
         }
         D3 xc = xyz(cp) - ref;
         F = F + Fs;
@@ -188,8 +277,15 @@ __device__ __forceinline__ void gatherForces(const StepArgs& a, const NodeView& 
     }
 }
 
-__global__ void __launch_bounds__(WDM_BLOCK) k_gather(StepArgs a, NodeView n, PartView p, GatherView gv) {
-    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+__global__ void __launch_bounds__(WDM_BLOCK) k_gather(const SceneDesc* D, const int32_t* idx, Map m) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const SceneDesc& S = D[idx ? idx[t] : t];
+    const StepArgs a = S.a;
+    const NodeView n = S.n;
+    const PartView p = S.p;
+    const GatherView gv = S.gv;
+    int i = (int)(lb * WDM_BLOCK + threadIdx.x);
     if (i >= n.nn) return;
     D3 F, T;
     gatherForces(a, n, p, gv, i, n.flags[i], n.velm[i].w, xyz(n.posr[i]), F, T);
@@ -201,13 +297,19 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_gather(StepArgs a, NodeView n, Pa
 // Per node: accelerations with blocked DOFs, 2nd-order non-viscous damping, GRADV2
 // velocity map, spherical rotation, energy terms, and the Verlet displacement check
 // of InsertionSortCollider::updateBboxes_doFullRun (:340-350) on the new position.
-// Partial energies per block: [nonviscDamp, grav, kinetic].  With GATHER the node's
-// force and torque are summed from the contacts of the previous ContactLoop here
-// instead of being read from the force/torque arrays.
-template <bool GATHER>
-__global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(StepArgs a, NodeView n, PartView p, GatherView gv, double* epart, uint32_t* orOut,
-                                                       uint32_t* creOut) {
-    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+// Partial energies per block: [nonviscDamp, grav, kinetic].  With GATHER (scenes whose
+// last ContactLoop has not been materialised) the node's force and torque are summed
+// from the contacts of that ContactLoop here instead of being read from the arrays.
+template <int GATHER>
+__global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(const SceneDesc* D, const int32_t* idx, Map m, double* epart) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const SceneDesc& S = D[idx ? idx[t] : t];
+    const StepArgs a = S.a;
+    const NodeView n = S.n;
+    const PartView p = S.p;
+    const GatherView gv = S.gv;
+    int i = (int)(lb * WDM_BLOCK + threadIdx.x);
     double ev[3] = {0., 0., 0.};
     uint32_t bits = 0;
     if (i < n.nn) {
@@ -254,11 +356,11 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(StepArgs a, NodeView n, 
                 }
                 if (a.periodic) {
                     switch (a.homoDeform) {  // Cell::pprevFluctVel, core/Cell.cpp:36-50
-                        case 2: case 3: pfv = vel - mv3(a.gradV, pos); break;
-                        case 4: pfv = vel - mv3(a.gradV, pos - vel * (dt / 2)); break;
+                        case 2: case 3: pfv = vel - mv3(ldM3(a.cell->gradV), pos); break;
+                        case 4: pfv = vel - mv3(ldM3(a.cell->gradV), pos - vel * (dt / 2)); break;
                         default: pfv = vel;
                     }
-                    pfav = gradv2 ? angVel - mk3(a.spinVec[0], a.spinVec[1], a.spinVec[2]) : angVel;
+                    pfav = gradv2 ? angVel - mk3(a.cell->spinVec[0], a.cell->spinVec[1], a.cell->spinVec[2]) : angVel;
                 } else {
                     pfv = vel;
                     pfav = angVel;
@@ -268,7 +370,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(StepArgs a, NodeView n, 
                     la.y *= 1 - damping * sgn(f.y * (pfv.y + 0.5 * dt * la.y));
                     la.z *= 1 - damping * sgn(f.z * (pfv.z + 0.5 * dt * la.z));
                 }
-                if (gradv2) vel = mv3(a.ImLL4hInv, (mv3(a.LmL, pos) + mv3(a.IpLL4h, vel)) + la * dt);
+                if (gradv2) vel = mv3(ldM3(a.cell->ImLL4hInv), (mv3(ldM3(a.cell->LmL), pos) + mv3(ldM3(a.cell->IpLL4h), vel)) + la * dt);
                 else vel = vel + la * dt;
                 if (inertia != 0.) {
                     if (blockedNone) aa = t / inertia;
@@ -283,16 +385,16 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(StepArgs a, NodeView n, 
                         aa.z *= 1 - damping * sgn(t.z * (pfav.z + 0.5 * dt * aa.z));
                     }
                     angVel = angVel + aa * dt;
-                    if (gradv2) angVel = angVel - mk3(a.deltaSpinVec[0], a.deltaSpinVec[1], a.deltaSpinVec[2]);
+                    if (gradv2) angVel = angVel - mk3(a.cell->deltaSpinVec[0], a.cell->deltaSpinVec[1], a.cell->deltaSpinVec[2]);
                 }
             } else {
-                if (gradv2) vel = mv3(a.ImLL4hInv, mv3(a.LmL, pos) + mv3(a.IpLL4h, vel));
+                if (gradv2) vel = mv3(ldM3(a.cell->ImLL4hInv), mv3(ldM3(a.cell->LmL), pos) + mv3(ldM3(a.cell->IpLL4h), vel));
             }
             if (a.homoDeform >= 0 && !gradv2) {  // applyPeriodicCorrections, :258-272
                 if (a.homoDeform == 2 || a.homoDeform == 3) {
-                    if (a.homoDeform == 3) vel = vel + mv3(a.midGradV, vel - la * (dt / 2.)) * dt;
-                    vel = vel + mv3(a.dGradV, pos);
-                } else if (a.homoDeform == 1) pos = pos + mv3(a.nextGradV, pos) * dt;
+                    if (a.homoDeform == 3) vel = vel + mv3(ldM3(a.cell->midGradV), vel - la * (dt / 2.)) * dt;
+                    vel = vel + mv3(ldM3(a.cell->dGradV), pos);
+                } else if (a.homoDeform == 1) pos = pos + mv3(ldM3(a.cell->nextGradV), pos) * dt;
             }
             if (a.reallyTrackEnergy && !(fl & F_ENERGY_SKIP)) {  // doKineticEnergy, :75-97
                 D3 cfv = pfv + la * (.5 * dt), cfav = pfav + aa * (.5 * dt);
@@ -329,23 +431,63 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(StepArgs a, NodeView n, 
             if (sqn3(d) > a.verletD2) bits |= VERLET_BIT;
         }
     }
-    blockSumWrite<3>(ev, bits, a.reallyTrackEnergy ? epart : nullptr, orOut, creOut);
+    blockSumWrite<3>(ev, bits, a.reallyTrackEnergy ? epart : nullptr, S.words, a.stamp, S.words + 1);
 }
 
-// sum per-block partials sequentially (block order) into the energy accumulators
-__global__ void k_energy_finalize(const double* part, int nblocks, int nv, const int* slots, double* E) {
-    int k = threadIdx.x;
-    if (k >= nv) return;
+// Per-block partial energies of each scene summed in a fixed order (a strided sequential
+// pass per thread, then a tree): the same for a scene alone or in any batch.
+// Resettable slots (kinetic from Leapfrog, elast from the laws) are assigned, the others
+// accumulated: EnergyTracker::resetResettables at the step prologue followed by the
+// engine's adds.
+__device__ __forceinline__ int energySlot(int nv, int k) {
+    if (nv == 3) return k == 0 ? E_DAMP : (k == 1 ? E_GRAV : E_KIN);
+    const int s[5] = {E_ELAST, E_PLAST, E_VISCN, E_VISCT, E_BROKEN};
+    return s[k];
+}
+__device__ __forceinline__ uint32_t loadWord(const uint32_t* w) {
+    return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// one block per (scene, slot); desc[t * stride] (or t when desc is null) is the SceneDesc
+// of the launch's t-th scene; spec: skip scenes whose Verlet trigger fired this step
+__global__ void __launch_bounds__(WDM_BLOCK) k_energy_finalize(const SceneDesc* D, const int32_t* desc, int stride, Map m,
+                                                              const double* part, int nv, int spec) {
+    __shared__ double sh[WDM_BLOCK];
+    const int t = blockIdx.x / nv, k = blockIdx.x % nv;
+    const SceneDesc& S = D[desc ? desc[(size_t)t * stride] : t];
+    const bool leap = nv == 3;
+    if (!S.a.trackEnergy) return;
+    if (spec && loadWord(S.words) == S.a.stamp) return;
+    const int slot = energySlot(nv, k);
+    if (leap && !S.a.reallyTrackEnergy) {
+        if (slot == E_KIN && threadIdx.x == 0) S.E[E_KIN] = 0.;
+        return;
+    }
     double s = 0;
-    for (int b = 0; b < nblocks; b++) s += part[(size_t)b * nv + k];
-    E[slots[k]] += s;
+    for (int64_t b = m.bprefix[t] + threadIdx.x; b < m.bprefix[t + 1]; b += WDM_BLOCK) s += part[(size_t)b * nv + k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = WDM_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (slot == E_KIN || slot == E_ELAST) S.E[slot] = sh[0];
+        else S.E[slot] += sh[0];
+    }
 }
 
 // ---------------- bounds: Bo1_Sphere_Aabb (pkg/dem/Sphere.cpp:50-79), Bo1_Wall_Aabb
 // (pkg/dem/Wall.cpp:24-33), Bo1_Facet_Aabb (pkg/dem/Facet.cpp:177-200), enlarged by
 // verletDist and with node positions saved (InsertionSortCollider.cpp:376-414) ----------------
-__global__ void __launch_bounds__(WDM_BLOCK) k_bounds(NodeView n, PartView p, double distFactor, double vd) {
-    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+// relaxMode: plain bounds (distFactor 1, no Verlet enlargement) for the overlap relaxation
+__global__ void __launch_bounds__(WDM_BLOCK) k_bounds(const SceneDesc* D, const FRScene* FR, Map m, int relaxMode) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const SceneDesc& S = D[FR[t].desc];
+    const NodeView n = S.n;
+    const PartView p = S.p;
+    const double distFactor = relaxMode ? 1. : S.a.distFactor, vd = relaxMode ? 0. : S.a.verletDist;
+    int i = (int)(lb * WDM_BLOCK + threadIdx.x);
     if (i >= p.np) return;
     int32_t pt = p.ptype[i];
     int sh = ptShape(pt);
@@ -393,14 +535,17 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_bounds(NodeView n, PartView p, do
 }
 
 // extent reduction over binned (finite-bound) particles: min corner, max corner, max extent
-__global__ void __launch_bounds__(WDM_BLOCK) k_extent(PartView p, const int32_t* binned, int nb, double* part) {
+__global__ void __launch_bounds__(WDM_BLOCK) k_extent(const SceneDesc* D, const FRScene* FR, Map m, double* part) {
     __shared__ double sh[7][WDM_BLOCK];
+    int64_t lb;
+    const SceneDesc& S = D[FR[mapBlock(m, lb)].desc];
+    const PartView p = S.p;
     int t = threadIdx.x;
-    int i = blockIdx.x * WDM_BLOCK + t;
+    int i = (int)(lb * WDM_BLOCK + t);
     const double inf = __builtin_inf();
     double v[7] = {inf, inf, inf, -inf, -inf, -inf, 0.};
-    if (i < nb) {
-        int id = binned[i];
+    if (i < S.nbinned) {
+        int id = S.binned[i];
         double4 a = p.bmin[id], b = p.bmax[id];
         v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = b.x; v[4] = b.y; v[5] = b.z;
         v[6] = fmax(fmax(b.x - a.x, b.y - a.y), b.z - a.z);
@@ -416,15 +561,27 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_extent(PartView p, const int32_t*
     }
     if (t == 0) for (int k = 0; k < 7; k++) part[(size_t)blockIdx.x * 7 + k] = sh[k][0];
 }
-
-struct GridArgs {
-    double origin[3];
-    double w[3];   // cell width per axis
-    int n[3];      // cells per axis
-    int periodic;
-    double L[3];     // periodic cell size
-    double invL[3];  // 1/L (prefilter only)
-};
+// per scene (one block each): min corner, max corner, max extent over its blocks
+__global__ void __launch_bounds__(WDM_BLOCK) k_extent_finalize(Map m, const double* part, double* out) {
+    __shared__ double sh[7][WDM_BLOCK];
+    const int t = blockIdx.x, tid = threadIdx.x;
+    const double inf = __builtin_inf();
+    double v[7] = {inf, inf, inf, -inf, -inf, -inf, 0.};
+    for (int64_t b = m.bprefix[t] + tid; b < m.bprefix[t + 1]; b += WDM_BLOCK) {
+        for (int k = 0; k < 3; k++) v[k] = fmin(v[k], part[(size_t)b * 7 + k]);
+        for (int k = 3; k < 7; k++) v[k] = fmax(v[k], part[(size_t)b * 7 + k]);
+    }
+    for (int k = 0; k < 7; k++) sh[k][tid] = v[k];
+    __syncthreads();
+    for (int w = WDM_BLOCK / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            for (int k = 0; k < 3; k++) sh[k][tid] = fmin(sh[k][tid], sh[k][tid + w]);
+            for (int k = 3; k < 7; k++) sh[k][tid] = fmax(sh[k][tid], sh[k][tid + w]);
+        }
+        __syncthreads();
+    }
+    if (tid < 7) out[(size_t)t * 7 + tid] = sh[tid][0];
+}
 
 __device__ __forceinline__ int cellCoord(const GridArgs& g, int d, double x) {
     int c;
@@ -438,14 +595,20 @@ __device__ __forceinline__ int cellCoord(const GridArgs& g, int d, double x) {
     return c < 0 ? 0 : (c >= g.n[d] ? g.n[d] - 1 : c);
 }
 
-__global__ void __launch_bounds__(WDM_BLOCK) k_cell_keys(PartView p, GridArgs g, const int32_t* binned, int nb, uint32_t* keys, int32_t* vals) {
-    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
-    if (i >= nb) return;
-    int id = binned[i];
-    double4 a = p.bmin[id];
+// key = scene's first cell + cell of the bound's minimum corner; sorting the keys of a
+// batch orders entries by scene, then by cell
+__global__ void __launch_bounds__(WDM_BLOCK) k_cell_keys(const SceneDesc* D, const FRScene* FR, Map m, uint32_t* keys, int32_t* vals) {
+    int64_t lb;
+    const FRScene& F = FR[mapBlock(m, lb)];
+    const SceneDesc& S = D[F.desc];
+    int i = (int)(lb * WDM_BLOCK + threadIdx.x);
+    if (i >= S.nbinned) return;
+    const GridArgs g = F.g;  // by value: the loops below store to global memory
+    int id = S.binned[i];
+    double4 a = S.p.bmin[id];
     int cx = cellCoord(g, 0, a.x), cy = cellCoord(g, 1, a.y), cz = cellCoord(g, 2, a.z);
-    keys[i] = (uint32_t)((cz * g.n[1] + cy) * g.n[0] + cx);
-    vals[i] = id;
+    keys[F.binStart + i] = (uint32_t)(F.cellBase + (cz * g.n[1] + cy) * g.n[0] + cx);
+    vals[F.binStart + i] = id;
 }
 
 __global__ void k_fill_i32(int32_t* a, int64_t n, int32_t v) {
@@ -453,12 +616,12 @@ __global__ void k_fill_i32(int32_t* a, int64_t n, int32_t v) {
     if (i < n) a[i] = v;
 }
 
-__global__ void __launch_bounds__(WDM_BLOCK) k_cell_ranges(const uint32_t* keys, int nb, int32_t* cstart, int32_t* cend) {
-    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+__global__ void __launch_bounds__(WDM_BLOCK) k_cell_ranges(const uint32_t* keys, int64_t nb, int32_t* cstart, int32_t* cend) {
+    int64_t i = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
     if (i >= nb) return;
     uint32_t k = keys[i];
-    if (i == 0 || keys[i - 1] != k) cstart[k] = i;
-    if (i == nb - 1 || keys[i + 1] != k) cend[k] = i + 1;
+    if (i == 0 || keys[i - 1] != k) cstart[k] = (int32_t)i;
+    if (i == nb - 1 || keys[i + 1] != k) cend[k] = (int32_t)(i + 1);
 }
 
 // InsertionSortCollider::cellWrap / cellWrapRel (InsertionSortCollider.cpp:650-660)
@@ -541,15 +704,29 @@ __device__ __forceinline__ bool periPrefilter(const double4& a0, const double4& 
     return true;
 }
 
+// Candidate keys of a batch: (t << 2 idBits) | (min id << idBits) | max id, t the scene's
+// index in the full run, so one sort orders them by scene, then like the contact keys.
 template <int MODE>
-__global__ void __launch_bounds__(WDM_BLOCK) k_pairs(PartView p, GridArgs g, const uint32_t* ckeys, const int32_t* cvals, int nb,
-                                                    const int32_t* cstart, const int32_t* cend, const int32_t* unb, int nunb,
-                                                    uint32_t loneMask, int32_t* counts, const int64_t* offs, uint64_t* okeys,
-                                                    int32_t* operiods, uint32_t* err) {
-    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
-    if (i >= nb) return;
+__global__ void __launch_bounds__(WDM_BLOCK) k_pairs(const SceneDesc* D, const FRScene* FR, Map m, const uint32_t* ckeys, const int32_t* cvals,
+                                                    const int32_t* cstart, const int32_t* cend, int32_t* counts, const int64_t* offs,
+                                                    uint64_t* okeys, int32_t* operiods, int idBits) {
+    int64_t lb;
+    const int tt = mapBlock(m, lb);
+    const FRScene& F = FR[tt];
+    const SceneDesc& S = D[F.desc];
+    const PartView p = S.p;
+    const GridArgs g = F.g;  // by value: the loops below store to global memory
+    const uint32_t loneMask = S.a.loneMask;
+    uint32_t* err = S.words + 2;
+    const int* unb = S.unb;
+    const int nunb = S.nunb;
+    const int64_t cellBase = F.cellBase;
+    int li = (int)(lb * WDM_BLOCK + threadIdx.x);
+    if (li >= S.nbinned) return;
+    const int64_t i = F.binStart + li;
+    const uint64_t tkey = (uint64_t)tt << (2 * idBits);
     int id = cvals[i];
-    uint32_t ck = ckeys[i];
+    uint32_t ck = (uint32_t)(ckeys[i] - cellBase);
     int cx = ck % g.n[0], cy = (ck / g.n[0]) % g.n[1], cz = ck / (g.n[0] * g.n[1]);
     double4 a0 = p.bmin[id], a1 = p.bmax[id];
     int sa = ptShape(p.ptype[id]);
@@ -564,11 +741,11 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pairs(PartView p, GridArgs g, con
                 if (g.periodic) {
                     z = (z + g.n[2]) % g.n[2]; y = (y + g.n[1]) % g.n[1]; x = (x + g.n[0]) % g.n[0];
                 } else if (z < 0 || z >= g.n[2] || y < 0 || y >= g.n[1] || x < 0 || x >= g.n[0]) continue;
-                uint32_t cell = (uint32_t)((z * g.n[1] + y) * g.n[0] + x);
+                int64_t cell = cellBase + (z * g.n[1] + y) * g.n[0] + x;
                 int s = cstart[cell];
                 if (s < 0) continue;
                 int e = cend[cell];
-                if (dx == 0 && dy == 0 && dz == 0) s = i + 1;  // own cell: later entries only
+                if (dx == 0 && dy == 0 && dz == 0) s = (int)i + 1;  // own cell: later entries only
                 for (int k = s; k < e; k++) {
                     int j = cvals[k];
                     double4 b0 = p.bmin[j], b1 = p.bmax[j];
@@ -584,7 +761,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pairs(PartView p, GridArgs g, con
                     if (!ov || !mayCollideD(p, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), loneMask)) continue;
                     if (MODE) {
                         int lo = id < j ? id : j, hi = id < j ? j : id;
-                        okeys[o + cnt] = ((uint64_t)(uint32_t)lo << 32) | (uint32_t)hi;
+                        okeys[o + cnt] = tkey | ((uint64_t)(uint32_t)lo << idBits) | (uint32_t)hi;
                         operiods[o + cnt] = packCd(per[0], per[1], per[2]);
                     }
                     cnt++;
@@ -603,7 +780,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pairs(PartView p, GridArgs g, con
         if (!ov || !mayCollideD(p, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), loneMask)) continue;
         if (MODE) {
             int lo = id < j ? id : j, hi = id < j ? j : id;
-            okeys[o + cnt] = ((uint64_t)(uint32_t)lo << 32) | (uint32_t)hi;
+            okeys[o + cnt] = tkey | ((uint64_t)(uint32_t)lo << idBits) | (uint32_t)hi;
             operiods[o + cnt] = packCd(per[0], per[1], per[2]);
         }
         cnt++;
@@ -620,19 +797,40 @@ __device__ __forceinline__ int64_t lowerBound64(const uint64_t* a, int64_t n, ui
     return lo;
 }
 
+// contact key (min id << 32 | max id) of a candidate key of a batch
+__device__ __forceinline__ uint64_t candLocal(uint64_t k, int idBits) {
+    const uint64_t msk = (1ull << idBits) - 1;
+    return (((k >> idBits) & msk) << 32) | (k & msk);
+}
+// first candidate of a scene's segment whose contact key is >= key
+__device__ __forceinline__ int64_t lowerBoundCand(const uint64_t* a, int64_t n, uint64_t key, int idBits) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (candLocal(a[mid], idBits) < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
 // old contacts that stay only because they are real (their bounds no longer overlap)
-__global__ void __launch_bounds__(WDM_BLOCK) k_flag_old(const uint64_t* okey, const uint32_t* oinfo, int64_t nold, const uint64_t* ckey,
-                                                       int64_t ncand, int32_t* flag) {
-    int64_t j = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
-    if (j >= nold) return;
-    uint32_t inf = oinfo[j];
+template <typename T>
+__global__ void __launch_bounds__(WDM_BLOCK) k_flag_old(const ConDesc<T>* C, const FRScene* FR, Map m, const uint64_t* ckey, int32_t* flag,
+                                                       int idBits) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const FRScene& F = FR[t];
+    const ConView<T> oc = C[t].cv;
+    int64_t j = lb * WDM_BLOCK + threadIdx.x;
+    if (j >= F.nOld) return;
+    uint32_t inf = oc.info[j];
     int f = 0;
     if ((inf & CI_REAL) && !(inf & CI_DEAD)) {
-        uint64_t k = okey[j];
-        int64_t lb = lowerBound64(ckey, ncand, k);
-        f = (lb < ncand && ckey[lb] == k) ? 0 : 1;
+        uint64_t k = oc.key[j];
+        const uint64_t* seg = ckey + F.candStart;
+        int64_t l = lowerBoundCand(seg, F.nCand, k, idBits);
+        f = (l < F.nCand && candLocal(seg[l], idBits) == k) ? 0 : 1;
     }
-    flag[j] = f;
+    flag[F.oldStart + j] = f;
 }
 
 template <typename T>
@@ -666,85 +864,160 @@ __device__ __forceinline__ bool dormantPair(const NodeView& n, const PartView& p
     return false;
 }
 
+// ooRank: exclusive prefix (batch-wide) of the flags of k_flag_old; a scene's old contacts
+// with index < j that stay only because they are real number ooRank[oldStart + j] - ooRank[oldStart]
+// A pair that touches now becomes real in the ContactLoop that follows the full run
+// (Cg2_Sphere_Sphere_L6Geom / Cg2_Wall_Sphere_L6Geom accept it): it gets its state row at
+// the full run, in key order, instead of one handed out in execution order by the law.
+// Only the row order depends on this test.
+__device__ __forceinline__ bool touchesNow(const StepArgs& a, const NodeView& n, const PartView& p, int A, int B, uint32_t ty, int32_t cdp) {
+    double4 pa = n.posr[A], pb = n.posr[B];
+    if (ty == CI_SS) {
+        D3 p2 = xyz(pb);
+        if (a.periodic) p2 = p2 + mv3(ldM3(a.cell->hSize), mk3((double)unpack10(cdp, 0), (double)unpack10(cdp, 10), (double)unpack10(cdp, 20)));
+        D3 r = p2 - xyz(pa);
+        double df = fabs(a.distFactor) * (pa.w + pb.w);
+        return sqn3(r) - df * df <= 0;
+    }
+    if (ty == CI_WS) {
+        int ax = ptAxis(p.ptype[A]);
+        return fabs(comp3(xyz(pb), ax) - comp3(xyz(pa), ax)) <= pb.w;
+    }
+    return false;
+}
+
 template <typename T>
-__global__ void __launch_bounds__(WDM_BLOCK) k_emit_cand(NodeView n, PartView p, ConView<T> oldc, int64_t nold, const int64_t* ooRank,
-                                                        const uint64_t* ckey, const int32_t* cper, int64_t ncand, ConView<T> nc,
-                                                        int32_t* oslot, int64_t* rflag, double vd, double df) {
-    int64_t i = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
-    if (i >= ncand) return;
-    uint64_t k = ckey[i];
-    int64_t lb = lowerBound64(oldc.key, nold, k);
-    int64_t d = i + ooRank[lb];
+__global__ void __launch_bounds__(WDM_BLOCK) k_emit_cand(const SceneDesc* D, const ConDesc<T>* C, const FRScene* FR, Map m,
+                                                        const int64_t* ooRank, const uint64_t* ckey, const int32_t* cper, int32_t* oslot,
+                                                        int64_t* rflag, int idBits) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const FRScene& F = FR[t];
+    const SceneDesc& S = D[F.desc];
+    const NodeView n = S.n;
+    const PartView p = S.p;
+    const ConView<T> oldc = C[t].cv, nc = C[t].cvNew;
+    const int64_t nold = F.nOld;
+    const double vd = S.a.verletDist, df = S.a.distFactor;
+    int64_t i = lb * WDM_BLOCK + threadIdx.x;
+    if (i >= F.nCand) return;
+    uint64_t k = candLocal(ckey[F.candStart + i], idBits);
+    int64_t l = lowerBound64(oldc.key, nold, k);
+    int64_t d = i + (ooRank[F.oldStart + l] - ooRank[F.oldStart]);
+    int64_t gd = F.newStart + d;
     nc.key[d] = k;
-    if (lb < nold && oldc.key[lb] == k && !(oldc.info[lb] & CI_DEAD)) {
-        copyContact(oldc, lb, nc, d);
-        uint32_t inf = oldc.info[lb];
+    if (l < nold && oldc.key[l] == k && !(oldc.info[l] & CI_DEAD)) {
+        copyContact(oldc, l, nc, d);
+        uint32_t inf = oldc.info[l];
         bool r = (inf & CI_REAL) != 0;
         uint32_t ni = (inf & ~(CI_PENDING | CI_DEAD | CI_DORMANT)) | CI_OVERLAP;
+        bool fresh = false;  // not real, touching: new row
         if (!r) {
-            int2 id = oldc.ids[lb];
-            if (dormantPair(n, p, id.x, id.y, ni & CI_TYPE_MASK, oldc.cd[lb], vd, df)) ni |= CI_DORMANT;
+            int2 id = oldc.ids[l];
+            if (dormantPair(n, p, id.x, id.y, ni & CI_TYPE_MASK, oldc.cd[l], vd, df)) ni |= CI_DORMANT;
+            else fresh = touchesNow(S.a, n, p, id.x, id.y, ni & CI_TYPE_MASK, oldc.cd[l]);
         }
         nc.info[d] = ni;
-        oslot[d] = r ? oldc.slot[lb] : -1;
-        rflag[d] = r ? 1 : 0;
+        oslot[gd] = r ? oldc.slot[l] : (fresh ? -2 : -1);
+        rflag[gd] = (r || fresh) ? 1 : 0;
     } else {
         int lo = (int)(k >> 32), hi = (int)(k & 0xffffffffu);
         int A, B, ty;
         canonical(lo, hi, ptShape(p.ptype[lo]), ptShape(p.ptype[hi]), A, B, ty);
         nc.ids[d] = make_int2(A, B);
-        int32_t pc = cper[i];  // periods computed in canonical order
+        int32_t pc = cper[F.candStart + i];  // periods computed in canonical order
         nc.cd[d] = pc;
-        nc.info[d] = (uint32_t)ty | CI_OVERLAP | (dormantPair(n, p, A, B, (uint32_t)ty, pc, vd, df) ? CI_DORMANT : 0u);
-        for (int s = 0; s < NSTATE; s++) nc.st[(size_t)s * nc.cap + d] = (T)0;
+        const bool dorm = dormantPair(n, p, A, B, (uint32_t)ty, pc, vd, df);
+        const bool fresh = !dorm && touchesNow(S.a, n, p, A, B, (uint32_t)ty, pc);
+        nc.info[d] = (uint32_t)ty | CI_OVERLAP | (dorm ? CI_DORMANT : 0u);
         nc.gfc[2 * d] = make_double4(0., 0., 0., 0.);
         nc.gfc[2 * d + 1] = make_double4(0., 0., 0., 0.);
-        oslot[d] = -1;
-        rflag[d] = 0;
+        oslot[gd] = fresh ? -2 : -1;
+        rflag[gd] = fresh ? 1 : 0;
     }
 }
 template <typename T>
-__global__ void __launch_bounds__(WDM_BLOCK) k_emit_oor(ConView<T> oldc, int64_t nold, const int32_t* flag, const int64_t* ooRank,
-                                                       const uint64_t* ckey, int64_t ncand, ConView<T> nc, int32_t* oslot,
-                                                       int64_t* rflag) {
-    int64_t j = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
-    if (j >= nold || !flag[j]) return;
+__global__ void __launch_bounds__(WDM_BLOCK) k_emit_oor(const ConDesc<T>* C, const FRScene* FR, Map m, const int32_t* flag,
+                                                       const int64_t* ooRank, const uint64_t* ckey, int32_t* oslot, int64_t* rflag,
+                                                       int idBits) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const FRScene& F = FR[t];
+    const ConView<T> oldc = C[t].cv, nc = C[t].cvNew;
+    int64_t j = lb * WDM_BLOCK + threadIdx.x;
+    if (j >= F.nOld || !flag[F.oldStart + j]) return;
     uint64_t k = oldc.key[j];
-    int64_t d = lowerBound64(ckey, ncand, k) + ooRank[j];
+    int64_t d = lowerBoundCand(ckey + F.candStart, F.nCand, k, idBits) + (ooRank[F.oldStart + j] - ooRank[F.oldStart]);
+    int64_t gd = F.newStart + d;
     nc.key[d] = k;
     copyContact(oldc, j, nc, d);
     nc.info[d] = oldc.info[j] & ~(CI_PENDING | CI_DEAD | CI_OVERLAP | CI_DORMANT);
-    oslot[d] = oldc.slot[j];
-    rflag[d] = 1;
+    oslot[gd] = oldc.slot[j];
+    rflag[gd] = 1;
 }
 
-// dense state rows for the real contacts of the merged list, in key order
+// dense state rows for the real (and touching) contacts of the merged list, in key order
+// (rscan: exclusive batch-wide prefix of rflag); oslot -2: row without state to copy
 template <typename T>
-__global__ void __launch_bounds__(WDM_BLOCK) k_copy_state(ConView<T> oldc, ConView<T> nc, int64_t nnew, const int32_t* oslot,
+__global__ void __launch_bounds__(WDM_BLOCK) k_copy_state(const ConDesc<T>* C, const FRScene* FR, Map m, const int32_t* oslot,
                                                          const int64_t* rscan) {
-    int64_t d = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
-    if (d >= nnew) return;
-    int32_t os = oslot[d];
-    if (os < 0) { nc.slot[d] = -1; return; }
-    int64_t ns = rscan[d];
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const FRScene& F = FR[t];
+    const ConView<T> oldc = C[t].cv, nc = C[t].cvNew;
+    int64_t d = lb * WDM_BLOCK + threadIdx.x;
+    if (d >= F.nNew) return;
+    int32_t os = oslot[F.newStart + d];
+    if (os == -1) { nc.slot[d] = -1; return; }
+    int64_t ns = rscan[F.newStart + d] - rscan[F.newStart];
     nc.slot[d] = (int32_t)ns;
+    if (os < 0) return;
     for (int k = 0; k < NSTATE; k++) nc.st[(size_t)k * nc.cap + ns] = oldc.st[(size_t)k * oldc.cap + os];
 }
-
-// CSR entries: key (particle << 32 | other), value contact index | isB << 31
-__global__ void __launch_bounds__(WDM_BLOCK) k_csr_entries(const int2* ids, int64_t nc, uint64_t* ek, uint32_t* ev) {
-    int64_t c = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
-    if (c >= nc) return;
-    int2 id = ids[c];
-    ek[2 * c] = ((uint64_t)(uint32_t)id.x << 32) | (uint32_t)id.y;
-    ev[2 * c] = (uint32_t)c;
-    ek[2 * c + 1] = ((uint64_t)(uint32_t)id.y << 32) | (uint32_t)id.x;
-    ev[2 * c + 1] = (uint32_t)c | 0x80000000u;
+// next free state row of each scene = number of real contacts carried over
+template <typename T>
+__global__ void k_set_counters(const ConDesc<T>* C, const FRScene* FR, int n, const int64_t* rscan) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const FRScene& F = FR[t];
+    *C[t].cvNew.counter = (int32_t)(rscan[F.newStart + F.nNew] - rscan[F.newStart]);
 }
-__global__ void __launch_bounds__(WDM_BLOCK) k_csr_offsets(const uint64_t* ek, int64_t ne, int np, int64_t* off) {
-    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+
+// CSR entries of the new lists: key (t, particle, other), value contact index | isB << 31
+template <typename T>
+__global__ void __launch_bounds__(WDM_BLOCK) k_csr_entries(const ConDesc<T>* C, const FRScene* FR, Map m, uint64_t* ek, uint32_t* ev,
+                                                          int idBits) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const FRScene& F = FR[t];
+    const int2* ids = C[t].cvNew.ids;
+    int64_t c = lb * WDM_BLOCK + threadIdx.x;
+    if (c >= F.nNew) return;
+    const uint64_t tkey = (uint64_t)t << (2 * idBits);
+    int2 id = ids[c];
+    int64_t g = F.newStart + c;
+    ek[2 * g] = tkey | ((uint64_t)(uint32_t)id.x << idBits) | (uint32_t)id.y;
+    ev[2 * g] = (uint32_t)c;
+    ek[2 * g + 1] = tkey | ((uint64_t)(uint32_t)id.y << idBits) | (uint32_t)id.x;
+    ev[2 * g + 1] = (uint32_t)c | 0x80000000u;
+}
+// copy each scene's sorted entry values into its own CSR array
+__global__ void __launch_bounds__(WDM_BLOCK) k_csr_scatter(const FRScene* FR, Map m, const uint32_t* ev) {
+    int64_t lb;
+    const FRScene& F = FR[mapBlock(m, lb)];
+    int64_t e = lb * WDM_BLOCK + threadIdx.x;
+    if (e >= 2 * F.nNew) return;
+    F.entV[e] = ev[2 * F.newStart + e];
+}
+__global__ void __launch_bounds__(WDM_BLOCK) k_csr_offsets(const SceneDesc* D, const FRScene* FR, Map m, const uint64_t* ek, int idBits) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const FRScene& F = FR[t];
+    const int np = D[F.desc].p.np;
+    int i = (int)(lb * WDM_BLOCK + threadIdx.x);
     if (i > np) return;
-    off[i] = lowerBound64(ek, ne, (uint64_t)(uint32_t)i << 32);
+    const uint64_t key = ((uint64_t)t << (2 * idBits)) | ((uint64_t)(uint32_t)i << idBits);
+    F.csrOff[i] = lowerBound64(ek + 2 * F.newStart, 2 * F.nNew, key);
 }
 
 // ---------------- ContactLoop::run, pkg/dem/ContactLoop.cpp:93-259 ----------------
@@ -776,9 +1049,21 @@ template <typename T> __device__ __forceinline__ V3T<T> trCol(const T* tr, int j
 template <typename T> __device__ __forceinline__ void trSetCol(T* tr, int j, const V3T<T>& c) { tr[j] = c.x; tr[3 + j] = c.y; tr[6 + j] = c.z; }
 
 template <typename T, int LAW>
-__global__ void __launch_bounds__(WDM_BLOCK) k_contacts(StepArgs a, NodeView n, PartView p, ConView<T> cv, int64_t nc, double* epart,
-                                                       uint32_t* creOut, uint32_t* err, uint64_t* realMask) {
-    int64_t c = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
+__global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, const ConDesc<T>* C, Map m, double* epart, int spec) {
+    int64_t lb;
+    const int tt = mapBlock(m, lb);
+    const ConDesc<T>& CD = C[tt];
+    const SceneDesc& S = D[CD.desc];
+    const StepArgs a = S.a;
+    const NodeView n = S.n;
+    const PartView p = S.p;
+    const ConView<T> cv = CD.cv;
+    const int64_t nc = CD.nc;
+    uint32_t* err = S.words + 2;
+    uint32_t* creOut = S.words + 1;
+    uint64_t* realMask = CD.realMask;
+    if (spec && loadWord(S.words) == a.stamp) return;  // the collider runs first: launched again afterwards
+    int64_t c = lb * WDM_BLOCK + threadIdx.x;
     double ev[5] = {0., 0., 0., 0., 0.};  // elast, plast, viscN, viscT, broken
     uint32_t bits = 0;
     bool realOut = false;  // contact is real after this step's law
@@ -801,7 +1086,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(StepArgs a, NodeView n, 
         int32_t cdp = 0;
         if (a.periodic) {
             cdp = cv.cd[c];
-            shift2 = mv3(a.hSize, mk3((double)unpack10(cdp, 0), (double)unpack10(cdp, 10), (double)unpack10(cdp, 20)));
+            shift2 = mv3(ldM3(a.cell->hSize), mk3((double)unpack10(cdp, 0), (double)unpack10(cdp, 10), (double)unpack10(cdp, 20)));
         }
         T tr[9];
         int32_t row = real ? cv.slot[c] : -1;
@@ -917,7 +1202,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(StepArgs a, NodeView n, 
             const double4 cprev = cv.gfc[2 * c + 1];
             const D3 prevContPt = xyz(cprev);
             D3 shiftVel2 = mk3(0., 0., 0.);
-            if (a.periodic) shiftVel2 = mv3(a.shiftVelM, mk3((double)unpack10(cdp, 0), (double)unpack10(cdp, 10), (double)unpack10(cdp, 20)));
+            if (a.periodic) shiftVel2 = mv3(ldM3(a.cell->shiftVelM), mk3((double)unpack10(cdp, 0), (double)unpack10(cdp, 10), (double)unpack10(cdp, 20)));
             V3T<T> normRotVec = cross3(prevNormal, currNormal);
             V3T<T> midNormal = (prevNormal + currNormal) * (T).5;
             if (wsqrt(sqn3(midNormal)) == 0) midNormal = currNormal;
@@ -1101,21 +1386,27 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(StepArgs a, NodeView n, 
     } while (0);
     // one bit per contact, one 64-bit word per wavefront: the gather skips non-real contacts
     uint64_t bal = __ballot(realOut);
-    int64_t w0 = blockIdx.x * (int64_t)WDM_BLOCK + (threadIdx.x & ~63);
+    int64_t w0 = lb * WDM_BLOCK + (threadIdx.x & ~63);
     if ((threadIdx.x & 63) == 0 && w0 < nc) realMask[w0 >> 6] = bal;
-    if (a.trackEnergy) blockSumWrite<5>(ev, bits, epart, nullptr, creOut);
+    if (a.trackEnergy) blockSumWrite<5>(ev, bits, epart, nullptr, 0u, creOut);
 }
 
 // ---------------- scene generation: Jacobi overlap relaxation ----------------
 // Each sphere moves by half of every overlap with its neighbours (found through the
 // same cell grid); all displacements are computed from the old positions, so the
 // result does not depend on thread order.
-__global__ void __launch_bounds__(WDM_BLOCK) k_relax(NodeView n, PartView p, GridArgs g, const uint32_t* ckeys, const int32_t* cvals, int nb,
+__global__ void __launch_bounds__(WDM_BLOCK) k_relax(const SceneDesc* D, const FRScene* FR, Map m, const uint32_t* ckeys, const int32_t* cvals,
                                                     const int32_t* cstart, const int32_t* cend, double4* disp) {
-    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
-    if (i >= nb) return;
+    int64_t lb;
+    const FRScene& F = FR[mapBlock(m, lb)];
+    const SceneDesc& S = D[F.desc];
+    const NodeView n = S.n;
+    const GridArgs g = F.g;  // by value: the loops below store to global memory
+    int li = (int)(lb * WDM_BLOCK + threadIdx.x);
+    if (li >= S.nbinned) return;
+    const int64_t i = F.binStart + li;
     int id = cvals[i];
-    uint32_t ck = ckeys[i];
+    uint32_t ck = (uint32_t)(ckeys[i] - F.cellBase);
     int cx = ck % g.n[0], cy = (ck / g.n[0]) % g.n[1], cz = ck / (g.n[0] * g.n[1]);
     double4 pi = n.posr[id];
     D3 d = mk3(0., 0., 0.);
@@ -1130,7 +1421,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_relax(NodeView n, PartView p, Gri
             for (int dx = -rx; dx <= rx; dx++) {
                 int x = cx + dx;
                 if (g.periodic) x = (x + g.n[0]) % g.n[0]; else if (x < 0 || x >= g.n[0]) continue;
-                uint32_t cell = (uint32_t)((z * g.n[1] + y) * g.n[0] + x);
+                int64_t cell = F.cellBase + (z * g.n[1] + y) * g.n[0] + x;
                 int s = cstart[cell];
                 if (s < 0) continue;
                 int e = cend[cell];
@@ -1153,10 +1444,13 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_relax(NodeView n, PartView p, Gri
     }
     disp[id] = make_double4(d.x, d.y, d.z, 0.);
 }
-__global__ void k_relax_apply(NodeView n, const int32_t* binned, int nb, const double4* disp) {
-    int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
-    if (i >= nb) return;
-    int id = binned[i];
+__global__ void __launch_bounds__(WDM_BLOCK) k_relax_apply(const SceneDesc* D, const FRScene* FR, Map m, const double4* disp) {
+    int64_t lb;
+    const SceneDesc& S = D[FR[mapBlock(m, lb)].desc];
+    const NodeView n = S.n;
+    int i = (int)(lb * WDM_BLOCK + threadIdx.x);
+    if (i >= S.nbinned) return;
+    int id = S.binned[i];
     double4 pr = n.posr[id];
     double4 d = disp[id];
     n.posr[id] = make_double4(pr.x + d.x, pr.y + d.y, pr.z + d.z, pr.w);

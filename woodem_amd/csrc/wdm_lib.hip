@@ -69,13 +69,6 @@ M3 toM3(const HM3& a) { M3 r; for (int i = 0; i < 3; i++) for (int j = 0; j < 3;
 void leviCivitaH(const HM3& m, double out[3]) { out[0] = m.m[1][2] - m.m[2][1]; out[1] = m.m[2][0] - m.m[0][2]; out[2] = m.m[0][1] - m.m[1][0]; }
 
 // ---------------- device buffers ----------------
-struct Ctx;
-struct DevMem {
-    Ctx* ctx = nullptr;
-    std::vector<std::pair<void*, size_t>> owned;
-    size_t bytes = 0;
-};
-
 template <typename T>
 struct DBuf {
     T* p = nullptr;
@@ -87,6 +80,7 @@ struct Timing {
     std::map<std::string, std::pair<double, int64_t>> acc;
 };
 
+struct Shared;
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -96,6 +90,7 @@ struct Ctx {
     int32_t nextId = 0;
     Timing timing;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Shared* sh = nullptr;  // batch scratch, descriptor tables, per-scene words
 };
 
 void* devAlloc(Ctx* c, size_t bytes) {
@@ -146,8 +141,8 @@ struct ConBuf {
         cap = nc;
     }
     ConView<T> view() const {
-        ConView<T> v; v.key = key.p; v.ids = ids.p; v.info = info.p; v.cd = cd.p; v.slot = slot.p; v.counter = counter.p;
-        v.st = st.p; v.gfc = gfc.p; v.cap = cap; return v;
+        ConView<T> v; v.key = gpc(key.p); v.ids = gpc(ids.p); v.info = gpc(info.p); v.cd = gpc(cd.p); v.slot = gpc(slot.p);
+        v.counter = counter.p; v.st = gpc(st.p); v.gfc = gpc(gfc.p); v.cap = cap; return v;
     }
     void free(Ctx* c) {
         release(c, key); release(c, ids); release(c, info); release(c, cd); release(c, st); release(c, gfc);
@@ -155,10 +150,115 @@ struct ConBuf {
     }
 };
 
+// ---------------- staged host->device tables ----------------
+// A device table written by stream-ordered copies from two alternating pinned staging
+// buffers: kernels queued before an upload read the previous contents, kernels queued
+// after it the new ones, and the host never overwrites a staging buffer whose copy is
+// still in flight.
+struct Staged {
+    uint8_t* dev = nullptr;
+    uint8_t* host[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    size_t cap = 0;
+    int k = 0;
+    // device base address valid for an upload of `bytes`
+    uint8_t* prepare(Ctx* c, size_t bytes) {
+        if (bytes <= cap && dev) return dev;
+        size_t nc = std::max<size_t>(bytes + bytes / 2, 4096);
+        HIPCHK(hipStreamSynchronize(c->stream));
+        free(c);
+        dev = (uint8_t*)devAlloc(c, nc);
+        for (int i = 0; i < 2; i++) {
+            HIPCHK(hipHostMalloc((void**)&host[i], nc, hipHostMallocDefault));
+            HIPCHK(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+            HIPCHK(hipEventRecord(ev[i], c->stream));
+        }
+        cap = nc;
+        return dev;
+    }
+    uint8_t* upload(Ctx* c, const void* src, size_t bytes) {
+        prepare(c, bytes);
+        k ^= 1;
+        HIPCHK(hipEventSynchronize(ev[k]));
+        if (bytes) std::memcpy(host[k], src, bytes);
+        if (bytes) HIPCHK(hipMemcpyAsync(dev, host[k], bytes, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipEventRecord(ev[k], c->stream));
+        return dev;
+    }
+    void free(Ctx* c) {
+        if (dev) devFree(c, dev, cap);
+        for (int i = 0; i < 2; i++) {
+            if (host[i]) (void)hipHostFree(host[i]);
+            if (ev[i]) (void)hipEventDestroy(ev[i]);
+            host[i] = nullptr;
+            ev[i] = nullptr;
+        }
+        dev = nullptr;
+        cap = 0;
+    }
+};
+
+// packs several host arrays into one upload (64-byte aligned members)
+struct Blob {
+    std::vector<uint8_t> b;
+    size_t add(const void* p, size_t n) {
+        size_t o = (b.size() + 63) & ~(size_t)63;
+        b.resize(o + n);
+        if (n) std::memcpy(b.data() + o, p, n);
+        return o;
+    }
+    template <typename T>
+    size_t addv(const std::vector<T>& v) { return add(v.data(), v.size() * sizeof(T)); }
+    template <typename T>
+    T* at(size_t off) { return (T*)(b.data() + off); }
+};
+
+// ---------------- per-context state shared by the scenes ----------------
+struct Shared {
+    DBuf<uint32_t> words;  // 4 per scene slot: Verlet trigger, created energy slots, device errors, 0
+    uint32_t* hostWords = nullptr;
+    size_t hostWordsCap = 0;
+    std::vector<int32_t> freeSlots;
+    int32_t nslots = 0;
+    Staged stDesc, stMapNode, stMapCon, stCon, stFR, stConFR, stIdx;
+    Staged stMapP, stMapB, stMapC, stMapO, stMapN, stMapE, stMapP1;
+    hipEvent_t evWords = nullptr;  // recorded after the copy of the words to the host
+    // energy partials per block
+    DBuf<double> epartL, epartC;
+    // collider full-run scratch, batch-wide
+    DBuf<double> extPart, extOut;
+    DBuf<uint32_t> ckeys, ckeys2;
+    DBuf<int32_t> cvals, cvals2, cstart, cend, cnt32, ooFlag, oslot;
+    DBuf<int64_t> counts, offs, ooFlag64, ooRank, rflag, rscan, starts;
+    DBuf<uint64_t> candK, candK2, entK, entK2;
+    DBuf<int32_t> candP, candP2;
+    DBuf<uint32_t> entV, entV2;
+    DBuf<uint8_t> temp;
+    DBuf<double4> disp;
+    void freeAll(Ctx* c) {
+        release(c, words);
+        if (hostWords) (void)hipHostFree(hostWords);
+        hostWords = nullptr;
+        hostWordsCap = 0;
+        for (Staged* s : {&stDesc, &stMapNode, &stMapCon, &stCon, &stFR, &stConFR, &stIdx, &stMapP, &stMapB, &stMapC, &stMapO, &stMapN,
+                          &stMapE, &stMapP1})
+            s->free(c);
+        release(c, epartL); release(c, epartC); release(c, extPart); release(c, extOut);
+        release(c, ckeys); release(c, ckeys2); release(c, cvals); release(c, cvals2); release(c, cstart); release(c, cend);
+        release(c, cnt32); release(c, ooFlag); release(c, oslot); release(c, counts); release(c, offs); release(c, ooFlag64);
+        release(c, ooRank); release(c, rflag); release(c, rscan); release(c, starts); release(c, candK); release(c, candK2);
+        release(c, entK); release(c, entK2); release(c, candP); release(c, candP2); release(c, entV); release(c, entV2);
+        release(c, temp); release(c, disp);
+        if (evWords) (void)hipEventDestroy(evWords);
+        evWords = nullptr;
+    }
+};
+
 // ---------------- Scene (core/Scene.hpp:27) with its DemField (pkg/dem/Particle.hpp:278) ----------------
 struct Scene {
     Ctx* ctx;
     int32_t np, nn, nmat;
+    int32_t slot = -1;  // index of the scene's words in Shared::words
     wdm_params P{};
     bool paramsSet = false, nodesSet = false, partsSet = false, matsSet = false;
     // host copies of static particle data
@@ -187,51 +287,35 @@ struct Scene {
     DBuf<uint32_t> pmask;
     DBuf<int4> fnodes;
     DBuf<double4> bmin, bmax, mat;
-    // contacts, double-buffered
+    // contacts, double-buffered (merge target of the next full run)
     ConBuf<double> cb64[2];
     ConBuf<float> cb32[2];
     int cur = 0;
     int64_t nc = 0;
-    // broadphase scratch
-    DBuf<uint32_t> ckeys, ckeys2;
-    DBuf<int32_t> cvals, cvals2, cstart, cend;
-    DBuf<int64_t> counts, offs, ooFlag64, ooRank;
-    DBuf<int32_t> ooFlag, cnt32, oslot;
-    DBuf<int64_t> rflag, rscan;
-    DBuf<uint64_t> candK, candK2;
-    DBuf<int32_t> candP, candP2;
-    DBuf<double> extPart;
-    DBuf<uint8_t> temp;
-    // CSR
-    DBuf<uint64_t> entK, entK2;
-    DBuf<uint32_t> entV, entV2;
+    // CSR contact -> particle, realness bits of the last ContactLoop
     DBuf<int64_t> csrOff;
+    DBuf<uint32_t> entV;
     DBuf<uint64_t> realMask;
-    // energy, flags
-    DBuf<double> E, epartL, epartC;
-    DBuf<int32_t> eslots;
-    DBuf<uint32_t> orL, ecre, err;
-    uint32_t* hostWords = nullptr;  // pinned: [0] leapfrog or-bits, [1] error bits, [2] contact or-bits
-    uint32_t createdBits = 0;
-    // relax scratch
-    DBuf<double4> disp;
+    // energy accumulators
+    DBuf<double> E;
 
     Scene(Ctx* c, int32_t np_, int32_t nn_, int32_t nm_) : ctx(c), np(np_), nn(nn_), nmat(nm_) {}
 
     bool mixed() const { return P.precision == WDM_PREC_MIXED; }
 
     NodeView nview() {
-        NodeView v; v.posr = posr.p; v.velm = velm.p; v.angI = angI.p; v.ori = ori.p; v.force = force.p; v.torque = torque.p;
-        v.lastPos = lastPos.p; v.flags = flags.p; v.checked = checked.p; v.nn = nn; return v;
+        NodeView v; v.posr = gpc(posr.p); v.velm = gpc(velm.p); v.angI = gpc(angI.p); v.ori = gpc(ori.p); v.force = gpc(force.p);
+        v.torque = gpc(torque.p); v.lastPos = gpc(lastPos.p); v.flags = gpc(flags.p); v.checked = gpc(checked.p); v.nn = nn; return v;
     }
     GatherView gview() {
         GatherView g;
-        g.off = csrOff.p; g.ent = entV2.p; g.realMask = realMask.p;
-        g.gfc = (P.precision == WDM_PREC_MIXED) ? cb32[cur].gfc.p : cb64[cur].gfc.p;
+        g.off = gpc(csrOff.p); g.ent = gpc(entV.p); g.realMask = gpc(realMask.p);
+        g.gfc = gpc(mixed() ? cb32[cur].gfc.p : cb64[cur].gfc.p);
         return g;
     }
     PartView pview() {
-        PartView v; v.ptype = ptype.p; v.mask = pmask.p; v.mat = pmat.p; v.fnodes = fnodes.p; v.bmin = bmin.p; v.bmax = bmax.p; v.np = np; return v;
+        PartView v; v.ptype = gpc(ptype.p); v.mask = gpc(pmask.p); v.mat = gpc(pmat.p); v.fnodes = gpc(fnodes.p); v.bmin = gpc(bmin.p);
+        v.bmax = gpc(bmax.p); v.np = np; return v;
     }
 
     void freeAll() {
@@ -239,13 +323,7 @@ struct Scene {
         release(ctx, lastPos); release(ctx, flags); release(ctx, checked); release(ctx, ptype); release(ctx, pmat); release(ctx, binned);
         release(ctx, unb); release(ctx, pmask); release(ctx, fnodes); release(ctx, bmin); release(ctx, bmax); release(ctx, mat);
         for (int i = 0; i < 2; i++) { cb64[i].free(ctx); cb32[i].free(ctx); }
-        release(ctx, ckeys); release(ctx, ckeys2); release(ctx, cvals); release(ctx, cvals2); release(ctx, cstart); release(ctx, cend);
-        release(ctx, counts); release(ctx, offs); release(ctx, ooFlag64); release(ctx, ooRank); release(ctx, ooFlag); release(ctx, cnt32); release(ctx, oslot); release(ctx, rflag); release(ctx, rscan);
-        release(ctx, candK); release(ctx, candK2); release(ctx, candP); release(ctx, candP2); release(ctx, extPart); release(ctx, temp);
-        release(ctx, entK); release(ctx, entK2); release(ctx, entV); release(ctx, entV2); release(ctx, csrOff); release(ctx, realMask);
-        release(ctx, E); release(ctx, epartL); release(ctx, epartC); release(ctx, eslots); release(ctx, orL); release(ctx, ecre); release(ctx, err);
-        release(ctx, disp);
-        if (hostWords) { (void)hipHostFree(hostWords); hostWords = nullptr; }
+        release(ctx, csrOff); release(ctx, entV); release(ctx, realMask); release(ctx, E);
     }
 };
 
@@ -269,36 +347,62 @@ void checkLaunch() {
 }
 
 // temp storage for hipcub calls
-void* tempFor(Scene& S, size_t bytes) { ensure(S.ctx, S.temp, bytes); return S.temp.p; }
+void* tempFor(Ctx* c, size_t bytes) { ensure(c, c->sh->temp, bytes); return c->sh->temp.p; }
 
 template <typename K, typename V>
-void sortPairs(Scene& S, K* kin, K* kout, V* vin, V* vout, int64_t n, int endBit) {
+void sortPairs(Ctx* c, K* kin, K* kout, V* vin, V* vout, int64_t n, int endBit) {
     if (n <= 0) return;
+    if (n > INT32_MAX) throw WdmError(WDM_E_ARG, "sort of more than 2^31 entries");
     size_t bytes = 0;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, (int)n, 0, endBit, S.ctx->stream));
-    void* t = tempFor(S, bytes);
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(t, bytes, kin, kout, vin, vout, (int)n, 0, endBit, S.ctx->stream));
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, (int)n, 0, endBit, c->stream));
+    void* t = tempFor(c, bytes);
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(t, bytes, kin, kout, vin, vout, (int)n, 0, endBit, c->stream));
 }
 // out[0] = 0, out[i+1] = sum(in[0..i])
-void scanInclusiveShift(Scene& S, const int64_t* in, int64_t* out, int64_t n) {
-    HIPCHK(hipMemsetAsync(out, 0, sizeof(int64_t), S.ctx->stream));
+void scanInclusiveShift(Ctx* c, const int64_t* in, int64_t* out, int64_t n) {
+    HIPCHK(hipMemsetAsync(out, 0, sizeof(int64_t), c->stream));
     if (n <= 0) return;
     size_t bytes = 0;
-    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, in, out + 1, (int)n, S.ctx->stream));
-    void* t = tempFor(S, bytes);
-    HIPCHK(hipcub::DeviceScan::InclusiveSum(t, bytes, in, out + 1, (int)n, S.ctx->stream));
-}
-int64_t readI64(Scene& S, const int64_t* p) {
-    int64_t v = 0;
-    HIPCHK(hipMemcpyAsync(&v, p, sizeof(v), hipMemcpyDeviceToHost, S.ctx->stream));
-    HIPCHK(hipStreamSynchronize(S.ctx->stream));
-    return v;
+    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, in, out + 1, (int)n, c->stream));
+    void* t = tempFor(c, bytes);
+    HIPCHK(hipcub::DeviceScan::InclusiveSum(t, bytes, in, out + 1, (int)n, c->stream));
 }
 
-__global__ void k_i64_to_i32_one(const int64_t* a, int32_t* b) { *b = (int32_t)*a; }
 __global__ void k_i32_to_i64(const int32_t* a, int64_t* b, int64_t n) {
     int64_t i = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
     if (i < n) b[i] = a[i];
+}
+__global__ void k_gather_i64(const int64_t* src, const int64_t* idx, int n, int64_t* out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = src[idx[i]];
+}
+
+// values of a batch-wide exclusive prefix at the first entry of each scene's segment (and the total)
+std::vector<int64_t> segmentStarts(Ctx* c, const int64_t* scan, const std::vector<int64_t>& firstEntry) {
+    Shared& X = *c->sh;
+    int n = (int)firstEntry.size();
+    const int64_t* idx = (const int64_t*)X.stIdx.upload(c, firstEntry.data(), n * sizeof(int64_t));
+    ensure(c, X.starts, n);
+    hipLaunchKernelGGL(k_gather_i64, dim3((n + 63) / 64), dim3(64), 0, c->stream, scan, idx, n, X.starts.p);
+    checkLaunch();
+    std::vector<int64_t> out(n);
+    HIPCHK(hipMemcpyAsync(out.data(), X.starts.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return out;
+}
+
+Map uploadMap(Ctx* c, Staged& s, const std::vector<int64_t>& bprefix) {
+    Map m;
+    m.bprefix = (const int64_t*)s.upload(c, bprefix.data(), bprefix.size() * sizeof(int64_t));
+    m.n = (int)bprefix.size() - 1;
+    return m;
+}
+// block prefix of per-scene element counts
+template <typename F>
+std::vector<int64_t> blockPrefix(int n, F&& count) {
+    std::vector<int64_t> bp(n + 1, 0);
+    for (int t = 0; t < n; t++) bp[t + 1] = bp[t] + (int64_t)nblk(count(t));
+    return bp;
 }
 
 // ---------------- Cell (core/Cell.cpp:28-120) ----------------
@@ -326,7 +430,37 @@ void cellSetNextGradV(Scene& S) {
     for (double& x : S.spinVec) x = x * .5;
 }
 
-StepArgs makeArgs(Scene& S) {
+void makeCellArgs(Scene& S, CellArgs& ca) {
+    std::memset(&ca, 0, sizeof(ca));
+    const double dt = S.P.dt;
+    HM3 pprevL = S.gradV, nnextL = S.nextGradV;
+    HM3 ImL = hzero(), IpL = hzero(), LmL = hzero();
+    double dsv[3] = {0, 0, 0};
+    if (S.periodic && S.homoDeform == 4) {
+        ImL = hinv(hsub(heye(), hdivs(hscale(hadd(nnextL, pprevL), dt), 4.)));
+        IpL = hadd(heye(), hdivs(hscale(hadd(nnextL, pprevL), dt), 4.));
+        LmL = hsub(nnextL, pprevL);
+        double l1[3], l2[3];
+        leviCivitaH(hscale(hsub(pprevL, htrans(pprevL)), .5), l1);
+        leviCivitaH(hscale(hsub(nnextL, htrans(nnextL)), .5), l2);
+        for (int k = 0; k < 3; k++) dsv[k] = l1[k] * -.5 + l2[k] * .5;
+    }
+    ca.ImLL4hInv = toM3(ImL); ca.IpLL4h = toM3(IpL); ca.LmL = toM3(LmL);
+    ca.dGradV = toM3(hsub(nnextL, pprevL));
+    ca.midGradV = toM3(hscale(hadd(pprevL, nnextL), .5));
+    ca.nextGradV = toM3(nnextL); ca.gradV = toM3(pprevL);
+    for (int k = 0; k < 3; k++) { ca.deltaSpinVec[k] = dsv[k]; ca.spinVec[k] = S.spinVec[k]; ca.cellSize[k] = S.size[k]; }
+    ca.hSize = toM3(S.hSize);
+    ca.gatherHSize = toM3(S.hSizeCL);
+    if (S.periodic && S.homoDeform == 4) ca.shiftVelM = toM3(hmul(S.gradV, S.pprevHsize));
+    else if (S.periodic && (S.homoDeform == 2 || S.homoDeform == 3)) ca.shiftVelM = toM3(hmul(S.gradV, S.hSize));
+    else ca.shiftVelM = toM3(hzero());
+}
+
+// nonzero tag of a step, written into the Verlet trigger word by a triggering Leapfrog
+uint32_t stampOf(int64_t step) { return (uint32_t)(step % 0x7fffffffLL) + 1u; }
+
+StepArgs makeStepArgs(Scene& S) {
     StepArgs a;
     std::memset(&a, 0, sizeof(a));
     const wdm_params& P = S.P;
@@ -337,30 +471,11 @@ StepArgs makeArgs(Scene& S) {
     a.homoDeform = S.periodic ? S.homoDeform : -1;
     a.trackEnergy = P.trackEnergy;
     a.reallyTrackEnergy = P.trackEnergy && (!S.periodic || S.step > 0 || hequal(S.gradV, heye()));
-    const double dt = P.dt;
-    HM3 pprevL = S.gradV, nnextL = S.nextGradV;
-    HM3 ImL = hzero(), IpL = hzero(), LmL = hzero();
-    double dsv[3] = {0, 0, 0};
-    if (a.homoDeform == 4) {
-        ImL = hinv(hsub(heye(), hdivs(hscale(hadd(nnextL, pprevL), dt), 4.)));
-        IpL = hadd(heye(), hdivs(hscale(hadd(nnextL, pprevL), dt), 4.));
-        LmL = hsub(nnextL, pprevL);
-        double l1[3], l2[3];
-        leviCivitaH(hscale(hsub(pprevL, htrans(pprevL)), .5), l1);
-        leviCivitaH(hscale(hsub(nnextL, htrans(nnextL)), .5), l2);
-        for (int k = 0; k < 3; k++) dsv[k] = l1[k] * -.5 + l2[k] * .5;
-    }
-    a.ImLL4hInv = toM3(ImL); a.IpLL4h = toM3(IpL); a.LmL = toM3(LmL);
-    a.dGradV = toM3(hsub(nnextL, pprevL));
-    a.midGradV = toM3(hscale(hadd(pprevL, nnextL), .5));
-    a.nextGradV = toM3(nnextL); a.gradV = toM3(pprevL);
-    for (int k = 0; k < 3; k++) { a.deltaSpinVec[k] = dsv[k]; a.spinVec[k] = S.spinVec[k]; a.cellSize[k] = S.size[k]; }
-    a.hSize = toM3(S.hSize);
-    a.gatherHSize = toM3(S.hSizeCL);
-    if (S.periodic && S.homoDeform == 4) a.shiftVelM = toM3(hmul(S.gradV, S.pprevHsize));
-    else if (S.periodic && (S.homoDeform == 2 || S.homoDeform == 3)) a.shiftVelM = toM3(hmul(S.gradV, S.hSize));
-    else a.shiftVelM = toM3(hzero());
+    a.verletDist = S.verletDist;
     a.verletD2 = S.verletDist * S.verletDist;
+    a.checkVerlet = S.lastLeapfrogVerletChecked ? 1 : 0;
+    a.stamp = stampOf(S.step);
+    a.gatherOn = S.forcesFromContacts ? 1 : 0;
     a.distFactor = P.distFactor;
     a.law = P.law; a.noSlip = P.noSlip; a.noBreak = P.noBreak; a.noFrict = P.noFrict; a.noAttraction = P.noAttraction;
     a.trsfRenorm = P.trsfRenorm;
@@ -374,8 +489,25 @@ StepArgs makeArgs(Scene& S) {
         a.viscAlpha = -std::sqrt(5) * le / (std::sqrt(le * le + PI * PI));
         a.hertzDamped = 1;
     } else { a.viscAlpha = 0; a.hertzDamped = 0; }
-    a.mat = S.mat.p;
+    a.mat = gpc(S.mat.p);
+    a.cell = nullptr;
     return a;
+}
+
+SceneDesc makeDesc(Scene& S) {
+    SceneDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.a = makeStepArgs(S);
+    d.n = S.nview();
+    d.p = S.pview();
+    d.gv = S.gview();
+    d.binned = gpc((const int32_t*)S.binned.p);
+    d.unb = gpc((const int32_t*)S.unb.p);
+    d.nbinned = (int)S.binnedH.size();
+    d.nunb = (int)S.unbH.size();
+    d.E = gpc(S.E.p);
+    d.words = S.ctx->sh->words.p + 4 * (size_t)S.slot;
+    return d;
 }
 
 void resolveVerlet(Scene& S) {
@@ -390,278 +522,510 @@ void resolveVerlet(Scene& S) {
     S.verletResolved = true;
 }
 
-void finalizeEnergy(Scene& S, DBuf<double>& part, int nblocks, int nv, int slotOffset) {
-    hipLaunchKernelGGL(k_energy_finalize, dim3(1), dim3(64), 0, S.ctx->stream, part.p, nblocks, nv, S.eslots.p + slotOffset, S.E.p);
-    checkLaunch();
+// the scenes one engine call works on, with their descriptor table on the device
+struct Batch {
+    Ctx* c;
+    std::vector<Scene*> ss;
+    const SceneDesc* D = nullptr;
+    Batch(Ctx* c_, std::vector<Scene*> s) : c(c_), ss(std::move(s)) {}
+    int n() const { return (int)ss.size(); }
+    // descriptor + cell tables of the scenes' current state (one upload)
+    void upload() {
+        Shared& X = *c->sh;
+        int n = (int)ss.size();
+        std::vector<SceneDesc> d(n);
+        std::vector<CellArgs> ca(n);
+        for (int i = 0; i < n; i++) { d[i] = makeDesc(*ss[i]); makeCellArgs(*ss[i], ca[i]); }
+        Blob b;
+        size_t od = b.addv(d), oc = b.addv(ca);
+        uint8_t* base = X.stDesc.prepare(c, b.b.size());
+        SceneDesc* dh = b.at<SceneDesc>(od);
+        for (int i = 0; i < n; i++) dh[i].a.cell = gpc((const CellArgs*)(base + oc) + i);
+        X.stDesc.upload(c, b.b.data(), b.b.size());
+        D = (const SceneDesc*)(base + od);
+    }
+    void ensureDesc() { if (!D) upload(); }
+};
+
+void throwDeviceError(uint32_t e) {
+    if (e & ERR_NAN) throw WdmError(WDM_E_NAN, "NaN force in contact");
+    if (e & ERR_OVERSIZE) throw WdmError(WDM_E_OVERSIZE, "Particle larger than half of the cell size encountered.");
+    if (e & ERR_CAPACITY) throw WdmError(WDM_E_STATE, "contact state rows exhausted");
+    throw WdmError(WDM_E_HIP, "device error");
+}
+// copy every scene's words to the host (synchronising) and raise the first device error among `ss`
+void readWords(Ctx* c, const std::vector<Scene*>& ss) {
+    Shared& X = *c->sh;
+    size_t n = 4 * (size_t)X.nslots;
+    if (n) HIPCHK(hipMemcpyAsync(X.hostWords, X.words.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (Scene* S : ss)
+        if (uint32_t e = X.hostWords[4 * (size_t)S->slot + 2]) throwDeviceError(e);
 }
 
 // ---------------- Leapfrog::run (pkg/dem/Leapfrog.cpp:103-255) ----------------
 struct Leapfrog {
-    static void run(Scene& S) {
-        Ctx* c = S.ctx;
-        resolveVerlet(S);
-        StepArgs a = makeArgs(S);
-        a.checkVerlet = (!S.needFull && S.verletDist > 0) ? 1 : 0;
-        S.lastLeapfrogVerletChecked = a.checkVerlet;
-        int nb = (int)nblk(S.nn);
-        ensure(c, S.epartL, (size_t)nb * 3 + 3);
-        HIPCHK(hipMemsetAsync(S.orL.p, 0, sizeof(uint32_t), c->stream));
-        {
-            KTimer t(c, "leapfrog");
-            if (S.forcesFromContacts)
-                hipLaunchKernelGGL(k_leapfrog<true>, dim3(nb), dim3(WDM_BLOCK), 0, c->stream, a, S.nview(), S.pview(), S.gview(), S.epartL.p, S.orL.p, S.ecre.p);
-            else
-                hipLaunchKernelGGL(k_leapfrog<false>, dim3(nb), dim3(WDM_BLOCK), 0, c->stream, a, S.nview(), S.pview(), S.gview(), S.epartL.p, S.orL.p, S.ecre.p);
-            checkLaunch();
+    // uploads the batch's descriptors; one launch for the scenes whose forces are gathered
+    // from their last ContactLoop, one for the others
+    static void run(Batch& B) {
+        Ctx* c = B.c;
+        Shared& X = *c->sh;
+        const int n = B.n();
+        bool anyEnergy = false;
+        std::vector<int32_t> idx[2];
+        for (int i = 0; i < n; i++) {
+            Scene* S = B.ss[i];
+            resolveVerlet(*S);
+            S->lastLeapfrogVerletChecked = (!S->needFull && S->verletDist > 0);
+            anyEnergy |= S->P.trackEnergy != 0;
+            idx[S->forcesFromContacts ? 1 : 0].push_back(i);
         }
-        if (a.reallyTrackEnergy) finalizeEnergy(S, S.epartL, nb, 3, 0);
+        B.upload();
+        Blob b;
+        size_t oi[2], ob[2];
+        std::vector<int64_t> bp[2];
+        for (int g = 0; g < 2; g++) {
+            bp[g] = blockPrefix((int)idx[g].size(), [&](int t) { return B.ss[idx[g][t]]->nn; });
+            oi[g] = b.addv(idx[g]);
+            ob[g] = b.addv(bp[g]);
+        }
+        uint8_t* base = X.stMapNode.upload(c, b.b.data(), b.b.size());
+        const int64_t nb0 = bp[0].back(), nb1 = bp[1].back();
+        ensure(c, X.epartL, (size_t)(nb0 + nb1) * 3 + 3);
+        for (int g = 0; g < 2; g++) {
+            const int ng = (int)idx[g].size();
+            if (ng == 0) continue;
+            Map m;
+            m.bprefix = (const int64_t*)(base + ob[g]);
+            m.n = ng;
+            const int32_t* id = (const int32_t*)(base + oi[g]);
+            double* ep = X.epartL.p + (g ? 3 * nb0 : 0);
+            const int64_t nbk = bp[g].back();
+            if (nbk > 0) {
+                KTimer t(c, "leapfrog");
+                if (g) hipLaunchKernelGGL(k_leapfrog<1>, dim3((unsigned)nbk), dim3(WDM_BLOCK), 0, c->stream, B.D, id, m, ep);
+                else hipLaunchKernelGGL(k_leapfrog<0>, dim3((unsigned)nbk), dim3(WDM_BLOCK), 0, c->stream, B.D, id, m, ep);
+                checkLaunch();
+            }
+            if (anyEnergy) {
+                hipLaunchKernelGGL(k_energy_finalize, dim3(3 * ng), dim3(WDM_BLOCK), 0, c->stream, B.D, id, 1, m, (const double*)ep, 3, 0);
+                checkLaunch();
+            }
+        }
     }
 };
 
 // ---------------- InsertionSortCollider::run (pkg/dem/InsertionSortCollider.cpp:445-646) ----------------
 struct InsertionSortCollider {
-    static void buildGrid(Scene& S, GridArgs& g, int& nbinned) {
-        Ctx* c = S.ctx;
-        int nb = (int)S.binnedH.size();
-        nbinned = nb;
-        std::memset(&g, 0, sizeof(g));
-        g.periodic = S.periodic;
-        for (int d = 0; d < 3; d++) { g.n[d] = 1; g.w[d] = 1; g.L[d] = S.size[d]; g.invL[d] = 1. / S.size[d]; }
-        if (nb == 0) return;
-        int nbk = (int)nblk(nb);
-        ensure(c, S.extPart, (size_t)nbk * 7);
-        hipLaunchKernelGGL(k_extent, dim3(nbk), dim3(WDM_BLOCK), 0, c->stream, S.pview(), (const int32_t*)S.binned.p, nb, S.extPart.p);
-        checkLaunch();
-        std::vector<double> h((size_t)nbk * 7);
-        HIPCHK(hipMemcpyAsync(h.data(), S.extPart.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY}, ext = 0;
-        for (int b = 0; b < nbk; b++) {
-            for (int d = 0; d < 3; d++) { mn[d] = std::min(mn[d], h[7 * b + d]); mx[d] = std::max(mx[d], h[7 * b + 3 + d]); }
-            ext = std::max(ext, h[7 * b + 6]);
+    // Grids of the scenes L[t] from their bound extents (one synchronisation), then the
+    // bound entries of all of them sorted by (scene, cell) with each cell's entry range.
+    static void buildGrids(Batch& B, const std::vector<int>& L, std::vector<FRScene>& F, const std::vector<int64_t>& bpB, const Map& mB,
+                           const FRScene*& FRd, int64_t& nbTot) {
+        Ctx* c = B.c;
+        Shared& X = *c->sh;
+        hipStream_t st = c->stream;
+        int nT = (int)L.size();
+        const int64_t nblkB = bpB[nT];
+        ensure(c, X.extPart, (size_t)nblkB * 7 + 7);
+        ensure(c, X.extOut, (size_t)nT * 7);
+        if (nblkB > 0) {
+            hipLaunchKernelGGL(k_extent, dim3((unsigned)nblkB), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, X.extPart.p);
+            checkLaunch();
         }
-        if (!(ext > 0) || !std::isfinite(ext)) ext = 1e-12;
-        double w = ext * 1.001;
-        const int64_t maxCells = std::max<int64_t>(4096, 2 * (int64_t)nb);
-        for (int it = 0; it < 200; it++) {
-            int64_t tot = 1;
-            for (int d = 0; d < 3; d++) {
-                if (S.periodic) {
-                    int n = (int)std::floor(S.size[d] / w);
-                    if (n < 3) n = 1;
-                    g.n[d] = n;
-                    g.w[d] = S.size[d] / n;
-                    g.origin[d] = 0;
-                } else {
-                    double e = mx[d] - mn[d];
-                    int n = (int)std::floor(e / w) + 1;
-                    if (n < 1) n = 1;
-                    g.n[d] = n;
-                    g.w[d] = w;
-                    g.origin[d] = mn[d];
+        hipLaunchKernelGGL(k_extent_finalize, dim3(nT), dim3(WDM_BLOCK), 0, st, mB, (const double*)X.extPart.p, X.extOut.p);
+        checkLaunch();
+        std::vector<double> h((size_t)nT * 7);
+        HIPCHK(hipMemcpyAsync(h.data(), X.extOut.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        int64_t cellBase = 0;
+        nbTot = 0;
+        for (int t = 0; t < nT; t++) {
+            Scene& S = *B.ss[L[t]];
+            GridArgs& g = F[t].g;
+            std::memset(&g, 0, sizeof(g));
+            g.periodic = S.periodic;
+            for (int d = 0; d < 3; d++) { g.n[d] = 1; g.w[d] = 1; g.L[d] = S.size[d]; g.invL[d] = 1. / S.size[d]; }
+            int nb = (int)S.binnedH.size();
+            if (nb > 0) {
+                const double* e = &h[(size_t)t * 7];
+                double ext = e[6];
+                if (!(ext > 0) || !std::isfinite(ext)) ext = 1e-12;
+                double w = ext * 1.001;
+                const int64_t maxCells = std::max<int64_t>(4096, 2 * (int64_t)nb);
+                for (int it = 0; it < 200; it++) {
+                    int64_t tot = 1;
+                    for (int d = 0; d < 3; d++) {
+                        if (S.periodic) {
+                            int n = (int)std::floor(S.size[d] / w);
+                            if (n < 3) n = 1;
+                            g.n[d] = n;
+                            g.w[d] = S.size[d] / n;
+                            g.origin[d] = 0;
+                        } else {
+                            double ee = e[3 + d] - e[d];
+                            int n = (int)std::floor(ee / w) + 1;
+                            if (n < 1) n = 1;
+                            g.n[d] = n;
+                            g.w[d] = w;
+                            g.origin[d] = e[d];
+                        }
+                        tot *= g.n[d];
+                    }
+                    if (tot <= maxCells) break;
+                    w *= 1.26;
                 }
-                tot *= g.n[d];
             }
-            if (tot <= maxCells) break;
-            w *= 1.26;
+            F[t].cellBase = cellBase;
+            F[t].binStart = nbTot;
+            cellBase += (int64_t)g.n[0] * g.n[1] * g.n[2];
+            nbTot += nb;
         }
-        int64_t G = (int64_t)g.n[0] * g.n[1] * g.n[2];
-        ensure(c, S.ckeys, nb); ensure(c, S.ckeys2, nb); ensure(c, S.cvals, nb); ensure(c, S.cvals2, nb);
-        ensure(c, S.cstart, G); ensure(c, S.cend, G);
-        hipLaunchKernelGGL(k_cell_keys, dim3(nbk), dim3(WDM_BLOCK), 0, c->stream, S.pview(), g, (const int32_t*)S.binned.p, nb, S.ckeys.p, S.cvals.p);
-        checkLaunch();
-        sortPairs(S, S.ckeys.p, S.ckeys2.p, S.cvals.p, S.cvals2.p, nb, bitsFor((uint64_t)G));
-        hipLaunchKernelGGL(k_fill_i32, dim3(nblk(G)), dim3(WDM_BLOCK), 0, c->stream, S.cstart.p, G, -1);
-        hipLaunchKernelGGL(k_cell_ranges, dim3(nbk), dim3(WDM_BLOCK), 0, c->stream, (const uint32_t*)S.ckeys2.p, nb, S.cstart.p, S.cend.p);
+        if (cellBase >= INT32_MAX || nbTot >= INT32_MAX) throw WdmError(WDM_E_ARG, "collider batch too large (2^31 cells or bounds)");
+        FRd = (const FRScene*)X.stFR.upload(c, F.data(), F.size() * sizeof(FRScene));
+        const int64_t G = cellBase;
+        ensure(c, X.ckeys, nbTot); ensure(c, X.ckeys2, nbTot); ensure(c, X.cvals, nbTot); ensure(c, X.cvals2, nbTot);
+        ensure(c, X.cstart, G); ensure(c, X.cend, G);
+        if (nblkB > 0) {
+            hipLaunchKernelGGL(k_cell_keys, dim3((unsigned)nblkB), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, X.ckeys.p, X.cvals.p);
+            checkLaunch();
+        }
+        sortPairs(c, X.ckeys.p, X.ckeys2.p, X.cvals.p, X.cvals2.p, nbTot, bitsFor((uint64_t)G));
+        hipLaunchKernelGGL(k_fill_i32, dim3(nblk(G)), dim3(WDM_BLOCK), 0, st, X.cstart.p, G, -1);
+        if (nbTot > 0)
+            hipLaunchKernelGGL(k_cell_ranges, dim3(nblk(nbTot)), dim3(WDM_BLOCK), 0, st, (const uint32_t*)X.ckeys2.p, nbTot, X.cstart.p, X.cend.p);
         checkLaunch();
     }
 
     template <typename T>
-    static void fullRun(Scene& S, ConBuf<T>* cb) {
-        Ctx* c = S.ctx;
+    static ConBuf<T>* bufs(Scene& S);
+
+    template <typename T>
+    static void fullRun(Batch& B, const std::vector<int>& L) {
+        Ctx* c = B.c;
+        Shared& X = *c->sh;
         hipStream_t st = c->stream;
+        const int nT = (int)L.size();
+        if (nT == 0) return;
+        auto S_ = [&](int t) -> Scene& { return *B.ss[L[t]]; };
+        int maxNp = 1;
+        for (int t = 0; t < nT; t++) maxNp = std::max(maxNp, S_(t).np);
+        const int idBits = bitsFor((uint64_t)maxNp);
+        const int tBits = bitsFor((uint64_t)(nT - 1));
+        if (2 * idBits + tBits > 64) throw WdmError(WDM_E_ARG, "collider batch too large for 64-bit pair keys");
+        const int keyBits = 2 * idBits + tBits;
+
+        std::vector<FRScene> F(nT);
+        std::memset(F.data(), 0, F.size() * sizeof(FRScene));
+        for (int t = 0; t < nT; t++) F[t].desc = L[t];
+        const FRScene* FRd = (const FRScene*)X.stFR.upload(c, F.data(), F.size() * sizeof(FRScene));
+        std::vector<int64_t> bpP = blockPrefix(nT, [&](int t) { return S_(t).np; });
+        std::vector<int64_t> bpB = blockPrefix(nT, [&](int t) { return (int64_t)S_(t).binnedH.size(); });
+        Map mP = uploadMap(c, X.stMapP, bpP), mB = uploadMap(c, X.stMapB, bpB);
+        if (bpP[nT] > 0) {
+            KTimer tm(c, "bounds");
+            hipLaunchKernelGGL(k_bounds, dim3((unsigned)bpP[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mP, 0);
+            checkLaunch();
+        }
+        int64_t nbTot = 0;
+        buildGrids(B, L, F, bpB, mB, FRd, nbTot);
+
+        // candidate pairs: count, scan, write, sort by (scene, min id, max id)
+        int64_t ncandTot = 0;
         {
-            KTimer t(c, "bounds");
-            hipLaunchKernelGGL(k_bounds, dim3(nblk(S.np)), dim3(WDM_BLOCK), 0, st, S.nview(), S.pview(), S.P.distFactor, S.verletDist);
-            checkLaunch();
-        }
-        GridArgs g;
-        int nb;
-        buildGrid(S, g, nb);
-        int64_t ncand = 0;
-        if (nb > 0) {
-            ensure(c, S.counts, nb); ensure(c, S.offs, (size_t)nb + 1);
-            ensure(c, S.cnt32, nb);
-            int nbk = (int)nblk(nb);
-            {
-                KTimer t(c, "pairs_count");
-                hipLaunchKernelGGL(k_pairs<0>, dim3(nbk), dim3(WDM_BLOCK), 0, st, S.pview(), g, (const uint32_t*)S.ckeys2.p, (const int32_t*)S.cvals2.p, nb,
-                                   (const int32_t*)S.cstart.p, (const int32_t*)S.cend.p, (const int32_t*)S.unb.p, (int)S.unbH.size(), S.P.loneMask,
-                                   S.cnt32.p, (const int64_t*)nullptr, (uint64_t*)nullptr, (int32_t*)nullptr, S.err.p);
+            ensure(c, X.cnt32, nbTot); ensure(c, X.counts, nbTot); ensure(c, X.offs, (size_t)nbTot + 1);
+            if (bpB[nT] > 0) {
+                KTimer tm(c, "pairs_count");
+                hipLaunchKernelGGL(k_pairs<0>, dim3((unsigned)bpB[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
+                                   (const int32_t*)X.cvals2.p, (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p, X.cnt32.p,
+                                   (const int64_t*)nullptr, (uint64_t*)nullptr, (int32_t*)nullptr, idBits);
+                checkLaunch();
+                hipLaunchKernelGGL(k_i32_to_i64, dim3(nblk(nbTot)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.cnt32.p, X.counts.p, nbTot);
                 checkLaunch();
             }
-            hipLaunchKernelGGL(k_i32_to_i64, dim3(nbk), dim3(WDM_BLOCK), 0, st, (const int32_t*)S.cnt32.p, S.counts.p, (int64_t)nb);
-            checkLaunch();
-            scanInclusiveShift(S, S.counts.p, S.offs.p, nb);
-            ncand = readI64(S, S.offs.p + nb);
-            ensure(c, S.candK, ncand); ensure(c, S.candK2, ncand); ensure(c, S.candP, ncand); ensure(c, S.candP2, ncand);
-            if (ncand > 0) {
-                KTimer t(c, "pairs_write");
-                hipLaunchKernelGGL(k_pairs<1>, dim3(nbk), dim3(WDM_BLOCK), 0, st, S.pview(), g, (const uint32_t*)S.ckeys2.p, (const int32_t*)S.cvals2.p, nb,
-                                   (const int32_t*)S.cstart.p, (const int32_t*)S.cend.p, (const int32_t*)S.unb.p, (int)S.unbH.size(), S.P.loneMask,
-                                   (int32_t*)nullptr, (const int64_t*)S.offs.p, S.candK.p, S.candP.p, S.err.p);
+            scanInclusiveShift(c, X.counts.p, X.offs.p, nbTot);
+            std::vector<int64_t> first(nT + 1);
+            for (int t = 0; t < nT; t++) first[t] = F[t].binStart;
+            first[nT] = nbTot;
+            std::vector<int64_t> cs = segmentStarts(c, X.offs.p, first);
+            for (int t = 0; t < nT; t++) { F[t].candStart = cs[t]; F[t].nCand = cs[t + 1] - cs[t]; }
+            ncandTot = cs[nT];
+            ensure(c, X.candK, ncandTot); ensure(c, X.candK2, ncandTot); ensure(c, X.candP, ncandTot); ensure(c, X.candP2, ncandTot);
+            if (ncandTot > 0) {
+                KTimer tm(c, "pairs_write");
+                hipLaunchKernelGGL(k_pairs<1>, dim3((unsigned)bpB[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
+                                   (const int32_t*)X.cvals2.p, (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p, (int32_t*)nullptr,
+                                   (const int64_t*)X.offs.p, X.candK.p, X.candP.p, idBits);
                 checkLaunch();
-                sortPairs(S, S.candK.p, S.candK2.p, S.candP.p, S.candP2.p, ncand, 32 + bitsFor((uint64_t)S.np));
+                sortPairs(c, X.candK.p, X.candK2.p, X.candP.p, X.candP2.p, ncandTot, keyBits);
             }
         }
-        // merge with the previous contact list
-        ConBuf<T>& oldb = cb[S.cur];
-        ConBuf<T>& newb = cb[1 - S.cur];
-        int64_t nold = S.nc;
-        ensure(c, S.ooFlag, std::max<int64_t>(nold, 1)); ensure(c, S.ooFlag64, std::max<int64_t>(nold, 1)); ensure(c, S.ooRank, (size_t)nold + 1);
-        int64_t noor = 0;
-        if (nold > 0) {
-            hipLaunchKernelGGL(k_flag_old, dim3(nblk(nold)), dim3(WDM_BLOCK), 0, st, (const uint64_t*)oldb.key.p, (const uint32_t*)oldb.info.p, nold,
-                               (const uint64_t*)S.candK2.p, ncand, S.ooFlag.p);
-            hipLaunchKernelGGL(k_i32_to_i64, dim3(nblk(nold)), dim3(WDM_BLOCK), 0, st, (const int32_t*)S.ooFlag.p, S.ooFlag64.p, nold);
+
+        // old contacts kept only because they are real
+        std::vector<ConDesc<T>> C(nT);
+        std::memset(C.data(), 0, C.size() * sizeof(ConDesc<T>));
+        int64_t nOldTot = 0;
+        for (int t = 0; t < nT; t++) {
+            Scene& S = S_(t);
+            ConBuf<T>* cb = bufs<T>(S);
+            if (cb[S.cur].cap == 0) cb[S.cur].reserve(c, 1);
+            C[t].cv = cb[S.cur].view();
+            C[t].nc = S.nc;
+            C[t].desc = L[t];
+            F[t].oldStart = nOldTot;
+            F[t].nOld = S.nc;
+            nOldTot += S.nc;
+        }
+        FRd = (const FRScene*)X.stFR.upload(c, F.data(), F.size() * sizeof(FRScene));
+        const ConDesc<T>* Cd = (const ConDesc<T>*)X.stConFR.upload(c, C.data(), C.size() * sizeof(ConDesc<T>));
+        std::vector<int64_t> bpO = blockPrefix(nT, [&](int t) { return F[t].nOld; });
+        Map mO = uploadMap(c, X.stMapO, bpO);
+        ensure(c, X.ooFlag, std::max<int64_t>(nOldTot, 1)); ensure(c, X.ooFlag64, std::max<int64_t>(nOldTot, 1));
+        ensure(c, X.ooRank, (size_t)nOldTot + 1);
+        std::vector<int64_t> noor(nT, 0);
+        if (nOldTot > 0) {
+            hipLaunchKernelGGL(k_flag_old<T>, dim3((unsigned)bpO[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mO, (const uint64_t*)X.candK2.p,
+                               X.ooFlag.p, idBits);
+            hipLaunchKernelGGL(k_i32_to_i64, dim3(nblk(nOldTot)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.ooFlag.p, X.ooFlag64.p, nOldTot);
             checkLaunch();
-            scanInclusiveShift(S, S.ooFlag64.p, S.ooRank.p, nold);
-            noor = readI64(S, S.ooRank.p + nold);
+            scanInclusiveShift(c, X.ooFlag64.p, X.ooRank.p, nOldTot);
+            std::vector<int64_t> first(nT + 1);
+            for (int t = 0; t < nT; t++) first[t] = F[t].oldStart;
+            first[nT] = nOldTot;
+            std::vector<int64_t> os = segmentStarts(c, X.ooRank.p, first);
+            for (int t = 0; t < nT; t++) noor[t] = os[t + 1] - os[t];
         } else {
-            HIPCHK(hipMemsetAsync(S.ooRank.p, 0, sizeof(int64_t), st));
+            HIPCHK(hipMemsetAsync(X.ooRank.p, 0, sizeof(int64_t), st));
         }
-        int64_t nnew = ncand + noor;
-        newb.reserve(c, std::max<int64_t>(nnew, 1));
-        if (oldb.cap == 0) oldb.reserve(c, 1);
-        ensure(c, S.oslot, std::max<int64_t>(nnew, 1)); ensure(c, S.rflag, std::max<int64_t>(nnew, 1));
-        ensure(c, S.rscan, (size_t)nnew + 1);
+
+        // merged lists
+        int64_t nNewTot = 0, noorTot = 0;
+        for (int t = 0; t < nT; t++) {
+            Scene& S = S_(t);
+            ConBuf<T>* cb = bufs<T>(S);
+            int64_t nnew = F[t].nCand + noor[t];
+            noorTot += noor[t];
+            cb[1 - S.cur].reserve(c, std::max<int64_t>(nnew, 1));
+            if (cb[S.cur].cap == 0) cb[S.cur].reserve(c, 1);
+            ensure(c, S.entV, (size_t)std::max<int64_t>(2 * nnew, 1));
+            C[t].cv = cb[S.cur].view();
+            C[t].cvNew = cb[1 - S.cur].view();
+            F[t].newStart = nNewTot;
+            F[t].nNew = nnew;
+            F[t].csrOff = gpc(S.csrOff.p);
+            F[t].entV = gpc(S.entV.p);
+            nNewTot += nnew;
+        }
+        FRd = (const FRScene*)X.stFR.upload(c, F.data(), F.size() * sizeof(FRScene));
+        Cd = (const ConDesc<T>*)X.stConFR.upload(c, C.data(), C.size() * sizeof(ConDesc<T>));
+        ensure(c, X.oslot, std::max<int64_t>(nNewTot, 1)); ensure(c, X.rflag, std::max<int64_t>(nNewTot, 1));
+        ensure(c, X.rscan, (size_t)nNewTot + 1);
+        std::vector<int64_t> bpC = blockPrefix(nT, [&](int t) { return F[t].nCand; });
+        std::vector<int64_t> bpN = blockPrefix(nT, [&](int t) { return F[t].nNew; });
+        Map mC = uploadMap(c, X.stMapC, bpC), mN = uploadMap(c, X.stMapN, bpN);
         {
-            KTimer t(c, "merge");
-            if (ncand > 0)
-                hipLaunchKernelGGL(k_emit_cand<T>, dim3(nblk(ncand)), dim3(WDM_BLOCK), 0, st, S.nview(), S.pview(), oldb.view(), nold,
-                                   (const int64_t*)S.ooRank.p, (const uint64_t*)S.candK2.p, (const int32_t*)S.candP2.p, ncand, newb.view(),
-                                   S.oslot.p, S.rflag.p, S.verletDist, S.P.distFactor);
-            if (noor > 0)
-                hipLaunchKernelGGL(k_emit_oor<T>, dim3(nblk(nold)), dim3(WDM_BLOCK), 0, st, oldb.view(), nold, (const int32_t*)S.ooFlag.p,
-                                   (const int64_t*)S.ooRank.p, (const uint64_t*)S.candK2.p, ncand, newb.view(), S.oslot.p, S.rflag.p);
+            KTimer tm(c, "merge");
+            if (ncandTot > 0)
+                hipLaunchKernelGGL(k_emit_cand<T>, dim3((unsigned)bpC[nT]), dim3(WDM_BLOCK), 0, st, B.D, Cd, FRd, mC, (const int64_t*)X.ooRank.p,
+                                   (const uint64_t*)X.candK2.p, (const int32_t*)X.candP2.p, X.oslot.p, X.rflag.p, idBits);
+            if (noorTot > 0)
+                hipLaunchKernelGGL(k_emit_oor<T>, dim3((unsigned)bpO[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mO, (const int32_t*)X.ooFlag.p,
+                                   (const int64_t*)X.ooRank.p, (const uint64_t*)X.candK2.p, X.oslot.p, X.rflag.p, idBits);
             checkLaunch();
-            scanInclusiveShift(S, S.rflag.p, S.rscan.p, nnew);
-            if (nnew > 0)
-                hipLaunchKernelGGL(k_copy_state<T>, dim3(nblk(nnew)), dim3(WDM_BLOCK), 0, st, oldb.view(), newb.view(), nnew,
-                                   (const int32_t*)S.oslot.p, (const int64_t*)S.rscan.p);
-            checkLaunch();
-            // next free row = number of real contacts carried over
-            hipLaunchKernelGGL(k_i64_to_i32_one, dim3(1), dim3(1), 0, st, (const int64_t*)S.rscan.p + nnew, newb.counter.p);
+            scanInclusiveShift(c, X.rflag.p, X.rscan.p, nNewTot);
+            if (nNewTot > 0)
+                hipLaunchKernelGGL(k_copy_state<T>, dim3((unsigned)bpN[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mN, (const int32_t*)X.oslot.p,
+                                   (const int64_t*)X.rscan.p);
+            hipLaunchKernelGGL(k_set_counters<T>, dim3((nT + 63) / 64), dim3(64), 0, st, Cd, FRd, nT, (const int64_t*)X.rscan.p);
             checkLaunch();
         }
-        S.cur = 1 - S.cur;
-        S.nc = nnew;
+        for (int t = 0; t < nT; t++) {
+            Scene& S = S_(t);
+            S.cur = 1 - S.cur;
+            S.nc = F[t].nNew;
+            S.nFullRuns++;
+            S.needFull = false;
+        }
         // CSR (contact -> particle)
-        int64_t ne = 2 * nnew;
-        ensure(c, S.entK, std::max<int64_t>(ne, 1)); ensure(c, S.entK2, std::max<int64_t>(ne, 1));
-        ensure(c, S.entV, std::max<int64_t>(ne, 1)); ensure(c, S.entV2, std::max<int64_t>(ne, 1));
-        ensure(c, S.csrOff, (size_t)S.np + 1);
         {
-            KTimer t(c, "csr");
-            if (nnew > 0) {
-                hipLaunchKernelGGL(k_csr_entries, dim3(nblk(nnew)), dim3(WDM_BLOCK), 0, st, (const int2*)newb.ids.p, nnew, S.entK.p, S.entV.p);
+            KTimer tm(c, "csr");
+            int64_t ne = 2 * nNewTot;
+            ensure(c, X.entK, std::max<int64_t>(ne, 1)); ensure(c, X.entK2, std::max<int64_t>(ne, 1));
+            ensure(c, X.entV, std::max<int64_t>(ne, 1)); ensure(c, X.entV2, std::max<int64_t>(ne, 1));
+            if (nNewTot > 0) {
+                hipLaunchKernelGGL(k_csr_entries<T>, dim3((unsigned)bpN[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mN, X.entK.p, X.entV.p, idBits);
                 checkLaunch();
-                sortPairs(S, S.entK.p, S.entK2.p, S.entV.p, S.entV2.p, ne, 32 + bitsFor((uint64_t)S.np));
+                sortPairs(c, X.entK.p, X.entK2.p, X.entV.p, X.entV2.p, ne, keyBits);
+                std::vector<int64_t> bpE = blockPrefix(nT, [&](int t) { return 2 * F[t].nNew; });
+                Map mE = uploadMap(c, X.stMapE, bpE);
+                hipLaunchKernelGGL(k_csr_scatter, dim3((unsigned)bpE[nT]), dim3(WDM_BLOCK), 0, st, FRd, mE, (const uint32_t*)X.entV2.p);
+                checkLaunch();
             }
-            hipLaunchKernelGGL(k_csr_offsets, dim3(nblk(S.np + 1)), dim3(WDM_BLOCK), 0, st, (const uint64_t*)S.entK2.p, ne, S.np, S.csrOff.p);
+            std::vector<int64_t> bp1 = blockPrefix(nT, [&](int t) { return (int64_t)S_(t).np + 1; });
+            Map m1 = uploadMap(c, X.stMapP1, bp1);
+            hipLaunchKernelGGL(k_csr_offsets, dim3((unsigned)bp1[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, m1, (const uint64_t*)X.entK2.p, idBits);
             checkLaunch();
         }
-        S.nFullRuns++;
-        S.needFull = false;
     }
 
-    static void run(Scene& S) {
-        Ctx* c = S.ctx;
-        resolveVerlet(S);
-        bool full = S.needFull || S.verletDist == 0;
-        if (!full && S.lastLeapfrogVerletChecked) {
-            // Verlet decision of updateBboxes_doFullRun (:340-350), computed by the Leapfrog kernel
-            HIPCHK(hipMemcpyAsync(S.hostWords, S.orL.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipMemcpyAsync(S.hostWords + 1, S.err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(hipStreamSynchronize(c->stream));
-            if (S.hostWords[1]) throwDeviceError(S.hostWords[1]);
-            full = (S.hostWords[0] & VERLET_BIT) != 0;
+    // scenes that need a full run this step: after an upload, with verletDist 0, or when
+    // the Leapfrog of this step moved a node by more than verletDist (words read by the caller)
+    static std::vector<int> fullList(Batch& B, bool wordsRead) {
+        std::vector<int> L;
+        for (int i = 0; i < B.n(); i++) {
+            Scene& S = *B.ss[i];
+            resolveVerlet(S);
+            bool full = S.needFull || S.verletDist == 0;
+            if (!full && S.lastLeapfrogVerletChecked && wordsRead) full = B.c->sh->hostWords[4 * (size_t)S.slot] == stampOf(S.step);
+            if (full) L.push_back(i);
         }
-        if (!full) return;  // pending removals are decided inside the contact kernel
-        if (S.mixed()) fullRun<float>(S, S.cb32);
-        else fullRun<double>(S, S.cb64);
+        return L;
     }
-
-    static void throwDeviceError(uint32_t e) {
-        if (e & ERR_NAN) throw WdmError(WDM_E_NAN, "NaN force in contact");
-        if (e & ERR_OVERSIZE) throw WdmError(WDM_E_OVERSIZE, "Particle larger than half of the cell size encountered.");
-        if (e & ERR_CAPACITY) throw WdmError(WDM_E_STATE, "contact state rows exhausted");
-        throw WdmError(WDM_E_HIP, "device error");
+    static void runList(Batch& B, const std::vector<int>& L) {
+        if (L.empty()) return;  // pending removals are decided inside the contact kernel
+        std::vector<int> L64, L32;
+        for (int i : L) (B.ss[i]->mixed() ? L32 : L64).push_back(i);
+        B.ensureDesc();
+        fullRun<double>(B, L64);
+        fullRun<float>(B, L32);
+    }
+    static void run(Batch& B) {
+        bool anyCheck = false;
+        for (Scene* S : B.ss) anyCheck |= (!S->needFull && S->lastLeapfrogVerletChecked);
+        if (anyCheck) readWords(B.c, B.ss);  // the Verlet decision of updateBboxes_doFullRun (:340-350)
+        runList(B, fullList(B, anyCheck));
     }
 };
+template <>
+ConBuf<double>* InsertionSortCollider::bufs<double>(Scene& S) { return S.cb64; }
+template <>
+ConBuf<float>* InsertionSortCollider::bufs<float>(Scene& S) { return S.cb32; }
 
 // ---------------- ContactLoop::run (pkg/dem/ContactLoop.cpp:93-259) ----------------
 struct ContactLoop {
+    // spec: launched before the collider's decision is known; a scene whose Verlet trigger
+    // fired skips it and is launched again after its full run
     template <typename T>
-    static void runT(Scene& S, ConBuf<T>* cb) {
-        Ctx* c = S.ctx;
-        StepArgs a = makeArgs(S);
-        ConBuf<T>& b = cb[S.cur];
-        if (b.cap == 0) b.reserve(c, 1);
-        int64_t nc = S.nc;
-        unsigned nbk = nblk(std::max<int64_t>(nc, 1));
-        ensure(c, S.epartC, (size_t)nbk * 5 + 5);
-        ensure(c, S.realMask, (size_t)nbk * (WDM_BLOCK / 64) + 1);
-        if (nc == 0) HIPCHK(hipMemsetAsync(S.realMask.p, 0, sizeof(uint64_t), c->stream));
-        if (nc > 0) {
-            KTimer t(c, "contacts");
-            if (S.P.law == WDM_LAW_HERTZ_DMT)
-                hipLaunchKernelGGL((k_contacts<T, 1>), dim3(nbk), dim3(WDM_BLOCK), 0, c->stream, a, S.nview(), S.pview(), b.view(), nc, S.epartC.p, S.ecre.p, S.err.p, S.realMask.p);
+    static void runGroup(Batch& B, const std::vector<int>& L, int law, bool spec) {
+        Ctx* c = B.c;
+        Shared& X = *c->sh;
+        const int n = (int)L.size();
+        if (n == 0) return;
+        std::vector<ConDesc<T>> C(n);
+        std::memset(C.data(), 0, C.size() * sizeof(ConDesc<T>));
+        bool anyEnergy = false;
+        for (int t = 0; t < n; t++) {
+            Scene& S = *B.ss[L[t]];
+            ConBuf<T>* cb = InsertionSortCollider::bufs<T>(S);
+            if (cb[S.cur].cap == 0) cb[S.cur].reserve(c, 1);
+            ensure(c, S.realMask, (size_t)nblk(std::max<int64_t>(S.nc, 1)) * (WDM_BLOCK / 64) + 1);
+            C[t].cv = cb[S.cur].view();
+            C[t].nc = S.nc;
+            C[t].realMask = gpc(S.realMask.p);
+            C[t].desc = L[t];
+            anyEnergy |= S.P.trackEnergy != 0;
+        }
+        std::vector<int64_t> bp = blockPrefix(n, [&](int t) { return C[t].nc; });
+        Blob b;
+        size_t oc = b.addv(C), om = b.addv(bp);
+        uint8_t* base = X.stCon.upload(c, b.b.data(), b.b.size());
+        const ConDesc<T>* Cd = (const ConDesc<T>*)(base + oc);
+        Map m;
+        m.bprefix = (const int64_t*)(base + om);
+        m.n = n;
+        ensure(c, X.epartC, (size_t)bp[n] * 5 + 5);
+        if (bp[n] > 0) {
+            KTimer tm(c, "contacts");
+            if (law == WDM_LAW_HERTZ_DMT)
+                hipLaunchKernelGGL((k_contacts<T, 1>), dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, c->stream, B.D, Cd, m, X.epartC.p, spec ? 1 : 0);
             else
-                hipLaunchKernelGGL((k_contacts<T, 0>), dim3(nbk), dim3(WDM_BLOCK), 0, c->stream, a, S.nview(), S.pview(), b.view(), nc, S.epartC.p, S.ecre.p, S.err.p, S.realMask.p);
+                hipLaunchKernelGGL((k_contacts<T, 0>), dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, c->stream, B.D, Cd, m, X.epartC.p, spec ? 1 : 0);
             checkLaunch();
         }
-        if (a.trackEnergy && nc > 0) finalizeEnergy(S, S.epartC, (int)nbk, 5, 3);
-        S.forcesFromContacts = true;  // applied by the next Leapfrog (or materialised on download)
-        S.hSizeCL = S.hSize;
+        if (anyEnergy) {
+            const int32_t* descs = (const int32_t*)((const uint8_t*)Cd + offsetof(ConDesc<T>, desc));
+            hipLaunchKernelGGL(k_energy_finalize, dim3(5 * n), dim3(WDM_BLOCK), 0, c->stream, B.D, descs, (int)(sizeof(ConDesc<T>) / 4), m,
+                               (const double*)X.epartC.p, 5, spec ? 1 : 0);
+            checkLaunch();
+        }
     }
-    static void run(Scene& S) {
-        if (S.needFull) throw WdmError(WDM_E_STATE, "ContactLoop before the collider's first run");
-        if (S.mixed()) runT<float>(S, S.cb32); else runT<double>(S, S.cb64);
+    static void runList(Batch& B, const std::vector<int>& L, bool spec) {
+        std::vector<int> g[2][2];  // [mixed][law]
+        for (int i : L) {
+            Scene& S = *B.ss[i];
+            if (S.needFull) throw WdmError(WDM_E_STATE, "ContactLoop before the collider's first run");
+            g[S.mixed() ? 1 : 0][S.P.law == WDM_LAW_HERTZ_DMT ? 1 : 0].push_back(i);
+        }
+        if (L.empty()) return;
+        B.ensureDesc();
+        runGroup<double>(B, g[0][0], WDM_LAW_IDEALELPL, spec);
+        runGroup<double>(B, g[0][1], WDM_LAW_HERTZ_DMT, spec);
+        runGroup<float>(B, g[1][0], WDM_LAW_IDEALELPL, spec);
+        runGroup<float>(B, g[1][1], WDM_LAW_HERTZ_DMT, spec);
+    }
+    static void done(Batch& B) {
+        for (Scene* S : B.ss) {
+            S->forcesFromContacts = true;  // applied by the next Leapfrog (or materialised on download)
+            S->hSizeCL = S->hSize;
+        }
+    }
+    static void run(Batch& B) {
+        std::vector<int> all(B.n());
+        for (int i = 0; i < B.n(); i++) all[i] = i;
+        runList(B, all, false);
+        done(B);
     }
 };
 
-void stepBegin(Scene& S) {
+void checkReady(Scene& S) {
     if (!(S.paramsSet && S.nodesSet && S.partsSet && S.matsSet)) throw WdmError(WDM_E_STATE, "scene not fully uploaded");
+}
+// Scene::doOneStep prologue (core/Scene.cpp:371-389): cell update
+void stepBeginCell(Scene& S) {
+    checkReady(S);
     if (S.periodic) cellIntegrateAndUpdate(S, S.P.dt);
-    if (S.P.trackEnergy) {  // EnergyTracker::resetResettables: kinetic and elast
-        HIPCHK(hipMemsetAsync(S.E.p + WDM_E_KINETIC, 0, sizeof(double) * 2, S.ctx->stream));
-    }
 }
 void stepEnd(Scene& S) {
     if (S.periodic) cellSetNextGradV(S);
     S.step++;
 }
-void doOneStep(Scene& S) {
-    stepBegin(S);
-    Leapfrog::run(S);
-    InsertionSortCollider::run(S);
-    ContactLoop::run(S);
-    stepEnd(S);
-}
-void syncAndCheck(Scene& S) {
-    Ctx* c = S.ctx;
-    HIPCHK(hipMemcpyAsync(S.hostWords + 1, S.err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(S.hostWords + 0, S.orL.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (S.hostWords[1]) InsertionSortCollider::throwDeviceError(S.hostWords[1]);
+// One timestep of every scene of the batch, engines in Woo's order, each launched once for
+// all scenes.  The ContactLoop of the scenes that can skip the collider is queued right
+// behind Leapfrog, before the host has read the Verlet triggers; scenes whose trigger
+// fired skip it on the device and run the collider and the ContactLoop afterwards.  The
+// host waits only for Leapfrog, so the GPU stays busy while it decides.
+void doOneStep(Batch& B) {
+    Ctx* c = B.c;
+    Shared& X = *c->sh;
+    for (Scene* S : B.ss) stepBeginCell(*S);
+    Leapfrog::run(B);
+    std::vector<int> spec;
+    for (int i = 0; i < B.n(); i++)
+        if (B.ss[i]->lastLeapfrogVerletChecked) spec.push_back(i);
+    if (!spec.empty()) {
+        HIPCHK(hipMemcpyAsync(X.hostWords, X.words.p, 4 * (size_t)X.nslots * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipEventRecord(X.evWords, c->stream));
+        ContactLoop::runList(B, spec, true);
+        HIPCHK(hipEventSynchronize(X.evWords));
+        for (Scene* S : B.ss)
+            if (uint32_t e = X.hostWords[4 * (size_t)S->slot + 2]) throwDeviceError(e);
+    }
+    std::vector<int> full = InsertionSortCollider::fullList(B, !spec.empty());
+    InsertionSortCollider::runList(B, full);
+    ContactLoop::runList(B, full, false);
+    ContactLoop::done(B);
+    for (Scene* S : B.ss) stepEnd(*S);
 }
 
 // DemData.force/torque after ContactLoop: gravity reset plus the contact sums
 void materialiseForces(Scene& S) {
-    StepArgs a = makeArgs(S);
-    KTimer t(S.ctx, "gather");
-    hipLaunchKernelGGL(k_gather, dim3(nblk(S.nn)), dim3(WDM_BLOCK), 0, S.ctx->stream, a, S.nview(), S.pview(), S.gview());
+    Ctx* c = S.ctx;
+    Batch B(c, {&S});
+    B.upload();
+    std::vector<int64_t> bp = blockPrefix(1, [&](int) { return S.nn; });
+    Map m = uploadMap(c, c->sh->stMapNode, bp);
+    if (bp[1] == 0) return;
+    KTimer t(c, "gather");
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)bp[1]), dim3(WDM_BLOCK), 0, c->stream, B.D, (const int32_t*)nullptr, m);
     checkLaunch();
 }
 
@@ -743,22 +1107,53 @@ int64_t countLive(Scene& S, ConBuf<T>* cb) {
 // scene generation: Jacobi overlap relaxation sweeps
 void relax(Scene& S, int iters) {
     Ctx* c = S.ctx;
-    int nb = (int)S.binnedH.size();
-    ensure(c, S.disp, std::max(S.nn, 1));
+    Shared& X = *c->sh;
+    if (S.binnedH.empty()) return;
+    ensure(c, X.disp, std::max(S.nn, 1));
+    Batch B(c, {&S});
+    B.upload();
+    std::vector<int> L = {0};
+    std::vector<int64_t> bpP = blockPrefix(1, [&](int) { return S.np; });
+    std::vector<int64_t> bpB = blockPrefix(1, [&](int) { return (int64_t)S.binnedH.size(); });
+    Map mP = uploadMap(c, X.stMapP, bpP), mB = uploadMap(c, X.stMapB, bpB);
     for (int it = 0; it < iters; it++) {
-        hipLaunchKernelGGL(k_bounds, dim3(nblk(S.np)), dim3(WDM_BLOCK), 0, c->stream, S.nview(), S.pview(), 1.0, 0.0);
+        std::vector<FRScene> F(1);
+        std::memset(F.data(), 0, sizeof(FRScene));
+        F[0].desc = 0;
+        const FRScene* FRd = (const FRScene*)X.stFR.upload(c, F.data(), sizeof(FRScene));
+        hipLaunchKernelGGL(k_bounds, dim3((unsigned)bpP[1]), dim3(WDM_BLOCK), 0, c->stream, B.D, FRd, mP, 1);
         checkLaunch();
-        GridArgs g;
-        int nbn;
-        InsertionSortCollider::buildGrid(S, g, nbn);
-        if (nb == 0) return;
-        hipLaunchKernelGGL(k_relax, dim3(nblk(nb)), dim3(WDM_BLOCK), 0, c->stream, S.nview(), S.pview(), g, (const uint32_t*)S.ckeys2.p,
-                           (const int32_t*)S.cvals2.p, nb, (const int32_t*)S.cstart.p, (const int32_t*)S.cend.p, S.disp.p);
-        hipLaunchKernelGGL(k_relax_apply, dim3(nblk(nb)), dim3(WDM_BLOCK), 0, c->stream, S.nview(), (const int32_t*)S.binned.p, nb, (const double4*)S.disp.p);
+        int64_t nbTot = 0;
+        InsertionSortCollider::buildGrids(B, L, F, bpB, mB, FRd, nbTot);
+        hipLaunchKernelGGL(k_relax, dim3((unsigned)bpB[1]), dim3(WDM_BLOCK), 0, c->stream, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
+                           (const int32_t*)X.cvals2.p, (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p, X.disp.p);
+        hipLaunchKernelGGL(k_relax_apply, dim3((unsigned)bpB[1]), dim3(WDM_BLOCK), 0, c->stream, B.D, FRd, mB, (const double4*)X.disp.p);
         checkLaunch();
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     S.needFull = true;
+}
+
+// scene words slot (Verlet trigger, created energy slots, device errors)
+int32_t takeSlot(Ctx* c) {
+    Shared& X = *c->sh;
+    int32_t s;
+    if (!X.freeSlots.empty()) { s = X.freeSlots.back(); X.freeSlots.pop_back(); }
+    else {
+        s = X.nslots;
+        size_t need = 4 * ((size_t)s + 1);
+        ensure(c, X.words, need, true);
+        if (X.hostWordsCap < X.words.n) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (X.hostWords) (void)hipHostFree(X.hostWords);
+            X.hostWords = nullptr;
+            HIPCHK(hipHostMalloc((void**)&X.hostWords, X.words.n * sizeof(uint32_t), hipHostMallocDefault));
+            X.hostWordsCap = X.words.n;
+        }
+        X.nslots = s + 1;
+    }
+    HIPCHK(hipMemsetAsync(X.words.p + 4 * (size_t)s, 0, 4 * sizeof(uint32_t), c->stream));
+    return s;
 }
 
 }  // namespace
@@ -776,8 +1171,11 @@ int wdm_create(wdm_ctx** out, int device, size_t budget) {
     wdm_ctx* c = new wdm_ctx();
     c->device = device;
     c->budget = budget;
+    c->sh = new Shared();
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->sh->evWords, hipEventDisableTiming) != hipSuccess) {
+        delete c->sh;
         delete c;
         return WDM_E_HIP;
     }
@@ -791,6 +1189,8 @@ void wdm_destroy(wdm_ctx* c) {
     (void)hipStreamSynchronize(c->stream);
     for (auto& kv : c->scenes) { kv.second->freeAll(); delete kv.second; }
     c->scenes.clear();
+    c->sh->freeAll(c);
+    delete c->sh;
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -804,14 +1204,15 @@ size_t wdm_scene_footprint(int32_t n_par, int32_t n_node, double cpp) {
     size_t parB = (size_t)n_par * (4 + 4 + 4 + 16 + 64 + 8 + 16);
     double ncon = cpp * n_par * 1.34;
     size_t conB = (size_t)(ncon * 2 * (8 + 8 + 4 + 4 + NSTATE * 8 + 64));
-    size_t csrB = (size_t)(ncon * 2 * 2 * (8 + 4));
-    size_t scratch = (size_t)n_par * 64 + (size_t)(ncon * 24);
+    size_t csrB = (size_t)(ncon * 2 * (4 + 1));
+    size_t scratch = (size_t)n_par * 64 + (size_t)(ncon * 48);
     return nodeB + parB + conB + csrB + scratch + (1u << 20);
 }
 
 int wdm_scene_alloc(wdm_ctx* c, int32_t np, int32_t nn, int32_t nm, int32_t* sid) {
     return guard(c, [&] {
         if (np < 0 || nn < np || nm < 1 || !sid) throw WdmError(WDM_E_ARG, "bad sizes (need n_node >= n_par, n_mat >= 1)");
+        if (np >= (1 << 30)) throw WdmError(WDM_E_ARG, "too many particles");
         Scene* S = new Scene(c, np, nn, nm);
         try {
             ensure(c, S->posr, std::max(nn, 1)); ensure(c, S->velm, std::max(nn, 1)); ensure(c, S->angI, std::max(nn, 1));
@@ -820,19 +1221,17 @@ int wdm_scene_alloc(wdm_ctx* c, int32_t np, int32_t nn, int32_t nm, int32_t* sid
             ensure(c, S->ptype, std::max(np, 1)); ensure(c, S->pmat, std::max(np, 1)); ensure(c, S->pmask, std::max(np, 1));
             ensure(c, S->fnodes, std::max(np, 1)); ensure(c, S->bmin, std::max(np, 1)); ensure(c, S->bmax, std::max(np, 1));
             ensure(c, S->mat, nm);
-            ensure(c, S->E, WDM_E_COUNT); ensure(c, S->eslots, 8); ensure(c, S->orL, 1); ensure(c, S->ecre, 1); ensure(c, S->err, 1);
+            ensure(c, S->E, WDM_E_COUNT);
             ensure(c, S->csrOff, (size_t)np + 1);
+            ensure(c, S->entV, 1);
+            ensure(c, S->realMask, 1);
             HIPCHK(hipMemsetAsync(S->E.p, 0, sizeof(double) * WDM_E_COUNT, c->stream));
-            HIPCHK(hipMemsetAsync(S->orL.p, 0, 4, c->stream));
-            HIPCHK(hipMemsetAsync(S->ecre.p, 0, 4, c->stream));
-            HIPCHK(hipMemsetAsync(S->err.p, 0, 4, c->stream));
             HIPCHK(hipMemsetAsync(S->csrOff.p, 0, sizeof(int64_t) * ((size_t)np + 1), c->stream));
-            int32_t slots[8] = {WDM_E_NONVISCDAMP, WDM_E_GRAV, WDM_E_KINETIC, WDM_E_ELAST, WDM_E_PLAST, WDM_E_VISCN, WDM_E_VISCT, WDM_E_BROKEN};
-            HIPCHK(hipMemcpyAsync(S->eslots.p, slots, sizeof(slots), hipMemcpyHostToDevice, c->stream));
-            HIPCHK(hipHostMalloc((void**)&S->hostWords, 16, hipHostMallocDefault));
-            std::memset(S->hostWords, 0, 16);
+            HIPCHK(hipMemsetAsync(S->realMask.p, 0, sizeof(uint64_t), c->stream));
+            S->slot = takeSlot(c);
             HIPCHK(hipStreamSynchronize(c->stream));
         } catch (...) {
+            if (S->slot >= 0) c->sh->freeSlots.push_back(S->slot);
             S->freeAll();
             delete S;
             throw;
@@ -847,6 +1246,7 @@ int wdm_scene_free(wdm_ctx* c, int32_t sid) {
         Scene& S = getScene(c, sid);
         HIPCHK(hipStreamSynchronize(c->stream));
         S.freeAll();
+        c->sh->freeSlots.push_back(S.slot);
         c->scenes.erase(sid);
         delete &S;
     });
@@ -988,21 +1388,43 @@ int wdm_set_params(wdm_ctx* c, int32_t sid, const wdm_params* p) {
         if (p->law == WDM_LAW_HERTZ_DMT && (p->gamma != 0 || p->alpha != 0)) throw WdmError(WDM_E_UNSUPPORTED, "adhesive Hertz (gamma/alpha != 0) is not supported");
         if (p->precision != WDM_PREC_FP64 && p->precision != WDM_PREC_MIXED) throw WdmError(WDM_E_ARG, "unknown precision");
         if (S.paramsSet && p->precision != S.P.precision && S.nc > 0) throw WdmError(WDM_E_STATE, "precision cannot change after contacts exist");
-        if (S.paramsSet && p->verletDist != S.P.verletDist) S.verletResolved = false;
+        S.verletResolved = false;
         S.P = *p;
         S.paramsSet = true;
     });
 }
 
-int wdm_step_begin(wdm_ctx* c, int32_t sid) { return guard(c, [&] { stepBegin(getScene(c, sid)); }); }
-int wdm_leapfrog_run(wdm_ctx* c, int32_t sid) { return guard(c, [&] { Leapfrog::run(getScene(c, sid)); }); }
-int wdm_collider_run(wdm_ctx* c, int32_t sid) { return guard(c, [&] { InsertionSortCollider::run(getScene(c, sid)); }); }
-int wdm_contact_loop_run(wdm_ctx* c, int32_t sid) { return guard(c, [&] { ContactLoop::run(getScene(c, sid)); }); }
+int wdm_step_begin(wdm_ctx* c, int32_t sid) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        stepBeginCell(S);
+        if (S.P.trackEnergy)  // EnergyTracker::resetResettables: kinetic and elast
+            HIPCHK(hipMemsetAsync(S.E.p + WDM_E_KINETIC, 0, sizeof(double) * 2, c->stream));
+    });
+}
+int wdm_leapfrog_run(wdm_ctx* c, int32_t sid) {
+    return guard(c, [&] {
+        Batch B(c, {&getScene(c, sid)});
+        Leapfrog::run(B);
+    });
+}
+int wdm_collider_run(wdm_ctx* c, int32_t sid) {
+    return guard(c, [&] {
+        Batch B(c, {&getScene(c, sid)});
+        InsertionSortCollider::run(B);
+    });
+}
+int wdm_contact_loop_run(wdm_ctx* c, int32_t sid) {
+    return guard(c, [&] {
+        Batch B(c, {&getScene(c, sid)});
+        ContactLoop::run(B);
+    });
+}
 int wdm_step_end(wdm_ctx* c, int32_t sid) {
     return guard(c, [&] {
         Scene& S = getScene(c, sid);
         stepEnd(S);
-        syncAndCheck(S);
+        readWords(c, {&S});
     });
 }
 
@@ -1010,11 +1432,19 @@ int wdm_step(wdm_ctx* c, const int32_t* scenes, int32_t ns, int32_t nsteps) {
     return guard(c, [&] {
         if (ns < 0 || nsteps < 0 || (ns > 0 && !scenes)) throw WdmError(WDM_E_ARG, "bad scene list");
         std::vector<Scene*> ss;
-        for (int i = 0; i < ns; i++) ss.push_back(&getScene(c, scenes[i]));
+        for (int i = 0; i < ns; i++) {
+            Scene* S = &getScene(c, scenes[i]);
+            if (std::find(ss.begin(), ss.end(), S) != ss.end()) throw WdmError(WDM_E_ARG, "scene listed twice");
+            checkReady(*S);
+            ss.push_back(S);
+        }
         if (c->timing.on) c->timing.acc.clear();
-        for (int k = 0; k < nsteps; k++)
-            for (Scene* S : ss) doOneStep(*S);
-        for (Scene* S : ss) syncAndCheck(*S);
+        if (ss.empty()) return;
+        for (int k = 0; k < nsteps; k++) {
+            Batch B(c, ss);
+            doOneStep(B);
+        }
+        readWords(c, ss);
     });
 }
 
@@ -1065,7 +1495,7 @@ int wdm_get_energy(wdm_ctx* c, int32_t sid, double* values, int32_t* created) {
         double e[WDM_E_COUNT];
         uint32_t cre = 0;
         HIPCHK(hipMemcpyAsync(e, S.E.p, sizeof(e), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(&cre, S.ecre.p, sizeof(cre), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(&cre, c->sh->words.p + 4 * (size_t)S.slot + 1, sizeof(cre), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         for (int k = 0; k < WDM_E_COUNT; k++) {
             if (values) values[k] = e[k];
