@@ -8,6 +8,12 @@ gravity, isotropic cell strain rate 1e-3/s, FP64.  Overlaps of the uniform
 placement are relaxed with 500 device sweeps before timing.  A step is one
 Scene::doOneStep: Leapfrog, InsertionSortCollider (Verlet-triggered), ContactLoop.
 
+--workload batch: 512 independent deposition boxes (woodem_amd.scenes.deposition_box,
+seeds 1000..1511: N_s = round(exp(U(ln 500, ln 2e4))), r ~ U[2, 6] mm, tanPhi ~
+U[.1, .8], damping ~ U[.05, .4]), FP64 state with FP32 contact arithmetic, stepped
+together by one wdm_step call per step batch; the same JSON line carries the
+sequential rate (each scene stepped alone, on a sample of the scenes) and the ratio.
+
 One JSON line on stdout.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
 import argparse
@@ -58,6 +64,95 @@ def cpu_baseline(ctx, n_sample, seconds):
                     n_sample, k, el))
 
 
+def bench_batch(args, ctx):
+    """Many small scenes stepped as one batch vs the same scenes stepped one by one."""
+    from woodem_amd import runner, scenes
+    from woodem_amd._native import PREC_MIXED
+    t0 = time.time()
+    arrs = [scenes.deposition_box(seed=1000 + s) for s in range(args.scenes)]
+    runs = [runner.HipRun(a, runner.params_from_meta(a, precision=PREC_MIXED), runner.cell_from_meta(a), ctx=ctx) for a in arrs]
+    sids = [r.sid for r in runs]
+    n_tot = sum(a.n_par for a in arrs)
+    log("batch: %d scenes, %d particles in total (%d..%d per scene), set up in %.1f s" % (
+        len(arrs), n_tot, min(a.n_par for a in arrs), max(a.n_par for a in arrs), time.time() - t0))
+    ctx.step(sids, args.warmup)
+    ctx.sync()
+    fr0 = sum(ctx.stats(s)["nFullRuns"] for s in sids)
+    t1 = time.perf_counter()
+    ctx.step(sids, args.steps)
+    ctx.sync()
+    el = time.perf_counter() - t1
+    fr1 = sum(ctx.stats(s)["nFullRuns"] for s in sids)
+    value = n_tot * args.steps / el
+    log("batched: %d steps in %.3f s: %.4g particle-steps/s, %d scene full runs" % (args.steps, el, value, fr1 - fr0))
+    # per-step latency of the whole batch
+    lat = []
+    for _ in range(args.latency_steps):
+        tt = time.perf_counter()
+        ctx.step(sids, 1)
+        lat.append(time.perf_counter() - tt)
+    lat = np.array(lat) * 1e3
+    # kernel time per step (all launches of a kind in one step summed)
+    ctx.enable_timing(True)
+    ctx.step(sids, args.timing_steps)
+    kt = ctx.kernel_times()
+    ctx.enable_timing(False)
+    n_con = n_real = 0
+    for s in sids:
+        c = ctx.contacts(s)
+        n_con += len(c["idA"])
+        n_real += int(c["isReal"].sum())
+    ms_step = kt.get("contacts", (float("nan"), 0))[0] / args.timing_steps
+    alg_bytes = BYTES_REAL["mixed"] * n_real + BYTES_POT["mixed"] * (n_con - n_real)
+    achieved = alg_bytes / (ms_step * 1e-3) / 1e9
+    kernels = {k: {"ms_per_step": v[0] / args.timing_steps, "launches": v[1]} for k, v in kt.items()}
+    log("kernels: %s" % json.dumps(kernels))
+    # sequential: a sample of the scenes, each stepped alone (same code path, batch of one)
+    pick = list(range(0, len(runs), max(1, len(runs) // args.seq_scenes)))[:args.seq_scenes]
+    seq_ps = 0
+    seq_t = 0.
+    for i in pick:
+        r = runner.HipRun(arrs[i], runner.params_from_meta(arrs[i], precision=PREC_MIXED), runner.cell_from_meta(arrs[i]), ctx=ctx)
+        r.step(args.warmup)
+        ctx.sync()
+        tt = time.perf_counter()
+        r.step(args.seq_steps)
+        ctx.sync()
+        seq_t += time.perf_counter() - tt
+        seq_ps += arrs[i].n_par * args.seq_steps
+        r.close()
+    seq = seq_ps / seq_t
+    log("sequential sample: %d scenes x %d steps: %.4g particle-steps/s; batched/sequential = %.2f" % (
+        len(pick), args.seq_steps, seq, value / seq))
+    for r in runs:
+        r.close()
+    return {
+        "metric": "particle-steps/s on 1 MI355X (batched scenes vs sequential); p50/p99 step latency",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64-state/f32-contact",
+        "data": "synthetic (seeded generator, seeds 1000..%d)" % (999 + len(arrs)),
+        "config": {"workload": "%d deposition boxes, %d particles in total, FP32 contact arithmetic, one batch" % (len(arrs), n_tot),
+                   "scenes": len(arrs), "n_particles": n_tot, "precision": "mixed",
+                   "contacts_potential": n_con, "contacts_real": n_real, "scene_full_runs_in_timed_region": fr1 - fr0},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)), "samples": len(lat)},
+        "roofline": {"bound": "hbm", "kernel": "k_contacts", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_step": alg_bytes, "ms_per_step": ms_step},
+        "kernels": kernels,
+        "sequential": {"value": seq, "unit": "particle-steps/s", "scenes": len(pick), "steps": args.seq_steps,
+                       "batched_over_sequential": value / seq},
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +167,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
+    ap.add_argument("--workload", choices=["periodic", "batch"], default="periodic")
+    ap.add_argument("--scenes", type=int, default=512, help="batch workload: number of scenes")
+    ap.add_argument("--seq-scenes", type=int, default=32, help="batch workload: scenes in the sequential sample")
+    ap.add_argument("--seq-steps", type=int, default=300, help="batch workload: steps per scene in the sequential sample")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB (5 times), for scaling FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
@@ -93,6 +192,9 @@ def main():
     from woodem_amd._native import PREC_FP64, PREC_MIXED, Context
 
     ctx = Context(0)
+    if args.workload == "batch":
+        print(json.dumps(bench_batch(args, ctx)), flush=True)
+        return
     t0 = time.time()
     a = scenes.periodic_hertz(n=args.n)
     b = scenes.relax_and_order(a, ctx, iters=args.relax)
