@@ -127,6 +127,17 @@ int wdm_upload_particles(wdm_ctx* ctx, int32_t scene, int32_t n, const int32_t* 
 /* FrictMat table (pkg/dem/FrictMat.hpp:18-28) */
 int wdm_upload_materials(wdm_ctx* ctx, int32_t scene, int32_t n, const double* density, const double* young,
                          const double* ktDivKn, const double* tanPhi);
+/* Clumps: ClumpData (pkg/dem/Clump.hpp:10-40) as built by ClumpData::makeClump
+   (pkg/dem/Clump.cpp:185-253).  Clump c has master node master[c] (flag
+   WDM_CLUMP_CLUMP) and members members[off[c]..off[c+1]) (flag WDM_CLUMP_CLUMPED),
+   each with relPos (3 doubles) and relOri (w, x, y, z) in the master's frame.
+   Leapfrog integrates masters (forces and torques summed from the members,
+   ClumpData::forceTorqueFromMembers) and places the members (applyToMembers,
+   pkg/dem/Clump.cpp:265-297); the collider does not pair particles of one clump.
+   Masters with free rotational DOFs need spherical inertia.  Call after
+   wdm_upload_nodes; nc = 0 removes the clumps. */
+int wdm_upload_clumps(wdm_ctx* ctx, int32_t scene, int32_t nc, const int32_t* master, const int32_t* off,
+                      const int32_t* members, const double* relPos3, const double* relOri4);
 /* Cell (core/Cell.hpp:116-160): hSize, gradV, nextGradV row-major 3x3 */
 int wdm_set_cell(wdm_ctx* ctx, int32_t scene, int32_t periodic, const double* hSize9, const double* gradV9,
                  const double* nextGradV9, int32_t homoDeform);

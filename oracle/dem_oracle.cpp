@@ -16,9 +16,11 @@
  *   - forces are applied deterministically (ContactLoop.deterministic,
  *     pkg/dem/ContactLoop.cpp:250-257), contacts visited in (min id, max id) order,
  *     so each particle sums its contacts in ascending order of the other id.
- *   - energies are summed sequentially in node order / contact order.
+ *   - energies are summed sequentially in node order / contact order;
+ *   - quaternion products and quaternion*vector follow Eigen 3's scalar formulas
+ *     (the reference's build may vectorise them, equal to a few ulp).
  * The "parity unpinned" items are listed in the module docstring of
- * tests/oracle_lib.py and in DESIGN.md.
+ * tests/oracle_lib.py.
  */
 #include "dem_oracle.h"
 
@@ -114,6 +116,14 @@ inline Qt qmul(const Qt& a, const Qt& b) {
     r.z = ((a.w * b.z + a.z * b.w) + a.x * b.y) - a.y * b.x;
     return r;
 }
+/* Quaternion * vector as Eigen 3 evaluates it (Quaternion::_transformVector):
+   t = 2 (q.vec x v); v' = (v + w t) + q.vec x t */
+inline V3 qrotate(const Qt& q, const V3& v) {
+    V3 u = v3(q.x, q.y, q.z);
+    V3 t = vcross(u, v);
+    t = vadd(t, t);
+    return vadd(vadd(v, vmul(t, q.w)), vcross(u, t));
+}
 inline Qt qnormalized(const Qt& q) {
     double n = std::sqrt(((q.w * q.w + q.x * q.x) + q.y * q.y) + q.z * q.z);
     Qt r; r.w = q.w / n; r.x = q.x / n; r.y = q.y / n; r.z = q.z / n; return r;
@@ -175,6 +185,11 @@ struct Oracle {
     std::vector<Qt> ori;
     std::vector<double> mass;
     std::vector<uint32_t> flags;
+    // clumps (ClumpData, pkg/dem/Clump.hpp): master node, members with relPos/relOri
+    std::vector<int> clumpMaster, clumpOff, clumpMem;  // per clump; offsets; member nodes
+    std::vector<V3> clumpRelPos;
+    std::vector<Qt> clumpRelOri;
+    std::vector<int> nodeMaster, nodeClump;  // per node: master node of a member (-1), clump index of a master (-1)
     // particles
     int np = 0;
     std::vector<int> shape, nodeIx, axis, sense, matId;  // nodeIx: 3 per particle
@@ -284,9 +299,16 @@ struct Oracle {
         #pragma omp parallel for schedule(guided)
         for (int i = 0; i < nn; i++) {
             uint32_t fl = flags[i];
-            if (fl & CLUMP_CLUMPED) continue;
-            if (fl & CLUMP_CLUMP) continue;  // clump masters are outside the restated path (rejected at upload)
+            if (fl & CLUMP_CLUMPED) continue;  // integrated through the clump's master node
+            const int ci = (fl & CLUMP_CLUMP) ? nodeClump[i] : -1;
             bool damp = (damping != 0. && !(fl & DAMPING_SKIP));
+            if (ci >= 0) {  // ClumpData::forceTorqueFromMembers, Clump.cpp:265-273
+                for (int k = clumpOff[ci]; k < clumpOff[ci + 1]; k++) {
+                    int mnode = clumpMem[k];
+                    force[i] = vadd(force[i], force[mnode]);
+                    torque[i] = vadd(torque[i], vadd(torque[mnode], vcross(vsub(pos[mnode], pos[i]), force[mnode])));
+                }
+            }
             V3 f = force[i], t = torque[i];
             bool blockedNone = (fl & DOF_ALL) == 0, blockedAll = (fl & DOF_ALL) == DOF_ALL;
             if (reallyTrackEnergy) {
@@ -354,6 +376,17 @@ struct Oracle {
             // reset, :240-245
             force[i] = (hasGravity && !(fl & GRAVITY_SKIP)) ? vmul(g, mass[i]) : vzero();
             torque[i] = vzero();
+            if (ci >= 0) {  // ClumpData::applyToMembers, Clump.cpp:276-289 (reset: members' force and torque zeroed)
+                for (int k = clumpOff[ci]; k < clumpOff[ci + 1]; k++) {
+                    int mnode = clumpMem[k];
+                    pos[mnode] = vadd(pos[i], qrotate(ori[i], clumpRelPos[k]));
+                    ori[mnode] = qmul(ori[i], clumpRelOri[k]);
+                    vel[mnode] = vadd(vel[i], vcross(angVel[i], vsub(pos[mnode], pos[i])));
+                    angVel[mnode] = angVel[i];
+                    force[mnode] = vzero();
+                    torque[mnode] = vzero();
+                }
+            }
         }
         if (reallyTrackEnergy) {
             for (int i = 0; i < nn; i++) {
@@ -427,12 +460,17 @@ struct Oracle {
         return true;
     }
 
-    /* Collider::mayCollide, pkg/dem/Collision.cpp:19-41 (clumps are outside the restated path) */
+    /* Collider::mayCollide, pkg/dem/Collision.cpp:19-41 */
     bool mayCollide(int a, int b) const {
         if (a == b || shape[a] == ORC_SHAPE_NONE || shape[b] == ORC_SHAPE_NONE) return false;
         if (!(mask[a] & mask[b])) return false;
         if ((mask[a] & mask[b] & P.loneMask) != 0) return false;
-        for (int i = 0; i < numNodes(a); i++) for (int j = 0; j < numNodes(b); j++) if (nodeOf(a, i) == nodeOf(b, j)) return false;
+        for (int i = 0; i < numNodes(a); i++)
+            for (int j = 0; j < numNodes(b); j++) {
+                int na = nodeOf(a, i), nb = nodeOf(b, j);
+                if (na == nb) return false;  // shared node
+                if ((flags[na] & CLUMP_CLUMPED) && (flags[nb] & CLUMP_CLUMPED) && nodeMaster[na] == nodeMaster[nb]) return false;  // same clump
+            }
         return true;
     }
 
@@ -1054,9 +1092,42 @@ int orc_set_nodes(void* h, int32_t n, const double* pos3, const double* ori4, co
         }
         o->ori[i].w = ori4[4 * i + 0]; o->ori[i].x = ori4[4 * i + 1]; o->ori[i].y = ori4[4 * i + 2]; o->ori[i].z = ori4[4 * i + 3];
         o->mass[i] = mass[i]; o->flags[i] = flags[i];
-        if (flags[i] & (CLUMP_CLUMP | CLUMP_CLUMPED)) { o->err = "clumps are outside the restated path"; return -1; }
         bool asph = !(o->inertia[i][0] == o->inertia[i][1] && o->inertia[i][1] == o->inertia[i][2]);
         if (asph && (flags[i] & DOF_RXRYRZ) != DOF_RXRYRZ) { o->err = "aspherical integrator is outside the restated path"; return -1; }
+    }
+    o->nodeMaster.assign(n, -1);
+    o->nodeClump.assign(n, -1);
+    o->clumpMaster.clear(); o->clumpOff.assign(1, 0); o->clumpMem.clear(); o->clumpRelPos.clear(); o->clumpRelOri.clear();
+    return 0;
+}
+/* ClumpData (pkg/dem/Clump.hpp): for clump c, master node master[c], members
+   members[off[c]..off[c+1]) with relPos (3 per member) and relOri (w, x, y, z per member);
+   node flags must mark masters CLUMP_CLUMP and members CLUMP_CLUMPED */
+int orc_set_clumps(void* h, int32_t nc, const int32_t* master, const int32_t* off, const int32_t* members, const double* relPos3,
+                   const double* relOri4) {
+    Oracle* o = O(h);
+    o->nodeMaster.assign(o->nn, -1);
+    o->nodeClump.assign(o->nn, -1);
+    o->clumpMaster.assign(master, master + nc);
+    o->clumpOff.assign(off, off + nc + 1);
+    int nm = nc > 0 ? off[nc] : 0;
+    o->clumpMem.assign(members, members + nm);
+    o->clumpRelPos.resize(nm); o->clumpRelOri.resize(nm);
+    for (int c = 0; c < nc; c++) {
+        int M = master[c];
+        if (M < 0 || M >= o->nn || !(o->flags[M] & CLUMP_CLUMP) || o->nodeClump[M] >= 0) { o->err = "bad clump master"; return -1; }
+        o->nodeClump[M] = c;
+        for (int k = off[c]; k < off[c + 1]; k++) {
+            int m = members[k];
+            if (m < 0 || m >= o->nn || !(o->flags[m] & CLUMP_CLUMPED) || o->nodeMaster[m] >= 0) { o->err = "bad clump member"; return -1; }
+            o->nodeMaster[m] = M;
+            o->clumpRelPos[k] = v3(relPos3[3 * k], relPos3[3 * k + 1], relPos3[3 * k + 2]);
+            o->clumpRelOri[k] = Qt{relOri4[4 * k], relOri4[4 * k + 1], relOri4[4 * k + 2], relOri4[4 * k + 3]};
+        }
+    }
+    for (int i = 0; i < o->nn; i++) {
+        if ((o->flags[i] & CLUMP_CLUMP) && o->nodeClump[i] < 0) { o->err = "clump master without members"; return -1; }
+        if ((o->flags[i] & CLUMP_CLUMPED) && o->nodeMaster[i] < 0) { o->err = "clumped node without master"; return -1; }
     }
     return 0;
 }
@@ -1102,6 +1173,12 @@ int orc_set_params(void* h, const orc_params* p) {
 }
 int orc_step(void* h, int32_t n) {
     Oracle* o = O(h);
+    for (int i = 0; i < o->nn; i++) {
+        if (((o->flags[i] & CLUMP_CLUMP) && o->nodeClump[i] < 0) || ((o->flags[i] & CLUMP_CLUMPED) && o->nodeMaster[i] < 0)) {
+            o->err = "clump flags set but clumps not given (orc_set_clumps)";
+            return -1;
+        }
+    }
     try { for (int i = 0; i < n; i++) o->doOneStep(); }
     catch (std::exception& e) { o->err = e.what(); return -1; }
     return 0;

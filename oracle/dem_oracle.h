@@ -64,6 +64,10 @@ int orc_set_materials(void* h, int32_t n, const double* density, const double* y
 int orc_set_cell(void* h, int32_t periodic, const double* hSize9, const double* gradV9,
                  const double* nextGradV9, int32_t homoDeform);
 int orc_set_params(void* h, const orc_params* p);
+/* clumps (ClumpData, pkg/dem/Clump.hpp): master node, member node lists (offsets
+   nc+1), members' relPos (3 each) and relOri (w, x, y, z); call after orc_set_nodes */
+int orc_set_clumps(void* h, int32_t nc, const int32_t* master, const int32_t* off, const int32_t* members,
+                   const double* relPos3, const double* relOri4);
 
 /* run n full timesteps (Scene::doOneStep) */
 int orc_step(void* h, int32_t n);

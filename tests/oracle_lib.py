@@ -59,6 +59,7 @@ def lib():
         "orc_set_materials": (C.c_int, [P, I32, pd, pd, pd, pd]),
         "orc_set_cell": (C.c_int, [P, I32, pd, pd, pd, I32]),
         "orc_set_params": (C.c_int, [P, C.POINTER(OrcParams)]),
+        "orc_set_clumps": (C.c_int, [P, I32, pi, pi, pi, pd, pd]),
         "orc_step": (C.c_int, [P, I32]),
         "orc_get_nodes": (C.c_int, [P, pd, pd, pd, pd, pd, pd]),
         "orc_num_contacts": (I64, [P]),
@@ -110,6 +111,10 @@ class OracleRun:
         self._chk(L.orc_set_particles(self.h, a.n_par, _p(a.shape, I32), _p(a.nodes, I32), _p(a.radius, D), _p(a.axis, I32),
                                       _p(a.sense, I32), _p(a.mask, U32), _p(a.mat, I32)))
         self._chk(L.orc_set_materials(self.h, a.n_mat, _p(a.density, D), _p(a.young, D), _p(a.ktDivKn, D), _p(a.tanPhi, D)))
+        c = a.clumps
+        if c:
+            self._chk(L.orc_set_clumps(self.h, len(c["master"]), _p(c["master"], I32), _p(c["off"], I32), _p(c["members"], I32),
+                                       _p(c["relPos"], D), _p(c["relOri"], D)))
         h9 = np.ascontiguousarray(cell["hSize"], np.float64).reshape(9)
         g9 = np.ascontiguousarray(cell["gradV"], np.float64).reshape(9)
         n9 = np.ascontiguousarray(cell["nextGradV"], np.float64).reshape(9)

@@ -28,6 +28,10 @@ def scene_set(ctx):
     out.append((d, runner.params_from_meta(d, trackEnergy=True), runner.cell_from_meta(d)))
     e = scenes.deposition_box(seed=15, n=700)
     out.append((e, runner.params_from_meta(e, precision=PREC_MIXED), runner.cell_from_meta(e)))
+    f = scenes.drum(seed=16, n=400, omega=2., gap=1e-4)
+    ns = f.meta["n_spheres"]
+    f.vel[:ns, 2] = -.6
+    out.append((f, runner.params_from_meta(f, trackEnergy=True), runner.cell_from_meta(f)))
     return out
 
 

@@ -62,6 +62,7 @@ def lib():
         "wdm_upload_nodes": (C.c_int, [P, I32, I32, pd, pd, pd, pd, pd, pd, pu, pd, pd]),
         "wdm_upload_particles": (C.c_int, [P, I32, I32, pi, pi, pd, pi, pi, pu, pi]),
         "wdm_upload_materials": (C.c_int, [P, I32, I32, pd, pd, pd, pd]),
+        "wdm_upload_clumps": (C.c_int, [P, I32, I32, pi, pi, pi, pd, pd]),
         "wdm_set_cell": (C.c_int, [P, I32, I32, pd, pd, pd, I32]),
         "wdm_set_params": (C.c_int, [P, I32, C.POINTER(WdmParams)]),
         "wdm_step_begin": (C.c_int, [P, I32]),
@@ -143,6 +144,10 @@ class Context:
                                           ptr(a.axis, I32), ptr(a.sense, I32), ptr(a.mask, U32), ptr(a.mat, I32)))
         self.check(L.wdm_upload_materials(h, sid, a.n_mat, ptr(a.density, D), ptr(a.young, D), ptr(a.ktDivKn, D),
                                           ptr(a.tanPhi, D)))
+        c = getattr(a, "clumps", None)
+        if c:
+            self.check(L.wdm_upload_clumps(h, sid, len(c["master"]), ptr(c["master"], I32), ptr(c["off"], I32),
+                                           ptr(c["members"], I32), ptr(c["relPos"], D), ptr(c["relOri"], D)))
 
     def set_cell(self, sid, periodic, hSize, gradV, nextGradV, homoDeform):
         D = C.c_double

@@ -87,7 +87,18 @@ struct NodeView {
     GP double4 *posr, *velm, *angI, *ori, *force, *torque, *lastPos;
     GP uint32_t* flags;
     GP uint8_t* checked;  // node belongs to a particle with a bound (Verlet check)
+    GP int32_t* master;   // clump master node of a clumped node, -1 otherwise (null: no clumps)
     int nn;
+};
+// clumps (ClumpData, pkg/dem/Clump.hpp): members of clump c are mem[off[c]..off[c+1]),
+// with their position and orientation relative to the master node
+struct ClumpView {
+    const GP int32_t* off;
+    const GP int32_t* mem;
+    const GP double4* relPos;   // (x, y, z, 0)
+    const GP double4* relOri;   // (w, x, y, z)
+    const GP int32_t* nodeClump;  // clump index of a master node, -1 otherwise
+    int nClumps, nMembers;
 };
 struct PartView {
     GP int32_t* ptype;
@@ -138,6 +149,7 @@ struct StepArgs {
     int checkVerlet;
     uint32_t stamp;  // nonzero tag of this step: the Verlet trigger word holds it when a node moved too far
     int gatherOn;  // node forces come from the contacts of the previous ContactLoop
+    int hasClumps;
     // contact
     double distFactor;
     int law, noSlip, noBreak, noFrict, noAttraction, trsfRenorm;
@@ -180,6 +192,7 @@ struct SceneDesc {
     int nbinned, nunb;
     GP double* E;       // energy accumulators [E_N]
     uint32_t* words;    // [0] Verlet trigger, [1] created energy slots, [2] device errors
+    ClumpView cl;
 };
 template <typename T>
 struct ConDesc {
@@ -252,7 +265,8 @@ __device__ void blockSumWrite(double (&v)[NV], uint32_t orbits, double* partial,
 // record, the periodic shift of the B side from the cellDist packed in its w slot.
 __device__ __forceinline__ void gatherForces(const StepArgs& a, const NodeView& n, const PartView& p, const GatherView& gv, int i,
                                              uint32_t fl, double m, const D3& pos, D3& F, D3& T) {
-    F = (a.hasGravity && !(fl & F_GRAVITY_SKIP)) ? mk3(a.g[0] * m, a.g[1] * m, a.g[2] * m) : mk3(0., 0., 0.);
+    // gravity acts on a clump's master only (ClumpData::applyToMembers resets the members)
+    F = (a.hasGravity && !(fl & (F_GRAVITY_SKIP | F_CLUMPED))) ? mk3(a.g[0] * m, a.g[1] * m, a.g[2] * m) : mk3(0., 0., 0.);
     T = mk3(0., 0., 0.);
     if (i >= p.np) return;
     int sh = ptShape(p.ptype[i]);
@@ -316,7 +330,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(const SceneDesc* D, cons
         uint32_t fl = n.flags[i];
         double4 pr = n.posr[i];
         D3 pos = xyz(pr);
-        if (!(fl & (F_CLUMPED | F_CLUMP))) {
+        if (!(fl & F_CLUMPED)) {  // clumped nodes move with their master (k_clump_members)
             const double dt = a.dt, damping = a.damping;
             double4 vm = n.velm[i], aw = n.angI[i];
             D3 vel = xyz(vm), angVel = xyz(aw);
@@ -327,6 +341,19 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(const SceneDesc* D, cons
             D3 g = mk3(a.g[0], a.g[1], a.g[2]);
             bool damp = (damping != 0. && !(fl & F_DAMPING_SKIP));
             bool blockedNone = (fl & F_DOF_ALL) == 0, blockedAll = (fl & F_DOF_ALL) == F_DOF_ALL;
+            // ClumpData::forceTorqueFromMembers, pkg/dem/Clump.cpp:265-273; skipped when nothing
+            // uses the sums (all DOFs blocked and no damping dissipation tracked)
+            if ((fl & F_CLUMP) && (!blockedAll || (a.reallyTrackEnergy && damp && !(fl & F_ENERGY_SKIP)))) {
+                const int c = S.cl.nodeClump[i];
+                for (int k = S.cl.off[c]; k < S.cl.off[c + 1]; k++) {
+                    const int mn = S.cl.mem[k];
+                    D3 fm, tm;
+                    if (GATHER) gatherForces(a, n, p, gv, mn, n.flags[mn], 0., xyz(n.posr[mn]), fm, tm);
+                    else { fm = xyz(n.force[mn]); tm = xyz(n.torque[mn]); }
+                    f = f + fm;
+                    t = t + (tm + cross3(xyz(n.posr[mn]) - pos, fm));
+                }
+            }
             if (a.reallyTrackEnergy) {
                 if (damp && !(fl & F_ENERGY_SKIP)) {
                     ev[0] = dot3(abs3(vel), abs3(f)) * damping * dt + dot3(abs3(angVel), abs3(t)) * damping * dt;
@@ -426,12 +453,57 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(const SceneDesc* D, cons
             n.velm[i] = make_double4(vel.x, vel.y, vel.z, mass);
             n.angI[i] = make_double4(angVel.x, angVel.y, angVel.z, inertia);
         }
-        if (a.checkVerlet && n.checked[i]) {
+        if (a.checkVerlet && n.checked[i] && !(fl & F_CLUMPED)) {  // members: checked after k_clump_members
             D3 d = xyz(n.lastPos[i]) - pos;
             if (sqn3(d) > a.verletD2) bits |= VERLET_BIT;
         }
     }
     blockSumWrite<3>(ev, bits, a.reallyTrackEnergy ? epart : nullptr, S.words, a.stamp, S.words + 1);
+}
+
+// quaternions are stored (w, x, y, z) in double4 (x, y, z, w); products and
+// quaternion*vector follow Eigen 3's scalar formulas
+__device__ __forceinline__ double4 qmul4(const double4& a, const double4& b) {
+    double4 r;
+    r.x = ((a.x * b.x - a.y * b.y) - a.z * b.z) - a.w * b.w;
+    r.y = ((a.x * b.y + a.y * b.x) + a.z * b.w) - a.w * b.z;
+    r.z = ((a.x * b.z + a.z * b.x) + a.w * b.y) - a.y * b.w;
+    r.w = ((a.x * b.w + a.w * b.x) + a.y * b.z) - a.z * b.y;
+    return r;
+}
+__device__ __forceinline__ D3 qrot4(const double4& q, const D3& v) {
+    D3 u = mk3(q.y, q.z, q.w);
+    D3 t = cross3(u, v);
+    t = t + t;
+    return (v + t * q.x) + cross3(u, t);
+}
+// ClumpData::applyToMembers, pkg/dem/Clump.cpp:276-289, after Leapfrog moved the masters,
+// then the collider's Verlet displacement check (InsertionSortCollider.cpp:340-350) on the
+// members' new positions
+__global__ void __launch_bounds__(WDM_BLOCK) k_clump_members(const SceneDesc* D, const int32_t* idx, Map m) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const SceneDesc& S = D[idx ? idx[t] : t];
+    const StepArgs a = S.a;
+    const NodeView n = S.n;
+    const ClumpView cl = S.cl;
+    const int k = (int)(lb * WDM_BLOCK + threadIdx.x);
+    bool trig = false;
+    if (k < cl.nMembers) {
+        const int mn = cl.mem[k];
+        const int M = n.master[mn];
+        const double4 P = n.posr[M], Q = n.ori[M], V = n.velm[M], W = n.angI[M];
+        const D3 pos = xyz(P) + qrot4(Q, xyz(cl.relPos[k]));
+        const double4 o = qmul4(Q, cl.relOri[k]);
+        const D3 vel = xyz(V) + cross3(xyz(W), pos - xyz(P));
+        const double4 pr = n.posr[mn], vm = n.velm[mn], aw = n.angI[mn];
+        n.posr[mn] = make_double4(pos.x, pos.y, pos.z, pr.w);
+        n.ori[mn] = o;
+        n.velm[mn] = make_double4(vel.x, vel.y, vel.z, vm.w);
+        n.angI[mn] = make_double4(W.x, W.y, W.z, aw.w);
+        if (a.checkVerlet && n.checked[mn]) trig = sqn3(xyz(n.lastPos[mn]) - pos) > a.verletD2;
+    }
+    if (__ballot(trig) != 0ull && (threadIdx.x & 63) == 0) atomicMax(S.words, a.stamp);
 }
 
 // Per-block partial energies of each scene summed in a fixed order (a strided sequential
@@ -665,17 +737,25 @@ __device__ __forceinline__ void canonical(int i, int j, int si, int sj, int& A, 
     A = i < j ? i : j; B = i < j ? j : i; type = CI_NONE;
 }
 
-// Collider::mayCollide, pkg/dem/Collision.cpp:19-41 (no clumps on this path)
-__device__ __forceinline__ bool mayCollideD(const PartView& p, int a, int b, int sa, int sb, uint32_t loneMask) {
+// Collider::mayCollide, pkg/dem/Collision.cpp:19-41; master: clump master per node (null
+// when the scene has no clumps)
+__device__ __forceinline__ bool mayCollideD(const PartView& p, const GP int32_t* master, int a, int b, int sa, int sb, uint32_t loneMask) {
     if (a == b || sa == 0 || sb == 0) return false;
     uint32_t ma = p.mask[a], mb = p.mask[b];
     if (!(ma & mb)) return false;
     if ((ma & mb & loneMask) != 0) return false;
-    if (sa == 3 || sb == 3) {
+    if (sa == 3 || sb == 3 || master) {
         int na[3], nb[3], ka, kb;
         if (sa == 3) { int4 f = p.fnodes[a]; na[0] = f.x; na[1] = f.y; na[2] = f.z; ka = 3; } else { na[0] = a; ka = 1; }
         if (sb == 3) { int4 f = p.fnodes[b]; nb[0] = f.x; nb[1] = f.y; nb[2] = f.z; kb = 3; } else { nb[0] = b; kb = 1; }
-        for (int x = 0; x < ka; x++) for (int y = 0; y < kb; y++) if (na[x] == nb[y]) return false;
+        for (int x = 0; x < ka; x++)
+            for (int y = 0; y < kb; y++) {
+                if (na[x] == nb[y]) return false;  // shared node
+                if (master) {                      // nodes of the same clump
+                    int mx = master[na[x]];
+                    if (mx >= 0 && mx == master[nb[y]]) return false;
+                }
+            }
     }
     return true;
 }
@@ -758,7 +838,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pairs(const SceneDesc* D, const F
                     const double4 &A0 = aIsId ? a0 : b0, &A1 = aIsId ? a1 : b1, &B0 = aIsId ? b0 : a0, &B1 = aIsId ? b1 : a1;
                     int per[3] = {0, 0, 0};
                     bool ov = g.periodic ? overlapPeri(A0, A1, B0, B1, g.L, per, err) : true;
-                    if (!ov || !mayCollideD(p, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), loneMask)) continue;
+                    if (!ov || !mayCollideD(p, S.n.master, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), loneMask)) continue;
                     if (MODE) {
                         int lo = id < j ? id : j, hi = id < j ? j : id;
                         okeys[o + cnt] = tkey | ((uint64_t)(uint32_t)lo << idBits) | (uint32_t)hi;
@@ -777,7 +857,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pairs(const SceneDesc* D, const F
         double4 A0 = p.bmin[A], A1 = p.bmax[A], B0 = p.bmin[B], B1 = p.bmax[B];
         int per[3] = {0, 0, 0};
         bool ov = g.periodic ? overlapPeri(A0, A1, B0, B1, g.L, per, err) : overlapAperiodic(A0, A1, B0, B1);
-        if (!ov || !mayCollideD(p, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), loneMask)) continue;
+        if (!ov || !mayCollideD(p, S.n.master, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), loneMask)) continue;
         if (MODE) {
             int lo = id < j ? id : j, hi = id < j ? j : id;
             okeys[o + cnt] = tkey | ((uint64_t)(uint32_t)lo << idBits) | (uint32_t)hi;

@@ -221,7 +221,7 @@ struct Shared {
     std::vector<int32_t> freeSlots;
     int32_t nslots = 0;
     Staged stDesc, stMapNode, stMapCon, stCon, stFR, stConFR, stIdx;
-    Staged stMapP, stMapB, stMapC, stMapO, stMapN, stMapE, stMapP1;
+    Staged stMapP, stMapB, stMapC, stMapO, stMapN, stMapE, stMapP1, stMapCl;
     hipEvent_t evWords = nullptr;  // recorded after the copy of the words to the host
     // energy partials per block
     DBuf<double> epartL, epartC;
@@ -241,7 +241,7 @@ struct Shared {
         hostWords = nullptr;
         hostWordsCap = 0;
         for (Staged* s : {&stDesc, &stMapNode, &stMapCon, &stCon, &stFR, &stConFR, &stIdx, &stMapP, &stMapB, &stMapC, &stMapO, &stMapN,
-                          &stMapE, &stMapP1})
+                          &stMapE, &stMapP1, &stMapCl})
             s->free(c);
         release(c, epartL); release(c, epartC); release(c, extPart); release(c, extOut);
         release(c, ckeys); release(c, ckeys2); release(c, cvals); release(c, cvals2); release(c, cstart); release(c, cend);
@@ -265,6 +265,7 @@ struct Scene {
     std::vector<int32_t> ptypeH;
     std::vector<double> radiusH;
     std::vector<int32_t> binnedH, unbH;
+    std::vector<uint32_t> flagsH;
     // cell (core/Cell.hpp)
     bool periodic = false;
     HM3 hSize = heye(), gradV = hzero(), nextGradV = hzero(), pprevHsize = heye(), trsf = heye();
@@ -298,6 +299,12 @@ struct Scene {
     DBuf<uint64_t> realMask;
     // energy accumulators
     DBuf<double> E;
+    // clumps
+    int32_t nClumps = 0, nMembers = 0;
+    int64_t clumpFlagged = 0;  // nodes flagged CLUMP or CLUMPED at upload
+    bool clumpsSet = true;     // flags and uploaded clumps agree
+    DBuf<int32_t> master, nodeClump, cOff, cMem;
+    DBuf<double4> cRelPos, cRelOri;
 
     Scene(Ctx* c, int32_t np_, int32_t nn_, int32_t nm_) : ctx(c), np(np_), nn(nn_), nmat(nm_) {}
 
@@ -305,7 +312,8 @@ struct Scene {
 
     NodeView nview() {
         NodeView v; v.posr = gpc(posr.p); v.velm = gpc(velm.p); v.angI = gpc(angI.p); v.ori = gpc(ori.p); v.force = gpc(force.p);
-        v.torque = gpc(torque.p); v.lastPos = gpc(lastPos.p); v.flags = gpc(flags.p); v.checked = gpc(checked.p); v.nn = nn; return v;
+        v.torque = gpc(torque.p); v.lastPos = gpc(lastPos.p); v.flags = gpc(flags.p); v.checked = gpc(checked.p);
+        v.master = nClumps > 0 ? gpc(master.p) : nullptr; v.nn = nn; return v;
     }
     GatherView gview() {
         GatherView g;
@@ -324,6 +332,7 @@ struct Scene {
         release(ctx, unb); release(ctx, pmask); release(ctx, fnodes); release(ctx, bmin); release(ctx, bmax); release(ctx, mat);
         for (int i = 0; i < 2; i++) { cb64[i].free(ctx); cb32[i].free(ctx); }
         release(ctx, csrOff); release(ctx, entV); release(ctx, realMask); release(ctx, E);
+        release(ctx, master); release(ctx, nodeClump); release(ctx, cOff); release(ctx, cMem); release(ctx, cRelPos); release(ctx, cRelOri);
     }
 };
 
@@ -476,6 +485,7 @@ StepArgs makeStepArgs(Scene& S) {
     a.checkVerlet = S.lastLeapfrogVerletChecked ? 1 : 0;
     a.stamp = stampOf(S.step);
     a.gatherOn = S.forcesFromContacts ? 1 : 0;
+    a.hasClumps = S.nClumps > 0 ? 1 : 0;
     a.distFactor = P.distFactor;
     a.law = P.law; a.noSlip = P.noSlip; a.noBreak = P.noBreak; a.noFrict = P.noFrict; a.noAttraction = P.noAttraction;
     a.trsfRenorm = P.trsfRenorm;
@@ -507,6 +517,15 @@ SceneDesc makeDesc(Scene& S) {
     d.nunb = (int)S.unbH.size();
     d.E = gpc(S.E.p);
     d.words = S.ctx->sh->words.p + 4 * (size_t)S.slot;
+    if (S.nClumps > 0) {
+        d.cl.off = gpc((const int32_t*)S.cOff.p);
+        d.cl.mem = gpc((const int32_t*)S.cMem.p);
+        d.cl.relPos = gpc((const double4*)S.cRelPos.p);
+        d.cl.relOri = gpc((const double4*)S.cRelOri.p);
+        d.cl.nodeClump = gpc((const int32_t*)S.nodeClump.p);
+        d.cl.nClumps = S.nClumps;
+        d.cl.nMembers = S.nMembers;
+    }
     return d;
 }
 
@@ -611,6 +630,22 @@ struct Leapfrog {
                 hipLaunchKernelGGL(k_energy_finalize, dim3(3 * ng), dim3(WDM_BLOCK), 0, c->stream, B.D, id, 1, m, (const double*)ep, 3, 0);
                 checkLaunch();
             }
+        }
+        // clump members follow their masters (ClumpData::applyToMembers)
+        std::vector<int32_t> cidx;
+        for (int i = 0; i < n; i++)
+            if (B.ss[i]->nMembers > 0) cidx.push_back(i);
+        if (!cidx.empty()) {
+            std::vector<int64_t> cbp = blockPrefix((int)cidx.size(), [&](int t) { return B.ss[cidx[t]]->nMembers; });
+            Blob cb;
+            size_t oi = cb.addv(cidx), ob = cb.addv(cbp);
+            uint8_t* cbase = X.stMapCl.upload(c, cb.b.data(), cb.b.size());
+            Map m;
+            m.bprefix = (const int64_t*)(cbase + ob);
+            m.n = (int)cidx.size();
+            KTimer t(c, "clump_members");
+            hipLaunchKernelGGL(k_clump_members, dim3((unsigned)cbp.back()), dim3(WDM_BLOCK), 0, c->stream, B.D, (const int32_t*)(cbase + oi), m);
+            checkLaunch();
         }
     }
 };
@@ -978,6 +1013,7 @@ struct ContactLoop {
 
 void checkReady(Scene& S) {
     if (!(S.paramsSet && S.nodesSet && S.partsSet && S.matsSet)) throw WdmError(WDM_E_STATE, "scene not fully uploaded");
+    if (!S.clumpsSet) throw WdmError(WDM_E_STATE, "clump flags of the nodes and the uploaded clumps disagree (wdm_upload_clumps)");
 }
 // Scene::doOneStep prologue (core/Scene.cpp:371-389): cell update
 void stepBeginCell(Scene& S) {
@@ -1260,8 +1296,9 @@ int wdm_upload_nodes(wdm_ctx* c, int32_t sid, int32_t n, const double* pos3, con
         std::vector<double4> pr(n), vm(n), aw(n), q(n), f(n), t(n);
         std::vector<double> rad(n, 0.);
         if (S.partsSet) for (int p = 0; p < S.np; p++) rad[p] = S.radiusH[p];
+        int64_t nclf = 0;
         for (int i = 0; i < n; i++) {
-            if (flags[i] & (WDM_CLUMP_CLUMP | WDM_CLUMP_CLUMPED)) throw WdmError(WDM_E_UNSUPPORTED, "clumps are not supported on this path");
+            if (flags[i] & (WDM_CLUMP_CLUMP | WDM_CLUMP_CLUMPED)) nclf++;
             const double* I = inertia3 + 3 * i;
             bool asph = !(I[0] == I[1] && I[1] == I[2]);
             if (asph && (flags[i] & 56u) != 56u) throw WdmError(WDM_E_UNSUPPORTED, "aspherical integrator is not supported (node " + std::to_string(i) + ")");
@@ -1284,6 +1321,10 @@ int wdm_upload_nodes(wdm_ctx* c, int32_t sid, int32_t n, const double* pos3, con
         S.nodesSet = true;
         S.needFull = true;
         S.forcesFromContacts = false;
+        S.clumpFlagged = nclf;
+        S.flagsH.assign(flags, flags + n);
+        S.nClumps = S.nMembers = 0;  // clumps are uploaded again after the nodes
+        S.clumpsSet = nclf == 0;
     });
 }
 
@@ -1355,6 +1396,58 @@ int wdm_upload_materials(wdm_ctx* c, int32_t sid, int32_t n, const double* densi
         HIPCHK(hipMemcpyAsync(S.mat.p, m.data(), n * sizeof(double4), hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         S.matsSet = true;
+    });
+}
+
+int wdm_upload_clumps(wdm_ctx* c, int32_t sid, int32_t nc, const int32_t* master, const int32_t* off, const int32_t* members,
+                      const double* relPos3, const double* relOri4) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        if (!S.nodesSet) throw WdmError(WDM_E_STATE, "upload nodes before clumps");
+        if (nc < 0 || (nc > 0 && (!master || !off || !members || !relPos3 || !relOri4))) throw WdmError(WDM_E_ARG, "clump arrays");
+        const int nn = S.nn;
+        std::vector<int32_t> nodeMaster(nn, -1), nodeClump(nn, -1);
+        if (nc > 0 && off[0] != 0) throw WdmError(WDM_E_ARG, "clump offsets must start at 0");
+        for (int k = 0; k < nc; k++) {
+            int M = master[k];
+            if (M < 0 || M >= nn || !(S.flagsH[M] & WDM_CLUMP_CLUMP) || nodeClump[M] >= 0) throw WdmError(WDM_E_ARG, "bad clump master node");
+            if (off[k + 1] < off[k]) throw WdmError(WDM_E_ARG, "clump offsets must not decrease");
+            nodeClump[M] = k;
+            for (int j = off[k]; j < off[k + 1]; j++) {
+                int m = members[j];
+                if (m < 0 || m >= nn || !(S.flagsH[m] & WDM_CLUMP_CLUMPED) || nodeMaster[m] >= 0 || (S.flagsH[m] & WDM_CLUMP_CLUMP))
+                    throw WdmError(WDM_E_ARG, "bad clump member node " + std::to_string(m));
+                nodeMaster[m] = M;
+            }
+        }
+        for (int i = 0; i < nn; i++) {
+            if ((S.flagsH[i] & WDM_CLUMP_CLUMP) && nodeClump[i] < 0) throw WdmError(WDM_E_ARG, "clump master node without a clump");
+            if ((S.flagsH[i] & WDM_CLUMP_CLUMPED) && nodeMaster[i] < 0) throw WdmError(WDM_E_ARG, "clumped node without a master");
+        }
+        const int nm = nc > 0 ? off[nc] : 0;
+        std::vector<double4> rp(nm), ro(nm);
+        for (int j = 0; j < nm; j++) {
+            rp[j] = make_double4(relPos3[3 * j], relPos3[3 * j + 1], relPos3[3 * j + 2], 0.);
+            ro[j] = make_double4(relOri4[4 * j], relOri4[4 * j + 1], relOri4[4 * j + 2], relOri4[4 * j + 3]);
+        }
+        hipStream_t st = c->stream;
+        if (nc > 0) {
+            ensure(c, S.master, nn); ensure(c, S.nodeClump, nn); ensure(c, S.cOff, (size_t)nc + 1);
+            ensure(c, S.cMem, std::max(nm, 1)); ensure(c, S.cRelPos, std::max(nm, 1)); ensure(c, S.cRelOri, std::max(nm, 1));
+            HIPCHK(hipMemcpyAsync(S.master.p, nodeMaster.data(), nn * sizeof(int32_t), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(S.nodeClump.p, nodeClump.data(), nn * sizeof(int32_t), hipMemcpyHostToDevice, st));
+            HIPCHK(hipMemcpyAsync(S.cOff.p, off, ((size_t)nc + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+            if (nm > 0) {
+                HIPCHK(hipMemcpyAsync(S.cMem.p, members, nm * sizeof(int32_t), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(S.cRelPos.p, rp.data(), nm * sizeof(double4), hipMemcpyHostToDevice, st));
+                HIPCHK(hipMemcpyAsync(S.cRelOri.p, ro.data(), nm * sizeof(double4), hipMemcpyHostToDevice, st));
+            }
+            HIPCHK(hipStreamSynchronize(st));
+        }
+        S.nClumps = nc;
+        S.nMembers = nm;
+        S.clumpsSet = true;
+        S.needFull = true;
     });
 }
 

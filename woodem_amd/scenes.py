@@ -178,3 +178,81 @@ def deposition_box(seed=0, n=None, rmin=2e-3, rmax=6e-3, phi=.40, density=3000.,
     a.meta.update(dict(kind="deposition_box", dt=dt, gravity=(0., 0., -9.81), damping=damping, law="IdealElPl",
                        periodic=False, n_spheres=N, seed=seed, side=side, tanPhi=tanPhi))
     return a
+
+
+def drum(seed=0, n=None, radius=.25, length=.1, segments=64, rings=8, rmin=1e-3, rmax=4e-3, omega=None, nmin=1000,
+         nmax=200000, density=3000., young=1e7, ktDivKn=.2, tanPhi=.5, damping=.2, gap=.5e-3):
+    """Rotating drum of the serving workload (the pattern of examples/mill.py:28-57): a
+    cylinder of `segments` x `rings` quads (2 facets each) around the x axis with two end
+    walls, all clumped to a central node at the origin (ClumpData::makeClump of massless
+    nodes: every DOF of the master blocked, pkg/dem/Clump.cpp:228-237) that spins at
+    omega ~ U[.5, 3] rad/s about x.  Spheres: count round(exp(U(ln 1e3, ln 2e5))) capped at
+    the drum's lattice capacity, r ~ U[1, 4] mm, on a cubic lattice of pitch 2 r_max + gap
+    inside the drum, lowest sites first.  Not fixed by the workload text and chosen here:
+    the lattice fill and its capacity cap (2e5 spheres of this size cannot fit the drum),
+    FrictMat E=1e7, rho=3000, ktDivKn=.2, tanPhi=.5, damping .2, IdealElPl,
+    dt = 0.3 r_min / sqrt(E/rho)."""
+    rng = np.random.default_rng(seed)
+    want = int(n) if n is not None else int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+    w = float(omega) if omega is not None else float(rng.uniform(.5, 3.))
+    pitch = 2 * rmax + gap
+    rin = radius * math.cos(math.pi / segments) - rmax - gap  # inside the flat facets
+    xs = np.arange(-.5 * length + rmax + gap, .5 * length - rmax - gap + 1e-12, pitch)
+    ys = np.arange(-rin, rin + 1e-12, pitch)
+    X, Y, Z = np.meshgrid(xs, ys, ys, indexing="ij")
+    sites = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    sites = sites[sites[:, 1] ** 2 + sites[:, 2] ** 2 <= rin * rin]
+    sites = sites[np.lexsort((sites[:, 0], sites[:, 1], sites[:, 2]))]  # lowest first
+    capacity = len(sites)
+    N = min(want, capacity)
+    r = rng.uniform(rmin, rmax, N)
+    a = _soa.spheres_arrays(sites[:N], r, density=density, young=young, ktDivKn=ktDivKn, tanPhi=tanPhi)
+    a = _soa.append_walls(a, [(-.5 * length, 0, 1), (.5 * length, 0, -1)])
+    nw = 2
+    # facet vertices: rings+1 circles of `segments` points
+    th = 2 * math.pi * np.arange(segments) / segments
+    xv = np.linspace(-.5 * length, .5 * length, rings + 1)
+    verts = np.array([(x, radius * math.cos(t), radius * math.sin(t)) for x in xv for t in th])
+    vid = lambda j, i: j * segments + (i % segments)
+    tris = []
+    for j in range(rings):
+        for i in range(segments):
+            tris.append((vid(j, i), vid(j + 1, i), vid(j + 1, i + 1)))
+            tris.append((vid(j, i), vid(j + 1, i + 1), vid(j, i + 1)))
+    nf = len(tris)
+    npar0 = a.n_par  # spheres + walls
+    npar = npar0 + nf
+    nv = len(verts)
+    master = npar + nv  # central node
+    nn = master + 1
+    ori = np.zeros((nn, 4))
+    ori[:, 0] = 1
+    pos_n = np.concatenate([a.pos, np.zeros((nf, 3)), verts, np.zeros((1, 3))])
+    flags = np.concatenate([a.flags, np.full(nf + nv + 1, _soa.DOF_ALL, np.uint32)])
+    members = np.concatenate([np.arange(N, N + nw), np.arange(npar, npar + nv)]).astype(np.int32)
+    flags[members] |= _soa.CLUMP_CLUMPED
+    flags[master] |= _soa.CLUMP_CLUMP
+    angVel = np.zeros((nn, 3))
+    angVel[master] = (w, 0., 0.)
+    # members move with the master from the start (ClumpData::applyToMembers)
+    for mi in members:
+        angVel[mi] = (w, 0., 0.)
+    vel = np.zeros((nn, 3))
+    vel[members] = np.cross(angVel[members], pos_n[members])
+    b = _soa.DemArrays(
+        pos=pos_n, ori=ori, vel=vel, angVel=angVel,
+        mass=np.concatenate([a.mass, np.zeros(nf + nv + 1)]), inertia=np.concatenate([a.inertia, np.zeros((nf + nv + 1, 3))]),
+        flags=flags, force=np.zeros((nn, 3)), torque=np.zeros((nn, 3)),
+        shape=np.concatenate([a.shape, np.full(nf, 3, np.int32)]),
+        nodes=np.concatenate([a.nodes, np.array(tris, np.int32) + npar]),
+        radius=np.concatenate([a.radius, np.zeros(nf)]), axis=np.concatenate([a.axis, np.zeros(nf, np.int32)]),
+        sense=np.concatenate([a.sense, np.zeros(nf, np.int32)]),
+        mask=np.concatenate([a.mask, np.full(nf, _soa.DEFAULT_BOUNDARY_MASK, np.uint32)]),
+        mat=np.zeros(npar, np.int32), density=a.density, young=a.young, ktDivKn=a.ktDivKn, tanPhi=a.tanPhi,
+        clumps=dict(master=np.array([master], np.int32), off=np.array([0, len(members)], np.int32), members=members,
+                    relPos=pos_n[members], relOri=ori[members])).normalized()
+    b.validate()
+    dt = 0.3 * float(r.min()) / math.sqrt(young / density)
+    b.meta.update(dict(kind="drum", dt=dt, gravity=(0., 0., -9.81), damping=damping, law="IdealElPl", periodic=False,
+                       n_spheres=N, n_requested=want, capacity=capacity, omega=w, n_facets=nf, seed=seed))
+    return b

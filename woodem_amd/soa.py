@@ -46,6 +46,8 @@ class DemArrays:
     ktDivKn: np.ndarray
     tanPhi: np.ndarray
     meta: dict = field(default_factory=dict)
+    # ClumpData (pkg/dem/Clump.hpp): dict(master[nc], off[nc+1], members[m], relPos[m,3], relOri[m,4]) or None
+    clumps: dict = None
 
     @property
     def n_node(self):
@@ -72,7 +74,17 @@ class DemArrays:
             shape=i32(self.shape, (p,)), nodes=i32(self.nodes, (p, 3)), radius=f64(self.radius, (p,)),
             axis=i32(self.axis, (p,)), sense=i32(self.sense, (p,)), mask=u32(self.mask, (p,)), mat=i32(self.mat, (p,)),
             density=f64(self.density, (m,)), young=f64(self.young, (m,)), ktDivKn=f64(self.ktDivKn, (m,)),
-            tanPhi=f64(self.tanPhi, (m,)), meta=dict(self.meta))
+            tanPhi=f64(self.tanPhi, (m,)), meta=dict(self.meta), clumps=self._norm_clumps())
+
+    def _norm_clumps(self):
+        c = self.clumps
+        if not c:
+            return None
+        nm = len(c["members"])
+        return dict(master=np.ascontiguousarray(c["master"], np.int32), off=np.ascontiguousarray(c["off"], np.int32),
+                    members=np.ascontiguousarray(c["members"], np.int32),
+                    relPos=np.ascontiguousarray(c["relPos"], np.float64).reshape(nm, 3),
+                    relOri=np.ascontiguousarray(c["relOri"], np.float64).reshape(nm, 4))
 
     def copy(self):
         return self.normalized()
