@@ -14,6 +14,13 @@ U[.1, .8], damping ~ U[.05, .4]), FP64 state with FP32 contact arithmetic, stepp
 together by one wdm_step call per step batch; the same JSON line carries the
 sequential rate (each scene stepped alone, on a sample of the scenes) and the ratio.
 
+--workload serve: a Poisson stream (20 requests/s, seeded) of rotating drums
+(woodem_amd.scenes.drum: 1,024 facets + 2 end walls clumped to a spinning centre node,
+1e3..2e5 spheres log-uniform capped at the drum's lattice capacity, r ~ U[1, 4] mm,
+IdealElPl, FP64), 10,000 steps per request, served by woodem_amd.serving.Server
+(admission by HBM footprint, continuous batching); value = aggregate particle-steps/s
+over the serving wall time, with per-request latency p50/p99.
+
 One JSON line on stdout.  --gpus N with N > 1 exits: this build runs on one GPU.
 """
 import argparse
@@ -69,7 +76,7 @@ def bench_batch(args, ctx):
     from woodem_amd import runner, scenes
     from woodem_amd._native import PREC_MIXED
     t0 = time.time()
-    arrs = [scenes.deposition_box(seed=1000 + s) for s in range(args.scenes)]
+    arrs = [scenes.relax_box(scenes.deposition_box(seed=1000 + s), ctx, iters=args.relax_box) for s in range(args.scenes)]
     runs = [runner.HipRun(a, runner.params_from_meta(a, precision=PREC_MIXED), runner.cell_from_meta(a), ctx=ctx) for a in arrs]
     sids = [r.sid for r in runs]
     n_tot = sum(a.n_par for a in arrs)
@@ -153,10 +160,57 @@ def bench_batch(args, ctx):
     }
 
 
+def bench_serve(args, ctx):
+    """Continuous batching of rotating-drum requests arriving as a Poisson stream."""
+    from woodem_amd import runner, scenes, serving
+    t0 = time.time()
+    arr = serving.poisson_arrivals(args.requests, args.rate, seed=args.seed)
+    reqs = []
+    for i, t in enumerate(arr):
+        d = scenes.drum(seed=args.seed * 100000 + i)
+        reqs.append(serving.Request(i, d, runner.params_from_meta(d), runner.cell_from_meta(d), args.req_steps, float(t)))
+    n_req = np.array([r.arrays.meta["n_requested"] for r in reqs])
+    n_sph = np.array([r.arrays.meta["n_spheres"] for r in reqs])
+    log("serve: %d drum requests over %.2f s, %d..%d spheres (mean %.0f; %d capped at the drum capacity), built in %.1f s" % (
+        len(reqs), arr[-1], n_sph.min(), n_sph.max(), n_sph.mean(), int((n_req > n_sph).sum()), time.time() - t0))
+    # warm-up: one small request through the same path (code objects, allocator)
+    w = scenes.drum(seed=1, n=2000)
+    serving.Server(ctx, args.hbm_budget, chunk=args.chunk).serve(
+        [serving.Request(-1, w, runner.params_from_meta(w), runner.cell_from_meta(w), 200, 0.)])
+    srv = serving.Server(ctx, args.hbm_budget, chunk=args.chunk, log=log)
+    srv.serve(reqs)
+    rep = srv.report()
+    log("served: %s" % json.dumps(rep))
+    return {
+        "metric": "particle-steps/s on 1 MI355X (served drums, continuous batching); p50/p99 latency",
+        "value": rep["particle_steps_per_s"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.req_steps,
+        "warmup": 200,
+        "ms_per_step": rep["step_ms"]["p50"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Poisson arrivals and drum generator, seed %d)" % args.seed,
+        "config": {"workload": "%d rotating-drum requests, Poisson %.0f/s, %d steps each, served with continuous batching" % (
+            len(reqs), args.rate, args.req_steps), "requests": len(reqs), "rate_per_s": args.rate, "chunk_steps": args.chunk,
+                   "spheres_mean": float(n_sph.mean()), "hbm_budget": args.hbm_budget},
+        "latency_s": rep["latency_s"],
+        "admission_wait_s": rep["admission_wait_s"],
+        "step_latency_ms": rep["step_ms"],
+        "max_resident": rep["max_resident"],
+        "wall_s": rep["wall_s"],
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 1000; 5000 for --workload batch)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
@@ -167,13 +221,22 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
-    ap.add_argument("--workload", choices=["periodic", "batch"], default="periodic")
+    ap.add_argument("--workload", choices=["periodic", "batch", "serve"], default="periodic")
+    ap.add_argument("--requests", type=int, default=100, help="serve workload: number of requests")
+    ap.add_argument("--rate", type=float, default=20., help="serve workload: mean arrivals per second")
+    ap.add_argument("--req-steps", type=int, default=10000, help="serve workload: steps per request")
+    ap.add_argument("--chunk", type=int, default=100, help="serve workload: steps per batched call")
+    ap.add_argument("--hbm-budget", type=float, default=200e9, help="serve workload: bytes the resident scenes may use")
+    ap.add_argument("--seed", type=int, default=5, help="serve workload: arrival and scene seed")
     ap.add_argument("--scenes", type=int, default=512, help="batch workload: number of scenes")
     ap.add_argument("--seq-scenes", type=int, default=32, help="batch workload: scenes in the sequential sample")
     ap.add_argument("--seq-steps", type=int, default=300, help="batch workload: steps per scene in the sequential sample")
+    ap.add_argument("--relax-box", type=int, default=200, help="batch workload: overlap relaxation sweeps per scene")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB (5 times), for scaling FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 5000 if args.workload == "batch" else 1000
     if args.calibrate_copy:
         import torch
         a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
@@ -194,6 +257,9 @@ def main():
     ctx = Context(0)
     if args.workload == "batch":
         print(json.dumps(bench_batch(args, ctx)), flush=True)
+        return
+    if args.workload == "serve":
+        print(json.dumps(bench_serve(args, ctx)), flush=True)
         return
     t0 = time.time()
     a = scenes.periodic_hertz(n=args.n)

@@ -26,7 +26,7 @@ def scene_set(ctx):
     out.append((c, runner.params_from_meta(c, trackEnergy=True), runner.cell_from_meta(c)))
     d = scenes.facet_floor(n=5, seed=14)
     out.append((d, runner.params_from_meta(d, trackEnergy=True), runner.cell_from_meta(d)))
-    e = scenes.deposition_box(seed=15, n=700)
+    e = scenes.relax_box(scenes.deposition_box(seed=15, n=700), ctx, iters=100)
     out.append((e, runner.params_from_meta(e, precision=PREC_MIXED), runner.cell_from_meta(e)))
     f = scenes.drum(seed=16, n=400, omega=2., gap=1e-4)
     ns = f.meta["n_spheres"]

@@ -1474,7 +1474,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
 // ---------------- scene generation: Jacobi overlap relaxation ----------------
 // Each sphere moves by half of every overlap with its neighbours (found through the
 // same cell grid); all displacements are computed from the old positions, so the
-// result does not depend on thread order.
+// result does not depend on thread order.  Walls then push spheres back to their side.
 __global__ void __launch_bounds__(WDM_BLOCK) k_relax(const SceneDesc* D, const FRScene* FR, Map m, const uint32_t* ckeys, const int32_t* cvals,
                                                     const int32_t* cstart, const int32_t* cend, double4* disp) {
     int64_t lb;
@@ -1533,5 +1533,17 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_relax_apply(const SceneDesc* D, c
     int id = S.binned[i];
     double4 pr = n.posr[id];
     double4 d = disp[id];
-    n.posr[id] = make_double4(pr.x + d.x, pr.y + d.y, pr.z + d.z, pr.w);
+    double x[3] = {pr.x + d.x, pr.y + d.y, pr.z + d.z};
+    if (ptShape(S.p.ptype[id]) == 1) {  // walls keep spheres on their side
+        for (int u = 0; u < S.nunb; u++) {
+            const int w = S.unb[u];
+            const int32_t pt = S.p.ptype[w];
+            const int ax = ptAxis(pt), se = ptSense(pt);
+            const double4 pw = n.posr[w];
+            const double c = ax == 0 ? pw.x : (ax == 1 ? pw.y : pw.z);
+            if (se > 0) x[ax] = fmax(x[ax], c + pr.w);
+            else if (se < 0) x[ax] = fmin(x[ax], c - pr.w);
+        }
+    }
+    n.posr[id] = make_double4(x[0], x[1], x[2], pr.w);
 }

@@ -14,12 +14,11 @@ periodic_hertz: N spheres, lognormal radii (median 1 mm, sigma .3, clipped to
   then renumbered along a Morton curve so that neighbours have nearby ids.
 deposition_box: one scene of the 512-scene batch workload: N = round(exp(U(ln 500,
   ln 2e4))) spheres, r ~ U[2, 6] mm, tanPhi ~ U[.1, .8], Leapfrog damping ~ U[.05, .4],
-  all drawn from the scene's own seed.  Not fixed by the workload text and chosen here:
-  the container is open at the top with a square base and height equal to the base
-  side, its volume being the solid volume / 0.40; the spheres start on a square lattice
-  of pitch 2 r_max + 0.5 mm (xy positions jittered inside their lattice cell) stacked
-  from the floor up, so the upper layers start above the rim and fall in; FrictMat
-  E=1e7, rho=3000, ktDivKn=.2, IdealElPl, gravity -9.81 z, dt = 0.3 r_min / sqrt(E/rho).
+  all drawn from the scene's own seed, in a closed cubic box (6 walls) whose volume is
+  the solid volume / 0.40.  Not fixed by the workload text and chosen here: the cube
+  shape; centres uniform in the box, overlaps then relaxed on the device with the walls
+  holding the spheres in (relax_box); FrictMat E=1e7, rho=3000, ktDivKn=.2, IdealElPl,
+  gravity -9.81 z, dt = 0.3 r_min / sqrt(E/rho).
 """
 import math
 
@@ -157,27 +156,38 @@ def facet_floor(n=6, tilt=.2, cells=6, size=.1, radius=4e-3, spacing=9e-3, seed=
 
 
 def deposition_box(seed=0, n=None, rmin=2e-3, rmax=6e-3, phi=.40, density=3000., young=1e7, ktDivKn=.2, nmin=500,
-                   nmax=20000, gap=.5e-3):
+                   nmax=20000):
+    """Initial state before relaxation (uniform centres, overlapping); see relax_box."""
     rng = np.random.default_rng(seed)
     N = int(n) if n is not None else int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
     r = rng.uniform(rmin, rmax, N)
     tanPhi = float(rng.uniform(.1, .8))
     damping = float(rng.uniform(.05, .4))
     side = (float(np.sum((4. / 3.) * math.pi * r ** 3)) / phi) ** (1. / 3.)
-    pitch = 2 * rmax + gap
-    nxy = max(1, int(math.floor(side / pitch)))
-    k = np.arange(N)
-    ix, iy, iz = k % nxy, (k // nxy) % nxy, k // (nxy * nxy)
-    jit = (rmax - r)[:, None] * rng.uniform(-1, 1, (N, 2))
-    x0 = .5 * (side - (nxy - 1) * pitch)
-    pos = np.stack([x0 + ix * pitch + jit[:, 0], x0 + iy * pitch + jit[:, 1], rmax + gap + iz * pitch], 1)
+    pos = r[:, None] + rng.uniform(0, 1, (N, 3)) * (side - 2 * r)[:, None]
     a = _soa.spheres_arrays(pos, r, density=density, young=young, ktDivKn=ktDivKn, tanPhi=tanPhi)
-    walls = [(0., 2, 1), (0., 0, 1), (side, 0, -1), (0., 1, 1), (side, 1, -1)]
+    walls = [(0., 0, 1), (side, 0, -1), (0., 1, 1), (side, 1, -1), (0., 2, 1), (side, 2, -1)]
     a = _soa.append_walls(a, walls)
     dt = 0.3 * float(r.min()) / math.sqrt(young / density)
     a.meta.update(dict(kind="deposition_box", dt=dt, gravity=(0., 0., -9.81), damping=damping, law="IdealElPl",
-                       periodic=False, n_spheres=N, seed=seed, side=side, tanPhi=tanPhi))
+                       periodic=False, n_spheres=N, seed=seed, side=side, tanPhi=tanPhi, phi=phi))
     return a
+
+
+def relax_box(a, ctx, iters=300):
+    """Relax the overlaps of a walled scene on the device (walls keep the spheres in)."""
+    sid = ctx.scene_alloc(a.n_par, a.n_node, a.n_mat)
+    try:
+        ctx.upload(sid, a)
+        ctx.set_cell(sid, False, np.eye(3), np.zeros((3, 3)), np.zeros((3, 3)), 4)
+        ctx.relax(sid, iters)
+        st = ctx.download_nodes(sid, a.n_node)
+    finally:
+        ctx.scene_free(sid)
+    b = a.copy()
+    b.pos = st["pos"]
+    b.meta["relaxIters"] = iters
+    return b
 
 
 def drum(seed=0, n=None, radius=.25, length=.1, segments=64, rings=8, rmin=1e-3, rmax=4e-3, omega=None, nmin=1000,
