@@ -110,3 +110,31 @@ def test_clump_flags_without_clumps_rejected():
     o = OL.OracleRun(b, p, cell)
     with pytest.raises(RuntimeError):
         o.step(1)
+
+
+@pytest.mark.gpu
+def test_free_clump_parity_on_gpu(hip_ctx):
+    """A translating two-sphere clump thrown onto a floor wall next to a free sphere:
+    member contacts drive the master (forceTorqueFromMembers) on the GPU as in the oracle."""
+    b = two_sphere_clump()
+    b.vel[4] = [0., 0., -1.5]
+    b.vel[2] = [0., 0., -1.5]
+    p = runner.make_params(dt=1e-5, damping=.2, gravity=(0, 0, -9.81), trackEnergy=True)
+    cell = dict(periodic=False, hSize=np.eye(3), gradV=np.zeros((3, 3)), nextGradV=np.zeros((3, 3)), homoDeform=4)
+    g = runner.HipRun(b, p, cell, ctx=hip_ctx)
+    o = OL.OracleRun(b, p, cell)
+    for k in (1, 2999, 1000):
+        g.step(k)
+        o.step(k)
+        ng, no = g.nodes(), o.nodes()
+        assert np.array_equal(OL.pair_keys(g.contacts()), OL.pair_keys(o.contacts()))
+        for f in ("pos", "vel", "angVel", "force", "torque"):
+            sc = max(float(np.abs(no[f]).max()), 1e-300)
+            assert float(np.abs(ng[f] - no[f]).max()) / sc <= 1e-9, (k, f)
+    # the clump and the free sphere bounced off the floor
+    assert no["vel"][4][2] > 0 and no["vel"][2][2] > 0 and no["vel"][0][2] > 0
+    vg, cg = g.energy()
+    vo, co = o.energy()
+    assert np.array_equal(cg, co)
+    assert np.allclose(vg, vo, rtol=1e-9, atol=1e-12 * np.abs(vo).max())
+    g.close()

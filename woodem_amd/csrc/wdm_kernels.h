@@ -98,6 +98,7 @@ struct ClumpView {
     const GP double4* relPos;   // (x, y, z, 0)
     const GP double4* relOri;   // (w, x, y, z)
     const GP int32_t* nodeClump;  // clump index of a master node, -1 otherwise
+    GP double4* ft;               // per clump: master force, torque with the members' sums (2 records)
     int nClumps, nMembers;
 };
 struct PartView {
@@ -335,24 +336,20 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(const SceneDesc* D, cons
             double4 vm = n.velm[i], aw = n.angI[i];
             D3 vel = xyz(vm), angVel = xyz(aw);
             const double mass = vm.w, inertia = aw.w;
-            D3 f, t;
-            if (GATHER) gatherForces(a, n, p, gv, i, fl, mass, pos, f, t);
-            else { f = xyz(n.force[i]); t = xyz(n.torque[i]); }
             D3 g = mk3(a.g[0], a.g[1], a.g[2]);
             bool damp = (damping != 0. && !(fl & F_DAMPING_SKIP));
             bool blockedNone = (fl & F_DOF_ALL) == 0, blockedAll = (fl & F_DOF_ALL) == F_DOF_ALL;
-            // ClumpData::forceTorqueFromMembers, pkg/dem/Clump.cpp:265-273; skipped when nothing
-            // uses the sums (all DOFs blocked and no damping dissipation tracked)
-            if ((fl & F_CLUMP) && (!blockedAll || (a.reallyTrackEnergy && damp && !(fl & F_ENERGY_SKIP)))) {
-                const int c = S.cl.nodeClump[i];
-                for (int k = S.cl.off[c]; k < S.cl.off[c + 1]; k++) {
-                    const int mn = S.cl.mem[k];
-                    D3 fm, tm;
-                    if (GATHER) gatherForces(a, n, p, gv, mn, n.flags[mn], 0., xyz(n.posr[mn]), fm, tm);
-                    else { fm = xyz(n.force[mn]); tm = xyz(n.torque[mn]); }
-                    f = f + fm;
-                    t = t + (tm + cross3(xyz(n.posr[mn]) - pos, fm));
-                }
+            // force and torque are read by the accelerations and by the damping dissipation
+            // only; a node with every DOF blocked (a wall) skips its contact sum otherwise
+            const bool needFT = !blockedAll || (a.reallyTrackEnergy && damp && !(fl & F_ENERGY_SKIP));
+            D3 f = mk3(0., 0., 0.), t = mk3(0., 0., 0.);
+            if (needFT) {
+                if (fl & F_CLUMP) {  // summed with the members by k_clump_forces
+                    const int c = S.cl.nodeClump[i];
+                    f = xyz(S.cl.ft[2 * c]);
+                    t = xyz(S.cl.ft[2 * c + 1]);
+                } else if (GATHER) gatherForces(a, n, p, gv, i, fl, mass, pos, f, t);
+                else { f = xyz(n.force[i]); t = xyz(n.torque[i]); }
             }
             if (a.reallyTrackEnergy) {
                 if (damp && !(fl & F_ENERGY_SKIP)) {
@@ -459,6 +456,44 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(const SceneDesc* D, cons
         }
     }
     blockSumWrite<3>(ev, bits, a.reallyTrackEnergy ? epart : nullptr, S.words, a.stamp, S.words + 1);
+}
+
+// ClumpData::forceTorqueFromMembers, pkg/dem/Clump.cpp:265-273, one thread per clump before
+// Leapfrog: the master's own force and torque, then each member's added in member order;
+// only for masters whose Leapfrog reads them (free DOFs, or damping dissipation tracked)
+__global__ void __launch_bounds__(WDM_BLOCK) k_clump_forces(const SceneDesc* D, const int32_t* idx, Map m) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const SceneDesc& S = D[idx ? idx[t] : t];
+    const StepArgs a = S.a;
+    const NodeView n = S.n;
+    const PartView p = S.p;
+    const GatherView gv = S.gv;
+    const ClumpView cl = S.cl;
+    const int c = (int)(lb * WDM_BLOCK + threadIdx.x);
+    if (c >= cl.nClumps) return;
+    int M = -1;
+    for (int i = cl.off[c]; i < cl.off[c + 1] && M < 0; i++) M = n.master[cl.mem[i]];
+    if (M < 0) return;  // clump without members: nothing to add
+    const uint32_t fl = n.flags[M];
+    const bool damp = a.damping != 0. && !(fl & F_DAMPING_SKIP);
+    const bool blockedAll = (fl & F_DOF_ALL) == F_DOF_ALL;
+    if (!(!blockedAll || (a.reallyTrackEnergy && damp && !(fl & F_ENERGY_SKIP)))) return;
+    const D3 P = xyz(n.posr[M]);
+    D3 f, tq;
+    if (a.gatherOn) gatherForces(a, n, p, gv, M, fl, n.velm[M].w, P, f, tq);
+    else { f = xyz(n.force[M]); tq = xyz(n.torque[M]); }
+    for (int k = cl.off[c]; k < cl.off[c + 1]; k++) {
+        const int mn = cl.mem[k];
+        const D3 xm = xyz(n.posr[mn]);
+        D3 fm, tm;
+        if (a.gatherOn) gatherForces(a, n, p, gv, mn, n.flags[mn], 0., xm, fm, tm);
+        else { fm = xyz(n.force[mn]); tm = xyz(n.torque[mn]); }
+        f = f + fm;
+        tq = tq + (tm + cross3(xm - P, fm));
+    }
+    cl.ft[2 * c] = make_double4(f.x, f.y, f.z, 0.);
+    cl.ft[2 * c + 1] = make_double4(tq.x, tq.y, tq.z, 0.);
 }
 
 // quaternions are stored (w, x, y, z) in double4 (x, y, z, w); products and
@@ -1064,21 +1099,30 @@ __global__ void k_set_counters(const ConDesc<T>* C, const FRScene* FR, int n, co
 }
 
 // CSR entries of the new lists: key (t, particle, other), value contact index | isB << 31
+// Dormant contacts cannot become real before the next full run and dead ones never again:
+// their entries get the key (t, np, 0), behind every particle of the scene, so the CSR
+// offsets leave them out of every particle's range.
 template <typename T>
-__global__ void __launch_bounds__(WDM_BLOCK) k_csr_entries(const ConDesc<T>* C, const FRScene* FR, Map m, uint64_t* ek, uint32_t* ev,
-                                                          int idBits) {
+__global__ void __launch_bounds__(WDM_BLOCK) k_csr_entries(const SceneDesc* D, const ConDesc<T>* C, const FRScene* FR, Map m, uint64_t* ek,
+                                                          uint32_t* ev, int idBits) {
     int64_t lb;
     const int t = mapBlock(m, lb);
     const FRScene& F = FR[t];
-    const int2* ids = C[t].cvNew.ids;
+    const ConView<T> nc = C[t].cvNew;
     int64_t c = lb * WDM_BLOCK + threadIdx.x;
     if (c >= F.nNew) return;
     const uint64_t tkey = (uint64_t)t << (2 * idBits);
-    int2 id = ids[c];
-    int64_t g = F.newStart + c;
-    ek[2 * g] = tkey | ((uint64_t)(uint32_t)id.x << idBits) | (uint32_t)id.y;
+    const int2 id = nc.ids[c];
+    const int64_t g = F.newStart + c;
+    if (nc.info[c] & (CI_DORMANT | CI_DEAD)) {
+        const uint64_t off = tkey | ((uint64_t)(uint32_t)D[F.desc].p.np << idBits);
+        ek[2 * g] = off;
+        ek[2 * g + 1] = off;
+    } else {
+        ek[2 * g] = tkey | ((uint64_t)(uint32_t)id.x << idBits) | (uint32_t)id.y;
+        ek[2 * g + 1] = tkey | ((uint64_t)(uint32_t)id.y << idBits) | (uint32_t)id.x;
+    }
     ev[2 * g] = (uint32_t)c;
-    ek[2 * g + 1] = tkey | ((uint64_t)(uint32_t)id.y << idBits) | (uint32_t)id.x;
     ev[2 * g + 1] = (uint32_t)c | 0x80000000u;
 }
 // copy each scene's sorted entry values into its own CSR array
@@ -1128,7 +1172,12 @@ __device__ __forceinline__ void initialLocalCoords(const V3T<T>& locX, T* tr) {
 template <typename T> __device__ __forceinline__ V3T<T> trCol(const T* tr, int j) { return mk3<T>(tr[j], tr[3 + j], tr[6 + j]); }
 template <typename T> __device__ __forceinline__ void trSetCol(T* tr, int j, const V3T<T>& c) { tr[j] = c.x; tr[3 + j] = c.y; tr[6 + j] = c.z; }
 
-template <typename T, int LAW>
+// GEOM: the Cg2 functors compiled in (G_SS | G_WS | G_FS), so that a scene of spheres only
+// does not carry the registers of the facet geometry
+#define G_SS 1
+#define G_WS 2
+#define G_FS 4
+template <typename T, int LAW, int GEOM>
 __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, const ConDesc<T>* C, Map m, double* epart, int spec) {
     int64_t lb;
     const int tt = mapBlock(m, lb);
@@ -1172,7 +1221,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
         int32_t row = real ? cv.slot[c] : -1;
         if (real) for (int k = 0; k < 9; k++) tr[k] = cv.st[(size_t)(S_T + k) * cv.cap + row];
         Geo<T> G;
-        if (ty == CI_SS) {  // Cg2_Sphere_Sphere_L6Geom::go, pkg/dem/Sphere.cpp:88-109
+        if ((GEOM & G_SS) && ty == CI_SS) {  // Cg2_Sphere_Sphere_L6Geom::go, pkg/dem/Sphere.cpp:88-109
             D3 p2 = xyz(pb) + shift2;
             D3 relPos = p2 - xyz(pa);
             double r1 = pa.w, r2 = pb.w;
@@ -1188,7 +1237,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
             G.pos1 = xyz(pa); G.pos2 = p2;
             double4 va = n.velm[id.x], vb = n.velm[id.y], wa = n.angI[id.x], wb = n.angI[id.y];
             G.vel1 = xyz(va); G.vel2 = xyz(vb); G.angVel1 = xyz(wa); G.angVel2 = xyz(wb);
-        } else if (ty == CI_WS) {  // Cg2_Wall_Sphere_L6Geom::go, pkg/dem/Wall.cpp:45-79
+        } else if ((GEOM & G_WS) && ty == CI_WS) {  // Cg2_Wall_Sphere_L6Geom::go, pkg/dem/Wall.cpp:45-79
             int32_t wt = p.ptype[id.x];
             int ax = ptAxis(wt), sn = ptSense(wt);
             double rad = pb.w;
@@ -1210,6 +1259,8 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
             G.pos1 = wallPos; G.pos2 = spherePos;
             double4 va = n.velm[id.x], vb = n.velm[id.y], wa = n.angI[id.x], wb = n.angI[id.y];
             G.vel1 = xyz(va); G.vel2 = xyz(vb); G.angVel1 = xyz(wa); G.angVel2 = xyz(wb);
+        } else if (!(GEOM & G_FS)) {
+            break;  // not in this scene (the host picks GEOM from the scene's shapes)
         } else {  // Cg2_Facet_Sphere_L6Geom::go, pkg/dem/Facet.cpp:235-288
             int4 fn = p.fnodes[id.x];
             double halfThick = pa.w, rad = pb.w;

@@ -221,7 +221,7 @@ struct Shared {
     std::vector<int32_t> freeSlots;
     int32_t nslots = 0;
     Staged stDesc, stMapNode, stMapCon, stCon, stFR, stConFR, stIdx;
-    Staged stMapP, stMapB, stMapC, stMapO, stMapN, stMapE, stMapP1, stMapCl;
+    Staged stMapP, stMapB, stMapC, stMapO, stMapN, stMapE, stMapP1, stMapCl, stMapClF;
     hipEvent_t evWords = nullptr;  // recorded after the copy of the words to the host
     // energy partials per block
     DBuf<double> epartL, epartC;
@@ -241,7 +241,7 @@ struct Shared {
         hostWords = nullptr;
         hostWordsCap = 0;
         for (Staged* s : {&stDesc, &stMapNode, &stMapCon, &stCon, &stFR, &stConFR, &stIdx, &stMapP, &stMapB, &stMapC, &stMapO, &stMapN,
-                          &stMapE, &stMapP1, &stMapCl})
+                          &stMapE, &stMapP1, &stMapCl, &stMapClF})
             s->free(c);
         release(c, epartL); release(c, epartC); release(c, extPart); release(c, extOut);
         release(c, ckeys); release(c, ckeys2); release(c, cvals); release(c, cvals2); release(c, cstart); release(c, cend);
@@ -266,6 +266,7 @@ struct Scene {
     std::vector<double> radiusH;
     std::vector<int32_t> binnedH, unbH;
     std::vector<uint32_t> flagsH;
+    bool hasFacets = false;
     // cell (core/Cell.hpp)
     bool periodic = false;
     HM3 hSize = heye(), gradV = hzero(), nextGradV = hzero(), pprevHsize = heye(), trsf = heye();
@@ -304,7 +305,7 @@ struct Scene {
     int64_t clumpFlagged = 0;  // nodes flagged CLUMP or CLUMPED at upload
     bool clumpsSet = true;     // flags and uploaded clumps agree
     DBuf<int32_t> master, nodeClump, cOff, cMem;
-    DBuf<double4> cRelPos, cRelOri;
+    DBuf<double4> cRelPos, cRelOri, cFT;
 
     Scene(Ctx* c, int32_t np_, int32_t nn_, int32_t nm_) : ctx(c), np(np_), nn(nn_), nmat(nm_) {}
 
@@ -332,7 +333,7 @@ struct Scene {
         release(ctx, unb); release(ctx, pmask); release(ctx, fnodes); release(ctx, bmin); release(ctx, bmax); release(ctx, mat);
         for (int i = 0; i < 2; i++) { cb64[i].free(ctx); cb32[i].free(ctx); }
         release(ctx, csrOff); release(ctx, entV); release(ctx, realMask); release(ctx, E);
-        release(ctx, master); release(ctx, nodeClump); release(ctx, cOff); release(ctx, cMem); release(ctx, cRelPos); release(ctx, cRelOri);
+        release(ctx, master); release(ctx, nodeClump); release(ctx, cOff); release(ctx, cMem); release(ctx, cRelPos); release(ctx, cRelOri); release(ctx, cFT);
     }
 };
 
@@ -523,6 +524,7 @@ SceneDesc makeDesc(Scene& S) {
         d.cl.relPos = gpc((const double4*)S.cRelPos.p);
         d.cl.relOri = gpc((const double4*)S.cRelOri.p);
         d.cl.nodeClump = gpc((const int32_t*)S.nodeClump.p);
+        d.cl.ft = gpc(S.cFT.p);
         d.cl.nClumps = S.nClumps;
         d.cl.nMembers = S.nMembers;
     }
@@ -600,6 +602,23 @@ struct Leapfrog {
             idx[S->forcesFromContacts ? 1 : 0].push_back(i);
         }
         B.upload();
+        // clump masters' force and torque with their members' (read by the Leapfrog below)
+        {
+            std::vector<int32_t> cidx;
+            for (int i = 0; i < n; i++)
+                if (B.ss[i]->nClumps > 0) cidx.push_back(i);
+            if (!cidx.empty()) {
+                std::vector<int64_t> cbp = blockPrefix((int)cidx.size(), [&](int t) { return B.ss[cidx[t]]->nClumps; });
+                Blob cb;
+                size_t ci = cb.addv(cidx), cbo = cb.addv(cbp);
+                uint8_t* cbase = X.stMapClF.upload(c, cb.b.data(), cb.b.size());
+                Map cm;
+                cm.bprefix = (const int64_t*)(cbase + cbo);
+                cm.n = (int)cidx.size();
+                hipLaunchKernelGGL(k_clump_forces, dim3((unsigned)cbp.back()), dim3(WDM_BLOCK), 0, c->stream, B.D, (const int32_t*)(cbase + ci), cm);
+                checkLaunch();
+            }
+        }
         Blob b;
         size_t oi[2], ob[2];
         std::vector<int64_t> bp[2];
@@ -887,7 +906,7 @@ struct InsertionSortCollider {
             ensure(c, X.entK, std::max<int64_t>(ne, 1)); ensure(c, X.entK2, std::max<int64_t>(ne, 1));
             ensure(c, X.entV, std::max<int64_t>(ne, 1)); ensure(c, X.entV2, std::max<int64_t>(ne, 1));
             if (nNewTot > 0) {
-                hipLaunchKernelGGL(k_csr_entries<T>, dim3((unsigned)bpN[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mN, X.entK.p, X.entV.p, idBits);
+                hipLaunchKernelGGL(k_csr_entries<T>, dim3((unsigned)bpN[nT]), dim3(WDM_BLOCK), 0, st, B.D, Cd, FRd, mN, X.entK.p, X.entV.p, idBits);
                 checkLaunch();
                 sortPairs(c, X.entK.p, X.entK2.p, X.entV.p, X.entV2.p, ne, keyBits);
                 std::vector<int64_t> bpE = blockPrefix(nT, [&](int t) { return 2 * F[t].nNew; });
@@ -939,8 +958,13 @@ ConBuf<float>* InsertionSortCollider::bufs<float>(Scene& S) { return S.cb32; }
 struct ContactLoop {
     // spec: launched before the collider's decision is known; a scene whose Verlet trigger
     // fired skips it and is launched again after its full run
+    template <typename T, int LAW, int GEOM>
+    static void launch(Batch& B, const ConDesc<T>* Cd, const Map& m, int64_t blocks, bool spec) {
+        hipLaunchKernelGGL((k_contacts<T, LAW, GEOM>), dim3((unsigned)blocks), dim3(WDM_BLOCK), 0, B.c->stream, B.D, Cd, m,
+                           B.c->sh->epartC.p, spec ? 1 : 0);
+    }
     template <typename T>
-    static void runGroup(Batch& B, const std::vector<int>& L, int law, bool spec) {
+    static void runGroup(Batch& B, const std::vector<int>& L, int law, int geom, bool spec) {
         Ctx* c = B.c;
         Shared& X = *c->sh;
         const int n = (int)L.size();
@@ -970,10 +994,10 @@ struct ContactLoop {
         ensure(c, X.epartC, (size_t)bp[n] * 5 + 5);
         if (bp[n] > 0) {
             KTimer tm(c, "contacts");
-            if (law == WDM_LAW_HERTZ_DMT)
-                hipLaunchKernelGGL((k_contacts<T, 1>), dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, c->stream, B.D, Cd, m, X.epartC.p, spec ? 1 : 0);
-            else
-                hipLaunchKernelGGL((k_contacts<T, 0>), dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, c->stream, B.D, Cd, m, X.epartC.p, spec ? 1 : 0);
+            const bool h = law == WDM_LAW_HERTZ_DMT;
+            if (geom == G_SS) { if (h) launch<T, 1, G_SS>(B, Cd, m, bp[n], spec); else launch<T, 0, G_SS>(B, Cd, m, bp[n], spec); }
+            else if (geom == (G_SS | G_WS)) { if (h) launch<T, 1, G_SS | G_WS>(B, Cd, m, bp[n], spec); else launch<T, 0, G_SS | G_WS>(B, Cd, m, bp[n], spec); }
+            else { if (h) launch<T, 1, G_SS | G_WS | G_FS>(B, Cd, m, bp[n], spec); else launch<T, 0, G_SS | G_WS | G_FS>(B, Cd, m, bp[n], spec); }
             checkLaunch();
         }
         if (anyEnergy) {
@@ -984,18 +1008,21 @@ struct ContactLoop {
         }
     }
     static void runList(Batch& B, const std::vector<int>& L, bool spec) {
-        std::vector<int> g[2][2];  // [mixed][law]
+        std::vector<int> g[2][2][3];  // [mixed][law][geometry: spheres, + walls, + facets]
         for (int i : L) {
             Scene& S = *B.ss[i];
             if (S.needFull) throw WdmError(WDM_E_STATE, "ContactLoop before the collider's first run");
-            g[S.mixed() ? 1 : 0][S.P.law == WDM_LAW_HERTZ_DMT ? 1 : 0].push_back(i);
+            g[S.mixed() ? 1 : 0][S.P.law == WDM_LAW_HERTZ_DMT ? 1 : 0][S.hasFacets ? 2 : (S.unbH.empty() ? 0 : 1)].push_back(i);
         }
         if (L.empty()) return;
         B.ensureDesc();
-        runGroup<double>(B, g[0][0], WDM_LAW_IDEALELPL, spec);
-        runGroup<double>(B, g[0][1], WDM_LAW_HERTZ_DMT, spec);
-        runGroup<float>(B, g[1][0], WDM_LAW_IDEALELPL, spec);
-        runGroup<float>(B, g[1][1], WDM_LAW_HERTZ_DMT, spec);
+        const int geoms[3] = {G_SS, G_SS | G_WS, G_SS | G_WS | G_FS};
+        for (int q = 0; q < 3; q++) {
+            runGroup<double>(B, g[0][0][q], WDM_LAW_IDEALELPL, geoms[q], spec);
+            runGroup<double>(B, g[0][1][q], WDM_LAW_HERTZ_DMT, geoms[q], spec);
+            runGroup<float>(B, g[1][0][q], WDM_LAW_IDEALELPL, geoms[q], spec);
+            runGroup<float>(B, g[1][1][q], WDM_LAW_HERTZ_DMT, geoms[q], spec);
+        }
     }
     static void done(Batch& B) {
         for (Scene* S : B.ss) {
@@ -1360,6 +1387,8 @@ int wdm_upload_particles(wdm_ctx* c, int32_t sid, int32_t n, const int32_t* shap
         }
         S.ptypeH = pt;
         S.radiusH.assign(radius, radius + n);
+        S.hasFacets = false;
+        for (int p = 0; p < n; p++) S.hasFacets |= shape[p] == WDM_SHAPE_FACET;
         hipStream_t st = c->stream;
         HIPCHK(hipMemcpyAsync(S.ptype.p, pt.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(S.pmask.p, mask, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
@@ -1434,6 +1463,7 @@ int wdm_upload_clumps(wdm_ctx* c, int32_t sid, int32_t nc, const int32_t* master
         if (nc > 0) {
             ensure(c, S.master, nn); ensure(c, S.nodeClump, nn); ensure(c, S.cOff, (size_t)nc + 1);
             ensure(c, S.cMem, std::max(nm, 1)); ensure(c, S.cRelPos, std::max(nm, 1)); ensure(c, S.cRelOri, std::max(nm, 1));
+            ensure(c, S.cFT, 2 * (size_t)nc);
             HIPCHK(hipMemcpyAsync(S.master.p, nodeMaster.data(), nn * sizeof(int32_t), hipMemcpyHostToDevice, st));
             HIPCHK(hipMemcpyAsync(S.nodeClump.p, nodeClump.data(), nn * sizeof(int32_t), hipMemcpyHostToDevice, st));
             HIPCHK(hipMemcpyAsync(S.cOff.p, off, ((size_t)nc + 1) * sizeof(int32_t), hipMemcpyHostToDevice, st));
