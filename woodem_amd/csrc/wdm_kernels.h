@@ -210,6 +210,9 @@ struct FRScene {
     int64_t cellBase, binStart, candStart, nCand, oldStart, nOld, newStart, nNew;
     GP int64_t* csrOff;
     GP uint32_t* entV;
+    double wLarge;  // bounds wider than this (facets larger than every sphere) skip the grid cells
+    int32_t nCells;      // grid cells of the scene; cell nCells lists its large bounds
+    int32_t largePairs;  // large bounds may pair with each other (masks allow it)
     int32_t desc, pad;
 };
 
@@ -643,51 +646,52 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_bounds(const SceneDesc* D, const 
 
 // extent reduction over binned (finite-bound) particles: min corner, max corner, max extent
 __global__ void __launch_bounds__(WDM_BLOCK) k_extent(const SceneDesc* D, const FRScene* FR, Map m, double* part) {
-    __shared__ double sh[7][WDM_BLOCK];
+    __shared__ double sh[8][WDM_BLOCK];
     int64_t lb;
     const SceneDesc& S = D[FR[mapBlock(m, lb)].desc];
     const PartView p = S.p;
     int t = threadIdx.x;
     int i = (int)(lb * WDM_BLOCK + t);
     const double inf = __builtin_inf();
-    double v[7] = {inf, inf, inf, -inf, -inf, -inf, 0.};
+    double v[8] = {inf, inf, inf, -inf, -inf, -inf, 0., 0.};
     if (i < S.nbinned) {
         int id = S.binned[i];
         double4 a = p.bmin[id], b = p.bmax[id];
         v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = b.x; v[4] = b.y; v[5] = b.z;
         v[6] = fmax(fmax(b.x - a.x, b.y - a.y), b.z - a.z);
+        if (ptShape(p.ptype[id]) == 1) v[7] = v[6];
     }
-    for (int k = 0; k < 7; k++) sh[k][t] = v[k];
+    for (int k = 0; k < 8; k++) sh[k][t] = v[k];
     __syncthreads();
     for (int s = WDM_BLOCK / 2; s > 0; s >>= 1) {
         if (t < s) {
             for (int k = 0; k < 3; k++) sh[k][t] = fmin(sh[k][t], sh[k][t + s]);
-            for (int k = 3; k < 7; k++) sh[k][t] = fmax(sh[k][t], sh[k][t + s]);
+            for (int k = 3; k < 8; k++) sh[k][t] = fmax(sh[k][t], sh[k][t + s]);
         }
         __syncthreads();
     }
-    if (t == 0) for (int k = 0; k < 7; k++) part[(size_t)blockIdx.x * 7 + k] = sh[k][0];
+    if (t == 0) for (int k = 0; k < 8; k++) part[(size_t)blockIdx.x * 8 + k] = sh[k][0];
 }
-// per scene (one block each): min corner, max corner, max extent over its blocks
+// per scene (one block each): min corner, max corner, max extent, max sphere extent over its blocks
 __global__ void __launch_bounds__(WDM_BLOCK) k_extent_finalize(Map m, const double* part, double* out) {
-    __shared__ double sh[7][WDM_BLOCK];
+    __shared__ double sh[8][WDM_BLOCK];
     const int t = blockIdx.x, tid = threadIdx.x;
     const double inf = __builtin_inf();
-    double v[7] = {inf, inf, inf, -inf, -inf, -inf, 0.};
+    double v[8] = {inf, inf, inf, -inf, -inf, -inf, 0., 0.};
     for (int64_t b = m.bprefix[t] + tid; b < m.bprefix[t + 1]; b += WDM_BLOCK) {
-        for (int k = 0; k < 3; k++) v[k] = fmin(v[k], part[(size_t)b * 7 + k]);
-        for (int k = 3; k < 7; k++) v[k] = fmax(v[k], part[(size_t)b * 7 + k]);
+        for (int k = 0; k < 3; k++) v[k] = fmin(v[k], part[(size_t)b * 8 + k]);
+        for (int k = 3; k < 8; k++) v[k] = fmax(v[k], part[(size_t)b * 8 + k]);
     }
-    for (int k = 0; k < 7; k++) sh[k][tid] = v[k];
+    for (int k = 0; k < 8; k++) sh[k][tid] = v[k];
     __syncthreads();
     for (int w = WDM_BLOCK / 2; w > 0; w >>= 1) {
         if (tid < w) {
             for (int k = 0; k < 3; k++) sh[k][tid] = fmin(sh[k][tid], sh[k][tid + w]);
-            for (int k = 3; k < 7; k++) sh[k][tid] = fmax(sh[k][tid], sh[k][tid + w]);
+            for (int k = 3; k < 8; k++) sh[k][tid] = fmax(sh[k][tid], sh[k][tid + w]);
         }
         __syncthreads();
     }
-    if (tid < 7) out[(size_t)t * 7 + tid] = sh[tid][0];
+    if (tid < 8) out[(size_t)t * 8 + tid] = sh[tid][0];
 }
 
 __device__ __forceinline__ int cellCoord(const GridArgs& g, int d, double x) {
@@ -712,9 +716,14 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_cell_keys(const SceneDesc* D, con
     if (i >= S.nbinned) return;
     const GridArgs g = F.g;  // by value: the loops below store to global memory
     int id = S.binned[i];
-    double4 a = S.p.bmin[id];
-    int cx = cellCoord(g, 0, a.x), cy = cellCoord(g, 1, a.y), cz = cellCoord(g, 2, a.z);
-    keys[F.binStart + i] = (uint32_t)(F.cellBase + (cz * g.n[1] + cy) * g.n[0] + cx);
+    double4 a = S.p.bmin[id], b = S.p.bmax[id];
+    int64_t cell;
+    if (fmax(fmax(b.x - a.x, b.y - a.y), b.z - a.z) > F.wLarge) cell = F.nCells;  // large: listed apart
+    else {
+        int cx = cellCoord(g, 0, a.x), cy = cellCoord(g, 1, a.y), cz = cellCoord(g, 2, a.z);
+        cell = (cz * g.n[1] + cy) * g.n[0] + cx;
+    }
+    keys[F.binStart + i] = (uint32_t)(F.cellBase + cell);
     vals[F.binStart + i] = id;
 }
 
@@ -795,10 +804,14 @@ __device__ __forceinline__ bool mayCollideD(const PartView& p, const GP int32_t*
     return true;
 }
 
-// Candidate pairs.  Thread i takes the i-th binned particle in cell order and visits
-// its own cell (later entries only) and the 13 "forward" neighbour cells, so every
-// unordered pair of binned particles is visited once; it also tests every unbinned
-// particle (walls).  A conservative minimum-image test on bound centres (periodic
+// Candidate pairs.  Thread i takes the i-th bound in (scene, cell) order.  A bound that
+// fits a grid cell visits its own cell (later entries only) and the 13 "forward"
+// neighbour cells, so every unordered pair of such bounds is visited once, and it tests
+// every unbinned particle (walls).  A large bound (wider than a cell: a facet bigger than
+// every sphere) is kept out of the cells; it is listed instead, by k_large_cells, in every
+// cell its bound covers plus margin (a small bound overlapping it starts at most one cell
+// before it), and a small bound tests the large ones listed in its own cell.  Large
+// bounds pair with each other by a direct loop when their masks allow it.  A conservative minimum-image test on bound centres (periodic
 // cells) rejects most candidates cheaply; the decision itself is the collider's own
 // closed-interval 3-axis overlap of the stored bounds plus mayCollide, so the set
 // equals the reference's and the grid only prunes.
@@ -821,10 +834,69 @@ __device__ __forceinline__ bool periPrefilter(const double4& a0, const double4& 
 
 // Candidate keys of a batch: (t << 2 idBits) | (min id << idBits) | max id, t the scene's
 // index in the full run, so one sort orders them by scene, then like the contact keys.
+// cell index along an axis without wrapping or clamping (large bounds' cell ranges)
+__device__ __forceinline__ int cellCoordRaw(const GridArgs& g, int d, double x) {
+    if (g.periodic) return (int)floor(x / g.L[d] * g.n[d]);
+    return (int)floor((x - g.origin[d]) / g.w[d]);
+}
+
+// cells listing a large bound: those its bound covers, two more at the low side and one at
+// the high side (a small bound starts at most one cell before it; one cell of rounding
+// margin each way), each cell once in a periodic grid
+__device__ __forceinline__ void largeRange(const GridArgs& g, const double4& a0, const double4& a1, int c0[3], int nc[3]) {
+    const double lo3[3] = {a0.x, a0.y, a0.z}, hi3[3] = {a1.x, a1.y, a1.z};
+    for (int d = 0; d < 3; d++) {
+        int l = cellCoordRaw(g, d, lo3[d]) - 2, h = cellCoordRaw(g, d, hi3[d]) + 1;
+        if (!g.periodic) { l = l < 0 ? 0 : l; h = h > g.n[d] - 1 ? g.n[d] - 1 : h; }
+        c0[d] = l;
+        nc[d] = h >= l ? h - l + 1 : 0;
+        if (g.periodic && nc[d] > g.n[d]) { nc[d] = g.n[d]; c0[d] = 0; }
+    }
+}
+// MODE 0: number of cells listing each large bound (0 for the others); MODE 1: the
+// entries (scene's first cell + cell, particle id) at offs[i]
+template <int MODE>
+__global__ void __launch_bounds__(WDM_BLOCK) k_large_cells(const SceneDesc* D, const FRScene* FR, Map m, const uint32_t* ckeys,
+                                                          const int32_t* cvals, int32_t* counts, const int64_t* offs, uint32_t* lkeys,
+                                                          int32_t* lvals) {
+    int64_t lb;
+    const FRScene& F = FR[mapBlock(m, lb)];
+    const SceneDesc& S = D[F.desc];
+    const GridArgs g = F.g;
+    const int li = (int)(lb * WDM_BLOCK + threadIdx.x);
+    if (li >= S.nbinned) return;
+    const int64_t i = F.binStart + li;
+    if ((int)(ckeys[i] - F.cellBase) != F.nCells) {
+        if (!MODE) counts[i] = 0;
+        return;
+    }
+    const int id = cvals[i];
+    int c0[3], nc[3];
+    largeRange(g, S.p.bmin[id], S.p.bmax[id], c0, nc);
+    if (!MODE) {
+        counts[i] = nc[0] * nc[1] * nc[2];
+        return;
+    }
+    int64_t o = offs[i];
+    for (int zz = 0; zz < nc[2]; zz++) {
+        const int z = g.periodic ? (((c0[2] + zz) % g.n[2]) + g.n[2]) % g.n[2] : c0[2] + zz;
+        for (int yy = 0; yy < nc[1]; yy++) {
+            const int y = g.periodic ? (((c0[1] + yy) % g.n[1]) + g.n[1]) % g.n[1] : c0[1] + yy;
+            for (int xx = 0; xx < nc[0]; xx++) {
+                const int x = g.periodic ? (((c0[0] + xx) % g.n[0]) + g.n[0]) % g.n[0] : c0[0] + xx;
+                lkeys[o] = (uint32_t)(F.cellBase + (z * g.n[1] + y) * g.n[0] + x);
+                lvals[o] = id;
+                o++;
+            }
+        }
+    }
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(WDM_BLOCK) k_pairs(const SceneDesc* D, const FRScene* FR, Map m, const uint32_t* ckeys, const int32_t* cvals,
-                                                    const int32_t* cstart, const int32_t* cend, int32_t* counts, const int64_t* offs,
-                                                    uint64_t* okeys, int32_t* operiods, int idBits) {
+                                                    const int32_t* cstart, const int32_t* cend, const int32_t* lstart, const int32_t* lend,
+                                                    const int32_t* lvals, int32_t* counts, const int64_t* offs, uint64_t* okeys,
+                                                    int32_t* operiods, int idBits) {
     int64_t lb;
     const int tt = mapBlock(m, lb);
     const FRScene& F = FR[tt];
@@ -836,70 +908,75 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pairs(const SceneDesc* D, const F
     const int* unb = S.unb;
     const int nunb = S.nunb;
     const int64_t cellBase = F.cellBase;
+    const int nCells = F.nCells;
     int li = (int)(lb * WDM_BLOCK + threadIdx.x);
     if (li >= S.nbinned) return;
     const int64_t i = F.binStart + li;
     const uint64_t tkey = (uint64_t)tt << (2 * idBits);
-    int id = cvals[i];
-    uint32_t ck = (uint32_t)(ckeys[i] - cellBase);
-    int cx = ck % g.n[0], cy = (ck / g.n[0]) % g.n[1], cz = ck / (g.n[0] * g.n[1]);
-    double4 a0 = p.bmin[id], a1 = p.bmax[id];
-    int sa = ptShape(p.ptype[id]);
+    const int id = cvals[i];
+    const uint32_t ck = (uint32_t)(ckeys[i] - cellBase);
+    const double4 a0 = p.bmin[id], a1 = p.bmax[id];
+    const int sa = ptShape(p.ptype[id]);
     int cnt = 0;
-    int64_t o = MODE ? offs[i] : 0;
-    // periodic axes with fewer than 3 cells hold a single cell (no neighbours); aperiodic axes always scan +-1
-    int rx = (!g.periodic || g.n[0] >= 3) ? 1 : 0, ry = (!g.periodic || g.n[1] >= 3) ? 1 : 0, rz = (!g.periodic || g.n[2] >= 3) ? 1 : 0;
-    for (int dz = 0; dz <= rz; dz++) {
-        for (int dy = (dz == 0 ? 0 : -ry); dy <= ry; dy++) {
-            for (int dx = ((dz == 0 && dy == 0) ? 0 : -rx); dx <= rx; dx++) {
-                int z = cz + dz, y = cy + dy, x = cx + dx;
-                if (g.periodic) {
-                    z = (z + g.n[2]) % g.n[2]; y = (y + g.n[1]) % g.n[1]; x = (x + g.n[0]) % g.n[0];
-                } else if (z < 0 || z >= g.n[2] || y < 0 || y >= g.n[1] || x < 0 || x >= g.n[0]) continue;
-                int64_t cell = cellBase + (z * g.n[1] + y) * g.n[0] + x;
-                int s = cstart[cell];
-                if (s < 0) continue;
-                int e = cend[cell];
-                if (dx == 0 && dy == 0 && dz == 0) s = (int)i + 1;  // own cell: later entries only
-                for (int k = s; k < e; k++) {
-                    int j = cvals[k];
-                    double4 b0 = p.bmin[j], b1 = p.bmax[j];
-                    if (g.periodic) { if (!periPrefilter(a0, a1, b0, b1, g)) continue; }
-                    else if (!overlapAperiodic(a0, a1, b0, b1)) continue;
-                    int sb = ptShape(p.ptype[j]);
-                    int A, B, ty;
-                    canonical(id, j, sa, sb, A, B, ty);
-                    const bool aIsId = (A == id);
-                    const double4 &A0 = aIsId ? a0 : b0, &A1 = aIsId ? a1 : b1, &B0 = aIsId ? b0 : a0, &B1 = aIsId ? b1 : a1;
-                    int per[3] = {0, 0, 0};
-                    bool ov = g.periodic ? overlapPeri(A0, A1, B0, B1, g.L, per, err) : true;
-                    if (!ov || !mayCollideD(p, S.n.master, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), loneMask)) continue;
-                    if (MODE) {
-                        int lo = id < j ? id : j, hi = id < j ? j : id;
-                        okeys[o + cnt] = tkey | ((uint64_t)(uint32_t)lo << idBits) | (uint32_t)hi;
-                        operiods[o + cnt] = packCd(per[0], per[1], per[2]);
-                    }
-                    cnt++;
-                }
-            }
+    const int64_t o = MODE ? offs[i] : 0;
+    // one candidate: prefilter, canonical order, exact overlap (with periods), mayCollide
+    auto tryPair = [&](int j, bool binned) {
+        const double4 b0 = p.bmin[j], b1 = p.bmax[j];
+        if (binned) {
+            if (g.periodic) { if (!periPrefilter(a0, a1, b0, b1, g)) return; }
+            else if (!overlapAperiodic(a0, a1, b0, b1)) return;
         }
-    }
-    for (int u = 0; u < nunb; u++) {
-        int j = unb[u];
-        int sb = ptShape(p.ptype[j]);
+        const int sb = ptShape(p.ptype[j]);
         int A, B, ty;
         canonical(id, j, sa, sb, A, B, ty);
-        double4 A0 = p.bmin[A], A1 = p.bmax[A], B0 = p.bmin[B], B1 = p.bmax[B];
+        const bool aIsId = (A == id);
+        const double4 &A0 = aIsId ? a0 : b0, &A1 = aIsId ? a1 : b1, &B0 = aIsId ? b0 : a0, &B1 = aIsId ? b1 : a1;
         int per[3] = {0, 0, 0};
-        bool ov = g.periodic ? overlapPeri(A0, A1, B0, B1, g.L, per, err) : overlapAperiodic(A0, A1, B0, B1);
-        if (!ov || !mayCollideD(p, S.n.master, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), loneMask)) continue;
+        const bool ov = g.periodic ? overlapPeri(A0, A1, B0, B1, g.L, per, err) : (binned || overlapAperiodic(A0, A1, B0, B1));
+        if (!ov || !mayCollideD(p, S.n.master, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), loneMask)) return;
         if (MODE) {
-            int lo = id < j ? id : j, hi = id < j ? j : id;
+            const int lo = id < j ? id : j, hi = id < j ? j : id;
             okeys[o + cnt] = tkey | ((uint64_t)(uint32_t)lo << idBits) | (uint32_t)hi;
             operiods[o + cnt] = packCd(per[0], per[1], per[2]);
         }
         cnt++;
+    };
+    if ((int)ck != nCells) {
+        const int cx = ck % g.n[0], cy = (ck / g.n[0]) % g.n[1], cz = ck / (g.n[0] * g.n[1]);
+        // periodic axes with fewer than 3 cells hold a single cell (no neighbours); aperiodic axes always scan +-1
+        const int rx = (!g.periodic || g.n[0] >= 3) ? 1 : 0, ry = (!g.periodic || g.n[1] >= 3) ? 1 : 0,
+                  rz = (!g.periodic || g.n[2] >= 3) ? 1 : 0;
+        for (int dz = 0; dz <= rz; dz++) {
+            for (int dy = (dz == 0 ? 0 : -ry); dy <= ry; dy++) {
+                for (int dx = ((dz == 0 && dy == 0) ? 0 : -rx); dx <= rx; dx++) {
+                    int z = cz + dz, y = cy + dy, x = cx + dx;
+                    if (g.periodic) {
+                        z = (z + g.n[2]) % g.n[2]; y = (y + g.n[1]) % g.n[1]; x = (x + g.n[0]) % g.n[0];
+                    } else if (z < 0 || z >= g.n[2] || y < 0 || y >= g.n[1] || x < 0 || x >= g.n[0]) continue;
+                    const int64_t cell = cellBase + (z * g.n[1] + y) * g.n[0] + x;
+                    int s = cstart[cell];
+                    if (s < 0) continue;
+                    const int e = cend[cell];
+                    if (dx == 0 && dy == 0 && dz == 0) s = (int)i + 1;  // own cell: later entries only
+                    for (int k = s; k < e; k++) tryPair(cvals[k], true);
+                }
+            }
+        }
+        if (lstart) {  // large bounds listed in the own cell
+            const int64_t own = cellBase + ck;
+            const int s = lstart[own];
+            if (s >= 0) {
+                const int e = lend[own];
+                for (int k = s; k < e; k++) tryPair(lvals[k], true);
+            }
+        }
+    } else {
+        if (F.largePairs) {  // large-large: later entries of the large list
+            const int e = cend[cellBase + nCells];
+            for (int k = (int)i + 1; k < e; k++) tryPair(cvals[k], true);
+        }
     }
+    for (int u = 0; u < nunb; u++) tryPair(unb[u], false);
     if (!MODE) counts[i] = cnt;
 }
 
@@ -1538,6 +1615,10 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_relax(const SceneDesc* D, const F
     const int64_t i = F.binStart + li;
     int id = cvals[i];
     uint32_t ck = (uint32_t)(ckeys[i] - F.cellBase);
+    if ((int)ck == F.nCells || ptShape(S.p.ptype[id]) != 1) {  // spheres only
+        disp[id] = make_double4(0., 0., 0., 0.);
+        return;
+    }
     int cx = ck % g.n[0], cy = (ck / g.n[0]) % g.n[1], cz = ck / (g.n[0] * g.n[1]);
     double4 pi = n.posr[id];
     D3 d = mk3(0., 0., 0.);

@@ -233,6 +233,9 @@ struct Shared {
     DBuf<uint64_t> candK, candK2, entK, entK2;
     DBuf<int32_t> candP, candP2;
     DBuf<uint32_t> entV, entV2;
+    DBuf<uint32_t> lkeys, lkeys2;  // large bounds listed per cell
+    DBuf<int32_t> lvals, lvals2, lstart, lend, lcnt32;
+    DBuf<int64_t> lcnt, loffs;
     DBuf<uint8_t> temp;
     DBuf<double4> disp;
     void freeAll(Ctx* c) {
@@ -249,6 +252,8 @@ struct Shared {
         release(c, ooRank); release(c, rflag); release(c, rscan); release(c, starts); release(c, candK); release(c, candK2);
         release(c, entK); release(c, entK2); release(c, candP); release(c, candP2); release(c, entV); release(c, entV2);
         release(c, temp); release(c, disp);
+        release(c, lkeys); release(c, lkeys2); release(c, lvals); release(c, lvals2); release(c, lstart); release(c, lend);
+        release(c, lcnt32); release(c, lcnt); release(c, loffs);
         if (evWords) (void)hipEventDestroy(evWords);
         evWords = nullptr;
     }
@@ -267,6 +272,14 @@ struct Scene {
     std::vector<int32_t> binnedH, unbH;
     std::vector<uint32_t> flagsH;
     bool hasFacets = false;
+    std::vector<uint32_t> facetMasks;  // distinct masks of the facets
+    // two facets may pass Collider::mayCollide's mask tests
+    bool facetMasksPair(uint32_t lone) const {
+        for (uint32_t a : facetMasks)
+            for (uint32_t b : facetMasks)
+                if ((a & b) && !(a & b & lone)) return true;
+        return false;
+    }
     // cell (core/Cell.hpp)
     bool periodic = false;
     HM3 hSize = heye(), gradV = hzero(), nextGradV = hzero(), pprevHsize = heye(), trsf = heye();
@@ -302,6 +315,7 @@ struct Scene {
     DBuf<double> E;
     // clumps
     int32_t nClumps = 0, nMembers = 0;
+    bool clumpFree = false;  // a clump master with a free DOF
     int64_t clumpFlagged = 0;  // nodes flagged CLUMP or CLUMPED at upload
     bool clumpsSet = true;     // flags and uploaded clumps agree
     DBuf<int32_t> master, nodeClump, cOff, cMem;
@@ -605,8 +619,10 @@ struct Leapfrog {
         // clump masters' force and torque with their members' (read by the Leapfrog below)
         {
             std::vector<int32_t> cidx;
-            for (int i = 0; i < n; i++)
-                if (B.ss[i]->nClumps > 0) cidx.push_back(i);
+            for (int i = 0; i < n; i++) {  // only where Leapfrog reads the sums
+                const Scene& S = *B.ss[i];
+                if (S.nClumps > 0 && (S.clumpFree || (S.P.trackEnergy && S.P.damping != 0))) cidx.push_back(i);
+            }
             if (!cidx.empty()) {
                 std::vector<int64_t> cbp = blockPrefix((int)cidx.size(), [&](int t) { return B.ss[cidx[t]]->nClumps; });
                 Blob cb;
@@ -680,15 +696,15 @@ struct InsertionSortCollider {
         hipStream_t st = c->stream;
         int nT = (int)L.size();
         const int64_t nblkB = bpB[nT];
-        ensure(c, X.extPart, (size_t)nblkB * 7 + 7);
-        ensure(c, X.extOut, (size_t)nT * 7);
+        ensure(c, X.extPart, (size_t)nblkB * 8 + 8);
+        ensure(c, X.extOut, (size_t)nT * 8);
         if (nblkB > 0) {
             hipLaunchKernelGGL(k_extent, dim3((unsigned)nblkB), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, X.extPart.p);
             checkLaunch();
         }
         hipLaunchKernelGGL(k_extent_finalize, dim3(nT), dim3(WDM_BLOCK), 0, st, mB, (const double*)X.extPart.p, X.extOut.p);
         checkLaunch();
-        std::vector<double> h((size_t)nT * 7);
+        std::vector<double> h((size_t)nT * 8);
         HIPCHK(hipMemcpyAsync(h.data(), X.extOut.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         int64_t cellBase = 0;
@@ -700,9 +716,11 @@ struct InsertionSortCollider {
             g.periodic = S.periodic;
             for (int d = 0; d < 3; d++) { g.n[d] = 1; g.w[d] = 1; g.L[d] = S.size[d]; g.invL[d] = 1. / S.size[d]; }
             int nb = (int)S.binnedH.size();
+            F[t].wLarge = INFINITY;
             if (nb > 0) {
-                const double* e = &h[(size_t)t * 7];
-                double ext = e[6];
+                const double* e = &h[(size_t)t * 8];
+                // cells fit the spheres; larger bounds (facets) are listed apart
+                double ext = e[7] > 0 ? e[7] : e[6];
                 if (!(ext > 0) || !std::isfinite(ext)) ext = 1e-12;
                 double w = ext * 1.001;
                 const int64_t maxCells = std::max<int64_t>(4096, 2 * (int64_t)nb);
@@ -728,10 +746,13 @@ struct InsertionSortCollider {
                     if (tot <= maxCells) break;
                     w *= 1.26;
                 }
+                F[t].wLarge = w;
             }
             F[t].cellBase = cellBase;
             F[t].binStart = nbTot;
-            cellBase += (int64_t)g.n[0] * g.n[1] * g.n[2];
+            F[t].nCells = g.n[0] * g.n[1] * g.n[2];
+            F[t].largePairs = S.facetMasksPair(S.P.loneMask) ? 1 : 0;
+            cellBase += (int64_t)F[t].nCells + 1;  // + the list of large bounds
             nbTot += nb;
         }
         if (cellBase >= INT32_MAX || nbTot >= INT32_MAX) throw WdmError(WDM_E_ARG, "collider batch too large (2^31 cells or bounds)");
@@ -748,6 +769,37 @@ struct InsertionSortCollider {
         if (nbTot > 0)
             hipLaunchKernelGGL(k_cell_ranges, dim3(nblk(nbTot)), dim3(WDM_BLOCK), 0, st, (const uint32_t*)X.ckeys2.p, nbTot, X.cstart.p, X.cend.p);
         checkLaunch();
+    }
+
+    // list every large bound (facets wider than a cell) in the cells it covers; false when
+    // the batch has none (one synchronisation for the entry count)
+    static bool listLarge(Batch& B, const std::vector<int64_t>& bpB, const Map& mB, const FRScene* FRd, int64_t nbTot, int64_t G) {
+        Ctx* c = B.c;
+        Shared& X = *c->sh;
+        hipStream_t st = c->stream;
+        const int nT = (int)bpB.size() - 1;
+        if (bpB[nT] == 0) return false;
+        ensure(c, X.lcnt32, nbTot); ensure(c, X.lcnt, nbTot); ensure(c, X.loffs, (size_t)nbTot + 1);
+        hipLaunchKernelGGL(k_large_cells<0>, dim3((unsigned)bpB[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
+                           (const int32_t*)X.cvals2.p, X.lcnt32.p, (const int64_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr);
+        hipLaunchKernelGGL(k_i32_to_i64, dim3(nblk(nbTot)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.lcnt32.p, X.lcnt.p, nbTot);
+        checkLaunch();
+        scanInclusiveShift(c, X.lcnt.p, X.loffs.p, nbTot);
+        int64_t nl = 0;
+        HIPCHK(hipMemcpyAsync(&nl, X.loffs.p + nbTot, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (nl == 0) return false;
+        if (nl >= INT32_MAX) throw WdmError(WDM_E_ARG, "too many large-bound cell entries");
+        ensure(c, X.lkeys, nl); ensure(c, X.lkeys2, nl); ensure(c, X.lvals, nl); ensure(c, X.lvals2, nl);
+        hipLaunchKernelGGL(k_large_cells<1>, dim3((unsigned)bpB[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
+                           (const int32_t*)X.cvals2.p, (int32_t*)nullptr, (const int64_t*)X.loffs.p, X.lkeys.p, X.lvals.p);
+        checkLaunch();
+        sortPairs(c, X.lkeys.p, X.lkeys2.p, X.lvals.p, X.lvals2.p, nl, bitsFor((uint64_t)G));
+        ensure(c, X.lstart, G); ensure(c, X.lend, G);
+        hipLaunchKernelGGL(k_fill_i32, dim3(nblk(G)), dim3(WDM_BLOCK), 0, st, X.lstart.p, G, -1);
+        hipLaunchKernelGGL(k_cell_ranges, dim3(nblk(nl)), dim3(WDM_BLOCK), 0, st, (const uint32_t*)X.lkeys2.p, nl, X.lstart.p, X.lend.p);
+        checkLaunch();
+        return true;
     }
 
     template <typename T>
@@ -782,6 +834,9 @@ struct InsertionSortCollider {
         }
         int64_t nbTot = 0;
         buildGrids(B, L, F, bpB, mB, FRd, nbTot);
+        bool anyFacets = false;
+        for (int t = 0; t < nT; t++) anyFacets |= S_(t).hasFacets;
+        const bool large = anyFacets && listLarge(B, bpB, mB, FRd, nbTot, F[nT - 1].cellBase + F[nT - 1].nCells + 1);
 
         // candidate pairs: count, scan, write, sort by (scene, min id, max id)
         int64_t ncandTot = 0;
@@ -790,8 +845,9 @@ struct InsertionSortCollider {
             if (bpB[nT] > 0) {
                 KTimer tm(c, "pairs_count");
                 hipLaunchKernelGGL(k_pairs<0>, dim3((unsigned)bpB[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
-                                   (const int32_t*)X.cvals2.p, (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p, X.cnt32.p,
-                                   (const int64_t*)nullptr, (uint64_t*)nullptr, (int32_t*)nullptr, idBits);
+                                   (const int32_t*)X.cvals2.p, (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p,
+                                   large ? (const int32_t*)X.lstart.p : nullptr, (const int32_t*)X.lend.p, (const int32_t*)X.lvals2.p,
+                                   X.cnt32.p, (const int64_t*)nullptr, (uint64_t*)nullptr, (int32_t*)nullptr, idBits);
                 checkLaunch();
                 hipLaunchKernelGGL(k_i32_to_i64, dim3(nblk(nbTot)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.cnt32.p, X.counts.p, nbTot);
                 checkLaunch();
@@ -807,8 +863,9 @@ struct InsertionSortCollider {
             if (ncandTot > 0) {
                 KTimer tm(c, "pairs_write");
                 hipLaunchKernelGGL(k_pairs<1>, dim3((unsigned)bpB[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
-                                   (const int32_t*)X.cvals2.p, (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p, (int32_t*)nullptr,
-                                   (const int64_t*)X.offs.p, X.candK.p, X.candP.p, idBits);
+                                   (const int32_t*)X.cvals2.p, (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p,
+                                   large ? (const int32_t*)X.lstart.p : nullptr, (const int32_t*)X.lend.p, (const int32_t*)X.lvals2.p,
+                                   (int32_t*)nullptr, (const int64_t*)X.offs.p, X.candK.p, X.candP.p, idBits);
                 checkLaunch();
                 sortPairs(c, X.candK.p, X.candK2.p, X.candP.p, X.candP2.p, ncandTot, keyBits);
             }
@@ -1388,7 +1445,12 @@ int wdm_upload_particles(wdm_ctx* c, int32_t sid, int32_t n, const int32_t* shap
         S.ptypeH = pt;
         S.radiusH.assign(radius, radius + n);
         S.hasFacets = false;
-        for (int p = 0; p < n; p++) S.hasFacets |= shape[p] == WDM_SHAPE_FACET;
+        S.facetMasks.clear();
+        for (int p = 0; p < n; p++)
+            if (shape[p] == WDM_SHAPE_FACET) {
+                S.hasFacets = true;
+                if (std::find(S.facetMasks.begin(), S.facetMasks.end(), mask[p]) == S.facetMasks.end()) S.facetMasks.push_back(mask[p]);
+            }
         hipStream_t st = c->stream;
         HIPCHK(hipMemcpyAsync(S.ptype.p, pt.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
         HIPCHK(hipMemcpyAsync(S.pmask.p, mask, n * sizeof(uint32_t), hipMemcpyHostToDevice, st));
@@ -1476,6 +1538,8 @@ int wdm_upload_clumps(wdm_ctx* c, int32_t sid, int32_t nc, const int32_t* master
         }
         S.nClumps = nc;
         S.nMembers = nm;
+        S.clumpFree = false;
+        for (int k = 0; k < nc; k++) S.clumpFree |= (S.flagsH[master[k]] & 63u) != 63u;
         S.clumpsSet = true;
         S.needFull = true;
     });
