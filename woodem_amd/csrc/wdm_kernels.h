@@ -815,7 +815,9 @@ __device__ __forceinline__ bool mayCollideD(const PartView& p, const GP int32_t*
 // cells) rejects most candidates cheaply; the decision itself is the collider's own
 // closed-interval 3-axis overlap of the stored bounds plus mayCollide, so the set
 // equals the reference's and the grid only prunes.
-// MODE 0 counts per thread, MODE 1 writes keys and packed periods at offs[i].
+// Each bound's work is split into PAIR_ITEMS threads (one per stencil cell, one for walls
+// and listed large bounds).  MODE 0 counts per item, MODE 1 writes keys and packed periods
+// at offs[item].
 __device__ __forceinline__ bool periPrefilter(const double4& a0, const double4& a1, const double4& b0, const double4& b1, const GridArgs& g) {
     // centres and half extents; reject when the minimum-image gap exceeds the half
     // extents by more than 1e-9 L on some axis (rounding here is ~1e-16 L)
@@ -892,6 +894,9 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_large_cells(const SceneDesc* D, c
     }
 }
 
+// work items per bound in k_pairs: the 14 cells of the half stencil, then walls and the
+// large bounds listed in the own cell
+#define PAIR_ITEMS 15
 template <int MODE>
 __global__ void __launch_bounds__(WDM_BLOCK) k_pairs(const SceneDesc* D, const FRScene* FR, Map m, const uint32_t* ckeys, const int32_t* cvals,
                                                     const int32_t* cstart, const int32_t* cend, const int32_t* lstart, const int32_t* lend,
@@ -909,16 +914,18 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pairs(const SceneDesc* D, const F
     const int nunb = S.nunb;
     const int64_t cellBase = F.cellBase;
     const int nCells = F.nCells;
-    int li = (int)(lb * WDM_BLOCK + threadIdx.x);
+    const int64_t item = lb * WDM_BLOCK + threadIdx.x;
+    const int li = (int)(item / PAIR_ITEMS), sItem = (int)(item % PAIR_ITEMS);
     if (li >= S.nbinned) return;
     const int64_t i = F.binStart + li;
+    const int64_t q = i * PAIR_ITEMS + sItem;  // batch-wide item index
     const uint64_t tkey = (uint64_t)tt << (2 * idBits);
     const int id = cvals[i];
     const uint32_t ck = (uint32_t)(ckeys[i] - cellBase);
     const double4 a0 = p.bmin[id], a1 = p.bmax[id];
     const int sa = ptShape(p.ptype[id]);
     int cnt = 0;
-    const int64_t o = MODE ? offs[i] : 0;
+    const int64_t o = MODE ? offs[q] : 0;
     // one candidate: prefilter, canonical order, exact overlap (with periods), mayCollide
     auto tryPair = [&](int j, bool binned) {
         const double4 b0 = p.bmin[j], b1 = p.bmax[j];
@@ -941,28 +948,41 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pairs(const SceneDesc* D, const F
         }
         cnt++;
     };
-    if ((int)ck != nCells) {
-        const int cx = ck % g.n[0], cy = (ck / g.n[0]) % g.n[1], cz = ck / (g.n[0] * g.n[1]);
-        // periodic axes with fewer than 3 cells hold a single cell (no neighbours); aperiodic axes always scan +-1
-        const int rx = (!g.periodic || g.n[0] >= 3) ? 1 : 0, ry = (!g.periodic || g.n[1] >= 3) ? 1 : 0,
-                  rz = (!g.periodic || g.n[2] >= 3) ? 1 : 0;
-        for (int dz = 0; dz <= rz; dz++) {
-            for (int dy = (dz == 0 ? 0 : -ry); dy <= ry; dy++) {
-                for (int dx = ((dz == 0 && dy == 0) ? 0 : -rx); dx <= rx; dx++) {
-                    int z = cz + dz, y = cy + dy, x = cx + dx;
-                    if (g.periodic) {
-                        z = (z + g.n[2]) % g.n[2]; y = (y + g.n[1]) % g.n[1]; x = (x + g.n[0]) % g.n[0];
-                    } else if (z < 0 || z >= g.n[2] || y < 0 || y >= g.n[1] || x < 0 || x >= g.n[0]) continue;
+    const bool large = (int)ck == nCells;
+    if (sItem < 14) {
+        if (!large) {
+            // half-stencil offset of this item: (0,0,0), (1,0,0), (-1..1,1,0), (-1..1,-1..1,1)
+            int dx, dy, dz;
+            if (sItem == 0) { dx = 0; dy = 0; dz = 0; }
+            else if (sItem == 1) { dx = 1; dy = 0; dz = 0; }
+            else if (sItem < 5) { dx = sItem - 3; dy = 1; dz = 0; }
+            else { dx = (sItem - 5) % 3 - 1; dy = (sItem - 5) / 3 - 1; dz = 1; }
+            // periodic axes with fewer than 3 cells hold a single cell (no neighbours); aperiodic axes always scan +-1
+            const bool okx = dx == 0 || !g.periodic || g.n[0] >= 3, oky = dy == 0 || !g.periodic || g.n[1] >= 3,
+                       okz = dz == 0 || !g.periodic || g.n[2] >= 3;
+            if (okx && oky && okz) {
+                const int cx = ck % g.n[0], cy = (ck / g.n[0]) % g.n[1], cz = ck / (g.n[0] * g.n[1]);
+                int z = cz + dz, y = cy + dy, x = cx + dx;
+                bool in = true;
+                if (g.periodic) {
+                    z = (z + g.n[2]) % g.n[2]; y = (y + g.n[1]) % g.n[1]; x = (x + g.n[0]) % g.n[0];
+                } else in = !(z < 0 || z >= g.n[2] || y < 0 || y >= g.n[1] || x < 0 || x >= g.n[0]);
+                if (in) {
                     const int64_t cell = cellBase + (z * g.n[1] + y) * g.n[0] + x;
                     int s = cstart[cell];
-                    if (s < 0) continue;
-                    const int e = cend[cell];
-                    if (dx == 0 && dy == 0 && dz == 0) s = (int)i + 1;  // own cell: later entries only
-                    for (int k = s; k < e; k++) tryPair(cvals[k], true);
+                    if (s >= 0) {
+                        const int e = cend[cell];
+                        if (sItem == 0) s = (int)i + 1;  // own cell: later entries only
+                        for (int k = s; k < e; k++) tryPair(cvals[k], true);
+                    }
                 }
             }
+        } else if (sItem == 0 && F.largePairs) {  // large-large: later entries of the large list
+            const int e = cend[cellBase + nCells];
+            for (int k = (int)i + 1; k < e; k++) tryPair(cvals[k], true);
         }
-        if (lstart) {  // large bounds listed in the own cell
+    } else {
+        if (!large && lstart) {  // large bounds listed in the own cell
             const int64_t own = cellBase + ck;
             const int s = lstart[own];
             if (s >= 0) {
@@ -970,14 +990,9 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pairs(const SceneDesc* D, const F
                 for (int k = s; k < e; k++) tryPair(lvals[k], true);
             }
         }
-    } else {
-        if (F.largePairs) {  // large-large: later entries of the large list
-            const int e = cend[cellBase + nCells];
-            for (int k = (int)i + 1; k < e; k++) tryPair(cvals[k], true);
-        }
+        for (int u = 0; u < nunb; u++) tryPair(unb[u], false);
     }
-    for (int u = 0; u < nunb; u++) tryPair(unb[u], false);
-    if (!MODE) counts[i] = cnt;
+    if (!MODE) counts[q] = cnt;
 }
 
 __device__ __forceinline__ int64_t lowerBound64(const uint64_t* a, int64_t n, uint64_t k) {
@@ -1176,8 +1191,11 @@ __global__ void k_set_counters(const ConDesc<T>* C, const FRScene* FR, int n, co
 }
 
 // CSR entries of the new lists: key (t, particle, other), value contact index | isB << 31
+// CSR keys are (t, particle): the contacts come in (min id, max id) order, so the entries
+// of a particle arrive with the other id ascending (first as max id, then as min id) and
+// a stable radix sort on (t, particle) alone leaves them in that order.
 // Dormant contacts cannot become real before the next full run and dead ones never again:
-// their entries get the key (t, np, 0), behind every particle of the scene, so the CSR
+// their entries get the key (t, np), behind every particle of the scene, so the CSR
 // offsets leave them out of every particle's range.
 template <typename T>
 __global__ void __launch_bounds__(WDM_BLOCK) k_csr_entries(const SceneDesc* D, const ConDesc<T>* C, const FRScene* FR, Map m, uint64_t* ek,
@@ -1188,16 +1206,16 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_csr_entries(const SceneDesc* D, c
     const ConView<T> nc = C[t].cvNew;
     int64_t c = lb * WDM_BLOCK + threadIdx.x;
     if (c >= F.nNew) return;
-    const uint64_t tkey = (uint64_t)t << (2 * idBits);
+    const uint64_t tkey = (uint64_t)t << idBits;
     const int2 id = nc.ids[c];
     const int64_t g = F.newStart + c;
     if (nc.info[c] & (CI_DORMANT | CI_DEAD)) {
-        const uint64_t off = tkey | ((uint64_t)(uint32_t)D[F.desc].p.np << idBits);
+        const uint64_t off = tkey | (uint32_t)D[F.desc].p.np;
         ek[2 * g] = off;
         ek[2 * g + 1] = off;
     } else {
-        ek[2 * g] = tkey | ((uint64_t)(uint32_t)id.x << idBits) | (uint32_t)id.y;
-        ek[2 * g + 1] = tkey | ((uint64_t)(uint32_t)id.y << idBits) | (uint32_t)id.x;
+        ek[2 * g] = tkey | (uint32_t)id.x;
+        ek[2 * g + 1] = tkey | (uint32_t)id.y;
     }
     ev[2 * g] = (uint32_t)c;
     ev[2 * g + 1] = (uint32_t)c | 0x80000000u;
@@ -1217,7 +1235,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_csr_offsets(const SceneDesc* D, c
     const int np = D[F.desc].p.np;
     int i = (int)(lb * WDM_BLOCK + threadIdx.x);
     if (i > np) return;
-    const uint64_t key = ((uint64_t)t << (2 * idBits)) | ((uint64_t)(uint32_t)i << idBits);
+    const uint64_t key = ((uint64_t)t << idBits) | (uint32_t)i;
     F.csrOff[i] = lowerBound64(ek + 2 * F.newStart, 2 * F.nNew, key);
 }
 

@@ -221,7 +221,7 @@ struct Shared {
     std::vector<int32_t> freeSlots;
     int32_t nslots = 0;
     Staged stDesc, stMapNode, stMapCon, stCon, stFR, stConFR, stIdx;
-    Staged stMapP, stMapB, stMapC, stMapO, stMapN, stMapE, stMapP1, stMapCl, stMapClF;
+    Staged stMapP, stMapB, stMapC, stMapO, stMapN, stMapE, stMapP1, stMapCl, stMapClF, stMapQ;
     hipEvent_t evWords = nullptr;  // recorded after the copy of the words to the host
     // energy partials per block
     DBuf<double> epartL, epartC;
@@ -244,7 +244,7 @@ struct Shared {
         hostWords = nullptr;
         hostWordsCap = 0;
         for (Staged* s : {&stDesc, &stMapNode, &stMapCon, &stCon, &stFR, &stConFR, &stIdx, &stMapP, &stMapB, &stMapC, &stMapO, &stMapN,
-                          &stMapE, &stMapP1, &stMapCl, &stMapClF})
+                          &stMapE, &stMapP1, &stMapCl, &stMapClF, &stMapQ})
             s->free(c);
         release(c, epartL); release(c, epartC); release(c, extPart); release(c, extOut);
         release(c, ckeys); release(c, ckeys2); release(c, cvals); release(c, cvals2); release(c, cstart); release(c, cend);
@@ -838,31 +838,35 @@ struct InsertionSortCollider {
         for (int t = 0; t < nT; t++) anyFacets |= S_(t).hasFacets;
         const bool large = anyFacets && listLarge(B, bpB, mB, FRd, nbTot, F[nT - 1].cellBase + F[nT - 1].nCells + 1);
 
-        // candidate pairs: count, scan, write, sort by (scene, min id, max id)
+        // candidate pairs: count, scan, write, sort by (scene, min id, max id); PAIR_ITEMS
+        // threads per bound
         int64_t ncandTot = 0;
         {
-            ensure(c, X.cnt32, nbTot); ensure(c, X.counts, nbTot); ensure(c, X.offs, (size_t)nbTot + 1);
-            if (bpB[nT] > 0) {
+            const int64_t nq = nbTot * PAIR_ITEMS;
+            ensure(c, X.cnt32, nq); ensure(c, X.counts, nq); ensure(c, X.offs, (size_t)nq + 1);
+            std::vector<int64_t> bpQ = blockPrefix(nT, [&](int t) { return (int64_t)S_(t).binnedH.size() * PAIR_ITEMS; });
+            Map mQ = uploadMap(c, X.stMapQ, bpQ);
+            if (bpQ[nT] > 0) {
                 KTimer tm(c, "pairs_count");
-                hipLaunchKernelGGL(k_pairs<0>, dim3((unsigned)bpB[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
+                hipLaunchKernelGGL(k_pairs<0>, dim3((unsigned)bpQ[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mQ, (const uint32_t*)X.ckeys2.p,
                                    (const int32_t*)X.cvals2.p, (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p,
                                    large ? (const int32_t*)X.lstart.p : nullptr, (const int32_t*)X.lend.p, (const int32_t*)X.lvals2.p,
                                    X.cnt32.p, (const int64_t*)nullptr, (uint64_t*)nullptr, (int32_t*)nullptr, idBits);
                 checkLaunch();
-                hipLaunchKernelGGL(k_i32_to_i64, dim3(nblk(nbTot)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.cnt32.p, X.counts.p, nbTot);
+                hipLaunchKernelGGL(k_i32_to_i64, dim3(nblk(nq)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.cnt32.p, X.counts.p, nq);
                 checkLaunch();
             }
-            scanInclusiveShift(c, X.counts.p, X.offs.p, nbTot);
+            scanInclusiveShift(c, X.counts.p, X.offs.p, nq);
             std::vector<int64_t> first(nT + 1);
-            for (int t = 0; t < nT; t++) first[t] = F[t].binStart;
-            first[nT] = nbTot;
+            for (int t = 0; t < nT; t++) first[t] = F[t].binStart * PAIR_ITEMS;
+            first[nT] = nq;
             std::vector<int64_t> cs = segmentStarts(c, X.offs.p, first);
             for (int t = 0; t < nT; t++) { F[t].candStart = cs[t]; F[t].nCand = cs[t + 1] - cs[t]; }
             ncandTot = cs[nT];
             ensure(c, X.candK, ncandTot); ensure(c, X.candK2, ncandTot); ensure(c, X.candP, ncandTot); ensure(c, X.candP2, ncandTot);
             if (ncandTot > 0) {
                 KTimer tm(c, "pairs_write");
-                hipLaunchKernelGGL(k_pairs<1>, dim3((unsigned)bpB[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
+                hipLaunchKernelGGL(k_pairs<1>, dim3((unsigned)bpQ[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mQ, (const uint32_t*)X.ckeys2.p,
                                    (const int32_t*)X.cvals2.p, (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p,
                                    large ? (const int32_t*)X.lstart.p : nullptr, (const int32_t*)X.lend.p, (const int32_t*)X.lvals2.p,
                                    (int32_t*)nullptr, (const int64_t*)X.offs.p, X.candK.p, X.candP.p, idBits);
@@ -965,7 +969,7 @@ struct InsertionSortCollider {
             if (nNewTot > 0) {
                 hipLaunchKernelGGL(k_csr_entries<T>, dim3((unsigned)bpN[nT]), dim3(WDM_BLOCK), 0, st, B.D, Cd, FRd, mN, X.entK.p, X.entV.p, idBits);
                 checkLaunch();
-                sortPairs(c, X.entK.p, X.entK2.p, X.entV.p, X.entV2.p, ne, keyBits);
+                sortPairs(c, X.entK.p, X.entK2.p, X.entV.p, X.entV2.p, ne, idBits + tBits);  // stable: other id order kept
                 std::vector<int64_t> bpE = blockPrefix(nT, [&](int t) { return 2 * F[t].nNew; });
                 Map mE = uploadMap(c, X.stMapE, bpE);
                 hipLaunchKernelGGL(k_csr_scatter, dim3((unsigned)bpE[nT]), dim3(WDM_BLOCK), 0, st, FRd, mE, (const uint32_t*)X.entV2.p);
