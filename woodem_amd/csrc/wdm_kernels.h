@@ -8,8 +8,12 @@
 //     mat i32, facet nodes int4; stored bounds bmin/bmax double4.
 //   contacts, C entries sorted by key = (min id << 32 | max id): ids int2 (functor
 //     order), info u32, packed cellDist i32, state row i32; state T[NSTATE][cap]
-//     component-major, rows of real contacts dense in key order after each full run,
-//     gather record gfc[2c] = (global force, 0), gfc[2c+1] = (contact point, uN).
+//     component-major (trsf columns 0-1, local force, phys), rows of real contacts dense
+//     in key order after each full run,
+//     contact point per state row (FP64); work list of the ContactLoop: contacts holding a
+//     state row first (item == row), then the other non-dormant ones, dormant contacts left
+//     out; gather record per item gfc[2i] = (global force, cellDist), gfc[2i+1] = (contact
+//     point, uN).
 //   CSR: off[P+1], entries u32 (contact index | isB << 31), per particle sorted by
 //     the other particle's id.
 // Batches: every kernel takes a table of per-scene descriptors (views of the arrays
@@ -45,10 +49,10 @@ __host__ __device__ inline GP T* gpc(T* p) { return (GP T*)p; }
 #define CI_DORMANT 64u  // cannot touch before the next full run (gap > 2 verletDist at this run)
 
 // contact state components
-#define S_T 0      // trsf, 9 entries row-major
-#define S_F 9      // local force, 3 entries
-#define S_P 12     // phys, 5 entries
-#define NSTATE 17
+#define S_T 0      // trsf columns 0 and 1 (x, y, z each); column 2 is their cross product
+#define S_F 6      // local force, 3 entries
+#define S_P 9      // phys, 5 entries
+#define NSTATE 14
 
 // device error bits
 #define ERR_NAN 1u
@@ -118,7 +122,11 @@ struct ConView {
     int32_t* counter;  // next free state row (rows are handed out densely at full runs)
     GP int32_t* cd;
     GP T* st;
-    GP double4* gfc;
+    GP double4* cpt;   // per state row: contact point of the last ContactLoop (x, y, z, 0)
+    GP int32_t* itm;   // per contact: its item in the ContactLoop's work list, -1 when dormant
+    GP int32_t* act;   // per item: the contact
+    int32_t* nItems;   // items in the work list
+    GP double4* gfc;   // per item: gather record
     int64_t cap;
 };
 
@@ -1044,8 +1052,6 @@ template <typename T>
 __device__ __forceinline__ void copyContact(const ConView<T>& src, int64_t s, const ConView<T>& dst, int64_t d) {
     dst.ids[d] = src.ids[s];
     dst.cd[d] = src.cd[s];
-    dst.gfc[2 * d] = src.gfc[2 * s];
-    dst.gfc[2 * d + 1] = src.gfc[2 * s + 1];
 }
 
 // merged list = candidates (bounds overlap) U old real contacts (InsertionSortCollider keeps
@@ -1137,8 +1143,6 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_emit_cand(const SceneDesc* D, con
         const bool dorm = dormantPair(n, p, A, B, (uint32_t)ty, pc, vd, df);
         const bool fresh = !dorm && touchesNow(S.a, n, p, A, B, (uint32_t)ty, pc);
         nc.info[d] = (uint32_t)ty | CI_OVERLAP | (dorm ? CI_DORMANT : 0u);
-        nc.gfc[2 * d] = make_double4(0., 0., 0., 0.);
-        nc.gfc[2 * d + 1] = make_double4(0., 0., 0., 0.);
         oslot[gd] = fresh ? -2 : -1;
         rflag[gd] = fresh ? 1 : 0;
     }
@@ -1165,9 +1169,10 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_emit_oor(const ConDesc<T>* C, con
 
 // dense state rows for the real (and touching) contacts of the merged list, in key order
 // (rscan: exclusive batch-wide prefix of rflag); oslot -2: row without state to copy
+// nrflag: 1 for a contact that enters the work list without a state row (not dormant)
 template <typename T>
 __global__ void __launch_bounds__(WDM_BLOCK) k_copy_state(const ConDesc<T>* C, const FRScene* FR, Map m, const int32_t* oslot,
-                                                         const int64_t* rscan) {
+                                                         const int64_t* rscan, int64_t* nrflag) {
     int64_t lb;
     const int t = mapBlock(m, lb);
     const FRScene& F = FR[t];
@@ -1175,22 +1180,51 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_copy_state(const ConDesc<T>* C, c
     int64_t d = lb * WDM_BLOCK + threadIdx.x;
     if (d >= F.nNew) return;
     int32_t os = oslot[F.newStart + d];
-    if (os == -1) { nc.slot[d] = -1; return; }
+    if (os == -1) {
+        nc.slot[d] = -1;
+        nrflag[F.newStart + d] = (nc.info[d] & CI_DORMANT) ? 0 : 1;
+        return;
+    }
+    nrflag[F.newStart + d] = 0;
     int64_t ns = rscan[F.newStart + d] - rscan[F.newStart];
     nc.slot[d] = (int32_t)ns;
     if (os < 0) return;
     for (int k = 0; k < NSTATE; k++) nc.st[(size_t)k * nc.cap + ns] = oldc.st[(size_t)k * oldc.cap + os];
+    nc.cpt[ns] = oldc.cpt[os];
 }
-// next free state row of each scene = number of real contacts carried over
+// work list of the ContactLoop: a contact with a state row is item `row`, the other
+// non-dormant ones follow in key order (nrscan: exclusive batch-wide prefix of nrflag)
 template <typename T>
-__global__ void k_set_counters(const ConDesc<T>* C, const FRScene* FR, int n, const int64_t* rscan) {
+__global__ void __launch_bounds__(WDM_BLOCK) k_items(const ConDesc<T>* C, const FRScene* FR, Map m, const int64_t* rscan,
+                                                    const int64_t* nrscan) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const FRScene& F = FR[t];
+    const ConView<T> nc = C[t].cvNew;
+    int64_t d = lb * WDM_BLOCK + threadIdx.x;
+    if (d >= F.nNew) return;
+    const int32_t row = nc.slot[d];
+    int64_t item = -1;
+    if (row >= 0) item = row;
+    else if (!(nc.info[d] & CI_DORMANT)) {
+        const int64_t nRows = rscan[F.newStart + F.nNew] - rscan[F.newStart];
+        item = nRows + (nrscan[F.newStart + d] - nrscan[F.newStart]);
+    }
+    nc.itm[d] = (int32_t)item;
+    if (item >= 0) nc.act[item] = (int32_t)d;
+}
+// next free state row of each scene = rows handed out (real and touching contacts); items
+template <typename T>
+__global__ void k_set_counters(const ConDesc<T>* C, const FRScene* FR, int n, const int64_t* rscan, const int64_t* nrscan) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const FRScene& F = FR[t];
-    *C[t].cvNew.counter = (int32_t)(rscan[F.newStart + F.nNew] - rscan[F.newStart]);
+    const int64_t nRows = rscan[F.newStart + F.nNew] - rscan[F.newStart];
+    *C[t].cvNew.counter = (int32_t)nRows;
+    *C[t].cvNew.nItems = (int32_t)(nRows + nrscan[F.newStart + F.nNew] - nrscan[F.newStart]);
 }
 
-// CSR entries of the new lists: key (t, particle, other), value contact index | isB << 31
+// CSR entries of the new lists: key (t, particle), value work-list item | isB << 31
 // CSR keys are (t, particle): the contacts come in (min id, max id) order, so the entries
 // of a particle arrive with the other id ascending (first as max id, then as min id) and
 // a stable radix sort on (t, particle) alone leaves them in that order.
@@ -1217,8 +1251,9 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_csr_entries(const SceneDesc* D, c
         ek[2 * g] = tkey | (uint32_t)id.x;
         ek[2 * g + 1] = tkey | (uint32_t)id.y;
     }
-    ev[2 * g] = (uint32_t)c;
-    ev[2 * g + 1] = (uint32_t)c | 0x80000000u;
+    const uint32_t it = (uint32_t)nc.itm[c];  // dormant (-1) entries sit past every range
+    ev[2 * g] = it;
+    ev[2 * g + 1] = it | 0x80000000u;
 }
 // copy each scene's sorted entry values into its own CSR array
 __global__ void __launch_bounds__(WDM_BLOCK) k_csr_scatter(const FRScene* FR, Map m, const uint32_t* ev) {
@@ -1282,16 +1317,19 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
     const NodeView n = S.n;
     const PartView p = S.p;
     const ConView<T> cv = CD.cv;
-    const int64_t nc = CD.nc;
     uint32_t* err = S.words + 2;
     uint32_t* creOut = S.words + 1;
     uint64_t* realMask = CD.realMask;
     if (spec && loadWord(S.words) == a.stamp) return;  // the collider runs first: launched again afterwards
-    int64_t c = lb * WDM_BLOCK + threadIdx.x;
+    // work list: dormant contacts are not in it; contacts with a state row come first, as
+    // item == row, so their state is read and written contiguously
+    const int64_t nItems = *cv.nItems;
+    const int64_t item = lb * WDM_BLOCK + threadIdx.x;
     double ev[5] = {0., 0., 0., 0., 0.};  // elast, plast, viscN, viscT, broken
     uint32_t bits = 0;
     bool realOut = false;  // contact is real after this step's law
-    if (c < nc) do {
+    if (item < nItems) do {
+        const int64_t c = cv.act[item];
         uint32_t info = cv.info[c];
         if (info & (CI_DEAD | CI_DORMANT)) break;
         uint32_t ty = info & CI_TYPE_MASK;
@@ -1314,7 +1352,15 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
         }
         T tr[9];
         int32_t row = real ? cv.slot[c] : -1;
-        if (real) for (int k = 0; k < 9; k++) tr[k] = cv.st[(size_t)(S_T + k) * cv.cap + row];
+        // L6Geom.trsf: only columns 0 and 1 are read by the update (L6Geom.cpp:110-118
+        // recomputes column 2 as their cross product), so only they are stored
+        if (real) {
+            for (int k = 0; k < 3; k++) {
+                tr[3 * k] = cv.st[(size_t)(S_T + k) * cv.cap + row];
+                tr[3 * k + 1] = cv.st[(size_t)(S_T + 3 + k) * cv.cap + row];
+                tr[3 * k + 2] = (T)0;
+            }
+        }
         Geo<T> G;
         if ((GEOM & G_SS) && ty == CI_SS) {  // Cg2_Sphere_Sphere_L6Geom::go, pkg/dem/Sphere.cpp:88-109
             D3 p2 = xyz(pb) + shift2;
@@ -1425,8 +1471,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
             gangVel = mk3<T>(0, 0, 0);
         } else {
             const V3T<T> currNormal = G.normal, prevNormal = trCol(tr, 0);
-            const double4 cprev = cv.gfc[2 * c + 1];
-            const D3 prevContPt = xyz(cprev);
+            const D3 prevContPt = xyz(cv.cpt[row]);
             D3 shiftVel2 = mk3(0., 0., 0.);
             if (a.periodic) shiftVel2 = mv3(ldM3(a.cell->shiftVelM), mk3((double)unpack10(cdp, 0), (double)unpack10(cdp, 10), (double)unpack10(cdp, 20)));
             V3T<T> normRotVec = cross3(prevNormal, currNormal);
@@ -1585,7 +1630,6 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
         }
         if (!keep) {  // ContactContainer::requestRemoval -> Contact::reset (ContactContainer.cpp:164-171)
             cv.info[c] = (info & ~CI_REAL) | CI_PENDING;
-            cv.gfc[2 * c] = make_double4(0., 0., 0., 0.);
             break;
         }
         // store geometry/physics state
@@ -1597,7 +1641,10 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
                 cv.slot[c] = row;
             }
         }
-        for (int k = 0; k < 9; k++) cv.st[(size_t)(S_T + k) * cv.cap + row] = tr[k];
+        for (int k = 0; k < 3; k++) {
+            cv.st[(size_t)(S_T + k) * cv.cap + row] = tr[3 * k];
+            cv.st[(size_t)(S_T + 3 + k) * cv.cap + row] = tr[3 * k + 1];
+        }
         for (int k = 0; k < 3; k++) cv.st[(size_t)(S_F + k) * cv.cap + row] = F[k];
         if (!real) {
             for (int k = 0; k < 5; k++) cv.st[(size_t)(S_P + k) * cv.cap + row] = ph[k];
@@ -1606,14 +1653,15 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
         // global force trsf*F (Contact::getForceTorqueBranch, Contact.cpp:24-34)
         V3T<T> fl = mk3<T>(F[0], F[1], F[2]);
         V3T<T> fg = mk3<T>(dot3(mk3<T>(tr[0], tr[1], tr[2]), fl), dot3(mk3<T>(tr[3], tr[4], tr[5]), fl), dot3(mk3<T>(tr[6], tr[7], tr[8]), fl));
-        cv.gfc[2 * c] = make_double4((double)fg.x, (double)fg.y, (double)fg.z, (double)cdp);
-        cv.gfc[2 * c + 1] = make_double4(G.contPt.x, G.contPt.y, G.contPt.z, (double)uN);
+        cv.gfc[2 * item] = make_double4((double)fg.x, (double)fg.y, (double)fg.z, (double)cdp);
+        cv.gfc[2 * item + 1] = make_double4(G.contPt.x, G.contPt.y, G.contPt.z, (double)uN);
+        cv.cpt[row] = make_double4(G.contPt.x, G.contPt.y, G.contPt.z, 0.);
         realOut = true;
     } while (0);
-    // one bit per contact, one 64-bit word per wavefront: the gather skips non-real contacts
+    // one bit per item, one 64-bit word per wavefront: the gather skips non-real contacts
     uint64_t bal = __ballot(realOut);
     int64_t w0 = lb * WDM_BLOCK + (threadIdx.x & ~63);
-    if ((threadIdx.x & 63) == 0 && w0 < nc) realMask[w0 >> 6] = bal;
+    if ((threadIdx.x & 63) == 0 && w0 < nItems) realMask[w0 >> 6] = bal;
     if (a.trackEnergy) blockSumWrite<5>(ev, bits, epart, nullptr, 0u, creOut);
 }
 

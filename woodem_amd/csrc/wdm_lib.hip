@@ -126,27 +126,28 @@ struct ConBuf {
     DBuf<uint64_t> key;
     DBuf<int2> ids;
     DBuf<uint32_t> info;
-    DBuf<int32_t> cd, slot, counter;
+    DBuf<int32_t> cd, slot, counter, itm, act, nItems;
     DBuf<T> st;
-    DBuf<double4> gfc;
+    DBuf<double4> cpt, gfc;
     int64_t cap = 0;
     void reserve(Ctx* c, int64_t n) {
         if (n <= cap && key.p) return;
         int64_t nc = std::max<int64_t>(n + n / 3, 1024);
-        release(c, key); release(c, ids); release(c, info); release(c, cd); release(c, st); release(c, gfc);
-        release(c, slot); release(c, counter);
+        free(c);
         ensure(c, key, nc); ensure(c, ids, nc); ensure(c, info, nc); ensure(c, cd, nc);
-        ensure(c, slot, nc); ensure(c, counter, 1);
-        ensure(c, st, (size_t)NSTATE * nc); ensure(c, gfc, 2 * (size_t)nc);
+        ensure(c, slot, nc); ensure(c, counter, 1); ensure(c, itm, nc); ensure(c, act, nc); ensure(c, nItems, 1);
+        ensure(c, st, (size_t)NSTATE * nc); ensure(c, cpt, nc); ensure(c, gfc, 2 * (size_t)nc);
+        HIPCHK(hipMemsetAsync(nItems.p, 0, sizeof(int32_t), c->stream));
         cap = nc;
     }
     ConView<T> view() const {
         ConView<T> v; v.key = gpc(key.p); v.ids = gpc(ids.p); v.info = gpc(info.p); v.cd = gpc(cd.p); v.slot = gpc(slot.p);
-        v.counter = counter.p; v.st = gpc(st.p); v.gfc = gpc(gfc.p); v.cap = cap; return v;
+        v.counter = counter.p; v.st = gpc(st.p); v.cpt = gpc(cpt.p); v.itm = gpc(itm.p); v.act = gpc(act.p); v.nItems = nItems.p;
+        v.gfc = gpc(gfc.p); v.cap = cap; return v;
     }
     void free(Ctx* c) {
         release(c, key); release(c, ids); release(c, info); release(c, cd); release(c, st); release(c, gfc);
-        release(c, slot); release(c, counter); cap = 0;
+        release(c, slot); release(c, counter); release(c, itm); release(c, act); release(c, nItems); release(c, cpt); cap = 0;
     }
 };
 
@@ -229,7 +230,7 @@ struct Shared {
     DBuf<double> extPart, extOut;
     DBuf<uint32_t> ckeys, ckeys2;
     DBuf<int32_t> cvals, cvals2, cstart, cend, cnt32, ooFlag, oslot;
-    DBuf<int64_t> counts, offs, ooFlag64, ooRank, rflag, rscan, starts;
+    DBuf<int64_t> counts, offs, ooFlag64, ooRank, rflag, rscan, starts, nrflag, nrscan;
     DBuf<uint64_t> candK, candK2, entK, entK2;
     DBuf<int32_t> candP, candP2;
     DBuf<uint32_t> entV, entV2;
@@ -249,7 +250,7 @@ struct Shared {
         release(c, epartL); release(c, epartC); release(c, extPart); release(c, extOut);
         release(c, ckeys); release(c, ckeys2); release(c, cvals); release(c, cvals2); release(c, cstart); release(c, cend);
         release(c, cnt32); release(c, ooFlag); release(c, oslot); release(c, counts); release(c, offs); release(c, ooFlag64);
-        release(c, ooRank); release(c, rflag); release(c, rscan); release(c, starts); release(c, candK); release(c, candK2);
+        release(c, ooRank); release(c, rflag); release(c, rscan); release(c, starts); release(c, nrflag); release(c, nrscan); release(c, candK); release(c, candK2);
         release(c, entK); release(c, entK2); release(c, candP); release(c, candP2); release(c, entV); release(c, entV2);
         release(c, temp); release(c, disp);
         release(c, lkeys); release(c, lkeys2); release(c, lvals); release(c, lvals2); release(c, lstart); release(c, lend);
@@ -934,6 +935,7 @@ struct InsertionSortCollider {
         Cd = (const ConDesc<T>*)X.stConFR.upload(c, C.data(), C.size() * sizeof(ConDesc<T>));
         ensure(c, X.oslot, std::max<int64_t>(nNewTot, 1)); ensure(c, X.rflag, std::max<int64_t>(nNewTot, 1));
         ensure(c, X.rscan, (size_t)nNewTot + 1);
+        ensure(c, X.nrflag, std::max<int64_t>(nNewTot, 1)); ensure(c, X.nrscan, (size_t)nNewTot + 1);
         std::vector<int64_t> bpC = blockPrefix(nT, [&](int t) { return F[t].nCand; });
         std::vector<int64_t> bpN = blockPrefix(nT, [&](int t) { return F[t].nNew; });
         Map mC = uploadMap(c, X.stMapC, bpC), mN = uploadMap(c, X.stMapN, bpN);
@@ -949,8 +951,15 @@ struct InsertionSortCollider {
             scanInclusiveShift(c, X.rflag.p, X.rscan.p, nNewTot);
             if (nNewTot > 0)
                 hipLaunchKernelGGL(k_copy_state<T>, dim3((unsigned)bpN[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mN, (const int32_t*)X.oslot.p,
-                                   (const int64_t*)X.rscan.p);
-            hipLaunchKernelGGL(k_set_counters<T>, dim3((nT + 63) / 64), dim3(64), 0, st, Cd, FRd, nT, (const int64_t*)X.rscan.p);
+                                   (const int64_t*)X.rscan.p, X.nrflag.p);
+            checkLaunch();
+            // ContactLoop work list: rows first, then the other non-dormant contacts
+            scanInclusiveShift(c, X.nrflag.p, X.nrscan.p, nNewTot);
+            if (nNewTot > 0)
+                hipLaunchKernelGGL(k_items<T>, dim3((unsigned)bpN[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mN, (const int64_t*)X.rscan.p,
+                                   (const int64_t*)X.nrscan.p);
+            hipLaunchKernelGGL(k_set_counters<T>, dim3((nT + 63) / 64), dim3(64), 0, st, Cd, FRd, nT, (const int64_t*)X.rscan.p,
+                               (const int64_t*)X.nrscan.p);
             checkLaunch();
         }
         for (int t = 0; t < nT; t++) {
@@ -1183,7 +1192,7 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
     int64_t n = S.nc;
     std::vector<int2> ids(n);
     std::vector<uint32_t> info(n);
-    std::vector<int32_t> cd(n), slot(n);
+    std::vector<int32_t> cd(n), slot(n), itm(n);
     std::vector<T> st((size_t)NSTATE * std::max<int64_t>(b.cap, 1));
     std::vector<double4> gfc(2 * (size_t)n);
     if (n > 0) {
@@ -1191,8 +1200,9 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
         HIPCHK(hipMemcpyAsync(info.data(), b.info.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(cd.data(), b.cd.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(slot.data(), b.slot.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(itm.data(), b.itm.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(st.data(), b.st.p, (size_t)NSTATE * b.cap * sizeof(T), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(gfc.data(), b.gfc.p, 2 * n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(gfc.data(), b.gfc.p, 2 * n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));  // items <= contacts
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     int64_t k = 0;
@@ -1207,9 +1217,22 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
         for (int d = 0; d < 3; d++) {
             if (forceLocal3) forceLocal3[3 * k + d] = real ? (double)st[(size_t)(S_F + d) * b.cap + row] : 0.;
         }
-        if (contPt3) { contPt3[3 * k] = gfc[2 * i + 1].x; contPt3[3 * k + 1] = gfc[2 * i + 1].y; contPt3[3 * k + 2] = gfc[2 * i + 1].z; }
-        if (uN) uN[k] = real ? gfc[2 * i + 1].w : NAN;
-        if (trsf9) for (int q = 0; q < 9; q++) trsf9[9 * k + q] = (real && row >= 0) ? (double)st[(size_t)(S_T + q) * b.cap + row] : NAN;
+        // contact point and uN exist for real contacts only (L6Geom); zeros otherwise
+        const double4 rec = (real && itm[i] >= 0) ? gfc[2 * (size_t)itm[i] + 1] : make_double4(0., 0., 0., 0.);
+        if (contPt3) { contPt3[3 * k] = rec.x; contPt3[3 * k + 1] = rec.y; contPt3[3 * k + 2] = rec.z; }
+        if (uN) uN[k] = real ? rec.w : NAN;
+        if (trsf9) {  // columns 0 and 1 stored, column 2 = col0 x col1 in the state's precision
+            if (real && row >= 0) {
+                T c0[3], c1[3];
+                for (int q = 0; q < 3; q++) { c0[q] = st[(size_t)(S_T + q) * b.cap + row]; c1[q] = st[(size_t)(S_T + 3 + q) * b.cap + row]; }
+                const T c2[3] = {c0[1] * c1[2] - c0[2] * c1[1], c0[2] * c1[0] - c0[0] * c1[2], c0[0] * c1[1] - c0[1] * c1[0]};
+                for (int q = 0; q < 3; q++) {
+                    trsf9[9 * k + 3 * q] = (double)c0[q];
+                    trsf9[9 * k + 3 * q + 1] = (double)c1[q];
+                    trsf9[9 * k + 3 * q + 2] = (double)c2[q];
+                }
+            } else for (int q = 0; q < 9; q++) trsf9[9 * k + q] = NAN;
+        }
         k++;
     }
     if (nOut) *nOut = k;
@@ -1327,7 +1350,7 @@ size_t wdm_scene_footprint(int32_t n_par, int32_t n_node, double cpp) {
     size_t nodeB = (size_t)n_node * (7 * 32 + 4 + 1);
     size_t parB = (size_t)n_par * (4 + 4 + 4 + 16 + 64 + 8 + 16);
     double ncon = cpp * n_par * 1.34;
-    size_t conB = (size_t)(ncon * 2 * (8 + 8 + 4 + 4 + NSTATE * 8 + 64));
+    size_t conB = (size_t)(ncon * 2 * (8 + 8 + 4 + 4 + 4 + 4 + NSTATE * 8 + 32 + 64));
     size_t csrB = (size_t)(ncon * 2 * (4 + 1));
     size_t scratch = (size_t)n_par * 64 + (size_t)(ncon * 48);
     return nodeB + parB + conB + csrB + scratch + (1u << 20);
