@@ -104,13 +104,14 @@ def bench_batch(args, ctx):
     ctx.step(sids, args.timing_steps)
     kt = ctx.kernel_times()
     ctx.enable_timing(False)
-    n_con = n_real = 0
+    n_con = n_real = n_active = 0
     for s in sids:
         c = ctx.contacts(s)
         n_con += len(c["idA"])
         n_real += int(c["isReal"].sum())
+        n_active += ctx.contact_work(s)[0]
     ms_step = kt.get("contacts", (float("nan"), 0))[0] / args.timing_steps
-    alg_bytes = BYTES_REAL["mixed"] * n_real + BYTES_POT["mixed"] * (n_con - n_real)
+    alg_bytes = BYTES_REAL["mixed"] * n_real + BYTES_POT["mixed"] * max(n_active - n_real, 0)
     achieved = alg_bytes / (ms_step * 1e-3) / 1e9
     kernels = {k: {"ms_per_step": v[0] / args.timing_steps, "launches": v[1]} for k, v in kt.items()}
     log("kernels: %s" % json.dumps(kernels))
@@ -298,13 +299,14 @@ def main():
     c = run.contacts()
     n_con = len(c["idA"])
     n_real = int(c["isReal"].sum())
+    n_active, _ = ctx.contact_work(run.sid)  # contacts the kernel reads (dormant ones it never touches)
     ms, launches = kt.get("contacts", (float("nan"), 0))
     avg_ms = ms / max(launches, 1)
-    alg_bytes = BYTES_REAL[args.precision] * n_real + BYTES_POT[args.precision] * (n_con - n_real)
+    alg_bytes = BYTES_REAL[args.precision] * n_real + BYTES_POT[args.precision] * max(n_active - n_real, 0)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     kernels = {k: {"avg_ms": v[0] / max(v[1], 1), "launches": v[1]} for k, v in kt.items()}
     log("kernels: %s" % json.dumps(kernels))
-    log("contacts: %d potential, %d real" % (n_con, n_real))
+    log("contacts: %d potential, %d real, %d processed by the ContactLoop" % (n_con, n_real, n_active))
 
     mixed = None
     if not args.no_mixed and args.precision == "fp64":
@@ -353,7 +355,7 @@ def main():
         "data": "synthetic (seeded generator, relaxed on device)",
         "config": {"workload": "periodic Hertz, %d spheres, phi=0.60, lognormal r (median 1 mm, sigma .3), 1000-step run" % b.n_par,
                    "n_particles": b.n_par, "precision": args.precision, "dt": b.meta["dt"],
-                   "contacts_potential": n_con, "contacts_real": n_real,
+                   "contacts_potential": n_con, "contacts_real": n_real, "contacts_processed": n_active,
                    "full_collider_runs_in_timed_region": st1["nFullRuns"] - st0["nFullRuns"]},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)), "samples": len(lat)},
         "roofline": {"bound": "hbm", "kernel": "k_contacts", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

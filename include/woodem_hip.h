@@ -161,6 +161,11 @@ int wdm_download_nodes(wdm_ctx* ctx, int32_t scene, double* pos3, double* ori4, 
                        double* force3, double* torque3);
 /* number of contacts currently held (real + potential) */
 int wdm_num_contacts(wdm_ctx* ctx, int32_t scene, int64_t* n);
+/* the ContactLoop's work at the last full run: contacts it processes (`active`: all but the
+   dormant potential contacts, whose bounds overlap but whose surfaces stay more than
+   2 verletDist apart until the next full run) and state rows handed out (`rows`); no
+   counterpart in the reference, where every contact is visited */
+int wdm_contact_work(wdm_ctx* ctx, int32_t scene, int64_t* active, int64_t* rows);
 /* contacts sorted by (min id, max id); ids in functor order (Wall/Facet first, else lower id);
    any output pointer may be NULL */
 int wdm_get_contacts(wdm_ctx* ctx, int32_t scene, int64_t cap, int32_t* idA, int32_t* idB, int32_t* cellDist3,

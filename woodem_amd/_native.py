@@ -73,6 +73,7 @@ def lib():
         "wdm_step": (C.c_int, [P, pi, I32, I32]),
         "wdm_download_nodes": (C.c_int, [P, I32, pd, pd, pd, pd, pd, pd]),
         "wdm_num_contacts": (C.c_int, [P, I32, pl]),
+        "wdm_contact_work": (C.c_int, [P, I32, pl, pl]),
         "wdm_get_contacts": (C.c_int, [P, I32, I64, pi, pi, pi, C.POINTER(U8), pd, pd, pd, pd, pl]),
         "wdm_get_energy": (C.c_int, [P, I32, pd, pi]),
         "wdm_get_stats": (C.c_int, [P, I32, pl, pl, pd]),
@@ -191,6 +192,12 @@ class Context:
         k = nout.value
         return dict(idA=idA[:k], idB=idB[:k], cellDist=cd[:k], isReal=real[:k].astype(bool), forceLocal=fl[:k],
                     contPt=cp[:k], trsf=tr[:k].reshape(-1, 3, 3), uN=un[:k])
+
+    def contact_work(self, sid):
+        """(contacts the ContactLoop processes, state rows) at the last full run"""
+        a, r = C.c_int64(), C.c_int64()
+        self.check(self.L.wdm_contact_work(self.h, sid, C.byref(a), C.byref(r)))
+        return a.value, r.value
 
     def energy(self, sid):
         v = np.zeros(8)

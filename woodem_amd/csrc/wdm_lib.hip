@@ -1694,6 +1694,20 @@ int wdm_num_contacts(wdm_ctx* c, int32_t sid, int64_t* n) {
     });
 }
 
+int wdm_contact_work(wdm_ctx* c, int32_t sid, int64_t* active, int64_t* rows) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        int32_t w[2] = {0, 0};
+        const int32_t* pa = S.mixed() ? S.cb32[S.cur].nItems.p : S.cb64[S.cur].nItems.p;
+        const int32_t* pr = S.mixed() ? S.cb32[S.cur].counter.p : S.cb64[S.cur].counter.p;
+        if (pa) HIPCHK(hipMemcpyAsync(&w[0], pa, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        if (pr) HIPCHK(hipMemcpyAsync(&w[1], pr, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (active) *active = w[0];
+        if (rows) *rows = w[1];
+    });
+}
+
 int wdm_get_contacts(wdm_ctx* c, int32_t sid, int64_t cap, int32_t* idA, int32_t* idB, int32_t* cellDist3, uint8_t* isReal, double* forceLocal3,
                      double* contPt3, double* trsf9, double* uN, int64_t* nOut) {
     return guard(c, [&] {
