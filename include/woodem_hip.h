@@ -156,6 +156,15 @@ int wdm_contact_loop_run(wdm_ctx* ctx, int32_t scene);
 int wdm_step_end(wdm_ctx* ctx, int32_t scene);
 int wdm_step(wdm_ctx* ctx, const int32_t* scenes, int32_t n_scenes, int32_t n_steps);
 
+/* Launch-shape knobs of wdm_step (no counterpart in the reference; results do not depend
+   on them): `window` steps between two looks of the host at the Verlet triggers, launched
+   as one captured HIP graph when `graphs` is nonzero (1 = step by step); `cell_factor`
+   (>= 1) scales the collider's grid cell width over the largest sphere bound.  The getter
+   also returns how many windows ran, graphs were captured and graph launches were made. */
+int wdm_set_tuning(wdm_ctx* ctx, int32_t window, int32_t graphs, double cell_factor);
+int wdm_get_tuning(wdm_ctx* ctx, int32_t* window, int32_t* graphs, double* cell_factor, int64_t* windows,
+                   int64_t* graph_captures, int64_t* graph_launches);
+
 /* Readback (synchronous). */
 int wdm_download_nodes(wdm_ctx* ctx, int32_t scene, double* pos3, double* ori4, double* vel3, double* angVel3,
                        double* force3, double* torque3);

@@ -81,6 +81,8 @@ def lib():
         "wdm_sync": (C.c_int, [P]),
         "wdm_kernel_times": (C.c_int, [P, I32, C.c_char_p, pd, pl, pi]),
         "wdm_enable_timing": (C.c_int, [P, I32]),
+        "wdm_set_tuning": (C.c_int, [P, I32, I32, D]),
+        "wdm_get_tuning": (C.c_int, [P, pi, pi, pd, pl, pl, pl]),
         "wdm_relax_overlaps": (C.c_int, [P, I32, I32]),
     }
     for name, (res, args) in sig.items():
@@ -218,6 +220,24 @@ class Context:
 
     def sync(self):
         self.check(self.L.wdm_sync(self.h))
+
+    def set_tuning(self, window=None, graphs=None, cell_factor=None):
+        """Launch-shape knobs of step(): steps per window (1 = step by step), captured
+        graphs on/off, collider cell width factor (>= 1); None keeps the current value.
+        Results do not depend on them."""
+        t = self.tuning()
+        w = t["window"] if window is None else int(window)
+        g = t["graphs"] if graphs is None else int(bool(graphs))
+        f = t["cell_factor"] if cell_factor is None else float(cell_factor)
+        self.check(self.L.wdm_set_tuning(self.h, w, g, f))
+
+    def tuning(self):
+        w, g = C.c_int32(), C.c_int32()
+        f = C.c_double()
+        nw, nc, nl = C.c_int64(), C.c_int64(), C.c_int64()
+        self.check(self.L.wdm_get_tuning(self.h, C.byref(w), C.byref(g), C.byref(f), C.byref(nw), C.byref(nc), C.byref(nl)))
+        return dict(window=w.value, graphs=g.value, cell_factor=f.value, windows=nw.value, graph_captures=nc.value,
+                    graph_launches=nl.value)
 
     def enable_timing(self, on=True):
         self.check(self.L.wdm_enable_timing(self.h, int(on)))

@@ -203,6 +203,42 @@ struct SceneDesc {
     uint32_t* words;    // [0] Verlet trigger, [1] created energy slots, [2] device errors
     ClumpView cl;
 };
+
+// Several steps of a batch launched back to back without the host (one captured graph):
+// the per-step fields of each scene's StepArgs come from this table, one row per step.
+// A scene whose Verlet trigger fires stops there: its trigger word then holds a stamp at
+// or above `gate` (the stamp of the window's first step), its later steps in the window
+// do nothing, and the host runs its collider before it resumes.
+struct StepVar {
+    int64_t step;
+    const GP CellArgs* cell;
+    uint32_t stamp;  // stamp of this step
+    uint32_t gate;   // stamp of the window's first step
+    int32_t reallyTrackEnergy;
+    int32_t idle;    // the scene has no step at this position of the window
+};
+__device__ __forceinline__ uint32_t loadWord(const uint32_t* w) {
+    return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Leapfrog side of a step (clump sums, Leapfrog, clump members, their energies): runs unless
+// the trigger fired at an earlier step of the window.  The word only moves from below the
+// gate to this step's stamp while the step runs, so every block decides alike.
+__device__ __forceinline__ bool leapActive(const StepVar& v, const uint32_t* words) {
+    if (v.idle) return false;
+    const uint32_t w = loadWord(words);
+    return w < v.gate || w == v.stamp;
+}
+// ContactLoop side: runs when no trigger fired in the window up to and including this step
+__device__ __forceinline__ bool contactActive(const StepVar& v, const uint32_t* words) {
+    return !v.idle && loadWord(words) < v.gate;
+}
+__device__ __forceinline__ void applyStepVar(StepArgs& a, const StepVar& v) {
+    a.step = v.step;
+    a.cell = v.cell;
+    a.stamp = v.stamp;
+    a.reallyTrackEnergy = v.reallyTrackEnergy;
+}
+
 template <typename T>
 struct ConDesc {
     ConView<T> cv;      // current contact list
@@ -327,11 +363,17 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_gather(const SceneDesc* D, const 
 // last ContactLoop has not been materialised) the node's force and torque are summed
 // from the contacts of that ContactLoop here instead of being read from the arrays.
 template <int GATHER>
-__global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(const SceneDesc* D, const int32_t* idx, Map m, double* epart) {
+__global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(const SceneDesc* D, const int32_t* idx, Map m, double* epart, const StepVar* sv) {
     int64_t lb;
     const int t = mapBlock(m, lb);
-    const SceneDesc& S = D[idx ? idx[t] : t];
-    const StepArgs a = S.a;
+    const int di = idx ? idx[t] : t;
+    const SceneDesc& S = D[di];
+    StepArgs a = S.a;
+    if (sv) {
+        const StepVar v = sv[di];
+        if (!leapActive(v, S.words)) return;
+        applyStepVar(a, v);
+    }
     const NodeView n = S.n;
     const PartView p = S.p;
     const GatherView gv = S.gv;
@@ -472,11 +514,17 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(const SceneDesc* D, cons
 // ClumpData::forceTorqueFromMembers, pkg/dem/Clump.cpp:265-273, one thread per clump before
 // Leapfrog: the master's own force and torque, then each member's added in member order;
 // only for masters whose Leapfrog reads them (free DOFs, or damping dissipation tracked)
-__global__ void __launch_bounds__(WDM_BLOCK) k_clump_forces(const SceneDesc* D, const int32_t* idx, Map m) {
+__global__ void __launch_bounds__(WDM_BLOCK) k_clump_forces(const SceneDesc* D, const int32_t* idx, Map m, const StepVar* sv) {
     int64_t lb;
     const int t = mapBlock(m, lb);
-    const SceneDesc& S = D[idx ? idx[t] : t];
-    const StepArgs a = S.a;
+    const int di = idx ? idx[t] : t;
+    const SceneDesc& S = D[di];
+    StepArgs a = S.a;
+    if (sv) {
+        const StepVar v = sv[di];
+        if (!leapActive(v, S.words)) return;
+        applyStepVar(a, v);
+    }
     const NodeView n = S.n;
     const PartView p = S.p;
     const GatherView gv = S.gv;
@@ -526,11 +574,17 @@ __device__ __forceinline__ D3 qrot4(const double4& q, const D3& v) {
 // ClumpData::applyToMembers, pkg/dem/Clump.cpp:276-289, after Leapfrog moved the masters,
 // then the collider's Verlet displacement check (InsertionSortCollider.cpp:340-350) on the
 // members' new positions
-__global__ void __launch_bounds__(WDM_BLOCK) k_clump_members(const SceneDesc* D, const int32_t* idx, Map m) {
+__global__ void __launch_bounds__(WDM_BLOCK) k_clump_members(const SceneDesc* D, const int32_t* idx, Map m, const StepVar* sv) {
     int64_t lb;
     const int t = mapBlock(m, lb);
-    const SceneDesc& S = D[idx ? idx[t] : t];
-    const StepArgs a = S.a;
+    const int di = idx ? idx[t] : t;
+    const SceneDesc& S = D[di];
+    StepArgs a = S.a;
+    if (sv) {
+        const StepVar v = sv[di];
+        if (!leapActive(v, S.words)) return;
+        applyStepVar(a, v);
+    }
     const NodeView n = S.n;
     const ClumpView cl = S.cl;
     const int k = (int)(lb * WDM_BLOCK + threadIdx.x);
@@ -562,21 +616,25 @@ __device__ __forceinline__ int energySlot(int nv, int k) {
     const int s[5] = {E_ELAST, E_PLAST, E_VISCN, E_VISCT, E_BROKEN};
     return s[k];
 }
-__device__ __forceinline__ uint32_t loadWord(const uint32_t* w) {
-    return __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 // one block per (scene, slot); desc[t * stride] (or t when desc is null) is the SceneDesc
-// of the launch's t-th scene; spec: skip scenes whose Verlet trigger fired this step
+// of the launch's t-th scene; spec: skip scenes whose Verlet trigger fired this step; sv:
+// the step's row of a window (gated like the kernel whose partials are summed)
 __global__ void __launch_bounds__(WDM_BLOCK) k_energy_finalize(const SceneDesc* D, const int32_t* desc, int stride, Map m,
-                                                              const double* part, int nv, int spec) {
+                                                              const double* part, int nv, int spec, const StepVar* sv) {
     __shared__ double sh[WDM_BLOCK];
     const int t = blockIdx.x / nv, k = blockIdx.x % nv;
-    const SceneDesc& S = D[desc ? desc[(size_t)t * stride] : t];
+    const int di = desc ? desc[(size_t)t * stride] : t;
+    const SceneDesc& S = D[di];
     const bool leap = nv == 3;
     if (!S.a.trackEnergy) return;
-    if (spec && loadWord(S.words) == S.a.stamp) return;
+    int really = S.a.reallyTrackEnergy;
+    if (sv) {
+        const StepVar v = sv[di];
+        if (!(leap ? leapActive(v, S.words) : contactActive(v, S.words))) return;
+        really = v.reallyTrackEnergy;
+    } else if (spec && loadWord(S.words) == S.a.stamp) return;
     const int slot = energySlot(nv, k);
-    if (leap && !S.a.reallyTrackEnergy) {
+    if (leap && !really) {
         if (slot == E_KIN && threadIdx.x == 0) S.E[E_KIN] = 0.;
         return;
     }
@@ -1308,19 +1366,24 @@ template <typename T> __device__ __forceinline__ void trSetCol(T* tr, int j, con
 #define G_WS 2
 #define G_FS 4
 template <typename T, int LAW, int GEOM>
-__global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, const ConDesc<T>* C, Map m, double* epart, int spec) {
+__global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, const ConDesc<T>* C, Map m, double* epart, int spec,
+                                                       const StepVar* sv) {
     int64_t lb;
     const int tt = mapBlock(m, lb);
     const ConDesc<T>& CD = C[tt];
     const SceneDesc& S = D[CD.desc];
-    const StepArgs a = S.a;
+    StepArgs a = S.a;
     const NodeView n = S.n;
     const PartView p = S.p;
     const ConView<T> cv = CD.cv;
     uint32_t* err = S.words + 2;
     uint32_t* creOut = S.words + 1;
     uint64_t* realMask = CD.realMask;
-    if (spec && loadWord(S.words) == a.stamp) return;  // the collider runs first: launched again afterwards
+    if (sv) {  // a step of a window: stopped scenes skip it
+        const StepVar v = sv[CD.desc];
+        if (!contactActive(v, S.words)) return;
+        applyStepVar(a, v);
+    } else if (spec && loadWord(S.words) == a.stamp) return;  // the collider runs first: launched again afterwards
     // work list: dormant contacts are not in it; contacts with a state row come first, as
     // item == row, so their state is read and written contiguously
     const int64_t nItems = *cv.nItems;

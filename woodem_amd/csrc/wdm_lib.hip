@@ -91,6 +91,11 @@ struct Ctx {
     Timing timing;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Shared* sh = nullptr;  // batch scratch, descriptor tables, per-scene words
+    // launch-shape knobs (results do not depend on them): steps per window, windows replayed
+    // as captured graphs, collider cell width over the largest sphere bound
+    int window = 16;
+    int graphs = 1;
+    double cellFactor = 1.;
 };
 
 void* devAlloc(Ctx* c, size_t bytes) {
@@ -222,7 +227,7 @@ struct Shared {
     std::vector<int32_t> freeSlots;
     int32_t nslots = 0;
     Staged stDesc, stMapNode, stMapCon, stCon, stFR, stConFR, stIdx;
-    Staged stMapP, stMapB, stMapC, stMapO, stMapN, stMapE, stMapP1, stMapCl, stMapClF, stMapQ;
+    Staged stMapP, stMapB, stMapC, stMapO, stMapN, stMapE, stMapP1, stMapQ;
     hipEvent_t evWords = nullptr;  // recorded after the copy of the words to the host
     // energy partials per block
     DBuf<double> epartL, epartC;
@@ -239,13 +244,27 @@ struct Shared {
     DBuf<int64_t> lcnt, loffs;
     DBuf<uint8_t> temp;
     DBuf<double4> disp;
+    // window of steps: its tables (device and pinned staging) and the last captured graph
+    DBuf<uint8_t> winDev;
+    uint8_t* winHost = nullptr;
+    size_t winHostCap = 0;
+    hipGraphExec_t winExec = nullptr;
+    std::vector<uint64_t> winKey;
+    int64_t graphLaunches = 0, graphCaptures = 0, windows = 0;
     void freeAll(Ctx* c) {
+        release(c, winDev);
+        if (winHost) (void)hipHostFree(winHost);
+        winHost = nullptr;
+        winHostCap = 0;
+        if (winExec) (void)hipGraphExecDestroy(winExec);
+        winExec = nullptr;
+        winKey.clear();
         release(c, words);
         if (hostWords) (void)hipHostFree(hostWords);
         hostWords = nullptr;
         hostWordsCap = 0;
         for (Staged* s : {&stDesc, &stMapNode, &stMapCon, &stCon, &stFR, &stConFR, &stIdx, &stMapP, &stMapB, &stMapC, &stMapO, &stMapN,
-                          &stMapE, &stMapP1, &stMapCl, &stMapClF, &stMapQ})
+                          &stMapE, &stMapP1, &stMapQ})
             s->free(c);
         release(c, epartL); release(c, epartC); release(c, extPart); release(c, extOut);
         release(c, ckeys); release(c, ckeys2); release(c, cvals); release(c, cvals2); release(c, cstart); release(c, cend);
@@ -600,89 +619,98 @@ void readWords(Ctx* c, const std::vector<Scene*>& ss) {
 }
 
 // ---------------- Leapfrog::run (pkg/dem/Leapfrog.cpp:103-255) ----------------
-struct Leapfrog {
-    // uploads the batch's descriptors; one launch for the scenes whose forces are gathered
-    // from their last ContactLoop, one for the others
-    static void run(Batch& B) {
+// The tables of one Leapfrog run over a batch (scenes per gather mode, clump scenes,
+// block prefixes), packed into a Blob; launched once per step, or once per step of a
+// window with that step's StepVar rows.
+struct LeapPlan {
+    std::vector<int32_t> idx[2], cfIdx, cmIdx;
+    std::vector<int64_t> bp[2], cfBp, cmBp;
+    size_t oIdx[2] = {0, 0}, oBp[2] = {0, 0}, oCfIdx = 0, oCfBp = 0, oCmIdx = 0, oCmBp = 0;
+    bool anyEnergy = false;
+    const uint8_t* base = nullptr;
+
+    void build(Batch& B, Blob& b) {
         Ctx* c = B.c;
-        Shared& X = *c->sh;
         const int n = B.n();
-        bool anyEnergy = false;
-        std::vector<int32_t> idx[2];
         for (int i = 0; i < n; i++) {
             Scene* S = B.ss[i];
             resolveVerlet(*S);
             S->lastLeapfrogVerletChecked = (!S->needFull && S->verletDist > 0);
             anyEnergy |= S->P.trackEnergy != 0;
             idx[S->forcesFromContacts ? 1 : 0].push_back(i);
+            // clump masters' sums only where Leapfrog reads them (free DOFs, damping dissipation)
+            if (S->nClumps > 0 && (S->clumpFree || (S->P.trackEnergy && S->P.damping != 0))) cfIdx.push_back(i);
+            if (S->nMembers > 0) cmIdx.push_back(i);
         }
-        B.upload();
-        // clump masters' force and torque with their members' (read by the Leapfrog below)
-        {
-            std::vector<int32_t> cidx;
-            for (int i = 0; i < n; i++) {  // only where Leapfrog reads the sums
-                const Scene& S = *B.ss[i];
-                if (S.nClumps > 0 && (S.clumpFree || (S.P.trackEnergy && S.P.damping != 0))) cidx.push_back(i);
-            }
-            if (!cidx.empty()) {
-                std::vector<int64_t> cbp = blockPrefix((int)cidx.size(), [&](int t) { return B.ss[cidx[t]]->nClumps; });
-                Blob cb;
-                size_t ci = cb.addv(cidx), cbo = cb.addv(cbp);
-                uint8_t* cbase = X.stMapClF.upload(c, cb.b.data(), cb.b.size());
-                Map cm;
-                cm.bprefix = (const int64_t*)(cbase + cbo);
-                cm.n = (int)cidx.size();
-                hipLaunchKernelGGL(k_clump_forces, dim3((unsigned)cbp.back()), dim3(WDM_BLOCK), 0, c->stream, B.D, (const int32_t*)(cbase + ci), cm);
-                checkLaunch();
-            }
-        }
-        Blob b;
-        size_t oi[2], ob[2];
-        std::vector<int64_t> bp[2];
         for (int g = 0; g < 2; g++) {
             bp[g] = blockPrefix((int)idx[g].size(), [&](int t) { return B.ss[idx[g][t]]->nn; });
-            oi[g] = b.addv(idx[g]);
-            ob[g] = b.addv(bp[g]);
+            oIdx[g] = b.addv(idx[g]);
+            oBp[g] = b.addv(bp[g]);
         }
-        uint8_t* base = X.stMapNode.upload(c, b.b.data(), b.b.size());
-        const int64_t nb0 = bp[0].back(), nb1 = bp[1].back();
-        ensure(c, X.epartL, (size_t)(nb0 + nb1) * 3 + 3);
+        cfBp = blockPrefix((int)cfIdx.size(), [&](int t) { return B.ss[cfIdx[t]]->nClumps; });
+        cmBp = blockPrefix((int)cmIdx.size(), [&](int t) { return B.ss[cmIdx[t]]->nMembers; });
+        oCfIdx = b.addv(cfIdx); oCfBp = b.addv(cfBp);
+        oCmIdx = b.addv(cmIdx); oCmBp = b.addv(cmBp);
+        ensure(c, c->sh->epartL, (size_t)(bp[0].back() + bp[1].back()) * 3 + 3);
+    }
+    Map map(size_t o, size_t n) const {
+        Map m;
+        m.bprefix = (const int64_t*)(base + o);
+        m.n = (int)n;
+        return m;
+    }
+    void sig(std::vector<uint64_t>& k) const {
+        for (int g = 0; g < 2; g++) k.insert(k.end(), {idx[g].size(), (uint64_t)bp[g].back(), oIdx[g], oBp[g]});
+        k.insert(k.end(), {cfIdx.size(), (uint64_t)cfBp.back(), oCfIdx, oCfBp, cmIdx.size(), (uint64_t)cmBp.back(), oCmIdx, oCmBp,
+                           (uint64_t)anyEnergy});
+    }
+    void launch(Ctx* c, const SceneDesc* D, const StepVar* sv) const {
+        Shared& X = *c->sh;
+        hipStream_t st = c->stream;
+        // clump masters' force and torque with their members' (read by the Leapfrog below)
+        if (cfBp.back() > 0) {
+            hipLaunchKernelGGL(k_clump_forces, dim3((unsigned)cfBp.back()), dim3(WDM_BLOCK), 0, st, D, (const int32_t*)(base + oCfIdx),
+                               map(oCfBp, cfIdx.size()), sv);
+            checkLaunch();
+        }
+        const int64_t nb0 = bp[0].back();
         for (int g = 0; g < 2; g++) {
             const int ng = (int)idx[g].size();
             if (ng == 0) continue;
-            Map m;
-            m.bprefix = (const int64_t*)(base + ob[g]);
-            m.n = ng;
-            const int32_t* id = (const int32_t*)(base + oi[g]);
+            const Map m = map(oBp[g], ng);
+            const int32_t* id = (const int32_t*)(base + oIdx[g]);
             double* ep = X.epartL.p + (g ? 3 * nb0 : 0);
             const int64_t nbk = bp[g].back();
             if (nbk > 0) {
                 KTimer t(c, "leapfrog");
-                if (g) hipLaunchKernelGGL(k_leapfrog<1>, dim3((unsigned)nbk), dim3(WDM_BLOCK), 0, c->stream, B.D, id, m, ep);
-                else hipLaunchKernelGGL(k_leapfrog<0>, dim3((unsigned)nbk), dim3(WDM_BLOCK), 0, c->stream, B.D, id, m, ep);
+                if (g) hipLaunchKernelGGL(k_leapfrog<1>, dim3((unsigned)nbk), dim3(WDM_BLOCK), 0, st, D, id, m, ep, sv);
+                else hipLaunchKernelGGL(k_leapfrog<0>, dim3((unsigned)nbk), dim3(WDM_BLOCK), 0, st, D, id, m, ep, sv);
                 checkLaunch();
             }
             if (anyEnergy) {
-                hipLaunchKernelGGL(k_energy_finalize, dim3(3 * ng), dim3(WDM_BLOCK), 0, c->stream, B.D, id, 1, m, (const double*)ep, 3, 0);
+                hipLaunchKernelGGL(k_energy_finalize, dim3(3 * ng), dim3(WDM_BLOCK), 0, st, D, id, 1, m, (const double*)ep, 3, 0, sv);
                 checkLaunch();
             }
         }
         // clump members follow their masters (ClumpData::applyToMembers)
-        std::vector<int32_t> cidx;
-        for (int i = 0; i < n; i++)
-            if (B.ss[i]->nMembers > 0) cidx.push_back(i);
-        if (!cidx.empty()) {
-            std::vector<int64_t> cbp = blockPrefix((int)cidx.size(), [&](int t) { return B.ss[cidx[t]]->nMembers; });
-            Blob cb;
-            size_t oi = cb.addv(cidx), ob = cb.addv(cbp);
-            uint8_t* cbase = X.stMapCl.upload(c, cb.b.data(), cb.b.size());
-            Map m;
-            m.bprefix = (const int64_t*)(cbase + ob);
-            m.n = (int)cidx.size();
+        if (cmBp.back() > 0) {
             KTimer t(c, "clump_members");
-            hipLaunchKernelGGL(k_clump_members, dim3((unsigned)cbp.back()), dim3(WDM_BLOCK), 0, c->stream, B.D, (const int32_t*)(cbase + oi), m);
+            hipLaunchKernelGGL(k_clump_members, dim3((unsigned)cmBp.back()), dim3(WDM_BLOCK), 0, st, D, (const int32_t*)(base + oCmIdx),
+                               map(oCmBp, cmIdx.size()), sv);
             checkLaunch();
         }
+    }
+};
+struct Leapfrog {
+    // uploads the batch's descriptors and the plan's tables, then launches
+    static void run(Batch& B) {
+        Ctx* c = B.c;
+        LeapPlan lp;
+        Blob b;
+        lp.build(B, b);
+        B.upload();
+        lp.base = c->sh->stMapNode.upload(c, b.b.data(), b.b.size());
+        lp.launch(c, B.D, nullptr);
     }
 };
 
@@ -691,7 +719,7 @@ struct InsertionSortCollider {
     // Grids of the scenes L[t] from their bound extents (one synchronisation), then the
     // bound entries of all of them sorted by (scene, cell) with each cell's entry range.
     static void buildGrids(Batch& B, const std::vector<int>& L, std::vector<FRScene>& F, const std::vector<int64_t>& bpB, const Map& mB,
-                           const FRScene*& FRd, int64_t& nbTot) {
+                           const FRScene*& FRd, int64_t& nbTot, double cellFactor) {
         Ctx* c = B.c;
         Shared& X = *c->sh;
         hipStream_t st = c->stream;
@@ -723,7 +751,7 @@ struct InsertionSortCollider {
                 // cells fit the spheres; larger bounds (facets) are listed apart
                 double ext = e[7] > 0 ? e[7] : e[6];
                 if (!(ext > 0) || !std::isfinite(ext)) ext = 1e-12;
-                double w = ext * 1.001;
+                double w = ext * 1.001 * cellFactor;
                 const int64_t maxCells = std::max<int64_t>(4096, 2 * (int64_t)nb);
                 for (int it = 0; it < 200; it++) {
                     int64_t tot = 1;
@@ -834,7 +862,7 @@ struct InsertionSortCollider {
             checkLaunch();
         }
         int64_t nbTot = 0;
-        buildGrids(B, L, F, bpB, mB, FRd, nbTot);
+        buildGrids(B, L, F, bpB, mB, FRd, nbTot, c->cellFactor);
         bool anyFacets = false;
         for (int t = 0; t < nT; t++) anyFacets |= S_(t).hasFacets;
         const bool large = anyFacets && listLarge(B, bpB, mB, FRd, nbTot, F[nT - 1].cellBase + F[nT - 1].nCells + 1);
@@ -1025,23 +1053,31 @@ template <>
 ConBuf<float>* InsertionSortCollider::bufs<float>(Scene& S) { return S.cb32; }
 
 // ---------------- ContactLoop::run (pkg/dem/ContactLoop.cpp:93-259) ----------------
-struct ContactLoop {
-    // spec: launched before the collider's decision is known; a scene whose Verlet trigger
-    // fired skips it and is launched again after its full run
-    template <typename T, int LAW, int GEOM>
-    static void launch(Batch& B, const ConDesc<T>* Cd, const Map& m, int64_t blocks, bool spec) {
-        hipLaunchKernelGGL((k_contacts<T, LAW, GEOM>), dim3((unsigned)blocks), dim3(WDM_BLOCK), 0, B.c->stream, B.D, Cd, m,
-                           B.c->sh->epartC.p, spec ? 1 : 0);
-    }
+// Scenes grouped by precision, law and the geometry functors they need, one launch per
+// group; the groups' ConDesc tables and block prefixes packed into a Blob.
+struct ContactPlan {
+    struct Group {
+        int mixed, law, geom, n;
+        size_t oC, oBp;
+        int64_t blocks;
+        bool anyEnergy;
+    };
+    std::vector<Group> gs;
+    const uint8_t* base = nullptr;
+
     template <typename T>
-    static void runGroup(Batch& B, const std::vector<int>& L, int law, int geom, bool spec) {
+    void addGroup(Batch& B, const std::vector<int>& L, int law, int geom, Blob& b) {
         Ctx* c = B.c;
-        Shared& X = *c->sh;
         const int n = (int)L.size();
         if (n == 0) return;
         std::vector<ConDesc<T>> C(n);
         std::memset(C.data(), 0, C.size() * sizeof(ConDesc<T>));
-        bool anyEnergy = false;
+        Group g;
+        g.mixed = sizeof(T) == 4;
+        g.law = law;
+        g.geom = geom;
+        g.n = n;
+        g.anyEnergy = false;
         for (int t = 0; t < n; t++) {
             Scene& S = *B.ss[L[t]];
             ConBuf<T>* cb = InsertionSortCollider::bufs<T>(S);
@@ -1051,48 +1087,79 @@ struct ContactLoop {
             C[t].nc = S.nc;
             C[t].realMask = gpc(S.realMask.p);
             C[t].desc = L[t];
-            anyEnergy |= S.P.trackEnergy != 0;
+            g.anyEnergy |= S.P.trackEnergy != 0;
         }
         std::vector<int64_t> bp = blockPrefix(n, [&](int t) { return C[t].nc; });
-        Blob b;
-        size_t oc = b.addv(C), om = b.addv(bp);
-        uint8_t* base = X.stCon.upload(c, b.b.data(), b.b.size());
-        const ConDesc<T>* Cd = (const ConDesc<T>*)(base + oc);
-        Map m;
-        m.bprefix = (const int64_t*)(base + om);
-        m.n = n;
-        ensure(c, X.epartC, (size_t)bp[n] * 5 + 5);
-        if (bp[n] > 0) {
-            KTimer tm(c, "contacts");
-            const bool h = law == WDM_LAW_HERTZ_DMT;
-            if (geom == G_SS) { if (h) launch<T, 1, G_SS>(B, Cd, m, bp[n], spec); else launch<T, 0, G_SS>(B, Cd, m, bp[n], spec); }
-            else if (geom == (G_SS | G_WS)) { if (h) launch<T, 1, G_SS | G_WS>(B, Cd, m, bp[n], spec); else launch<T, 0, G_SS | G_WS>(B, Cd, m, bp[n], spec); }
-            else { if (h) launch<T, 1, G_SS | G_WS | G_FS>(B, Cd, m, bp[n], spec); else launch<T, 0, G_SS | G_WS | G_FS>(B, Cd, m, bp[n], spec); }
-            checkLaunch();
-        }
-        if (anyEnergy) {
-            const int32_t* descs = (const int32_t*)((const uint8_t*)Cd + offsetof(ConDesc<T>, desc));
-            hipLaunchKernelGGL(k_energy_finalize, dim3(5 * n), dim3(WDM_BLOCK), 0, c->stream, B.D, descs, (int)(sizeof(ConDesc<T>) / 4), m,
-                               (const double*)X.epartC.p, 5, spec ? 1 : 0);
-            checkLaunch();
-        }
+        g.oC = b.addv(C);
+        g.oBp = b.addv(bp);
+        g.blocks = bp[n];
+        ensure(c, c->sh->epartC, (size_t)bp[n] * 5 + 5);
+        gs.push_back(g);
     }
-    static void runList(Batch& B, const std::vector<int>& L, bool spec) {
+    void build(Batch& B, const std::vector<int>& L, Blob& b) {
         std::vector<int> g[2][2][3];  // [mixed][law][geometry: spheres, + walls, + facets]
         for (int i : L) {
             Scene& S = *B.ss[i];
             if (S.needFull) throw WdmError(WDM_E_STATE, "ContactLoop before the collider's first run");
             g[S.mixed() ? 1 : 0][S.P.law == WDM_LAW_HERTZ_DMT ? 1 : 0][S.hasFacets ? 2 : (S.unbH.empty() ? 0 : 1)].push_back(i);
         }
-        if (L.empty()) return;
-        B.ensureDesc();
         const int geoms[3] = {G_SS, G_SS | G_WS, G_SS | G_WS | G_FS};
         for (int q = 0; q < 3; q++) {
-            runGroup<double>(B, g[0][0][q], WDM_LAW_IDEALELPL, geoms[q], spec);
-            runGroup<double>(B, g[0][1][q], WDM_LAW_HERTZ_DMT, geoms[q], spec);
-            runGroup<float>(B, g[1][0][q], WDM_LAW_IDEALELPL, geoms[q], spec);
-            runGroup<float>(B, g[1][1][q], WDM_LAW_HERTZ_DMT, geoms[q], spec);
+            addGroup<double>(B, g[0][0][q], WDM_LAW_IDEALELPL, geoms[q], b);
+            addGroup<double>(B, g[0][1][q], WDM_LAW_HERTZ_DMT, geoms[q], b);
+            addGroup<float>(B, g[1][0][q], WDM_LAW_IDEALELPL, geoms[q], b);
+            addGroup<float>(B, g[1][1][q], WDM_LAW_HERTZ_DMT, geoms[q], b);
         }
+    }
+    void sig(std::vector<uint64_t>& k) const {
+        for (const Group& g : gs)
+            k.insert(k.end(), {(uint64_t)g.mixed, (uint64_t)g.law, (uint64_t)g.geom, (uint64_t)g.n, g.oC, g.oBp, (uint64_t)g.blocks,
+                               (uint64_t)g.anyEnergy});
+    }
+    template <typename T, int LAW, int GEOM>
+    void launchT(Ctx* c, const SceneDesc* D, const Group& g, int spec, const StepVar* sv) const {
+        Map m;
+        m.bprefix = (const int64_t*)(base + g.oBp);
+        m.n = g.n;
+        const ConDesc<T>* Cd = (const ConDesc<T>*)(base + g.oC);
+        if (g.blocks > 0) {
+            KTimer tm(c, "contacts");
+            hipLaunchKernelGGL((k_contacts<T, LAW, GEOM>), dim3((unsigned)g.blocks), dim3(WDM_BLOCK), 0, c->stream, D, Cd, m,
+                               c->sh->epartC.p, spec, sv);
+            checkLaunch();
+        }
+        if (g.anyEnergy) {
+            const int32_t* descs = (const int32_t*)((const uint8_t*)Cd + offsetof(ConDesc<T>, desc));
+            hipLaunchKernelGGL(k_energy_finalize, dim3(5 * g.n), dim3(WDM_BLOCK), 0, c->stream, D, descs, (int)(sizeof(ConDesc<T>) / 4), m,
+                               (const double*)c->sh->epartC.p, 5, spec, sv);
+            checkLaunch();
+        }
+    }
+    template <typename T, int LAW>
+    void launchG(Ctx* c, const SceneDesc* D, const Group& g, int spec, const StepVar* sv) const {
+        if (g.geom == G_SS) launchT<T, LAW, G_SS>(c, D, g, spec, sv);
+        else if (g.geom == (G_SS | G_WS)) launchT<T, LAW, G_SS | G_WS>(c, D, g, spec, sv);
+        else launchT<T, LAW, G_SS | G_WS | G_FS>(c, D, g, spec, sv);
+    }
+    // spec: launched before the collider's decision is known; a scene whose Verlet trigger
+    // fired skips it and is launched again after its full run
+    void launch(Ctx* c, const SceneDesc* D, bool spec, const StepVar* sv) const {
+        for (const Group& g : gs) {
+            const bool h = g.law == WDM_LAW_HERTZ_DMT;
+            if (g.mixed) { if (h) launchG<float, 1>(c, D, g, spec, sv); else launchG<float, 0>(c, D, g, spec, sv); }
+            else { if (h) launchG<double, 1>(c, D, g, spec, sv); else launchG<double, 0>(c, D, g, spec, sv); }
+        }
+    }
+};
+struct ContactLoop {
+    static void runList(Batch& B, const std::vector<int>& L, bool spec) {
+        if (L.empty()) return;
+        ContactPlan cp;
+        Blob b;
+        cp.build(B, L, b);
+        B.ensureDesc();
+        cp.base = B.c->sh->stCon.upload(B.c, b.b.data(), b.b.size());
+        cp.launch(B.c, B.D, spec, nullptr);
     }
     static void done(Batch& B) {
         for (Scene* S : B.ss) {
@@ -1147,6 +1214,177 @@ void doOneStep(Batch& B) {
     ContactLoop::runList(B, full, false);
     ContactLoop::done(B);
     for (Scene* S : B.ss) stepEnd(*S);
+}
+
+// ---------------- windows: several steps launched together ----------------
+// Between two collider runs a scene's step is Leapfrog then ContactLoop with no host
+// decision, so K steps of a batch are launched back to back (one captured graph, replayed
+// while the batch's tables keep their shape) and the host looks at the Verlet triggers
+// once per window.  A scene whose trigger fires stops at that step (StepVar gate); after
+// the window the host finishes those steps (collider, ContactLoop) for all stopped
+// scenes in one batched pass.  Each scene runs the same kernels on the same data in the
+// same order as step by step, so its trajectory does not change.
+
+// cell state before a step's prologue (periodic scenes), to rewind a scene that stopped
+struct CellSnap {
+    HM3 hSize, gradV, pprevHsize, trsf, hSizeCL;
+    double size[3], spinVec[3];
+};
+CellSnap snapCell(const Scene& S) {
+    CellSnap z;
+    z.hSize = S.hSize; z.gradV = S.gradV; z.pprevHsize = S.pprevHsize; z.trsf = S.trsf; z.hSizeCL = S.hSizeCL;
+    for (int k = 0; k < 3; k++) { z.size[k] = S.size[k]; z.spinVec[k] = S.spinVec[k]; }
+    return z;
+}
+void restoreCell(Scene& S, const CellSnap& z) {
+    S.hSize = z.hSize; S.gradV = z.gradV; S.pprevHsize = z.pprevHsize; S.trsf = z.trsf; S.hSizeCL = z.hSizeCL;
+    for (int k = 0; k < 3; k++) { S.size[k] = z.size[k]; S.spinVec[k] = z.spinVec[k]; }
+}
+
+// a scene can join a window of K steps: collider state current, forces from its contacts,
+// a Verlet distance, and stamps that do not wrap inside the window
+bool windowable(Scene& S, int K) {
+    resolveVerlet(S);
+    if (S.needFull || !(S.verletDist > 0) || !S.forcesFromContacts) return false;
+    return (S.step % 0x7fffffffLL) + K < 0x7fffffffLL;
+}
+
+// K steps of the scenes ss (scene i has left[i] >= 1 steps to go); returns the steps each
+// completed
+std::vector<int64_t> runWindow(Ctx* c, const std::vector<Scene*>& ss, const std::vector<int64_t>& left, int K) {
+    Shared& X = *c->sh;
+    hipStream_t st = c->stream;
+    const int n = (int)ss.size();
+    Batch B(c, ss);
+    for (Scene* S : ss) checkReady(*S);
+    LeapPlan lp;
+    ContactPlan cp;
+    Blob b;
+    lp.build(B, b);  // sets the Verlet check flag of every scene (all are checked here)
+    std::vector<int> all(n);
+    for (int i = 0; i < n; i++) all[i] = i;
+    cp.build(B, all, b);
+    // descriptors from the state at the window's start; the per-step fields come from the rows
+    std::vector<SceneDesc> d(n);
+    for (int i = 0; i < n; i++) d[i] = makeDesc(*ss[i]);
+    std::vector<int64_t> s0(n);
+    std::vector<uint32_t> gate(n);
+    for (int i = 0; i < n; i++) { s0[i] = ss[i]->step; gate[i] = stampOf(ss[i]->step); }
+    // the host side of each step (cell prologue, cell of the ContactLoop, epilogue), run ahead
+    std::vector<StepVar> sv((size_t)K * n);
+    std::memset(sv.data(), 0, sv.size() * sizeof(StepVar));
+    std::vector<CellArgs> cells;
+    std::vector<size_t> cellOf((size_t)K * n, 0), cell0(n, 0);
+    std::vector<std::vector<CellSnap>> snaps(n);
+    for (int j = 0; j < K; j++) {
+        for (int i = 0; i < n; i++) {
+            Scene& S = *ss[i];
+            StepVar& v = sv[(size_t)j * n + i];
+            v.gate = gate[i];
+            if (j >= left[i]) { v.idle = 1; continue; }
+            if (S.periodic) snaps[i].push_back(snapCell(S));
+            stepBeginCell(S);
+            if (S.periodic || j == 0) {
+                CellArgs ca;
+                makeCellArgs(S, ca);
+                cells.push_back(ca);
+            }
+            cellOf[(size_t)j * n + i] = (S.periodic || j == 0) ? cells.size() - 1 : cellOf[i];
+            if (j == 0) cell0[i] = cells.size() - 1;
+            v.step = S.step;
+            v.stamp = stampOf(S.step);
+            v.reallyTrackEnergy = S.P.trackEnergy && (!S.periodic || S.step > 0 || hequal(S.gradV, heye()));
+            S.hSizeCL = S.hSize;  // ContactLoop::done
+            stepEnd(S);
+        }
+    }
+    const size_t oD = b.addv(d), oCells = b.addv(cells), oSV = b.addv(sv);
+    const size_t bytes = b.b.size();
+    ensure(c, X.winDev, bytes);
+    if (X.winHostCap < bytes) {
+        HIPCHK(hipStreamSynchronize(st));
+        if (X.winHost) (void)hipHostFree(X.winHost);
+        X.winHost = nullptr;
+        X.winHostCap = 0;
+        size_t cap = bytes + bytes / 2;
+        HIPCHK(hipHostMalloc((void**)&X.winHost, cap, hipHostMallocDefault));
+        X.winHostCap = cap;
+    }
+    uint8_t* base = X.winDev.p;
+    const CellArgs* cellsD = (const CellArgs*)(base + oCells);
+    SceneDesc* dh = b.at<SceneDesc>(oD);
+    for (int i = 0; i < n; i++) dh[i].a.cell = gpc(cellsD + cell0[i]);
+    StepVar* svh = b.at<StepVar>(oSV);
+    for (size_t k = 0; k < (size_t)K * n; k++)
+        if (!svh[k].idle) svh[k].cell = gpc(cellsD + cellOf[k]);
+    std::memcpy(X.winHost, b.b.data(), bytes);  // the previous window has finished (synchronised below)
+    HIPCHK(hipMemcpyAsync(base, X.winHost, bytes, hipMemcpyHostToDevice, st));
+    lp.base = base;
+    cp.base = base;
+    const SceneDesc* D = (const SceneDesc*)(base + oD);
+    const StepVar* SV = (const StepVar*)(base + oSV);
+    auto launchAll = [&]() {
+        for (int j = 0; j < K; j++) {
+            lp.launch(c, D, SV + (size_t)j * n);
+            cp.launch(c, D, true, SV + (size_t)j * n);
+        }
+    };
+    if (c->graphs && !c->timing.on) {
+        std::vector<uint64_t> key = {(uint64_t)K, (uint64_t)n, (uint64_t)(uintptr_t)base, oD, oSV,
+                                     (uint64_t)(uintptr_t)X.epartL.p, (uint64_t)(uintptr_t)X.epartC.p};
+        lp.sig(key);
+        cp.sig(key);
+        if (!X.winExec || key != X.winKey) {
+            if (X.winExec) { HIPCHK(hipGraphExecDestroy(X.winExec)); X.winExec = nullptr; }
+            X.winKey.clear();
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            try {
+                launchAll();
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIPCHK(hipStreamEndCapture(st, &g));
+            hipError_t e = hipGraphInstantiate(&X.winExec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) { X.winExec = nullptr; throw WdmError(WDM_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e)); }
+            X.winKey = key;
+            X.graphCaptures++;
+        }
+        HIPCHK(hipGraphLaunch(X.winExec, st));
+        X.graphLaunches++;
+    } else launchAll();
+    X.windows++;
+    readWords(c, ss);  // raises device errors
+    // where each scene stopped
+    std::vector<int64_t> done(n);
+    std::vector<Scene*> stopped;
+    for (int i = 0; i < n; i++) {
+        Scene& S = *ss[i];
+        const uint32_t w = X.hostWords[4 * (size_t)S.slot];
+        const int64_t active = std::min<int64_t>(K, left[i]);
+        if (w >= gate[i]) {
+            const int64_t jf = (int64_t)(w - gate[i]);
+            if (jf >= active) throw WdmError(WDM_E_STATE, "Verlet trigger outside the window");
+            S.step = s0[i] + jf;
+            if (S.periodic) restoreCell(S, snaps[i][jf]);
+            stepBeginCell(S);  // mid-step: Leapfrog done, collider and ContactLoop to run
+            stopped.push_back(&S);
+            done[i] = jf + 1;
+        } else done[i] = active;
+    }
+    if (!stopped.empty()) {
+        Batch F(c, stopped);
+        std::vector<int> L(stopped.size());
+        for (size_t i = 0; i < stopped.size(); i++) L[i] = (int)i;
+        InsertionSortCollider::runList(F, L);
+        ContactLoop::runList(F, L, false);
+        ContactLoop::done(F);
+        for (Scene* S : stopped) stepEnd(*S);
+    }
+    return done;
 }
 
 // DemData.force/torque after ContactLoop: gravity reset plus the contact sums
@@ -1271,7 +1509,7 @@ void relax(Scene& S, int iters) {
         hipLaunchKernelGGL(k_bounds, dim3((unsigned)bpP[1]), dim3(WDM_BLOCK), 0, c->stream, B.D, FRd, mP, 1);
         checkLaunch();
         int64_t nbTot = 0;
-        InsertionSortCollider::buildGrids(B, L, F, bpB, mB, FRd, nbTot);
+        InsertionSortCollider::buildGrids(B, L, F, bpB, mB, FRd, nbTot, 1.);
         hipLaunchKernelGGL(k_relax, dim3((unsigned)bpB[1]), dim3(WDM_BLOCK), 0, c->stream, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
                            (const int32_t*)X.cvals2.p, (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p, X.disp.p);
         hipLaunchKernelGGL(k_relax_apply, dim3((unsigned)bpB[1]), dim3(WDM_BLOCK), 0, c->stream, B.D, FRd, mB, (const double4*)X.disp.p);
@@ -1654,9 +1892,26 @@ int wdm_step(wdm_ctx* c, const int32_t* scenes, int32_t ns, int32_t nsteps) {
         }
         if (c->timing.on) c->timing.acc.clear();
         if (ss.empty()) return;
-        for (int k = 0; k < nsteps; k++) {
-            Batch B(c, ss);
-            doOneStep(B);
+        std::vector<int64_t> left(ss.size(), nsteps);
+        for (;;) {
+            std::vector<Scene*> act;
+            std::vector<int64_t> la;
+            std::vector<size_t> ai;
+            int64_t most = 0;
+            for (size_t i = 0; i < ss.size(); i++)
+                if (left[i] > 0) { act.push_back(ss[i]); la.push_back(left[i]); ai.push_back(i); most = std::max(most, left[i]); }
+            if (act.empty()) break;
+            const int K = (int)std::min<int64_t>(c->window, most);
+            bool win = K > 1;
+            for (Scene* S : act) win = win && windowable(*S, K);
+            if (win) {
+                std::vector<int64_t> done = runWindow(c, act, la, K);
+                for (size_t k = 0; k < act.size(); k++) left[ai[k]] -= done[k];
+            } else {
+                Batch B(c, act);
+                doOneStep(B);
+                for (size_t k = 0; k < act.size(); k++) left[ai[k]] -= 1;
+            }
         }
         readWords(c, ss);
     });
@@ -1753,6 +2008,26 @@ int wdm_get_cell(wdm_ctx* c, int32_t sid, double* hSize9, double* gradV9) {
 
 int wdm_sync(wdm_ctx* c) { return guard(c, [&] { HIPCHK(hipStreamSynchronize(c->stream)); }); }
 
+int wdm_set_tuning(wdm_ctx* c, int32_t window, int32_t graphs, double cellFactor) {
+    return guard(c, [&] {
+        if (window < 1 || window > 4096) throw WdmError(WDM_E_ARG, "window must be in [1, 4096] steps");
+        if (!(cellFactor >= 1. && cellFactor <= 16.)) throw WdmError(WDM_E_ARG, "cell_factor must be in [1, 16]");
+        c->window = window;
+        c->graphs = graphs != 0;
+        c->cellFactor = cellFactor;
+    });
+}
+int wdm_get_tuning(wdm_ctx* c, int32_t* window, int32_t* graphs, double* cellFactor, int64_t* windows, int64_t* captures,
+                   int64_t* launches) {
+    return guard(c, [&] {
+        if (window) *window = c->window;
+        if (graphs) *graphs = c->graphs;
+        if (cellFactor) *cellFactor = c->cellFactor;
+        if (windows) *windows = c->sh->windows;
+        if (captures) *captures = c->sh->graphCaptures;
+        if (launches) *launches = c->sh->graphLaunches;
+    });
+}
 int wdm_enable_timing(wdm_ctx* c, int32_t on) { return guard(c, [&] { c->timing.on = on != 0; c->timing.acc.clear(); }); }
 
 int wdm_kernel_times(wdm_ctx* c, int32_t cap, char* names, double* ms, int64_t* launches, int32_t* nOut) {
