@@ -71,6 +71,31 @@ def cpu_baseline(ctx, n_sample, seconds):
                     n_sample, k, el))
 
 
+def timed_kernels(ctx, sids, steps):
+    """HIP-event kernel times over `steps` steps launched one at a time (window 1), so that
+    every launch of a step is counted once."""
+    t = ctx.tuning()
+    ctx.set_tuning(window=1)
+    ctx.enable_timing(True)
+    ctx.step(sids, steps)
+    kt = ctx.kernel_times()
+    ctx.enable_timing(False)
+    ctx.set_tuning(window=t["window"])
+    return kt
+
+
+def launch_config(ctx, tune):
+    t = ctx.tuning()
+    out = {"window_steps": t["window"], "graphs": bool(t["graphs"]), "cell_factor": t["cell_factor"], "block": 256}
+    if tune:
+        out["autotuned"] = True
+        out["autotune_steps"] = tune["steps"]
+        out["autotune_trials"] = [[x["window"], x["cell_factor"], round(x["ms_per_step"], 4)] for x in tune["trials"]]
+    else:
+        out["autotuned"] = False
+    return out
+
+
 def bench_batch(args, ctx):
     """Many small scenes stepped as one batch vs the same scenes stepped one by one."""
     from woodem_amd import runner, scenes
@@ -84,6 +109,13 @@ def bench_batch(args, ctx):
         len(arrs), n_tot, min(a.n_par for a in arrs), max(a.n_par for a in arrs), time.time() - t0))
     ctx.step(sids, args.warmup)
     ctx.sync()
+    tune = None
+    if args.autotune:
+        from woodem_amd.autotune import autotune
+        tt = time.perf_counter()
+        tune = autotune(ctx, sids, log=log)
+        log("autotune: window %d, cell factor %.2f (%d steps, %.1f s)" % (tune["window"], tune["cell_factor"], tune["steps"],
+                                                                           time.perf_counter() - tt))
     fr0 = sum(ctx.stats(s)["nFullRuns"] for s in sids)
     t1 = time.perf_counter()
     ctx.step(sids, args.steps)
@@ -99,11 +131,8 @@ def bench_batch(args, ctx):
         ctx.step(sids, 1)
         lat.append(time.perf_counter() - tt)
     lat = np.array(lat) * 1e3
-    # kernel time per step (all launches of a kind in one step summed)
-    ctx.enable_timing(True)
-    ctx.step(sids, args.timing_steps)
-    kt = ctx.kernel_times()
-    ctx.enable_timing(False)
+    # kernel time per step (all launches of a kind in one step summed), one step at a time
+    kt = timed_kernels(ctx, sids, args.timing_steps)
     n_con = n_real = n_active = 0
     for s in sids:
         c = ctx.contacts(s)
@@ -149,7 +178,8 @@ def bench_batch(args, ctx):
         "data": "synthetic (seeded generator, seeds 1000..%d)" % (999 + len(arrs)),
         "config": {"workload": "%d deposition boxes, %d particles in total, FP32 contact arithmetic, one batch" % (len(arrs), n_tot),
                    "scenes": len(arrs), "n_particles": n_tot, "precision": "mixed",
-                   "contacts_potential": n_con, "contacts_real": n_real, "scene_full_runs_in_timed_region": fr1 - fr0},
+                   "contacts_potential": n_con, "contacts_real": n_real, "scene_full_runs_in_timed_region": fr1 - fr0,
+                   "launch": launch_config(ctx, tune)},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)), "samples": len(lat)},
         "roofline": {"bound": "hbm", "kernel": "k_contacts", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -197,7 +227,7 @@ def bench_serve(args, ctx):
         "data": "synthetic (seeded Poisson arrivals and drum generator, seed %d)" % args.seed,
         "config": {"workload": "%d rotating-drum requests, Poisson %.0f/s, %d steps each, served with continuous batching" % (
             len(reqs), args.rate, args.req_steps), "requests": len(reqs), "rate_per_s": args.rate, "chunk_steps": args.chunk,
-                   "spheres_mean": float(n_sph.mean()), "hbm_budget": args.hbm_budget},
+                   "spheres_mean": float(n_sph.mean()), "hbm_budget": args.hbm_budget, "launch": launch_config(ctx, None)},
         "latency_s": rep["latency_s"],
         "admission_wait_s": rep["admission_wait_s"],
         "step_latency_ms": rep["step_ms"],
@@ -233,11 +263,18 @@ def main():
     ap.add_argument("--seq-scenes", type=int, default=32, help="batch workload: scenes in the sequential sample")
     ap.add_argument("--seq-steps", type=int, default=300, help="batch workload: steps per scene in the sequential sample")
     ap.add_argument("--relax-box", type=int, default=200, help="batch workload: overlap relaxation sweeps per scene")
+    ap.add_argument("--window", type=int, default=16, help="steps launched together as one graph (1 = step by step)")
+    ap.add_argument("--no-graphs", action="store_true", help="launch windows without capturing graphs")
+    ap.add_argument("--cell-factor", type=float, default=1.0, help="collider cell width over the largest sphere bound")
+    ap.add_argument("--autotune", type=int, default=None,
+                    help="batch workload: pick window and cell factor by the autotuner (default 1 for batch, 0 otherwise)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB (5 times), for scaling FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = 5000 if args.workload == "batch" else 1000
+    if args.autotune is None:
+        args.autotune = 1 if args.workload == "batch" else 0
     if args.calibrate_copy:
         import torch
         a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
@@ -256,6 +293,7 @@ def main():
     from woodem_amd._native import PREC_FP64, PREC_MIXED, Context
 
     ctx = Context(0)
+    ctx.set_tuning(window=args.window, graphs=not args.no_graphs, cell_factor=args.cell_factor)
     if args.workload == "batch":
         print(json.dumps(bench_batch(args, ctx)), flush=True)
         return
@@ -292,10 +330,7 @@ def main():
     lat = np.array(lat) * 1e3
 
     # kernel timing with HIP events on the library's stream
-    ctx.enable_timing(True)
-    run.step(args.timing_steps)
-    kt = ctx.kernel_times()
-    ctx.enable_timing(False)
+    kt = timed_kernels(ctx, [run.sid], args.timing_steps)
     c = run.contacts()
     n_con = len(c["idA"])
     n_real = int(c["isReal"].sum())
@@ -319,10 +354,7 @@ def main():
         rm.step(args.steps)
         ctx.sync()
         elm = time.perf_counter() - tm
-        ctx.enable_timing(True)
-        rm.step(args.timing_steps)
-        ktm = ctx.kernel_times()
-        ctx.enable_timing(False)
+        ktm = timed_kernels(ctx, [rm.sid], args.timing_steps)
         cm = rm.contacts()
         msm, lam = ktm.get("contacts", (float("nan"), 0))
         mixed = {"value": b.n_par * args.steps / elm, "ms_per_step": elm / args.steps * 1e3,
@@ -356,7 +388,7 @@ def main():
         "config": {"workload": "periodic Hertz, %d spheres, phi=0.60, lognormal r (median 1 mm, sigma .3), 1000-step run" % b.n_par,
                    "n_particles": b.n_par, "precision": args.precision, "dt": b.meta["dt"],
                    "contacts_potential": n_con, "contacts_real": n_real, "contacts_processed": n_active,
-                   "full_collider_runs_in_timed_region": st1["nFullRuns"] - st0["nFullRuns"]},
+                   "full_collider_runs_in_timed_region": st1["nFullRuns"] - st0["nFullRuns"], "launch": launch_config(ctx, None)},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)), "samples": len(lat)},
         "roofline": {"bound": "hbm", "kernel": "k_contacts", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -124,7 +124,10 @@ struct ConView {
     GP T* st;
     GP double4* cpt;   // per state row: contact point of the last ContactLoop (x, y, z, 0)
     GP int32_t* itm;   // per contact: its item in the ContactLoop's work list, -1 when dormant
-    GP int32_t* act;   // per item: the contact
+    // per item, so that the ContactLoop reads them contiguously: (idA, idB, info, cellDist)
+    // and (state row, contact); info and the row are written to both places
+    GP int4* irec;
+    GP int2* iaux;
     int32_t* nItems;   // items in the work list
     GP double4* gfc;   // per item: gather record
     int64_t cap;
@@ -1269,7 +1272,11 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_items(const ConDesc<T>* C, const 
         item = nRows + (nrscan[F.newStart + d] - nrscan[F.newStart]);
     }
     nc.itm[d] = (int32_t)item;
-    if (item >= 0) nc.act[item] = (int32_t)d;
+    if (item >= 0) {
+        const int2 id = nc.ids[d];
+        nc.irec[item] = make_int4(id.x, id.y, (int32_t)nc.info[d], nc.cd[d]);
+        nc.iaux[item] = make_int2(row, (int32_t)d);
+    }
 }
 // next free state row of each scene = rows handed out (real and touching contacts); items
 template <typename T>
@@ -1392,29 +1399,33 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
     uint32_t bits = 0;
     bool realOut = false;  // contact is real after this step's law
     if (item < nItems) do {
-        const int64_t c = cv.act[item];
-        uint32_t info = cv.info[c];
+        const int4 rec = cv.irec[item];
+        const int2 aux = cv.iaux[item];
+        const int64_t c = aux.y;
+        GP int32_t* infoItem = (GP int32_t*)(cv.irec + item) + 2;
+        uint32_t info = (uint32_t)rec.z;
         if (info & (CI_DEAD | CI_DORMANT)) break;
         uint32_t ty = info & CI_TYPE_MASK;
         if (info & CI_PENDING) {  // ContactContainer::removePending (ContactContainer.hpp:78-93)
             info &= ~CI_PENDING;
             if (!(info & CI_OVERLAP)) info |= CI_DEAD;
             cv.info[c] = info;
+            *infoItem = (int32_t)info;
             if (info & CI_DEAD) break;
         }
         if (ty == CI_NONE) break;
         const bool real = (info & CI_REAL) != 0;
-        const int2 id = cv.ids[c];
+        const int2 id = make_int2(rec.x, rec.y);
         const double dt = a.dt;
         double4 pa = n.posr[id.x], pb = n.posr[id.y];
         D3 shift2 = mk3(0., 0., 0.);
         int32_t cdp = 0;
         if (a.periodic) {
-            cdp = cv.cd[c];
+            cdp = rec.w;
             shift2 = mv3(ldM3(a.cell->hSize), mk3((double)unpack10(cdp, 0), (double)unpack10(cdp, 10), (double)unpack10(cdp, 20)));
         }
         T tr[9];
-        int32_t row = real ? cv.slot[c] : -1;
+        int32_t row = real ? aux.x : -1;
         // L6Geom.trsf: only columns 0 and 1 are read by the update (L6Geom.cpp:110-118
         // recomputes column 2 as their cross product), so only they are stored
         if (real) {
@@ -1692,16 +1703,19 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
             }
         }
         if (!keep) {  // ContactContainer::requestRemoval -> Contact::reset (ContactContainer.cpp:164-171)
-            cv.info[c] = (info & ~CI_REAL) | CI_PENDING;
+            const uint32_t ni = (info & ~CI_REAL) | CI_PENDING;
+            cv.info[c] = ni;
+            *infoItem = (int32_t)ni;
             break;
         }
         // store geometry/physics state
         if (!real) {  // a contact becoming real keeps its earlier row or takes the next free one
-            row = cv.slot[c];
+            row = aux.x;
             if (row < 0) {
                 row = atomicAdd(cv.counter, 1);
                 if (row >= cv.cap) { atomicOr(err, ERR_CAPACITY); break; }
                 cv.slot[c] = row;
+                cv.iaux[item].x = row;
             }
         }
         for (int k = 0; k < 3; k++) {
@@ -1712,6 +1726,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
         if (!real) {
             for (int k = 0; k < 5; k++) cv.st[(size_t)(S_P + k) * cv.cap + row] = ph[k];
             cv.info[c] = info | CI_REAL;
+            *infoItem = (int32_t)(info | CI_REAL);
         }
         // global force trsf*F (Contact::getForceTorqueBranch, Contact.cpp:24-34)
         V3T<T> fl = mk3<T>(F[0], F[1], F[2]);

@@ -131,7 +131,9 @@ struct ConBuf {
     DBuf<uint64_t> key;
     DBuf<int2> ids;
     DBuf<uint32_t> info;
-    DBuf<int32_t> cd, slot, counter, itm, act, nItems;
+    DBuf<int32_t> cd, slot, counter, itm, nItems;
+    DBuf<int4> irec;
+    DBuf<int2> iaux;
     DBuf<T> st;
     DBuf<double4> cpt, gfc;
     int64_t cap = 0;
@@ -140,19 +142,19 @@ struct ConBuf {
         int64_t nc = std::max<int64_t>(n + n / 3, 1024);
         free(c);
         ensure(c, key, nc); ensure(c, ids, nc); ensure(c, info, nc); ensure(c, cd, nc);
-        ensure(c, slot, nc); ensure(c, counter, 1); ensure(c, itm, nc); ensure(c, act, nc); ensure(c, nItems, 1);
+        ensure(c, slot, nc); ensure(c, counter, 1); ensure(c, itm, nc); ensure(c, irec, nc); ensure(c, iaux, nc); ensure(c, nItems, 1);
         ensure(c, st, (size_t)NSTATE * nc); ensure(c, cpt, nc); ensure(c, gfc, 2 * (size_t)nc);
         HIPCHK(hipMemsetAsync(nItems.p, 0, sizeof(int32_t), c->stream));
         cap = nc;
     }
     ConView<T> view() const {
         ConView<T> v; v.key = gpc(key.p); v.ids = gpc(ids.p); v.info = gpc(info.p); v.cd = gpc(cd.p); v.slot = gpc(slot.p);
-        v.counter = counter.p; v.st = gpc(st.p); v.cpt = gpc(cpt.p); v.itm = gpc(itm.p); v.act = gpc(act.p); v.nItems = nItems.p;
+        v.counter = counter.p; v.st = gpc(st.p); v.cpt = gpc(cpt.p); v.itm = gpc(itm.p); v.irec = gpc(irec.p); v.iaux = gpc(iaux.p); v.nItems = nItems.p;
         v.gfc = gpc(gfc.p); v.cap = cap; return v;
     }
     void free(Ctx* c) {
         release(c, key); release(c, ids); release(c, info); release(c, cd); release(c, st); release(c, gfc);
-        release(c, slot); release(c, counter); release(c, itm); release(c, act); release(c, nItems); release(c, cpt); cap = 0;
+        release(c, slot); release(c, counter); release(c, itm); release(c, irec); release(c, iaux); release(c, nItems); release(c, cpt); cap = 0;
     }
 };
 
