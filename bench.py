@@ -107,15 +107,22 @@ def bench_batch(args, ctx):
     n_tot = sum(a.n_par for a in arrs)
     log("batch: %d scenes, %d particles in total (%d..%d per scene), set up in %.1f s" % (
         len(arrs), n_tot, min(a.n_par for a in arrs), max(a.n_par for a in arrs), time.time() - t0))
-    ctx.step(sids, args.warmup)
-    ctx.sync()
     tune = None
     if args.autotune:
+        # tuned on a copy of the batch (same scenes, own state), so that the timed steps
+        # below start from the same state as an untuned run
         from woodem_amd.autotune import autotune
         tt = time.perf_counter()
-        tune = autotune(ctx, sids, log=log)
-        log("autotune: window %d, cell factor %.2f (%d steps, %.1f s)" % (tune["window"], tune["cell_factor"], tune["steps"],
-                                                                           time.perf_counter() - tt))
+        twins = [runner.HipRun(a, runner.params_from_meta(a, precision=PREC_MIXED), runner.cell_from_meta(a), ctx=ctx) for a in arrs]
+        tsids = [r.sid for r in twins]
+        ctx.step(tsids, args.warmup)
+        tune = autotune(ctx, tsids, steps=args.tune_steps, warm=10, windows=(4, 8, 16, 32, 64, 128), log=log)
+        for r in twins:
+            r.close()
+        log("autotune: window %d, cell factor %.2f (%d steps on a copy of the batch, %.1f s)" % (
+            tune["window"], tune["cell_factor"], tune["steps"], time.perf_counter() - tt))
+    ctx.step(sids, args.warmup)
+    ctx.sync()
     fr0 = sum(ctx.stats(s)["nFullRuns"] for s in sids)
     t1 = time.perf_counter()
     ctx.step(sids, args.steps)
@@ -268,6 +275,7 @@ def main():
     ap.add_argument("--cell-factor", type=float, default=1.0, help="collider cell width over the largest sphere bound")
     ap.add_argument("--autotune", type=int, default=None,
                     help="batch workload: pick window and cell factor by the autotuner (default 1 for batch, 0 otherwise)")
+    ap.add_argument("--tune-steps", type=int, default=100, help="autotuner: timed steps per candidate and round")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB (5 times), for scaling FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()

@@ -181,6 +181,8 @@ struct GridArgs {
     int periodic;
     double L[3];     // periodic cell size
     double invL[3];  // 1/L (prefilter only)
+    float Lf[3], invLf[3];  // the same in FP32 (FP32 prefilter)
+    float pfMargin;         // FP32 prefilter slack: far above its rounding (~1e-7 of the grid span)
 };
 
 
@@ -796,6 +798,40 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_cell_keys(const SceneDesc* D, con
     vals[F.binStart + i] = id;
 }
 
+// FP32 box of each bound entry in (scene, cell) order: centre (wrapped into the periodic
+// cell, or relative to the grid origin) and the largest half extent.  k_pairs scans a
+// neighbour cell's entries contiguously through these 16 bytes and reads a bound's exact
+// FP64 corners only when the FP32 test cannot rule the pair out.
+__global__ void __launch_bounds__(WDM_BLOCK) k_cell_boxes(const SceneDesc* D, const FRScene* FR, Map m, const int32_t* cvals, float4* boxes) {
+    int64_t lb;
+    const FRScene& F = FR[mapBlock(m, lb)];
+    const SceneDesc& S = D[F.desc];
+    const int li = (int)(lb * WDM_BLOCK + threadIdx.x);
+    if (li >= S.nbinned) return;
+    const int64_t i = F.binStart + li;
+    const int id = cvals[i];
+    const double4 a = S.p.bmin[id], b = S.p.bmax[id];
+    const double c3[3] = {.5 * (a.x + b.x), .5 * (a.y + b.y), .5 * (a.z + b.z)};
+    double r[3];
+    for (int d = 0; d < 3; d++) {
+        if (F.g.periodic) r[d] = c3[d] - F.g.L[d] * floor(c3[d] * F.g.invL[d]);
+        else r[d] = c3[d] - F.g.origin[d];
+    }
+    const double h = fmax(fmax(b.x - a.x, b.y - a.y), b.z - a.z) * .5;
+    boxes[i] = make_float4((float)r[0], (float)r[1], (float)r[2], (float)h);
+}
+// the FP32 test: false only when the boxes are apart by more than the slack on some axis
+__device__ __forceinline__ bool boxesMayOverlap(const float4& a, const float4& b, const GridArgs& g) {
+    const float lim = a.w + b.w + g.pfMargin;
+    float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+    if (g.periodic) {
+        dx -= g.Lf[0] * rintf(dx * g.invLf[0]);
+        dy -= g.Lf[1] * rintf(dy * g.invLf[1]);
+        dz -= g.Lf[2] * rintf(dz * g.invLf[2]);
+    }
+    return !(fabsf(dx) > lim || fabsf(dy) > lim || fabsf(dz) > lim);
+}
+
 __global__ void k_fill_i32(int32_t* a, int64_t n, int32_t v) {
     int64_t i = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
     if (i < n) a[i] = v;
@@ -880,13 +916,17 @@ __device__ __forceinline__ bool mayCollideD(const PartView& p, const GP int32_t*
 // every sphere) is kept out of the cells; it is listed instead, by k_large_cells, in every
 // cell its bound covers plus margin (a small bound overlapping it starts at most one cell
 // before it), and a small bound tests the large ones listed in its own cell.  Large
-// bounds pair with each other by a direct loop when their masks allow it.  A conservative minimum-image test on bound centres (periodic
-// cells) rejects most candidates cheaply; the decision itself is the collider's own
-// closed-interval 3-axis overlap of the stored bounds plus mayCollide, so the set
-// equals the reference's and the grid only prunes.
+// bounds pair with each other by a direct loop when their masks allow it.  Conservative
+// tests reject most candidates cheaply (FP32 boxes, then an FP64 minimum-image test on
+// bound centres); the decision itself is the collider's own closed-interval 3-axis overlap
+// of the stored bounds plus mayCollide, so the set equals the reference's and the grid
+// only prunes.
 // Each bound's work is split into PAIR_ITEMS threads (one per stencil cell, one for walls
-// and listed large bounds).  MODE 0 counts per item, MODE 1 writes keys and packed periods
-// at offs[item].
+// and listed large bounds).  A wavefront scans its stencil cells in lockstep and queues
+// the candidates that survive the FP32 test in LDS; the exact tests then run over the
+// queue with every lane busy (run per thread inside the scan, they would serialise the
+// wavefront on its few surviving lanes).  MODE 0 counts per item, MODE 1 writes keys and
+// packed periods at offs[item] (in queue order: the candidates are sorted afterwards).
 __device__ __forceinline__ bool periPrefilter(const double4& a0, const double4& a1, const double4& b0, const double4& b1, const GridArgs& g) {
     // centres and half extents; reject when the minimum-image gap exceeds the half
     // extents by more than 1e-9 L on some axis (rounding here is ~1e-16 L)
@@ -963,105 +1003,179 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_large_cells(const SceneDesc* D, c
     }
 }
 
+// one candidate of bound `id` (corners a0, a1, shape sa) with particle j: canonical order,
+// the collider's exact overlap (with periods) and mayCollide; a binned j must pass the
+// FP64 prefilter first (it has finite bounds)
+__device__ __forceinline__ bool pairTest(const SceneDesc& S, const GridArgs& g, int id, const double4& a0, const double4& a1, int sa, int j,
+                                         bool binned, uint32_t* err, int per[3]) {
+    const PartView& p = S.p;
+    const double4 b0 = p.bmin[j], b1 = p.bmax[j];
+    if (binned) {
+        if (g.periodic) { if (!periPrefilter(a0, a1, b0, b1, g)) return false; }
+        else if (!overlapAperiodic(a0, a1, b0, b1)) return false;
+    }
+    const int sb = ptShape(p.ptype[j]);
+    int A, B, ty;
+    canonical(id, j, sa, sb, A, B, ty);
+    const bool aIsId = (A == id);
+    const double4 &A0 = aIsId ? a0 : b0, &A1 = aIsId ? a1 : b1, &B0 = aIsId ? b0 : a0, &B1 = aIsId ? b1 : a1;
+    per[0] = per[1] = per[2] = 0;
+    const bool ov = g.periodic ? overlapPeri(A0, A1, B0, B1, g.L, per, err) : (binned || overlapAperiodic(A0, A1, B0, B1));
+    return ov && mayCollideD(p, S.n.master, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), S.a.loneMask);
+}
+
 // work items per bound in k_pairs: the 14 cells of the half stencil, then walls and the
 // large bounds listed in the own cell
 #define PAIR_ITEMS 15
+// wavefront queue of candidates that passed the FP32 box test: (owner thread, partner's entry)
+#define PAIR_QUEUE 512
 template <int MODE>
 __global__ void __launch_bounds__(WDM_BLOCK) k_pairs(const SceneDesc* D, const FRScene* FR, Map m, const uint32_t* ckeys, const int32_t* cvals,
                                                     const int32_t* cstart, const int32_t* cend, const int32_t* lstart, const int32_t* lend,
-                                                    const int32_t* lvals, int32_t* counts, const int64_t* offs, uint64_t* okeys,
-                                                    int32_t* operiods, int idBits) {
+                                                    const int32_t* lvals, const float4* boxes, int32_t* counts, const int64_t* offs,
+                                                    uint64_t* okeys, int32_t* operiods, int idBits) {
+    __shared__ int2 queue[WDM_BLOCK / 64][PAIR_QUEUE];
+    __shared__ int accepted[WDM_BLOCK];  // per thread: pairs found through the queue
+    __shared__ int ownId[WDM_BLOCK];
+    __shared__ int64_t ownOff[WDM_BLOCK];
     int64_t lb;
     const int tt = mapBlock(m, lb);
     const FRScene& F = FR[tt];
     const SceneDesc& S = D[F.desc];
     const PartView p = S.p;
     const GridArgs g = F.g;  // by value: the loops below store to global memory
-    const uint32_t loneMask = S.a.loneMask;
     uint32_t* err = S.words + 2;
     const int* unb = S.unb;
     const int nunb = S.nunb;
     const int64_t cellBase = F.cellBase;
     const int nCells = F.nCells;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t item = lb * WDM_BLOCK + threadIdx.x;
     const int li = (int)(item / PAIR_ITEMS), sItem = (int)(item % PAIR_ITEMS);
-    if (li >= S.nbinned) return;
-    const int64_t i = F.binStart + li;
+    // every lane stays to the end: the box scan below is a wavefront-wide loop
+    const bool valid = li < S.nbinned;
+    const int64_t i = F.binStart + (valid ? li : 0);
     const int64_t q = i * PAIR_ITEMS + sItem;  // batch-wide item index
     const uint64_t tkey = (uint64_t)tt << (2 * idBits);
-    const int id = cvals[i];
-    const uint32_t ck = (uint32_t)(ckeys[i] - cellBase);
-    const double4 a0 = p.bmin[id], a1 = p.bmax[id];
-    const int sa = ptShape(p.ptype[id]);
-    int cnt = 0;
-    const int64_t o = MODE ? offs[q] : 0;
-    // one candidate: prefilter, canonical order, exact overlap (with periods), mayCollide
-    auto tryPair = [&](int j, bool binned) {
-        const double4 b0 = p.bmin[j], b1 = p.bmax[j];
-        if (binned) {
-            if (g.periodic) { if (!periPrefilter(a0, a1, b0, b1, g)) return; }
-            else if (!overlapAperiodic(a0, a1, b0, b1)) return;
-        }
-        const int sb = ptShape(p.ptype[j]);
-        int A, B, ty;
-        canonical(id, j, sa, sb, A, B, ty);
-        const bool aIsId = (A == id);
-        const double4 &A0 = aIsId ? a0 : b0, &A1 = aIsId ? a1 : b1, &B0 = aIsId ? b0 : a0, &B1 = aIsId ? b1 : a1;
-        int per[3] = {0, 0, 0};
-        const bool ov = g.periodic ? overlapPeri(A0, A1, B0, B1, g.L, per, err) : (binned || overlapAperiodic(A0, A1, B0, B1));
-        if (!ov || !mayCollideD(p, S.n.master, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), loneMask)) return;
-        if (MODE) {
-            const int lo = id < j ? id : j, hi = id < j ? j : id;
-            okeys[o + cnt] = tkey | ((uint64_t)(uint32_t)lo << idBits) | (uint32_t)hi;
-            operiods[o + cnt] = packCd(per[0], per[1], per[2]);
-        }
-        cnt++;
-    };
+    const int id = valid ? cvals[i] : 0;
+    const uint32_t ck = valid ? (uint32_t)(ckeys[i] - cellBase) : 0u;
     const bool large = (int)ck == nCells;
-    if (sItem < 14) {
-        if (!large) {
-            // half-stencil offset of this item: (0,0,0), (1,0,0), (-1..1,1,0), (-1..1,-1..1,1)
-            int dx, dy, dz;
-            if (sItem == 0) { dx = 0; dy = 0; dz = 0; }
-            else if (sItem == 1) { dx = 1; dy = 0; dz = 0; }
-            else if (sItem < 5) { dx = sItem - 3; dy = 1; dz = 0; }
-            else { dx = (sItem - 5) % 3 - 1; dy = (sItem - 5) / 3 - 1; dz = 1; }
-            // periodic axes with fewer than 3 cells hold a single cell (no neighbours); aperiodic axes always scan +-1
-            const bool okx = dx == 0 || !g.periodic || g.n[0] >= 3, oky = dy == 0 || !g.periodic || g.n[1] >= 3,
-                       okz = dz == 0 || !g.periodic || g.n[2] >= 3;
-            if (okx && oky && okz) {
-                const int cx = ck % g.n[0], cy = (ck / g.n[0]) % g.n[1], cz = ck / (g.n[0] * g.n[1]);
-                int z = cz + dz, y = cy + dy, x = cx + dx;
-                bool in = true;
-                if (g.periodic) {
-                    z = (z + g.n[2]) % g.n[2]; y = (y + g.n[1]) % g.n[1]; x = (x + g.n[0]) % g.n[0];
-                } else in = !(z < 0 || z >= g.n[2] || y < 0 || y >= g.n[1] || x < 0 || x >= g.n[0]);
-                if (in) {
-                    const int64_t cell = cellBase + (z * g.n[1] + y) * g.n[0] + x;
-                    int s = cstart[cell];
-                    if (s >= 0) {
-                        const int e = cend[cell];
-                        if (sItem == 0) s = (int)i + 1;  // own cell: later entries only
-                        for (int k = s; k < e; k++) tryPair(cvals[k], true);
+    const int64_t o = (MODE && valid) ? offs[q] : 0;
+    accepted[threadIdx.x] = 0;
+    ownId[threadIdx.x] = id;
+    ownOff[threadIdx.x] = o;
+    // stencil items of a bound that fits a cell: the entries of one neighbour cell, tested
+    // through their FP32 boxes in a wavefront-wide loop; survivors are queued
+    int s = 0, len = 0;
+    float4 own = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (valid && sItem < 14 && !large) {
+        // half-stencil offset of this item: (0,0,0), (1,0,0), (-1..1,1,0), (-1..1,-1..1,1)
+        int dx, dy, dz;
+        if (sItem == 0) { dx = 0; dy = 0; dz = 0; }
+        else if (sItem == 1) { dx = 1; dy = 0; dz = 0; }
+        else if (sItem < 5) { dx = sItem - 3; dy = 1; dz = 0; }
+        else { dx = (sItem - 5) % 3 - 1; dy = (sItem - 5) / 3 - 1; dz = 1; }
+        // periodic axes with fewer than 3 cells hold a single cell (no neighbours); aperiodic axes always scan +-1
+        const bool okx = dx == 0 || !g.periodic || g.n[0] >= 3, oky = dy == 0 || !g.periodic || g.n[1] >= 3,
+                   okz = dz == 0 || !g.periodic || g.n[2] >= 3;
+        if (okx && oky && okz) {
+            const int cx = ck % g.n[0], cy = (ck / g.n[0]) % g.n[1], cz = ck / (g.n[0] * g.n[1]);
+            int z = cz + dz, y = cy + dy, x = cx + dx;
+            bool in = true;
+            if (g.periodic) {
+                z = (z + g.n[2]) % g.n[2]; y = (y + g.n[1]) % g.n[1]; x = (x + g.n[0]) % g.n[0];
+            } else in = !(z < 0 || z >= g.n[2] || y < 0 || y >= g.n[1] || x < 0 || x >= g.n[0]);
+            if (in) {
+                const int64_t cell = cellBase + (z * g.n[1] + y) * g.n[0] + x;
+                s = cstart[cell];
+                if (s >= 0) {
+                    const int e = cend[cell];
+                    if (sItem == 0) s = (int)i + 1;  // own cell: later entries only
+                    len = e > s ? e - s : 0;
+                    own = boxes[i];
+                } else s = 0;
+            }
+        }
+    }
+    int most = len;
+    for (int w = 32; w > 0; w >>= 1) most = max(most, __shfl_xor(most, w));
+    int queued = 0;  // the same in every lane
+    // exact tests of the queued candidates, one per lane
+    auto flush = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int b = 0; b < queued; b += 64) {
+            if (b + lane < queued) {
+                const int2 e = queue[wv][b + lane];
+                const int oid = ownId[e.x];
+                const int j = cvals[e.y];
+                int per[3];
+                if (pairTest(S, g, oid, p.bmin[oid], p.bmax[oid], ptShape(p.ptype[oid]), j, true, err, per)) {
+                    const int slot = atomicAdd(&accepted[e.x], 1);
+                    if (MODE) {
+                        const int lo = oid < j ? oid : j, hi = oid < j ? j : oid;
+                        okeys[ownOff[e.x] + slot] = tkey | ((uint64_t)(uint32_t)lo << idBits) | (uint32_t)hi;
+                        operiods[ownOff[e.x] + slot] = packCd(per[0], per[1], per[2]);
                     }
                 }
             }
-        } else if (sItem == 0 && F.largePairs) {  // large-large: later entries of the large list
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        queued = 0;
+    };
+    // four entries per round, their loads issued together; a round queues at most 256
+    for (int t0 = 0; t0 < most; t0 += 4) {
+        if (queued > PAIR_QUEUE - 256) flush();
+        float4 bx[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (t0 + u < len) bx[u] = boxes[s + t0 + u];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const bool pass = t0 + u < len && boxesMayOverlap(own, bx[u], g);
+            const uint64_t bal = __ballot(pass);
+            if (pass) {
+                const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                queue[wv][queued + rank] = make_int2((int)threadIdx.x, s + t0 + u);  // (owner, entry)
+            }
+            queued += (int)__popcll(bal);
+        }
+    }
+    flush();
+    // the other items test their partners directly (walls, listed large bounds, large-large)
+    int cnt = 0;
+    if (valid && (sItem == 14 || (large && sItem == 0 && F.largePairs))) {
+        const double4 a0 = p.bmin[id], a1 = p.bmax[id];
+        const int sa = ptShape(p.ptype[id]);
+        auto tryPair = [&](int j, bool binned) {
+            int per[3];
+            if (!pairTest(S, g, id, a0, a1, sa, j, binned, err, per)) return;
+            if (MODE) {
+                const int lo = id < j ? id : j, hi = id < j ? j : id;
+                okeys[o + cnt] = tkey | ((uint64_t)(uint32_t)lo << idBits) | (uint32_t)hi;
+                operiods[o + cnt] = packCd(per[0], per[1], per[2]);
+            }
+            cnt++;
+        };
+        if (sItem == 0) {  // large-large: later entries of the large list
             const int e = cend[cellBase + nCells];
             for (int k = (int)i + 1; k < e; k++) tryPair(cvals[k], true);
-        }
-    } else {
-        if (!large && lstart) {  // large bounds listed in the own cell
-            const int64_t own = cellBase + ck;
-            const int s = lstart[own];
-            if (s >= 0) {
-                const int e = lend[own];
-                for (int k = s; k < e; k++) tryPair(lvals[k], true);
+        } else {
+            if (!large && lstart) {  // large bounds listed in the own cell
+                const int64_t oc = cellBase + ck;
+                const int ls = lstart[oc];
+                if (ls >= 0) {
+                    const int le = lend[oc];
+                    for (int k = ls; k < le; k++) tryPair(lvals[k], true);
+                }
             }
+            for (int u = 0; u < nunb; u++) tryPair(unb[u], false);
         }
-        for (int u = 0; u < nunb; u++) tryPair(unb[u], false);
     }
-    if (!MODE) counts[q] = cnt;
+    if (!MODE && valid) counts[q] = cnt + accepted[threadIdx.x];
 }
 
 __device__ __forceinline__ int64_t lowerBound64(const uint64_t* a, int64_t n, uint64_t k) {

@@ -244,6 +244,7 @@ struct Shared {
     DBuf<uint32_t> lkeys, lkeys2;  // large bounds listed per cell
     DBuf<int32_t> lvals, lvals2, lstart, lend, lcnt32;
     DBuf<int64_t> lcnt, loffs;
+    DBuf<float4> cbox;  // FP32 boxes of the bound entries in cell order
     DBuf<uint8_t> temp;
     DBuf<double4> disp;
     // window of steps: its tables (device and pinned staging) and the last captured graph
@@ -275,7 +276,7 @@ struct Shared {
         release(c, entK); release(c, entK2); release(c, candP); release(c, candP2); release(c, entV); release(c, entV2);
         release(c, temp); release(c, disp);
         release(c, lkeys); release(c, lkeys2); release(c, lvals); release(c, lvals2); release(c, lstart); release(c, lend);
-        release(c, lcnt32); release(c, lcnt); release(c, loffs);
+        release(c, lcnt32); release(c, lcnt); release(c, loffs); release(c, cbox);
         if (evWords) (void)hipEventDestroy(evWords);
         evWords = nullptr;
     }
@@ -779,6 +780,13 @@ struct InsertionSortCollider {
                 }
                 F[t].wLarge = w;
             }
+            double span = 0;
+            for (int d = 0; d < 3; d++) {
+                g.Lf[d] = (float)g.L[d];
+                g.invLf[d] = (float)g.invL[d];
+                span = std::max(span, S.periodic ? g.L[d] : g.n[d] * g.w[d]);
+            }
+            g.pfMargin = (float)(4e-6 * span);
             F[t].cellBase = cellBase;
             F[t].binStart = nbTot;
             F[t].nCells = g.n[0] * g.n[1] * g.n[2];
@@ -799,6 +807,9 @@ struct InsertionSortCollider {
         hipLaunchKernelGGL(k_fill_i32, dim3(nblk(G)), dim3(WDM_BLOCK), 0, st, X.cstart.p, G, -1);
         if (nbTot > 0)
             hipLaunchKernelGGL(k_cell_ranges, dim3(nblk(nbTot)), dim3(WDM_BLOCK), 0, st, (const uint32_t*)X.ckeys2.p, nbTot, X.cstart.p, X.cend.p);
+        ensure(c, X.cbox, std::max<int64_t>(nbTot, 1));
+        if (nblkB > 0)
+            hipLaunchKernelGGL(k_cell_boxes, dim3((unsigned)nblkB), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const int32_t*)X.cvals2.p, X.cbox.p);
         checkLaunch();
     }
 
@@ -882,7 +893,7 @@ struct InsertionSortCollider {
                 hipLaunchKernelGGL(k_pairs<0>, dim3((unsigned)bpQ[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mQ, (const uint32_t*)X.ckeys2.p,
                                    (const int32_t*)X.cvals2.p, (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p,
                                    large ? (const int32_t*)X.lstart.p : nullptr, (const int32_t*)X.lend.p, (const int32_t*)X.lvals2.p,
-                                   X.cnt32.p, (const int64_t*)nullptr, (uint64_t*)nullptr, (int32_t*)nullptr, idBits);
+                                   (const float4*)X.cbox.p, X.cnt32.p, (const int64_t*)nullptr, (uint64_t*)nullptr, (int32_t*)nullptr, idBits);
                 checkLaunch();
                 hipLaunchKernelGGL(k_i32_to_i64, dim3(nblk(nq)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.cnt32.p, X.counts.p, nq);
                 checkLaunch();
@@ -900,7 +911,7 @@ struct InsertionSortCollider {
                 hipLaunchKernelGGL(k_pairs<1>, dim3((unsigned)bpQ[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mQ, (const uint32_t*)X.ckeys2.p,
                                    (const int32_t*)X.cvals2.p, (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p,
                                    large ? (const int32_t*)X.lstart.p : nullptr, (const int32_t*)X.lend.p, (const int32_t*)X.lvals2.p,
-                                   (int32_t*)nullptr, (const int64_t*)X.offs.p, X.candK.p, X.candP.p, idBits);
+                                   (const float4*)X.cbox.p, (int32_t*)nullptr, (const int64_t*)X.offs.p, X.candK.p, X.candP.p, idBits);
                 checkLaunch();
                 sortPairs(c, X.candK.p, X.candK2.p, X.candP.p, X.candP2.p, ncandTot, keyBits);
             }
