@@ -58,7 +58,6 @@ __host__ __device__ inline GP T* gpc(T* p) { return (GP T*)p; }
 #define ERR_NAN 1u
 #define ERR_OVERSIZE 2u
 #define ERR_CELLDIST 4u
-#define ERR_CAPACITY 8u
 
 // node flags
 #define F_DOF_ALL 63u
@@ -119,7 +118,7 @@ struct ConView {
     GP int2* ids;
     GP uint32_t* info;
     GP int32_t* slot;  // row of the contact's geometry/physics state, -1 when it has none
-    int32_t* counter;  // next free state row (rows are handed out densely at full runs)
+    int32_t* counter;  // state rows handed out at the last full run (real and touching contacts)
     GP int32_t* cd;
     GP T* st;
     GP double4* cpt;   // per state row: contact point of the last ContactLoop (x, y, z, 0)
@@ -316,6 +315,9 @@ __device__ void blockSumWrite(double (&v)[NV], uint32_t orbits, double* partial,
 // node sums its real contacts in ascending order of the other particle's id (CSR
 // order); F = trsf*Flocal and the contact point come from the contact's gather
 // record, the periodic shift of the B side from the cellDist packed in its w slot.
+#ifndef GATHER_CHUNK
+#define GATHER_CHUNK 2
+#endif
 __device__ __forceinline__ void gatherForces(const StepArgs& a, const NodeView& n, const PartView& p, const GatherView& gv, int i,
                                              uint32_t fl, double m, const D3& pos, D3& F, D3& T) {
     // gravity acts on a clump's master only (ClumpData::applyToMembers resets the members)
@@ -324,23 +326,42 @@ __device__ __forceinline__ void gatherForces(const StepArgs& a, const NodeView& 
     if (i >= p.np) return;
     int sh = ptShape(p.ptype[i]);
     if (sh != 1 && sh != 2) return;
-    int64_t e0 = gv.off[i], e1 = gv.off[i + 1];
-    for (int64_t e = e0; e < e1; e++) {
-        uint32_t v = gv.ent[e];
-        uint32_t c = v & 0x7fffffffu;
-        if (!((gv.realMask[c >> 6] >> (c & 63)) & 1ull)) continue;  // non-real: no force (ContactLoop.cpp:188)
-        bool isB = (v >> 31) != 0;
-        double4 fg = gv.gfc[2 * (size_t)c], cp = gv.gfc[2 * (size_t)c + 1];
-        double sign = isB ? -1. : 1.;
-        D3 Fs = mk3(fg.x * sign, fg.y * sign, fg.z * sign);
-        D3 ref = pos;
-        if (isB && a.periodic) {
-            int32_t q = (int32_t)fg.w;
-            ref = pos + mv3(ldM3(a.cell->gatherHSize), mk3((double)unpack10(q, 0), (double)unpack10(q, 10), (double)unpack10(q, 20)));
+    const int64_t e0 = gv.off[i], e1 = gv.off[i + 1];
+    // GATHER_CHUNK entries at a time: their loads are issued together (entry, realness
+    // word, gather record), then summed in entry order
+    for (int64_t eb = e0; eb < e1; eb += GATHER_CHUNK) {
+        uint32_t v[GATHER_CHUNK];
+        bool real[GATHER_CHUNK];
+        double4 fg[GATHER_CHUNK], cp[GATHER_CHUNK];
+#pragma unroll
+        for (int u = 0; u < GATHER_CHUNK; u++) v[u] = eb + u < e1 ? gv.ent[eb + u] : 0u;
+#pragma unroll
+        for (int u = 0; u < GATHER_CHUNK; u++) {
+            const uint32_t c = v[u] & 0x7fffffffu;
+            real[u] = eb + u < e1 && ((gv.realMask[c >> 6] >> (c & 63)) & 1ull);  // non-real: no force (ContactLoop.cpp:188)
         }
-        D3 xc = xyz(cp) - ref;
-        F = F + Fs;
-        T = T + cross3(xc, Fs);
+#pragma unroll
+        for (int u = 0; u < GATHER_CHUNK; u++)
+            if (real[u]) {
+                const size_t c = v[u] & 0x7fffffffu;
+                fg[u] = gv.gfc[2 * c];
+                cp[u] = gv.gfc[2 * c + 1];
+            }
+#pragma unroll
+        for (int u = 0; u < GATHER_CHUNK; u++) {
+            if (!real[u]) continue;
+            const bool isB = (v[u] >> 31) != 0;
+            const double sign = isB ? -1. : 1.;
+            D3 Fs = mk3(fg[u].x * sign, fg[u].y * sign, fg[u].z * sign);
+            D3 ref = pos;
+            if (isB && a.periodic) {
+                int32_t q = (int32_t)fg[u].w;
+                ref = pos + mv3(ldM3(a.cell->gatherHSize), mk3((double)unpack10(q, 0), (double)unpack10(q, 10), (double)unpack10(q, 20)));
+            }
+            D3 xc = xyz(cp[u]) - ref;
+            F = F + Fs;
+            T = T + cross3(xc, Fs);
+        }
     }
 }
 
@@ -1367,8 +1388,9 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_copy_state(const ConDesc<T>* C, c
     for (int k = 0; k < NSTATE; k++) nc.st[(size_t)k * nc.cap + ns] = oldc.st[(size_t)k * oldc.cap + os];
     nc.cpt[ns] = oldc.cpt[os];
 }
-// work list of the ContactLoop: a contact with a state row is item `row`, the other
-// non-dormant ones follow in key order (nrscan: exclusive batch-wide prefix of nrflag)
+// work list of the ContactLoop: a contact given a state row at the full run (real or
+// touching) is item `row`, the other non-dormant ones follow in key order (nrscan:
+// exclusive batch-wide prefix of nrflag) and take the row of their item
 template <typename T>
 __global__ void __launch_bounds__(WDM_BLOCK) k_items(const ConDesc<T>* C, const FRScene* FR, Map m, const int64_t* rscan,
                                                     const int64_t* nrscan) {
@@ -1384,15 +1406,16 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_items(const ConDesc<T>* C, const 
     else if (!(nc.info[d] & CI_DORMANT)) {
         const int64_t nRows = rscan[F.newStart + F.nNew] - rscan[F.newStart];
         item = nRows + (nrscan[F.newStart + d] - nrscan[F.newStart]);
+        nc.slot[d] = (int32_t)item;  // its state row if it becomes real before the next full run
     }
     nc.itm[d] = (int32_t)item;
     if (item >= 0) {
         const int2 id = nc.ids[d];
         nc.irec[item] = make_int4(id.x, id.y, (int32_t)nc.info[d], nc.cd[d]);
-        nc.iaux[item] = make_int2(row, (int32_t)d);
+        nc.iaux[item] = make_int2((int32_t)item, (int32_t)d);
     }
 }
-// next free state row of each scene = rows handed out (real and touching contacts); items
+// state rows handed out at the full run (real and touching contacts) and items of each scene
 template <typename T>
 __global__ void k_set_counters(const ConDesc<T>* C, const FRScene* FR, int n, const int64_t* rscan, const int64_t* nrscan) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1822,16 +1845,9 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
             *infoItem = (int32_t)ni;
             break;
         }
-        // store geometry/physics state
-        if (!real) {  // a contact becoming real keeps its earlier row or takes the next free one
-            row = aux.x;
-            if (row < 0) {
-                row = atomicAdd(cv.counter, 1);
-                if (row >= cv.cap) { atomicOr(err, ERR_CAPACITY); break; }
-                cv.slot[c] = row;
-                cv.iaux[item].x = row;
-            }
-        }
+        // store geometry/physics state; every item owns the state row of its own index, so
+        // contacts becoming real between full runs read and write their state contiguously
+        if (!real) row = aux.x;
         for (int k = 0; k < 3; k++) {
             cv.st[(size_t)(S_T + k) * cv.cap + row] = tr[3 * k];
             cv.st[(size_t)(S_T + 3 + k) * cv.cap + row] = tr[3 * k + 1];

@@ -608,7 +608,6 @@ struct Batch {
 void throwDeviceError(uint32_t e) {
     if (e & ERR_NAN) throw WdmError(WDM_E_NAN, "NaN force in contact");
     if (e & ERR_OVERSIZE) throw WdmError(WDM_E_OVERSIZE, "Particle larger than half of the cell size encountered.");
-    if (e & ERR_CAPACITY) throw WdmError(WDM_E_STATE, "contact state rows exhausted");
     throw WdmError(WDM_E_HIP, "device error");
 }
 // copy every scene's words to the host (synchronising) and raise the first device error among `ss`
