@@ -7,13 +7,14 @@
 //   particle arrays, P entries: ptype i32 (shape | axis<<4 | (sense+1)<<6), mask u32,
 //     mat i32, facet nodes int4; stored bounds bmin/bmax double4.
 //   contacts, C entries sorted by key = (min id << 32 | max id): ids int2 (functor
-//     order), info u32, packed cellDist i32, state row i32; state T[NSTATE][cap]
-//     component-major (trsf columns 0-1, local force, phys), rows of real contacts dense
-//     in key order after each full run,
-//     contact point per state row (FP64); work list of the ContactLoop: contacts holding a
-//     state row first (item == row), then the other non-dormant ones, dormant contacts left
-//     out; gather record per item gfc[2i] = (global force, cellDist), gfc[2i+1] = (contact
-//     point, uN).
+//     order), info u32, packed cellDist i32, state row i32; state (trsf columns 0-1,
+//     local force, phys): the rows handed out at a full run (real and touching contacts,
+//     dense in key order) component-major T[NSTATE][cap], the rows after them
+//     contact-major T[NSTRIDE] each; contact point per state row (FP64); work list of the
+//     ContactLoop: contacts given a row at the full run first (item == row), then the other
+//     non-dormant ones (item i owns row i), dormant contacts left out; per item the record
+//     (ids, info, cellDist) and (row, contact), and the gather record gfc[2i] = (global
+//     force, cellDist), gfc[2i+1] = (contact point, uN).
 //   CSR: off[P+1], entries u32 (contact index | isB << 31), per particle sorted by
 //     the other particle's id.
 // Batches: every kernel takes a table of per-scene descriptors (views of the arrays
@@ -53,6 +54,7 @@ __host__ __device__ inline GP T* gpc(T* p) { return (GP T*)p; }
 #define S_F 6      // local force, 3 entries
 #define S_P 9      // phys, 5 entries
 #define NSTATE 14
+#define NSTRIDE 16  // values per contact-major state row (NSTATE used): 128 bytes in FP64
 
 // device error bits
 #define ERR_NAN 1u
@@ -119,6 +121,7 @@ struct ConView {
     GP uint32_t* info;
     GP int32_t* slot;  // row of the contact's geometry/physics state, -1 when it has none
     int32_t* counter;  // state rows handed out at the last full run (real and touching contacts)
+    GP T* st2;         // state of the rows from *counter on (items that were not touching at the full run)
     GP int32_t* cd;
     GP T* st;
     GP double4* cpt;   // per state row: contact point of the last ContactLoop (x, y, z, 0)
@@ -131,6 +134,16 @@ struct ConView {
     GP double4* gfc;   // per item: gather record
     int64_t cap;
 };
+
+// State value k of row `row`.  Rows handed out at the full run (nR of them: real and
+// touching contacts, dense in key order) are component-major, so a wavefront of them reads
+// each value with one contiguous access; the rows after them belong to items that may
+// become real later, scattered among items that stay apart, and are contact-major, so that
+// such a contact reads one 128-byte line instead of a line per value.
+template <typename T>
+__device__ __forceinline__ GP T* stAt(const ConView<T>& cv, int64_t row, int k, int64_t nR) {
+    return row < nR ? cv.st + (size_t)k * cv.cap + row : cv.st2 + (size_t)(row - nR) * NSTRIDE + k;
+}
 
 __device__ __forceinline__ M3 ldM3(const GP M3& x) {
     M3 r;
@@ -1385,7 +1398,8 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_copy_state(const ConDesc<T>* C, c
     int64_t ns = rscan[F.newStart + d] - rscan[F.newStart];
     nc.slot[d] = (int32_t)ns;
     if (os < 0) return;
-    for (int k = 0; k < NSTATE; k++) nc.st[(size_t)k * nc.cap + ns] = oldc.st[(size_t)k * oldc.cap + os];
+    const int64_t nROld = *oldc.counter;
+    for (int k = 0; k < NSTATE; k++) nc.st[(size_t)k * nc.cap + ns] = *stAt(oldc, os, k, nROld);
     nc.cpt[ns] = oldc.cpt[os];
 }
 // work list of the ContactLoop: a contact given a state row at the full run (real or
@@ -1531,6 +1545,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
     // work list: dormant contacts are not in it; contacts with a state row come first, as
     // item == row, so their state is read and written contiguously
     const int64_t nItems = *cv.nItems;
+    const int64_t nR = *cv.counter;
     const int64_t item = lb * WDM_BLOCK + threadIdx.x;
     double ev[5] = {0., 0., 0., 0., 0.};  // elast, plast, viscN, viscT, broken
     uint32_t bits = 0;
@@ -1567,8 +1582,8 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
         // recomputes column 2 as their cross product), so only they are stored
         if (real) {
             for (int k = 0; k < 3; k++) {
-                tr[3 * k] = cv.st[(size_t)(S_T + k) * cv.cap + row];
-                tr[3 * k + 1] = cv.st[(size_t)(S_T + 3 + k) * cv.cap + row];
+                tr[3 * k] = *stAt(cv, row, S_T + k, nR);
+                tr[3 * k + 1] = *stAt(cv, row, S_T + 3 + k, nR);
                 tr[3 * k + 2] = (T)0;
             }
         }
@@ -1758,8 +1773,8 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
             }
             F[0] = F[1] = F[2] = 0;
         } else {
-            for (int k = 0; k < 5; k++) ph[k] = cv.st[(size_t)(S_P + k) * cv.cap + row];
-            for (int k = 0; k < 3; k++) F[k] = cv.st[(size_t)(S_F + k) * cv.cap + row];
+            for (int k = 0; k < 5; k++) ph[k] = *stAt(cv, row, S_P + k, nR);
+            for (int k = 0; k < 3; k++) F[k] = *stAt(cv, row, S_F + k, nR);
         }
         if (LAW == 0) {  // Law2_L6Geom_FrictPhys_IdealElPl::go, pkg/dem/IdealElPl.cpp:12-122
             const T kn = ph[0], kt = ph[1], tanPhi = ph[2];
@@ -1849,12 +1864,12 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
         // contacts becoming real between full runs read and write their state contiguously
         if (!real) row = aux.x;
         for (int k = 0; k < 3; k++) {
-            cv.st[(size_t)(S_T + k) * cv.cap + row] = tr[3 * k];
-            cv.st[(size_t)(S_T + 3 + k) * cv.cap + row] = tr[3 * k + 1];
+            *stAt(cv, row, S_T + k, nR) = tr[3 * k];
+            *stAt(cv, row, S_T + 3 + k, nR) = tr[3 * k + 1];
         }
-        for (int k = 0; k < 3; k++) cv.st[(size_t)(S_F + k) * cv.cap + row] = F[k];
+        for (int k = 0; k < 3; k++) *stAt(cv, row, S_F + k, nR) = F[k];
         if (!real) {
-            for (int k = 0; k < 5; k++) cv.st[(size_t)(S_P + k) * cv.cap + row] = ph[k];
+            for (int k = 0; k < 5; k++) *stAt(cv, row, S_P + k, nR) = ph[k];
             cv.info[c] = info | CI_REAL;
             *infoItem = (int32_t)(info | CI_REAL);
         }

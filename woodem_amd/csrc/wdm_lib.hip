@@ -134,7 +134,7 @@ struct ConBuf {
     DBuf<int32_t> cd, slot, counter, itm, nItems;
     DBuf<int4> irec;
     DBuf<int2> iaux;
-    DBuf<T> st;
+    DBuf<T> st, st2;
     DBuf<double4> cpt, gfc;
     int64_t cap = 0;
     void reserve(Ctx* c, int64_t n) {
@@ -143,17 +143,18 @@ struct ConBuf {
         free(c);
         ensure(c, key, nc); ensure(c, ids, nc); ensure(c, info, nc); ensure(c, cd, nc);
         ensure(c, slot, nc); ensure(c, counter, 1); ensure(c, itm, nc); ensure(c, irec, nc); ensure(c, iaux, nc); ensure(c, nItems, 1);
-        ensure(c, st, (size_t)NSTATE * nc); ensure(c, cpt, nc); ensure(c, gfc, 2 * (size_t)nc);
+        ensure(c, st, (size_t)NSTATE * nc); ensure(c, st2, (size_t)NSTRIDE * nc); ensure(c, cpt, nc); ensure(c, gfc, 2 * (size_t)nc);
         HIPCHK(hipMemsetAsync(nItems.p, 0, sizeof(int32_t), c->stream));
+        HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(int32_t), c->stream));
         cap = nc;
     }
     ConView<T> view() const {
         ConView<T> v; v.key = gpc(key.p); v.ids = gpc(ids.p); v.info = gpc(info.p); v.cd = gpc(cd.p); v.slot = gpc(slot.p);
-        v.counter = counter.p; v.st = gpc(st.p); v.cpt = gpc(cpt.p); v.itm = gpc(itm.p); v.irec = gpc(irec.p); v.iaux = gpc(iaux.p); v.nItems = nItems.p;
+        v.counter = counter.p; v.st = gpc(st.p); v.st2 = gpc(st2.p); v.cpt = gpc(cpt.p); v.itm = gpc(itm.p); v.irec = gpc(irec.p); v.iaux = gpc(iaux.p); v.nItems = nItems.p;
         v.gfc = gpc(gfc.p); v.cap = cap; return v;
     }
     void free(Ctx* c) {
-        release(c, key); release(c, ids); release(c, info); release(c, cd); release(c, st); release(c, gfc);
+        release(c, key); release(c, ids); release(c, info); release(c, cd); release(c, st); release(c, st2); release(c, gfc);
         release(c, slot); release(c, counter); release(c, itm); release(c, irec); release(c, iaux); release(c, nItems); release(c, cpt); cap = 0;
     }
 };
@@ -1443,8 +1444,9 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
     std::vector<int2> ids(n);
     std::vector<uint32_t> info(n);
     std::vector<int32_t> cd(n), slot(n), itm(n);
-    std::vector<T> st((size_t)NSTATE * std::max<int64_t>(b.cap, 1));
+    std::vector<T> st((size_t)NSTATE * std::max<int64_t>(b.cap, 1)), st2((size_t)NSTRIDE * std::max<int64_t>(b.cap, 1));
     std::vector<double4> gfc(2 * (size_t)n);
+    int32_t nR = 0;
     if (n > 0) {
         HIPCHK(hipMemcpyAsync(ids.data(), b.ids.p, n * sizeof(int2), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(info.data(), b.info.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
@@ -1452,9 +1454,13 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
         HIPCHK(hipMemcpyAsync(slot.data(), b.slot.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(itm.data(), b.itm.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(st.data(), b.st.p, (size_t)NSTATE * b.cap * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(st2.data(), b.st2.p, (size_t)NSTRIDE * b.cap * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(&nR, b.counter.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(gfc.data(), b.gfc.p, 2 * n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));  // items <= contacts
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+    // state value q of a row (the layout of the kernels' stAt)
+    auto stv = [&](int64_t row, int q) -> T { return row < nR ? st[(size_t)q * b.cap + row] : st2[(size_t)(row - nR) * NSTRIDE + q]; };
     int64_t k = 0;
     for (int64_t i = 0; i < n && k < cap; i++) {
         if (info[i] & CI_DEAD) continue;
@@ -1465,7 +1471,7 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
         if (isReal) isReal[k] = real ? 1 : 0;
         int64_t row = slot[i];
         for (int d = 0; d < 3; d++) {
-            if (forceLocal3) forceLocal3[3 * k + d] = real ? (double)st[(size_t)(S_F + d) * b.cap + row] : 0.;
+            if (forceLocal3) forceLocal3[3 * k + d] = real ? (double)stv(row, S_F + d) : 0.;
         }
         // contact point and uN exist for real contacts only (L6Geom); zeros otherwise
         const double4 rec = (real && itm[i] >= 0) ? gfc[2 * (size_t)itm[i] + 1] : make_double4(0., 0., 0., 0.);
@@ -1474,7 +1480,7 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
         if (trsf9) {  // columns 0 and 1 stored, column 2 = col0 x col1 in the state's precision
             if (real && row >= 0) {
                 T c0[3], c1[3];
-                for (int q = 0; q < 3; q++) { c0[q] = st[(size_t)(S_T + q) * b.cap + row]; c1[q] = st[(size_t)(S_T + 3 + q) * b.cap + row]; }
+                for (int q = 0; q < 3; q++) { c0[q] = stv(row, S_T + q); c1[q] = stv(row, S_T + 3 + q); }
                 const T c2[3] = {c0[1] * c1[2] - c0[2] * c1[1], c0[2] * c1[0] - c0[0] * c1[2], c0[0] * c1[1] - c0[1] * c1[0]};
                 for (int q = 0; q < 3; q++) {
                     trsf9[9 * k + 3 * q] = (double)c0[q];
@@ -1600,7 +1606,7 @@ size_t wdm_scene_footprint(int32_t n_par, int32_t n_node, double cpp) {
     size_t nodeB = (size_t)n_node * (7 * 32 + 4 + 1);
     size_t parB = (size_t)n_par * (4 + 4 + 4 + 16 + 64 + 8 + 16);
     double ncon = cpp * n_par * 1.34;
-    size_t conB = (size_t)(ncon * 2 * (8 + 8 + 4 + 4 + 4 + 4 + NSTATE * 8 + 32 + 64));
+    size_t conB = (size_t)(ncon * 2 * (8 + 8 + 4 + 4 + 4 + 4 + (NSTATE + NSTRIDE) * 8 + 32 + 64));
     size_t csrB = (size_t)(ncon * 2 * (4 + 1));
     size_t scratch = (size_t)n_par * 64 + (size_t)(ncon * 48);
     return nodeB + parB + conB + csrB + scratch + (1u << 20);
