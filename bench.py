@@ -71,6 +71,18 @@ def cpu_baseline(ctx, n_sample, seconds):
                     n_sample, k, el))
 
 
+def measured_traffic(args):
+    """HBM bytes per k_contacts launch (read + write) of this workload, from the rocprofv3
+    FETCH_SIZE/WRITE_SIZE runs recorded in profiles/ (scaled by a device copy of known
+    size); None for other sizes or precisions."""
+    path = os.path.join(ROOT, "profiles", "traffic_periodic_hertz_1M_fp64.json")
+    if args.n != 1_000_000 or args.precision != "fp64" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t["hbm_read_bytes_per_launch"] + t["hbm_write_bytes_per_launch"]
+
+
 def timed_kernels(ctx, sids, steps):
     """HIP-event kernel times over `steps` steps launched one at a time (window 1), so that
     every launch of a step is counted once."""
@@ -254,7 +266,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--relax", type=int, default=500)
     ap.add_argument("--latency-steps", type=int, default=200)
-    ap.add_argument("--timing-steps", type=int, default=50)
+    ap.add_argument("--timing-steps", type=int, default=300,
+                    help="steps timed kernel by kernel with HIP events (one at a time; spans collider intervals)")
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -399,7 +412,7 @@ def main():
                    "full_collider_runs_in_timed_region": st1["nFullRuns"] - st0["nFullRuns"], "launch": launch_config(ctx, None)},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)), "samples": len(lat)},
         "roofline": {"bound": "hbm", "kernel": "k_contacts", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
                      "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_ms},
         "kernels": kernels,
         "mixed_precision": mixed,
