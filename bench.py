@@ -381,8 +381,11 @@ def main():
         mixed = {"value": b.n_par * args.steps / elm, "ms_per_step": elm / args.steps * 1e3,
                  "speedup_vs_fp64": (b.n_par * args.steps / elm) / value, "contacts_avg_ms": msm / max(lam, 1),
                  "contacts_real": int(cm["isReal"].sum())}
-        log("mixed precision: %s" % json.dumps(mixed))
         rm.close()
+        from woodem_amd.precision import mixed_error_report
+        mixed["errors_vs_fp64"] = mixed_error_report(ctx)
+        log("mixed precision errors: %s" % json.dumps(mixed["errors_vs_fp64"]))
+        log("mixed precision: %s" % json.dumps(mixed))
         run = None
 
     cpu = None

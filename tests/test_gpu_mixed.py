@@ -63,3 +63,15 @@ def test_mixed_long_run_statistics(hip_ctx):
     assert abs(int(n64) - int(n32)) <= max(3, .05 * n64)
     g64.close()
     g32.close()
+
+
+def test_mixed_error_report(hip_ctx):
+    """The report bench.py carries for the mixed-precision line (config-1 scene, 1 and 1000
+    steps): one-step force error within the workload's 1e-5, same contact set at one step."""
+    from woodem_amd.precision import mixed_error_report
+    r = mixed_error_report(hip_ctx, settle=6000, long_steps=200)
+    s1 = r["step_1"]
+    assert s1["contact_set_equal"] and s1["real_fp64"] > 1000
+    assert s1["force_rel_max"] <= 1e-5 and s1["force_rel_mean"] <= s1["force_rel_max"] and s1["torque_rel_max"] <= 1e-5
+    assert 0 <= r["step_200"]["realness_agreement"] <= 1
+    assert abs(r["energy"]["balance_drift_fp64"]) < 1e-2
