@@ -963,6 +963,12 @@ struct InsertionSortCollider {
             noorTot += noor[t];
             cb[1 - S.cur].reserve(c, std::max<int64_t>(nnew, 1));
             if (cb[S.cur].cap == 0) cb[S.cur].reserve(c, 1);
+            if (S.nc == 0 && cb[S.cur].cap < cb[1 - S.cur].cap) {
+                // first full run: the list that holds nothing yet gets the same capacity, so
+                // that the next full run (its merge target) does not reallocate
+                HIPCHK(hipStreamSynchronize(st));
+                cb[S.cur].reserve(c, std::max<int64_t>(nnew, 1));
+            }
             ensure(c, S.entV, (size_t)std::max<int64_t>(2 * nnew, 1));
             C[t].cv = cb[S.cur].view();
             C[t].cvNew = cb[1 - S.cur].view();
