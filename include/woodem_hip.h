@@ -154,7 +154,38 @@ int wdm_leapfrog_run(wdm_ctx* ctx, int32_t scene);
 int wdm_collider_run(wdm_ctx* ctx, int32_t scene);
 int wdm_contact_loop_run(wdm_ctx* ctx, int32_t scene);
 int wdm_step_end(wdm_ctx* ctx, int32_t scene);
+/* DynDt::run (pkg/dem/DynDt.cpp:72-97) when PeriodicEngine::isActivated (core/Engine.cpp:169-195)
+   says so at the current step: the engine call after ContactLoop (see wdm_set_dyndt) */
+int wdm_dyndt_run(wdm_ctx* ctx, int32_t scene);
 int wdm_step(wdm_ctx* ctx, const int32_t* scenes, int32_t n_scenes, int32_t n_steps);
+
+/* DynDt (pkg/dem/DynDt.hpp:20-26, DynDt.cpp:72-97) placed after ContactLoop in the engine
+   list, with the PeriodicEngine attributes and state it runs by (core/Engine.hpp:141-153,
+   PeriodicEngine::isActivated core/Engine.cpp:169-195).  The state fields (nDone,
+   stepLast, stepPrev, virtLast, virtPrev) and `dt` (the value a dryRun would apply) are
+   both set and read back, so a host-side engine object keeps them between calls as the
+   reference's attributes do.  realPeriod (wall-clock time) is not supported: the steps it
+   would fire at are not known when a window of steps is launched.  A new dt takes effect
+   at the step's epilogue (core/Scene.cpp:400).  Rejected (WDM_E_UNSUPPORTED): scenes whose
+   facet nodes have free DOFs and are not clumped (their stiffness comes from the facets). */
+typedef struct wdm_dyndt {
+    double maxRelInc;   /* DynDt.maxRelInc (1e-4); 1+maxRelInc must differ from 1 */
+    double dtSafety;    /* Scene.dtSafety (.9) */
+    double virtPeriod;  /* PeriodicEngine.virtPeriod (<= 0: off) */
+    double virtLast, virtPrev;
+    double dt;          /* DynDt.dt: the dryRun result (NaN otherwise) */
+    int64_t stepPeriod; /* PeriodicEngine.stepPeriod (<= 0: off) */
+    int64_t nDo, nDone, stepLast, stepPrev;
+    int32_t enabled;    /* the engine is in the list and not dead */
+    int32_t stepModulo, initRun, dryRun;
+} wdm_dyndt;
+int wdm_set_dyndt(wdm_ctx* ctx, int32_t scene, const wdm_dyndt* dyndt);
+int wdm_get_dyndt(wdm_ctx* ctx, int32_t scene, wdm_dyndt* dyndt);
+/* DynDt::critDt_compute (DynDt.cpp:61-70) on the contacts of the last ContactLoop:
+   sqrt(min over nodes of the squared critical step); +inf when no node has stiffness */
+int wdm_dyndt_crit(wdm_ctx* ctx, int32_t scene, double* crit_dt);
+/* Scene.time and Scene.dt (DynDt may have changed dt) */
+int wdm_get_time(wdm_ctx* ctx, int32_t scene, double* time, double* dt);
 
 /* Launch-shape knobs of wdm_step (no counterpart in the reference; results do not depend
    on them): `window` steps between two looks of the host at the Verlet triggers, launched

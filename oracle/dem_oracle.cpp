@@ -1051,6 +1051,92 @@ struct Oracle {
         }
     }
 
+    /* ---------- DynDt, pkg/dem/DynDt.cpp:16-97 ---------- */
+    orc_dyndt dd{};
+    double time = 0, nextDt = NAN;
+    /* inner loop of DynDt::nodalStiffAdd (:19-34) over the contacts of particle p, in
+       contact-map order (other id ascending); the contact node's orientation is
+       Quaternionr(trsf) (L6Geom.cpp:69,216), which rotates the x axis to trsf column 0 */
+    void stiffAddParticle(int p, V3& kt, V3& kr) {
+        if (adjDirty) rebuildAdjacency();
+        for (long k = adjOff[p]; k < adjOff[p + 1]; k++) {
+            const Contact& C = *adjC[k];
+            if (!C.isReal()) continue;
+            const int ix = (C.idA == p) ? 0 : 1;
+            const V3 n = mcol(C.trsf, 0);
+            const V3 n2 = v3(n[0] * n[0], n[1] * n[1], n[2] * n[2]);
+            for (int a = 0; a < 3; a++) kt[a] = kt[a] + (n2[a] * (C.kn - C.kt) + C.kt);
+            const double lk = (C.lens[ix] * C.lens[ix]) * C.kt;
+            kr[0] = kr[0] + lk * (n2[1] + n2[2]);
+            kr[1] = kr[1] + lk * (n2[2] + n2[0]);
+            kr[2] = kr[2] + lk * (n2[0] + n2[1]);
+        }
+    }
+    bool uninodalAt(int i) const { return i < np && (shape[i] == ORC_SHAPE_SPHERE || shape[i] == ORC_SHAPE_WALL) && nodeOf(i) == i; }
+    /* DynDt::nodalCritDtSq, :39-58 */
+    double nodalCritDtSq(int i) {
+        const uint32_t fl = flags[i];
+        if ((fl & DOF_ALL) == DOF_ALL) return INF;
+        V3 kt = vzero(), kr = vzero();
+        if (uninodalAt(i)) stiffAddParticle(i, kt, kr);
+        if (fl & CLUMP_CLUMP) {
+            const int c = nodeClump[i];
+            for (int k = clumpOff[c]; k < clumpOff[c + 1]; k++)
+                if (uninodalAt(clumpMem[k])) stiffAddParticle(clumpMem[k], kt, kr);
+        }
+        double ret = INF;
+        for (int a = 0; a < 3; a++) if (kt[a] != 0 && mass[i] > 0. && !(fl & axisDOF(a, false))) ret = std::min(ret, mass[i] / std::abs(kt[a]));
+        for (int a = 0; a < 3; a++) if (kr[a] != 0 && inertia[i][a] > 0. && !(fl & axisDOF(a, true))) ret = std::min(ret, inertia[i][a] / std::abs(kr[a]));
+        return 2 * ret;
+    }
+    /* DynDt::critDt_stiffness, :61-70, over DemField.nodes (clump members are not among
+       them; facet vertex nodes must be blocked or clumped: their facets' contacts are not
+       restated) */
+    double critDtStiffness() {
+        std::vector<char> facetNode(nn, 0);
+        for (int p = 0; p < np; p++)
+            if (shape[p] == ORC_SHAPE_FACET) for (int k = 0; k < 3; k++) facetNode[nodeOf(p, k)] = 1;
+        double ret = INF;
+        for (int i = 0; i < nn; i++) {
+            if (flags[i] & CLUMP_CLUMPED) {
+                if (facetNode[i] && (flags[nodeMaster[i]] & DOF_ALL) != DOF_ALL) throw std::runtime_error("DynDt: facet nodes with free DOFs are not restated");
+                continue;
+            }
+            if (i < np && !uninodalAt(i)) continue;  // no node of its own (facet particles)
+            if (facetNode[i] && (flags[i] & DOF_ALL) != DOF_ALL) throw std::runtime_error("DynDt: facet nodes with free DOFs are not restated");
+            ret = std::min(ret, nodalCritDtSq(i));
+        }
+        return std::sqrt(ret);
+    }
+    /* PeriodicEngine::isActivated, core/Engine.cpp:169-195 (no wall-clock criterion) */
+    bool ddActivate() {
+        const long stepNow = step;
+        const double virtNow = time;
+        const bool initNow = dd.stepLast < 0;
+        if (initNow) { dd.virtLast = virtNow; dd.stepLast = stepNow; }
+        if ((dd.nDo < 0 || dd.nDone < dd.nDo) &&
+            ((dd.virtPeriod > 0 && virtNow - dd.virtLast >= dd.virtPeriod) ||
+             (dd.stepPeriod > 0 && ((stepNow - dd.stepLast >= dd.stepPeriod) || (dd.stepModulo && stepNow % dd.stepPeriod == 0))))) {
+            dd.virtPrev = dd.virtLast; dd.virtLast = virtNow;
+            dd.stepPrev = dd.stepLast; dd.stepLast = stepNow;
+            if (initNow && !dd.initRun) return false;
+            dd.nDone++;
+            return true;
+        }
+        return false;
+    }
+    /* DynDt::run, :72-97 */
+    void dynDtRun() {
+        const double crDt = critDtStiffness();
+        if (std::isinf(crDt)) return;
+        const long nSteps = step - dd.stepPrev;
+        const double maxNewDt = P.dt * std::pow(1. + dd.maxRelInc, nSteps);
+        const double nd = std::min(crDt * dd.dtSafety, maxNewDt);
+        if (!dd.dryRun) {
+            if (nd != P.dt) { dd.dt = NAN; nextDt = nd; }
+        } else dd.dt = nd;
+    }
+
     /* Scene::doOneStep, core/Scene.cpp:319-424 */
     void doOneStep() {
         if (periodic) cellIntegrateAndUpdate(P.dt);
@@ -1058,8 +1144,11 @@ struct Oracle {
         leapfrog();
         collider();
         contactLoop();
+        if (dd.enabled && ddActivate()) dynDtRun();
         if (periodic) cellSetNextGradV();
         step++;
+        time += P.dt;
+        if (!std::isnan(nextDt)) { P.dt = nextDt; nextDt = NAN; }
     }
 };
 
@@ -1252,6 +1341,18 @@ int64_t orc_bruteforce_pairs(void* h, int64_t cap, int32_t* idMin, int32_t* idMa
         }
     return n;
 }
+int orc_set_dyndt(void* h, const orc_dyndt* d) {
+    if (d->enabled && 1. + d->maxRelInc == 1.) { O(h)->err = "DynDt: maxRelInc too small (1.0+maxRelInc==1.0)"; return -1; }
+    O(h)->dd = *d;
+    return 0;
+}
+int orc_get_dyndt(void* h, orc_dyndt* d) { *d = O(h)->dd; return 0; }
+int orc_dyndt_crit(void* h, double* critDt) {
+    try { *critDt = O(h)->critDtStiffness(); }
+    catch (std::exception& e) { O(h)->err = e.what(); return -1; }
+    return 0;
+}
+int orc_get_time(void* h, double* time, double* dt) { *time = O(h)->time; *dt = O(h)->P.dt; return 0; }
 double orc_pwave_dt(void* h) {
     Oracle* o = O(h);
     double dt = INF;

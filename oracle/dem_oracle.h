@@ -69,6 +69,20 @@ int orc_set_params(void* h, const orc_params* p);
 int orc_set_clumps(void* h, int32_t nc, const int32_t* master, const int32_t* off, const int32_t* members,
                    const double* relPos3, const double* relOri4);
 
+/* DynDt (pkg/dem/DynDt.hpp:20-26) after ContactLoop, with the PeriodicEngine attributes and
+   state it runs by (core/Engine.hpp:141-153); realPeriod is not restated */
+typedef struct orc_dyndt {
+    double maxRelInc, dtSafety, virtPeriod, virtLast, virtPrev, dt;
+    int64_t stepPeriod, nDo, nDone, stepLast, stepPrev;
+    int32_t enabled, stepModulo, initRun, dryRun;
+} orc_dyndt;
+int orc_set_dyndt(void* h, const orc_dyndt* d);
+int orc_get_dyndt(void* h, orc_dyndt* d);
+/* DynDt::critDt_compute (pkg/dem/DynDt.cpp:61-70) on the current contacts */
+int orc_dyndt_crit(void* h, double* critDt);
+/* Scene.time, Scene.dt */
+int orc_get_time(void* h, double* time, double* dt);
+
 /* run n full timesteps (Scene::doOneStep) */
 int orc_step(void* h, int32_t n);
 

@@ -7,7 +7,8 @@ Parity-pinning status of the oracle's functions (details in DESIGN.md):
     bounds, contact set (vs brute force), sphere/wall/facet L6Geom creation,
     FrictPhys, IdealElPl slip cap and dissipation, Hertz collision time,
     restitution, ballistic leapfrog, damping, GRADV2 cell, incident velocity
-    under PBC, energy balance;
+    under PBC, energy balance, DynDt (single-contact stiffness closed forms,
+    maxRelInc cap, dryRun, PeriodicEngine activation);
   parity unpinned (no reference vector exists; checked only by invariants):
     multi-step L6Geom trsf evolution with renormalisation, IdealElPl Ft history
     over many steps, summation order of forces and energies.
@@ -69,6 +70,10 @@ def lib():
         "orc_get_cell": (C.c_int, [P, pd, pd]),
         "orc_bruteforce_pairs": (I64, [P, I64, pi, pi]),
         "orc_pwave_dt": (D, [P]),
+        "orc_set_dyndt": (C.c_int, [P, C.c_void_p]),
+        "orc_get_dyndt": (C.c_int, [P, C.c_void_p]),
+        "orc_dyndt_crit": (C.c_int, [P, pd]),
+        "orc_get_time": (C.c_int, [P, pd, pd]),
     }
     for n, (r, a) in sig.items():
         f = getattr(L, n)
@@ -121,6 +126,10 @@ class OracleRun:
         self._chk(L.orc_set_cell(self.h, int(cell["periodic"]), _p(h9, D), _p(g9, D), _p(n9, D), int(cell["homoDeform"])))
         op = params if isinstance(params, OrcParams) else orc_params_from(params, nThreads)
         self._chk(L.orc_set_params(self.h, C.byref(op)))
+
+    def set_params(self, params, nThreads=0):
+        op = params if isinstance(params, OrcParams) else orc_params_from(params, nThreads)
+        self._chk(self.L.orc_set_params(self.h, C.byref(op)))
 
     def _chk(self, rc):
         if rc != 0:
@@ -175,6 +184,26 @@ class OracleRun:
         a, b = np.empty(n, np.int32), np.empty(n, np.int32)
         self.L.orc_bruteforce_pairs(self.h, n, _p(a, C.c_int32), _p(b, C.c_int32))
         return a, b
+
+    def set_dyndt(self, d):
+        """DynDt engine (a woodem_amd._native.WdmDynDt: orc_dyndt has the same layout)."""
+        self._chk(self.L.orc_set_dyndt(self.h, C.addressof(d)))
+
+    def dyndt(self):
+        from woodem_amd._native import WdmDynDt
+        d = WdmDynDt()
+        self._chk(self.L.orc_get_dyndt(self.h, C.addressof(d)))
+        return d
+
+    def dyndt_crit(self):
+        v = C.c_double()
+        self._chk(self.L.orc_dyndt_crit(self.h, C.byref(v)))
+        return v.value
+
+    def time(self):
+        t, d = C.c_double(), C.c_double()
+        self._chk(self.L.orc_get_time(self.h, C.byref(t), C.byref(d)))
+        return t.value, d.value
 
     def pwave_dt(self):
         return float(self.L.orc_pwave_dt(self.h))

@@ -59,7 +59,13 @@ def test_hertz_engines_params():
 def test_unsupported_configurations_raise():
     S = small_scene()
     S.engines = utils.defaultEngines(damping=.2)  # DynDt included by default (dynDtPeriod=100)
+    S._params()
+    S.engines[-1].realPeriod = 5.  # wall-clock periodicity cannot be known ahead of a window
     with pytest.raises(NotImplementedError):
+        S.engines[-1]._native(.9)
+    S.engines = utils.defaultEngines(damping=.2)
+    S.engines = [S.engines[-1]] + S.engines[:-1]  # DynDt must follow ContactLoop
+    with pytest.raises(ValueError):
         S._params()
     S.engines = utils.defaultEngines(damping=.2, dynDtPeriod=0)
     S.engines[2].lawFunctors = [woo.Law2_L6Geom_HertzPhys_DMT()]

@@ -32,6 +32,31 @@ class WdmParams(C.Structure):
     ]
 
 
+class WdmDynDt(C.Structure):
+    """wdm_dyndt: DynDt attributes with its PeriodicEngine attributes and state."""
+    _fields_ = [
+        ("maxRelInc", C.c_double), ("dtSafety", C.c_double), ("virtPeriod", C.c_double), ("virtLast", C.c_double),
+        ("virtPrev", C.c_double), ("dt", C.c_double),
+        ("stepPeriod", C.c_int64), ("nDo", C.c_int64), ("nDone", C.c_int64), ("stepLast", C.c_int64), ("stepPrev", C.c_int64),
+        ("enabled", C.c_int32), ("stepModulo", C.c_int32), ("initRun", C.c_int32), ("dryRun", C.c_int32),
+    ]
+
+
+DYNDT_FIELDS = [f for f, _ in WdmDynDt._fields_]
+
+
+def make_dyndt(**kw):
+    """WdmDynDt with the reference's defaults (DynDt.hpp:22-24, core/Engine.hpp:141-153)
+    overridden by keyword arguments."""
+    d = WdmDynDt()
+    defaults = dict(maxRelInc=1e-4, dtSafety=.9, virtPeriod=0., virtLast=float("nan"), virtPrev=-1., dt=float("nan"),
+                    stepPeriod=1, nDo=-1, nDone=0, stepLast=-1, stepPrev=-1, enabled=1, stepModulo=1, initRun=1, dryRun=0)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        setattr(d, k, v)
+    return d
+
+
 class WdmError(RuntimeError):
     """Error returned by the C ABI; `code` is the WDM_E_* value."""
 
@@ -84,6 +109,11 @@ def lib():
         "wdm_set_tuning": (C.c_int, [P, I32, I32, D]),
         "wdm_get_tuning": (C.c_int, [P, pi, pi, pd, pl, pl, pl]),
         "wdm_relax_overlaps": (C.c_int, [P, I32, I32]),
+        "wdm_set_dyndt": (C.c_int, [P, I32, C.POINTER(WdmDynDt)]),
+        "wdm_get_dyndt": (C.c_int, [P, I32, C.POINTER(WdmDynDt)]),
+        "wdm_dyndt_crit": (C.c_int, [P, I32, pd]),
+        "wdm_dyndt_run": (C.c_int, [P, I32]),
+        "wdm_get_time": (C.c_int, [P, I32, pd, pd]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -255,6 +285,30 @@ class Context:
             nm = raw[64 * i:64 * i + 64].split(b"\0", 1)[0].decode()
             out[nm] = (float(ms[i]), int(la[i]))
         return out
+
+    def set_dyndt(self, sid, d):
+        """DynDt engine of scene `sid` (a WdmDynDt, see make_dyndt; enabled=0 removes it)."""
+        self.check(self.L.wdm_set_dyndt(self.h, sid, C.byref(d)))
+
+    def dyndt(self, sid):
+        d = WdmDynDt()
+        self.check(self.L.wdm_get_dyndt(self.h, sid, C.byref(d)))
+        return d
+
+    def dyndt_crit(self, sid):
+        """DynDt::critDt_compute on the contacts of the last ContactLoop."""
+        v = C.c_double()
+        self.check(self.L.wdm_dyndt_crit(self.h, sid, C.byref(v)))
+        return v.value
+
+    def dyndt_run(self, sid):
+        self.check(self.L.wdm_dyndt_run(self.h, sid))
+
+    def time(self, sid):
+        """(Scene.time, Scene.dt)"""
+        t, d = C.c_double(), C.c_double()
+        self.check(self.L.wdm_get_time(self.h, sid, C.byref(t), C.byref(d)))
+        return t.value, d.value
 
     def relax(self, sid, iters):
         self.check(self.L.wdm_relax_overlaps(self.h, sid, int(iters)))

@@ -101,8 +101,12 @@ class Scene(_dem._Attrs):
     def _params(self):
         eng = [e for e in self.engines if not getattr(e, "dead", False)]
         kinds = [type(e) for e in eng]
-        if any(isinstance(e, _dem.DynDt) for e in eng):
-            raise NotImplementedError("DynDt is not implemented on this path; use defaultEngines(dynDtPeriod=0) and a fixed dt")
+        dyn = [e for e in eng if isinstance(e, _dem.DynDt)]
+        if dyn:
+            if len(dyn) > 1 or eng[-1] is not dyn[0]:
+                raise ValueError("DynDt must be the last engine (after ContactLoop)")
+            eng = eng[:-1]
+            kinds = kinds[:-1]
         if len(eng) != 3 or not (isinstance(eng[0], _dem.Leapfrog) and isinstance(eng[1], _dem.InsertionSortCollider)
                                  and isinstance(eng[2], _dem.ContactLoop)):
             raise ValueError("engines must be [Leapfrog, InsertionSortCollider, ContactLoop] (got %s)" % [k.__name__ for k in kinds])
@@ -153,11 +157,37 @@ class Scene(_dem._Attrs):
             raise ValueError("unsupported Cp2 functor %s" % type(cp2).__name__)
         if cp2.tanPhi is not None:
             raise ValueError("MatchMaker tanPhi is outside this path")
-        if math.isnan(p.dt) or p.dt < 0:
-            # Scene::doOneStep auto-dt: dtSafety * min critDt (core/Scene.cpp:333-355)
-            p.dt = self.dtSafety * S_dem.critDt()
+        if math.isnan(p.dt) or math.isinf(p.dt) or p.dt < 0:
+            # Scene::doOneStep auto-dt: dtSafety * min critDt of the engines and fields
+            # (core/Scene.cpp:333-355): DemField's p-wave step, DynDt's stiffness step once
+            # there are contacts
+            crit = S_dem.critDt()
+            if dyn and self._sid is not None:
+                crit = min(crit, self._ctx.dyndt_crit(self._sid))
+            p.dt = self.dtSafety * crit
             self.dt = p.dt
         return p
+
+    def _dyndt_engine(self):
+        d = [e for e in self.engines if isinstance(e, _dem.DynDt) and not getattr(e, "dead", False)]
+        return d[0] if d else None
+
+    def _set_dyndt(self):
+        from ._native import make_dyndt
+        e = self._dyndt_engine()
+        self._ctx.set_dyndt(self._sid, e._native(self.dtSafety) if e else make_dyndt(enabled=0))
+
+    def _after_run(self):
+        """Scene.dt/time and the DynDt engine's state after steps on the device."""
+        t, dt = self._ctx.time(self._sid)
+        self._time, self.dt = t, dt
+        e = self._dyndt_engine()
+        if e is not None:
+            e._update(self._ctx.dyndt(self._sid))
+
+    @property
+    def time(self):
+        return getattr(self, "_time", 0.)
 
     def _ensure(self):
         f = self.dem
@@ -174,11 +204,17 @@ class Scene(_dem._Attrs):
         c = self.cell
         self._ctx.set_cell(self._sid, self.periodic, c.hSize, c.gradV, c.nextGradV, c.homoDeform)
         self._ctx.set_params(self._sid, self._params())
+        self._set_dyndt()
 
     def run(self, nSteps=1, wait=True):
         first = self._sid is None or self.dem._dirty
-        self._ensure() if first else self._ctx.set_params(self._sid, self._params())
+        if first:
+            self._ensure()
+        else:
+            self._ctx.set_params(self._sid, self._params())
+            self._set_dyndt()
         self._ctx.step([self._sid], int(nSteps))
+        self._after_run()
         self._stale = True
         self._con_cache = None
         h, g = self._ctx.cell(self._sid)

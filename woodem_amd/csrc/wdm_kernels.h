@@ -54,8 +54,9 @@ __host__ __device__ inline GP T* gpc(T* p) { return (GP T*)p; }
 #define S_T 0      // trsf columns 0 and 1 (x, y, z each); column 2 is their cross product
 #define S_F 6      // local force, 3 entries
 #define S_P 9      // phys, 5 entries
-#define NSTATE 14
-#define NSTRIDE 16  // values per contact-major state row (NSTATE used): 128 bytes in FP64
+#define S_L 14     // L6Geom.lens, 2 entries (written at creation, read by DynDt)
+#define NSTATE 16
+#define NSTRIDE 16  // values per contact-major state row: 128 bytes in FP64
 
 // device error bits
 #define ERR_NAN 1u
@@ -1928,6 +1929,8 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
         for (int k = 0; k < 3; k++) *stAt(cv, row, S_F + k, nR) = F[k];
         if (!real) {
             for (int k = 0; k < 5; k++) *stAt(cv, row, S_P + k, nR) = ph[k];
+            *stAt(cv, row, S_L, nR) = lens0;
+            *stAt(cv, row, S_L + 1, nR) = lens1;
             cv.info[c] = info | CI_REAL;
             *infoItem = (int32_t)(info | CI_REAL);
         }
@@ -1944,6 +1947,106 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
     int64_t w0 = lb * WDM_BLOCK + (threadIdx.x & ~63);
     if ((threadIdx.x & 63) == 0 && w0 < nItems) realMask[w0 >> 6] = bal;
     if (a.trackEnergy) blockSumWrite<5>(ev, bits, epart, nullptr, 0u, creOut);
+}
+
+// ---------------- DynDt, pkg/dem/DynDt.cpp:16-70 ----------------
+// nodalStiffAdd (:16-36) for the contacts of particle q: every real contact adds
+// n2*(kn-kt)+kt to the translational and lens^2*kt*(n2y+n2z, n2z+n2x, n2x+n2y) to the
+// rotational stiffness, n2 the squared components of the contact normal (trsf column 0,
+// which the contact node's orientation rotates the x axis to) and lens the L6Geom branch of
+// q's side.  Contacts come in contact-map order (other particle's id ascending: the CSR
+// order).  HertzPhys stiffnesses are those of its last law call (Hertz.cpp:69,157).
+template <typename T, int LAW>
+__device__ __forceinline__ void stiffAdd(const ConView<T>& cv, int64_t nR, const GatherView& gv, int q, double (&kt)[3],
+                                         double (&kr)[3]) {
+    const int64_t e0 = gv.off[q], e1 = gv.off[q + 1];
+    for (int64_t e = e0; e < e1; e++) {
+        const uint32_t v = gv.ent[e];
+        const uint32_t c = v & 0x7fffffffu;  // work-list item, which owns state row c
+        if (!((gv.realMask[c >> 6] >> (c & 63)) & 1ull)) continue;  // !C->isReal()
+        const int ix = (int)(v >> 31);
+        const double nx = (double)*stAt(cv, c, S_T + 0, nR), ny = (double)*stAt(cv, c, S_T + 1, nR), nz = (double)*stAt(cv, c, S_T + 2, nR);
+        double kn, kT;
+        if (LAW == 0) {
+            kn = (double)*stAt(cv, c, S_P + 0, nR);
+            kT = (double)*stAt(cv, c, S_P + 1, nR);
+        } else {
+            const double uN = gv.gfc[2 * (size_t)c + 1].w;
+            const double kn0 = (double)*stAt(cv, c, S_P + 0, nR) * sqrt((double)*stAt(cv, c, S_P + 1, nR));
+            kn = (3 / 2.) * kn0 * sqrt(-uN);
+            kT = (double)*stAt(cv, c, S_P + 3, nR) * sqrt(uN < 0 ? -uN : 0.);
+        }
+        const double l = (double)*stAt(cv, c, S_L + ix, nR);
+        const double n2x = nx * nx, n2y = ny * ny, n2z = nz * nz;
+        kt[0] = kt[0] + (n2x * (kn - kT) + kT);
+        kt[1] = kt[1] + (n2y * (kn - kT) + kT);
+        kt[2] = kt[2] + (n2z * (kn - kT) + kT);
+        const double lk = (l * l) * kT;
+        kr[0] = kr[0] + lk * (n2y + n2z);
+        kr[1] = kr[1] + lk * (n2z + n2x);
+        kr[2] = kr[2] + lk * (n2x + n2y);
+    }
+}
+// Per node of DemField.nodes (clump members are not among them; the dummy node of a facet
+// particle is no node of the reference): nodalCritDtSq (:39-58), the squared critical step
+// 2 min(m/|k_t,i|, I/|k_r,i|) over the free DOFs, infinite for a node with every DOF
+// blocked; a clump master adds its members' contacts after its own (:48-51).  The scene's
+// minimum (critDt_stiffness, :61-70) is an exact min, reduced per block and then with an
+// atomic min on the bit pattern (non-negative doubles order like their bits), so it does
+// not depend on the order.  out[t] must hold +inf before the launch.
+template <typename T, int LAW>
+__global__ void __launch_bounds__(WDM_BLOCK) k_dyndt(const SceneDesc* D, const ConDesc<T>* C, Map m, unsigned long long* out) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const ConDesc<T>& CD = C[t];
+    const SceneDesc& S = D[CD.desc];
+    const NodeView n = S.n;
+    const PartView p = S.p;
+    const GatherView gv = S.gv;
+    const ClumpView cl = S.cl;
+    const ConView<T> cv = CD.cv;
+    const int64_t nR = *cv.counter;
+    const int i = (int)(lb * WDM_BLOCK + threadIdx.x);
+    double r = INFINITY;
+    if (i < n.nn) {
+        const uint32_t fl = n.flags[i];
+        const int sh = i < p.np ? ptShape(p.ptype[i]) : 0;
+        const bool own = sh == 1 || sh == 2;  // uninodal particle on this node
+        const bool isNode = !(fl & F_CLUMPED) && (own || i >= p.np);
+        if (isNode && (fl & F_DOF_ALL) != F_DOF_ALL) {
+            double kt[3] = {0., 0., 0.}, kr[3] = {0., 0., 0.};
+            if (own) stiffAdd<T, LAW>(cv, nR, gv, i, kt, kr);
+            if ((fl & F_CLUMP) && cl.nodeClump) {
+                const int c = cl.nodeClump[i];
+                for (int k = cl.off[c]; k < cl.off[c + 1]; k++) {
+                    const int q = cl.mem[k];
+                    if (q < p.np) {
+                        const int sq = ptShape(p.ptype[q]);
+                        if (sq == 1 || sq == 2) stiffAdd<T, LAW>(cv, nR, gv, q, kt, kr);
+                    }
+                }
+            }
+            const double mass = n.velm[i].w, inertia = n.angI[i].w;
+            double ret = INFINITY;
+            for (int a = 0; a < 3; a++)
+                if (kt[a] != 0 && mass > 0. && !(fl & (1u << a))) { const double x = mass / fabs(kt[a]); if (x < ret) ret = x; }
+            for (int a = 0; a < 3; a++)
+                if (kr[a] != 0 && inertia > 0. && !(fl & (8u << a))) { const double x = inertia / fabs(kr[a]); if (x < ret) ret = x; }
+            r = 2 * ret;
+        }
+    }
+    __shared__ double sm[WDM_BLOCK];
+    sm[threadIdx.x] = r;
+    __syncthreads();
+    for (int s = WDM_BLOCK / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) { const double o = sm[threadIdx.x + s]; if (o < sm[threadIdx.x]) sm[threadIdx.x] = o; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMin(out + t, (unsigned long long)__double_as_longlong(sm[0]));
+}
+__global__ void k_fill_u64(unsigned long long* a, int n, unsigned long long v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
 }
 
 // ---------------- scene generation: Jacobi overlap relaxation ----------------

@@ -229,7 +229,7 @@ struct Shared {
     size_t hostWordsCap = 0;
     std::vector<int32_t> freeSlots;
     int32_t nslots = 0;
-    Staged stDesc, stMapNode, stMapCon, stCon, stFR, stConFR, stIdx;
+    Staged stDesc, stMapNode, stMapCon, stCon, stFR, stConFR, stIdx, stDyn;
     Staged stMapP, stMapB, stMapC, stMapO, stMapN, stMapE, stMapP1, stMapQ;
     hipEvent_t evWords = nullptr;  // recorded after the copy of the words to the host
     // energy partials per block
@@ -248,6 +248,7 @@ struct Shared {
     DBuf<float4> cbox;  // FP32 boxes of the bound entries in cell order
     DBuf<uint8_t> temp;
     DBuf<double4> disp;
+    DBuf<unsigned long long> ddOut;  // DynDt: per scene, bits of the minimum squared critical step
     // window of steps: its tables (device and pinned staging) and the last captured graph
     DBuf<uint8_t> winDev;
     uint8_t* winHost = nullptr;
@@ -267,7 +268,7 @@ struct Shared {
         if (hostWords) (void)hipHostFree(hostWords);
         hostWords = nullptr;
         hostWordsCap = 0;
-        for (Staged* s : {&stDesc, &stMapNode, &stMapCon, &stCon, &stFR, &stConFR, &stIdx, &stMapP, &stMapB, &stMapC, &stMapO, &stMapN,
+        for (Staged* s : {&stDesc, &stMapNode, &stMapCon, &stCon, &stFR, &stConFR, &stIdx, &stDyn, &stMapP, &stMapB, &stMapC, &stMapO, &stMapN,
                           &stMapE, &stMapP1, &stMapQ})
             s->free(c);
         release(c, epartL); release(c, epartC); release(c, extPart); release(c, extOut);
@@ -275,7 +276,7 @@ struct Shared {
         release(c, cnt32); release(c, ooFlag); release(c, oslot); release(c, counts); release(c, offs); release(c, ooFlag64);
         release(c, ooRank); release(c, rflag); release(c, rscan); release(c, starts); release(c, nrflag); release(c, nrscan); release(c, candK); release(c, candK2);
         release(c, entK); release(c, entK2); release(c, candP); release(c, candP2); release(c, entV); release(c, entV2);
-        release(c, temp); release(c, disp);
+        release(c, temp); release(c, disp); release(c, ddOut);
         release(c, lkeys); release(c, lkeys2); release(c, lvals); release(c, lvals2); release(c, lstart); release(c, lend);
         release(c, lcnt32); release(c, lcnt); release(c, loffs); release(c, cbox);
         if (evWords) (void)hipEventDestroy(evWords);
@@ -311,6 +312,10 @@ struct Scene {
     int homoDeform = 4;
     // counters
     int64_t step = 0, nFullRuns = 0;
+    double time = 0, nextDt = NAN;  // Scene.time, Scene.nextDt (core/Scene.hpp)
+    wdm_dyndt dd{};                 // DynDt after ContactLoop (enabled 0: not in the engine list)
+    bool contactsRun = false;       // a ContactLoop ran on the current contact list
+    std::vector<int32_t> facetNodesH, nodeMasterH;  // nodes of facets; clump master of each node (-1)
     double verletDist = 0;
     bool verletResolved = false;
     bool needFull = true;
@@ -1183,6 +1188,7 @@ struct ContactLoop {
     static void done(Batch& B) {
         for (Scene* S : B.ss) {
             S->forcesFromContacts = true;  // applied by the next Leapfrog (or materialised on download)
+            S->contactsRun = true;
             S->hSizeCL = S->hSize;
         }
     }
@@ -1203,10 +1209,135 @@ void stepBeginCell(Scene& S) {
     checkReady(S);
     if (S.periodic) cellIntegrateAndUpdate(S, S.P.dt);
 }
+// Scene::doOneStep epilogue (core/Scene.cpp:395-400)
 void stepEnd(Scene& S) {
     if (S.periodic) cellSetNextGradV(S);
     S.step++;
+    S.time += S.P.dt;
+    if (!std::isnan(S.nextDt)) { S.P.dt = S.nextDt; S.nextDt = NAN; }
 }
+// ---------------- DynDt (pkg/dem/DynDt.cpp:16-97) ----------------
+// PeriodicEngine::isActivated, core/Engine.cpp:169-195, without the wall-clock criterion
+bool ddActivate(wdm_dyndt& d, int64_t stepNow, double virtNow) {
+    const bool initNow = d.stepLast < 0;
+    if (initNow) { d.virtLast = virtNow; d.stepLast = stepNow; }
+    if ((d.nDo < 0 || d.nDone < d.nDo) &&
+        ((d.virtPeriod > 0 && virtNow - d.virtLast >= d.virtPeriod) ||
+         (d.stepPeriod > 0 && ((stepNow - d.stepLast >= d.stepPeriod) || (d.stepModulo && stepNow % d.stepPeriod == 0))))) {
+        d.virtPrev = d.virtLast; d.virtLast = virtNow;
+        d.stepPrev = d.stepLast; d.stepLast = stepNow;
+        if (initNow && !d.initRun) return false;
+        d.nDone++;
+        return true;
+    }
+    return false;
+}
+struct DynDtEngine {
+    // facet nodes get their stiffness from the facets' contacts, which are not summed here:
+    // they must be fully blocked, or clumped to a fully blocked master
+    static void checkSupported(const Scene& S) {
+        for (int32_t q : S.facetNodesH) {
+            uint32_t f = S.flagsH[q];
+            if ((f & WDM_CLUMP_CLUMPED) && q < (int32_t)S.nodeMasterH.size() && S.nodeMasterH[q] >= 0) f = S.flagsH[S.nodeMasterH[q]];
+            else if (f & WDM_CLUMP_CLUMPED) continue;  // clumps not uploaded yet: checked again later
+            if ((f & 63u) != 63u) throw WdmError(WDM_E_UNSUPPORTED, "DynDt: facet nodes with free DOFs are outside the implemented path");
+        }
+    }
+    template <typename T, int LAW>
+    static void launchGroup(Batch& B, const std::vector<int>& L, unsigned long long* out) {
+        Ctx* c = B.c;
+        const int n = (int)L.size();
+        if (n == 0) return;
+        std::vector<ConDesc<T>> C(n);
+        std::memset(C.data(), 0, C.size() * sizeof(ConDesc<T>));
+        for (int t = 0; t < n; t++) {
+            Scene& S = *B.ss[L[t]];
+            ConBuf<T>* cb = InsertionSortCollider::bufs<T>(S);
+            C[t].cv = cb[S.cur].view();
+            C[t].nc = S.nc;
+            C[t].realMask = gpc(S.realMask.p);
+            C[t].desc = L[t];
+        }
+        std::vector<int64_t> bp = blockPrefix(n, [&](int t) { return B.ss[L[t]]->nn; });
+        Blob b;
+        const size_t oC = b.addv(C), oB = b.addv(bp);
+        const uint8_t* base = c->sh->stDyn.upload(c, b.b.data(), b.b.size());
+        Map m;
+        m.bprefix = (const int64_t*)(base + oB);
+        m.n = n;
+        if (bp[n] > 0) {
+            KTimer tm(c, "dyndt");
+            hipLaunchKernelGGL((k_dyndt<T, LAW>), dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, c->stream, B.D, (const ConDesc<T>*)(base + oC), m, out);
+            checkLaunch();
+        }
+    }
+    // DynDt::critDt_stiffness (DynDt.cpp:61-70) of each scene, from the contacts of its last
+    // ContactLoop (+inf before the first one); one launch per precision and law, one synchronisation
+    static std::vector<double> crit(Ctx* c, const std::vector<Scene*>& ss) {
+        Shared& X = *c->sh;
+        const int n = (int)ss.size();
+        std::vector<double> out(n, INFINITY);
+        std::vector<int> g[2][2];
+        for (int i = 0; i < n; i++) {
+            checkSupported(*ss[i]);
+            if (!ss[i]->needFull && ss[i]->contactsRun) g[ss[i]->mixed() ? 1 : 0][ss[i]->P.law == WDM_LAW_HERTZ_DMT ? 1 : 0].push_back(i);
+        }
+        if (g[0][0].empty() && g[0][1].empty() && g[1][0].empty() && g[1][1].empty()) return out;
+        Batch B(c, ss);
+        B.upload();
+        ensure(c, X.ddOut, (size_t)n);
+        hipLaunchKernelGGL(k_fill_u64, dim3((n + 63) / 64), dim3(64), 0, c->stream, X.ddOut.p, n, 0x7ff0000000000000ULL);
+        checkLaunch();
+        // the groups' scenes take consecutive output slots, group after group
+        std::vector<int> slotOf(n, -1);
+        int next = 0;
+        for (int mx = 0; mx < 2; mx++)
+            for (int lw = 0; lw < 2; lw++) {
+                const std::vector<int>& L = g[mx][lw];
+                if (L.empty()) continue;
+                unsigned long long* o = X.ddOut.p + next;
+                for (int i : L) slotOf[i] = next++;
+                if (mx) { if (lw) launchGroup<float, 1>(B, L, o); else launchGroup<float, 0>(B, L, o); }
+                else { if (lw) launchGroup<double, 1>(B, L, o); else launchGroup<double, 0>(B, L, o); }
+            }
+        std::vector<unsigned long long> h(n);
+        HIPCHK(hipMemcpyAsync(h.data(), X.ddOut.p, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < n; i++) {
+            if (slotOf[i] < 0) continue;
+            double v;
+            std::memcpy(&v, &h[slotOf[i]], sizeof(double));
+            out[i] = std::sqrt(v);
+        }
+        return out;
+    }
+    // DynDt::run (DynDt.cpp:72-97) for scenes whose engine fired at step dd.stepLast
+    static void run(Ctx* c, const std::vector<Scene*>& ss) {
+        if (ss.empty()) return;
+        const std::vector<double> cr = crit(c, ss);
+        for (size_t k = 0; k < ss.size(); k++) {
+            Scene& S = *ss[k];
+            wdm_dyndt& d = S.dd;
+            const double crDt = cr[k];
+            if (std::isinf(crDt)) continue;  // no timestep computed: keep the current value
+            const int64_t nSteps = d.stepLast - d.stepPrev;
+            const double maxNewDt = S.P.dt * std::pow(1. + d.maxRelInc, (double)nSteps);
+            const double cand = crDt * d.dtSafety;
+            const double nextDt = maxNewDt < cand ? maxNewDt : cand;  // std::min(cand, maxNewDt)
+            if (!d.dryRun) {
+                if (nextDt != S.P.dt) { d.dt = NAN; S.nextDt = nextDt; }
+            } else d.dt = nextDt;
+        }
+    }
+    // the engine loop's turn of DynDt (after ContactLoop) at the scenes' current step
+    static void fire(Ctx* c, const std::vector<Scene*>& ss) {
+        std::vector<Scene*> L;
+        for (Scene* S : ss)
+            if (S->dd.enabled && ddActivate(S->dd, S->step, S->time)) L.push_back(S);
+        run(c, L);
+    }
+};
+
 // One timestep of every scene of the batch, engines in Woo's order, each launched once for
 // all scenes.  The ContactLoop of the scenes that can skip the collider is queued right
 // behind Leapfrog, before the host has read the Verlet triggers; scenes whose trigger
@@ -1232,6 +1363,7 @@ void doOneStep(Batch& B) {
     InsertionSortCollider::runList(B, full);
     ContactLoop::runList(B, full, false);
     ContactLoop::done(B);
+    DynDtEngine::fire(c, B.ss);
     for (Scene* S : B.ss) stepEnd(*S);
 }
 
@@ -1295,12 +1427,17 @@ std::vector<int64_t> runWindow(Ctx* c, const std::vector<Scene*>& ss, const std:
     std::vector<CellArgs> cells;
     std::vector<size_t> cellOf((size_t)K * n, 0), cell0(n, 0);
     std::vector<std::vector<CellSnap>> snaps(n);
+    // Scene.time and the DynDt state before each step's engine loop (to rewind a scene that
+    // stops); a scene whose DynDt fires at step j ends its window there (cutAt)
+    std::vector<std::vector<double>> tSnap(n);
+    std::vector<std::vector<wdm_dyndt>> ddSnap(n);
+    std::vector<int> cutAt(n, -1);
     for (int j = 0; j < K; j++) {
         for (int i = 0; i < n; i++) {
             Scene& S = *ss[i];
             StepVar& v = sv[(size_t)j * n + i];
             v.gate = gate[i];
-            if (j >= left[i]) { v.idle = 1; continue; }
+            if (j >= left[i] || (cutAt[i] >= 0 && j > cutAt[i])) { v.idle = 1; continue; }
             if (S.periodic) snaps[i].push_back(snapCell(S));
             stepBeginCell(S);
             if (S.periodic || j == 0) {
@@ -1314,6 +1451,11 @@ std::vector<int64_t> runWindow(Ctx* c, const std::vector<Scene*>& ss, const std:
             v.stamp = stampOf(S.step);
             v.reallyTrackEnergy = S.P.trackEnergy && (!S.periodic || S.step > 0 || hequal(S.gradV, heye()));
             S.hSizeCL = S.hSize;  // ContactLoop::done
+            tSnap[i].push_back(S.time);
+            if (S.dd.enabled) {
+                ddSnap[i].push_back(S.dd);
+                if (ddActivate(S.dd, S.step, S.time)) cutAt[i] = j;
+            }
             stepEnd(S);
         }
     }
@@ -1379,20 +1521,26 @@ std::vector<int64_t> runWindow(Ctx* c, const std::vector<Scene*>& ss, const std:
     readWords(c, ss);  // raises device errors
     // where each scene stopped
     std::vector<int64_t> done(n);
-    std::vector<Scene*> stopped;
+    std::vector<Scene*> stopped, fired;
     for (int i = 0; i < n; i++) {
         Scene& S = *ss[i];
         const uint32_t w = X.hostWords[4 * (size_t)S.slot];
-        const int64_t active = std::min<int64_t>(K, left[i]);
+        int64_t active = std::min<int64_t>(K, left[i]);
+        if (cutAt[i] >= 0) active = std::min<int64_t>(active, cutAt[i] + 1);
         if (w >= gate[i]) {
             const int64_t jf = (int64_t)(w - gate[i]);
             if (jf >= active) throw WdmError(WDM_E_STATE, "Verlet trigger outside the window");
             S.step = s0[i] + jf;
+            S.time = tSnap[i][jf];
+            if (S.dd.enabled) S.dd = ddSnap[i][jf];
             if (S.periodic) restoreCell(S, snaps[i][jf]);
             stepBeginCell(S);  // mid-step: Leapfrog done, collider and ContactLoop to run
             stopped.push_back(&S);
             done[i] = jf + 1;
-        } else done[i] = active;
+        } else {
+            done[i] = active;
+            if (cutAt[i] >= 0) fired.push_back(&S);  // DynDt fired at the scene's last step (its epilogue ran ahead)
+        }
     }
     if (!stopped.empty()) {
         Batch F(c, stopped);
@@ -1401,8 +1549,15 @@ std::vector<int64_t> runWindow(Ctx* c, const std::vector<Scene*>& ss, const std:
         InsertionSortCollider::runList(F, L);
         ContactLoop::runList(F, L, false);
         ContactLoop::done(F);
-        for (Scene* S : stopped) stepEnd(*S);
     }
+    // DynDt of every scene whose engine fires at its last step of the window, in one pass
+    const size_t nAhead = fired.size();
+    for (Scene* S : stopped)
+        if (S->dd.enabled && ddActivate(S->dd, S->step, S->time)) fired.push_back(S);
+    DynDtEngine::run(c, fired);
+    for (size_t k = 0; k < nAhead; k++)
+        if (!std::isnan(fired[k]->nextDt)) { fired[k]->P.dt = fired[k]->nextDt; fired[k]->nextDt = NAN; }
+    for (Scene* S : stopped) stepEnd(*S);
     return done;
 }
 
@@ -1541,6 +1696,7 @@ void relax(Scene& S, int iters) {
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     S.needFull = true;
+        S.contactsRun = false;
 }
 
 // scene words slot (Verlet trigger, created energy slots, device errors)
@@ -1693,6 +1849,7 @@ int wdm_upload_nodes(wdm_ctx* c, int32_t sid, int32_t n, const double* pos3, con
         HIPCHK(hipStreamSynchronize(st));
         S.nodesSet = true;
         S.needFull = true;
+        S.contactsRun = false;
         S.forcesFromContacts = false;
         S.clumpFlagged = nclf;
         S.flagsH.assign(flags, flags + n);
@@ -1733,6 +1890,10 @@ int wdm_upload_particles(wdm_ctx* c, int32_t sid, int32_t n, const int32_t* shap
         }
         S.ptypeH = pt;
         S.radiusH.assign(radius, radius + n);
+        S.facetNodesH.clear();
+        for (int p = 0; p < n; p++)
+            if (shape[p] == WDM_SHAPE_FACET)
+                for (int k = 0; k < 3; k++) S.facetNodesH.push_back(nodes3[3 * p + k]);
         S.hasFacets = false;
         S.facetMasks.clear();
         for (int p = 0; p < n; p++)
@@ -1761,6 +1922,7 @@ int wdm_upload_particles(wdm_ctx* c, int32_t sid, int32_t n, const int32_t* shap
         HIPCHK(hipStreamSynchronize(st));
         S.partsSet = true;
         S.needFull = true;
+        S.contactsRun = false;
         S.verletResolved = false;
         S.nc = 0;
     });
@@ -1827,10 +1989,12 @@ int wdm_upload_clumps(wdm_ctx* c, int32_t sid, int32_t nc, const int32_t* master
         }
         S.nClumps = nc;
         S.nMembers = nm;
+        S.nodeMasterH = nodeMaster;
         S.clumpFree = false;
         for (int k = 0; k < nc; k++) S.clumpFree |= (S.flagsH[master[k]] & 63u) != 63u;
         S.clumpsSet = true;
         S.needFull = true;
+        S.contactsRun = false;
     });
 }
 
@@ -1853,6 +2017,7 @@ int wdm_set_cell(wdm_ctx* c, int32_t sid, int32_t periodic, const double* hSize9
         for (double& x : S.spinVec) x = x * .5;
         if (S.periodic) cellIntegrateAndUpdate(S, 0);  // Cell::postLoad
         S.needFull = true;
+        S.contactsRun = false;
     });
 }
 
@@ -1901,6 +2066,45 @@ int wdm_step_end(wdm_ctx* c, int32_t sid) {
         Scene& S = getScene(c, sid);
         stepEnd(S);
         readWords(c, {&S});
+    });
+}
+int wdm_dyndt_run(wdm_ctx* c, int32_t sid) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        DynDtEngine::fire(c, {&S});
+    });
+}
+int wdm_set_dyndt(wdm_ctx* c, int32_t sid, const wdm_dyndt* d) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        if (!d) throw WdmError(WDM_E_ARG, "null DynDt attributes");
+        if (d->enabled) {
+            if (1. + d->maxRelInc == 1.) throw WdmError(WDM_E_ARG, "DynDt: maxRelInc too small (1.0+maxRelInc==1.0)");  // DynDt::postLoad
+            if (!(d->dtSafety > 0)) throw WdmError(WDM_E_ARG, "DynDt: dtSafety must be positive");
+            DynDtEngine::checkSupported(S);
+        }
+        S.dd = *d;
+    });
+}
+int wdm_get_dyndt(wdm_ctx* c, int32_t sid, wdm_dyndt* d) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        if (!d) throw WdmError(WDM_E_ARG, "null DynDt output");
+        *d = S.dd;
+    });
+}
+int wdm_dyndt_crit(wdm_ctx* c, int32_t sid, double* crit) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        if (!crit) throw WdmError(WDM_E_ARG, "null output");
+        *crit = DynDtEngine::crit(c, {&S})[0];
+    });
+}
+int wdm_get_time(wdm_ctx* c, int32_t sid, double* time, double* dt) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        if (time) *time = S.time;
+        if (dt) *dt = S.P.dt;
     });
 }
 

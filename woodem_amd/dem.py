@@ -427,8 +427,29 @@ class ContactLoop(Engine):
 
 
 class DynDt(Engine):
-    """pkg/dem/DynDt.hpp; adaptive dt is not implemented on this path (configs use fixed dt)."""
-    _defaults = dict(stepPeriod=100)
+    """pkg/dem/DynDt.hpp:20-26 with the PeriodicEngine attributes and state it runs by
+    (core/Engine.hpp:141-153), placed after ContactLoop.  The state attributes (nDone,
+    stepLast, stepPrev, virtLast, virtPrev) and `dt` (the dryRun result) are updated by
+    Scene.run.  realPeriod (wall-clock time) is not supported on this path."""
+    _defaults = dict(maxRelInc=1e-4, dryRun=False, dt=float("nan"), stepPeriod=1, virtPeriod=0., realPeriod=0.,
+                     stepModulo=True, nDo=-1, nDone=0, initRun=True, virtLast=float("nan"), virtPrev=-1., stepLast=-1,
+                     stepPrev=-1)
+    _state = ("nDone", "stepLast", "stepPrev", "virtLast", "virtPrev", "dt")
+
+    def _native(self, dtSafety):
+        from ._native import make_dyndt
+        if self.realPeriod > 0:
+            raise NotImplementedError("DynDt.realPeriod (wall-clock periodicity) is not supported on this path")
+        if 1. + self.maxRelInc == 1.:
+            raise ValueError("DynDt: maxRelInc too small (1.0+maxRelInc==1.0)")  # DynDt::postLoad, DynDt.cpp:12-14
+        return make_dyndt(maxRelInc=self.maxRelInc, dtSafety=dtSafety, virtPeriod=self.virtPeriod, virtLast=self.virtLast,
+                          virtPrev=self.virtPrev, dt=self.dt, stepPeriod=self.stepPeriod, nDo=self.nDo, nDone=self.nDone,
+                          stepLast=self.stepLast, stepPrev=self.stepPrev, enabled=0 if self.dead else 1,
+                          stepModulo=int(bool(self.stepModulo)), initRun=int(bool(self.initRun)), dryRun=int(bool(self.dryRun)))
+
+    def _update(self, d):
+        for k in self._state:
+            setattr(self, k, getattr(d, k))
 
 
 # convenience: module-level constructor of a bare field with particles

@@ -25,7 +25,7 @@ def defaultMaterial():
 def defaultEngines(damping=0., gravity=None, verletDist=-.05, kinSplit=False, dontCollect=False, noSlip=False, noBreak=False,
                    cp2=None, law=None, model=None, grid=False, dynDtPeriod=100, cpKw={}, lawKw={}, precision="fp64"):
     """Canonical timestep engines, py/utils.py:55-83: Leapfrog(reset=True), InsertionSortCollider,
-    ContactLoop, and DynDt when dynDtPeriod > 0 (DynDt is not implemented here: pass 0)."""
+    ContactLoop, and DynDt(stepPeriod=dynDtPeriod) when dynDtPeriod > 0."""
     if gravity:
         raise ValueError("gravity MUST NOT be specified anymore, set DemField.gravity=... instead.")
     if model or grid:
