@@ -361,6 +361,13 @@ def main():
     alg_bytes = BYTES_REAL[args.precision] * n_real + BYTES_POT[args.precision] * max(n_active - n_real, 0)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
     kernels = {k: {"avg_ms": v[0] / max(v[1], 1), "launches": v[1]} for k, v in kt.items()}
+    # DynDt (utils.defaultEngines adds it with stepPeriod=100): its kernel on the same contacts
+    ctx.enable_timing(True)
+    for _ in range(10):
+        crit = ctx.dyndt_crit(run.sid)
+    kd = ctx.kernel_times().get("dyndt", (float("nan"), 0))
+    ctx.enable_timing(False)
+    kernels["dyndt"] = {"avg_ms": kd[0] / max(kd[1], 1), "launches": kd[1], "crit_dt": crit}
     log("kernels: %s" % json.dumps(kernels))
     log("contacts: %d potential, %d real, %d processed by the ContactLoop" % (n_con, n_real, n_active))
 

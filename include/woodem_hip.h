@@ -210,6 +210,24 @@ int wdm_contact_work(wdm_ctx* ctx, int32_t scene, int64_t* active, int64_t* rows
    any output pointer may be NULL */
 int wdm_get_contacts(wdm_ctx* ctx, int32_t scene, int64_t cap, int32_t* idA, int32_t* idB, int32_t* cellDist3,
                      uint8_t* isReal, double* forceLocal3, double* contPt3, double* trsf9, double* uN, int64_t* n_out);
+/* phys (FrictPhys: kn, kt, tanPhi, 0, 0; HertzPhys: K, R, tanPhi, kt0, alpha_sqrtMK) and
+   L6Geom.lens of the contacts in wdm_get_contacts order (NaN for non-real contacts) */
+int wdm_get_contact_state(wdm_ctx* ctx, int32_t scene, int64_t cap, double* phys5, double* lens2, int64_t* n_out);
+
+/* Checkpoint restore (no counterpart on the reference's hot path: Woo saves whole scenes
+   with boost::serialization, lib/object/ObjectIO.hpp:49-90).  wdm_upload_contacts makes
+   `n` real contacts (ids in functor order, any order of contacts) with their L6Geom and
+   phys state the scene's contact list; the next step runs a collider full run, which keeps
+   them and adds the potential contacts, so stepping on continues bit for bit as if the
+   scene had not been saved (node forces/torques of the checkpoint go to wdm_upload_nodes).
+   wdm_set_energy restores EnergyTracker values and created slots, wdm_set_time Scene.step
+   and Scene.time. */
+int wdm_upload_contacts(wdm_ctx* ctx, int32_t scene, int64_t n, const int32_t* idA, const int32_t* idB,
+                        const int32_t* cellDist3, const double* trsf9, const double* forceLocal3,
+                        const double* phys5, const double* lens2, const double* contPt3);
+int wdm_set_energy(wdm_ctx* ctx, int32_t scene, const double* values, const int32_t* created);
+int wdm_set_time(wdm_ctx* ctx, int32_t scene, int64_t step, double time);
+
 /* EnergyTracker values and whether each slot exists */
 int wdm_get_energy(wdm_ctx* ctx, int32_t scene, double* values, int32_t* created);
 /* Scene.step, InsertionSortCollider.nFullRuns, resolved verletDist */

@@ -1,0 +1,154 @@
+"""Checkpoints (npz) and VTK output (woodem_amd.snapshot).
+
+CPU: the VTK writer on the oracle's state (through a small adapter with the Context calls it
+makes): every array read back from the XML equals the state it was written from, names as
+VtkExport's (pkg/dem/VtkExport.cpp:132-260), periodic points wrapped into the cell.
+GPU: a checkpoint restored into a new scene steps on bit for bit like the saved scene
+(positions, velocities, orientations, forces, real contacts with their local forces and
+trsf, step, time, dt, DynDt counters; energies up to the summation order of their block
+sums), for a box with walls and energy tracking, the periodic Hertz packing, a rotating
+drum with facets and clumps, and mixed precision.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+from woodem_amd import runner, scenes, snapshot, soa
+
+
+class OracleCtx:
+    """the Context calls vtk_export makes, answered by an OracleRun"""
+
+    def __init__(self, o):
+        self.o = o
+
+    def stats(self, sid):
+        return self.o.stats()
+
+    def cell(self, sid):
+        return self.o.cell()
+
+    def download_nodes(self, sid, n):
+        return self.o.nodes()
+
+    def contacts(self, sid):
+        return self.o.contacts()
+
+
+class OracleRunView:
+    def __init__(self, a, cell, o):
+        self.ctx, self.sid, self.a, self.cell0 = OracleCtx(o), 0, a.normalized(), cell
+
+
+def test_vtk_export_box(tmp_path):
+    a = scenes.box_lattice(nx=5, ny=4, nz=4)
+    p, cell = runner.params_from_meta(a), runner.cell_from_meta(a)
+    o = OL.OracleRun(a, p, cell)
+    o.step(2600)
+    run = OracleRunView(a, cell, o)
+    files = snapshot.vtk_export(run, str(tmp_path / "out/box."))
+    assert set(files) == {"spheres", "con"}  # no facets: no mesh file
+    assert files["spheres"].endswith("box.spheres.2600.vtu") and files["con"].endswith("box.con.2600.vtp")
+    s = snapshot.read_vtk_arrays(files["spheres"])
+    nd = o.nodes()
+    sph = np.nonzero(a.shape == soa.SHAPE_SPHERE)[0]
+    assert np.array_equal(s["Points"], nd["pos"][sph])
+    assert np.array_equal(s["radius"], a.radius[sph]) and np.array_equal(s["mass"], a.mass[sph])
+    assert np.array_equal(s["id"], sph) and np.array_equal(s["vel"], nd["vel"][sph])
+    assert np.array_equal(s["angVel"], nd["angVel"][sph]) and np.array_equal(s["matId"], a.mat[sph])
+    assert np.array_equal(s["types"], np.ones(len(sph)))  # VTK_VERTEX
+    c = o.contacts()
+    r = c["isReal"]
+    assert r.sum() > 10
+    k = snapshot.read_vtk_arrays(files["con"])
+    assert np.array_equal(k["Fn"], c["forceLocal"][r, 0])
+    assert np.allclose(k["|Ft|"], np.hypot(c["forceLocal"][r, 1], c["forceLocal"][r, 2]), rtol=1e-15, atol=0)
+    pts = k["Points"].reshape(-1, 2, 3)
+    # sphere ends at the centres, wall ends at the contact point (walls come first in a contact)
+    wall = a.shape[c["idA"][r]] == soa.SHAPE_WALL
+    assert wall.sum() > 0 and (~wall).sum() > 0
+    assert np.array_equal(pts[~wall, 0], nd["pos"][c["idA"][r][~wall]])
+    assert np.array_equal(pts[wall, 0], c["contPt"][r][wall])
+    assert np.array_equal(pts[:, 1], nd["pos"][c["idB"][r]])
+
+
+def test_vtk_export_periodic_wraps_points(tmp_path):
+    a = scenes.periodic_hertz(n=400, seed=3)
+    L = a.meta["L"]
+    a.pos[:7] += np.array([L, -L, 2 * L])  # outside the cell: exported wrapped into [0, L)
+    p, cell = runner.params_from_meta(a), runner.cell_from_meta(a)
+    o = OL.OracleRun(a, p, cell)
+    run = OracleRunView(a, cell, o)
+    files = snapshot.vtk_export(run, str(tmp_path / "p."), what=("spheres",))
+    s = snapshot.read_vtk_arrays(files["spheres"])
+    assert np.all(s["Points"] >= 0) and np.all(s["Points"] < L)
+    h = o.cell()[0]
+    want = o.nodes()["pos"] - np.floor(o.nodes()["pos"] / np.diag(h)) * np.diag(h)
+    assert np.array_equal(s["Points"], want)
+
+
+def test_checkpoint_rejects_other_files(tmp_path):
+    f = tmp_path / "x.npz"
+    np.savez(f, meta=np.array('{"schema": "something else"}'))
+    with pytest.raises(ValueError):
+        snapshot.read(str(f))
+
+
+# ---------------- GPU: continuation after a restore ----------------
+def _compare(r1, r2):
+    n1, n2 = r1.nodes(), r2.nodes()
+    for q in ("pos", "vel", "angVel", "ori", "force", "torque"):
+        assert np.array_equal(n1[q], n2[q]), q
+    c1, c2 = r1.contacts(), r2.contacts()
+    k1 = OL.pair_keys({k: v[c1["isReal"]] for k, v in c1.items()})
+    k2 = OL.pair_keys({k: v[c2["isReal"]] for k, v in c2.items()})
+    assert np.array_equal(k1, k2)
+    # both lists are sorted by (min id, max id)
+    assert np.array_equal(c1["forceLocal"][c1["isReal"]], c2["forceLocal"][c2["isReal"]])
+    assert np.array_equal(c1["trsf"][c1["isReal"]], c2["trsf"][c2["isReal"]])
+    # energies are block sums in work-list order, which depends on the step of the last collider
+    # full run (the restore adds one): equal up to summation order
+    e1, e2 = r1.energy(), r2.energy()
+    assert np.allclose(e1[0], e2[0], rtol=1e-12, atol=0) and np.array_equal(e1[1], e2[1])
+    assert r1.stats()["step"] == r2.stats()["step"]
+    assert r1.ctx.time(r1.sid) == r2.ctx.time(r2.sid)
+
+
+def _scene(kind):
+    from woodem_amd._native import PREC_MIXED, Context
+    ctx = Context(0)
+    if kind == "box":
+        a = scenes.box_lattice(nx=8, ny=6, nz=5)
+        p = runner.params_from_meta(a, trackEnergy=True)
+    elif kind == "periodic":
+        a = scenes.relax_and_order(scenes.periodic_hertz(n=4000, seed=4), ctx, iters=100)
+        p = runner.params_from_meta(a, trackEnergy=True)
+    elif kind == "mixed":
+        a = scenes.relax_and_order(scenes.periodic_hertz(n=4000, seed=4), ctx, iters=100)
+        p = runner.params_from_meta(a, precision=PREC_MIXED)
+    else:
+        a = scenes.drum(n=3000, seed=7)
+        p = runner.params_from_meta(a)
+    return ctx, a, p, runner.cell_from_meta(a)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["box", "periodic", "mixed", "drum"])
+def test_checkpoint_continues_bit_for_bit(kind, tmp_path):
+    from woodem_amd._native import make_dyndt
+    ctx, a, p, cell = _scene(kind)
+    r1 = runner.HipRun(a, p, cell, ctx=ctx)
+    if kind == "box":
+        ctx.set_dyndt(r1.sid, make_dyndt(stepPeriod=50, maxRelInc=1e-3))
+    r1.step(2700 if kind == "box" else 300)
+    f = snapshot.save(r1, str(tmp_path / ("%s.npz" % kind)))
+    r2 = snapshot.load(f, ctx=ctx)
+    for n in (1, 37, 400):
+        r1.step(n)
+        r2.step(n)
+        _compare(r1, r2)
+    if kind == "box":
+        d1, d2 = ctx.dyndt(r1.sid), ctx.dyndt(r2.sid)
+        assert (d1.nDone, d1.stepLast, d1.stepPrev) == (d2.nDone, d2.stepLast, d2.stepPrev)
+    r1.close()
+    r2.close()

@@ -114,6 +114,10 @@ def lib():
         "wdm_dyndt_crit": (C.c_int, [P, I32, pd]),
         "wdm_dyndt_run": (C.c_int, [P, I32]),
         "wdm_get_time": (C.c_int, [P, I32, pd, pd]),
+        "wdm_get_contact_state": (C.c_int, [P, I32, I64, pd, pd, pl]),
+        "wdm_upload_contacts": (C.c_int, [P, I32, I64, pi, pi, pi, pd, pd, pd, pd, pd]),
+        "wdm_set_energy": (C.c_int, [P, I32, pd, pi]),
+        "wdm_set_time": (C.c_int, [P, I32, I64, D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -309,6 +313,35 @@ class Context:
         t, d = C.c_double(), C.c_double()
         self.check(self.L.wdm_get_time(self.h, sid, C.byref(t), C.byref(d)))
         return t.value, d.value
+
+    def contact_state(self, sid):
+        """phys (5 values) and L6Geom.lens (2) of the contacts, in contacts() order"""
+        n = C.c_int64()
+        self.check(self.L.wdm_num_contacts(self.h, sid, C.byref(n)))
+        ph = np.empty((n.value, 5))
+        ln = np.empty((n.value, 2))
+        k = C.c_int64()
+        self.check(self.L.wdm_get_contact_state(self.h, sid, n.value, ptr(ph, C.c_double), ptr(ln, C.c_double), C.byref(k)))
+        return ph[:k.value], ln[:k.value]
+
+    def upload_contacts(self, sid, idA, idB, cellDist, trsf, forceLocal, phys, lens, contPt):
+        """real contacts with their state (checkpoint restore)"""
+        i32 = lambda x: np.ascontiguousarray(x, np.int32)
+        f64 = lambda x: np.ascontiguousarray(x, np.float64)
+        idA, idB, cd = i32(idA), i32(idB), i32(cellDist).reshape(-1, 3)
+        tr, fl, ph, ln, cp = f64(trsf).reshape(-1, 9), f64(forceLocal), f64(phys), f64(lens), f64(contPt)
+        n = len(idA)
+        self.check(self.L.wdm_upload_contacts(self.h, sid, n, ptr(idA, C.c_int32), ptr(idB, C.c_int32), ptr(cd, C.c_int32),
+                                              ptr(tr, C.c_double), ptr(fl, C.c_double), ptr(ph, C.c_double),
+                                              ptr(ln, C.c_double), ptr(cp, C.c_double)))
+
+    def set_energy(self, sid, values, created):
+        v = np.ascontiguousarray(values, np.float64)
+        c = np.ascontiguousarray(np.asarray(created).astype(np.int32))
+        self.check(self.L.wdm_set_energy(self.h, sid, ptr(v, C.c_double), ptr(c, C.c_int32)))
+
+    def set_time(self, sid, step, time):
+        self.check(self.L.wdm_set_time(self.h, sid, int(step), float(time)))
 
     def relax(self, sid, iters):
         self.check(self.L.wdm_relax_overlaps(self.h, sid, int(iters)))

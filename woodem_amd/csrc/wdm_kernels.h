@@ -939,7 +939,7 @@ __device__ __forceinline__ bool overlapPeri(const double4& a0, const double4& a1
 }
 
 // functor order: Wall/Facet first, otherwise the lower id
-__device__ __forceinline__ void canonical(int i, int j, int si, int sj, int& A, int& B, int& type) {
+WDM_HD void canonical(int i, int j, int si, int sj, int& A, int& B, int& type) {
     if (si == 1 && sj == 1) { A = i < j ? i : j; B = i < j ? j : i; type = CI_SS; return; }
     if ((si == 2 || si == 3) && sj == 1) { A = i; B = j; type = si == 2 ? CI_WS : CI_FS; return; }
     if ((sj == 2 || sj == 3) && si == 1) { A = j; B = i; type = sj == 2 ? CI_WS : CI_FS; return; }

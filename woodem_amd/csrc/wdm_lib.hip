@@ -1598,7 +1598,8 @@ int guard(Ctx* c, F&& f) {
 
 template <typename T>
 void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_t* idB, int32_t* cellDist3, uint8_t* isReal,
-                      double* forceLocal3, double* contPt3, double* trsf9, double* uN, int64_t* nOut) {
+                      double* forceLocal3, double* contPt3, double* trsf9, double* uN, int64_t* nOut, double* phys5 = nullptr,
+                      double* lens2 = nullptr) {
     Ctx* c = S.ctx;
     ConBuf<T>& b = cb[S.cur];
     int64_t n = S.nc;
@@ -1634,6 +1635,8 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
         for (int d = 0; d < 3; d++) {
             if (forceLocal3) forceLocal3[3 * k + d] = real ? (double)stv(row, S_F + d) : 0.;
         }
+        if (phys5) for (int d = 0; d < 5; d++) phys5[5 * k + d] = real ? (double)stv(row, S_P + d) : NAN;
+        if (lens2) for (int d = 0; d < 2; d++) lens2[2 * k + d] = real ? (double)stv(row, S_L + d) : NAN;
         // contact point and uN exist for real contacts only (L6Geom); zeros otherwise
         const double4 rec = (real && itm[i] >= 0) ? gfc[2 * (size_t)itm[i] + 1] : make_double4(0., 0., 0., 0.);
         if (contPt3) { contPt3[3 * k] = rec.x; contPt3[3 * k + 1] = rec.y; contPt3[3 * k + 2] = rec.z; }
@@ -1655,6 +1658,76 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
     if (nOut) *nOut = k;
 }
 
+// real contacts of a checkpoint become the scene's contact list (state rows in key order);
+// the next step runs a collider full run, which keeps them and adds the potential contacts
+template <typename T>
+void uploadContacts(Scene& S, ConBuf<T>* cb, int64_t n, const int32_t* idA, const int32_t* idB, const int32_t* cellDist3,
+                    const double* trsf9, const double* forceLocal3, const double* phys5, const double* lens2, const double* contPt3) {
+    Ctx* c = S.ctx;
+    std::vector<uint64_t> key(n);
+    std::vector<int64_t> ord(n);
+    for (int64_t k = 0; k < n; k++) {
+        const int a = idA[k], b = idB[k];
+        if (a < 0 || b < 0 || a >= S.np || b >= S.np || a == b) throw WdmError(WDM_E_ARG, "contact ids out of range");
+        const int sa = S.ptypeH[a] & 15, sb = S.ptypeH[b] & 15;
+        int A, B, ty;
+        canonical(a, b, sa, sb, A, B, ty);
+        if (ty == (int)CI_NONE || A != a || B != b) throw WdmError(WDM_E_ARG, "contact " + std::to_string(k) + ": ids not in functor order or no functor");
+        for (int q = 0; q < 3; q++)
+            if (cellDist3[3 * k + q] < -511 || cellDist3[3 * k + q] > 511) throw WdmError(WDM_E_ARG, "cellDist out of range");
+        const int lo = std::min(a, b), hi = std::max(a, b);
+        key[k] = ((uint64_t)(uint32_t)lo << 32) | (uint32_t)hi;
+        ord[k] = k;
+    }
+    std::sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return key[x] < key[y]; });
+    for (int64_t k = 1; k < n; k++)
+        if (key[ord[k]] == key[ord[k - 1]]) throw WdmError(WDM_E_ARG, "duplicate contact");
+    ConBuf<T>& b = cb[S.cur];
+    b.reserve(c, std::max<int64_t>(n, 1));
+    const int64_t cap = b.cap;
+    std::vector<uint64_t> hk(n);
+    std::vector<int2> ids(n);
+    std::vector<uint32_t> info(n);
+    std::vector<int32_t> cd(n), slot(n);
+    std::vector<T> st((size_t)NSTATE * cap, (T)0);
+    std::vector<double4> cpt(std::max<int64_t>(n, 1));
+    for (int64_t r = 0; r < n; r++) {
+        const int64_t k = ord[r];
+        hk[r] = key[k];
+        ids[r] = make_int2(idA[k], idB[k]);
+        int A, B, ty;
+        canonical(idA[k], idB[k], S.ptypeH[idA[k]] & 15, S.ptypeH[idB[k]] & 15, A, B, ty);
+        info[r] = (uint32_t)ty | CI_REAL;
+        cd[r] = packCd(cellDist3[3 * k], cellDist3[3 * k + 1], cellDist3[3 * k + 2]);
+        slot[r] = (int32_t)r;
+        const double* t = trsf9 + 9 * k;
+        for (int q = 0; q < 3; q++) {
+            st[(size_t)(S_T + q) * cap + r] = (T)t[3 * q];          // column 0
+            st[(size_t)(S_T + 3 + q) * cap + r] = (T)t[3 * q + 1];  // column 1
+            st[(size_t)(S_F + q) * cap + r] = (T)forceLocal3[3 * k + q];
+        }
+        for (int q = 0; q < 5; q++) st[(size_t)(S_P + q) * cap + r] = (T)phys5[5 * k + q];
+        for (int q = 0; q < 2; q++) st[(size_t)(S_L + q) * cap + r] = (T)lens2[2 * k + q];
+        cpt[r] = make_double4(contPt3[3 * k], contPt3[3 * k + 1], contPt3[3 * k + 2], 0.);
+    }
+    hipStream_t st0 = c->stream;
+    const int32_t nR = (int32_t)n, zero = 0;
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(b.key.p, hk.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, st0));
+        HIPCHK(hipMemcpyAsync(b.ids.p, ids.data(), n * sizeof(int2), hipMemcpyHostToDevice, st0));
+        HIPCHK(hipMemcpyAsync(b.info.p, info.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st0));
+        HIPCHK(hipMemcpyAsync(b.cd.p, cd.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st0));
+        HIPCHK(hipMemcpyAsync(b.slot.p, slot.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st0));
+        HIPCHK(hipMemcpyAsync(b.st.p, st.data(), st.size() * sizeof(T), hipMemcpyHostToDevice, st0));
+        HIPCHK(hipMemcpyAsync(b.cpt.p, cpt.data(), n * sizeof(double4), hipMemcpyHostToDevice, st0));
+    }
+    HIPCHK(hipMemcpyAsync(b.counter.p, &nR, sizeof(int32_t), hipMemcpyHostToDevice, st0));
+    HIPCHK(hipMemcpyAsync(b.nItems.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, st0));
+    HIPCHK(hipStreamSynchronize(st0));
+    S.nc = n;
+    S.needFull = true;
+    S.contactsRun = false;
+}
 template <typename T>
 int64_t countLive(Scene& S, ConBuf<T>* cb) {
     ConBuf<T>& b = cb[S.cur];
@@ -2200,6 +2273,47 @@ int wdm_get_contacts(wdm_ctx* c, int32_t sid, int64_t cap, int32_t* idA, int32_t
     });
 }
 
+int wdm_get_contact_state(wdm_ctx* c, int32_t sid, int64_t cap, double* phys5, double* lens2, int64_t* nOut) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        if (S.mixed()) downloadContacts(S, S.cb32, cap, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nOut, phys5, lens2);
+        else downloadContacts(S, S.cb64, cap, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nOut, phys5, lens2);
+    });
+}
+int wdm_upload_contacts(wdm_ctx* c, int32_t sid, int64_t n, const int32_t* idA, const int32_t* idB, const int32_t* cellDist3,
+                        const double* trsf9, const double* forceLocal3, const double* phys5, const double* lens2, const double* contPt3) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        if (!S.partsSet || !S.paramsSet) throw WdmError(WDM_E_STATE, "upload particles and parameters before contacts");
+        if (n < 0 || (n > 0 && (!idA || !idB || !cellDist3 || !trsf9 || !forceLocal3 || !phys5 || !lens2 || !contPt3)))
+            throw WdmError(WDM_E_ARG, "contact arrays");
+        if (n >= INT32_MAX) throw WdmError(WDM_E_ARG, "too many contacts");
+        if (S.mixed()) uploadContacts(S, S.cb32, n, idA, idB, cellDist3, trsf9, forceLocal3, phys5, lens2, contPt3);
+        else uploadContacts(S, S.cb64, n, idA, idB, cellDist3, trsf9, forceLocal3, phys5, lens2, contPt3);
+    });
+}
+int wdm_set_energy(wdm_ctx* c, int32_t sid, const double* values, const int32_t* created) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        if (!values || !created) throw WdmError(WDM_E_ARG, "energy arrays");
+        uint32_t cre = 0;
+        for (int k = 0; k < WDM_E_COUNT; k++) cre |= created[k] ? (1u << k) : 0u;
+        HIPCHK(hipMemcpyAsync(S.E.p, values, WDM_E_COUNT * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->sh->words.p + 4 * (size_t)S.slot + 1, &cre, sizeof(cre), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+int wdm_set_time(wdm_ctx* c, int32_t sid, int64_t step, double time) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        if (step < 0) throw WdmError(WDM_E_ARG, "negative step");
+        S.step = step;
+        S.time = time;
+        const uint32_t z = 0;  // Verlet trigger: stamps of the old step count are meaningless now
+        HIPCHK(hipMemcpyAsync(c->sh->words.p + 4 * (size_t)S.slot, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
 int wdm_get_energy(wdm_ctx* c, int32_t sid, double* values, int32_t* created) {
     return guard(c, [&] {
         Scene& S = getScene(c, sid);

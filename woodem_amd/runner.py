@@ -46,6 +46,7 @@ class HipRun:
     def __init__(self, arrays, params, cell, ctx=None):
         self.ctx = ctx or Context(0)
         self.a = arrays.normalized()
+        self.params, self.cell0 = params, cell
         self.sid = self.ctx.scene_alloc(self.a.n_par, self.a.n_node, self.a.n_mat)
         self.ctx.upload(self.sid, self.a)
         self.ctx.set_cell(self.sid, cell["periodic"], cell["hSize"], cell["gradV"], cell["nextGradV"], cell["homoDeform"])
