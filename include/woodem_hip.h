@@ -228,6 +228,31 @@ int wdm_upload_contacts(wdm_ctx* ctx, int32_t scene, int64_t n, const int32_t* i
 int wdm_set_energy(wdm_ctx* ctx, int32_t scene, const double* values, const int32_t* created);
 int wdm_set_time(wdm_ctx* ctx, int32_t scene, int64_t step, double time);
 
+/* Particles removed and inserted in place.  The scene keeps its particle slots: a removed
+   particle (DemField::removeParticle, pkg/dem/Particle.cpp:445-493; spheres and walls, not
+   clumped) leaves a hole of shape NONE whose node is fully blocked, massless and at rest, and
+   its contacts are dropped; the forces of the last ContactLoop stay with its partners.
+   Inserted spheres take ids as ParticleContainer::insert does (ParticleContainer.cpp:23-72):
+   the holes first, in the order they were freed, then the next id past the last particle in
+   use, up to n_par (allocate spare particles of shape NONE for growth); their nodes start
+   with zero force and torque, as a new DemData.  Both make the next step run a collider full
+   run (the particle set changed, InsertionSortCollider.cpp:475-492). */
+int wdm_remove_particles(wdm_ctx* ctx, int32_t scene, int32_t n, const int32_t* ids);
+int wdm_insert_spheres(wdm_ctx* ctx, int32_t scene, int32_t n, const double* pos3, const double* vel3,
+                       const double* angVel3, const double* radius, const double* mass, const double* inertia,
+                       const int32_t* mat, const uint32_t* mask, int32_t* ids_out);
+/* holes in use order and the end of the particles in use (ParticleContainer.freeIds, size) */
+int wdm_get_free_ids(wdm_ctx* ctx, int32_t scene, int32_t cap, int32_t* ids, int32_t* n, int32_t* par_end);
+int wdm_set_free_ids(wdm_ctx* ctx, int32_t scene, int32_t n, const int32_t* ids, int32_t par_end);
+/* Particle.mask of particles (Outlet's marking mode, pkg/dem/Outlet.cpp:95-101) */
+int wdm_set_masks(wdm_ctx* ctx, int32_t scene, int32_t n, const int32_t* ids, const uint32_t* masks);
+/* The particles Outlet::run (pkg/dem/Outlet.cpp:12-40) would delete (markMask 0) or mark for a
+   BoxOutlet (Outlet.hpp:56: the closed box lo..hi; `inside` selects the particles in it
+   rather than out of it; `mask` as Outlet.mask), uninodal particles only, ascending ids;
+   n = how many (ids holds the first `cap`). */
+int wdm_select_box(wdm_ctx* ctx, int32_t scene, const double* lo3, const double* hi3, int32_t inside, uint32_t mask,
+                   uint32_t mark_mask, int32_t cap, int32_t* ids, int32_t* n);
+
 /* EnergyTracker values and whether each slot exists */
 int wdm_get_energy(wdm_ctx* ctx, int32_t scene, double* values, int32_t* created);
 /* Scene.step, InsertionSortCollider.nFullRuns, resolved verletDist */

@@ -25,6 +25,7 @@
 #include "dem_oracle.h"
 
 #include <algorithm>
+#include <deque>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -641,10 +642,15 @@ struct Oracle {
                 Bounds& b = BBj[i];
                 const int id = b.id;
                 const Aabb& a = aabb[id];
-                b.hasBB = a.has;
-                b.coord = (a.has ? (b.isMin ? a.mn[j] : a.mx[j]) : pos[nodeOf(id)][j]) - (periodic ? BBj.cellDim * b.period : 0.);
-                b.isInf = false;
-                if (periodic && std::isinf(b.coord)) { b.period = (b.coord < 0 ? 0 : 1); b.coord = 0.; b.isInf = true; }
+                if (shape[id] != ORC_SHAPE_NONE) {
+                    b.hasBB = a.has;
+                    b.coord = (a.has ? (b.isMin ? a.mn[j] : a.mx[j]) : pos[nodeOf(id)][j]) - (periodic ? BBj.cellDim * b.period : 0.);
+                    b.isInf = false;
+                    if (periodic && std::isinf(b.coord)) { b.period = (b.coord < 0 ? 0 : 1); b.coord = 0.; b.isInf = true; }
+                } else {  // vanished particle (:534-541): no bound; -inf (0 if periodic) at an initial sort, else kept
+                    b.hasBB = false;
+                    if (doInitSort) b.coord = periodic ? 0. : -INF;
+                }
                 if (doInitSort && periodic) { if (!b.isInf) { int per; b.coord = cellWrap(b.coord, 0, BBj.cellDim, per); b.period = per; } }
             }
         }
@@ -1051,6 +1057,57 @@ struct Oracle {
         }
     }
 
+    /* ---------- particles removed and inserted in place ---------- */
+    /* ParticleContainer ids (pkg/dem/ParticleContainer.cpp:23-72): holes reused first, in the
+       order they were freed (freeIds is a FIFO list), then the id past the last one in use */
+    int parEnd = 0;
+    std::deque<int> freeIds;
+    void initIds() {
+        parEnd = 0;
+        for (int p = 0; p < np; p++) if (shape[p] != ORC_SHAPE_NONE) parEnd = p + 1;
+        freeIds.clear();
+        for (int p = 0; p < parEnd; p++) if (shape[p] == ORC_SHAPE_NONE) freeIds.push_back(p);
+    }
+    /* DemField::removeParticle, pkg/dem/Particle.cpp:445-493: the particle's contacts are
+       removed (ContactContainer::remove); the slot stays as a hole whose node is fully
+       blocked, massless, at rest and skipped by the energy terms, so Leapfrog changes nothing
+       for it (the reference removes the node from DemField.nodes) */
+    void removeParticle(int id) {
+        for (auto it = con.begin(); it != con.end();) {
+            if (it->second.idA == id || it->second.idB == id) it = con.erase(it);
+            else ++it;
+        }
+        pending.erase(std::remove_if(pending.begin(), pending.end(), [&](uint64_t k) { return (int)(k >> 32) == id || (int)(k & 0xffffffffu) == id; }),
+                      pending.end());
+        shape[id] = ORC_SHAPE_NONE;
+        const int nd = nodeOf(id);
+        flags[nd] = DOF_ALL | ENERGY_SKIP | GRAVITY_SKIP | DAMPING_SKIP;
+        vel[nd] = vzero(); angVel[nd] = vzero(); mass[nd] = 0; inertia[nd] = vzero();
+        aabb[id] = Aabb();
+        freeIds.push_back(id);
+        adjDirty = true;
+        contactsDirty = true;  // the particle count changed: full run (InsertionSortCollider.cpp:475-492)
+    }
+    /* ParticleContainer::insert of a new sphere: its node starts with zero force and torque,
+       as a new DemData */
+    int insertSphere(const V3& p, const V3& v, const V3& w, double r, double m, double I, int mat, uint32_t msk) {
+        int id;
+        if (!freeIds.empty()) { id = freeIds.front(); freeIds.pop_front(); }
+        else id = parEnd++;
+        shape[id] = ORC_SHAPE_SPHERE;
+        nodeIx[3 * id] = id; nodeIx[3 * id + 1] = -1; nodeIx[3 * id + 2] = -1;
+        radius[id] = r; axis[id] = 0; sense[id] = 0; mask[id] = msk; matId[id] = mat;
+        pos[id] = p; vel[id] = v; angVel[id] = w;
+        ori[id] = Qt{1, 0, 0, 0};
+        mass[id] = m; inertia[id] = v3(I, I, I);
+        flags[id] = 0;
+        force[id] = vzero(); torque[id] = vzero();
+        aabb[id] = Aabb();
+        adjDirty = true;
+        contactsDirty = true;
+        return id;
+    }
+
     /* ---------- DynDt, pkg/dem/DynDt.cpp:16-97 ---------- */
     orc_dyndt dd{};
     double time = 0, nextDt = NAN;
@@ -1227,6 +1284,7 @@ int orc_set_particles(void* h, int32_t n, const int32_t* shape, const int32_t* n
     o->shape.assign(shape, shape + n); o->nodeIx.assign(nodes3, nodes3 + 3 * n); o->radius.assign(radius, radius + n);
     o->axis.assign(axis, axis + n); o->sense.assign(sense, sense + n); o->mask.assign(mask, mask + n); o->matId.assign(mat, mat + n);
     o->contactsDirty = true; o->con.clear(); o->adjDirty = true; o->aabb.assign(n, Aabb());
+    o->initIds();
     return 0;
 }
 int orc_set_materials(void* h, int32_t n, const double* density, const double* young, const double* ktDivKn, const double* tanPhi) {
@@ -1340,6 +1398,43 @@ int64_t orc_bruteforce_pairs(void* h, int64_t cap, int32_t* idMin, int32_t* idMa
             n++;
         }
     return n;
+}
+int orc_remove_particles(void* h, int32_t n, const int32_t* ids) {
+    Oracle* o = O(h);
+    std::vector<char> seen(o->np, 0);
+    for (int k = 0; k < n; k++) {
+        const int id = ids[k];
+        if (id < 0 || id >= o->np || o->shape[id] == ORC_SHAPE_NONE) { o->err = "particle does not exist"; return -1; }
+        if (o->shape[id] == ORC_SHAPE_FACET) { o->err = "removing facets is not restated"; return -1; }
+        if (o->flags[o->nodeOf(id)] & (CLUMP_CLUMPED | CLUMP_CLUMP)) { o->err = "a node is clumped, remove the clump itself instead!"; return -1; }
+        if (seen[id]++) { o->err = "particle listed twice"; return -1; }
+    }
+    for (int k = 0; k < n; k++) o->removeParticle(ids[k]);
+    return 0;
+}
+int orc_insert_spheres(void* h, int32_t n, const double* pos3, const double* vel3, const double* angVel3, const double* radius,
+                       const double* mass, const double* inertia, const int32_t* mat, const uint32_t* mask, int32_t* idsOut) {
+    Oracle* o = O(h);
+    if ((long)o->freeIds.size() + (o->np - o->parEnd) < n) { o->err = "no free particle slot"; return -1; }
+    for (int k = 0; k < n; k++) {
+        V3 p = v3(pos3[3 * k], pos3[3 * k + 1], pos3[3 * k + 2]);
+        V3 v = vel3 ? v3(vel3[3 * k], vel3[3 * k + 1], vel3[3 * k + 2]) : vzero();
+        V3 w = angVel3 ? v3(angVel3[3 * k], angVel3[3 * k + 1], angVel3[3 * k + 2]) : vzero();
+        idsOut[k] = o->insertSphere(p, v, w, radius[k], mass[k], inertia[k], mat[k], mask[k]);
+    }
+    return 0;
+}
+int orc_set_masks(void* h, int32_t n, const int32_t* ids, const uint32_t* masks) {
+    for (int k = 0; k < n; k++) O(h)->mask[ids[k]] = masks[k];
+    return 0;
+}
+int orc_get_free_ids(void* h, int32_t cap, int32_t* ids, int32_t* n, int32_t* parEnd) {
+    Oracle* o = O(h);
+    int k = 0;
+    for (int q : o->freeIds) { if (k < cap) ids[k] = q; k++; }
+    *n = k;
+    *parEnd = o->parEnd;
+    return 0;
 }
 int orc_set_dyndt(void* h, const orc_dyndt* d) {
     if (d->enabled && 1. + d->maxRelInc == 1.) { O(h)->err = "DynDt: maxRelInc too small (1.0+maxRelInc==1.0)"; return -1; }

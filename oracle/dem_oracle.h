@@ -80,6 +80,13 @@ int orc_set_dyndt(void* h, const orc_dyndt* d);
 int orc_get_dyndt(void* h, orc_dyndt* d);
 /* DynDt::critDt_compute (pkg/dem/DynDt.cpp:61-70) on the current contacts */
 int orc_dyndt_crit(void* h, double* critDt);
+/* DemField::removeParticle (pkg/dem/Particle.cpp:445-493) with the slot kept as a hole;
+   ParticleContainer::insert of spheres (ParticleContainer.cpp:23-72: freed ids first, FIFO) */
+int orc_remove_particles(void* h, int32_t n, const int32_t* ids);
+int orc_insert_spheres(void* h, int32_t n, const double* pos3, const double* vel3, const double* angVel3, const double* radius,
+                       const double* mass, const double* inertia, const int32_t* mat, const uint32_t* mask, int32_t* idsOut);
+int orc_set_masks(void* h, int32_t n, const int32_t* ids, const uint32_t* masks);
+int orc_get_free_ids(void* h, int32_t cap, int32_t* ids, int32_t* n, int32_t* parEnd);
 /* Scene.time, Scene.dt */
 int orc_get_time(void* h, double* time, double* dt);
 

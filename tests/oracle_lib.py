@@ -74,6 +74,10 @@ def lib():
         "orc_get_dyndt": (C.c_int, [P, C.c_void_p]),
         "orc_dyndt_crit": (C.c_int, [P, pd]),
         "orc_get_time": (C.c_int, [P, pd, pd]),
+        "orc_remove_particles": (C.c_int, [P, I32, pi]),
+        "orc_insert_spheres": (C.c_int, [P, I32, pd, pd, pd, pd, pd, pd, pi, pu, pi]),
+        "orc_set_masks": (C.c_int, [P, I32, pi, pu]),
+        "orc_get_free_ids": (C.c_int, [P, I32, pi, pi, pi]),
     }
     for n, (r, a) in sig.items():
         f = getattr(L, n)
@@ -204,6 +208,35 @@ class OracleRun:
         t, d = C.c_double(), C.c_double()
         self._chk(self.L.orc_get_time(self.h, C.byref(t), C.byref(d)))
         return t.value, d.value
+
+    def remove_particles(self, ids):
+        ids = np.ascontiguousarray(ids, np.int32)
+        self._chk(self.L.orc_remove_particles(self.h, len(ids), _p(ids, C.c_int32)))
+
+    def insert_spheres(self, pos, radius, mass, inertia, mat, mask, vel=None, angVel=None):
+        f64 = lambda x: None if x is None else np.ascontiguousarray(x, np.float64)
+        pos, vel, angVel = f64(pos).reshape(-1, 3), f64(vel), f64(angVel)
+        n = len(pos)
+        bc = lambda x, t: np.ascontiguousarray(np.broadcast_to(np.asarray(x, t), (n,)))
+        radius, mass, inertia = bc(radius, np.float64), bc(mass, np.float64), bc(inertia, np.float64)
+        mat, mask = bc(mat, np.int32), bc(mask, np.uint32)
+        ids = np.empty(n, np.int32)
+        D = C.c_double
+        self._chk(self.L.orc_insert_spheres(self.h, n, _p(pos, D), _p(vel, D), _p(angVel, D), _p(radius, D), _p(mass, D),
+                                            _p(inertia, D), _p(mat, C.c_int32), _p(mask, C.c_uint32), _p(ids, C.c_int32)))
+        return ids
+
+    def set_masks(self, ids, masks):
+        ids = np.ascontiguousarray(ids, np.int32)
+        masks = np.ascontiguousarray(np.broadcast_to(np.asarray(masks, np.uint32), ids.shape))
+        self._chk(self.L.orc_set_masks(self.h, len(ids), _p(ids, C.c_int32), _p(masks, C.c_uint32)))
+
+    def free_ids(self):
+        n, e = C.c_int32(), C.c_int32()
+        self._chk(self.L.orc_get_free_ids(self.h, 0, None, C.byref(n), C.byref(e)))
+        ids = np.empty(n.value, np.int32)
+        self._chk(self.L.orc_get_free_ids(self.h, n.value, _p(ids, C.c_int32), C.byref(n), C.byref(e)))
+        return ids, e.value
 
     def pwave_dt(self):
         return float(self.L.orc_pwave_dt(self.h))

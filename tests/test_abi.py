@@ -45,3 +45,22 @@ def test_footprint_is_monotone():
     a = Context.footprint(1000, 1000, 6.0)
     b = Context.footprint(2000, 2000, 6.0)
     assert 0 < a < b
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libwoodem_hip.so not built")
+def test_kernel_parameters_have_no_address_space():
+    """A kernel parameter declared in the global address space is mangled with it (U3AS1) in
+    the device pass only; the host then launches a name the code object lacks.  Every kernel
+    of the built library must take plain pointers (qualified pointers only inside structs)."""
+    data = open(LIB, "rb").read()
+    kernels = set(m.group(1) for m in re.finditer(rb"(_Z[0-9A-Za-z_]+)\.kd\x00", data))
+    ours = [k for k in kernels if re.match(rb"_Z\d+k_", k)]
+    assert len(ours) > 20, "kernel symbols not found in the library's code object"
+    bad = sorted(k.decode() for k in kernels if b"U3AS" in k)
+    assert not bad, bad
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libwoodem_hip.so not built")
+def test_dyndt_struct_layout():
+    from woodem_amd._native import WdmDynDt
+    assert ctypes.sizeof(WdmDynDt) == 6 * 8 + 5 * 8 + 4 * 4

@@ -118,6 +118,12 @@ def lib():
         "wdm_upload_contacts": (C.c_int, [P, I32, I64, pi, pi, pi, pd, pd, pd, pd, pd]),
         "wdm_set_energy": (C.c_int, [P, I32, pd, pi]),
         "wdm_set_time": (C.c_int, [P, I32, I64, D]),
+        "wdm_remove_particles": (C.c_int, [P, I32, I32, pi]),
+        "wdm_insert_spheres": (C.c_int, [P, I32, I32, pd, pd, pd, pd, pd, pd, pi, pu, pi]),
+        "wdm_get_free_ids": (C.c_int, [P, I32, I32, pi, pi, pi]),
+        "wdm_set_free_ids": (C.c_int, [P, I32, I32, pi, I32]),
+        "wdm_set_masks": (C.c_int, [P, I32, I32, pi, pu]),
+        "wdm_select_box": (C.c_int, [P, I32, pd, pd, I32, U32, U32, I32, pi, pi]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -342,6 +348,53 @@ class Context:
 
     def set_time(self, sid, step, time):
         self.check(self.L.wdm_set_time(self.h, sid, int(step), float(time)))
+
+    def remove_particles(self, sid, ids):
+        """DemField::removeParticle of each id (the slot stays as a hole)"""
+        ids = np.ascontiguousarray(ids, np.int32)
+        self.check(self.L.wdm_remove_particles(self.h, sid, len(ids), ptr(ids, C.c_int32)))
+
+    def insert_spheres(self, sid, pos, radius, mass, inertia, mat, mask, vel=None, angVel=None):
+        """new spheres in free slots (ParticleContainer ids); returns their ids"""
+        f64 = lambda x: None if x is None else np.ascontiguousarray(x, np.float64)
+        pos, vel, angVel = f64(pos).reshape(-1, 3), f64(vel), f64(angVel)
+        n = len(pos)
+        bc = lambda x, t: np.ascontiguousarray(np.broadcast_to(np.asarray(x, t), (n,)))
+        radius, mass, inertia = bc(radius, np.float64), bc(mass, np.float64), bc(inertia, np.float64)
+        mat, mask = bc(mat, np.int32), bc(mask, np.uint32)
+        ids = np.empty(n, np.int32)
+        self.check(self.L.wdm_insert_spheres(self.h, sid, n, ptr(pos, C.c_double), ptr(vel, C.c_double), ptr(angVel, C.c_double),
+                                             ptr(radius, C.c_double), ptr(mass, C.c_double), ptr(inertia, C.c_double),
+                                             ptr(mat, C.c_int32), ptr(mask, C.c_uint32), ptr(ids, C.c_int32)))
+        return ids
+
+    def free_ids(self, sid):
+        """(holes in reuse order, end of the particles in use)"""
+        n, e = C.c_int32(), C.c_int32()
+        self.check(self.L.wdm_get_free_ids(self.h, sid, 0, None, C.byref(n), C.byref(e)))
+        ids = np.empty(n.value, np.int32)
+        self.check(self.L.wdm_get_free_ids(self.h, sid, n.value, ptr(ids, C.c_int32), C.byref(n), C.byref(e)))
+        return ids, e.value
+
+    def set_free_ids(self, sid, ids, par_end):
+        ids = np.ascontiguousarray(ids, np.int32)
+        self.check(self.L.wdm_set_free_ids(self.h, sid, len(ids), ptr(ids, C.c_int32), int(par_end)))
+
+    def set_masks(self, sid, ids, masks):
+        ids = np.ascontiguousarray(ids, np.int32)
+        masks = np.ascontiguousarray(np.broadcast_to(np.asarray(masks, np.uint32), ids.shape))
+        self.check(self.L.wdm_set_masks(self.h, sid, len(ids), ptr(ids, C.c_int32), ptr(masks, C.c_uint32)))
+
+    def select_box(self, sid, lo, hi, inside=False, mask=0, markMask=0):
+        """ids Outlet::run would delete (or mark) for a BoxOutlet, ascending"""
+        lo, hi = np.ascontiguousarray(lo, np.float64), np.ascontiguousarray(hi, np.float64)
+        n = C.c_int32()
+        self.check(self.L.wdm_select_box(self.h, sid, ptr(lo, C.c_double), ptr(hi, C.c_double), int(bool(inside)), int(mask),
+                                         int(markMask), 0, None, C.byref(n)))
+        ids = np.empty(n.value, np.int32)
+        self.check(self.L.wdm_select_box(self.h, sid, ptr(lo, C.c_double), ptr(hi, C.c_double), int(bool(inside)), int(mask),
+                                         int(markMask), n.value, ptr(ids, C.c_int32), C.byref(n)))
+        return ids
 
     def relax(self, sid, iters):
         self.check(self.L.wdm_relax_overlaps(self.h, sid, int(iters)))

@@ -41,12 +41,14 @@ class Cell(_dem._Attrs):
 class EnergyTracker:
     """EnergyTracker, core/EnergyTracker.hpp:10-84 (read-only view after a run)."""
 
-    def __init__(self, values=None, created=None):
+    def __init__(self, values=None, created=None, extra=None):
         self._v = dict()
         if values is not None:
             for k, name in enumerate(ENERGY_NAMES):
                 if created[k]:
                     self._v[name] = float(values[k])
+        for name, v in (extra or {}).items():  # terms added by engines on the host (Outlet's kinOutlet)
+            self._v[name] = float(v)
 
     def __getitem__(self, k):
         return self._v[k]
@@ -83,6 +85,7 @@ class Scene(_dem._Attrs):
         self._uploaded_sig = None
         self._stale = True
         self._con_cache = None
+        self._host_energy = {}
 
     @property
     def dem(self):
@@ -101,12 +104,18 @@ class Scene(_dem._Attrs):
     def _params(self):
         eng = [e for e in self.engines if not getattr(e, "dead", False)]
         kinds = [type(e) for e in eng]
-        dyn = [e for e in eng if isinstance(e, _dem.DynDt)]
-        if dyn:
-            if len(dyn) > 1 or eng[-1] is not dyn[0]:
-                raise ValueError("DynDt must be the last engine (after ContactLoop)")
-            eng = eng[:-1]
-            kinds = kinds[:-1]
+        # engines after ContactLoop: DynDt (on the device, inside each step), then host-side
+        # engines (BoxOutlet) that Scene.run pauses for
+        tail = []
+        while eng and isinstance(eng[-1], (_dem.DynDt, _dem.BoxOutlet)):
+            tail.insert(0, eng.pop())
+            kinds.pop()
+        dyn = [e for e in tail if isinstance(e, _dem.DynDt)]
+        if len(dyn) > 1 or (dyn and tail[0] is not dyn[0]):
+            raise ValueError("DynDt must follow ContactLoop directly (before any Outlet)")
+        for e in tail:
+            if isinstance(e, _dem.BoxOutlet) and (e.realPeriod > 0 or e.virtPeriod > 0):
+                raise NotImplementedError("BoxOutlet runs by stepPeriod on this path")
         if len(eng) != 3 or not (isinstance(eng[0], _dem.Leapfrog) and isinstance(eng[1], _dem.InsertionSortCollider)
                                  and isinstance(eng[2], _dem.ContactLoop)):
             raise ValueError("engines must be [Leapfrog, InsertionSortCollider, ContactLoop] (got %s)" % [k.__name__ for k in kinds])
@@ -168,6 +177,9 @@ class Scene(_dem._Attrs):
             self.dt = p.dt
         return p
 
+    def _host_engines(self):
+        return [e for e in self.engines if isinstance(e, _dem.BoxOutlet) and not getattr(e, "dead", False)]
+
     def _dyndt_engine(self):
         d = [e for e in self.engines if isinstance(e, _dem.DynDt) and not getattr(e, "dead", False)]
         return d[0] if d else None
@@ -213,7 +225,32 @@ class Scene(_dem._Attrs):
         else:
             self._ctx.set_params(self._sid, self._params())
             self._set_dyndt()
-        self._ctx.step([self._sid], int(nSteps))
+        host = self._host_engines()
+        if not host:
+            self._ctx.step([self._sid], int(nSteps))
+        else:
+            # step on the device up to each step a host engine fires on, then run it there (the
+            # reference runs it after ContactLoop of that step; the epilogue in between changes
+            # no state it reads)
+            left = int(nSteps)
+            while left > 0:
+                s0 = self._ctx.stats(self._sid)["step"]
+                fire, fired = None, []
+                for j in range(left):
+                    fired = [e for e in host if e._activate(s0 + j, None)]
+                    if fired:
+                        fire = j
+                        break
+                if fire is None:
+                    self._ctx.step([self._sid], left)
+                    break
+                if fire > 0:
+                    self._ctx.step([self._sid], fire)
+                t, dt = self._ctx.time(self._sid)
+                self._ctx.step([self._sid], 1)
+                for e in fired:
+                    e._run(self, s0 + fire, t, dt)
+                left -= fire + 1
         self._after_run()
         self._stale = True
         self._con_cache = None
@@ -245,7 +282,7 @@ class Scene(_dem._Attrs):
         if self._sid is None:
             return EnergyTracker()
         v, c = self._ctx.energy(self._sid)
-        return EnergyTracker(v, c)
+        return EnergyTracker(v, c, self._host_energy)
 
     def stats(self):
         return self._ctx.stats(self._sid)

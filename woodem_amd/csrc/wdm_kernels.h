@@ -2049,6 +2049,81 @@ __global__ void k_fill_u64(unsigned long long* a, int n, unsigned long long v) {
     if (i < n) a[i] = v;
 }
 
+// ---------------- particles removed and inserted in place ----------------
+// A removed particle (DemField::removeParticle, pkg/dem/Particle.cpp:445-493) keeps its slot as
+// a hole: no shape, its node fully blocked, massless, at rest and skipped by the energy terms,
+// so Leapfrog changes nothing for it; its contacts are dropped.  ParticleContainer reuses the
+// holes first (ParticleContainer.cpp:23-72), which wdm_insert_spheres does.
+// Kernel parameters are plain pointers: an address-space-qualified parameter type is mangled
+// differently in the device pass, so the host would launch a name the code object lacks.
+#define F_INERT (F_DOF_ALL | F_ENERGY_SKIP | F_GRAVITY_SKIP | F_DAMPING_SKIP)
+__global__ void k_remove_particles(int32_t* ptype, uint32_t* flags, double4* velm, double4* angI, uint8_t* checked, const int32_t* ids,
+                                   int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int i = ids[k];
+    ptype[i] = 0;
+    flags[i] = F_INERT;
+    velm[i] = make_double4(0., 0., 0., 0.);
+    angI[i] = make_double4(0., 0., 0., 0.);
+    checked[i] = 0;
+}
+// contacts with a removed particle (ContactContainer::remove, ContactContainer.cpp:124-150)
+__global__ void k_kill_contacts(uint32_t* info, const int2* ids, int64_t nc, const int32_t* ptype) {
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int2 id = ids[c];
+    if (ptShape(ptype[id.x]) == 0 || ptShape(ptype[id.y]) == 0) info[c] = CI_DEAD;
+}
+// a new sphere in slot `id`: node at rest in its given state, force and torque zero (a new
+// DemData's; gravity enters through the force reset of its first Leapfrog, Leapfrog.cpp:240-245)
+struct InsRec {
+    double4 posr, velm, angI;
+    int32_t id, mat;
+    uint32_t mask;
+    int32_t pad;
+};
+__global__ void k_insert_spheres(NodeView n, PartView p, const InsRec* rec, int cnt) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    const InsRec r = rec[k];
+    const int i = r.id;
+    n.posr[i] = r.posr;
+    n.velm[i] = r.velm;
+    n.angI[i] = r.angI;
+    n.ori[i] = make_double4(1., 0., 0., 0.);
+    n.force[i] = make_double4(0., 0., 0., 0.);
+    n.torque[i] = make_double4(0., 0., 0., 0.);
+    n.lastPos[i] = make_double4(r.posr.x, r.posr.y, r.posr.z, 0.);
+    n.flags[i] = 0u;
+    n.checked[i] = 1;
+    p.ptype[i] = 1;  // Sphere
+    p.mask[i] = r.mask;
+    p.mat[i] = r.mat;
+    p.fnodes[i] = make_int4(i, -1, -1, 0);
+}
+__global__ void k_set_masks(uint32_t* pmask, const int32_t* ids, const uint32_t* masks, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) pmask[ids[k]] = masks[k];
+}
+// BoxOutlet::isInside (pkg/dem/Outlet.hpp:56, AlignedBox3r::contains: closed box) and the
+// particle tests of Outlet::run (Outlet.cpp:16-40) for uninodal particles: 1 when the
+// particle is to be deleted (markMask 0) or marked
+__global__ void k_select_box(const double4* posr, const int32_t* ptype, const uint32_t* pmask, int np, double3 lo, double3 hi,
+                             int inside, uint32_t mask, uint32_t markMask, uint8_t* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const int sh = ptShape(ptype[i]);
+    uint8_t sel = 0;
+    if (sh == 1 || sh == 2) {
+        const double4 q = posr[i];
+        const bool in = q.x >= lo.x && q.x <= hi.x && q.y >= lo.y && q.y <= hi.y && q.z >= lo.z && q.z <= hi.z;
+        const uint32_t m = pmask[i];
+        sel = (inside != 0) == in && !(mask && !(mask & m)) && !(markMask != 0 && (markMask & m) == markMask);
+    }
+    out[i] = sel;
+}
+
 // ---------------- scene generation: Jacobi overlap relaxation ----------------
 // Each sphere moves by half of every overlap with its neighbours (found through the
 // same cell grid); all displacements are computed from the old positions, so the

@@ -8,6 +8,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <deque>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -316,6 +317,10 @@ struct Scene {
     wdm_dyndt dd{};                 // DynDt after ContactLoop (enabled 0: not in the engine list)
     bool contactsRun = false;       // a ContactLoop ran on the current contact list
     std::vector<int32_t> facetNodesH, nodeMasterH;  // nodes of facets; clump master of each node (-1)
+    // ParticleContainer ids (pkg/dem/ParticleContainer.cpp:23-72): particles in use end before
+    // parEnd; holes below it are reused first, in the order they were freed
+    int32_t parEnd = 0;
+    std::deque<int32_t> freeIds;
     double verletDist = 0;
     bool verletResolved = false;
     bool needFull = true;
@@ -1967,6 +1972,12 @@ int wdm_upload_particles(wdm_ctx* c, int32_t sid, int32_t n, const int32_t* shap
         for (int p = 0; p < n; p++)
             if (shape[p] == WDM_SHAPE_FACET)
                 for (int k = 0; k < 3; k++) S.facetNodesH.push_back(nodes3[3 * p + k]);
+        S.parEnd = 0;
+        for (int p = 0; p < n; p++)
+            if (shape[p] != WDM_SHAPE_NONE) S.parEnd = p + 1;
+        S.freeIds.clear();
+        for (int p = 0; p < S.parEnd; p++)
+            if (shape[p] == WDM_SHAPE_NONE) S.freeIds.push_back(p);
         S.hasFacets = false;
         S.facetMasks.clear();
         for (int p = 0; p < n; p++)
@@ -2290,6 +2301,170 @@ int wdm_upload_contacts(wdm_ctx* c, int32_t sid, int64_t n, const int32_t* idA, 
         if (n >= INT32_MAX) throw WdmError(WDM_E_ARG, "too many contacts");
         if (S.mixed()) uploadContacts(S, S.cb32, n, idA, idB, cellDist3, trsf9, forceLocal3, phys5, lens2, contPt3);
         else uploadContacts(S, S.cb64, n, idA, idB, cellDist3, trsf9, forceLocal3, phys5, lens2, contPt3);
+    });
+}
+// ---------------- particles removed and inserted in place ----------------
+int wdm_remove_particles(wdm_ctx* c, int32_t sid, int32_t n, const int32_t* ids) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        checkReady(S);
+        if (n < 0 || (n > 0 && !ids)) throw WdmError(WDM_E_ARG, "particle ids");
+        std::vector<char> seen(S.np, 0);
+        for (int k = 0; k < n; k++) {
+            const int id = ids[k];
+            if (id < 0 || id >= S.np) throw WdmError(WDM_E_ARG, "particle id out of range");
+            const int sh = S.ptypeH[id] & 15;
+            if (sh == WDM_SHAPE_NONE) throw WdmError(WDM_E_ARG, "#" + std::to_string(id) + " does not exist");
+            if (sh == WDM_SHAPE_FACET) throw WdmError(WDM_E_UNSUPPORTED, "removing facets is outside the implemented path");
+            if (S.flagsH[id] & (WDM_CLUMP_CLUMPED | WDM_CLUMP_CLUMP))
+                throw WdmError(WDM_E_UNSUPPORTED, "#" + std::to_string(id) + ": a node is clumped, remove the clump itself instead!");  // Particle.cpp:451
+            if (seen[id]++) throw WdmError(WDM_E_ARG, "particle listed twice");
+        }
+        if (n == 0) return;
+        // forces of the last ContactLoop stay with the partners (Woo's ContactLoop applied them
+        // before the removal): the next Leapfrog gathers them through the current CSR and
+        // realness bits, which the dropped contacts keep until the next full run
+        hipStream_t st = c->stream;
+        Shared& X = *c->sh;
+        const int32_t* dIds = (const int32_t*)X.stDyn.upload(c, ids, n * sizeof(int32_t));
+        hipLaunchKernelGGL(k_remove_particles, dim3((n + 255) / 256), dim3(256), 0, st, S.ptype.p, S.flags.p, S.velm.p, S.angI.p,
+                           S.checked.p, dIds, n);
+        checkLaunch();
+        const int64_t nc = S.nc;
+        if (nc > 0) {
+            uint32_t* info = S.mixed() ? S.cb32[S.cur].info.p : S.cb64[S.cur].info.p;
+            const int2* cids = S.mixed() ? S.cb32[S.cur].ids.p : S.cb64[S.cur].ids.p;
+            hipLaunchKernelGGL(k_kill_contacts, dim3(nblk(nc)), dim3(WDM_BLOCK), 0, st, info, cids, nc, (const int32_t*)S.ptype.p);
+            checkLaunch();
+        }
+        for (int k = 0; k < n; k++) {
+            S.ptypeH[ids[k]] = 0;
+            S.flagsH[ids[k]] = 63u | WDM_ENERGY_SKIP | WDM_GRAVITY_SKIP | WDM_DAMPING_SKIP;
+            S.freeIds.push_back(ids[k]);
+        }
+        auto keep = [&](std::vector<int32_t>& v) { v.erase(std::remove_if(v.begin(), v.end(), [&](int32_t q) { return seen[q] != 0; }), v.end()); };
+        keep(S.binnedH);
+        keep(S.unbH);
+        if (!S.binnedH.empty()) HIPCHK(hipMemcpyAsync(S.binned.p, S.binnedH.data(), S.binnedH.size() * 4, hipMemcpyHostToDevice, st));
+        if (!S.unbH.empty()) HIPCHK(hipMemcpyAsync(S.unb.p, S.unbH.data(), S.unbH.size() * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        // the particle set changed: the collider runs a full run (InsertionSortCollider.cpp:475-492)
+        S.needFull = true;
+        S.contactsRun = false;
+    });
+}
+int wdm_insert_spheres(wdm_ctx* c, int32_t sid, int32_t n, const double* pos3, const double* vel3, const double* angVel3,
+                       const double* radius, const double* mass, const double* inertia, const int32_t* mat, const uint32_t* mask,
+                       int32_t* idsOut) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        checkReady(S);
+        if (n < 0 || (n > 0 && (!pos3 || !radius || !mass || !inertia || !mat || !mask || !idsOut))) throw WdmError(WDM_E_ARG, "sphere arrays");
+        if ((int64_t)S.freeIds.size() + (S.np - S.parEnd) < n)
+            throw WdmError(WDM_E_ARG, "no free particle slot (allocate the scene with spare particles of shape NONE)");
+        for (int k = 0; k < n; k++) {
+            if (!(radius[k] > 0)) throw WdmError(WDM_E_ARG, "sphere radius must be positive");
+            if (mat[k] < 0 || mat[k] >= S.nmat) throw WdmError(WDM_E_ARG, "bad material index");
+        }
+        if (n == 0) return;
+        // the node forces of the last ContactLoop go to the arrays, where the new nodes get zero
+        if (S.forcesFromContacts) {
+            materialiseForces(S);
+            S.forcesFromContacts = false;
+        }
+        std::vector<InsRec> rec(n);
+        for (int k = 0; k < n; k++) {
+            int32_t id;
+            if (!S.freeIds.empty()) { id = S.freeIds.front(); S.freeIds.pop_front(); }
+            else id = S.parEnd++;
+            idsOut[k] = id;
+            InsRec& r = rec[k];
+            r.posr = make_double4(pos3[3 * k], pos3[3 * k + 1], pos3[3 * k + 2], radius[k]);
+            r.velm = vel3 ? make_double4(vel3[3 * k], vel3[3 * k + 1], vel3[3 * k + 2], mass[k]) : make_double4(0., 0., 0., mass[k]);
+            r.angI = angVel3 ? make_double4(angVel3[3 * k], angVel3[3 * k + 1], angVel3[3 * k + 2], inertia[k]) : make_double4(0., 0., 0., inertia[k]);
+            r.id = id; r.mat = mat[k]; r.mask = mask[k]; r.pad = 0;
+            S.ptypeH[id] = WDM_SHAPE_SPHERE;
+            S.radiusH[id] = radius[k];
+            S.flagsH[id] = 0;
+            S.binnedH.push_back(id);
+        }
+        hipStream_t st = c->stream;
+        const InsRec* d = (const InsRec*)c->sh->stDyn.upload(c, rec.data(), n * sizeof(InsRec));
+        hipLaunchKernelGGL(k_insert_spheres, dim3((n + 255) / 256), dim3(256), 0, st, S.nview(), S.pview(), d, n);
+        checkLaunch();
+        ensure(c, S.binned, std::max<size_t>(S.binnedH.size(), 1));
+        HIPCHK(hipMemcpyAsync(S.binned.p, S.binnedH.data(), S.binnedH.size() * 4, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        S.needFull = true;
+        S.contactsRun = false;
+    });
+}
+int wdm_get_free_ids(wdm_ctx* c, int32_t sid, int32_t cap, int32_t* ids, int32_t* n, int32_t* parEnd) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        if (!n || !parEnd || cap < 0 || (cap > 0 && !ids)) throw WdmError(WDM_E_ARG, "output arrays");
+        int32_t k = 0;
+        for (int32_t q : S.freeIds) { if (k < cap) ids[k] = q; k++; }
+        *n = k;
+        *parEnd = S.parEnd;
+    });
+}
+int wdm_set_free_ids(wdm_ctx* c, int32_t sid, int32_t n, const int32_t* ids, int32_t parEnd) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        if (n < 0 || (n > 0 && !ids) || parEnd < 0 || parEnd > S.np) throw WdmError(WDM_E_ARG, "free ids");
+        std::vector<char> hole(S.np, 0);
+        for (int p = 0; p < parEnd; p++) hole[p] = (S.ptypeH[p] & 15) == WDM_SHAPE_NONE;
+        for (int p = parEnd; p < S.np; p++)
+            if ((S.ptypeH[p] & 15) != WDM_SHAPE_NONE) throw WdmError(WDM_E_ARG, "a particle in use lies past par_end");
+        int32_t cnt = 0;
+        for (int k = 0; k < n; k++) {
+            if (ids[k] < 0 || ids[k] >= parEnd || hole[ids[k]] != 1) throw WdmError(WDM_E_ARG, "free ids must list each hole below par_end once");
+            hole[ids[k]] = 2;
+            cnt++;
+        }
+        for (int p = 0; p < parEnd; p++)
+            if (hole[p] == 1) throw WdmError(WDM_E_ARG, "free ids must list each hole below par_end once");
+        S.freeIds.assign(ids, ids + n);
+        S.parEnd = parEnd;
+        (void)cnt;
+    });
+}
+int wdm_set_masks(wdm_ctx* c, int32_t sid, int32_t n, const int32_t* ids, const uint32_t* masks) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        if (n < 0 || (n > 0 && (!ids || !masks))) throw WdmError(WDM_E_ARG, "mask arrays");
+        for (int k = 0; k < n; k++)
+            if (ids[k] < 0 || ids[k] >= S.np || (S.ptypeH[ids[k]] & 15) == WDM_SHAPE_NONE) throw WdmError(WDM_E_ARG, "particle id");
+        if (n == 0) return;
+        Blob b;
+        const size_t oi = b.add(ids, n * sizeof(int32_t)), om = b.add(masks, n * sizeof(uint32_t));
+        const uint8_t* base = c->sh->stDyn.upload(c, b.b.data(), b.b.size());
+        hipLaunchKernelGGL(k_set_masks, dim3((n + 255) / 256), dim3(256), 0, c->stream, S.pmask.p, (const int32_t*)(base + oi),
+                           (const uint32_t*)(base + om), n);
+        checkLaunch();
+        HIPCHK(hipStreamSynchronize(c->stream));
+    });
+}
+int wdm_select_box(wdm_ctx* c, int32_t sid, const double* lo3, const double* hi3, int32_t inside, uint32_t mask, uint32_t markMask,
+                   int32_t cap, int32_t* ids, int32_t* nOut) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        if (!lo3 || !hi3 || !nOut || cap < 0 || (cap > 0 && !ids)) throw WdmError(WDM_E_ARG, "box arrays");
+        ensure(c, c->sh->temp, (size_t)std::max(S.np, 1));
+        uint8_t* flag = c->sh->temp.p;
+        if (S.np > 0)
+            hipLaunchKernelGGL(k_select_box, dim3(nblk(S.np)), dim3(WDM_BLOCK), 0, c->stream, (const double4*)S.posr.p,
+                               (const int32_t*)S.ptype.p, (const uint32_t*)S.pmask.p, S.np, make_double3(lo3[0], lo3[1], lo3[2]),
+                               make_double3(hi3[0], hi3[1], hi3[2]), inside, mask, markMask, flag);
+        checkLaunch();
+        std::vector<uint8_t> h(S.np);
+        if (S.np > 0) HIPCHK(hipMemcpyAsync(h.data(), flag, S.np, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        int32_t k = 0;
+        for (int i = 0; i < S.np; i++)
+            if (h[i]) { if (k < cap) ids[k] = i; k++; }
+        *nOut = k;
     });
 }
 int wdm_set_energy(wdm_ctx* c, int32_t sid, const double* values, const int32_t* created) {

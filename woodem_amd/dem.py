@@ -164,6 +164,15 @@ class ParticleContainer:
     def __iter__(self):
         return iter(self._p)
 
+    def exists(self, i):
+        return 0 <= i < len(self._p) and self._p[i] is not None
+
+    def _removed(self, ids):
+        """particles removed on the device (DemField::removeParticle): their slots become None,
+        as ParticleContainer::remove leaves them (ParticleContainer.cpp:121-126)"""
+        for i in ids:
+            self._p[int(i)] = None
+
 
 class ContactView:
     """One contact of DemField.con (Contact + L6Geom + FrictPhys data, read-only)."""
@@ -239,8 +248,9 @@ class DemField(_Attrs):
     def _compile(self):
         if getattr(self, "_from_arrays", False) and not len(self.par):
             return self._arrays
-        pars = list(self.par)
-        n_par = len(pars)
+        slots = list(self.par)
+        n_par = len(slots)
+        pars = [p for p in slots if p is not None]  # holes (removed particles) become inert slots
         mats, matIx = [], {}
         for p in pars:
             if id(p.material) not in matIx:
@@ -271,6 +281,9 @@ class DemField(_Attrs):
         for p in pars:
             for n in p.nodes:
                 allnodes[node_ix[id(n)]] = n
+        for i, p in enumerate(slots):
+            if p is None:  # the hole's node: fully blocked, massless, skipped by the energy terms
+                allnodes[i] = Node(dem=DemData(flags=_soa.DOF_ALL | _soa.ENERGY_SKIP | _soa.GRAVITY_SKIP | _soa.DAMPING_SKIP))
         pos = np.array([n.pos for n in allnodes], dtype=np.float64).reshape(nn, 3)
         ori = np.array([n.ori for n in allnodes], dtype=np.float64).reshape(nn, 4)
         vel = np.array([n.dem.vel for n in allnodes], dtype=np.float64).reshape(nn, 3)
@@ -303,7 +316,7 @@ class DemField(_Attrs):
                 nodes[i, k] = node_ix[id(n)]
             mask[i] = p.mask
             mat[i] = matIx[id(p.material)]
-        self._node_of = np.array([node_ix[id(p.nodes[0])] for p in pars], dtype=np.int64)
+        self._node_of = np.array([node_ix[id(p.nodes[0])] if p is not None else i for i, p in enumerate(slots)], dtype=np.int64)
         arr = _soa.DemArrays(pos=pos, ori=ori, vel=vel, angVel=angVel, mass=mass, inertia=inertia, flags=flags,
                              force=force, torque=torque, shape=shape, nodes=nodes, radius=radius, axis=axis, sense=sense,
                              mask=mask, mat=mat,
@@ -314,6 +327,13 @@ class DemField(_Attrs):
         arr.validate()
         self._arrays = arr
         return arr
+
+    def _mark_removed(self, ids):
+        """the compiled arrays follow a removal on the device (shape NONE, inert node)"""
+        a = self._arrays
+        a.shape[ids] = 0
+        a.flags[a.nodes[ids, 0]] = _soa.DOF_ALL | _soa.ENERGY_SKIP | _soa.GRAVITY_SKIP | _soa.DAMPING_SKIP
+        a.mass[a.nodes[ids, 0]] = 0.
 
     def _refresh(self):
         if self._scene is not None:
@@ -424,6 +444,79 @@ class ContactLoop(Engine):
         super().__init__(**kw)
         if geo is not None:
             self.geoFunctors, self.phyFunctors, self.lawFunctors = list(geo), list(phy), list(law)
+
+
+class PeriodicEngine(Engine):
+    """PeriodicEngine attributes and its activation rule (core/Engine.hpp:129-153,
+    core/Engine.cpp:169-195); realPeriod (wall-clock time) is not supported on this path."""
+    _periodic = dict(stepPeriod=1, virtPeriod=0., realPeriod=0., stepModulo=True, nDo=-1, nDone=0, initRun=True,
+                     virtLast=float("nan"), virtPrev=-1., stepLast=-1, stepPrev=-1)
+
+    def _activate(self, stepNow, virtNow):
+        """PeriodicEngine::isActivated at step `stepNow` (with its side effects)"""
+        if self.realPeriod > 0:
+            raise NotImplementedError("%s.realPeriod is not supported on this path" % type(self).__name__)
+        if self.virtPeriod > 0 and virtNow is None:
+            raise NotImplementedError("%s.virtPeriod is not supported here (use stepPeriod)" % type(self).__name__)
+        initNow = self.stepLast < 0
+        if initNow:
+            self.virtLast, self.stepLast = virtNow, stepNow
+        if (self.nDo < 0 or self.nDone < self.nDo) and (
+                (self.virtPeriod > 0 and virtNow - self.virtLast >= self.virtPeriod) or
+                (self.stepPeriod > 0 and ((stepNow - self.stepLast >= self.stepPeriod) or
+                                          (self.stepModulo and stepNow % self.stepPeriod == 0)))):
+            self.virtPrev, self.virtLast = self.virtLast, virtNow
+            self.stepPrev, self.stepLast = self.stepLast, stepNow
+            if initNow and not self.initRun:
+                return False
+            self.nDone += 1
+            return True
+        return False
+
+
+class BoxOutlet(PeriodicEngine):
+    """BoxOutlet (pkg/dem/Outlet.hpp:52-60) with the Outlet attributes (Outlet.hpp:22-38) and
+    Outlet::run (Outlet.cpp:12-110) for uninodal particles: deletes (markMask 0) or marks the
+    particles outside `box` (inside it with inside=True) matching `mask`.  It runs after the
+    engines on the device; Scene.run pauses there at the steps it fires on.  The particles'
+    kinetic energy goes to Scene.energy['kinOutlet'] when Scene.trackEnergy is set."""
+    _defaults = dict(box=((0., 0., 0.), (0., 0., 0.)), inside=False, mask=0b0100, markMask=0, save=False, num=0, mass=0.,
+                     currRate=float("nan"), currRateSmooth=1., diamMassTime=lambda: [], **PeriodicEngine._periodic)
+
+    def _run(self, scene, step, time, dt):
+        ctx, sid, a = scene._ctx, scene._sid, scene.dem._arrays
+        lo, hi = np.asarray(self.box[0], np.float64), np.asarray(self.box[1], np.float64)
+        ids = ctx.select_box(sid, lo, hi, self.inside, self.mask, self.markMask)
+        deleting = self.markMask == 0
+        m = a.mass[ids] if len(ids) else np.zeros(0)
+        if deleting and scene.trackEnergy and len(ids):
+            nd = ctx.download_nodes(sid, a.n_node)
+            v, w = nd["vel"][ids], nd["angVel"][ids]
+            ek = .5 * m * np.einsum("ij,ij->i", v, v) + .5 * np.einsum("ij,ij->i", w, a.inertia[ids] * w)
+            scene._host_energy["kinOutlet"] = scene._host_energy.get("kinOutlet", 0.) + float(np.sum(ek))
+        stepMass = float(np.sum(m))
+        self.num += len(ids)
+        self.mass += stepMass
+        if self.save:
+            for i, mi in zip(ids, m):
+                self.diamMassTime.append((2 * float(a.radius[i]), float(mi), time))
+        if len(ids):
+            if deleting:
+                ctx.remove_particles(sid, ids)
+                scene.dem.par._removed(ids)
+                scene.dem._mark_removed(ids)
+            else:
+                ctx.set_masks(sid, ids, a.mask[ids] | self.markMask)
+                a.mask[ids] |= self.markMask
+        span = self.stepPeriod if (self.stepPrev < 0 and self.stepPeriod > 0) else step - self.stepPrev
+        den = span * dt
+        # the reference divides in C++: 0/0 is NaN (the first run, fired at the step counting starts)
+        rate = stepMass / den if den != 0 else (math.nan if stepMass == 0 else math.copysign(math.inf, stepMass))
+        if math.isnan(self.currRate) or self.stepPrev < 0:
+            self.currRate = rate
+        else:
+            self.currRate = (1 - self.currRateSmooth) * self.currRate + self.currRateSmooth * rate
+        return ids
 
 
 class DynDt(Engine):

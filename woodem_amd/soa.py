@@ -163,6 +163,35 @@ def append_walls(a, walls, mat=0, mask=DEFAULT_BOUNDARY_MASK):
         density=a.density, young=a.young, ktDivKn=a.ktDivKn, tanPhi=a.tanPhi, meta=dict(a.meta)).normalized()
 
 
+def with_spare(a, k):
+    """Arrays with `k` spare particle slots (shape NONE) after the particles, for particles
+    inserted later (Context.insert_spheres): each slot has its node (index == slot), fully
+    blocked, massless and skipped by the energy terms; facet nodes move up by k."""
+    a = a.normalized()
+    n0, m = a.n_par, a.n_node
+    inert = DOF_ALL | ENERGY_SKIP | GRAVITY_SKIP | DAMPING_SKIP
+    ins = lambda x, fill: np.concatenate([x[:n0], fill, x[n0:]], 0)
+    z3 = np.zeros((k, 3))
+    ori = np.zeros((k, 4))
+    ori[:, 0] = 1
+    nodes = a.nodes.copy()
+    nodes[(a.shape == SHAPE_FACET)[:, None] & (nodes >= n0)] += k
+    out = DemArrays(
+        pos=ins(a.pos, z3), ori=ins(a.ori, ori), vel=ins(a.vel, z3), angVel=ins(a.angVel, z3), mass=ins(a.mass, np.zeros(k)),
+        inertia=ins(a.inertia, z3), flags=ins(a.flags, np.full(k, inert, np.uint32)), force=ins(a.force, z3),
+        torque=ins(a.torque, z3), shape=np.concatenate([a.shape, np.zeros(k, np.int32)]),
+        nodes=np.concatenate([nodes, -np.ones((k, 3), np.int32)]), radius=np.concatenate([a.radius, np.zeros(k)]),
+        axis=np.concatenate([a.axis, np.zeros(k, np.int32)]), sense=np.concatenate([a.sense, np.zeros(k, np.int32)]),
+        mask=np.concatenate([a.mask, np.zeros(k, np.uint32)]), mat=np.concatenate([a.mat, np.zeros(k, np.int32)]),
+        density=a.density, young=a.young, ktDivKn=a.ktDivKn, tanPhi=a.tanPhi, meta=dict(a.meta))
+    if a.clumps:
+        c = dict(a.clumps)
+        bump = lambda x: np.where(np.asarray(x) >= n0, np.asarray(x) + k, np.asarray(x))
+        c["master"], c["members"] = bump(c["master"]), bump(c["members"])
+        out.clumps = c
+    return out.normalized()
+
+
 def permute_particles(a, perm):
     """Relabel uninodal particles (no facets) with a new order: new id i = old id perm[i]."""
     a = a.normalized()
