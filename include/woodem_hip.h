@@ -253,6 +253,19 @@ int wdm_set_masks(wdm_ctx* ctx, int32_t scene, int32_t n, const int32_t* ids, co
 int wdm_select_box(wdm_ctx* ctx, int32_t scene, const double* lo3, const double* hi3, int32_t inside, uint32_t mask,
                    uint32_t mark_mask, int32_t cap, int32_t* ids, int32_t* n);
 
+/* SpherePack::makeCloud (lib/sphere-pack/SpherePack.cpp:135-248) on the context's device:
+   a random loose packing in the box mn..mx (periodic: the cell of that size), radii from
+   rMean/rRelFuzz, num/porosity or a PSD (psdSizes diameters, psdCumm cumulative, nPsd
+   points; distributeMass as the reference), placed sphere after sphere in the reference's
+   order.  *rng_state is boost::minstd_rand's state (1..2^31-2), advanced by the draws (the
+   reference's generator is a static, seeded once per process).  Spheres go to c3/r (cap of
+   them; with num < 0 the packing stops at cap spheres or at the first sphere without a free
+   spot); sheared cells (hSize) are not supported. */
+int wdm_make_cloud(wdm_ctx* ctx, const double* mn3, const double* mx3, double rMean, double rRelFuzz, int32_t num,
+                   int32_t periodic, double porosity, const double* psdSizes, const double* psdCumm, int32_t nPsd,
+                   int32_t distributeMass, uint32_t* rng_state, int64_t cap, double* c3, double* r, int64_t* n_out,
+                   double* appliedPsdScaling);
+
 /* EnergyTracker values and whether each slot exists */
 int wdm_get_energy(wdm_ctx* ctx, int32_t scene, double* values, int32_t* created);
 /* Scene.step, InsertionSortCollider.nFullRuns, resolved verletDist */

@@ -1211,6 +1211,99 @@ struct Oracle {
 
 Oracle* O(void* h) { return static_cast<Oracle*>(h); }
 
+/* ---------- SpherePack::makeCloud, lib/sphere-pack/SpherePack.cpp:135-248 (no hSize) ----------
+   The generator: boost::minstd_rand (x <- 48271 x mod 2^31-1) through boost::uniform_real(0,1),
+   u = (x - 1)/(2^31 - 3), redrawn when it reaches 1 (Boost's pre-1.47 uniform_real; the
+   reference pins no Boost version). */
+struct CloudGen {
+    uint32_t x;
+    double rnd() {
+        for (;;) {
+            x = (uint32_t)(((uint64_t)x * 48271u) % 2147483647u);
+            double u = (double)(x - 1u) / 2147483645.;
+            if (u < 1.) return u;
+        }
+    }
+};
+struct Sph { V3 c; double r; };
+double cloudPow3Interp(double x, double a, double b) { return std::pow(x * (std::pow(b, -2) - std::pow(a, -2)) + std::pow(a, -2), -1. / 2); }  // SpherePack.hpp:86
+int cloudGetPiece(double x, const std::vector<double>& cumm, double& norm) {  // SpherePack.cpp:283-291
+    int sz = (int)cumm.size(), i = 0;
+    while (i < sz && cumm[i] <= x) i++;
+    if ((i == sz - 1) && cumm[i] <= x) { i = sz - 2; norm = 1.; return i; }
+    i--;
+    norm = (x - cumm[i]) / (cumm[i + 1] - cumm[i]);
+    return i;
+}
+double cloudWrapRel(double x, double x0, double x1) { double xNorm = (x - x0) / (x1 - x0); return (xNorm - std::floor(xNorm)) * (x1 - x0); }  // SpherePack.hpp:32-35
+long cloudMake(std::vector<Sph>& pack, CloudGen& g, V3 mn, V3 mx, double rMean, double rRelFuzz, int num, bool periodic, double porosity,
+               const std::vector<double>& psdSizes, const std::vector<double>& psdCumm, bool distributeMass, long cap, double& applied, int depth) {
+    if (depth > 200) throw std::runtime_error("makeCloud: no packing after 200 porosity increases");
+    std::vector<double> psdRadii, psdCumm2;
+    V3 size = vsub(mx, mn);
+    double volume = size[0] * (size[1] * size[2]);
+    if (!volume) throw std::invalid_argument("The box defined as null volume.");
+    int mode = -1; bool err = false;
+    if (porosity <= 0) porosity = 0.5;
+    if (rMean > 0) mode = 0;
+    else if (num > 0 && psdSizes.size() == 0) { mode = 1; rMean = std::pow(volume * (1 - porosity) / (PI * (4 / 3.) * (1 + rRelFuzz * rRelFuzz) * num), 1 / 3.); }
+    if (psdSizes.size() > 0) {
+        err = (mode >= 0); mode = 2;
+        if (psdSizes.size() <= 1) throw std::invalid_argument("psdSizes must have at least 2 items");
+        for (size_t i = 0; i < psdSizes.size(); i++) {
+            psdRadii.push_back(.5 * psdSizes[i]);
+            if (distributeMass) {
+                if (i == 0) psdCumm2.push_back(0);
+                else psdCumm2.push_back(psdCumm2[i - 1] + 3.0 * volume * (1 - porosity) / PI * (psdCumm[i] - psdCumm[i - 1]) / (psdSizes[i] - psdSizes[i - 1]) *
+                                                          (std::pow(psdSizes[i - 1], -2) - std::pow(psdSizes[i], -2)));
+            }
+        }
+        if (distributeMass) {
+            if (num > 0 && psdCumm2.back() < num) {
+                applied = std::pow(psdCumm2.back() / num, 1. / 3.);
+                for (double& q : psdRadii) q *= applied;
+            }
+            for (size_t i = 1; i < psdSizes.size(); i++) psdCumm2[i] /= psdCumm2.back();
+        }
+    }
+    if (err || mode < 0) throw std::invalid_argument("makeCloud: radius specification");
+    const int maxTry = 1000;
+    double r = 0;
+    for (long i = 0; (i < num) || (num < 0); i++) {
+        if (num < 0 && i >= cap) return i;  // the stop at the output's capacity (the product's rule)
+        double norm, rand;
+        if (num > 0) rand = ((double)num - (double)i + 0.5) / ((double)num + 1.);
+        else rand = g.rnd();
+        if (mode != 2) r = distributeMass ? cloudPow3Interp(rand, rMean * (1 - rRelFuzz), rMean * (1 + rRelFuzz)) : rMean * (2 * (rand - .5) * rRelFuzz + 1);
+        else if (distributeMass) { int piece = cloudGetPiece(rand, psdCumm2, norm); r = cloudPow3Interp(norm, psdRadii[piece], psdRadii[piece + 1]); }
+        else { int piece = cloudGetPiece(rand, psdCumm, norm); r = psdRadii[piece] + norm * (psdRadii[piece + 1] - psdRadii[piece]); }
+        int t;
+        for (t = 0; t < maxTry; ++t) {
+            V3 c;
+            if (!periodic) { for (int axis = 0; axis < 3; axis++) c[axis] = mn[axis] + r + (size[axis] - 2 * r) * g.rnd(); }
+            else { for (int axis = 0; axis < 3; axis++) c[axis] = g.rnd(); for (int axis = 0; axis < 3; axis++) c[axis] = mn[axis] + size[axis] * c[axis]; }
+            bool overlap = false;
+            for (const Sph& q : pack) {
+                V3 dr;
+                for (int axis = 0; axis < 3; axis++)
+                    dr[axis] = periodic ? std::min(cloudWrapRel(c[axis], q.c[axis], q.c[axis] + size[axis]), cloudWrapRel(q.c[axis], c[axis], c[axis] + size[axis]))
+                                        : q.c[axis] - c[axis];
+                if ((q.r + r) * (q.r + r) >= vsqn(dr)) { overlap = true; break; }
+            }
+            if (!overlap) { pack.push_back(Sph{c, r}); break; }
+        }
+        if (t == maxTry) {
+            if (num > 0 && mode != 0) {
+                double nextPoro = porosity + (1 - porosity) / 10.;
+                pack.clear();
+                return cloudMake(pack, g, mn, mx, -1., rRelFuzz, num, periodic, nextPoro, psdSizes, psdCumm, distributeMass, cap, applied, depth + 1);
+            }
+            return i;
+        }
+    }
+    return (long)pack.size();
+}
+
 }  // namespace
 
 extern "C" {
@@ -1434,6 +1527,27 @@ int orc_get_free_ids(void* h, int32_t cap, int32_t* ids, int32_t* n, int32_t* pa
     for (int q : o->freeIds) { if (k < cap) ids[k] = q; k++; }
     *n = k;
     *parEnd = o->parEnd;
+    return 0;
+}
+int orc_make_cloud(const double* mn3, const double* mx3, double rMean, double rRelFuzz, int32_t num, int32_t periodic, double porosity,
+                   const double* psdSizes, const double* psdCumm, int32_t nPsd, int32_t distributeMass, uint32_t* state, int64_t cap,
+                   double* c3, double* r, int64_t* nOut, double* appliedPsdScaling) {
+    try {
+        std::vector<Sph> pack;
+        CloudGen g{*state};
+        double applied = 1.;
+        std::vector<double> ps(psdSizes, psdSizes + (nPsd > 0 ? nPsd : 0)), pc(psdCumm, psdCumm + (nPsd > 0 ? nPsd : 0));
+        cloudMake(pack, g, v3(mn3[0], mn3[1], mn3[2]), v3(mx3[0], mx3[1], mx3[2]), rMean, rRelFuzz, num, periodic != 0, porosity, ps, pc,
+                  distributeMass != 0, cap, applied, 0);
+        if ((int64_t)pack.size() > cap) return -1;
+        for (size_t i = 0; i < pack.size(); i++) {
+            for (int d = 0; d < 3; d++) c3[3 * i + d] = pack[i].c[d];
+            r[i] = pack[i].r;
+        }
+        *nOut = (int64_t)pack.size();
+        *state = g.x;
+        if (appliedPsdScaling) *appliedPsdScaling = applied;
+    } catch (std::exception&) { return -1; }
     return 0;
 }
 int orc_set_dyndt(void* h, const orc_dyndt* d) {

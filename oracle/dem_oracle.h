@@ -87,6 +87,11 @@ int orc_insert_spheres(void* h, int32_t n, const double* pos3, const double* vel
                        const double* mass, const double* inertia, const int32_t* mat, const uint32_t* mask, int32_t* idsOut);
 int orc_set_masks(void* h, int32_t n, const int32_t* ids, const uint32_t* masks);
 int orc_get_free_ids(void* h, int32_t cap, int32_t* ids, int32_t* n, int32_t* parEnd);
+/* SpherePack::makeCloud (lib/sphere-pack/SpherePack.cpp:135-248), sequential, no hSize;
+   *state: boost::minstd_rand state (1..2^31-2), advanced by the draws */
+int orc_make_cloud(const double* mn3, const double* mx3, double rMean, double rRelFuzz, int32_t num, int32_t periodic, double porosity,
+                   const double* psdSizes, const double* psdCumm, int32_t nPsd, int32_t distributeMass, uint32_t* state, int64_t cap,
+                   double* c3, double* r, int64_t* nOut, double* appliedPsdScaling);
 /* Scene.time, Scene.dt */
 int orc_get_time(void* h, double* time, double* dt);
 

@@ -78,6 +78,7 @@ def lib():
         "orc_insert_spheres": (C.c_int, [P, I32, pd, pd, pd, pd, pd, pd, pi, pu, pi]),
         "orc_set_masks": (C.c_int, [P, I32, pi, pu]),
         "orc_get_free_ids": (C.c_int, [P, I32, pi, pi, pi]),
+        "orc_make_cloud": (C.c_int, [pd, pd, D, D, I32, I32, D, pd, pd, I32, I32, pu, I64, pd, pd, pl, pd]),
     }
     for n, (r, a) in sig.items():
         f = getattr(L, n)
@@ -258,3 +259,23 @@ def pair_keys(c):
     lo = np.minimum(c["idA"], c["idB"]).astype(np.int64)
     hi = np.maximum(c["idA"], c["idB"]).astype(np.int64)
     return np.sort((lo << 32) | hi)
+
+
+def make_cloud(mn, mx, rMean=-1, rRelFuzz=0, num=-1, periodic=False, porosity=.5, psdSizes=(), psdCumm=(), distributeMass=False,
+               state=1, cap=100000):
+    """oracle SpherePack::makeCloud: (centres, radii, final generator state, appliedPsdScaling)"""
+    L = lib()
+    f64 = lambda x: np.ascontiguousarray(x, np.float64)
+    mn, mx, ps, pc = f64(mn), f64(mx), f64(psdSizes), f64(psdCumm)
+    st = np.array([state], np.uint32)
+    c = np.empty((cap, 3))
+    r = np.empty(cap)
+    n = C.c_int64()
+    sc = C.c_double()
+    D = C.c_double
+    rc = L.orc_make_cloud(_p(mn, D), _p(mx, D), float(rMean), float(rRelFuzz), int(num), int(bool(periodic)), float(porosity),
+                          _p(ps, D) if len(ps) else None, _p(pc, D) if len(pc) else None, len(ps), int(bool(distributeMass)),
+                          _p(st, C.c_uint32), int(cap), _p(c, D), _p(r, D), C.byref(n), C.byref(sc))
+    if rc != 0:
+        raise RuntimeError("oracle makeCloud failed")
+    return c[:n.value], r[:n.value], int(st[0]), sc.value

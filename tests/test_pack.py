@@ -1,0 +1,109 @@
+"""SpherePack::makeCloud (lib/sphere-pack/SpherePack.cpp:135-248) on the device.
+
+The oracle restates the reference's sequential algorithm (every candidate tested against
+every placed sphere); it is pinned by closed forms and invariants: the radius sequence for
+num > 0 ((num - i + 1/2)/(num + 1) through the mode's rule, decreasing), the mean radius
+from the porosity, no two spheres overlapping (periodic: minimum image), every centre inside
+its box (aperiodic: at least r from the walls), the generator's sequence (Park-Miller
+x <- 48271 x mod 2^31-1: after 10000 draws from 1 the state is 399268537, the known value of
+the minimal standard's 10000th output from seed 1), a PSD's radii within its range and the
+retry at a higher porosity when the box cannot hold num spheres.  The device packing is
+compared with the oracle's sphere for sphere, bit for bit.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as OL
+
+
+def min_gap(c, r, L=None):
+    d = c[:, None, :] - c[None, :, :]
+    if L is not None:
+        d = d - np.round(d / L) * L
+    dist = np.sqrt((d ** 2).sum(-1))
+    np.fill_diagonal(dist, np.inf)
+    return (dist - r[:, None] - r[None, :]).min()
+
+
+def test_generator_is_minimal_standard():
+    # 10000 draws of x <- 48271 x mod (2^31 - 1) from x0 = 1 give 399268537 (C++11 minstd_rand's
+    # required 10000th value); makeCloud of one sphere in a box draws 3 numbers per attempt
+    x = 1
+    for _ in range(10000):
+        x = (48271 * x) % 2147483647
+    assert x == 399268537
+    c, r, st, _ = OL.make_cloud((0, 0, 0), (1, 1, 1), rMean=.1, num=1, state=1)
+    x = 1
+    for _ in range(3):
+        x = (48271 * x) % 2147483647
+    assert st == x
+    u = [(48271 ** k % 2147483647 - 1) / 2147483645. for k in (1, 2, 3)]
+    assert np.array_equal(c[0], [.1 + (1 - .2) * u[0], .1 + (1 - .2) * u[1], .1 + (1 - .2) * u[2]])
+
+
+def test_radius_sequence_rmean():
+    num, rMean, fuzz = 200, .004, .25
+    c, r, st, _ = OL.make_cloud((0, 0, 0), (.1, .1, .1), rMean=rMean, rRelFuzz=fuzz, num=num, state=11)
+    assert len(r) == num
+    k = np.arange(num)
+    want = rMean * (2 * ((num - k + .5) / (num + 1.) - .5) * fuzz + 1)
+    assert np.array_equal(r, want)
+    assert np.all(np.diff(r) < 0)
+    assert min_gap(c, r) > 0
+    assert np.all(c - r[:, None] >= 0) and np.all(c + r[:, None] <= .1)
+
+
+def test_mean_radius_from_porosity():
+    # RDIST_NUM: rMean = (V (1-p) / (4/3 pi (1+f^2) num))^(1/3)
+    num, p, f = 100, .8, .1
+    c, r, st, _ = OL.make_cloud((0, 0, 0), (.2, .1, .1), rRelFuzz=f, num=num, porosity=p, state=5)
+    rm = (.2 * .1 * .1 * (1 - p) / (math.pi * 4 / 3. * (1 + f * f) * num)) ** (1 / 3.)
+    k = np.arange(num)
+    assert np.allclose(r, rm * (2 * ((num - k + .5) / (num + 1.) - .5) * f + 1), rtol=1e-15, atol=0)
+    assert r.mean() == pytest.approx(rm * (1 + f / (num + 1)), rel=1e-14)  # the sequence's mean
+
+
+def test_periodic_minimum_image_and_psd():
+    L = np.array([.05, .06, .07])
+    sizes, cumm = [.004, .006, .01], [0., .3, 1.]
+    c, r, st, _ = OL.make_cloud((0, 0, 0), L, psdSizes=sizes, psdCumm=cumm, num=150, periodic=True, state=3)
+    assert len(r) == 150
+    assert np.all(r >= .002 - 1e-15) and np.all(r <= .005 + 1e-15)
+    assert min_gap(c, r, L) > 0
+    assert np.all(c >= 0) and np.all(c < L + 1e-15)
+
+
+def test_retry_with_higher_porosity():
+    # 2000 spheres at porosity .3 do not fit a random loose packing: the radii shrink until they do
+    c, r, st, _ = OL.make_cloud((0, 0, 0), (.05, .05, .05), num=2000, porosity=.3, state=9)
+    assert len(r) == 2000
+    rm0 = (.05 ** 3 * .7 / (math.pi * 4 / 3. * 2000)) ** (1 / 3.)
+    assert r.mean() < rm0
+    assert min_gap(c, r) > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["rmean", "num", "psd_periodic", "psd_mass", "stochastic"])
+def test_gpu_make_cloud_equals_oracle(case):
+    from woodem_amd._native import Context
+    ctx = Context(0)
+    kw = dict(rmean=dict(mn=(0, 0, 0), mx=(.1, .1, .1), rMean=.003, rRelFuzz=.3, num=3000),
+              num=dict(mn=(-.1, 0, .2), mx=(.1, .05, .3), rRelFuzz=.2, num=4000, porosity=.7),
+              psd_periodic=dict(mn=(0, 0, 0), mx=(.05, .06, .07), psdSizes=[.002, .004, .006], psdCumm=[0, .4, 1], num=1200,
+                                periodic=True),
+              psd_mass=dict(mn=(0, 0, 0), mx=(.08, .08, .08), psdSizes=[.002, .003, .008], psdCumm=[0, .5, 1], num=3000,
+                            distributeMass=True),
+              stochastic=dict(mn=(0, 0, 0), mx=(.07, .07, .07), rMean=.003, rRelFuzz=.4, num=-1))[case]
+    mn, mx = kw.pop("mn"), kw.pop("mx")
+    cap = 20000
+    o_c, o_r, o_st, o_sc = OL.make_cloud(mn, mx, state=12345, cap=cap, **kw)
+    st = np.array([12345], np.uint32)
+    args = dict(rMean=-1, rRelFuzz=0, num=-1, periodic=False, porosity=.5, psdSizes=[], psdCumm=[], distributeMass=False)
+    args.update(kw)
+    g_c, g_r, g_sc = ctx.make_cloud(mn, mx, args["rMean"], args["rRelFuzz"], args["num"], args["periodic"], args["porosity"],
+                                    args["psdSizes"], args["psdCumm"], args["distributeMass"], st, cap)
+    assert len(g_r) == len(o_r) > 100
+    assert np.array_equal(g_r, o_r) and np.array_equal(g_c, o_c)
+    assert int(st[0]) == o_st and g_sc == o_sc

@@ -124,6 +124,7 @@ def lib():
         "wdm_set_free_ids": (C.c_int, [P, I32, I32, pi, I32]),
         "wdm_set_masks": (C.c_int, [P, I32, I32, pi, pu]),
         "wdm_select_box": (C.c_int, [P, I32, pd, pd, I32, U32, U32, I32, pi, pi]),
+        "wdm_make_cloud": (C.c_int, [P, pd, pd, D, D, I32, I32, D, pd, pd, I32, I32, pu, I64, pd, pd, pl, pd]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -395,6 +396,21 @@ class Context:
         self.check(self.L.wdm_select_box(self.h, sid, ptr(lo, C.c_double), ptr(hi, C.c_double), int(bool(inside)), int(mask),
                                          int(markMask), n.value, ptr(ids, C.c_int32), C.byref(n)))
         return ids
+
+    def make_cloud(self, mn, mx, rMean, rRelFuzz, num, periodic, porosity, psdSizes, psdCumm, distributeMass, state, cap):
+        """SpherePack::makeCloud on the device; `state` (uint32 array of 1) is advanced in place.
+        Returns (centres, radii, appliedPsdScaling)."""
+        f64 = lambda x: np.ascontiguousarray(x, np.float64)
+        mn, mx, ps, pc = f64(mn), f64(mx), f64(psdSizes), f64(psdCumm)
+        c = np.empty((cap, 3))
+        r = np.empty(cap)
+        n = C.c_int64()
+        sc = C.c_double()
+        self.check(self.L.wdm_make_cloud(self.h, ptr(mn, C.c_double), ptr(mx, C.c_double), float(rMean), float(rRelFuzz), int(num),
+                                         int(bool(periodic)), float(porosity), ptr(ps, C.c_double) if len(ps) else None,
+                                         ptr(pc, C.c_double) if len(pc) else None, len(ps), int(bool(distributeMass)),
+                                         ptr(state, C.c_uint32), int(cap), ptr(c, C.c_double), ptr(r, C.c_double), C.byref(n), C.byref(sc)))
+        return c[:n.value], r[:n.value], sc.value
 
     def relax(self, sid, iters):
         self.check(self.L.wdm_relax_overlaps(self.h, sid, int(iters)))

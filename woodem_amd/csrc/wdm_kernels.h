@@ -2124,6 +2124,151 @@ __global__ void k_select_box(const double4* posr, const int32_t* ptype, const ui
     out[i] = sel;
 }
 
+// ---------------- SpherePack::makeCloud (lib/sphere-pack/SpherePack.cpp:135-248) ----------------
+// Random loose packing, sphere after sphere in the reference's order: each sphere tries up
+// to maxTry positions drawn from the generator and takes the first that overlaps no sphere
+// placed before it, (rj + r)^2 >= |cj - c|^2 being an overlap; the periodic distance is the
+// smaller of the two wrapped differences per axis (SpherePack::cellWrapRel, SpherePack.hpp:32-35).
+// The generator is boost::minstd_rand (Park-Miller, x <- 48271 x mod 2^31-1, values 1..2^31-2)
+// read through boost::uniform_real(0,1): u = (x-1)/(2^31-3), redrawn when u reaches 1.  One
+// workgroup runs the sequence: a sphere's candidate is tested by the threads against the
+// spheres in the 27 grid cells around it (cells at least 2 rmax wide, so they hold every
+// sphere that can overlap it), which gives the same decisions as testing all placed spheres.
+struct CloudArgs {
+    double mn[3], size[3];
+    double h[3];      // grid cell widths
+    int nc[3];        // grid cells per axis
+    int periodic;
+    int num;          // > 0: radii[i] for sphere i (the reference's deterministic sequence); < 0: drawn
+    const double* radii;
+    // radius rule for num < 0 (SpherePack.cpp:191-209): mode 0 uniform/pow3 in rMean(1 +- rRelFuzz), 1 PSD
+    int mode, distributeMass, nPsd;
+    double rMean, rRelFuzz;
+    const double* psdR;   // PSD radii
+    const double* psdC;   // cumulative (psdCumm, or psdCumm2 with distributeMass)
+    int64_t cap;      // spheres the output holds
+    int maxTry;
+};
+WDM_HD uint32_t mstdNext(uint32_t x) { return (uint32_t)(((uint64_t)x * 48271u) % 2147483647u); }
+WDM_HD double mstdUniform(uint32_t& x) {
+    for (;;) {
+        x = mstdNext(x);
+        const double u = (double)(x - 1u) / 2147483645.;
+        if (u < 1.) return u;
+    }
+}
+// SpherePack::pow3Interp (SpherePack.hpp:86)
+WDM_HD double pow3Interp(double x, double a, double b) { return pow(x * (pow(b, -2.) - pow(a, -2.)) + pow(a, -2.), -1. / 2); }
+// SpherePack::psdGetPiece (SpherePack.cpp:283-291)
+WDM_HD int psdGetPiece(double x, const double* cumm, int sz, double& norm) {
+    int i = 0;
+    while (i < sz && cumm[i] <= x) i++;
+    if ((i == sz - 1) && cumm[i] <= x) { norm = 1.; return sz - 2; }
+    i--;
+    norm = (x - cumm[i]) / (cumm[i + 1] - cumm[i]);
+    return i;
+}
+__device__ double cloudRadius(const CloudArgs& A, double rnd) {
+    if (A.mode == 0) return A.distributeMass ? pow3Interp(rnd, A.rMean * (1 - A.rRelFuzz), A.rMean * (1 + A.rRelFuzz))
+                                             : A.rMean * (2 * (rnd - .5) * A.rRelFuzz + 1);
+    double norm;
+    const int piece = psdGetPiece(rnd, A.psdC, A.nPsd, norm);
+    return A.distributeMass ? pow3Interp(norm, A.psdR[piece], A.psdR[piece + 1]) : A.psdR[piece] + norm * (A.psdR[piece + 1] - A.psdR[piece]);
+}
+__device__ __forceinline__ double cloudWrapRel(double x, double x0, double x1) {
+    const double xNorm = (x - x0) / (x1 - x0);
+    return (xNorm - floor(xNorm)) * (x1 - x0);
+}
+__device__ __forceinline__ int cloudCell(const CloudArgs& A, int d, double x) {
+    int c = (int)floor((x - A.mn[d]) / A.h[d]);
+    return c < 0 ? 0 : (c >= A.nc[d] ? A.nc[d] - 1 : c);
+}
+// out[0]: spheres placed; out[1]: 1 when a sphere found no free spot in maxTry tries
+__global__ void __launch_bounds__(256) k_make_cloud(CloudArgs A, uint32_t* state, int32_t* head, int32_t* next, double4* sph, int64_t* out) {
+    __shared__ double4 cand;
+    __shared__ int cc[3];
+    __shared__ int done, placed;
+    __shared__ uint32_t st;
+    const int tid = threadIdx.x;
+    if (tid == 0) { st = *state; done = 0; }
+    __syncthreads();
+    int64_t i = 0;
+    for (;; i++) {
+        if ((A.num > 0 && i >= A.num) || i >= A.cap) break;
+        if (tid == 0) {
+            uint32_t x = st;
+            cand.w = A.num > 0 ? A.radii[i] : cloudRadius(A, mstdUniform(x));
+            st = x;
+            placed = 0;
+        }
+        __syncthreads();
+        const double r = cand.w;
+        int t = 0;
+        for (; t < A.maxTry; t++) {
+            if (tid == 0) {
+                uint32_t x = st;
+                double c[3];
+                for (int d = 0; d < 3; d++) {
+                    const double u = mstdUniform(x);
+                    c[d] = A.periodic ? A.mn[d] + A.size[d] * u : (A.mn[d] + r) + (A.size[d] - 2 * r) * u;
+                }
+                st = x;
+                cand = make_double4(c[0], c[1], c[2], r);
+                for (int d = 0; d < 3; d++) cc[d] = cloudCell(A, d, c[d]);
+            }
+            __syncthreads();
+            bool ov = false;
+            if (tid < 27) {
+                int o[3] = {tid % 3 - 1, (tid / 3) % 3 - 1, tid / 9 - 1};
+                int cell[3];
+                bool ok = true;
+                for (int d = 0; d < 3; d++) {
+                    int q = cc[d] + o[d];
+                    // periodic axes of 3+ cells wrap; with fewer cells (and aperiodic axes) each cell
+                    // in range is visited once
+                    if (A.periodic && A.nc[d] >= 3) q = (q + A.nc[d]) % A.nc[d];
+                    else if (q < 0 || q >= A.nc[d]) ok = false;
+                    cell[d] = q;
+                }
+                if (ok) {
+                    const double4 cd = cand;
+                    for (int j = head[(cell[2] * A.nc[1] + cell[1]) * A.nc[0] + cell[0]]; j >= 0 && !ov; j = next[j]) {
+                        const double4 p = sph[j];
+                        double dr[3];
+                        const double pc[3] = {p.x, p.y, p.z}, ccd[3] = {cd.x, cd.y, cd.z};
+                        for (int d = 0; d < 3; d++)
+                            dr[d] = A.periodic ? fmin(cloudWrapRel(ccd[d], pc[d], pc[d] + A.size[d]), cloudWrapRel(pc[d], ccd[d], ccd[d] + A.size[d]))
+                                               : pc[d] - ccd[d];
+                        const double rr = p.w + r;
+                        ov = rr * rr >= (dr[0] * dr[0] + dr[1] * dr[1]) + dr[2] * dr[2];
+                    }
+                }
+            }
+            if (!__syncthreads_or(ov)) {
+                if (tid == 0) {
+                    const int cell = (cc[2] * A.nc[1] + cc[1]) * A.nc[0] + cc[0];
+                    sph[i] = cand;
+                    next[i] = head[cell];
+                    head[cell] = (int32_t)i;
+                    placed = 1;
+                }
+                __syncthreads();
+                break;
+            }
+        }
+        if (t == A.maxTry) {
+            if (tid == 0) done = 1;
+            __syncthreads();
+            break;
+        }
+    }
+    if (tid == 0) {
+        out[0] = i;
+        out[1] = done;
+        *state = st;
+    }
+}
+
 // ---------------- scene generation: Jacobi overlap relaxation ----------------
 // Each sphere moves by half of every overlap with its neighbours (found through the
 // same cell grid); all displacements are computed from the old positions, so the

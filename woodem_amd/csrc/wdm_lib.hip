@@ -2467,6 +2467,149 @@ int wdm_select_box(wdm_ctx* c, int32_t sid, const double* lo3, const double* hi3
         *nOut = k;
     });
 }
+// ---------------- SpherePack::makeCloud on the device ----------------
+// The set-up of SpherePack::makeCloud (lib/sphere-pack/SpherePack.cpp:135-187): radius mode,
+// mean radius from the porosity, PSD radii with the mass-weighted cumulative curve and its
+// scaling, and the deterministic radius sequence for num > 0 (:191-209); placement runs in
+// k_make_cloud; a sphere without a free spot makes the reference retry with the porosity
+// raised by a tenth of what is left (:236-241), the generator going on where it stopped.
+int wdm_make_cloud(wdm_ctx* c, const double* mn3, const double* mx3, double rMean, double rRelFuzz, int32_t num, int32_t periodic,
+                   double porosity, const double* psdSizes, const double* psdCumm, int32_t nPsd, int32_t distributeMass,
+                   uint32_t* rngState, int64_t cap, double* c3, double* rOut, int64_t* nOut, double* appliedPsdScaling) {
+    return guard(c, [&] {
+        if (!mn3 || !mx3 || !rngState || !nOut || cap <= 0 || !c3 || !rOut || (nPsd > 0 && (!psdSizes || !psdCumm)))
+            throw WdmError(WDM_E_ARG, "makeCloud arguments");
+        if (*rngState == 0 || *rngState >= 2147483647u) throw WdmError(WDM_E_ARG, "generator state must be in 1..2^31-2");
+        if (num > 0 && num > cap) throw WdmError(WDM_E_ARG, "cap below num");
+        if (cap >= INT32_MAX) throw WdmError(WDM_E_ARG, "cap too large");
+        const double PI = 3.14159265358979323846;
+        double size[3], mn[3];
+        for (int d = 0; d < 3; d++) { mn[d] = mn3[d]; size[d] = mx3[d] - mn3[d]; }
+        const double volume = size[0] * (size[1] * size[2]);  // det of diag(size), Eigen's cofactor order
+        if (!volume) throw WdmError(WDM_E_ARG, "The box defined as null volume. Define at least maxCorner of the box, or hSize if periodic.");
+        if (porosity <= 0) porosity = .5;
+        uint32_t state = *rngState;
+        double scaling = 1.;
+        int64_t n = 0;
+        std::vector<double4> sphH;
+        for (int attempt = 0;; attempt++) {
+            if (attempt > 200) throw WdmError(WDM_E_STATE, "makeCloud: no packing after 200 porosity increases");
+            int mode = -1;  // 0 rMean, 1 num, 2 PSD
+            bool err = false;
+            double rM = attempt == 0 ? rMean : -1.;
+            if (rM > 0) mode = 0;
+            else if (num > 0 && nPsd == 0) {
+                mode = 1;
+                rM = std::pow(volume * (1 - porosity) / (PI * (4 / 3.) * (1 + rRelFuzz * rRelFuzz) * num), 1 / 3.);
+            }
+            std::vector<double> psdR, psdC2;
+            if (nPsd > 0) {
+                err = mode >= 0;
+                mode = 2;
+                if (nPsd <= 1) throw WdmError(WDM_E_ARG, "SpherePack.makeCloud: psdSizes must have at least 2 items");
+                if (psdCumm[0] != 0. && psdCumm[nPsd - 1] != 1.) throw WdmError(WDM_E_ARG, "SpherePack.makeCloud: first and last items of psdCumm *must* be exactly 0 and 1.");
+                for (int i = 0; i < nPsd; i++) {
+                    psdR.push_back(.5 * psdSizes[i]);
+                    if (distributeMass) {
+                        if (i == 0) psdC2.push_back(0);
+                        else psdC2.push_back(psdC2[i - 1] + 3.0 * volume * (1 - porosity) / PI * (psdCumm[i] - psdCumm[i - 1]) / (psdSizes[i] - psdSizes[i - 1]) *
+                                                                  (std::pow(psdSizes[i - 1], -2) - std::pow(psdSizes[i], -2)));
+                    }
+                    if (i > 0 && (psdSizes[i - 1] > psdSizes[i] || psdCumm[i - 1] > psdCumm[i]))
+                        throw WdmError(WDM_E_ARG, "SpherePack:makeCloud: psdSizes and psdCumm must be both non-decreasing.");
+                }
+                if (distributeMass) {
+                    if (num > 0 && psdC2[nPsd - 1] < num) {
+                        scaling = std::pow(psdC2[nPsd - 1] / num, 1. / 3.);
+                        for (int i = 0; i < nPsd; i++) psdR[i] *= scaling;
+                    }
+                    for (int i = 1; i < nPsd; i++) psdC2[i] /= psdC2[nPsd - 1];
+                }
+            }
+            if (err || mode < 0)
+                throw WdmError(WDM_E_ARG, "SpherePack.makeCloud: at least one of rMean, porosity, psdSizes & psdCumm arguments must be specified. rMean can't be combined with psdSizes.");
+            const double* cum = distributeMass ? psdC2.data() : psdCumm;
+            auto radius = [&](double rnd) -> double {
+                if (mode != 2) return distributeMass ? pow3Interp(rnd, rM * (1 - rRelFuzz), rM * (1 + rRelFuzz)) : rM * (2 * (rnd - .5) * rRelFuzz + 1);
+                double norm;
+                const int piece = psdGetPiece(rnd, cum, nPsd, norm);
+                return distributeMass ? pow3Interp(norm, psdR[piece], psdR[piece + 1]) : psdR[piece] + norm * (psdR[piece + 1] - psdR[piece]);
+            };
+            std::vector<double> radii;
+            double rmax = 0;
+            if (num > 0) {
+                radii.resize(num);
+                for (int i = 0; i < num; i++) { radii[i] = radius(((double)num - (double)i + 0.5) / ((double)num + 1.)); rmax = std::max(rmax, radii[i]); }
+            } else if (mode != 2) rmax = rM * (1 + std::fabs(rRelFuzz));
+            else for (double q : psdR) rmax = std::max(rmax, q);
+            if (!(rmax > 0)) throw WdmError(WDM_E_ARG, "makeCloud: radii must be positive");
+            CloudArgs A;
+            std::memset(&A, 0, sizeof(A));
+            int64_t cells = 1;
+            for (int d = 0; d < 3; d++) {
+                A.mn[d] = mn[d];
+                A.size[d] = size[d];
+                A.nc[d] = std::max(1, (int)std::floor(size[d] / (2 * rmax)));
+            }
+            for (;;) {
+                cells = (int64_t)A.nc[0] * A.nc[1] * A.nc[2];
+                if (cells <= (1 << 25)) break;
+                for (int d = 0; d < 3; d++) A.nc[d] = std::max(1, A.nc[d] / 2);
+            }
+            for (int d = 0; d < 3; d++) A.h[d] = size[d] / A.nc[d];
+            A.periodic = periodic ? 1 : 0;
+            A.num = num;
+            A.mode = mode == 2 ? 1 : 0;
+            A.distributeMass = distributeMass ? 1 : 0;
+            A.nPsd = nPsd;
+            A.rMean = rM;
+            A.rRelFuzz = rRelFuzz;
+            A.cap = num > 0 ? num : cap;
+            A.maxTry = 1000;
+            Blob b;
+            const size_t oR = b.addv(radii), oPR = b.addv(psdR);
+            std::vector<double> cumv(cum ? cum : psdR.data(), (cum ? cum : psdR.data()) + (nPsd > 0 ? nPsd : 0));
+            const size_t oPC = b.addv(cumv);
+            const size_t oS = b.add(&state, sizeof(state));
+            DBuf<uint8_t> tb, cellsB, nextB, sphB, outB;
+            ensure(c, tb, std::max<size_t>(b.b.size(), 64));
+            ensure(c, cellsB, (size_t)cells * 4);
+            ensure(c, nextB, (size_t)A.cap * 4);
+            ensure(c, sphB, (size_t)A.cap * sizeof(double4));
+            ensure(c, outB, 16);
+            hipStream_t st = c->stream;
+            HIPCHK(hipMemcpyAsync(tb.p, b.b.data(), b.b.size(), hipMemcpyHostToDevice, st));
+            A.radii = (const double*)(tb.p + oR);
+            A.psdR = (const double*)(tb.p + oPR);
+            A.psdC = (const double*)(tb.p + oPC);
+            hipLaunchKernelGGL(k_fill_i32, dim3(nblk(cells)), dim3(WDM_BLOCK), 0, st, (int32_t*)cellsB.p, cells, -1);
+            hipLaunchKernelGGL(k_make_cloud, dim3(1), dim3(256), 0, st, A, (uint32_t*)(tb.p + oS), (int32_t*)cellsB.p, (int32_t*)nextB.p,
+                               (double4*)sphB.p, (int64_t*)outB.p);
+            checkLaunch();
+            int64_t out[2];
+            HIPCHK(hipMemcpyAsync(out, outB.p, sizeof(out), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(&state, tb.p + oS, sizeof(state), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            n = out[0];
+            const bool retry = out[1] && num > 0 && mode != 0;
+            if (!retry && n > 0) {
+                sphH.resize(n);
+                HIPCHK(hipMemcpy(sphH.data(), sphB.p, n * sizeof(double4), hipMemcpyDeviceToHost));
+            }
+            release(c, tb); release(c, cellsB); release(c, nextB); release(c, sphB); release(c, outB);
+            if (!retry) break;
+            porosity = porosity + (1 - porosity) / 10.;
+        }
+        if (n > cap) throw WdmError(WDM_E_ARG, "cap below the spheres placed");
+        for (int64_t i = 0; i < n; i++) {
+            c3[3 * i] = sphH[i].x; c3[3 * i + 1] = sphH[i].y; c3[3 * i + 2] = sphH[i].z;
+            rOut[i] = sphH[i].w;
+        }
+        *nOut = n;
+        *rngState = state;
+        if (appliedPsdScaling) *appliedPsdScaling = scaling;
+    });
+}
 int wdm_set_energy(wdm_ctx* c, int32_t sid, const double* values, const int32_t* created) {
     return guard(c, [&] {
         Scene& S = getScene(c, sid);

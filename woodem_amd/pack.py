@@ -1,0 +1,94 @@
+"""SpherePack (lib/sphere-pack/SpherePack.hpp:20-120, py/pack.py) for building scenes:
+makeCloud runs on the device (wdm_make_cloud), sphere after sphere in the reference's order,
+so its result is the reference algorithm's on the same generator state.
+
+The reference's generator is a function-level static (SpherePack.cpp:136-137): the first
+makeCloud call of a process seeds it (seed, or the clock when seed is 0) and later calls go
+on from where it stopped, whatever seed they pass.  This module keeps that state.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import soa
+
+_M = 2147483647
+_state = None
+
+
+def _generator(seed):
+    global _state
+    if _state is None:
+        s = (int(seed) if seed != 0 else int(time.time())) % _M
+        _state = s if s != 0 else 1
+    return np.array([_state], np.uint32)
+
+
+def reset_generator():
+    """forget the generator state (the next makeCloud seeds it again)"""
+    global _state
+    _state = None
+
+
+class SpherePack:
+    """A set of spheres (centres `c`, radii `r`); `cellSize` is nonzero for periodic packs."""
+
+    def __init__(self):
+        self.c = np.zeros((0, 3))
+        self.r = np.zeros(0)
+        self.cellSize = np.zeros(3)
+        self.appliedPsdScaling = 1.
+
+    def __len__(self):
+        return len(self.r)
+
+    def __iter__(self):
+        return iter(zip(map(tuple, self.c), self.r))
+
+    def makeCloud(self, minCorner=(0., 0., 0.), maxCorner=(0., 0., 0.), rMean=-1, rRelFuzz=0, num=-1, periodic=False, porosity=.5,
+                  psdSizes=(), psdCumm=(), distributeMass=False, seed=0, hSize=None, ctx=None, cap=None):
+        """SpherePack::makeCloud (SpherePack.cpp:135-248, defaults of py/_packSpheres.cpp:29);
+        returns the number of spheres.  With num < 0 the packing stops at `cap` spheres (by
+        default the count that would fill the box densely) or at the first sphere without a
+        free spot.  Sheared cells (hSize) are not supported."""
+        if hSize is not None and np.any(np.asarray(hSize) != 0):
+            raise NotImplementedError("makeCloud with hSize (sheared cells) is not supported")
+        if len(self):
+            raise NotImplementedError("makeCloud into a non-empty SpherePack is not supported")
+        from ._native import Context
+        global _state
+        ctx = ctx or Context(0)
+        mn, mx = np.asarray(minCorner, np.float64), np.asarray(maxCorner, np.float64)
+        if cap is None:
+            if num > 0:
+                cap = num
+            else:
+                if len(psdSizes):
+                    rmin = .5 * min(x for x in psdSizes if x > 0)
+                else:
+                    rm = rMean if rMean > 0 else 1.
+                    rmin = rm * max(1e-3, 1 - abs(rRelFuzz))
+                cap = int(min(2 ** 30, float(np.prod(mx - mn)) * .74 / (4 / 3. * math.pi * rmin ** 3) + 16))
+        st = _generator(seed)
+        c, r, sc = ctx.make_cloud(mn, mx, rMean, rRelFuzz, num, periodic, porosity, psdSizes, psdCumm, distributeMass, st, int(cap))
+        _state = int(st[0])
+        self.c, self.r, self.appliedPsdScaling = c, r, sc
+        self.cellSize = (mx - mn) if periodic else np.zeros(3)
+        return len(r)
+
+    def toArrays(self, density=1e3, young=1e7, ktDivKn=.2, tanPhi=.5, mask=soa.DEFAULT_MOVABLE_MASK):
+        """the spheres as scene arrays (free spheres, one FrictMat)"""
+        return soa.spheres_arrays(self.c, self.r, density=density, young=young, ktDivKn=ktDivKn, tanPhi=tanPhi, mask=mask)
+
+    def toSimulation(self, S, mat=None, mask=None):
+        """add the spheres to S.dem (SpherePack.toSimulation, py/pack.py); returns their ids"""
+        from . import utils
+        from .dem import DemField
+        kw = dict(mask=DemField.defaultMovableMask if mask is None else mask)
+        if mat is not None:
+            kw["mat"] = mat
+        if np.any(self.cellSize != 0):
+            S.periodic = True
+            S.cell.setBox(self.cellSize)
+        return S.dem.par.add([utils.sphere(tuple(c), float(r), **kw) for c, r in zip(self.c, self.r)])
