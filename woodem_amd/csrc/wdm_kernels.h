@@ -2183,88 +2183,242 @@ __device__ __forceinline__ int cloudCell(const CloudArgs& A, int d, double x) {
     int c = (int)floor((x - A.mn[d]) / A.h[d]);
     return c < 0 ? 0 : (c >= A.nc[d] ? A.nc[d] - 1 : c);
 }
+// Placement (SpherePack.cpp:211-246) evaluated speculatively, one window of tries at a time.
+// The reference tries candidates one after the other: try t of sphere i takes the next three
+// draws (for num < 0 a sphere first takes one draw for its radius), and is accepted when it
+// overlaps no sphere placed before it.  Which sphere a try belongs to depends on how many
+// earlier tries were accepted.  A window fixes the draws of its next tries by jumping the
+// generator ahead (x_k = x_0 48271^k mod 2^31-1) and evaluates, in parallel, each try under a
+// few hypotheses on that count:
+//   num > 0: try j of the window with a = lo_j .. lo_j+2 acceptances before it (sphere i+a,
+//            radius radii[i+a], draws 3j..3j+2), lo_j following the last window's rate;
+//   num < 0: try j with none accepted before it (sphere i, draws s0+3j..) and with all
+//            accepted before it (sphere i+j, radius draw s0+4j-1, draws s0+4j..).
+// Each candidate is tested against the cell grid of the spheres placed before the window, by 27
+// threads (one per neighbour cell).  One wave then walks the tries in order, exactly as the
+// reference does, taking the candidate of the hypothesis that holds and testing it against the
+// spheres accepted earlier in the window; the window ends where no evaluated hypothesis holds.
+// Results (spheres, their order, the generator state, the failing sphere) are those of the
+// sequential loop; only the number of rounds changes.
+#define CLOUD_H 3                           // hypotheses per try (num > 0)
+#define CLOUD_W 12                          // tries per window (num > 0)
+#define CLOUD_W2 18                         // tries per family (num < 0)
+#define CLOUD_SLOTS 36                      // candidates per window: CLOUD_W x CLOUD_H = 2 x CLOUD_W2
+#define CLOUD_Q 80                          // draws generated per window (num < 0 uses at most 1 + 4 x 18 + 3)
+#define CLOUD_THREADS (CLOUD_SLOTS * 27)    // 972 threads test cells; the block has 1024
+__device__ __forceinline__ bool cloudOverlap(const CloudArgs& A, const double4 p, const double4 c) {
+    // p placed before c (SpherePack.cpp:222-229): periodic distance is the shorter way round
+    double dr[3];
+    const double pc[3] = {p.x, p.y, p.z}, cc[3] = {c.x, c.y, c.z};
+    for (int d = 0; d < 3; d++)
+        dr[d] = A.periodic ? fmin(cloudWrapRel(cc[d], pc[d], pc[d] + A.size[d]), cloudWrapRel(pc[d], cc[d], cc[d] + A.size[d])) : pc[d] - cc[d];
+    const double rr = p.w + c.w;
+    return rr * rr >= (dr[0] * dr[0] + dr[1] * dr[1]) + dr[2] * dr[2];
+}
+__device__ __forceinline__ double cloudU(uint32_t x) { return (double)(x - 1u) / 2147483645.; }
+__device__ __forceinline__ int cloudLo(double p, int j) {
+    const int cj = (int)floor(p * j + .5);
+    const int lo = cj - 1 < j - (CLOUD_H - 1) ? cj - 1 : j - (CLOUD_H - 1);
+    return lo < 0 ? 0 : lo;
+}
 // out[0]: spheres placed; out[1]: 1 when a sphere found no free spot in maxTry tries
-__global__ void __launch_bounds__(256) k_make_cloud(CloudArgs A, uint32_t* state, int32_t* head, int32_t* next, double4* sph, int64_t* out) {
-    __shared__ double4 cand;
-    __shared__ int cc[3];
-    __shared__ int done, placed;
+__global__ void __launch_bounds__(1024) k_make_cloud(CloudArgs A, uint32_t* state, int32_t* head, int32_t* next, double4* sph, int64_t* out) {
+    __shared__ uint32_t pw[CLOUD_Q];   // 48271^(k+1) mod 2^31-1
+    __shared__ uint32_t xs[CLOUD_Q];   // generator state after good draw k of the window
+    __shared__ double4 cnd[CLOUD_SLOTS];
+    __shared__ int on[CLOUD_SLOTS], hit[CLOUD_SLOTS];
     __shared__ uint32_t st;
+    __shared__ int64_t placed;
+    __shared__ int tries, needR, fin, failed;
+    __shared__ double rCur, rate;
     const int tid = threadIdx.x;
-    if (tid == 0) { st = *state; done = 0; }
-    __syncthreads();
-    int64_t i = 0;
-    for (;; i++) {
-        if ((A.num > 0 && i >= A.num) || i >= A.cap) break;
-        if (tid == 0) {
-            uint32_t x = st;
-            cand.w = A.num > 0 ? A.radii[i] : cloudRadius(A, mstdUniform(x));
-            st = x;
-            placed = 0;
+    const uint32_t M = 2147483647u;
+    if (tid < CLOUD_Q) {
+        uint64_t b = 48271u, r = 1;
+        for (int e = tid + 1; e; e >>= 1) {
+            if (e & 1) r = r * b % M;
+            b = b * b % M;
         }
-        __syncthreads();
-        const double r = cand.w;
-        int t = 0;
-        for (; t < A.maxTry; t++) {
+        pw[tid] = (uint32_t)r;
+    }
+    if (tid == 0) {
+        st = *state;
+        placed = 0;
+        tries = 0;
+        needR = 1;
+        fin = 0;
+        failed = 0;
+        rCur = 0.;
+        rate = 1.;
+    }
+    const int64_t lim = A.cap;  // num when num > 0
+    __syncthreads();
+    for (;;) {
+        if (fin) break;
+        // draws of the window; a state of 2^31-2 maps to 1 and is drawn again (rare: sequential)
+        const uint32_t x0 = st;
+        bool bad = false;
+        if (tid < CLOUD_Q) {
+            const uint32_t x = (uint32_t)((uint64_t)x0 * pw[tid] % M);
+            xs[tid] = x;
+            bad = x == M - 1;
+        }
+        if (__syncthreads_or(bad)) {
             if (tid == 0) {
-                uint32_t x = st;
-                double c[3];
-                for (int d = 0; d < 3; d++) {
-                    const double u = mstdUniform(x);
-                    c[d] = A.periodic ? A.mn[d] + A.size[d] * u : (A.mn[d] + r) + (A.size[d] - 2 * r) * u;
+                uint32_t y = x0;
+                for (int k = 0; k < CLOUD_Q; k++) {
+                    do y = mstdNext(y); while (y == M - 1);
+                    xs[k] = y;
                 }
-                st = x;
-                cand = make_double4(c[0], c[1], c[2], r);
-                for (int d = 0; d < 3; d++) cc[d] = cloudCell(A, d, c[d]);
             }
             __syncthreads();
-            bool ov = false;
-            if (tid < 27) {
-                int o[3] = {tid % 3 - 1, (tid / 3) % 3 - 1, tid / 9 - 1};
-                int cell[3];
-                bool ok = true;
+        }
+        const int64_t i0 = placed;
+        const int s0 = A.num > 0 ? 0 : needR;
+        const double p = rate;
+        // candidates
+        if (tid < CLOUD_SLOTS) {
+            const int s = tid;
+            int ok = 0, off = 0;
+            double r = 0.;
+            if (A.num > 0) {
+                const int j = s / CLOUD_H, a = cloudLo(p, j) + s % CLOUD_H;
+                ok = a <= j && i0 + a < lim;
+                if (ok) { r = A.radii[i0 + a]; off = 3 * j; }
+            } else if (s < CLOUD_W2) {  // none accepted before try s
+                ok = i0 < lim;
+                r = needR ? cloudRadius(A, cloudU(xs[0])) : rCur;
+                off = s0 + 3 * s;
+            } else {  // all accepted before try j
+                const int j = s - CLOUD_W2 + 1;
+                ok = i0 + j < lim;
+                off = s0 + 4 * j;
+                if (ok) r = cloudRadius(A, cloudU(xs[off - 1]));
+            }
+            if (ok) {
+                double c[3];
                 for (int d = 0; d < 3; d++) {
-                    int q = cc[d] + o[d];
+                    const double u = cloudU(xs[off + d]);
+                    c[d] = A.periodic ? A.mn[d] + A.size[d] * u : (A.mn[d] + r) + (A.size[d] - 2 * r) * u;
+                }
+                cnd[s] = make_double4(c[0], c[1], c[2], r);
+            }
+            on[s] = ok;
+            hit[s] = 0;
+        }
+        __syncthreads();
+        // grid test: thread (slot, neighbour cell)
+        if (tid < CLOUD_THREADS) {
+            const int s = tid / 27, o = tid % 27;
+            if (on[s]) {
+                const double4 cd = cnd[s];
+                const double ccd[3] = {cd.x, cd.y, cd.z};
+                const int oo[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
+                int cell[3];
+                bool inr = true;
+                for (int d = 0; d < 3; d++) {
+                    int q = cloudCell(A, d, ccd[d]) + oo[d];
                     // periodic axes of 3+ cells wrap; with fewer cells (and aperiodic axes) each cell
                     // in range is visited once
                     if (A.periodic && A.nc[d] >= 3) q = (q + A.nc[d]) % A.nc[d];
-                    else if (q < 0 || q >= A.nc[d]) ok = false;
+                    else if (q < 0 || q >= A.nc[d]) inr = false;
                     cell[d] = q;
                 }
-                if (ok) {
-                    const double4 cd = cand;
-                    for (int j = head[(cell[2] * A.nc[1] + cell[1]) * A.nc[0] + cell[0]]; j >= 0 && !ov; j = next[j]) {
-                        const double4 p = sph[j];
-                        double dr[3];
-                        const double pc[3] = {p.x, p.y, p.z}, ccd[3] = {cd.x, cd.y, cd.z};
-                        for (int d = 0; d < 3; d++)
-                            dr[d] = A.periodic ? fmin(cloudWrapRel(ccd[d], pc[d], pc[d] + A.size[d]), cloudWrapRel(pc[d], ccd[d], ccd[d] + A.size[d]))
-                                               : pc[d] - ccd[d];
-                        const double rr = p.w + r;
-                        ov = rr * rr >= (dr[0] * dr[0] + dr[1] * dr[1]) + dr[2] * dr[2];
+                if (inr) {
+                    bool ov = false;
+                    for (int j = head[(cell[2] * A.nc[1] + cell[1]) * A.nc[0] + cell[0]]; j >= 0 && !ov; j = next[j]) ov = cloudOverlap(A, sph[j], cd);
+                    if (ov) hit[s] = 1;
+                }
+            }
+        }
+        __syncthreads();
+        // walk the tries in order (wave 0; every value below is uniform across the wave except `mine`)
+        if (tid < 64) {
+            const int lane = tid;
+            double4 mine = make_double4(0., 0., 0., 0.);  // accepted candidate number `lane` of the window
+            int na = 0, t = tries, used = 0, nr = needR, fl = 0, fn = 0, triesUsed = 0;
+            double rc = rCur;
+            if (A.num > 0) {
+                for (int j = 0; j < CLOUD_W; j++) {
+                    if (i0 + na >= lim) { fn = 1; break; }
+                    const int lo = cloudLo(p, j);
+                    if (na < lo || na >= lo + CLOUD_H) break;
+                    const int s = j * CLOUD_H + (na - lo);
+                    const double4 c = cnd[s];
+                    const bool ov = hit[s] || __any(lane < na && cloudOverlap(A, mine, c));
+                    used = 3 * (j + 1);
+                    triesUsed = j + 1;
+                    if (!ov) {
+                        if (lane == na) mine = c;
+                        na++;
+                        t = 0;
+                    } else if (++t == A.maxTry) {
+                        fl = 1;
+                        fn = 1;
+                        break;
+                    }
+                }
+                if (!fn && i0 + na >= lim) fn = 1;
+            } else {
+                int fam = -1;  // -1 undecided (try 0), 0 none accepted, 1 all accepted
+                for (int j = 0;; j++) {
+                    if (i0 + na >= lim) { fn = 1; break; }
+                    if (fam <= 0 && j >= CLOUD_W2) break;
+                    if (fam == 1 && j > CLOUD_W2) break;
+                    const int s = fam == 1 ? CLOUD_W2 - 1 + j : j;
+                    const double4 c = cnd[s];
+                    const bool ov = hit[s] || __any(lane < na && cloudOverlap(A, mine, c));
+                    triesUsed = j + 1;
+                    if (!ov) {
+                        if (lane == na) mine = c;
+                        na++;
+                        t = 0;
+                        nr = 1;
+                        if (fam == 1) used = s0 + 4 * j + 3;
+                        else used = s0 + 3 * (j + 1);
+                        if (fam == 0) break;  // the next sphere's radius draw follows a run of misses
+                        fam = 1;
+                    } else {
+                        if (fam == 1) {  // first try of sphere i0+j missed
+                            used = s0 + 4 * j + 3;
+                            nr = 0;
+                            rc = c.w;
+                            if (++t == A.maxTry) { fl = 1; fn = 1; }
+                            break;
+                        }
+                        fam = 0;
+                        used = s0 + 3 * (j + 1);
+                        nr = 0;
+                        rc = c.w;
+                        if (++t == A.maxTry) { fl = 1; fn = 1; break; }
                     }
                 }
             }
-            if (!__syncthreads_or(ov)) {
-                if (tid == 0) {
-                    const int cell = (cc[2] * A.nc[1] + cc[1]) * A.nc[0] + cc[0];
-                    sph[i] = cand;
-                    next[i] = head[cell];
-                    head[cell] = (int32_t)i;
-                    placed = 1;
-                }
-                __syncthreads();
-                break;
+            // spheres accepted in the window go into the grid
+            if (lane < na) {
+                const int64_t idx = i0 + lane;
+                sph[idx] = mine;
+                const int cell = (cloudCell(A, 2, mine.z) * A.nc[1] + cloudCell(A, 1, mine.y)) * A.nc[0] + cloudCell(A, 0, mine.x);
+                next[idx] = atomicExch(&head[cell], (int32_t)idx);
+            }
+            if (lane == 0) {
+                if (used > 0) st = xs[used - 1];
+                placed = i0 + na;
+                tries = t;
+                needR = nr;
+                rCur = rc;
+                fin = fn;
+                failed = fl;
+                if (triesUsed > 0) rate = (double)na / triesUsed;
             }
         }
-        if (t == A.maxTry) {
-            if (tid == 0) done = 1;
-            __syncthreads();
-            break;
-        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     if (tid == 0) {
-        out[0] = i;
-        out[1] = done;
+        out[0] = placed;
+        out[1] = failed;
         *state = st;
     }
 }

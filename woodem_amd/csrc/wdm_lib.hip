@@ -2583,7 +2583,7 @@ int wdm_make_cloud(wdm_ctx* c, const double* mn3, const double* mx3, double rMea
             A.psdR = (const double*)(tb.p + oPR);
             A.psdC = (const double*)(tb.p + oPC);
             hipLaunchKernelGGL(k_fill_i32, dim3(nblk(cells)), dim3(WDM_BLOCK), 0, st, (int32_t*)cellsB.p, cells, -1);
-            hipLaunchKernelGGL(k_make_cloud, dim3(1), dim3(256), 0, st, A, (uint32_t*)(tb.p + oS), (int32_t*)cellsB.p, (int32_t*)nextB.p,
+            hipLaunchKernelGGL(k_make_cloud, dim3(1), dim3(1024), 0, st, A, (uint32_t*)(tb.p + oS), (int32_t*)cellsB.p, (int32_t*)nextB.p,
                                (double4*)sphB.p, (int64_t*)outB.p);
             checkLaunch();
             int64_t out[2];
