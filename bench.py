@@ -71,6 +71,38 @@ def cpu_baseline(ctx, n_sample, seconds):
                     n_sample, k, el))
 
 
+def cloud_args(num):
+    """SpherePack.makeCloud workload: num spheres, rRelFuzz .3, porosity .7 (mean radius from
+    the porosity), in a cube of 0.27 m per 100,000 spheres"""
+    L = 0.27 * (num / 1e5) ** (1. / 3.)
+    return np.zeros(3), np.full(3, L), dict(rRelFuzz=.3, num=num, porosity=.7)
+
+
+def cpu_make_cloud(n_sample):
+    """the oracle's makeCloud (all placed spheres tested per try, as SpherePack.cpp:222-231)"""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as OL
+    mn, mx, kw = cloud_args(n_sample)
+    t = time.perf_counter()
+    _, r, _, _ = OL.make_cloud(mn, mx, state=7, cap=n_sample, **kw)
+    el = time.perf_counter() - t
+    return dict(value=len(r) / el, unit="spheres/s", cores=1, kind="port",
+                sample="makeCloud of %d spheres (same density; the cost per sphere grows with the count), %.2f s" % (len(r), el))
+
+
+def gpu_make_cloud(ctx, num):
+    mn, mx, kw = cloud_args(num)
+    st = np.array([7], np.uint32)
+    ctx.make_cloud(mn, mx, -1., kw["rRelFuzz"], 2000, False, kw["porosity"], [], [], False, st, num)  # warm-up
+    st = np.array([7], np.uint32)
+    ctx.sync()
+    t = time.perf_counter()
+    _, r, _ = ctx.make_cloud(mn, mx, -1., kw["rRelFuzz"], num, False, kw["porosity"], [], [], False, st, num)
+    el = time.perf_counter() - t
+    return dict(value=len(r) / el, unit="spheres/s", seconds=el, spheres=len(r),
+                workload="SpherePack.makeCloud, num=%d, rRelFuzz .3, porosity .7, %.3f m cube, generator state 7" % (num, mx[0]))
+
+
 def measured_traffic(args):
     """HBM bytes per k_contacts launch (read + write) of this workload, from the rocprofv3
     FETCH_SIZE/WRITE_SIZE runs recorded in profiles/ (scaled by a device copy of known
@@ -271,6 +303,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cloud-n", type=int, default=100_000, help="spheres of the timed device makeCloud (0: skip)")
+    ap.add_argument("--cloud-cpu-sample", type=int, default=20_000, help="spheres of the oracle makeCloud sample")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
     ap.add_argument("--workload", choices=["periodic", "batch", "serve"], default="periodic")
     ap.add_argument("--requests", type=int, default=100, help="serve workload: number of requests")
@@ -369,6 +403,8 @@ def main():
     ctx.enable_timing(False)
     kernels["dyndt"] = {"avg_ms": kd[0] / max(kd[1], 1), "launches": kd[1], "crit_dt": crit}
     log("kernels: %s" % json.dumps(kernels))
+    gen = gpu_make_cloud(ctx, args.cloud_n) if args.cloud_n > 0 else None
+    log("scene generation: %s" % json.dumps(gen))
     log("contacts: %d potential, %d real, %d processed by the ContactLoop" % (n_con, n_real, n_active))
 
     mixed = None
@@ -399,6 +435,8 @@ def main():
     if not args.no_cpu:
         try:
             cpu = cpu_baseline(ctx, args.cpu_sample, args.cpu_seconds)
+            if gen is not None:
+                gen["cpu_baseline"] = cpu_make_cloud(args.cloud_cpu_sample)
             log("cpu baseline: %s" % json.dumps(cpu))
         except Exception as e:  # reported, not fatal for the GPU figure
             log("cpu baseline failed: %r" % (e,))
@@ -425,6 +463,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
                      "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_ms},
         "kernels": kernels,
+        "scene_generation": gen,
         "mixed_precision": mixed,
         "cpu_baseline": cpu,
     }

@@ -79,6 +79,11 @@ struct DBuf {
 struct Timing {
     bool on = false;
     std::map<std::string, std::pair<double, int64_t>> acc;
+    // event pairs (start, end) of launches not read yet, and their kernel names: events are read
+    // when the times are asked for, so no timed launch waits for the queue to drain and a start
+    // event completes when the GPU reaches the kernel, not while it idles for the launch
+    std::vector<hipEvent_t> ev;
+    std::vector<const char*> pend;
 };
 
 struct Shared;
@@ -385,17 +390,36 @@ struct Scene {
     }
 };
 
-struct KTimer {  // HIP-event timing of one launch on the context stream
-    Ctx* c; const char* name; bool on;
-    KTimer(Ctx* c_, const char* n) : c(c_), name(n), on(c_->timing.on) { if (on) (void)hipEventRecord(c->ev0, c->stream); }
+void timingFlush(Ctx* c) {  // reads the pending event pairs into the per-kernel totals
+    Timing& T = c->timing;
+    for (size_t k = 0; k < T.pend.size(); k++) {
+        (void)hipEventSynchronize(T.ev[2 * k + 1]);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, T.ev[2 * k], T.ev[2 * k + 1]);
+        auto& a = T.acc[T.pend[k]];
+        a.first += ms;
+        a.second += 1;
+    }
+    T.pend.clear();
+}
+struct KTimer {  // HIP-event timing of one launch (or phase) on the context stream
+    Ctx* c; const char* name; bool on; size_t k = 0;
+    KTimer(Ctx* c_, const char* n) : c(c_), name(n), on(c_->timing.on) {
+        if (!on) return;
+        Timing& T = c->timing;
+        if (T.pend.size() >= 4096) timingFlush(c);
+        k = T.pend.size();
+        while (T.ev.size() < 2 * k + 2) {
+            hipEvent_t e = nullptr;
+            HIPCHK(hipEventCreate(&e));
+            T.ev.push_back(e);
+        }
+        (void)hipEventRecord(T.ev[2 * k], c->stream);
+    }
     ~KTimer() {
         if (!on) return;
-        (void)hipEventRecord(c->ev1, c->stream);
-        (void)hipEventSynchronize(c->ev1);
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-        auto& a = c->timing.acc[name];
-        a.first += ms; a.second += 1;
+        (void)hipEventRecord(c->timing.ev[2 * k + 1], c->stream);
+        c->timing.pend.push_back(name);
     }
 };
 
@@ -1836,6 +1860,7 @@ void wdm_destroy(wdm_ctx* c) {
     delete c->sh;
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (hipEvent_t e : c->timing.ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -2202,7 +2227,7 @@ int wdm_step(wdm_ctx* c, const int32_t* scenes, int32_t ns, int32_t nsteps) {
             checkReady(*S);
             ss.push_back(S);
         }
-        if (c->timing.on) c->timing.acc.clear();
+        if (c->timing.on) { c->timing.pend.clear(); c->timing.acc.clear(); }
         if (ss.empty()) return;
         std::vector<int64_t> left(ss.size(), nsteps);
         for (;;) {
@@ -2688,10 +2713,17 @@ int wdm_get_tuning(wdm_ctx* c, int32_t* window, int32_t* graphs, double* cellFac
         if (launches) *launches = c->sh->graphLaunches;
     });
 }
-int wdm_enable_timing(wdm_ctx* c, int32_t on) { return guard(c, [&] { c->timing.on = on != 0; c->timing.acc.clear(); }); }
+int wdm_enable_timing(wdm_ctx* c, int32_t on) {
+    return guard(c, [&] {
+        timingFlush(c);
+        c->timing.on = on != 0;
+        c->timing.acc.clear();
+    });
+}
 
 int wdm_kernel_times(wdm_ctx* c, int32_t cap, char* names, double* ms, int64_t* launches, int32_t* nOut) {
     return guard(c, [&] {
+        timingFlush(c);
         int k = 0;
         for (auto& kv : c->timing.acc) {
             if (k >= cap) break;
