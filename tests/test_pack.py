@@ -4,7 +4,8 @@ The oracle restates the reference's sequential algorithm (every candidate tested
 every placed sphere); it is pinned by closed forms and invariants: the radius sequence for
 num > 0 ((num - i + 1/2)/(num + 1) through the mode's rule, decreasing), the mean radius
 from the porosity, no two spheres overlapping (periodic: minimum image), every centre inside
-its box (aperiodic: at least r from the walls), the generator's sequence (Park-Miller
+its box (aperiodic: at least r from the walls), r^-2 affine in the quantile with distributeMass,
+the solid volume V (1-p) that a mass-weighted PSD with its scaling fills, the generator's sequence (Park-Miller
 x <- 48271 x mod 2^31-1: after 10000 draws from 1 the state is 399268537, the known value of
 the minimal standard's 10000th output from seed 1), a PSD's radii within its range and the
 retry at a higher porosity when the box cannot hold num spheres.  The device packing is
@@ -73,6 +74,33 @@ def test_periodic_minimum_image_and_psd():
     assert np.all(r >= .002 - 1e-15) and np.all(r <= .005 + 1e-15)
     assert min_gap(c, r, L) > 0
     assert np.all(c >= 0) and np.all(c < L + 1e-15)
+
+
+def test_distribute_mass_pow3():
+    # distributeMass: the number density goes as r^-3 on [a, b], so r^-2 is affine in the quantile
+    # (pow3Interp, SpherePack.hpp:86): r = a at quantile 0, b at 1
+    num, rMean, f = 400, .003, .4
+    c, r, st, _ = OL.make_cloud((0, 0, 0), (.1, .1, .1), rMean=rMean, rRelFuzz=f, num=num, distributeMass=True, state=2)
+    a, b = rMean * (1 - f), rMean * (1 + f)
+    q = (num - np.arange(num) + .5) / (num + 1.)
+    assert np.allclose(r ** -2, q * (b ** -2 - a ** -2) + a ** -2, rtol=1e-13, atol=0)
+    assert min_gap(c, r) > 0
+
+
+def test_psd_mass_scaling_fills_the_solid_volume():
+    # a PSD linear in diameter between d0 and d1 holds N = 3 V (1-p) / (pi (d1-d0)) (d0^-2 - d1^-2)
+    # spheres (the integral of the mass fraction over the sphere volume pi d^3 / 6); asking for more
+    # scales the radii by (N/num)^(1/3), and the num spheres then fill V (1-p) of solid
+    V, p, d0, d1 = .06 ** 3, .8, .002, .006
+    N = 3 * V * (1 - p) / (math.pi * (d1 - d0)) * (d0 ** -2 - d1 ** -2)
+    num = int(N * 1.5)
+    c, r, st, sc = OL.make_cloud((0, 0, 0), (.06, .06, .06), psdSizes=[d0, d1], psdCumm=[0, 1], num=num, porosity=p,
+                                 distributeMass=True, state=4)
+    assert len(r) == num
+    assert sc == pytest.approx((N / num) ** (1 / 3.), rel=1e-13)
+    assert np.all(r >= sc * d0 / 2 * (1 - 1e-15)) and np.all(r <= sc * d1 / 2 * (1 + 1e-15))
+    assert (4 / 3. * math.pi * r ** 3).sum() == pytest.approx(V * (1 - p), rel=1e-3)
+    assert min_gap(c, r) > 0
 
 
 def test_retry_with_higher_porosity():
