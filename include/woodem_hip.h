@@ -266,6 +266,17 @@ int wdm_make_cloud(wdm_ctx* ctx, const double* mn3, const double* mx3, double rM
                    int32_t distributeMass, uint32_t* rng_state, int64_t cap, double* c3, double* r, int64_t* n_out,
                    double* appliedPsdScaling);
 
+/* pack.regularOrtho / pack.regularHexa (py/pack.py:323-350) on the context's device: the
+   lattice over the predicate's aabb (ortho: numpy.arange(mn+radius, mx-radius, 2 radius+gap)
+   per axis; hexa: spacing a = 2 radius + gap, rows a sqrt(3)/2, layers a sqrt(6)/3, alternate
+   rows and layers shifted), the points where the predicate holds with pad = radius, in the
+   order of itertools.product (z fastest).  kind 0 inAlignedBox (params mn3, mx3), 1 inSphere
+   (centre3, r), 2 inCylinder (c1_3, c2_3, r) (py/_packPredicates.cpp:96-205).  c3 receives
+   n_out centres (cap of them; a larger lattice sets n_out and returns WDM_E_ARG); with c3
+   NULL only n_out is set. */
+int wdm_regular_pack(wdm_ctx* ctx, int32_t hexa, int32_t kind, const double* params, double radius, double gap, int64_t cap,
+                     double* c3, int64_t* n_out);
+
 /* EnergyTracker values and whether each slot exists */
 int wdm_get_energy(wdm_ctx* ctx, int32_t scene, double* values, int32_t* created);
 /* Scene.step, InsertionSortCollider.nFullRuns, resolved verletDist */

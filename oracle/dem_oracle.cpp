@@ -1304,6 +1304,82 @@ long cloudMake(std::vector<Sph>& pack, CloudGen& g, V3 mn, V3 mx, double rMean, 
     return (long)pack.size();
 }
 
+// ---- regular packings (py/pack.py:323-350) and their predicates (py/_packPredicates.cpp) ----
+// numpy.arange(start, stop, step) for doubles: length ceil((stop-start)/step); a[0] = start,
+// a[1] = start + step, a[i] = start + i*(a[1] - a[0]) (numpy's DOUBLE_fill)
+std::vector<double> npArange(double start, double stop, double step) {
+    std::vector<double> a;
+    const double len = std::ceil((stop - start) / step);
+    if (!(len > 0)) return a;
+    const long n = (long)len;
+    a.resize(n);
+    a[0] = start;
+    if (n > 1) a[1] = start + step;
+    for (long i = 2; i < n; i++) a[i] = a[0] + (double)i * (a[1] - a[0]);
+    return a;
+}
+struct LatPred {
+    int kind;
+    double p[7];
+    bool operator()(V3 pt, double pad) const {
+        if (kind == 0)  // inAlignedBox (:136-148)
+            return p[0] + pad <= pt[0] && p[3] - pad >= pt[0] && p[1] + pad <= pt[1] && p[4] - pad >= pt[1] && p[2] + pad <= pt[2] &&
+                   p[5] - pad >= pt[2];
+        if (kind == 1) {  // inSphere (:96-104)
+            const V3 d = vsub(pt, v3(p[0], p[1], p[2]));
+            return std::sqrt(vsqn(d)) - pad <= p[3] - pad;
+        }
+        // inCylinder (:189-205)
+        const V3 c1 = v3(p[0], p[1], p[2]), c2 = v3(p[3], p[4], p[5]), c12 = vsub(c2, c1);
+        const double ht = std::sqrt(vsqn(c12));
+        const double u = (vdot(pt, c12) - vdot(c1, c12)) / (ht * ht);
+        if ((u * ht < 0 + pad) || (u * ht > ht - pad)) return false;
+        const double axisDist = std::sqrt(vsqn(vcross(vsub(pt, c1), vsub(pt, c2)))) / ht;
+        if (axisDist > p[6] - pad) return false;
+        return true;
+    }
+    void aabb(V3& mn, V3& mx) const {
+        if (kind == 0) { mn = v3(p[0], p[1], p[2]); mx = v3(p[3], p[4], p[5]); return; }
+        if (kind == 1) { mn = v3(p[0] - p[3], p[1] - p[3], p[2] - p[3]); mx = v3(p[0] + p[3], p[1] + p[3], p[2] + p[3]); return; }
+        const V3 A = v3(p[0], p[1], p[2]), B = v3(p[3], p[4], p[5]);
+        const double ht = std::sqrt(vsqn(vsub(B, A)));
+        auto sq = [](double x) { return x * x; };
+        const V3 k = v3(std::sqrt(sq(A[1] - B[1]) + sq(A[2] - B[2])) / ht, std::sqrt(sq(A[0] - B[0]) + sq(A[2] - B[2])) / ht,
+                        std::sqrt(sq(A[0] - B[0]) + sq(A[1] - B[1])) / ht);
+        for (int d = 0; d < 3; d++) { mn[d] = std::min(A[d], B[d]) - p[6] * k[d]; mx[d] = std::max(A[d], B[d]) + p[6] * k[d]; }
+    }
+};
+std::vector<V3> regularOrtho(const LatPred& pred, double radius, double gap) {  // py/pack.py:323-332
+    V3 mn, mx;
+    pred.aabb(mn, mx);
+    std::vector<double> ax[3];
+    for (int i = 0; i < 3; i++) ax[i] = npArange(mn[i] + radius, mx[i] - radius, 2 * radius + gap);
+    std::vector<V3> ret;
+    for (double x : ax[0])
+        for (double y : ax[1])
+            for (double z : ax[2])
+                if (pred(v3(x, y, z), radius)) ret.push_back(v3(x, y, z));
+    return ret;
+}
+std::vector<V3> regularHexa(const LatPred& pred, double radius, double gap) {  // py/pack.py:334-350
+    const double a = 2 * radius + gap;
+    const double hy = a * std::sqrt(3.) / 2., hz = a * std::sqrt(6.) / 3.;
+    V3 mn, mx;
+    pred.aabb(mn, mx);
+    const V3 dim = vsub(mx, mn);
+    const long ni = (long)(dim[0] / a) + 1, nj = (long)(dim[1] / hy) + 1, nk = (long)(dim[2] / hz) + 1;
+    std::vector<V3> ret;
+    for (long i = 0; i < ni; i++)
+        for (long j = 0; j < nj; j++)
+            for (long k = 0; k < nk; k++) {
+                double x = mn[0] + radius + (double)i * a, y = mn[1] + radius + (double)j * hy, z = mn[2] + radius + (double)k * hz;
+                if (j % 2 == 0) x += (k % 2 == 0) ? a / 2. : -a / 2.;
+                if (k % 2 != 0) { x += a / 2.; y += hy / 2.; }
+                if (pred(v3(x, y, z), radius)) ret.push_back(v3(x, y, z));
+            }
+    return ret;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1548,6 +1624,17 @@ int orc_make_cloud(const double* mn3, const double* mx3, double rMean, double rR
         *state = g.x;
         if (appliedPsdScaling) *appliedPsdScaling = applied;
     } catch (std::exception&) { return -1; }
+    return 0;
+}
+int orc_regular_pack(int32_t hexa, int32_t kind, const double* params, double radius, double gap, int64_t cap, double* c3, int64_t* nOut) {
+    LatPred pr;
+    pr.kind = kind;
+    for (int q = 0; q < 7; q++) pr.p[q] = q < (kind == 0 ? 6 : (kind == 1 ? 4 : 7)) ? params[q] : 0.;
+    const std::vector<V3> pts = hexa ? regularHexa(pr, radius, gap) : regularOrtho(pr, radius, gap);
+    *nOut = (int64_t)pts.size();
+    if ((int64_t)pts.size() > cap) return -1;
+    for (size_t i = 0; i < pts.size(); i++)
+        for (int d = 0; d < 3; d++) c3[3 * i + d] = pts[i][d];
     return 0;
 }
 int orc_set_dyndt(void* h, const orc_dyndt* d) {

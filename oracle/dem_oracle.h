@@ -92,6 +92,8 @@ int orc_get_free_ids(void* h, int32_t cap, int32_t* ids, int32_t* n, int32_t* pa
 int orc_make_cloud(const double* mn3, const double* mx3, double rMean, double rRelFuzz, int32_t num, int32_t periodic, double porosity,
                    const double* psdSizes, const double* psdCumm, int32_t nPsd, int32_t distributeMass, uint32_t* state, int64_t cap,
                    double* c3, double* r, int64_t* nOut, double* appliedPsdScaling);
+/* pack.regularOrtho / regularHexa (py/pack.py:323-350); kind 0 inAlignedBox, 1 inSphere, 2 inCylinder */
+int orc_regular_pack(int32_t hexa, int32_t kind, const double* params, double radius, double gap, int64_t cap, double* c3, int64_t* nOut);
 /* Scene.time, Scene.dt */
 int orc_get_time(void* h, double* time, double* dt);
 

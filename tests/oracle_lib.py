@@ -79,6 +79,7 @@ def lib():
         "orc_set_masks": (C.c_int, [P, I32, pi, pu]),
         "orc_get_free_ids": (C.c_int, [P, I32, pi, pi, pi]),
         "orc_make_cloud": (C.c_int, [pd, pd, D, D, I32, I32, D, pd, pd, I32, I32, pu, I64, pd, pd, pl, pd]),
+        "orc_regular_pack": (C.c_int, [I32, I32, pd, D, D, I64, pd, pl]),
     }
     for n, (r, a) in sig.items():
         f = getattr(L, n)
@@ -279,3 +280,16 @@ def make_cloud(mn, mx, rMean=-1, rRelFuzz=0, num=-1, periodic=False, porosity=.5
     if rc != 0:
         raise RuntimeError("oracle makeCloud failed")
     return c[:n.value], r[:n.value], int(st[0]), sc.value
+
+
+def regular_pack(hexa, kind, params, radius, gap, cap=1000000):
+    """oracle pack.regularOrtho (hexa false) / regularHexa: the centres, in the reference's order"""
+    L = lib()
+    prm = np.zeros(7)
+    prm[:len(params)] = params
+    c = np.empty((cap, 3))
+    n = C.c_int64()
+    if L.orc_regular_pack(int(bool(hexa)), int(kind), _p(prm, C.c_double), float(radius), float(gap), int(cap), _p(c, C.c_double),
+                          C.byref(n)) != 0:
+        raise RuntimeError("oracle regular packing: %d points exceed cap" % n.value)
+    return c[:n.value]

@@ -135,3 +135,85 @@ def test_gpu_make_cloud_equals_oracle(case):
     assert len(g_r) == len(o_r) > 100
     assert np.array_equal(g_r, o_r) and np.array_equal(g_c, o_c)
     assert int(st[0]) == o_st and g_sc == o_sc
+
+
+# ---------------- pack.regularOrtho / regularHexa (py/pack.py:323-350) ----------------
+def test_ortho_axes_follow_numpy_arange():
+    # the reference lays the grid out with numpy.arange(mn+r, mx-r, 2r+gap) per axis and
+    # visits itertools.product(xx, yy, zz): z fastest
+    mn, mx, r, gap = np.array([-.013, .2, 1.]), np.array([.1, .31, 1.07]), .004, .0013
+    c = OL.regular_pack(0, 0, list(mn) + list(mx), r, gap)
+    ax = [np.arange(mn[i] + r, mx[i] - r, 2 * r + gap) for i in range(3)]
+    want = np.array([(x, y, z) for x in ax[0] for y in ax[1] for z in ax[2]])
+    assert np.array_equal(c, want)  # every lattice point of a box lies inside it with pad r
+
+
+def test_in_sphere_pad_cancels():
+    # inSphere tests |p - c| - pad <= R - pad (py/_packPredicates.cpp:103): the pad cancels, so
+    # the spheres are those whose centres lie in the ball (their surfaces may cross it)
+    cen, R, r, gap = np.array([.01, -.02, .03]), .05, .004, .001
+    c = OL.regular_pack(0, 1, list(cen) + [R], r, gap)
+    ax = [np.arange(cen[i] - R + r, cen[i] + R - r, 2 * r + gap) for i in range(3)]
+    lat = np.array([(x, y, z) for x in ax[0] for y in ax[1] for z in ax[2]])
+    d = np.linalg.norm(lat - cen, axis=1)
+    sure = np.abs(d - R) > 1e-12 * R
+    inside = set(map(tuple, lat[sure & (d < R)]))
+    got = set(map(tuple, c))
+    assert inside <= got and not (got & set(map(tuple, lat[sure & (d > R)])))
+    assert np.max(np.linalg.norm(c - cen, axis=1)) > R - r  # some do reach out of the ball
+
+
+def test_in_cylinder_against_axis_projection():
+    # inCylinder keeps centres at least r inside both bases and at most R - r from the axis
+    c1, c2, R, r, gap = np.array([0., 0., 0.]), np.array([.02, .03, .09]), .025, .003, .0005
+    c = OL.regular_pack(0, 2, list(c1) + list(c2) + [R], r, gap)
+    assert len(c) > 50
+    e = (c2 - c1) / np.linalg.norm(c2 - c1)
+    t = (c - c1) @ e
+    rad = np.linalg.norm((c - c1) - t[:, None] * e, axis=1)
+    ht = np.linalg.norm(c2 - c1)
+    tol = 1e-12
+    assert np.all(t >= r - tol) and np.all(t <= ht - r + tol) and np.all(rad <= R - r + tol)
+    # and every lattice point that satisfies it clearly is kept
+    k = np.array([np.sqrt((c1[1] - c2[1]) ** 2 + (c1[2] - c2[2]) ** 2), np.sqrt((c1[0] - c2[0]) ** 2 + (c1[2] - c2[2]) ** 2),
+                  np.sqrt((c1[0] - c2[0]) ** 2 + (c1[1] - c2[1]) ** 2)]) / ht
+    mn, mx = np.minimum(c1, c2) - R * k, np.maximum(c1, c2) + R * k
+    ax = [np.arange(mn[i] + r, mx[i] - r, 2 * r + gap) for i in range(3)]
+    lat = np.array([(x, y, z) for x in ax[0] for y in ax[1] for z in ax[2]])
+    tl = (lat - c1) @ e
+    rl = np.linalg.norm((lat - c1) - tl[:, None] * e, axis=1)
+    clear = (tl > r + 1e-9) & (tl < ht - r - 1e-9) & (rl < R - r - 1e-9)
+    assert set(map(tuple, lat[clear])) <= set(map(tuple, c))
+
+
+def test_hexa_neighbour_distances():
+    # regularHexa's rows are a sqrt(3)/2 apart and its layers a sqrt(6)/3, alternate rows shifted
+    # by a/2 and odd layers by (a/2, hy/2) (py/pack.py:341-348); hy/2 rather than hy/3 makes each
+    # inner sphere's 12 nearest neighbours 2 at sqrt(41/48) a, 6 at a and 4 at sqrt(53/48) a
+    r, gap = .01, .002
+    a = 2 * r + gap
+    c = OL.regular_pack(1, 0, [0, 0, 0, .3, .3, .3], r, gap)
+    inner = np.all((c > .08) & (c < .22), axis=1)
+    d = np.linalg.norm(c[inner][:, None, :] - c[None, :, :], axis=2)
+    d.sort(axis=1)
+    nn = d[:, 1:13] / a
+    want = np.array([np.sqrt(41 / 48.)] * 2 + [1.] * 6 + [np.sqrt(53 / 48.)] * 4)
+    assert inner.sum() > 100
+    assert np.allclose(nn, want[None, :], rtol=1e-12, atol=0)
+    assert np.all(d[:, 13] / a > np.sqrt(53 / 48.) + 1e-3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("hexa", [0, 1])
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_gpu_regular_pack_equals_oracle(hexa, kind):
+    from woodem_amd._native import Context
+    ctx = Context(0)
+    params = {0: [-.05, 0., .1, .07, .09, .2], 1: [.01, -.02, .03, .06], 2: [0., 0., 0., .02, .03, .09, .025]}[kind]
+    for r, gap in ((.003, .0005), (.0021, 0.)):
+        want = OL.regular_pack(hexa, kind, params, r, gap)
+        got = ctx.regular_pack(hexa, kind, params, r, gap)
+        assert len(want) > 100
+        assert np.array_equal(got, want)
+    with pytest.raises(RuntimeError):
+        ctx.regular_pack(hexa, kind, params, .003, .0005, cap=10)

@@ -125,6 +125,7 @@ def lib():
         "wdm_set_masks": (C.c_int, [P, I32, I32, pi, pu]),
         "wdm_select_box": (C.c_int, [P, I32, pd, pd, I32, U32, U32, I32, pi, pi]),
         "wdm_make_cloud": (C.c_int, [P, pd, pd, D, D, I32, I32, D, pd, pd, I32, I32, pu, I64, pd, pd, pl, pd]),
+        "wdm_regular_pack": (C.c_int, [P, I32, I32, pd, D, D, I64, pd, pl]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -411,6 +412,19 @@ class Context:
                                          ptr(pc, C.c_double) if len(pc) else None, len(ps), int(bool(distributeMass)),
                                          ptr(state, C.c_uint32), int(cap), ptr(c, C.c_double), ptr(r, C.c_double), C.byref(n), C.byref(sc)))
         return c[:n.value], r[:n.value], sc.value
+
+    def regular_pack(self, hexa, kind, params, radius, gap, cap=None):
+        """pack.regularOrtho (hexa false) / regularHexa on the device for predicate `kind`
+        (0 inAlignedBox, 1 inSphere, 2 inCylinder) with its parameters; returns the centres
+        (all of them, or an error when there are more than `cap`)."""
+        prm = np.ascontiguousarray(params, np.float64)
+        n = C.c_int64()
+        args = (int(bool(hexa)), int(kind), ptr(prm, C.c_double), float(radius), float(gap))
+        self.check(self.L.wdm_regular_pack(self.h, *args, 0, None, C.byref(n)))
+        m = n.value if cap is None else min(n.value, int(cap))
+        c = np.empty((max(m, 1), 3))
+        self.check(self.L.wdm_regular_pack(self.h, *args, m, ptr(c, C.c_double), C.byref(n)))
+        return c[:n.value]
 
     def relax(self, sid, iters):
         self.check(self.L.wdm_relax_overlaps(self.h, sid, int(iters)))

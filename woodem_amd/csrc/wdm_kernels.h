@@ -2423,6 +2423,74 @@ __global__ void __launch_bounds__(1024) k_make_cloud(CloudArgs A, uint32_t* stat
     }
 }
 
+// ---------------- regular packings: pack.regularOrtho / pack.regularHexa ----------------
+// Lattice points in the order of itertools.product(xx, yy, zz) (z fastest), kept where the
+// predicate holds with the sphere radius as pad (py/pack.py:323-350); predicates
+// inAlignedBox, inSphere, inCylinder (py/_packPredicates.cpp:96-104, 136-148, 189-205).
+struct LatticeArgs {
+    int hexa, kind;   // kind: 0 inAlignedBox(mn, mx), 1 inSphere(centre, r), 2 inCylinder(c1, c2, r)
+    int64_t n[3];     // points per axis
+    double start[3], a1[3];  // ortho: numpy.arange(start, stop, step) holds start, start+step, then start + i (a1 - start)
+    double mn[3], radius, a, hy, hz;  // hexa
+    double p[7];      // predicate: box mn3 mx3 / centre3 r / c1_3 c2_3 r
+};
+WDM_HD double arangeAt(const LatticeArgs& L, int d, int64_t i) {
+    if (i == 0) return L.start[d];
+    if (i == 1) return L.a1[d];
+    return L.start[d] + (double)i * (L.a1[d] - L.start[d]);
+}
+WDM_HD void latticePoint(const LatticeArgs& L, int64_t idx, double& x, double& y, double& z) {
+    const int64_t k = idx % L.n[2], t = idx / L.n[2], j = t % L.n[1], i = t / L.n[1];
+    if (!L.hexa) {
+        x = arangeAt(L, 0, i);
+        y = arangeAt(L, 1, j);
+        z = arangeAt(L, 2, k);
+        return;
+    }
+    x = (L.mn[0] + L.radius) + (double)i * L.a;
+    y = (L.mn[1] + L.radius) + (double)j * L.hy;
+    z = (L.mn[2] + L.radius) + (double)k * L.hz;
+    if (j % 2 == 0) x += (k % 2 == 0) ? L.a / 2. : -L.a / 2.;
+    if (k % 2 != 0) {
+        x += L.a / 2.;
+        y += L.hy / 2.;
+    }
+}
+WDM_HD bool latticePred(const LatticeArgs& L, double x, double y, double z) {
+    const double pad = L.radius;
+    const double* p = L.p;
+    if (L.kind == 0)
+        return p[0] + pad <= x && p[3] - pad >= x && p[1] + pad <= y && p[4] - pad >= y && p[2] + pad <= z && p[5] - pad >= z;
+    if (L.kind == 1) {
+        const double dx = x - p[0], dy = y - p[1], dz = z - p[2];
+        return sqrt((dx * dx + dy * dy) + dz * dz) - pad <= p[3] - pad;
+    }
+    const double cx = p[3] - p[0], cy = p[4] - p[1], cz = p[5] - p[2];
+    const double ht = sqrt((cx * cx + cy * cy) + cz * cz);
+    const double u = (((x * cx + y * cy) + z * cz) - ((p[0] * cx + p[1] * cy) + p[2] * cz)) / (ht * ht);
+    if ((u * ht < 0 + pad) || (u * ht > ht - pad)) return false;
+    const double ax = x - p[0], ay = y - p[1], az = z - p[2], bx = x - p[3], by = y - p[4], bz = z - p[5];
+    const double qx = ay * bz - az * by, qy = az * bx - ax * bz, qz = ax * by - ay * bx;
+    const double axisDist = sqrt((qx * qx + qy * qy) + qz * qz) / ht;
+    return !(axisDist > p[6] - pad);
+}
+__global__ void __launch_bounds__(WDM_BLOCK) k_lattice_flags(LatticeArgs L, int64_t total, uint8_t* flags) {
+    const int64_t idx = (int64_t)blockIdx.x * WDM_BLOCK + threadIdx.x;
+    if (idx >= total) return;
+    double x, y, z;
+    latticePoint(L, idx, x, y, z);
+    flags[idx] = latticePred(L, x, y, z) ? 1 : 0;
+}
+__global__ void __launch_bounds__(WDM_BLOCK) k_lattice_write(LatticeArgs L, const int64_t* sel, const int64_t* nSel, double* out) {
+    const int64_t i = (int64_t)blockIdx.x * WDM_BLOCK + threadIdx.x;
+    if (i >= *nSel) return;
+    double x, y, z;
+    latticePoint(L, sel[i], x, y, z);
+    out[3 * i] = x;
+    out[3 * i + 1] = y;
+    out[3 * i + 2] = z;
+}
+
 // ---------------- scene generation: Jacobi overlap relaxation ----------------
 // Each sphere moves by half of every overlap with its neighbours (found through the
 // same cell grid); all displacements are computed from the old positions, so the

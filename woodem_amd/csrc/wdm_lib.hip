@@ -2635,6 +2635,109 @@ int wdm_make_cloud(wdm_ctx* c, const double* mn3, const double* mx3, double rMea
         if (appliedPsdScaling) *appliedPsdScaling = scaling;
     });
 }
+// pack.regularOrtho / pack.regularHexa (py/pack.py:323-350) for the predicates inAlignedBox,
+// inSphere and inCylinder: the predicate's aabb (py/_packPredicates.cpp), the lattice over it
+// as the reference lays it out, the points where the predicate holds (pad = radius) in the
+// reference's order.
+int wdm_regular_pack(wdm_ctx* c, int32_t hexa, int32_t kind, const double* params, double radius, double gap, int64_t cap, double* c3,
+                     int64_t* nOut) {
+    return guard(c, [&] {
+        if (!params || !nOut || cap < 0) throw WdmError(WDM_E_ARG, "regular packing arguments");
+        if (kind < 0 || kind > 2) throw WdmError(WDM_E_ARG, "predicate kind must be 0 (inAlignedBox), 1 (inSphere) or 2 (inCylinder)");
+        LatticeArgs L;
+        std::memset(&L, 0, sizeof(L));
+        L.hexa = hexa ? 1 : 0;
+        L.kind = kind;
+        const int np = kind == 0 ? 6 : (kind == 1 ? 4 : 7);
+        for (int q = 0; q < np; q++) L.p[q] = params[q];
+        double mn[3], mx[3];
+        if (kind == 0) {
+            for (int d = 0; d < 3; d++) { mn[d] = params[d]; mx[d] = params[3 + d]; }
+        } else if (kind == 1) {
+            for (int d = 0; d < 3; d++) { mn[d] = params[d] - params[3]; mx[d] = params[d] + params[3]; }
+        } else {  // inCylinder::aabb
+            const double* A = params;
+            const double* B = params + 3;
+            const double cx = B[0] - A[0], cy = B[1] - A[1], cz = B[2] - A[2];
+            const double ht = std::sqrt((cx * cx + cy * cy) + cz * cz);
+            if (!(ht > 0)) throw WdmError(WDM_E_ARG, "inCylinder: the axis has zero length");
+            const double k[3] = {std::sqrt((A[1] - B[1]) * (A[1] - B[1]) + (A[2] - B[2]) * (A[2] - B[2])) / ht,
+                                 std::sqrt((A[0] - B[0]) * (A[0] - B[0]) + (A[2] - B[2]) * (A[2] - B[2])) / ht,
+                                 std::sqrt((A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1])) / ht};
+            for (int d = 0; d < 3; d++) {
+                mn[d] = std::min(A[d], B[d]) - params[6] * k[d];
+                mx[d] = std::max(A[d], B[d]) + params[6] * k[d];
+            }
+        }
+        for (int d = 0; d < 3; d++)
+            if (!std::isfinite(mn[d]) || !std::isfinite(mx[d])) throw WdmError(WDM_E_ARG, "Aabb of the predicate must not be infinite");
+        L.radius = radius;
+        int64_t total = 1;
+        if (!L.hexa) {
+            const double step = 2 * radius + gap;
+            for (int d = 0; d < 3; d++) {
+                L.start[d] = mn[d] + radius;
+                L.a1[d] = L.start[d] + step;
+                const double len = std::ceil(((mx[d] - radius) - L.start[d]) / step);  // numpy.arange's length
+                L.n[d] = (len > 0 && std::isfinite(len)) ? (int64_t)len : 0;
+            }
+        } else {
+            L.a = 2 * radius + gap;
+            L.hy = L.a * std::sqrt(3.) / 2.;
+            L.hz = L.a * std::sqrt(6.) / 3.;
+            const double h[3] = {L.a, L.hy, L.hz};
+            for (int d = 0; d < 3; d++) {
+                L.mn[d] = mn[d];
+                const double q = (mx[d] - mn[d]) / h[d];
+                if (!(q >= 0) || !std::isfinite(q)) throw WdmError(WDM_E_ARG, "regularHexa: lattice spacing");
+                L.n[d] = (int64_t)q + 1;  // range(0, int(dim/h) + 1)
+            }
+        }
+        for (int d = 0; d < 3; d++) {
+            if (L.n[d] > ((int64_t)1 << 31)) throw WdmError(WDM_E_ARG, "lattice too large");
+            total *= L.n[d];
+            if (total > INT32_MAX) throw WdmError(WDM_E_ARG, "lattice too large (more than 2^31-1 points)");
+        }
+        *nOut = 0;
+        if (total == 0) return;
+        hipStream_t st = c->stream;
+        DBuf<uint8_t> flags;
+        DBuf<int64_t> sel, nsel;
+        DBuf<double> outB;
+        ensure(c, flags, (size_t)total);
+        ensure(c, sel, (size_t)total);
+        ensure(c, nsel, 1);
+        hipLaunchKernelGGL(k_lattice_flags, dim3(nblk(total)), dim3(WDM_BLOCK), 0, st, L, total, flags.p);
+        checkLaunch();
+        hipcub::CountingInputIterator<int64_t> it(0);
+        size_t bytes = 0;
+        HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, bytes, it, flags.p, sel.p, nsel.p, (int)total, st));
+        void* t = tempFor(c, bytes);
+        HIPCHK(hipcub::DeviceSelect::Flagged(t, bytes, it, flags.p, sel.p, nsel.p, (int)total, st));
+        int64_t n = 0;
+        HIPCHK(hipMemcpyAsync(&n, nsel.p, sizeof(n), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (!c3) {  // count only
+            release(c, flags); release(c, sel); release(c, nsel);
+            *nOut = n;
+            return;
+        }
+        if (n > cap) {
+            release(c, flags); release(c, sel); release(c, nsel);
+            *nOut = n;
+            throw WdmError(WDM_E_ARG, "cap below the spheres of the lattice");
+        }
+        if (n > 0) {
+            ensure(c, outB, (size_t)n * 3);
+            hipLaunchKernelGGL(k_lattice_write, dim3(nblk(n)), dim3(WDM_BLOCK), 0, st, L, (const int64_t*)sel.p, (const int64_t*)nsel.p, outB.p);
+            checkLaunch();
+            HIPCHK(hipMemcpyAsync(c3, outB.p, (size_t)n * 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+        }
+        release(c, flags); release(c, sel); release(c, nsel); release(c, outB);
+        *nOut = n;
+    });
+}
 int wdm_set_energy(wdm_ctx* c, int32_t sid, const double* values, const int32_t* created) {
     return guard(c, [&] {
         Scene& S = getScene(c, sid);

@@ -92,3 +92,60 @@ class SpherePack:
             S.periodic = True
             S.cell.setBox(self.cellSize)
         return S.dem.par.add([utils.sphere(tuple(c), float(r), **kw) for c, r in zip(self.c, self.r)])
+
+
+# ---------------- predicates and regular packings ----------------
+class Predicate:
+    """A solid that regularOrtho/regularHexa clip their lattice to (py/_packPredicates.cpp);
+    `kind` and `params` are what the device evaluates."""
+    kind = -1
+    params = ()
+
+
+class inAlignedBox(Predicate):
+    """Axis-aligned box (py/_packPredicates.cpp:136-148)"""
+    kind = 0
+
+    def __init__(self, minAABB, maxAABB):
+        self.params = tuple(float(x) for x in minAABB) + tuple(float(x) for x in maxAABB)
+
+    def aabb(self):
+        return self.params[:3], self.params[3:]
+
+
+class inSphere(Predicate):
+    """Sphere (py/_packPredicates.cpp:96-104)"""
+    kind = 1
+
+    def __init__(self, center, radius):
+        self.params = tuple(float(x) for x in center) + (float(radius),)
+
+
+class inCylinder(Predicate):
+    """Cylinder between the centres of its bases (py/_packPredicates.cpp:189-205)"""
+    kind = 2
+
+    def __init__(self, centerBottom, centerTop, radius):
+        self.params = tuple(float(x) for x in centerBottom) + tuple(float(x) for x in centerTop) + (float(radius),)
+
+
+def _regular(hexa, predicate, radius, gap, ctx, cap, kw):
+    from . import utils
+    from ._native import Context
+    if not isinstance(predicate, Predicate) or predicate.kind < 0:
+        raise NotImplementedError("regular packings support the predicates inAlignedBox, inSphere and inCylinder")
+    ctx = ctx or Context(0)
+    c = ctx.regular_pack(hexa, predicate.kind, predicate.params, radius, gap, cap)
+    return [utils.sphere(tuple(x), radius=radius, **kw) for x in c]
+
+
+def regularOrtho(predicate, radius, gap, ctx=None, cap=None, **kw):
+    """Spheres on an orthogonal grid clipped to the predicate, separated by `gap`
+    (py/pack.py:323-332), computed on the device; kw go to utils.sphere."""
+    return _regular(False, predicate, radius, gap, ctx, cap, kw)
+
+
+def regularHexa(predicate, radius, gap, ctx=None, cap=None, **kw):
+    """Spheres on a hexagonal grid clipped to the predicate, separated by `gap`
+    (py/pack.py:334-350), computed on the device; kw go to utils.sphere."""
+    return _regular(True, predicate, radius, gap, ctx, cap, kw)
