@@ -2190,22 +2190,23 @@ __device__ __forceinline__ int cloudCell(const CloudArgs& A, int d, double x) {
 // earlier tries were accepted.  A window fixes the draws of its next tries by jumping the
 // generator ahead (x_k = x_0 48271^k mod 2^31-1) and evaluates, in parallel, each try under a
 // few hypotheses on that count:
-//   num > 0: try j of the window with a = lo_j .. lo_j+2 acceptances before it (sphere i+a,
+//   num > 0: try j of the window with a = lo_j .. lo_j+3 acceptances before it (sphere i+a,
 //            radius radii[i+a], draws 3j..3j+2), lo_j following the last window's rate;
 //   num < 0: try j with none accepted before it (sphere i, draws s0+3j..) and with all
 //            accepted before it (sphere i+j, radius draw s0+4j-1, draws s0+4j..).
-// Each candidate is tested against the cell grid of the spheres placed before the window, by 27
-// threads (one per neighbour cell).  One wave then walks the tries in order, exactly as the
-// reference does, taking the candidate of the hypothesis that holds and testing it against the
-// spheres accepted earlier in the window; the window ends where no evaluated hypothesis holds.
-// Results (spheres, their order, the generator state, the failing sphere) are those of the
-// sequential loop; only the number of rounds changes.
-#define CLOUD_H 3                           // hypotheses per try (num > 0)
-#define CLOUD_W 12                          // tries per window (num > 0)
-#define CLOUD_W2 18                         // tries per family (num < 0)
-#define CLOUD_SLOTS 36                      // candidates per window: CLOUD_W x CLOUD_H = 2 x CLOUD_W2
-#define CLOUD_Q 80                          // draws generated per window (num < 0 uses at most 1 + 4 x 18 + 3)
-#define CLOUD_THREADS (CLOUD_SLOTS * 27)    // 972 threads test cells; the block has 1024
+// Each candidate is tested against the cell grid of the spheres placed before the window by 9
+// threads (one row of 3 neighbour cells each), and against every candidate of an earlier try
+// of the window (a bit mask per candidate).  One wave then walks the tries in order, exactly as
+// the reference does, taking the candidate of the hypothesis that holds; it overlaps the
+// window's accepted spheres when its mask meets theirs.  The window ends where no evaluated
+// hypothesis holds.  Results (spheres, their order, the generator state, the failing sphere)
+// are those of the sequential loop; only the number of rounds changes.
+#define CLOUD_TPC 9                         // threads per candidate: one row of 3 neighbour cells each
+#define CLOUD_SLOTS 112                     // candidates per window
+#define CLOUD_H 4                           // hypotheses per try (num > 0)
+#define CLOUD_W (CLOUD_SLOTS / CLOUD_H)     // 28 tries per window (num > 0)
+#define CLOUD_W2 (CLOUD_SLOTS / 2)          // 56 tries per family (num < 0)
+#define CLOUD_Q 232                         // draws generated per window (num < 0 uses at most 1 + 4 x 56 + 3)
 __device__ __forceinline__ bool cloudOverlap(const CloudArgs& A, const double4 p, const double4 c) {
     // p placed before c (SpherePack.cpp:222-229): periodic distance is the shorter way round
     double dr[3];
@@ -2221,12 +2222,18 @@ __device__ __forceinline__ int cloudLo(double p, int j) {
     const int lo = cj - 1 < j - (CLOUD_H - 1) ? cj - 1 : j - (CLOUD_H - 1);
     return lo < 0 ? 0 : lo;
 }
+__device__ __forceinline__ int cloudTry(const CloudArgs& A, int s) {  // the window's try a slot evaluates
+    return A.num > 0 ? s / CLOUD_H : (s < CLOUD_W2 ? s : s - (CLOUD_W2 - 1));
+}
 // out[0]: spheres placed; out[1]: 1 when a sphere found no free spot in maxTry tries
 __global__ void __launch_bounds__(1024) k_make_cloud(CloudArgs A, uint32_t* state, int32_t* head, int32_t* next, double4* sph, int64_t* out) {
     __shared__ uint32_t pw[CLOUD_Q];   // 48271^(k+1) mod 2^31-1
     __shared__ uint32_t xs[CLOUD_Q];   // generator state after good draw k of the window
     __shared__ double4 cnd[CLOUD_SLOTS];
+    __shared__ unsigned long long ovm[CLOUD_SLOTS][2];  // candidates of earlier tries it overlaps
     __shared__ int on[CLOUD_SLOTS], hit[CLOUD_SLOTS];
+    __shared__ int4 ccell[CLOUD_SLOTS];  // cell of each candidate
+    __shared__ int loJ[CLOUD_W];         // lo_j of the window's tries (num > 0)
     __shared__ uint32_t st;
     __shared__ int64_t placed;
     __shared__ int tries, needR, fin, failed;
@@ -2285,15 +2292,19 @@ __global__ void __launch_bounds__(1024) k_make_cloud(CloudArgs A, uint32_t* stat
                 const int j = s / CLOUD_H, a = cloudLo(p, j) + s % CLOUD_H;
                 ok = a <= j && i0 + a < lim;
                 if (ok) { r = A.radii[i0 + a]; off = 3 * j; }
-            } else if (s < CLOUD_W2) {  // none accepted before try s
-                ok = i0 < lim;
-                r = needR ? cloudRadius(A, cloudU(xs[0])) : rCur;
-                off = s0 + 3 * s;
-            } else {  // all accepted before try j
-                const int j = s - CLOUD_W2 + 1;
-                ok = i0 + j < lim;
-                off = s0 + 4 * j;
-                if (ok) r = cloudRadius(A, cloudU(xs[off - 1]));
+            } else {
+                int ri = -1;  // draw of the radius, -1: the current sphere's
+                if (s < CLOUD_W2) {  // none accepted before try s
+                    ok = i0 < lim;
+                    off = s0 + 3 * s;
+                    if (needR) ri = 0;
+                } else {  // all accepted before try j
+                    const int j = s - (CLOUD_W2 - 1);
+                    ok = i0 + j < lim;
+                    off = s0 + 4 * j;
+                    ri = off - 1;
+                }
+                if (ok) r = ri >= 0 ? cloudRadius(A, cloudU(xs[ri])) : rCur;
             }
             if (ok) {
                 double c[3];
@@ -2302,21 +2313,25 @@ __global__ void __launch_bounds__(1024) k_make_cloud(CloudArgs A, uint32_t* stat
                     c[d] = A.periodic ? A.mn[d] + A.size[d] * u : (A.mn[d] + r) + (A.size[d] - 2 * r) * u;
                 }
                 cnd[s] = make_double4(c[0], c[1], c[2], r);
+                ccell[s] = make_int4(cloudCell(A, 0, c[0]), cloudCell(A, 1, c[1]), cloudCell(A, 2, c[2]), 0);
             }
+            if (s < CLOUD_W) loJ[s] = cloudLo(p, s);
             on[s] = ok;
             hit[s] = 0;
+            ovm[s][0] = 0;
+            ovm[s][1] = 0;
         }
         __syncthreads();
-        // grid test: thread (slot, neighbour cell)
-        if (tid < CLOUD_THREADS) {
-            const int s = tid / 27, o = tid % 27;
+        // grid test: thread (slot, row of 3 neighbour cells); the three list heads are loaded together
+        if (tid < CLOUD_SLOTS * CLOUD_TPC) {
+            const int s = tid / CLOUD_TPC, row = tid % CLOUD_TPC;
             if (on[s]) {
                 const double4 cd = cnd[s];
                 const double ccd[3] = {cd.x, cd.y, cd.z};
-                const int oo[3] = {o % 3 - 1, (o / 3) % 3 - 1, o / 9 - 1};
+                const int oo[3] = {0, row % 3 - 1, row / 3 - 1};
                 int cell[3];
                 bool inr = true;
-                for (int d = 0; d < 3; d++) {
+                for (int d = 1; d < 3; d++) {
                     int q = cloudCell(A, d, ccd[d]) + oo[d];
                     // periodic axes of 3+ cells wrap; with fewer cells (and aperiodic axes) each cell
                     // in range is visited once
@@ -2325,32 +2340,147 @@ __global__ void __launch_bounds__(1024) k_make_cloud(CloudArgs A, uint32_t* stat
                     cell[d] = q;
                 }
                 if (inr) {
+                    const int cx = cloudCell(A, 0, ccd[0]);
+                    const int64_t base = ((int64_t)cell[2] * A.nc[1] + cell[1]) * A.nc[0];
+                    int cur[3];
+                    for (int k = 0; k < 3; k++) {
+                        int q = cx + k - 1;
+                        if (A.periodic && A.nc[0] >= 3) q = (q + A.nc[0]) % A.nc[0];
+                        cur[k] = (q >= 0 && q < A.nc[0]) ? head[base + q] : -1;
+                    }
                     bool ov = false;
-                    for (int j = head[(cell[2] * A.nc[1] + cell[1]) * A.nc[0] + cell[0]]; j >= 0 && !ov; j = next[j]) ov = cloudOverlap(A, sph[j], cd);
+                    for (int k = 0; k < 3 && !ov; k++)
+                        for (int j = cur[k]; j >= 0 && !ov; j = next[j]) ov = cloudOverlap(A, sph[j], cd);
                     if (ov) hit[s] = 1;
                 }
             }
         }
+        // window test: thread (slot, 16 slots of earlier tries)
+        if (tid < CLOUD_SLOTS * 7) {
+            const int s = tid / 7, ch = tid % 7;
+            if (on[s]) {
+                const int ts = cloudTry(A, s);
+                const double4 cd = cnd[s];
+                const int4 cs = ccell[s];
+                unsigned long long bits = 0;
+                for (int b = 0; b < 16; b++) {
+                    const int q = ch * 16 + b;
+                    if (q >= CLOUD_SLOTS || !on[q] || cloudTry(A, q) >= ts) continue;
+                    // spheres that can touch lie in neighbouring cells (cells are at least 2 r_max wide)
+                    const int4 cq = ccell[q];
+                    const int dc[3] = {cq.x - cs.x, cq.y - cs.y, cq.z - cs.z};
+                    bool near = true;
+                    for (int d = 0; d < 3; d++) {
+                        const int ad = dc[d] < 0 ? -dc[d] : dc[d];
+                        near = near && (ad <= 1 || (A.periodic && (A.nc[d] < 3 || ad == A.nc[d] - 1)));
+                    }
+                    if (near && cloudOverlap(A, cnd[q], cd)) bits |= 1ull << (q & 63);
+                }
+                if (bits) atomicOr(&ovm[s][ch >> 2], bits);
+            }
+        }
         __syncthreads();
-        // walk the tries in order (wave 0; every value below is uniform across the wave except `mine`)
+        // walk the tries in order (wave 0; every value below is uniform across the wave except
+        // `mine` and the per-lane copies of the window's tests, which the walk reads by lane)
         if (tid < 64) {
             const int lane = tid;
-            double4 mine = make_double4(0., 0., 0., 0.);  // accepted candidate number `lane` of the window
-            int na = 0, t = tries, used = 0, nr = needR, fl = 0, fn = 0, triesUsed = 0;
-            double rc = rCur;
-            if (A.num > 0) {
+            const bool hasB = lane + 64 < CLOUD_SLOTS;
+            const int hA = hit[lane], hB = hasB ? hit[lane + 64] : 1;
+            const int myLo = lane < CLOUD_W ? loJ[lane] : 0;
+            const unsigned long long mA0 = ovm[lane][0], mA1 = ovm[lane][1];
+            const unsigned long long mB0 = hasB ? ovm[lane + 64][0] : 0ull, mB1 = hasB ? ovm[lane + 64][1] : 0ull;
+            int mine = 0;  // slot of the window's accepted sphere number `lane`
+            unsigned long long acc0 = 0, acc1 = 0;  // accepted slots
+            // the walk's state is uniform: kept in scalar registers
+            int na = 0, t = __builtin_amdgcn_readfirstlane(tries), used = 0, nr = __builtin_amdgcn_readfirstlane(needR), fl = 0, fn = 0,
+                triesUsed = 0, rcSlot = -1;
+            const int64_t i0s = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(i0 >> 32)) << 32) |
+                                          (uint32_t)__builtin_amdgcn_readfirstlane((int)i0));
+            const int s0s = __builtin_amdgcn_readfirstlane(s0);
+            auto rl = [](uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); };
+            auto rl64 = [&](unsigned long long v, int l) {
+                return ((unsigned long long)rl((uint32_t)(v >> 32), l) << 32) | rl((uint32_t)v, l);
+            };
+            auto overlapsAccepted = [&](int s) {  // s is uniform: its lane's copy is read with v_readlane
+                const int l = s & 63;
+                const bool b = s >= 64;
+                const int h = (int)rl((uint32_t)(b ? hB : hA), l);
+                const unsigned long long m0 = rl64(b ? mB0 : mA0, l), m1 = rl64(b ? mB1 : mA1, l);
+                return h != 0 || (m0 & acc0) != 0 || (m1 & acc1) != 0;
+            };
+            auto accept = [&](int s) {
+                if (lane == na) mine = s;
+                na++;
+                if (s < 64) acc0 |= 1ull << s;
+                else acc1 |= 1ull << (s - 64);
+            };
+            // num > 0 without overlaps between the window's candidates (the usual case), and no
+            // sphere that could reach maxTry in the window: the walk's only state is na, and try j
+            // maps na in its band lo_j..lo_j+3 to na + (candidate hit ? 0 : 1), or ends the walk.
+            // The maps are composed by an inclusive scan over the wave (try j on lane j), which gives
+            // na after every try; the walk stops at the first try whose map ends it.
+            bool fastDone = false;
+            int insSlot = -1;
+            int64_t insIdx = 0;
+            if (A.num > 0 && !__any((mA0 | mA1 | mB0 | mB1) != 0ull) && t + CLOUD_W < A.maxTry) {
+                const int j = lane;
+                uint32_t T = 0xffffffffu;  // 4 entries of int8: na after try j for na = startLo + k, -1 when it ends
+                int startLo = myLo;          // lo of the first try of the lane's composed range
+                if (j < CLOUD_W) {
+                    T = 0;
+                    for (int k = 0; k < 4; k++) {
+                        const int nin = myLo + k;
+                        int o = -1;
+                        if (nin <= j && i0s + nin < lim) o = nin + (hit[j * CLOUD_H + k] ? 0 : 1);
+                        T |= (uint32_t)(o & 0xff) << (8 * k);
+                    }
+                }
+                for (int off = 1; off < 32; off <<= 1) {  // 28 tries fit in 32 lanes
+                    const uint32_t P = (uint32_t)__shfl_up((int)T, off);
+                    const int pLo = __shfl_up(startLo, off);
+                    if (j >= off && j < CLOUD_W) {
+                        uint32_t R = 0;
+                        for (int k = 0; k < 4; k++) {
+                            const int pk = (int)(int8_t)((P >> (8 * k)) & 0xffu);
+                            int o = -1;
+                            if (pk >= 0 && pk - startLo >= 0 && pk - startLo < 4) o = (int)(int8_t)((T >> (8 * (pk - startLo))) & 0xffu);
+                            R |= (uint32_t)(o & 0xff) << (8 * k);
+                        }
+                        T = R;
+                        startLo = pLo;
+                    }
+                }
+                const int X = (int)(int8_t)(T & 0xffu);  // na after try j (the range starts at try 0, na 0)
+                int Xp = __shfl_up(X, 1);
+                if (j == 0) Xp = 0;
+                const unsigned long long stopM = __ballot(j < CLOUD_W && X < 0);
+                const int jstar = stopM ? __ffsll((long long)stopM) - 1 : CLOUD_W;
+                const int naF = jstar == 0 ? 0 : __builtin_amdgcn_readlane(X, jstar - 1);
+                const bool accepted = j < jstar && X > Xp;
+                const unsigned long long accM = __ballot(accepted);
+                if (accepted) {
+                    insSlot = j * CLOUD_H + (Xp - myLo);
+                    insIdx = i0s + Xp;
+                }
+                na = naF;
+                used = 3 * jstar;
+                triesUsed = jstar;
+                t = accM ? jstar - 1 - (63 - __clzll((long long)accM)) : t + jstar;
+                fn = i0s + naF >= lim ? 1 : 0;
+                fastDone = true;
+            }
+            if (fastDone) {
+            } else if (A.num > 0) {
                 for (int j = 0; j < CLOUD_W; j++) {
-                    if (i0 + na >= lim) { fn = 1; break; }
-                    const int lo = cloudLo(p, j);
+                    if (i0s + na >= lim) { fn = 1; break; }
+                    const int lo = __builtin_amdgcn_readlane(myLo, j);
                     if (na < lo || na >= lo + CLOUD_H) break;
                     const int s = j * CLOUD_H + (na - lo);
-                    const double4 c = cnd[s];
-                    const bool ov = hit[s] || __any(lane < na && cloudOverlap(A, mine, c));
+                    const bool ov = overlapsAccepted(s);
                     used = 3 * (j + 1);
                     triesUsed = j + 1;
                     if (!ov) {
-                        if (lane == na) mine = c;
-                        na++;
+                        accept(s);
                         t = 0;
                     } else if (++t == A.maxTry) {
                         fl = 1;
@@ -2358,47 +2488,52 @@ __global__ void __launch_bounds__(1024) k_make_cloud(CloudArgs A, uint32_t* stat
                         break;
                     }
                 }
-                if (!fn && i0 + na >= lim) fn = 1;
+                if (!fn && i0s + na >= lim) fn = 1;
             } else {
                 int fam = -1;  // -1 undecided (try 0), 0 none accepted, 1 all accepted
                 for (int j = 0;; j++) {
-                    if (i0 + na >= lim) { fn = 1; break; }
+                    if (i0s + na >= lim) { fn = 1; break; }
                     if (fam <= 0 && j >= CLOUD_W2) break;
                     if (fam == 1 && j > CLOUD_W2) break;
                     const int s = fam == 1 ? CLOUD_W2 - 1 + j : j;
-                    const double4 c = cnd[s];
-                    const bool ov = hit[s] || __any(lane < na && cloudOverlap(A, mine, c));
+                    const bool ov = overlapsAccepted(s);
                     triesUsed = j + 1;
                     if (!ov) {
-                        if (lane == na) mine = c;
-                        na++;
+                        accept(s);
                         t = 0;
                         nr = 1;
-                        if (fam == 1) used = s0 + 4 * j + 3;
-                        else used = s0 + 3 * (j + 1);
+                        rcSlot = -1;
+                        if (fam == 1) used = s0s + 4 * j + 3;
+                        else used = s0s + 3 * (j + 1);
                         if (fam == 0) break;  // the next sphere's radius draw follows a run of misses
                         fam = 1;
                     } else {
                         if (fam == 1) {  // first try of sphere i0+j missed
-                            used = s0 + 4 * j + 3;
+                            used = s0s + 4 * j + 3;
                             nr = 0;
-                            rc = c.w;
+                            rcSlot = s;
                             if (++t == A.maxTry) { fl = 1; fn = 1; }
                             break;
                         }
                         fam = 0;
-                        used = s0 + 3 * (j + 1);
+                        used = s0s + 3 * (j + 1);
                         nr = 0;
-                        rc = c.w;
+                        rcSlot = s;
                         if (++t == A.maxTry) { fl = 1; fn = 1; break; }
                     }
                 }
             }
+            const double rc = rcSlot >= 0 ? cnd[rcSlot].w : rCur;  // the radius of a sphere left mid-tries
+            if (!fastDone && lane < na) {
+                insSlot = mine;
+                insIdx = i0s + lane;
+            }
             // spheres accepted in the window go into the grid
-            if (lane < na) {
-                const int64_t idx = i0 + lane;
-                sph[idx] = mine;
-                const int cell = (cloudCell(A, 2, mine.z) * A.nc[1] + cloudCell(A, 1, mine.y)) * A.nc[0] + cloudCell(A, 0, mine.x);
+            if (insSlot >= 0) {
+                const double4 c = cnd[insSlot];
+                const int64_t idx = insIdx;
+                sph[idx] = c;
+                const int64_t cell = ((int64_t)cloudCell(A, 2, c.z) * A.nc[1] + cloudCell(A, 1, c.y)) * A.nc[0] + cloudCell(A, 0, c.x);
                 next[idx] = atomicExch(&head[cell], (int32_t)idx);
             }
             if (lane == 0) {
