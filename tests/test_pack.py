@@ -217,3 +217,37 @@ def test_gpu_regular_pack_equals_oracle(hexa, kind):
         assert np.array_equal(got, want)
     with pytest.raises(RuntimeError):
         ctx.regular_pack(hexa, kind, params, .003, .0005, cap=10)
+
+
+@pytest.mark.gpu
+def test_gpu_cloud_to_simulation_steps_like_oracle():
+    # the Woo scripting path: SpherePack.makeCloud (seeded once per process, as the reference's
+    # static generator), toSimulation into a walled box, defaultEngines, Scene.run; the cloud and
+    # the trajectories equal the oracle's
+    import woodem_amd as woo
+    from woodem_amd import pack, utils
+    pack.reset_generator()
+    sp = pack.SpherePack()
+    box = ((0, 0, 0), (.05, .05, .05))
+    n = sp.makeCloud(box[0], box[1], rMean=.002, rRelFuzz=.2, num=400, seed=3)
+    c, r, st, _ = OL.make_cloud(box[0], box[1], rMean=.002, rRelFuzz=.2, num=400, state=3)
+    assert n == 400 and np.array_equal(sp.c, c) and np.array_equal(sp.r, r)
+    sp2 = pack.SpherePack()  # the generator goes on where the first cloud left it
+    sp2.makeCloud(box[0], box[1], rMean=.002, num=10, seed=99)
+    c2, r2, _, _ = OL.make_cloud(box[0], box[1], rMean=.002, num=10, state=st)
+    assert np.array_equal(sp2.c, c2)
+    S = woo.Scene(fields=[woo.DemField(gravity=(0, 0, -9.81))], dt=2e-5)
+    S.dem.par.add(utils.wallBox(box))
+    ids = sp.toSimulation(S)
+    assert len(ids) == 400
+    S.engines = utils.defaultEngines(damping=.2, dynDtPeriod=0)
+    S.run(600)
+    a = S.dem._compile()
+    o = OL.OracleRun(a, S._params(), dict(periodic=False, hSize=np.eye(3), gradV=np.zeros((3, 3)), nextGradV=np.zeros((3, 3)),
+                                          homoDeform=4))
+    o.step(600)
+    no = o.nodes()
+    for i in ids[::7]:
+        assert np.array_equal(S.dem.par[i].pos, no["pos"][i])
+    assert S.dem.con.countReal() == int(o.contacts()["isReal"].sum()) > 0
+    pack.reset_generator()
