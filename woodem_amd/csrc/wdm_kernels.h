@@ -2232,7 +2232,7 @@ __global__ void __launch_bounds__(1024) k_make_cloud(CloudArgs A, uint32_t* stat
     __shared__ double4 cnd[CLOUD_SLOTS];
     __shared__ unsigned long long ovm[CLOUD_SLOTS][2];  // candidates of earlier tries it overlaps
     __shared__ int on[CLOUD_SLOTS], hit[CLOUD_SLOTS];
-    __shared__ int4 ccell[CLOUD_SLOTS];  // cell of each candidate
+    __shared__ int4 ccell[CLOUD_SLOTS];  // cell of each candidate; .w its try, INT_MAX when the slot is off
     __shared__ int loJ[CLOUD_W];         // lo_j of the window's tries (num > 0)
     __shared__ uint32_t st;
     __shared__ int64_t placed;
@@ -2313,8 +2313,8 @@ __global__ void __launch_bounds__(1024) k_make_cloud(CloudArgs A, uint32_t* stat
                     c[d] = A.periodic ? A.mn[d] + A.size[d] * u : (A.mn[d] + r) + (A.size[d] - 2 * r) * u;
                 }
                 cnd[s] = make_double4(c[0], c[1], c[2], r);
-                ccell[s] = make_int4(cloudCell(A, 0, c[0]), cloudCell(A, 1, c[1]), cloudCell(A, 2, c[2]), 0);
-            }
+                ccell[s] = make_int4(cloudCell(A, 0, c[0]), cloudCell(A, 1, c[1]), cloudCell(A, 2, c[2]), cloudTry(A, s));
+            } else ccell[s] = make_int4(0, 0, 0, INT_MAX);
             if (s < CLOUD_W) loJ[s] = cloudLo(p, s);
             on[s] = ok;
             hit[s] = 0;
@@ -2359,22 +2359,36 @@ __global__ void __launch_bounds__(1024) k_make_cloud(CloudArgs A, uint32_t* stat
         if (tid < CLOUD_SLOTS * 7) {
             const int s = tid / 7, ch = tid % 7;
             if (on[s]) {
-                const int ts = cloudTry(A, s);
-                const double4 cd = cnd[s];
                 const int4 cs = ccell[s];
                 unsigned long long bits = 0;
-                for (int b = 0; b < 16; b++) {
-                    const int q = ch * 16 + b;
-                    if (q >= CLOUD_SLOTS || !on[q] || cloudTry(A, q) >= ts) continue;
-                    // spheres that can touch lie in neighbouring cells (cells are at least 2 r_max wide)
-                    const int4 cq = ccell[q];
-                    const int dc[3] = {cq.x - cs.x, cq.y - cs.y, cq.z - cs.z};
-                    bool near = true;
-                    for (int d = 0; d < 3; d++) {
-                        const int ad = dc[d] < 0 ? -dc[d] : dc[d];
-                        near = near && (ad <= 1 || (A.periodic && (A.nc[d] < 3 || ad == A.nc[d] - 1)));
+                for (int h = 0; h < 2; h++) {  // 8 slots at a time: their loads are issued together
+                    int4 cq[8];
+#pragma unroll
+                    for (int b = 0; b < 8; b++) {
+                        const int q = ch * 16 + h * 8 + b;
+                        cq[b] = ccell[q < CLOUD_SLOTS ? q : CLOUD_SLOTS - 1];
                     }
-                    if (near && cloudOverlap(A, cnd[q], cd)) bits |= 1ull << (q & 63);
+                    unsigned near = 0;
+#pragma unroll
+                    for (int b = 0; b < 8; b++) {
+                        const int q = ch * 16 + h * 8 + b;
+                        // an earlier try's candidate; spheres that can touch lie in neighbouring cells
+                        // (cells are at least 2 r_max wide)
+                        const int dc[3] = {cq[b].x - cs.x, cq[b].y - cs.y, cq[b].z - cs.z};
+                        bool nb = q < CLOUD_SLOTS && cq[b].w < cs.w;
+                        for (int d = 0; d < 3; d++) {
+                            const int ad = dc[d] < 0 ? -dc[d] : dc[d];
+                            nb = nb && (ad <= 1 || (A.periodic && (A.nc[d] < 3 || ad == A.nc[d] - 1)));
+                        }
+                        near |= nb ? 1u << b : 0u;
+                    }
+                    if (near) {
+                        const double4 cd = cnd[s];
+                        for (int b = 0; b < 8; b++) {
+                            const int q = ch * 16 + h * 8 + b;
+                            if (((near >> b) & 1u) && cloudOverlap(A, cnd[q], cd)) bits |= 1ull << (q & 63);
+                        }
+                    }
                 }
                 if (bits) atomicOr(&ovm[s][ch >> 2], bits);
             }
