@@ -103,6 +103,77 @@ def gpu_make_cloud(ctx, num):
                 workload="SpherePack.makeCloud, num=%d, rRelFuzz .3, porosity .7, %.3f m cube, generator state 7" % (num, mx[0]))
 
 
+def scene_io(ctx, run):
+    """Checkpoint (woodem_amd.snapshot npz) save and restore, and VTK export, of the timed scene;
+    host files under a temporary folder that is removed afterwards."""
+    import shutil
+    import tempfile
+    from woodem_amd import snapshot
+    d = tempfile.mkdtemp(prefix="wdm_bench_")
+    try:
+        ctx.sync()
+        t = time.perf_counter()
+        f = snapshot.save(run, os.path.join(d, "scene.npz"))
+        ts = time.perf_counter() - t
+        size = os.path.getsize(f)
+        t = time.perf_counter()
+        r2 = snapshot.load(f, ctx=ctx)
+        ctx.sync()
+        tl = time.perf_counter() - t
+        r2.close()
+        t = time.perf_counter()
+        files = snapshot.vtk_export(run, os.path.join(d, "vtk", "scene."))
+        tv = time.perf_counter() - t
+        vsize = sum(os.path.getsize(x) for x in files.values())
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"checkpoint": {"bytes": size, "save_s": ts, "load_s": tl, "save_GBps": size / ts / 1e9, "load_GBps": size / tl / 1e9,
+                           "what": "nodes, particles, materials, contacts with their state, energy, cell, engine state (npz)"},
+            "vtk": {"bytes": vsize, "s": tv, "files": sorted(files), "format": "VTK XML, inline base64 binary"}}
+
+
+def insertion(ctx, n_spare=20000):
+    """Particles inserted in place (DemField.par.add during a run, ParticleContainer ids): a
+    200,000-sphere box (100 x 100 x 20 lattice, r 5 mm, gravity, walls) with spare slots gets
+    n_spare spheres in one wdm_insert_spheres call (two lattice layers above the packing); the
+    step after it runs a full collider pass (the new particles need bounds).  Also the
+    BoxOutlet selection (wdm_select_box) over the box's lower half."""
+    from woodem_amd import runner, scenes, soa
+    a = scenes.box_lattice(nx=100, ny=100, nz=20, box=((0., 0., 0.), (1.2, 1.2, 1.0)))
+    a = soa.with_spare(a, n_spare)
+    p = runner.params_from_meta(a)
+    run = runner.HipRun(a, p, runner.cell_from_meta(a), ctx=ctx)
+    run.step(20)
+    ctx.sync()
+    t = time.perf_counter()
+    run.step(1)
+    ctx.sync()
+    t_step = time.perf_counter() - t
+    r, rho = 5e-3, 3000.
+    m = 4. / 3. * math.pi * r ** 3 * rho
+    inertia = .4 * m * r * r
+    g = np.arange(100) * 11e-3 + .055
+    xx, yy, zz = np.meshgrid(g, g, [.6, .611], indexing="ij")
+    pos = np.stack([xx.ravel(), yy.ravel(), zz.ravel()], 1)[:n_spare]
+    t = time.perf_counter()
+    ids = ctx.insert_spheres(run.sid, pos, r, m, inertia, 0, soa.DEFAULT_MOVABLE_MASK)
+    ctx.sync()
+    t_ins = time.perf_counter() - t
+    full0 = run.stats()["nFullRuns"]
+    t = time.perf_counter()
+    run.step(1)
+    ctx.sync()
+    t_after = time.perf_counter() - t
+    full1 = run.stats()["nFullRuns"]
+    t = time.perf_counter()
+    sel = ctx.select_box(run.sid, (0., 0., 0.), (1.2, 1.2, .15), inside=True)
+    t_sel = time.perf_counter() - t
+    run.close()
+    return {"scene": "box_lattice 100x100x20 (200,000 spheres, 6 walls) + %d spare slots" % n_spare, "inserted": int(len(ids)),
+            "insert_s": t_ins, "inserted_per_s": len(ids) / t_ins, "step_before_s": t_step, "step_after_s": t_after,
+            "full_collider_runs_in_step_after": full1 - full0, "select_box_s": t_sel, "selected": int(len(sel))}
+
+
 def measured_traffic(args):
     """HBM bytes per k_contacts launch (read + write) of this workload, from the rocprofv3
     FETCH_SIZE/WRITE_SIZE runs recorded in profiles/ (scaled by a device copy of known
@@ -303,6 +374,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-io", action="store_true", help="skip the checkpoint/VTK and insertion measurements")
     ap.add_argument("--cloud-n", type=int, default=100_000, help="spheres of the timed device makeCloud (0: skip)")
     ap.add_argument("--cloud-cpu-sample", type=int, default=20_000, help="spheres of the oracle makeCloud sample")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
@@ -405,6 +477,11 @@ def main():
     log("kernels: %s" % json.dumps(kernels))
     gen = gpu_make_cloud(ctx, args.cloud_n) if args.cloud_n > 0 else None
     log("scene generation: %s" % json.dumps(gen))
+    io = None
+    if not args.no_io:
+        io = scene_io(ctx, run)
+        io["insertion"] = insertion(ctx)
+        log("scene I/O and insertion: %s" % json.dumps(io))
     log("contacts: %d potential, %d real, %d processed by the ContactLoop" % (n_con, n_real, n_active))
 
     mixed = None
@@ -464,6 +541,7 @@ def main():
                      "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_ms},
         "kernels": kernels,
         "scene_generation": gen,
+        "scene_io": io,
         "mixed_precision": mixed,
         "cpu_baseline": cpu,
     }

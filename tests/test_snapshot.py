@@ -87,6 +87,24 @@ def test_vtk_export_periodic_wraps_points(tmp_path):
     assert np.array_equal(s["Points"], want)
 
 
+def test_vtk_binary_and_ascii_agree(tmp_path):
+    # the inline binary form (base64 of a UInt32 byte count and the little-endian values) holds
+    # the same numbers as the ascii form, bit for bit
+    a = scenes.box_lattice(nx=5, ny=4, nz=4)
+    p, cell = runner.params_from_meta(a), runner.cell_from_meta(a)
+    o = OL.OracleRun(a, p, cell)
+    o.step(2600)
+    run = OracleRunView(a, cell, o)
+    fb = snapshot.vtk_export(run, str(tmp_path / "b."))
+    fa = snapshot.vtk_export(run, str(tmp_path / "a."), binary=False)
+    for part in ("spheres", "con"):
+        assert 'format="binary"' in open(fb[part]).read() and 'format="ascii"' in open(fa[part]).read()
+        b, c = snapshot.read_vtk_arrays(fb[part]), snapshot.read_vtk_arrays(fa[part])
+        assert sorted(b) == sorted(c)
+        for k in c:
+            assert np.array_equal(b[k], c[k]), (part, k)
+
+
 def test_checkpoint_rejects_other_files(tmp_path):
     f = tmp_path / "x.npz"
     np.savez(f, meta=np.array('{"schema": "something else"}'))

@@ -38,8 +38,10 @@ the contact point for walls and facets) with cell data Fn and |Ft| (`{out}con.{s
 (Cell::canonicalizePt) and a contact's line starts at the wrapped point of its first
 particle.
 """
+import base64
 import json
 import os
+import struct
 
 import numpy as np
 
@@ -154,7 +156,16 @@ def _ifmt(x):
     return " ".join("%d" % v for v in np.asarray(x).ravel())
 
 
-def _arr(name, data, ncomp, kind="Float64"):
+_VTK_DTYPES = {"Float64": "<f8", "Float32": "<f4", "Int32": "<i4", "Int64": "<i8", "UInt8": "u1", "UInt32": "<u4"}
+
+
+def _arr(name, data, ncomp, kind="Float64", binary=True):
+    """A DataArray: VTK's inline binary form (base64 of a UInt32 byte count followed by the
+    little-endian values) or ascii"""
+    if binary:
+        raw = np.ascontiguousarray(np.asarray(data).ravel(), _VTK_DTYPES[kind]).tobytes()
+        body = base64.b64encode(struct.pack("<I", len(raw)) + raw).decode("ascii")
+        return '<DataArray type="%s" Name="%s" NumberOfComponents="%d" format="binary">%s</DataArray>' % (kind, name, ncomp, body)
     body = _fmt(data) if kind == "Float64" else _ifmt(data)
     return '<DataArray type="%s" Name="%s" NumberOfComponents="%d" format="ascii">%s</DataArray>' % (kind, name, ncomp, body)
 
@@ -165,9 +176,13 @@ def _wrap(pos, h):
     return pos - np.floor(pos / L) * L
 
 
-def vtk_export(run, out, what=("spheres", "mesh", "con"), sphereSphereOnly=False, nanValue=0.):
+def vtk_export(run, out, what=("spheres", "mesh", "con"), sphereSphereOnly=False, nanValue=0., binary=True):
     """Write the scene's current state as VTK XML files named like VtkExport's; returns
-    {'spheres': path, 'mesh': path, 'con': path} for the parts written."""
+    {'spheres': path, 'mesh': path, 'con': path} for the parts written.  Arrays are inline
+    base64 binary (ascii with binary=False)."""
+    def arr(*x, **k):  # every array of this export in the chosen form
+        return _arr(*x, binary=binary, **k)
+
     ctx, sid, a = run.ctx, run.sid, run.a
     step = ctx.stats(sid)["step"]
     h, _ = ctx.cell(sid)
@@ -187,12 +202,12 @@ def vtk_export(run, out, what=("spheres", "mesh", "con"), sphereSphereOnly=False
         nodes = a.nodes[sph, 0]
         xml = ['<?xml version="1.0"?>', '<VTKFile type="UnstructuredGrid" version="0.1" byte_order="LittleEndian">',
                "<UnstructuredGrid>", '<Piece NumberOfPoints="%d" NumberOfCells="%d">' % (n, n),
-               "<Points>" + _arr("Points", spos, 3) + "</Points>",
-               "<Cells>" + _arr("connectivity", np.arange(n), 1, "Int32") + _arr("offsets", np.arange(1, n + 1), 1, "Int32")
-               + _arr("types", np.ones(n, np.int32), 1, "UInt8") + "</Cells>",
-               "<PointData>", _arr("radius", a.radius[sph], 1), _arr("mass", a.mass[nodes], 1), _arr("id", sph, 1, "Int32"),
-               _arr("mask", a.mask[sph], 1, "Int32"), _arr("vel", nd["vel"][nodes], 3), _arr("angVel", nd["angVel"][nodes], 3),
-               _arr("matId", a.mat[sph], 1, "Int32"), "</PointData>", "</Piece>", "</UnstructuredGrid>", "</VTKFile>"]
+               "<Points>" + arr("Points", spos, 3) + "</Points>",
+               "<Cells>" + arr("connectivity", np.arange(n), 1, "Int32") + arr("offsets", np.arange(1, n + 1), 1, "Int32")
+               + arr("types", np.ones(n, np.int32), 1, "UInt8") + "</Cells>",
+               "<PointData>", arr("radius", a.radius[sph], 1), arr("mass", a.mass[nodes], 1), arr("id", sph, 1, "Int32"),
+               arr("mask", a.mask[sph], 1, "Int32"), arr("vel", nd["vel"][nodes], 3), arr("angVel", nd["angVel"][nodes], 3),
+               arr("matId", a.mat[sph], 1, "Int32"), "</PointData>", "</Piece>", "</UnstructuredGrid>", "</VTKFile>"]
         fn = "%sspheres.%d.vtu" % (out, step)
         with open(fn, "w") as f:
             f.write("\n".join(xml) + "\n")
@@ -206,10 +221,10 @@ def vtk_export(run, out, what=("spheres", "mesh", "con"), sphereSphereOnly=False
         nf = len(fac)
         xml = ['<?xml version="1.0"?>', '<VTKFile type="UnstructuredGrid" version="0.1" byte_order="LittleEndian">',
                "<UnstructuredGrid>", '<Piece NumberOfPoints="%d" NumberOfCells="%d">' % (len(vn), nf),
-               "<Points>" + _arr("Points", pos[vn], 3) + "</Points>",
-               "<Cells>" + _arr("connectivity", conn, 1, "Int32") + _arr("offsets", 3 * np.arange(1, nf + 1), 1, "Int32")
-               + _arr("types", np.full(nf, 5, np.int32), 1, "UInt8") + "</Cells>",
-               "<CellData>", _arr("vel", vel, 3), _arr("matId", a.mat[fac], 1, "Int32"), "</CellData>",
+               "<Points>" + arr("Points", pos[vn], 3) + "</Points>",
+               "<Cells>" + arr("connectivity", conn, 1, "Int32") + arr("offsets", 3 * np.arange(1, nf + 1), 1, "Int32")
+               + arr("types", np.full(nf, 5, np.int32), 1, "UInt8") + "</Cells>",
+               "<CellData>", arr("vel", vel, 3), arr("matId", a.mat[fac], 1, "Int32"), "</CellData>",
                "</Piece>", "</UnstructuredGrid>", "</VTKFile>"]
         fn = "%smesh.%d.vtu" % (out, step)
         with open(fn, "w") as f:
@@ -240,10 +255,10 @@ def vtk_export(run, out, what=("spheres", "mesh", "con"), sphereSphereOnly=False
         Ft = np.nan_to_num(np.sqrt(fl[:, 1] ** 2 + fl[:, 2] ** 2), nan=nanValue)
         xml = ['<?xml version="1.0"?>', '<VTKFile type="PolyData" version="0.1" byte_order="LittleEndian">', "<PolyData>",
                '<Piece NumberOfPoints="%d" NumberOfVerts="0" NumberOfLines="%d" NumberOfStrips="0" NumberOfPolys="0">' % (2 * m, m),
-               "<Points>" + _arr("Points", pts, 3) + "</Points>",
-               "<Lines>" + _arr("connectivity", np.arange(2 * m), 1, "Int32") + _arr("offsets", 2 * np.arange(1, m + 1), 1, "Int32")
+               "<Points>" + arr("Points", pts, 3) + "</Points>",
+               "<Lines>" + arr("connectivity", np.arange(2 * m), 1, "Int32") + arr("offsets", 2 * np.arange(1, m + 1), 1, "Int32")
                + "</Lines>",
-               "<CellData>", _arr("Fn", Fn, 1), _arr("|Ft|", Ft, 1), "</CellData>", "</Piece>", "</PolyData>", "</VTKFile>"]
+               "<CellData>", arr("Fn", Fn, 1), arr("|Ft|", Ft, 1), "</CellData>", "</Piece>", "</PolyData>", "</VTKFile>"]
         fn = "%scon.%d.vtp" % (out, step)
         with open(fn, "w") as f:
             f.write("\n".join(xml) + "\n")
@@ -257,7 +272,13 @@ def read_vtk_arrays(path):
     out = {}
     for e in ET.parse(path).getroot().iter("DataArray"):
         kind = e.get("type")
-        vals = np.array((e.text or "").split(), dtype=np.float64 if kind == "Float64" else np.int64)
+        if e.get("format") == "binary":
+            raw = base64.b64decode((e.text or "").strip())
+            nbytes = struct.unpack("<I", raw[:4])[0]
+            vals = np.frombuffer(raw[4:4 + nbytes], _VTK_DTYPES[kind])
+            vals = vals.astype(np.float64 if kind.startswith("Float") else np.int64)
+        else:
+            vals = np.array((e.text or "").split(), dtype=np.float64 if kind == "Float64" else np.int64)
         nc = int(e.get("NumberOfComponents", "1"))
         out[e.get("Name")] = vals.reshape(-1, nc) if nc > 1 else vals
     return out
