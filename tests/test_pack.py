@@ -251,3 +251,39 @@ def test_gpu_cloud_to_simulation_steps_like_oracle():
         assert np.array_equal(S.dem.par[i].pos, no["pos"][i])
     assert S.dem.con.countReal() == int(o.contacts()["isReal"].sum()) > 0
     pack.reset_generator()
+
+
+# ---------------- SpherePack text files (SpherePack.cpp:73-129) ----------------
+def test_sphere_pack_file_format(tmp_path):
+    from woodem_amd import pack
+    sp = pack.SpherePack()
+    sp.c = np.array([[0.0123456789, 1e-5, 2.], [1., 2.5, 3.25]])
+    sp.r = np.array([0.001, 1234567.])
+    sp.cellSize = np.array([1., 2., 3.5])
+    sp.userData = "made by a test"
+    f = str(tmp_path / "a.txt")
+    sp.save(f)
+    # C++ streams' default format: 6 significant digits, exponent form where %g uses it
+    assert open(f).read() == ("##PERIODIC:: 1 2 3.5\n##USER-DATA:: made by a test\n"
+                              "0.0123457 1e-05 2 0.001\n1 2.5 3.25 1.23457e+06\n")
+    sp2 = pack.SpherePack()
+    sp2.load(f)
+    assert sp2.userData == "made by a test" and np.array_equal(sp2.cellSize, [1., 2., 3.5])
+    assert np.array_equal(sp2.c, [[0.0123457, 1e-5, 2.], [1., 2.5, 3.25]]) and np.array_equal(sp2.r, [0.001, 1.23457e6])
+    assert not sp2.hasClumps()
+    g = tmp_path / "b.txt"
+    g.write_text("0 0 0 1 3\n\t1 1 1 .5 3\n\n2 2 2 .25 -1\n")  # tabs and blank lines are skipped
+    sp3 = pack.SpherePack()
+    sp3.loadTxt(str(g))
+    assert list(sp3.clumpId) == [3, 3, -1] and sp3.hasClumps()
+    sp3.saveTxt(str(tmp_path / "c.txt"))
+    assert open(str(tmp_path / "c.txt")).read() == "0 0 0 1 3\n1 1 1 0.5 3\n2 2 2 0.25 -1\n"
+    bad = tmp_path / "d.txt"
+    bad.write_text("0 0 0 1\n1 1 1 1 2\n")
+    with pytest.raises(ValueError):
+        sp3.load(str(bad))
+    bad.write_text("##PERIODIC:: 1 2\n")
+    with pytest.raises(ValueError):
+        sp3.load(str(bad))
+    with pytest.raises(ValueError):
+        sp3.load(str(tmp_path / "missing.txt"))

@@ -37,8 +37,10 @@ class SpherePack:
     def __init__(self):
         self.c = np.zeros((0, 3))
         self.r = np.zeros(0)
+        self.clumpId = np.zeros(0, np.int64)  # -1: not in a clump
         self.cellSize = np.zeros(3)
         self.appliedPsdScaling = 1.
+        self.userData = ""
 
     def __len__(self):
         return len(self.r)
@@ -74,8 +76,69 @@ class SpherePack:
         c, r, sc = ctx.make_cloud(mn, mx, rMean, rRelFuzz, num, periodic, porosity, psdSizes, psdCumm, distributeMass, st, int(cap))
         _state = int(st[0])
         self.c, self.r, self.appliedPsdScaling = c, r, sc
+        self.clumpId = np.full(len(r), -1, np.int64)
         self.cellSize = (mx - mn) if periodic else np.zeros(3)
         return len(r)
+
+    def hasClumps(self):
+        return bool(len(self.clumpId)) and bool(np.any(self.clumpId >= 0))
+
+    def save(self, fileName):
+        """SpherePack::toFile (SpherePack.cpp:114-129): optional `##PERIODIC:: x y z` and
+        `##USER-DATA:: ...` lines, then `x y z r` (and the clump id when any sphere has one) per
+        sphere, numbers as C++ streams write them by default (%g, 6 significant digits)."""
+        if "\n" in self.userData:
+            raise RuntimeError("SpherePack.userData must not contain newline.")
+        clumps = self.hasClumps()
+        with open(fileName, "w") as f:
+            if np.any(self.cellSize != 0):
+                f.write("##PERIODIC:: %g %g %g\n" % tuple(self.cellSize))
+            if self.userData:
+                f.write("##USER-DATA:: %s\n" % self.userData)
+            for i in range(len(self.r)):
+                f.write("%g %g %g %g" % (self.c[i, 0], self.c[i, 1], self.c[i, 2], self.r[i]))
+                if clumps:
+                    f.write(" %d" % self.clumpId[i])
+                f.write("\n")
+
+    saveTxt = save
+
+    def load(self, fileName):
+        """SpherePack::fromFile (SpherePack.cpp:73-112): the current spheres are replaced; every
+        sphere line has 4 columns, or 5 with a clump id, all lines alike."""
+        import os
+        if not os.path.exists(fileName):
+            raise ValueError("File with spheres `%s' doesn't exist." % fileName)
+        c, r, cid, ncols = [], [], [], 0
+        self.cellSize = np.zeros(3)
+        with open(fileName) as f:
+            for lineNo, line in enumerate(f, 1):
+                line = line.rstrip("\n")
+                if line.startswith("##USER-DATA:: "):
+                    self.userData = line[14:]
+                    continue
+                tok = line.replace("\t", " ").split()
+                if not tok:
+                    continue
+                if tok[0] == "##PERIODIC::":
+                    if len(tok) != 4:
+                        raise ValueError("Spheres file %s:%d contains ##PERIODIC::, but the line is malformed." % (fileName, lineNo))
+                    self.cellSize = np.array([float(x) for x in tok[1:]])
+                    continue
+                if len(tok) not in (4, 5):
+                    raise ValueError("%s:%d: line has %d columns (must be 4 or 5)." % (fileName, lineNo, len(tok)))
+                ncols = ncols or len(tok)
+                if len(tok) != ncols:
+                    raise ValueError("%s:%d: all line must have the same number of columns (previous had %d, this one has %d)"
+                                     % (fileName, lineNo, ncols, len(tok)))
+                c.append([float(x) for x in tok[:3]])
+                r.append(float(tok[3]))
+                cid.append(int(tok[4]) if len(tok) == 5 else -1)
+        self.c = np.array(c, np.float64).reshape(-1, 3)
+        self.r = np.array(r, np.float64)
+        self.clumpId = np.array(cid, np.int64)
+
+    loadTxt = load
 
     def toArrays(self, density=1e3, young=1e7, ktDivKn=.2, tanPhi=.5, mask=soa.DEFAULT_MOVABLE_MASK):
         """the spheres as scene arrays (free spheres, one FrictMat)"""
