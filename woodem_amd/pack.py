@@ -145,7 +145,10 @@ class SpherePack:
         return soa.spheres_arrays(self.c, self.r, density=density, young=young, ktDivKn=ktDivKn, tanPhi=tanPhi, mask=mask)
 
     def toSimulation(self, S, mat=None, mask=None):
-        """add the spheres to S.dem (SpherePack.toSimulation, py/pack.py); returns their ids"""
+        """add the spheres to S.dem (SpherePack.toSimulation, py/pack.py); returns their ids.
+        Clumped spheres (clump ids) are not supported by this front end: they raise."""
+        if self.hasClumps():
+            raise NotImplementedError("toSimulation of a SpherePack with clumps")
         from . import utils
         from .dem import DemField
         kw = dict(mask=DemField.defaultMovableMask if mask is None else mask)
