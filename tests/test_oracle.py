@@ -3,7 +3,9 @@
 Reference sources: py/utils.py:41-42 (pWaveDt golden), py/tests/hertz.py:12-46 (Hertz
 collision time), py/tests/pbc.py:35-105 (cell integration, incident velocity),
 py/tests/energy.py:70-90 (energy keys), pkg/dem/Sphere.cpp:24-31 (mass/inertia),
-InsertionSortCollider.cpp:52-58 (closed-interval overlap; checked by brute force).
+InsertionSortCollider.cpp:52-58 (closed-interval overlap; checked by brute force); multi-step
+L6Geom frame and IdealElPl shear history against rigid-body closed forms (a sphere spinning on a
+wall, a sliding sphere that starts rolling at 5/7 of its speed).
 """
 import math
 
@@ -317,3 +319,66 @@ def test_energy_balance_box():
     tot = float(np.sum(v))
     s = float(np.sum(np.abs(v)))
     assert s > 0 and abs(tot) / s < 2e-2
+
+
+def _sphere_on_floor(r=.01, density=1e3, young=1e7, tanPhi=.3, ktDivKn=.2, g=9.81):
+    """one sphere resting on the wall z=0 at its static overlap m g / kn"""
+    m = 4. / 3. * math.pi * r ** 3 * density
+    l = r
+    kn = young * math.pi * r * r / (2 * l)
+    a = soa.spheres_arrays(np.array([[.1, .1, r - m * g / kn]]), r, density=density, young=young, tanPhi=tanPhi, ktDivKn=ktDivKn)
+    return soa.append_walls(a, [(0., 2, 1)]), m
+
+
+def test_l6geom_twist_follows_the_mean_spin():
+    """L6Geom.trsf of a wall-sphere contact whose sphere spins about the normal (L6Geom.cpp:80-118):
+    the normal stays e_z, the contact point does not slide (its lever arm is parallel to the
+    spin), and the local frame turns about the normal by half the sum of the two bodies' spins
+    (normTwistVec = n dt/2 n.(w1+w2)): after N steps column 1, initially -e_x
+    (setInitialLocalCoords), is (-cos p, -sin p, 0), p = N w dt / 2, with the renormalisation
+    every 100 steps.  A wrong sign, a transposed update or the full spin instead of the mean
+    fails it."""
+    a, m = _sphere_on_floor()
+    w, dt, N = 10., 1e-5, 20000
+    a.angVel[0] = (0., 0., w)
+    o = OL.OracleRun(a, runner.make_params(dt=dt, damping=0., gravity=(0, 0, -9.81)), NOCELL)
+    o.step(1)  # the contact is created (trsf from setInitialLocalCoords)
+    t0 = o.contacts()["trsf"][0]
+    assert np.allclose(t0[:, 1], [-1, 0, 0], atol=1e-15) and np.allclose(t0[:, 0], [0, 0, 1], atol=0)
+    o.step(N)
+    c = o.contacts()
+    assert c["isReal"][0]
+    t = c["trsf"][0]
+    p = N * w * dt / 2
+    assert np.allclose(t[:, 0], [0, 0, 1], atol=1e-15)
+    assert np.allclose(t[:, 1], [-math.cos(p), -math.sin(p), 0], atol=1e-8)
+    assert np.allclose(t[:, 2], [math.sin(p), -math.cos(p), 0], atol=1e-8)
+    assert np.allclose(t.T @ t, np.eye(3), atol=1e-12)
+    assert o.nodes()["angVel"][0, 2] == w  # no torque about the normal: the spin is kept exactly
+
+
+def test_sliding_sphere_rolls_at_five_sevenths():
+    """IdealElPl shear history over thousands of steps (IdealElPl.cpp:40-77): a solid sphere
+    launched at v0 without spin on a floor with Coulomb friction tanPhi slides with deceleration
+    tanPhi g while friction spins it up (torque r F, inertia 2/5 m r^2), until it rolls at
+    5/7 v0 after t* = 2 v0 / (7 tanPhi g); then it keeps rolling (w r = v)."""
+    mu, v0, dt = .3, .1, 2e-5
+    a, m = _sphere_on_floor(tanPhi=mu)
+    a.vel[0] = (v0, 0., 0.)
+    o = OL.OracleRun(a, runner.make_params(dt=dt, damping=0., gravity=(0, 0, -9.81)), NOCELL)
+    ts = 2 * v0 / (7 * mu * 9.81)
+    nh = int(round(ts / 2 / dt))
+    o.step(nh)
+    vh = o.nodes()["vel"][0, 0]
+    assert vh == pytest.approx(v0 - mu * 9.81 * nh * dt, rel=.02)  # still sliding, decelerating at mu g
+    o.step(int(round(3 * ts / dt)))
+    vs, ws = [], []
+    for _ in range(200):
+        o.step(5)
+        nd = o.nodes()
+        vs.append(nd["vel"][0, 0])
+        ws.append(nd["angVel"][0, 1])
+    v, w = np.mean(vs), np.mean(ws)
+    assert v == pytest.approx(5 / 7. * v0, rel=.01)
+    assert w * .01 == pytest.approx(v, rel=.01)
+    assert o.contacts()["isReal"][0]
