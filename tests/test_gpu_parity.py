@@ -203,3 +203,36 @@ def test_drum_parity(hip_ctx):
     assert np.array_equal(cg, cr)
     assert np.allclose(vg, vo, rtol=1e-9, atol=1e-12 * np.abs(vo).max())
     g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["twist", "rolling"])
+def test_closed_form_scenes_equal_oracle(hip_ctx, case):
+    # the scenes that pin the oracle's multi-step contact history to rigid-body closed forms
+    # (a sphere spinning on a wall; a sliding sphere that starts to roll): the device's
+    # trajectories and contact frames are the oracle's, bit for bit
+    import math
+    from test_oracle import NOCELL, _sphere_on_floor
+    a, m = _sphere_on_floor(tanPhi=.3)
+    if case == "twist":
+        a.angVel[0] = (0., 0., 10.)
+        dt, n = 1e-5, 20000
+    else:
+        a.vel[0] = (.1, 0., 0.)
+        dt, n = 2e-5, 3000
+    p = runner.make_params(dt=dt, damping=0., gravity=(0, 0, -9.81))
+    g = runner.HipRun(a, p, NOCELL, ctx=hip_ctx)
+    o = OL.OracleRun(a, p, NOCELL)
+    g.step(n)
+    o.step(n)
+    ng, no = g.nodes(), o.nodes()
+    for q in ("pos", "vel", "angVel", "ori", "force", "torque"):
+        assert np.array_equal(ng[q], no[q]), q
+    cg, co = g.contacts(), o.contacts()
+    assert np.array_equal(cg["trsf"], co["trsf"]) and np.array_equal(cg["forceLocal"], co["forceLocal"])
+    if case == "rolling":
+        assert ng["angVel"][0, 1] * .01 == pytest.approx(5 / 7. * .1, rel=.02)
+    else:
+        ph = (n - 1) * 10. * dt / 2  # the contact's frame is created in the first step, then turns in each later one
+        assert np.allclose(cg["trsf"][0][:, 1], [-math.cos(ph), -math.sin(ph), 0], atol=1e-8)
+    g.close()
