@@ -1632,26 +1632,42 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
     Ctx* c = S.ctx;
     ConBuf<T>& b = cb[S.cur];
     int64_t n = S.nc;
-    std::vector<int2> ids(n);
-    std::vector<uint32_t> info(n);
-    std::vector<int32_t> cd(n), slot(n), itm(n);
-    std::vector<T> st((size_t)NSTATE * std::max<int64_t>(b.cap, 1)), st2((size_t)NSTRIDE * std::max<int64_t>(b.cap, 1));
-    std::vector<double4> gfc(2 * (size_t)n);
-    int32_t nR = 0;
+    // only the rows and items in use are copied (rows of the full run: nR; items: nI; the
+    // buffers' capacity can be far larger), into host buffers that are not cleared first
+    int32_t nR = 0, nI = 0;
     if (n > 0) {
-        HIPCHK(hipMemcpyAsync(ids.data(), b.ids.p, n * sizeof(int2), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(info.data(), b.info.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(cd.data(), b.cd.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(slot.data(), b.slot.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(itm.data(), b.itm.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(st.data(), b.st.p, (size_t)NSTATE * b.cap * sizeof(T), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(st2.data(), b.st2.p, (size_t)NSTRIDE * b.cap * sizeof(T), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(&nR, b.counter.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(gfc.data(), b.gfc.p, 2 * n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));  // items <= contacts
+        HIPCHK(hipMemcpyAsync(&nI, b.nItems.p, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (nR < 0 || nI < 0 || nR > b.cap || nI > b.cap) throw WdmError(WDM_E_STATE, "contact list counters out of range");
+    }
+    // rows of the items after the full run's rows (none right after a checkpoint upload, which
+    // hands out rows only and builds the items at the next full run)
+    const int32_t nTail = nI > nR ? nI - nR : 0;
+    const size_t nn = (size_t)std::max<int64_t>(n, 1);
+    std::unique_ptr<int2[]> ids(new int2[nn]);
+    std::unique_ptr<uint32_t[]> info(new uint32_t[nn]);
+    std::unique_ptr<int32_t[]> cd(new int32_t[nn]), slot(new int32_t[nn]), itm(new int32_t[nn]);
+    std::unique_ptr<T[]> st(new T[(size_t)NSTATE * std::max<int32_t>(nR, 1)]), st2(new T[(size_t)NSTRIDE * std::max<int32_t>(nTail, 1)]);
+    std::unique_ptr<double4[]> gfc(new double4[2 * (size_t)std::max<int32_t>(nI, 1)]);
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(ids.get(), b.ids.p, n * sizeof(int2), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(info.get(), b.info.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(cd.get(), b.cd.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(slot.get(), b.slot.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(itm.get(), b.itm.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        if (nR > 0)  // component-major: value q of the first nR rows
+            HIPCHK(hipMemcpy2DAsync(st.get(), (size_t)nR * sizeof(T), b.st.p, (size_t)b.cap * sizeof(T), (size_t)nR * sizeof(T), NSTATE,
+                                    hipMemcpyDeviceToHost, c->stream));
+        if (nTail > 0) HIPCHK(hipMemcpyAsync(st2.get(), b.st2.p, (size_t)NSTRIDE * nTail * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+        if (nI > 0) HIPCHK(hipMemcpyAsync(gfc.get(), b.gfc.p, 2 * (size_t)nI * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
-    // state value q of a row (the layout of the kernels' stAt)
-    auto stv = [&](int64_t row, int q) -> T { return row < nR ? st[(size_t)q * b.cap + row] : st2[(size_t)(row - nR) * NSTRIDE + q]; };
+    // state value q of a row (the layout of the kernels' stAt, compacted to the rows in use)
+    auto stv = [&](int64_t row, int q) -> T {
+        if (row < 0 || row >= (int64_t)nR + nTail) throw WdmError(WDM_E_STATE, "contact state row out of range");
+        return row < nR ? st[(size_t)q * nR + row] : st2[(size_t)(row - nR) * NSTRIDE + q];
+    };
     int64_t k = 0;
     for (int64_t i = 0; i < n && k < cap; i++) {
         if (info[i] & CI_DEAD) continue;
@@ -1667,7 +1683,8 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
         if (phys5) for (int d = 0; d < 5; d++) phys5[5 * k + d] = real ? (double)stv(row, S_P + d) : NAN;
         if (lens2) for (int d = 0; d < 2; d++) lens2[2 * k + d] = real ? (double)stv(row, S_L + d) : NAN;
         // contact point and uN exist for real contacts only (L6Geom); zeros otherwise
-        const double4 rec = (real && itm[i] >= 0) ? gfc[2 * (size_t)itm[i] + 1] : make_double4(0., 0., 0., 0.);
+        // (an item beyond the list: the items are built at the next full run)
+        const double4 rec = (real && itm[i] >= 0 && itm[i] < nI) ? gfc[2 * (size_t)itm[i] + 1] : make_double4(0., 0., 0., 0.);
         if (contPt3) { contPt3[3 * k] = rec.x; contPt3[3 * k + 1] = rec.y; contPt3[3 * k + 2] = rec.z; }
         if (uN) uN[k] = real ? rec.w : NAN;
         if (trsf9) {  // columns 0 and 1 stored, column 2 = col0 x col1 in the state's precision
