@@ -287,3 +287,23 @@ def test_sphere_pack_file_format(tmp_path):
         sp3.load(str(bad))
     with pytest.raises(ValueError):
         sp3.load(str(tmp_path / "missing.txt"))
+
+
+def test_pack_refusals_before_any_device_call():
+    # the host checks that run before the library is asked for anything
+    from woodem_amd import pack
+    with pytest.raises(NotImplementedError):
+        pack.regularOrtho(object(), .01, 0.)  # a predicate outside inAlignedBox/inSphere/inCylinder
+    sp = pack.SpherePack()
+    with pytest.raises(NotImplementedError):
+        sp.makeCloud((0, 0, 0), (1, 1, 1), .1, hSize=np.diag([1., 1., 1.]) + .1)  # sheared cells
+    sp.c, sp.r = np.zeros((1, 3)), np.ones(1)
+    with pytest.raises(NotImplementedError):
+        sp.makeCloud((0, 0, 0), (1, 1, 1), .1)  # into a non-empty pack
+    sp.clumpId = np.array([0])
+    with pytest.raises(NotImplementedError):
+        sp.toSimulation(None)  # clumps
+    b = pack.inAlignedBox((0, 0, 0), (1, 2, 3))
+    assert b.kind == 0 and b.params == (0., 0., 0., 1., 2., 3.) and b.aabb() == ((0., 0., 0.), (1., 2., 3.))
+    assert pack.inSphere((1, 2, 3), .5).params == (1., 2., 3., .5)
+    assert pack.inCylinder((0, 0, 0), (0, 0, 1), .2).params == (0., 0., 0., 0., 0., 1., .2)
