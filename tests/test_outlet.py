@@ -93,7 +93,6 @@ def test_select_box_semantics_in_python():
 
 # ---------------- HIP path against the oracle ----------------
 @pytest.mark.gpu
-@pytest.mark.skip(reason="not run on the GPU again this round: wdm_remove_particles aborted twice on MI355X; the cause, shown on the CPU, was kernels with address-space-qualified parameters, mangled differently in the host and device passes (the host launched a name the code object lacks); kernels now take plain pointers")
 def test_gpu_remove_insert_equals_oracle():
     from woodem_amd._native import Context
     ctx = Context(0)
@@ -131,7 +130,6 @@ def test_gpu_remove_insert_equals_oracle():
 
 
 @pytest.mark.gpu
-@pytest.mark.skip(reason="not run on the GPU again this round: wdm_remove_particles aborted twice on MI355X; the cause, shown on the CPU, was kernels with address-space-qualified parameters, mangled differently in the host and device passes (the host launched a name the code object lacks); kernels now take plain pointers")
 def test_gpu_box_outlet_scene_equals_oracle():
     """spheres falling through an open floor; a BoxOutlet deletes those below it every 20 steps"""
     import woodem_amd as woo
