@@ -5,10 +5,13 @@ Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
 Parity-pinning status of the oracle's functions (details in DESIGN.md):
   pinned by reference known answers / closed forms: pWaveDt, Sphere mass/inertia,
     bounds, contact set (vs brute force), sphere/wall/facet L6Geom creation,
-    FrictPhys, IdealElPl slip cap and dissipation, Hertz collision time,
-    restitution, ballistic leapfrog, damping, GRADV2 cell, incident velocity
-    under PBC, energy balance, DynDt (single-contact stiffness closed forms,
-    maxRelInc cap, dryRun, PeriodicEngine activation);
+    FrictPhys, IdealElPl slip cap, plastic dissipation (friction work per step) and
+    elastic energy, Hertz collision time, restitution, Hertz/DMT elast (Hertz
+    potential, also against the elastic force's work), viscN, viscT and plast (dashpot
+    work, released shear-spring energy) step by step, collision energy balance,
+    ballistic leapfrog, damping, rotation composition about non-parallel axes, GRADV2
+    cell, incident velocity under PBC, energy balance, DynDt (single-contact stiffness
+    closed forms, maxRelInc cap, dryRun, PeriodicEngine activation);
   parity unpinned (no reference vector exists; checked only by invariants):
     multi-step L6Geom trsf evolution with renormalisation, IdealElPl Ft history
     over many steps, summation order of forces and energies.
