@@ -154,6 +154,13 @@ int wdm_leapfrog_run(wdm_ctx* ctx, int32_t scene);
 int wdm_collider_run(wdm_ctx* ctx, int32_t scene);
 int wdm_contact_loop_run(wdm_ctx* ctx, int32_t scene);
 int wdm_step_end(wdm_ctx* ctx, int32_t scene);
+/* A scene whose kernels raise an error (NaN force, oversized particle) stops, as a Woo scene's
+   run thread stops on an exception and reports it at Scene.wait() (core/Scene.cpp:90-95): in a
+   wdm_step over several scenes the others take all their steps, and the call then returns the
+   stopped scene's code.  The stopped scene refuses further steps (WDM_E_STATE) and can still
+   be read back; free it with wdm_scene_free.  *code = its WDM_E_* code (0: running);
+   wdm_last_error() then holds the message. */
+int wdm_scene_error(wdm_ctx* ctx, int32_t scene, int32_t* code);
 /* DynDt::run (pkg/dem/DynDt.cpp:72-97) when PeriodicEngine::isActivated (core/Engine.cpp:169-195)
    says so at the current step: the engine call after ContactLoop (see wdm_set_dyndt) */
 int wdm_dyndt_run(wdm_ctx* ctx, int32_t scene);

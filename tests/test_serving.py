@@ -35,10 +35,21 @@ class FakeCtx:
     def set_params(self, sid, p):
         pass
 
+    fail_at = {}  # sid -> number of the step call in which that scene stops with a device error
+    failed = ()
+
     def step(self, sids, n):
         assert len(set(sids)) == len(sids) and all(s in self.res for s in sids)
+        assert not any(s in self.failed for s in sids), "a stopped scene was stepped again"
         self.calls.append((tuple(sids), n))
         self.clock.t += n * self.k * sum(self.res[s] for s in sids) * 1e-3
+        bad = [s for s in sids if self.fail_at.get(s) == len(self.calls)]
+        if bad:
+            self.failed = tuple(self.failed) + tuple(bad)
+            raise RuntimeError("WDM_E_NAN (-4): scene(s) %s stopped by a device error" % bad)
+
+    def scene_error(self, sid):
+        return (-4, "NaN force in contact") if sid in self.failed else (0, "")
 
     def download_nodes(self, sid, n_node):
         return {"sid": sid}
@@ -116,11 +127,54 @@ def test_uneven_step_counts_finish_exactly():
     assert all(r.done_steps == 250 for r in done)
 
 
-def test_oversized_request_rejected():
+def test_oversized_request_rejected_alone():
+    """a request larger than the whole budget ends with an error; the others are served"""
     clk = Clock()
-    srv = serving.Server(FakeCtx(clk), hbm_budget=1000, chunk=10, clock=clk, sleep=clk.sleep)
-    with pytest.raises(ValueError):
-        srv.serve(reqs([10], [0.]))
+    ctx = FakeCtx(clk)
+    srv = serving.Server(ctx, hbm_budget=10000 * 1000, chunk=10, clock=clk, sleep=clk.sleep)
+    done = srv.serve(reqs([10, 20000, 30], [0., 0., .001], steps=50))
+    by = {r.rid: r for r in done}
+    assert by[1].error and "budget" in by[1].error and by[1].result is None and by[1].sid is None
+    assert by[0].error is None and by[2].error is None and by[0].done_steps == by[2].done_steps == 50
+    assert not ctx.res  # every slot freed
+    rep = srv.report()
+    assert rep["requests"] == 2 and rep["rejected"] == 1 and rep["evicted"] == 0
+
+
+def test_failed_upload_frees_its_slot():
+    clk = Clock()
+    ctx = FakeCtx(clk)
+    orig = ctx.upload
+
+    def upload(sid, a):
+        if a.n_par == 77:
+            raise RuntimeError("WDM_E_ARG (-1): node arrays")
+        orig(sid, a)
+
+    ctx.upload = upload
+    srv = serving.Server(ctx, hbm_budget=10 ** 9, chunk=10, clock=clk, sleep=clk.sleep)
+    done = srv.serve(reqs([10, 77, 30], [0., 0., 0.], steps=40))
+    by = {r.rid: r for r in done}
+    assert "admission failed" in by[1].error and by[0].done_steps == by[2].done_steps == 40
+    assert not ctx.res
+
+
+def test_scene_stopped_by_device_error_is_evicted_alone():
+    """one scene of a batched call stops (NaN force): it is evicted with its error after the
+    steps it took; the others, which took that call's steps too, run to completion"""
+    clk = Clock()
+    ctx = FakeCtx(clk)
+    ctx.fail_at = {2: 3}  # scene id 2 (the second request) stops in the third call
+    srv = serving.Server(ctx, hbm_budget=10 ** 9, chunk=10, clock=clk, sleep=clk.sleep)
+    done = srv.serve(reqs([10, 20, 30], [0., 0., 0.], steps=100))
+    by = {r.rid: r for r in done}
+    assert by[1].error.startswith("NaN force in contact") and by[1].result is None and by[1].done_steps == 20
+    assert by[0].done_steps == by[2].done_steps == 100 and by[0].result and by[2].result
+    assert all(2 not in c[0] for c in ctx.calls[3:])
+    assert not ctx.res
+    rep = srv.report()
+    assert rep["requests"] == 2 and rep["evicted"] == 1
+    assert rep["particle_steps"] == 100 * 10 + 100 * 30 + 20 * 20
 
 
 def drum_request(rid, seed, n, arrival, steps):
@@ -158,3 +212,46 @@ def test_served_results_equal_alone(hip_ctx):
         g.close()
         for f in ("pos", "vel", "angVel", "ori", "force", "torque"):
             assert np.array_equal(r.result[f], n[f]), (r.rid, f)
+
+
+@pytest.mark.gpu
+def test_nan_drum_is_evicted_and_the_others_equal_alone(hip_ctx):
+    """A drum whose material has a NaN Young's modulus gets NaN contact forces at its first
+    contact (IdealElPl.cpp:76-80 raises there): it is evicted with WDM_E_NAN, and the drums
+    stepped in the same calls end bit for bit as when stepped alone."""
+    clk = Clock()
+    steps = 400
+    specs = [(0, 31, 300, 0.), (1, 32, 200, 0.), (2, 33, 250, 0.)]
+
+    def mk():
+        rs = [drum_request(i, s, n, t, steps) for i, s, n, t in specs]
+        bad = rs[1].arrays
+        bad.young[:] = np.nan
+        bad.pos[1] = bad.pos[0] + np.array([0., 0., .5 * (bad.radius[0] + bad.radius[1])])  # overlapping at the start
+        return rs
+
+    orig_step = hip_ctx.step
+
+    def timed_step(sids, n):
+        try:
+            orig_step(sids, n)
+        finally:
+            clk.t += n * 1e-5
+
+    hip_ctx.step = timed_step
+    try:
+        srv = serving.Server(hip_ctx, hbm_budget=8 << 30, chunk=50, clock=clk, sleep=clk.sleep)
+        done = srv.serve(mk())
+    finally:
+        hip_ctx.step = orig_step
+    by = {r.rid: r for r in done}
+    assert by[1].error and "NaN" in by[1].error and by[1].result is None and by[1].done_steps < steps
+    assert srv.report()["evicted"] == 1
+    for rid in (0, 2):
+        alone = mk()[rid]
+        g = runner.HipRun(alone.arrays, alone.params, alone.cell, ctx=hip_ctx)
+        g.step(steps)
+        n = g.nodes()
+        g.close()
+        for f in ("pos", "vel", "angVel", "ori", "force", "torque"):
+            assert np.array_equal(by[rid].result[f], n[f]), (rid, f)

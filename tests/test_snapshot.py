@@ -170,3 +170,30 @@ def test_checkpoint_continues_bit_for_bit(kind, tmp_path):
         assert (d1.nDone, d1.stepLast, d1.stepPrev) == (d2.nDone, d2.stepLast, d2.stepPrev)
     r1.close()
     r2.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["box", "periodic"])
+def test_checkpoint_of_a_restored_scene(kind, tmp_path):
+    """save, load, save again before any step, load that: the second checkpoint holds the same
+    contact points as the first (they live in the contacts' state rows, not in the gather
+    records a ContactLoop writes), and the scene restored from it steps on bit for bit like the
+    uninterrupted run (L6Geom's update reads the previous contact point)."""
+    ctx, a, p, cell = _scene(kind)
+    r1 = runner.HipRun(a, p, cell, ctx=ctx)
+    r1.step(2700 if kind == "box" else 300)
+    f1 = snapshot.save(r1, str(tmp_path / "first.npz"))
+    r2 = snapshot.load(f1, ctx=ctx)
+    f2 = snapshot.save(r2, str(tmp_path / "second.npz"))
+    r2.close()
+    c1, c2 = snapshot.read(f1)[3], snapshot.read(f2)[3]
+    assert len(c1["idA"]) > 10
+    for k in ("idA", "idB", "cellDist", "trsf", "forceLocal", "phys", "lens", "contPt"):
+        assert np.array_equal(c1[k], c2[k]), k
+    r3 = snapshot.load(f2, ctx=ctx)
+    for n in (1, 37, 400):
+        r1.step(n)
+        r3.step(n)
+        _compare(r1, r3)
+    r1.close()
+    r3.close()

@@ -95,6 +95,7 @@ def lib():
         "wdm_collider_run": (C.c_int, [P, I32]),
         "wdm_contact_loop_run": (C.c_int, [P, I32]),
         "wdm_step_end": (C.c_int, [P, I32]),
+        "wdm_scene_error": (C.c_int, [P, I32, pi]),
         "wdm_step": (C.c_int, [P, pi, I32, I32]),
         "wdm_download_nodes": (C.c_int, [P, I32, pd, pd, pd, pd, pd, pd]),
         "wdm_num_contacts": (C.c_int, [P, I32, pl]),
@@ -205,8 +206,42 @@ class Context:
         self.check(self.L.wdm_set_params(self.h, sid, C.byref(p)))
 
     def step(self, sids, n):
+        """n whole steps of every listed scene (Scene::doOneStep).  A scene stopped by a device
+        error does not keep the others from their steps; the call then raises (scene_error
+        names the stopped scenes)."""
         arr = np.ascontiguousarray(sids, dtype=np.int32)
         self.check(self.L.wdm_step(self.h, ptr(arr, C.c_int32), len(arr), int(n)))
+
+    def scene_error(self, sid):
+        """(WDM_E_* code, message) of the device error that stopped scene `sid`; (0, "") while it runs"""
+        code = C.c_int32()
+        self.check(self.L.wdm_scene_error(self.h, sid, C.byref(code)))
+        return code.value, (self.L.wdm_last_error(self.h).decode() if code.value else "")
+
+    # ---- one engine at a time: the calls a host Scene::doOneStep makes (core/Scene.cpp:319-424) ----
+    def step_begin(self, sid):
+        """prologue: cell update, EnergyTracker resettables"""
+        self.check(self.L.wdm_step_begin(self.h, sid))
+
+    def leapfrog_run(self, sid):
+        """Leapfrog::run (pkg/dem/Leapfrog.cpp:103-255)"""
+        self.check(self.L.wdm_leapfrog_run(self.h, sid))
+
+    def collider_run(self, sid):
+        """InsertionSortCollider::run (pkg/dem/InsertionSortCollider.cpp:445-646)"""
+        self.check(self.L.wdm_collider_run(self.h, sid))
+
+    def contact_loop_run(self, sid):
+        """ContactLoop::run (pkg/dem/ContactLoop.cpp:93-259)"""
+        self.check(self.L.wdm_contact_loop_run(self.h, sid))
+
+    def dyndt_engine_run(self, sid):
+        """DynDt after ContactLoop, when its PeriodicEngine fires at this step"""
+        self.check(self.L.wdm_dyndt_run(self.h, sid))
+
+    def step_end(self, sid):
+        """epilogue: step += 1, time += dt, next dt"""
+        self.check(self.L.wdm_step_end(self.h, sid))
 
     def download_nodes(self, sid, n_node):
         D = C.c_double

@@ -126,7 +126,7 @@ struct ConView {
     GP T* st2;         // state of the rows from *counter on (items that were not touching at the full run)
     GP int32_t* cd;
     GP T* st;
-    GP double4* cpt;   // per state row: contact point of the last ContactLoop (x, y, z, 0)
+    GP double4* cpt;   // per state row: contact point and uN of the last ContactLoop (x, y, z, uN)
     GP int32_t* itm;   // per contact: its item in the ContactLoop's work list, -1 when dormant
     // per item, so that the ContactLoop reads them contiguously: (idA, idB, info, cellDist)
     // and (state row, contact); info and the row are written to both places
@@ -395,7 +395,8 @@ __device__ __forceinline__ void gatherNode(const StepArgs& a, const PartView& p,
     gatherForces(a, gv, e0, e1, sums, fl, m, pos, F, T);
 }
 
-__global__ void __launch_bounds__(WDM_BLOCK) k_gather(const SceneDesc* D, const int32_t* idx, Map m) {
+// node forces into the arrays: the reset values plus (sums != 0) the last ContactLoop's sums
+__global__ void __launch_bounds__(WDM_BLOCK) k_gather(const SceneDesc* D, const int32_t* idx, Map m, int sums) {
     int64_t lb;
     const int t = mapBlock(m, lb);
     const SceneDesc& S = D[idx ? idx[t] : t];
@@ -406,7 +407,8 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_gather(const SceneDesc* D, const 
     int i = (int)(lb * WDM_BLOCK + threadIdx.x);
     if (i >= n.nn) return;
     D3 F, T;
-    gatherNode(a, p, gv, i, n.flags[i], n.velm[i].w, xyz(n.posr[i]), F, T);
+    if (sums) gatherNode(a, p, gv, i, n.flags[i], n.velm[i].w, xyz(n.posr[i]), F, T);
+    else gatherForces(a, gv, 0, 0, false, n.flags[i], n.velm[i].w, xyz(n.posr[i]), F, T);
     n.force[i] = make_double4(F.x, F.y, F.z, 0.);
     n.torque[i] = make_double4(T.x, T.y, T.z, 0.);
 }
@@ -1939,7 +1941,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
         V3T<T> fg = mk3<T>(dot3(mk3<T>(tr[0], tr[1], tr[2]), fl), dot3(mk3<T>(tr[3], tr[4], tr[5]), fl), dot3(mk3<T>(tr[6], tr[7], tr[8]), fl));
         cv.gfc[2 * item] = make_double4((double)fg.x, (double)fg.y, (double)fg.z, (double)cdp);
         cv.gfc[2 * item + 1] = make_double4(G.contPt.x, G.contPt.y, G.contPt.z, (double)uN);
-        cv.cpt[row] = make_double4(G.contPt.x, G.contPt.y, G.contPt.z, 0.);
+        cv.cpt[row] = make_double4(G.contPt.x, G.contPt.y, G.contPt.z, (double)uN);
         realOut = true;
     } while (0);
     // one bit per item, one 64-bit word per wavefront: the gather skips non-real contacts

@@ -294,7 +294,10 @@ struct Shared {
 struct Scene {
     Ctx* ctx;
     int32_t np, nn, nmat;
+    int32_t id = -1;    // scene id of the ABI
     int32_t slot = -1;  // index of the scene's words in Shared::words
+    int32_t failed = 0;    // WDM_E_* of the device error that stopped the scene (0: running)
+    std::string failText;
     wdm_params P{};
     bool paramsSet = false, nodesSet = false, partsSet = false, matsSet = false;
     // host copies of static particle data
@@ -331,6 +334,9 @@ struct Scene {
     bool needFull = true;
     bool lastLeapfrogVerletChecked = false;
     bool forcesFromContacts = false;  // node forces are the sums of the last ContactLoop (not materialised)
+    // a Leapfrog ran after the last ContactLoop: the node forces are its reset values (m g, no
+    // torque; Leapfrog.cpp:240-245), which the arrays do not hold yet
+    bool forcesReset = false;
     HM3 hSizeCL = heye();             // cell hSize at the last ContactLoop (branch vectors of the B side)
     // nodes
     DBuf<double4> posr, velm, angI, ori, force, torque, lastPos;
@@ -640,19 +646,41 @@ struct Batch {
     void ensureDesc() { if (!D) upload(); }
 };
 
-void throwDeviceError(uint32_t e) {
-    if (e & ERR_NAN) throw WdmError(WDM_E_NAN, "NaN force in contact");
-    if (e & ERR_OVERSIZE) throw WdmError(WDM_E_OVERSIZE, "Particle larger than half of the cell size encountered.");
-    throw WdmError(WDM_E_HIP, "device error");
+// the reference's exception for a device error word (IdealElPl.cpp:76-80, InsertionSortCollider.cpp:945-949)
+int deviceErrorCode(uint32_t e) { return (e & ERR_NAN) ? WDM_E_NAN : ((e & ERR_OVERSIZE) ? WDM_E_OVERSIZE : WDM_E_HIP); }
+const char* deviceErrorText(uint32_t e) {
+    if (e & ERR_NAN) return "NaN force in contact";
+    if (e & ERR_OVERSIZE) return "Particle larger than half of the cell size encountered.";
+    return "device error";
 }
-// copy every scene's words to the host (synchronising) and raise the first device error among `ss`
-void readWords(Ctx* c, const std::vector<Scene*>& ss) {
+// A scene whose kernels flagged an error stops, as a Woo scene's run thread stops on an
+// exception (core/Scene.cpp:90-95): it takes no more steps and keeps its state for reading
+// until it is freed; the other scenes of a batch go on.  Marks the scenes of `ss` whose words
+// (last copied to the host) carry an error; returns those newly stopped.
+std::vector<Scene*> markFailed(Ctx* c, const std::vector<Scene*>& ss) {
+    std::vector<Scene*> out;
+    for (Scene* S : ss) {
+        const uint32_t e = c->sh->hostWords[4 * (size_t)S->slot + 2];
+        if (e && !S->failed) {
+            S->failed = deviceErrorCode(e);
+            S->failText = deviceErrorText(e);
+            out.push_back(S);
+        }
+    }
+    return out;
+}
+// copy every scene's words to the host (synchronising), mark the scenes of `ss` with a device
+// error as stopped and, with `raise`, raise the first one's error
+std::vector<Scene*> readWords(Ctx* c, const std::vector<Scene*>& ss, bool raise = true) {
     Shared& X = *c->sh;
     size_t n = 4 * (size_t)X.nslots;
     if (n) HIPCHK(hipMemcpyAsync(X.hostWords, X.words.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (Scene* S : ss)
-        if (uint32_t e = X.hostWords[4 * (size_t)S->slot + 2]) throwDeviceError(e);
+    std::vector<Scene*> f = markFailed(c, ss);
+    if (raise)
+        for (Scene* S : ss)
+            if (S->failed) throw WdmError(S->failed, S->failText);
+    return f;
 }
 
 // ---------------- Leapfrog::run (pkg/dem/Leapfrog.cpp:103-255) ----------------
@@ -738,16 +766,27 @@ struct LeapPlan {
         }
     }
 };
+void materialiseForces(Scene& S, bool sums);
 struct Leapfrog {
-    // uploads the batch's descriptors and the plan's tables, then launches
+    // uploads the batch's descriptors and the plan's tables, then launches; afterwards the node
+    // forces are the reset values (Leapfrog.cpp:240-245) until a ContactLoop adds its sums
     static void run(Batch& B) {
         Ctx* c = B.c;
+        for (Scene* S : B.ss)
+            if (S->forcesReset) {  // a second Leapfrog without a ContactLoop reads the reset forces
+                materialiseForces(*S, false);
+                S->forcesReset = false;
+            }
         LeapPlan lp;
         Blob b;
         lp.build(B, b);
         B.upload();
         lp.base = c->sh->stMapNode.upload(c, b.b.data(), b.b.size());
         lp.launch(c, B.D, nullptr);
+        for (Scene* S : B.ss) {
+            S->forcesFromContacts = false;
+            S->forcesReset = true;
+        }
     }
 };
 
@@ -1217,6 +1256,7 @@ struct ContactLoop {
     static void done(Batch& B) {
         for (Scene* S : B.ss) {
             S->forcesFromContacts = true;  // applied by the next Leapfrog (or materialised on download)
+            S->forcesReset = false;
             S->contactsRun = true;
             S->hSizeCL = S->hSize;
         }
@@ -1230,6 +1270,8 @@ struct ContactLoop {
 };
 
 void checkReady(Scene& S) {
+    if (S.failed)
+        throw WdmError(WDM_E_STATE, "scene " + std::to_string(S.id) + " was stopped by a device error (" + S.failText + "): free it");
     if (!(S.paramsSet && S.nodesSet && S.partsSet && S.matsSet)) throw WdmError(WDM_E_STATE, "scene not fully uploaded");
     if (!S.clumpsSet) throw WdmError(WDM_E_STATE, "clump flags of the nodes and the uploaded clumps disagree (wdm_upload_clumps)");
 }
@@ -1385,14 +1427,18 @@ void doOneStep(Batch& B) {
         HIPCHK(hipEventRecord(X.evWords, c->stream));
         ContactLoop::runList(B, spec, true);
         HIPCHK(hipEventSynchronize(X.evWords));
-        for (Scene* S : B.ss)
-            if (uint32_t e = X.hostWords[4 * (size_t)S->slot + 2]) throwDeviceError(e);
+        markFailed(c, B.ss);  // a scene with a device error stops here; the others go on
     }
-    std::vector<int> full = InsertionSortCollider::fullList(B, !spec.empty());
+    std::vector<int> full;
+    for (int i : InsertionSortCollider::fullList(B, !spec.empty()))
+        if (!B.ss[i]->failed) full.push_back(i);
     InsertionSortCollider::runList(B, full);
     ContactLoop::runList(B, full, false);
     ContactLoop::done(B);
-    DynDtEngine::fire(c, B.ss);
+    std::vector<Scene*> live;
+    for (Scene* S : B.ss)
+        if (!S->failed) live.push_back(S);
+    DynDtEngine::fire(c, live);
     for (Scene* S : B.ss) stepEnd(*S);
 }
 
@@ -1547,12 +1593,16 @@ std::vector<int64_t> runWindow(Ctx* c, const std::vector<Scene*>& ss, const std:
         X.graphLaunches++;
     } else launchAll();
     X.windows++;
-    readWords(c, ss);  // raises device errors
+    readWords(c, ss, false);  // scenes with a device error stop; the others finish the window
     // where each scene stopped
     std::vector<int64_t> done(n);
     std::vector<Scene*> stopped, fired;
     for (int i = 0; i < n; i++) {
         Scene& S = *ss[i];
+        if (S.failed) {
+            done[i] = left[i];
+            continue;
+        }
         const uint32_t w = X.hostWords[4 * (size_t)S.slot];
         int64_t active = std::min<int64_t>(K, left[i]);
         if (cutAt[i] >= 0) active = std::min<int64_t>(active, cutAt[i] + 1);
@@ -1590,8 +1640,9 @@ std::vector<int64_t> runWindow(Ctx* c, const std::vector<Scene*>& ss, const std:
     return done;
 }
 
-// DemData.force/torque after ContactLoop: gravity reset plus the contact sums
-void materialiseForces(Scene& S) {
+// DemData.force/torque into the arrays: after a ContactLoop the gravity reset plus the contact
+// sums (sums), after a Leapfrog the reset alone
+void materialiseForces(Scene& S, bool sums) {
     Ctx* c = S.ctx;
     Batch B(c, {&S});
     B.upload();
@@ -1599,8 +1650,14 @@ void materialiseForces(Scene& S) {
     Map m = uploadMap(c, c->sh->stMapNode, bp);
     if (bp[1] == 0) return;
     KTimer t(c, "gather");
-    hipLaunchKernelGGL(k_gather, dim3((unsigned)bp[1]), dim3(WDM_BLOCK), 0, c->stream, B.D, (const int32_t*)nullptr, m);
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)bp[1]), dim3(WDM_BLOCK), 0, c->stream, B.D, (const int32_t*)nullptr, m, sums ? 1 : 0);
     checkLaunch();
+}
+// the arrays receive the node forces, whichever engine ran last
+void materialiseAll(Scene& S) {
+    if (S.forcesFromContacts) materialiseForces(S, true);
+    else if (S.forcesReset) materialiseForces(S, false);
+    S.forcesReset = false;
 }
 
 Scene& getScene(Ctx* c, int32_t id) {
@@ -1644,23 +1701,24 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
     // rows of the items after the full run's rows (none right after a checkpoint upload, which
     // hands out rows only and builds the items at the next full run)
     const int32_t nTail = nI > nR ? nI - nR : 0;
+    const int64_t nRows = (int64_t)nR + nTail;
     const size_t nn = (size_t)std::max<int64_t>(n, 1);
     std::unique_ptr<int2[]> ids(new int2[nn]);
     std::unique_ptr<uint32_t[]> info(new uint32_t[nn]);
-    std::unique_ptr<int32_t[]> cd(new int32_t[nn]), slot(new int32_t[nn]), itm(new int32_t[nn]);
+    std::unique_ptr<int32_t[]> cd(new int32_t[nn]), slot(new int32_t[nn]);
     std::unique_ptr<T[]> st(new T[(size_t)NSTATE * std::max<int32_t>(nR, 1)]), st2(new T[(size_t)NSTRIDE * std::max<int32_t>(nTail, 1)]);
-    std::unique_ptr<double4[]> gfc(new double4[2 * (size_t)std::max<int32_t>(nI, 1)]);
+    // contact point and uN of each state row (written by the ContactLoop, or by a checkpoint upload)
+    std::unique_ptr<double4[]> cpt(new double4[(size_t)std::max<int64_t>(nRows, 1)]);
     if (n > 0) {
         HIPCHK(hipMemcpyAsync(ids.get(), b.ids.p, n * sizeof(int2), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(info.get(), b.info.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(cd.get(), b.cd.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipMemcpyAsync(slot.get(), b.slot.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipMemcpyAsync(itm.get(), b.itm.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         if (nR > 0)  // component-major: value q of the first nR rows
             HIPCHK(hipMemcpy2DAsync(st.get(), (size_t)nR * sizeof(T), b.st.p, (size_t)b.cap * sizeof(T), (size_t)nR * sizeof(T), NSTATE,
                                     hipMemcpyDeviceToHost, c->stream));
         if (nTail > 0) HIPCHK(hipMemcpyAsync(st2.get(), b.st2.p, (size_t)NSTRIDE * nTail * sizeof(T), hipMemcpyDeviceToHost, c->stream));
-        if (nI > 0) HIPCHK(hipMemcpyAsync(gfc.get(), b.gfc.p, 2 * (size_t)nI * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+        if (nRows > 0) HIPCHK(hipMemcpyAsync(cpt.get(), b.cpt.p, (size_t)nRows * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     // state value q of a row (the layout of the kernels' stAt, compacted to the rows in use)
@@ -1682,11 +1740,12 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
         }
         if (phys5) for (int d = 0; d < 5; d++) phys5[5 * k + d] = real ? (double)stv(row, S_P + d) : NAN;
         if (lens2) for (int d = 0; d < 2; d++) lens2[2 * k + d] = real ? (double)stv(row, S_L + d) : NAN;
-        // contact point and uN exist for real contacts only (L6Geom); zeros otherwise
-        // (an item beyond the list: the items are built at the next full run)
-        const double4 rec = (real && itm[i] >= 0 && itm[i] < nI) ? gfc[2 * (size_t)itm[i] + 1] : make_double4(0., 0., 0., 0.);
+        // contact point and uN exist for real contacts only (L6Geom): those of the state row, kept
+        // across full runs and checkpoint uploads; zeros (and NaN uN) otherwise
+        if (real && (row < 0 || row >= nRows)) throw WdmError(WDM_E_STATE, "contact state row out of range");
+        const double4 rec = real ? cpt[row] : make_double4(0., 0., 0., NAN);
         if (contPt3) { contPt3[3 * k] = rec.x; contPt3[3 * k + 1] = rec.y; contPt3[3 * k + 2] = rec.z; }
-        if (uN) uN[k] = real ? rec.w : NAN;
+        if (uN) uN[k] = rec.w;
         if (trsf9) {  // columns 0 and 1 stored, column 2 = col0 x col1 in the state's precision
             if (real && row >= 0) {
                 T c0[3], c1[3];
@@ -1754,7 +1813,8 @@ void uploadContacts(Scene& S, ConBuf<T>* cb, int64_t n, const int32_t* idA, cons
         }
         for (int q = 0; q < 5; q++) st[(size_t)(S_P + q) * cap + r] = (T)phys5[5 * k + q];
         for (int q = 0; q < 2; q++) st[(size_t)(S_L + q) * cap + r] = (T)lens2[2 * k + q];
-        cpt[r] = make_double4(contPt3[3 * k], contPt3[3 * k + 1], contPt3[3 * k + 2], 0.);
+        // uN is not part of a checkpoint: unknown (NaN) until the next ContactLoop computes it
+        cpt[r] = make_double4(contPt3[3 * k], contPt3[3 * k + 1], contPt3[3 * k + 2], NAN);
     }
     hipStream_t st0 = c->stream;
     const int32_t nR = (int32_t)n, zero = 0;
@@ -1922,7 +1982,17 @@ int wdm_scene_alloc(wdm_ctx* c, int32_t np, int32_t nn, int32_t nm, int32_t* sid
             throw;
         }
         *sid = c->nextId++;
+        S->id = *sid;
         c->scenes[*sid] = S;
+    });
+}
+
+int wdm_scene_error(wdm_ctx* c, int32_t sid, int32_t* code) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        if (!code) throw WdmError(WDM_E_ARG, "null output");
+        *code = S.failed;
+        if (S.failed) c->err = S.failText;
     });
 }
 
@@ -1971,6 +2041,7 @@ int wdm_upload_nodes(wdm_ctx* c, int32_t sid, int32_t n, const double* pos3, con
         S.needFull = true;
         S.contactsRun = false;
         S.forcesFromContacts = false;
+        S.forcesReset = false;
         S.clumpFlagged = nclf;
         S.flagsH.assign(flags, flags + n);
         S.nClumps = S.nMembers = 0;  // clumps are uploaded again after the nodes
@@ -2253,7 +2324,9 @@ int wdm_step(wdm_ctx* c, const int32_t* scenes, int32_t ns, int32_t nsteps) {
             std::vector<size_t> ai;
             int64_t most = 0;
             for (size_t i = 0; i < ss.size(); i++)
-                if (left[i] > 0) { act.push_back(ss[i]); la.push_back(left[i]); ai.push_back(i); most = std::max(most, left[i]); }
+                if (left[i] > 0 && !ss[i]->failed) {
+                    act.push_back(ss[i]); la.push_back(left[i]); ai.push_back(i); most = std::max(most, left[i]);
+                }
             if (act.empty()) break;
             const int K = (int)std::min<int64_t>(c->window, most);
             bool win = K > 1;
@@ -2267,7 +2340,16 @@ int wdm_step(wdm_ctx* c, const int32_t* scenes, int32_t ns, int32_t nsteps) {
                 for (size_t k = 0; k < act.size(); k++) left[ai[k]] -= 1;
             }
         }
-        readWords(c, ss);
+        readWords(c, ss, false);
+        // every scene without an error has taken its steps; report the ones that stopped
+        std::string ids;
+        int code = 0;
+        for (Scene* S : ss)
+            if (S->failed) {
+                ids += (ids.empty() ? "" : ", ") + std::to_string(S->id);
+                if (!code) code = S->failed;
+            }
+        if (code) throw WdmError(code, "scene(s) " + ids + " stopped by a device error (wdm_scene_error); the other scenes took their steps");
     });
 }
 
@@ -2275,7 +2357,7 @@ int wdm_download_nodes(wdm_ctx* c, int32_t sid, double* pos3, double* ori4, doub
     return guard(c, [&] {
         Scene& S = getScene(c, sid);
         int n = S.nn;
-        if (S.forcesFromContacts && (force3 || torque3)) materialiseForces(S);
+        if (force3 || torque3) materialiseAll(S);
         std::vector<double4> a(n);
         auto get = [&](DBuf<double4>& b, double* out, int comps, bool quat) {
             if (!out) return;
@@ -2409,11 +2491,10 @@ int wdm_insert_spheres(wdm_ctx* c, int32_t sid, int32_t n, const double* pos3, c
             if (mat[k] < 0 || mat[k] >= S.nmat) throw WdmError(WDM_E_ARG, "bad material index");
         }
         if (n == 0) return;
-        // the node forces of the last ContactLoop go to the arrays, where the new nodes get zero
-        if (S.forcesFromContacts) {
-            materialiseForces(S);
-            S.forcesFromContacts = false;
-        }
+        // the node forces (of the last ContactLoop, or Leapfrog's reset) go to the arrays, where
+        // the new nodes get zero
+        materialiseAll(S);
+        S.forcesFromContacts = false;
         std::vector<InsRec> rec(n);
         for (int k = 0; k < n; k++) {
             int32_t id;

@@ -9,6 +9,12 @@ finished ones are read back and freed.  Each scene's trajectory does not depend 
 what else is in the batch (per-scene block layout and reduction order), so a result
 is the same alone and in the mix.
 
+A failing request is confined to itself, as a failing Woo scene is to its run thread
+(core/Scene.cpp:90-95): a request larger than the whole budget, or whose upload fails, is
+rejected on its own (its slot freed); a scene stopped by a device error during a batched
+step (NaN force, oversized particle) is evicted with its error while the others, which
+took their steps, go on.  Such requests finish with `error` set and no result.
+
 The scene of each request is built by the client (the payload); the server's clock
 covers admission, upload, stepping and read-back.
 """
@@ -26,6 +32,7 @@ class Request:
         self.admitted = self.finished = None
         self.footprint = 0
         self.result = None
+        self.error = None  # why the request ended without a result
 
     @property
     def particles(self):
@@ -42,10 +49,11 @@ class Server:
     """Continuous-batching loop over one wdm context.
 
     ctx: woodem_amd._native.Context (or an object with the same scene_alloc/upload/
-    set_cell/set_params/step/download_nodes/scene_free/footprint methods);
+    set_cell/set_params/step/scene_error/download_nodes/scene_free/footprint methods);
     hbm_budget: bytes the resident scenes may occupy; max_slots: resident scenes at most;
-    chunk: steps per batched call; a request with fewer steps left than a chunk is
-    stepped alone for its remainder, so every request runs exactly its steps.
+    chunk: steps per batched call at most; every resident scene steps in every call, a
+    call being shortened to the fewest steps a resident request has left, so every request
+    runs exactly its steps and none waits for the others.
     contacts_per_particle: the contact density assumed for the footprint."""
 
     def __init__(self, ctx, hbm_budget, max_slots=1024, chunk=100, contacts_per_particle=8., clock=time.perf_counter,
@@ -60,6 +68,7 @@ class Server:
         self.chunk_ms = []  # wall time per batched step, one entry per chunk
         self.particle_steps = 0
         self.max_resident = 0
+        self.rejected = self.evicted = 0
         self.t0 = None
         self.log, self.log_every = log, float(log_every)
 
@@ -68,28 +77,51 @@ class Server:
         a = req.arrays
         return int(self.ctx.footprint(a.n_par, a.n_node, self.cpp))
 
+    def _reject(self, req, why):
+        req.error = why
+        req.finished = self.clock() - self.t0
+        self.finished.append(req)
+        self.rejected += 1
+        if self.log:
+            self.log("request %r rejected: %s" % (req.rid, why))
+
     def _admit(self, now):
         while self.queue and self.queue[0].arrival <= now:
             req = self.queue[0]
             fp = self.footprint(req)
             if fp > self.budget:
-                raise ValueError("request %r needs %d bytes, more than the whole budget" % (req.rid, fp))
+                self.queue.pop(0)
+                self._reject(req, "needs %d bytes of HBM, more than the whole budget of %d" % (fp, self.budget))
+                continue
             if len(self.resident) >= self.max_slots or self.used + fp > self.budget:
                 return  # head of line waits for memory or a slot (arrival order is kept)
             self.queue.pop(0)
+            try:
+                req.sid = self.ctx.scene_alloc(req.arrays.n_par, req.arrays.n_node, req.arrays.n_mat)
+                self.ctx.upload(req.sid, req.arrays)
+                c = req.cell
+                self.ctx.set_cell(req.sid, c["periodic"], c["hSize"], c["gradV"], c["nextGradV"], c["homoDeform"])
+                self.ctx.set_params(req.sid, req.params)
+            except Exception as e:  # this request only: its slot is freed, the others carry on
+                if req.sid is not None:
+                    self.ctx.scene_free(req.sid)
+                    req.sid = None
+                self._reject(req, "admission failed: %s" % e)
+                continue
             req.footprint = fp
-            req.sid = self.ctx.scene_alloc(req.arrays.n_par, req.arrays.n_node, req.arrays.n_mat)
-            self.ctx.upload(req.sid, req.arrays)
-            c = req.cell
-            self.ctx.set_cell(req.sid, c["periodic"], c["hSize"], c["gradV"], c["nextGradV"], c["homoDeform"])
-            self.ctx.set_params(req.sid, req.params)
             req.admitted = self.clock() - self.t0
             self.resident.append(req)
             self.used += fp
             self.max_resident = max(self.max_resident, len(self.resident))
 
-    def _retire(self, req):
-        req.result = self.ctx.download_nodes(req.sid, req.arrays.n_node)
+    def _retire(self, req, error=None):
+        if error is None:
+            req.result = self.ctx.download_nodes(req.sid, req.arrays.n_node)
+        else:
+            req.error = error
+            self.evicted += 1
+            if self.log:
+                self.log("request %r evicted after %d steps: %s" % (req.rid, req.done_steps, error))
         self.ctx.scene_free(req.sid)
         req.sid = None
         req.finished = self.clock() - self.t0
@@ -99,10 +131,23 @@ class Server:
 
     def _step(self, reqs, n):
         t = self.clock()
-        self.ctx.step([r.sid for r in reqs], n)
+        failed = {}
+        try:
+            self.ctx.step([r.sid for r in reqs], n)
+        except Exception as e:
+            # the scenes without an error took their n steps; the stopped ones are evicted
+            for r in reqs:
+                code, msg = self.ctx.scene_error(r.sid)
+                if code:
+                    failed[r.rid] = "%s (%d)" % (msg, code)
+            if not failed:
+                raise e
         dt = self.clock() - t
         self.chunk_ms.append(dt / n * 1e3)
         for r in reqs:
+            if r.rid in failed:
+                self._retire(r, error=failed[r.rid])
+                continue
             r.done_steps += n
             self.particle_steps += r.particles * n
 
@@ -124,23 +169,20 @@ class Server:
                 if wait > 0:
                     self.sleep(wait)
                 continue
-            # scenes that need less than a chunk finish alone; the rest step together
-            short = [r for r in self.resident if r.steps - r.done_steps < self.chunk]
-            for r in short:
-                self._step([r], r.steps - r.done_steps)
-            full = [r for r in self.resident if r.steps - r.done_steps >= self.chunk]
-            if full:
-                self._step(full, self.chunk)
+            # every resident scene steps; the call ends where the first of them is done
+            n = min(self.chunk, min(r.steps - r.done_steps for r in self.resident))
+            self._step(list(self.resident), n)
             for r in [r for r in self.resident if r.done_steps >= r.steps]:
                 self._retire(r)
         return self.finished
 
     def report(self):
         wall = (self.clock() - self.t0) if self.t0 is not None else float("nan")
-        lat = np.array([r.finished - r.arrival for r in self.finished])
-        wait = np.array([r.admitted - r.arrival for r in self.finished])
+        ok = [r for r in self.finished if r.error is None]
+        lat = np.array([r.finished - r.arrival for r in ok])
+        wait = np.array([r.admitted - r.arrival for r in ok])
         cms = np.array(self.chunk_ms) if self.chunk_ms else np.array([math.nan])
-        return dict(requests=len(self.finished), wall_s=wall, particle_steps=self.particle_steps,
+        return dict(requests=len(ok), rejected=self.rejected, evicted=self.evicted, wall_s=wall, particle_steps=self.particle_steps,
                     particle_steps_per_s=self.particle_steps / wall if wall > 0 else math.nan,
                     latency_s={"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))} if len(lat) else None,
                     admission_wait_s={"p50": float(np.percentile(wait, 50)), "p99": float(np.percentile(wait, 99))} if len(wait) else None,
