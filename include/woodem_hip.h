@@ -259,6 +259,14 @@ int wdm_set_masks(wdm_ctx* ctx, int32_t scene, int32_t n, const int32_t* ids, co
    n = how many (ids holds the first `cap`). */
 int wdm_select_box(wdm_ctx* ctx, int32_t scene, const double* lo3, const double* hi3, int32_t inside, uint32_t mask,
                    uint32_t mark_mask, int32_t cap, int32_t* ids, int32_t* n);
+/* wdm_select_box that allocates nothing (no counterpart in the reference): for callers that
+   cannot allocate device memory (code under HIP graph capture, tight serving loops).  Same
+   arguments and result; the selection is compacted on the device in memory the library
+   reserves once per GPU (16.1 MiB, held until the process ends), so scenes of at most
+   4,194,304 particles are accepted (larger ones: WDM_E_ARG).  Calls that overlap on one GPU
+   share that memory and race. */
+int wdm_select_box_noalloc(wdm_ctx* ctx, int32_t scene, const double* lo3, const double* hi3, int32_t inside, uint32_t mask,
+                           uint32_t mark_mask, int32_t cap, int32_t* ids, int32_t* n);
 
 /* SpherePack::makeCloud (lib/sphere-pack/SpherePack.cpp:135-248) on the context's device:
    a random loose packing in the box mn..mx (periodic: the cell of that size), radii from

@@ -125,6 +125,7 @@ def lib():
         "wdm_set_free_ids": (C.c_int, [P, I32, I32, pi, I32]),
         "wdm_set_masks": (C.c_int, [P, I32, I32, pi, pu]),
         "wdm_select_box": (C.c_int, [P, I32, pd, pd, I32, U32, U32, I32, pi, pi]),
+        "wdm_select_box_noalloc": (C.c_int, [P, I32, pd, pd, I32, U32, U32, I32, pi, pi]),
         "wdm_make_cloud": (C.c_int, [P, pd, pd, D, D, I32, I32, D, pd, pd, I32, I32, pu, I64, pd, pd, pl, pd]),
         "wdm_regular_pack": (C.c_int, [P, I32, I32, pd, D, D, I64, pd, pl]),
     }
@@ -432,6 +433,17 @@ class Context:
         self.check(self.L.wdm_select_box(self.h, sid, ptr(lo, C.c_double), ptr(hi, C.c_double), int(bool(inside)), int(mask),
                                          int(markMask), n.value, ptr(ids, C.c_int32), C.byref(n)))
         return ids
+
+    def select_box_noalloc(self, sid, lo, hi, inside=False, mask=0, markMask=0, cap=None):
+        """select_box through wdm_select_box_noalloc (no device allocation; scenes of at most
+        4,194,304 particles); returns (the first `cap` ids, how many were selected)"""
+        lo, hi = np.ascontiguousarray(lo, np.float64), np.ascontiguousarray(hi, np.float64)
+        cap = int(cap) if cap is not None else 1 << 22
+        ids = np.empty(max(cap, 1), np.int32)
+        n = C.c_int32()
+        self.check(self.L.wdm_select_box_noalloc(self.h, sid, ptr(lo, C.c_double), ptr(hi, C.c_double), int(bool(inside)), int(mask),
+                                                 int(markMask), cap, ptr(ids, C.c_int32), C.byref(n)))
+        return ids[:min(n.value, cap)], n.value
 
     def make_cloud(self, mn, mx, rMean, rRelFuzz, num, periodic, porosity, psdSizes, psdCumm, distributeMass, state, cap):
         """SpherePack::makeCloud on the device; `state` (uint32 array of 1) is advanced in place.

@@ -2111,19 +2111,71 @@ __global__ void k_set_masks(uint32_t* pmask, const int32_t* ids, const uint32_t*
 // BoxOutlet::isInside (pkg/dem/Outlet.hpp:56, AlignedBox3r::contains: closed box) and the
 // particle tests of Outlet::run (Outlet.cpp:16-40) for uninodal particles: 1 when the
 // particle is to be deleted (markMask 0) or marked
+__device__ __forceinline__ bool boxSelected(const double4* posr, const int32_t* ptype, const uint32_t* pmask, int i, double3 lo,
+                                            double3 hi, int inside, uint32_t mask, uint32_t markMask) {
+    const int sh = ptShape(ptype[i]);
+    if (!(sh == 1 || sh == 2)) return false;
+    const double4 q = posr[i];
+    const bool in = q.x >= lo.x && q.x <= hi.x && q.y >= lo.y && q.y <= hi.y && q.z >= lo.z && q.z <= hi.z;
+    const uint32_t m = pmask[i];
+    return (inside != 0) == in && !(mask && !(mask & m)) && !(markMask != 0 && (markMask & m) == markMask);
+}
 __global__ void k_select_box(const double4* posr, const int32_t* ptype, const uint32_t* pmask, int np, double3 lo, double3 hi,
                              int inside, uint32_t mask, uint32_t markMask, uint8_t* out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
-    const int sh = ptShape(ptype[i]);
-    uint8_t sel = 0;
-    if (sh == 1 || sh == 2) {
-        const double4 q = posr[i];
-        const bool in = q.x >= lo.x && q.x <= hi.x && q.y >= lo.y && q.y <= hi.y && q.z >= lo.z && q.z <= hi.z;
-        const uint32_t m = pmask[i];
-        sel = (inside != 0) == in && !(mask && !(mask & m)) && !(markMask != 0 && (markMask & m) == markMask);
+    out[i] = boxSelected(posr, ptype, pmask, i, lo, hi, inside, mask, markMask) ? 1 : 0;
+}
+// The same selection compacted on the device (ascending ids, no host pass): per block of
+// WDM_BLOCK particles the number selected, one workgroup's exclusive scan of those counts,
+// then each block writes its ids at its offset in thread order.
+__global__ void __launch_bounds__(WDM_BLOCK) k_select_count(const double4* posr, const int32_t* ptype, const uint32_t* pmask, int np,
+                                                           double3 lo, double3 hi, int inside, uint32_t mask, uint32_t markMask,
+                                                           int32_t* counts) {
+    const int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+    const bool s = i < np && boxSelected(posr, ptype, pmask, i, lo, hi, inside, mask, markMask);
+    const int c = __syncthreads_count(s ? 1 : 0);
+    if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+// one workgroup of 1024 threads: offs[b] = sum of counts[0..b), *total = sum of all (nb <= 1024 * SCAN_PER_THREAD)
+#define SCAN_PER_THREAD 16
+__global__ void __launch_bounds__(1024) k_select_scan(const int32_t* counts, int nb, int32_t* offs, int32_t* total) {
+    __shared__ int32_t part[1024];
+    const int t = threadIdx.x, b0 = t * SCAN_PER_THREAD;
+    int32_t v[SCAN_PER_THREAD], s = 0;
+    for (int k = 0; k < SCAN_PER_THREAD; k++) {
+        v[k] = b0 + k < nb ? counts[b0 + k] : 0;
+        s += v[k];
     }
-    out[i] = sel;
+    part[t] = s;
+    __syncthreads();
+    for (int w = 1; w < 1024; w <<= 1) {  // inclusive Hillis-Steele scan of the thread sums
+        const int32_t add = t >= w ? part[t - w] : 0;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    int32_t o = part[t] - s;  // exclusive prefix of this thread's first block
+    for (int k = 0; k < SCAN_PER_THREAD; k++) {
+        if (b0 + k < nb) offs[b0 + k] = o;
+        o += v[k];
+    }
+    if (t == 1023) *total = part[t];
+}
+__global__ void __launch_bounds__(WDM_BLOCK) k_select_write(const double4* posr, const int32_t* ptype, const uint32_t* pmask, int np,
+                                                           double3 lo, double3 hi, int inside, uint32_t mask, uint32_t markMask,
+                                                           const int32_t* offs, int32_t* ids) {
+    __shared__ int32_t waveBase[WDM_BLOCK / 64];
+    const int i = blockIdx.x * WDM_BLOCK + threadIdx.x;
+    const bool s = i < np && boxSelected(posr, ptype, pmask, i, lo, hi, inside, mask, markMask);
+    const uint64_t bal = __ballot(s);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) waveBase[wv] = __popcll(bal);
+    __syncthreads();
+    int base = offs[blockIdx.x];
+    for (int k = 0; k < wv; k++) base += waveBase[k];
+    const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+    if (s) ids[base + rank] = i;
 }
 
 // ---------------- SpherePack::makeCloud (lib/sphere-pack/SpherePack.cpp:135-248) ----------------

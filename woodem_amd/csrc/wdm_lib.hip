@@ -14,12 +14,26 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/woodem_hip.h"
 #include "wdm_kernels.h"
+
+// Device memory of wdm_select_box_noalloc: per-block counts and offsets of the selection, the
+// total and the selected ids, for scenes of up to NOALLOC_MAX_PARTICLES (4,194,304) particles
+// (16.1 MiB).  One copy per GPU, held until the process ends.  In host code the name is a
+// stand-in, not a device address: the host uses the address hipGetSymbolAddress gives for
+// each device (scratchInit).  Calls that overlap on one GPU share it and race.
+#define NOALLOC_MAX_BLOCKS (1024 * SCAN_PER_THREAD)
+#define NOALLOC_MAX_PARTICLES (NOALLOC_MAX_BLOCKS * WDM_BLOCK)
+#define NOALLOC_OFF_OFFS (4 * NOALLOC_MAX_BLOCKS)
+#define NOALLOC_OFF_TOTAL (8 * NOALLOC_MAX_BLOCKS)
+#define NOALLOC_OFF_IDS (8 * NOALLOC_MAX_BLOCKS + 256)
+#define NOALLOC_BYTES (NOALLOC_OFF_IDS + 4 * (size_t)NOALLOC_MAX_PARTICLES)
+static __device__ __align__(256) uint8_t scratch[NOALLOC_BYTES];
 
 namespace {
 
@@ -102,6 +116,7 @@ struct Ctx {
     int window = 16;
     int graphs = 1;
     double cellFactor = 1.;
+    hipError_t scratchErr = hipSuccess;  // looking up the device address of `scratch` at wdm_create
 };
 
 void* devAlloc(Ctx* c, size_t bytes) {
@@ -1900,11 +1915,74 @@ int32_t takeSlot(Ctx* c) {
     return s;
 }
 
+// ---------------- the selection of an outlet without allocation ----------------
+// The address of `scratch` on each device (declared at the top of this file), looked up at
+// wdm_create; kernels and copies use it, never the array's name.
+#define NOALLOC_MAX_DEVICES 64
+std::mutex g_scratchMutex;
+uint8_t* g_scratchPtr[NOALLOC_MAX_DEVICES] = {};
+// the device address of `scratch` on the current device `dev` (looked up once per device)
+hipError_t scratchInit(int dev) {
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratchMutex);
+    if (g_scratchPtr[dev]) return hipSuccess;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratchPtr[dev] = (uint8_t*)p;
+    return hipSuccess;
+}
+uint8_t* scratchOf(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratchMutex);
+    return (dev >= 0 && dev < NOALLOC_MAX_DEVICES) ? g_scratchPtr[dev] : nullptr;
+}
+
 }  // namespace
 
 struct wdm_ctx : Ctx {};
 
 extern "C" {
+
+int wdm_select_box_noalloc(wdm_ctx* c, int32_t sid, const double* lo3, const double* hi3, int32_t inside, uint32_t mask, uint32_t markMask,
+                           int32_t cap, int32_t* ids, int32_t* nOut) {
+    return guard(c, [&] {
+        Scene& S = getScene(c, sid);
+        if (!lo3 || !hi3 || !nOut || cap < 0 || (cap > 0 && !ids)) throw WdmError(WDM_E_ARG, "box arrays");
+        if (S.np > NOALLOC_MAX_PARTICLES)
+            throw WdmError(WDM_E_ARG, "wdm_select_box_noalloc: " + std::to_string(S.np) + " particles, more than its limit of " +
+                                          std::to_string(NOALLOC_MAX_PARTICLES) + " (use wdm_select_box)");
+        uint8_t* base = scratchOf(c->device);
+        if (!base)
+            throw WdmError(WDM_E_HIP, std::string("wdm_select_box_noalloc: hipGetSymbolAddress of its device memory failed: ") +
+                                          hipGetErrorString(c->scratchErr));
+        int32_t* counts = (int32_t*)base;
+        int32_t* offs = (int32_t*)(base + NOALLOC_OFF_OFFS);
+        int32_t* total = (int32_t*)(base + NOALLOC_OFF_TOTAL);
+        int32_t* dIds = (int32_t*)(base + NOALLOC_OFF_IDS);
+        const int nb = (int)nblk(S.np);
+        const double3 lo = make_double3(lo3[0], lo3[1], lo3[2]), hi = make_double3(hi3[0], hi3[1], hi3[2]);
+        hipStream_t st = c->stream;
+        // every count, offset and the total are written by the kernels below before they are read
+        if (nb > 0)
+            hipLaunchKernelGGL(k_select_count, dim3(nb), dim3(WDM_BLOCK), 0, st, (const double4*)S.posr.p, (const int32_t*)S.ptype.p,
+                               (const uint32_t*)S.pmask.p, S.np, lo, hi, inside, mask, markMask, counts);
+        hipLaunchKernelGGL(k_select_scan, dim3(1), dim3(1024), 0, st, (const int32_t*)counts, nb, offs, total);
+        if (nb > 0)
+            hipLaunchKernelGGL(k_select_write, dim3(nb), dim3(WDM_BLOCK), 0, st, (const double4*)S.posr.p, (const int32_t*)S.ptype.p,
+                               (const uint32_t*)S.pmask.p, S.np, lo, hi, inside, mask, markMask, (const int32_t*)offs, dIds);
+        checkLaunch();
+        int32_t n = 0;
+        HIPCHK(hipMemcpyAsync(&n, total, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (n < 0 || n > S.np) throw WdmError(WDM_E_STATE, "selection count out of range");
+        const int32_t k = std::min(n, cap);
+        if (k > 0) {
+            HIPCHK(hipMemcpyAsync(ids, dIds, (size_t)k * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+        }
+        *nOut = n;
+    });
+}
 
 int wdm_create(wdm_ctx** out, int device, size_t budget) {
     if (!out) return WDM_E_ARG;
@@ -1923,6 +2001,9 @@ int wdm_create(wdm_ctx** out, int device, size_t budget) {
         delete c;
         return WDM_E_HIP;
     }
+    // the device address of the no-allocation entry point's memory, once per device, with that
+    // device current; a failure is kept and reported by that entry point only
+    c->scratchErr = scratchInit(device);
     *out = c;
     return WDM_OK;
 }
