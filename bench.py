@@ -46,29 +46,65 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(ctx, n_sample, seconds):
-    """Oracle (C++ restatement, OpenMP) on a bounded sample of the same workload."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as OL
-    from woodem_amd import runner, scenes
-    a = scenes.periodic_hertz(n=n_sample)
-    b = scenes.relax_and_order(a, ctx, iters=500)
-    p = runner.params_from_meta(b)
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = min(threads, 16)
-    o = OL.OracleRun(b, p, runner.cell_from_meta(b), nThreads=threads)
-    o.step(10)  # warm-up (includes the initial sort)
+CPU_LABEL = "Woo-semantics CPU restatement (oracle/, C++17 + OpenMP)"
+# a one-GPU box grants a job 16 host CPUs, whatever its affinity mask or nproc show
+CPU_SHARE = 16
+
+
+def cpu_setup():
+    """Threads of the host baseline and the OpenMP placement, set before the OpenMP runtime
+    starts (the oracle library is loaded after this): every CPU of the affinity mask up to the
+    box's share, threads bound close to each other on cores."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = max(1, min(aff, CPU_SHARE))
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(threads=threads, affinity_cpus=aff, cpu_model=model, omp_proc_bind=os.environ["OMP_PROC_BIND"],
+                omp_places=os.environ["OMP_PLACES"])
+
+
+def oracle_rate(OL, arrays, params, cell, threads, seconds, warm=10, chunk=5):
+    """particle-steps/s of the oracle stepping `arrays` on `threads` threads: `warm` untimed
+    steps (the initial sort), then chunks of steps until `seconds` have passed"""
+    o = OL.OracleRun(arrays, params, cell, nThreads=threads)
+    o.step(warm)
     t0 = time.perf_counter()
     k = 0
     while True:
-        o.step(5)
-        k += 5
+        o.step(chunk)
+        k += chunk
         el = time.perf_counter() - t0
         if el > seconds:
             break
-    return dict(value=n_sample * k / el, unit="particle-steps/s", cores=threads, kind="port",
-                sample="periodic Hertz, %d spheres (same PSD, phi=0.60, relaxed), %d steps after 10 warm-up steps, %.1f s" % (
-                    n_sample, k, el))
+    o.close()
+    return arrays.n_par * k / el, k, el
+
+
+def cpu_baseline(ctx, n_sample, seconds, cpu):
+    """The oracle on a bounded sample of the timed workload (same generator, PSD, packing
+    fraction, relaxation and numbering, fewer spheres), on every thread of the box's share and
+    on one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as OL
+    from woodem_amd import runner, scenes
+    b = scenes.relax_and_order(scenes.periodic_hertz(n=n_sample), ctx, iters=500)
+    p, cell = runner.params_from_meta(b), runner.cell_from_meta(b)
+    v, k, el = oracle_rate(OL, b, p, cell, cpu["threads"], seconds)
+    v1, k1, el1 = oracle_rate(OL, b, p, cell, 1, seconds / 3)
+    return dict(value=v, unit="particle-steps/s", cores=cpu["threads"], kind="port", label=CPU_LABEL,
+                value_1_thread=v1, cpu_model=cpu["cpu_model"], affinity_cpus=cpu["affinity_cpus"],
+                omp={"OMP_PROC_BIND": cpu["omp_proc_bind"], "OMP_PLACES": cpu["omp_places"]},
+                sample="periodic Hertz, %d spheres (the timed workload's generator, PSD, phi=0.60, relaxed), %d steps in "
+                       "%.1f s after 10 warm-up steps (1 thread: %d steps in %.1f s)" % (n_sample, k, el, k1, el1))
 
 
 def cloud_args(num):
@@ -174,16 +210,19 @@ def insertion(ctx, n_spare=20000):
             "full_collider_runs_in_step_after": full1 - full0, "select_box_s": t_sel, "selected": int(len(sel))}
 
 
-def measured_traffic(args):
-    """HBM bytes per k_contacts launch (read + write) of this workload, from the rocprofv3
-    FETCH_SIZE/WRITE_SIZE runs recorded in profiles/ (scaled by a device copy of known
-    size); None for other sizes or precisions."""
+def measured_traffic(args, n_real, n_active):
+    """HBM bytes per k_contacts launch (read + write) from the rocprofv3 FETCH_SIZE/WRITE_SIZE
+    runs in profiles/, when they were taken at this run's roofline steps (the same real and
+    processed contact counts: the steps are deterministic); otherwise None, with the reason."""
     path = os.path.join(ROOT, "profiles", "traffic_periodic_hertz_1M_fp64.json")
     if args.n != 1_000_000 or args.precision != "fp64" or not os.path.exists(path):
-        return None
+        return None, "no counter profile for this workload"
     with open(path) as f:
         t = json.load(f)
-    return t["hbm_read_bytes_per_launch"] + t["hbm_write_bytes_per_launch"]
+    if t.get("real_contacts") != n_real or t.get("processed_contacts") != n_active:
+        return None, "the counter profile was taken at other steps (%s real, %s processed contacts)" % (
+            t.get("real_contacts"), t.get("processed_contacts"))
+    return t["hbm_read_bytes_per_launch"] + t["hbm_write_bytes_per_launch"], "profiles/" + t.get("source", "")
 
 
 def timed_kernels(ctx, sids, steps):
@@ -334,6 +373,23 @@ def bench_serve(args, ctx):
     srv.serve(reqs)
     rep = srv.report()
     log("served: %s" % json.dumps(rep))
+    # sequential: a sample of the requests, each stepped alone from its own state (no batching)
+    pick = list(range(0, len(reqs), max(1, len(reqs) // args.seq_requests)))[:args.seq_requests]
+    sps = st = 0.
+    for i in pick:
+        d = reqs[i].arrays
+        r = runner.HipRun(d, reqs[i].params, reqs[i].cell, ctx=ctx)
+        r.step(args.chunk)
+        ctx.sync()
+        tt = time.perf_counter()
+        r.step(args.seq_steps)
+        ctx.sync()
+        st += time.perf_counter() - tt
+        sps += d.n_par * args.seq_steps
+        r.close()
+    seq = {"value": sps / st, "unit": "particle-steps/s", "requests": len(pick), "steps": args.seq_steps,
+           "served_over_sequential": rep["particle_steps_per_s"] / (sps / st)}
+    log("serve sequential sample: %s" % json.dumps(seq))
     return {
         "metric": "particle-steps/s on 1 MI355X (served drums, continuous batching); p50/p99 latency",
         "value": rep["particle_steps_per_s"],
@@ -349,7 +405,9 @@ def bench_serve(args, ctx):
         "data": "synthetic (seeded Poisson arrivals and drum generator, seed %d)" % args.seed,
         "config": {"workload": "%d rotating-drum requests, Poisson %.0f/s, %d steps each, served with continuous batching" % (
             len(reqs), args.rate, args.req_steps), "requests": len(reqs), "rate_per_s": args.rate, "chunk_steps": args.chunk,
-                   "spheres_mean": float(n_sph.mean()), "hbm_budget": args.hbm_budget, "launch": launch_config(ctx, None)},
+                   "spheres_mean": float(n_sph.mean()), "hbm_budget": args.hbm_budget, "launch": launch_config(ctx, None),
+                   "rejected": rep["rejected"], "evicted": rep["evicted"]},
+        "sequential": seq,
         "latency_s": rep["latency_s"],
         "admission_wait_s": rep["admission_wait_s"],
         "step_latency_ms": rep["step_ms"],
@@ -357,6 +415,222 @@ def bench_serve(args, ctx):
         "wall_s": rep["wall_s"],
         "roofline": None,
         "cpu_baseline": None,
+    }
+
+
+def bench_cfg1(ctx, cpu, steps=2000, cpu_on=True):
+    """Config 1 (10,000 spheres r 5 mm on a jittered 25 x 20 x 20 lattice in a box of 6 walls,
+    IdealElPl, gravity, damping .2, dt 0.3 of the p-wave step, 2,000 steps) on the GPU in FP64
+    and in mixed precision (config 3's speed is mixed over FP64 on this scene), and the oracle's
+    rate on every thread of the box's share and on one thread, over the same 2,000 steps."""
+    from woodem_amd import runner, scenes
+    from woodem_amd._native import PREC_FP64, PREC_MIXED
+    a = scenes.box_lattice()
+    cell = runner.cell_from_meta(a)
+    out = {"workload": "cfg1: box_lattice 25x20x20, %d particles (10,000 spheres + 6 walls), IdealElPl, %d steps" % (a.n_par, steps),
+           "n_particles": a.n_par, "steps": steps}
+    for name, prec in (("fp64", PREC_FP64), ("mixed", PREC_MIXED)):
+        r = runner.HipRun(a, runner.params_from_meta(a, precision=prec), cell, ctx=ctx)
+        r.step(1)  # first step: uploads, first collider run
+        ctx.sync()
+        t = time.perf_counter()
+        r.step(steps - 1)
+        ctx.sync()
+        el = time.perf_counter() - t
+        out[name] = {"value": a.n_par * (steps - 1) / el, "ms_per_step": el / (steps - 1) * 1e3,
+                     "full_collider_runs": r.stats()["nFullRuns"]}
+        r.close()
+    out["cfg3_mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    if cpu_on:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as OL
+        p = runner.params_from_meta(a)
+        res = {}
+        for nt in (cpu["threads"], 1):
+            o = OL.OracleRun(a, p, cell, nThreads=nt)
+            o.step(1)
+            t = time.perf_counter()
+            o.step(steps - 1)
+            el = time.perf_counter() - t
+            o.close()
+            res[nt] = a.n_par * (steps - 1) / el
+        out["cpu_baseline"] = {"value": res[cpu["threads"]], "value_1_thread": res[1], "cores": cpu["threads"],
+                               "unit": "particle-steps/s", "kind": "port", "label": CPU_LABEL, "cpu_model": cpu["cpu_model"]}
+        out["gpu_fp64_over_cpu"] = out["fp64"]["value"] / res[cpu["threads"]]
+    return out
+
+
+def bench_periodic(args, ctx):
+    """The default line: the 1M-sphere periodic Hertz scene in FP64 (or mixed), timed over
+    exactly --steps steps after --warmup steps, with its roofline, latency, a sustained rate over
+    the workload's 1,000 steps (collider full runs included), and the other configurations'
+    legs."""
+    from woodem_amd import runner, scenes
+    from woodem_amd._native import PREC_FP64, PREC_MIXED
+    cpu = cpu_setup()
+    t0 = time.time()
+    a = scenes.periodic_hertz(n=args.n)
+    b = scenes.relax_and_order(a, ctx, iters=args.relax)
+    log("scene: %d spheres, L=%.4f m, dt=%.3e s, relaxed in %.1f s" % (b.n_par, b.meta["L"], b.meta["dt"], time.time() - t0))
+    prec = PREC_MIXED if args.precision == "mixed" else PREC_FP64
+    p = runner.params_from_meta(b, precision=prec)
+    run = runner.HipRun(b, p, runner.cell_from_meta(b), ctx=ctx)
+    run.step(args.warmup)
+    ctx.sync()
+    log("warm-up done: %s" % run.stats())
+
+    st0 = run.stats()
+    t1 = time.perf_counter()
+    run.step(args.steps)
+    ctx.sync()
+    el = time.perf_counter() - t1
+    st1 = run.stats()
+    value = b.n_par * args.steps / el
+    log("timed %d steps in %.3f s: %.4g particle-steps/s, full collider runs %d" % (
+        args.steps, el, value, st1["nFullRuns"] - st0["nFullRuns"]))
+
+    # per-step latency (each call synchronises)
+    lat = []
+    for _ in range(args.latency_steps):
+        tt = time.perf_counter()
+        run.step(1)
+        lat.append(time.perf_counter() - tt)
+    lat = np.array(lat if lat else [np.nan]) * 1e3
+
+    # kernel timing with HIP events on the library's stream, one step at a time
+    kt = timed_kernels(ctx, [run.sid], args.timing_steps)
+    c = run.contacts()
+    n_con = len(c["idA"])
+    n_real = int(c["isReal"].sum())
+    n_active, _ = ctx.contact_work(run.sid)  # contacts the kernel reads (dormant ones it never touches)
+    ms, launches = kt.get("contacts", (float("nan"), 0))
+    avg_ms = ms / max(launches, 1)
+    alg_bytes = BYTES_REAL[args.precision] * n_real + BYTES_POT[args.precision] * max(n_active - n_real, 0)
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(args, n_real, n_active)
+    kernels = {k: {"avg_ms": v[0] / max(v[1], 1), "launches": v[1]} for k, v in kt.items()}
+    log("kernels: %s" % json.dumps(kernels))
+
+    # sustained: the workload's 1,000 steps in windows, collider full runs included
+    sus = None
+    if args.sustained_steps > 0:
+        f0 = run.stats()["nFullRuns"]
+        ctx.sync()
+        t = time.perf_counter()
+        run.step(args.sustained_steps)
+        ctx.sync()
+        els = time.perf_counter() - t
+        sus = {"value": b.n_par * args.sustained_steps / els, "steps": args.sustained_steps,
+               "ms_per_step": els / args.sustained_steps * 1e3, "full_collider_runs": run.stats()["nFullRuns"] - f0}
+        log("sustained: %s" % json.dumps(sus))
+
+    # DynDt (utils.defaultEngines adds it with stepPeriod=100): its kernel on the same contacts
+    ctx.enable_timing(True)
+    for _ in range(10):
+        crit = ctx.dyndt_crit(run.sid)
+    kd = ctx.kernel_times().get("dyndt", (float("nan"), 0))
+    ctx.enable_timing(False)
+    kernels["dyndt"] = {"avg_ms": kd[0] / max(kd[1], 1), "launches": kd[1], "crit_dt": crit}
+    gen = gpu_make_cloud(ctx, args.cloud_n) if args.cloud_n > 0 else None
+    if gen:
+        log("scene generation: %s" % json.dumps(gen))
+    io = None
+    if not args.no_io:
+        io = scene_io(ctx, run)
+        io["insertion"] = insertion(ctx)
+        log("scene I/O and insertion: %s" % json.dumps(io))
+    log("contacts: %d potential, %d real, %d processed by the ContactLoop" % (n_con, n_real, n_active))
+    run.close()
+
+    mixed = None
+    if not args.no_mixed and args.precision == "fp64" and sus is not None:
+        pm = runner.params_from_meta(b, precision=PREC_MIXED)
+        rm = runner.HipRun(b, pm, runner.cell_from_meta(b), ctx=ctx)
+        # to the step where the FP64 sustained leg started (on its own trajectory), then the same steps timed
+        rm.step(args.warmup + args.steps + args.latency_steps + args.timing_steps)
+        ctx.sync()
+        f0 = rm.stats()["nFullRuns"]
+        tm = time.perf_counter()
+        rm.step(args.sustained_steps)
+        ctx.sync()
+        elm = time.perf_counter() - tm
+        ktm = timed_kernels(ctx, [rm.sid], args.timing_steps)
+        cm = rm.contacts()
+        msm, lam = ktm.get("contacts", (float("nan"), 0))
+        mval = b.n_par * args.sustained_steps / elm
+        mixed = {"value": mval, "ms_per_step": elm / args.sustained_steps * 1e3, "steps": args.sustained_steps,
+                 "full_collider_runs": rm.stats()["nFullRuns"] - f0, "speedup_vs_fp64": mval / sus["value"],
+                 "contacts_avg_ms": msm / max(lam, 1), "contacts_real": int(cm["isReal"].sum()),
+                 "contacts_processed": rm.ctx.contact_work(rm.sid)[0]}
+        rm.close()
+        from woodem_amd.precision import mixed_error_report
+        mixed["errors_vs_fp64"] = mixed_error_report(ctx)
+        log("mixed precision: %s" % json.dumps(mixed))
+
+    configs = {}
+    if not args.no_legs:
+        configs["cfg1"] = bench_cfg1(ctx, cpu, cpu_on=not args.no_cpu)
+        log("cfg1: %s" % json.dumps(configs["cfg1"]))
+        bargs = argparse.Namespace(**vars(args))
+        bargs.scenes, bargs.steps, bargs.warmup, bargs.autotune = 512, args.batch_steps, 20, 0
+        bargs.seq_scenes, bargs.seq_steps, bargs.latency_steps, bargs.timing_steps = 16, 200, 20, 50
+        ctx.set_tuning(window=16, graphs=not args.no_graphs, cell_factor=1.0)
+        bt = bench_batch(bargs, ctx)
+        configs["cfg4"] = {"workload": bt["config"]["workload"] + ", %d timed steps" % bt["steps"], "value": bt["value"],
+                           "unit": bt["unit"], "ms_per_step": bt["ms_per_step"], "sequential": bt["sequential"],
+                           "latency_ms": bt["latency_ms"], "roofline": bt["roofline"],
+                           "launch": bt["config"]["launch"], "scene_full_runs": bt["config"]["scene_full_runs_in_timed_region"]}
+        log("cfg4: %s" % json.dumps(configs["cfg4"]))
+        sargs = argparse.Namespace(**vars(args))
+        sv = bench_serve(sargs, ctx)
+        configs["cfg5"] = {"workload": sv["config"]["workload"], "value": sv["value"], "unit": sv["unit"],
+                           "latency_s": sv["latency_s"], "step_latency_ms": sv["step_latency_ms"],
+                           "admission_wait_s": sv["admission_wait_s"], "max_resident": sv["max_resident"], "wall_s": sv["wall_s"],
+                           "sequential": sv.get("sequential"), "rejected": sv["config"].get("rejected"),
+                           "evicted": sv["config"].get("evicted")}
+        log("cfg5: %s" % json.dumps(configs["cfg5"]))
+        ctx.set_tuning(window=args.window, graphs=not args.no_graphs, cell_factor=args.cell_factor)
+
+    cpub = None
+    if not args.no_cpu:
+        try:
+            cpub = cpu_baseline(ctx, args.cpu_sample, args.cpu_seconds, cpu)
+            if gen is not None:
+                gen["cpu_baseline"] = cpu_make_cloud(args.cloud_cpu_sample)
+            log("cpu baseline: %s" % json.dumps(cpub))
+        except Exception as e:  # reported, not fatal for the GPU figure
+            log("cpu baseline failed: %r" % (e,))
+
+    return {
+        "metric": "particle-steps/s on 1 MI355X (FP64 vs mixed, vs host OpenMP); p50/p99 step latency",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f64-state/f32-contact",
+        "data": "synthetic (seeded generator, relaxed on device)",
+        "config": {"workload": "cfg2: periodic Hertz, %d spheres, phi=0.60, lognormal r (median 1 mm, sigma .3), %d timed steps "
+                               "after %d warm-up steps" % (b.n_par, args.steps, args.warmup),
+                   "n_particles": b.n_par, "precision": args.precision, "dt": b.meta["dt"],
+                   "contacts_potential": n_con, "contacts_real": n_real, "contacts_processed": n_active,
+                   "full_collider_runs_in_timed_region": st1["nFullRuns"] - st0["nFullRuns"], "launch": launch_config(ctx, None)},
+        "sustained": sus,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)), "samples": args.latency_steps},
+        "roofline": {"bound": "hbm", "kernel": "k_contacts", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
+                     "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_ms, "launches": launches,
+                     "contacts_real": n_real, "contacts_processed": n_active},
+        "kernels": kernels,
+        "mixed_precision": mixed,
+        "configs": configs,
+        "scene_generation": gen,
+        "scene_io": io,
+        "cpu_baseline": cpub,
     }
 
 
@@ -395,6 +669,11 @@ def main():
     ap.add_argument("--autotune", type=int, default=None,
                     help="batch workload: pick window and cell factor by the autotuner (default 1 for batch, 0 otherwise)")
     ap.add_argument("--tune-steps", type=int, default=100, help="autotuner: timed steps per candidate and round")
+    ap.add_argument("--seq-requests", type=int, default=8, help="serve workload: requests in the sequential sample")
+    ap.add_argument("--sustained-steps", type=int, default=1000,
+                    help="default line: steps of the sustained rate (windows with their collider full runs; 0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=500, help="default line: timed steps of the cfg4 leg")
+    ap.add_argument("--no-legs", action="store_true", help="default line: skip the cfg1/cfg3, cfg4 and cfg5 legs")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain device copy of 1 GiB (5 times), for scaling FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
@@ -403,13 +682,24 @@ def main():
     if args.autotune is None:
         args.autotune = 1 if args.workload == "batch" else 0
     if args.calibrate_copy:
+        # kernels of known HBM traffic for scaling the FETCH_SIZE/WRITE_SIZE counters: a 1 GiB
+        # copy (1 GiB read, 1 GiB written), a 1 GiB sum (read only) and a gather of 2^22 random
+        # 64-byte rows of a 1 GiB table (256 MiB read in single lines, 256 MiB written)
         import torch
         a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         b = torch.empty_like(a)
         for _ in range(5):
             b.copy_(a)
+        s = None
+        for _ in range(5):
+            s = a.sum()
+        rows = a.view(-1, 8)
+        idx = torch.randint(0, rows.shape[0], (1 << 22,), device="cuda", generator=torch.Generator("cuda").manual_seed(1))
+        for _ in range(5):
+            g = rows.index_select(0, idx)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_per_launch": a.numel() * 8}))
+        print(json.dumps({"copy_bytes_read": a.numel() * 8, "copy_bytes_written": a.numel() * 8, "sum_bytes_read": a.numel() * 8,
+                          "gather_bytes_read": g.numel() * 8, "gather_bytes_written": g.numel() * 8, "sum": float(s)}))
         return
     if args.gpus != 1:
         print("bench.py: this build runs the timestep on one GPU only; --gpus %d is not supported" % args.gpus,
@@ -427,127 +717,7 @@ def main():
     if args.workload == "serve":
         print(json.dumps(bench_serve(args, ctx)), flush=True)
         return
-    t0 = time.time()
-    a = scenes.periodic_hertz(n=args.n)
-    b = scenes.relax_and_order(a, ctx, iters=args.relax)
-    log("scene: %d spheres, L=%.4f m, dt=%.3e s, relaxed in %.1f s" % (b.n_par, b.meta["L"], b.meta["dt"], time.time() - t0))
-    prec = PREC_MIXED if args.precision == "mixed" else PREC_FP64
-    p = runner.params_from_meta(b, precision=prec)
-    run = runner.HipRun(b, p, runner.cell_from_meta(b), ctx=ctx)
-    run.step(args.warmup)
-    ctx.sync()
-    log("warm-up done: %s" % run.stats())
-
-    st0 = run.stats()
-    t1 = time.perf_counter()
-    run.step(args.steps)
-    ctx.sync()
-    el = time.perf_counter() - t1
-    st1 = run.stats()
-    value = b.n_par * args.steps / el
-    log("timed %d steps in %.3f s: %.4g particle-steps/s, full collider runs %d" % (
-        args.steps, el, value, st1["nFullRuns"] - st0["nFullRuns"]))
-
-    # per-step latency (each call synchronises)
-    lat = []
-    for _ in range(args.latency_steps):
-        tt = time.perf_counter()
-        run.step(1)
-        lat.append(time.perf_counter() - tt)
-    lat = np.array(lat) * 1e3
-
-    # kernel timing with HIP events on the library's stream
-    kt = timed_kernels(ctx, [run.sid], args.timing_steps)
-    c = run.contacts()
-    n_con = len(c["idA"])
-    n_real = int(c["isReal"].sum())
-    n_active, _ = ctx.contact_work(run.sid)  # contacts the kernel reads (dormant ones it never touches)
-    ms, launches = kt.get("contacts", (float("nan"), 0))
-    avg_ms = ms / max(launches, 1)
-    alg_bytes = BYTES_REAL[args.precision] * n_real + BYTES_POT[args.precision] * max(n_active - n_real, 0)
-    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    kernels = {k: {"avg_ms": v[0] / max(v[1], 1), "launches": v[1]} for k, v in kt.items()}
-    # DynDt (utils.defaultEngines adds it with stepPeriod=100): its kernel on the same contacts
-    ctx.enable_timing(True)
-    for _ in range(10):
-        crit = ctx.dyndt_crit(run.sid)
-    kd = ctx.kernel_times().get("dyndt", (float("nan"), 0))
-    ctx.enable_timing(False)
-    kernels["dyndt"] = {"avg_ms": kd[0] / max(kd[1], 1), "launches": kd[1], "crit_dt": crit}
-    log("kernels: %s" % json.dumps(kernels))
-    gen = gpu_make_cloud(ctx, args.cloud_n) if args.cloud_n > 0 else None
-    log("scene generation: %s" % json.dumps(gen))
-    io = None
-    if not args.no_io:
-        io = scene_io(ctx, run)
-        io["insertion"] = insertion(ctx)
-        log("scene I/O and insertion: %s" % json.dumps(io))
-    log("contacts: %d potential, %d real, %d processed by the ContactLoop" % (n_con, n_real, n_active))
-
-    mixed = None
-    if not args.no_mixed and args.precision == "fp64":
-        run.close()
-        pm = runner.params_from_meta(b, precision=PREC_MIXED)
-        rm = runner.HipRun(b, pm, runner.cell_from_meta(b), ctx=ctx)
-        rm.step(args.warmup)
-        ctx.sync()
-        tm = time.perf_counter()
-        rm.step(args.steps)
-        ctx.sync()
-        elm = time.perf_counter() - tm
-        ktm = timed_kernels(ctx, [rm.sid], args.timing_steps)
-        cm = rm.contacts()
-        msm, lam = ktm.get("contacts", (float("nan"), 0))
-        mixed = {"value": b.n_par * args.steps / elm, "ms_per_step": elm / args.steps * 1e3,
-                 "speedup_vs_fp64": (b.n_par * args.steps / elm) / value, "contacts_avg_ms": msm / max(lam, 1),
-                 "contacts_real": int(cm["isReal"].sum())}
-        rm.close()
-        from woodem_amd.precision import mixed_error_report
-        mixed["errors_vs_fp64"] = mixed_error_report(ctx)
-        log("mixed precision errors: %s" % json.dumps(mixed["errors_vs_fp64"]))
-        log("mixed precision: %s" % json.dumps(mixed))
-        run = None
-
-    cpu = None
-    if not args.no_cpu:
-        try:
-            cpu = cpu_baseline(ctx, args.cpu_sample, args.cpu_seconds)
-            if gen is not None:
-                gen["cpu_baseline"] = cpu_make_cloud(args.cloud_cpu_sample)
-            log("cpu baseline: %s" % json.dumps(cpu))
-        except Exception as e:  # reported, not fatal for the GPU figure
-            log("cpu baseline failed: %r" % (e,))
-
-    out = {
-        "metric": "particle-steps/s on 1 MI355X (FP64 vs mixed, vs host OpenMP); p50/p99 step latency",
-        "value": value,
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": el / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64-state/f32-contact",
-        "data": "synthetic (seeded generator, relaxed on device)",
-        "config": {"workload": "periodic Hertz, %d spheres, phi=0.60, lognormal r (median 1 mm, sigma .3), 1000-step run" % b.n_par,
-                   "n_particles": b.n_par, "precision": args.precision, "dt": b.meta["dt"],
-                   "contacts_potential": n_con, "contacts_real": n_real, "contacts_processed": n_active,
-                   "full_collider_runs_in_timed_region": st1["nFullRuns"] - st0["nFullRuns"], "launch": launch_config(ctx, None)},
-        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)), "samples": len(lat)},
-        "roofline": {"bound": "hbm", "kernel": "k_contacts", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args),
-                     "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_ms},
-        "kernels": kernels,
-        "scene_generation": gen,
-        "scene_io": io,
-        "mixed_precision": mixed,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
-    if run is not None:
-        run.close()
+    print(json.dumps(bench_periodic(args, ctx)), flush=True)
 
 
 if __name__ == "__main__":

@@ -274,17 +274,28 @@ struct Shared {
     DBuf<uint8_t> winDev;
     uint8_t* winHost = nullptr;
     size_t winHostCap = 0;
+    hipGraph_t winGraph = nullptr;  // the captured graph, kept as long as its executable
     hipGraphExec_t winExec = nullptr;
     std::vector<uint64_t> winKey;
     int64_t graphLaunches = 0, graphCaptures = 0, windows = 0;
+    // The window graph goes before any memory its kernel arguments point to (the window tables,
+    // the energy partial sums) or any scene is freed, so that no executable graph outlives what
+    // it references: the queue is drained, then the executable and its graph are destroyed.
+    void dropGraph(Ctx* c) {
+        if (!winExec && !winGraph) return;
+        (void)hipStreamSynchronize(c->stream);
+        if (winExec) (void)hipGraphExecDestroy(winExec);
+        if (winGraph) (void)hipGraphDestroy(winGraph);
+        winExec = nullptr;
+        winGraph = nullptr;
+        winKey.clear();
+    }
     void freeAll(Ctx* c) {
+        dropGraph(c);
         release(c, winDev);
         if (winHost) (void)hipHostFree(winHost);
         winHost = nullptr;
         winHostCap = 0;
-        if (winExec) (void)hipGraphExecDestroy(winExec);
-        winExec = nullptr;
-        winKey.clear();
         release(c, words);
         if (hostWords) (void)hipHostFree(hostWords);
         hostWords = nullptr;
@@ -410,6 +421,13 @@ struct Scene {
         release(ctx, master); release(ctx, nodeClump); release(ctx, cOff); release(ctx, cMem); release(ctx, cRelPos); release(ctx, cRelOri); release(ctx, cFT);
     }
 };
+
+// grows a buffer that the window graph's kernel arguments point to: the graph goes first
+template <typename T>
+void ensureGraphArg(Ctx* c, DBuf<T>& b, size_t n) {
+    if (!b.p || b.n < n) c->sh->dropGraph(c);
+    ensure(c, b, n);
+}
 
 void timingFlush(Ctx* c) {  // reads the pending event pairs into the per-kernel totals
     Timing& T = c->timing;
@@ -731,7 +749,7 @@ struct LeapPlan {
         cmBp = blockPrefix((int)cmIdx.size(), [&](int t) { return B.ss[cmIdx[t]]->nMembers; });
         oCfIdx = b.addv(cfIdx); oCfBp = b.addv(cfBp);
         oCmIdx = b.addv(cmIdx); oCmBp = b.addv(cmBp);
-        ensure(c, c->sh->epartL, (size_t)(bp[0].back() + bp[1].back()) * 3 + 3);
+        ensureGraphArg(c, c->sh->epartL, (size_t)(bp[0].back() + bp[1].back()) * 3 + 3);
     }
     Map map(size_t o, size_t n) const {
         Map m;
@@ -1200,7 +1218,7 @@ struct ContactPlan {
         g.oC = b.addv(C);
         g.oBp = b.addv(bp);
         g.blocks = bp[n];
-        ensure(c, c->sh->epartC, (size_t)bp[n] * 5 + 5);
+        ensureGraphArg(c, c->sh->epartC, (size_t)bp[n] * 5 + 5);
         gs.push_back(g);
     }
     void build(Batch& B, const std::vector<int>& L, Blob& b) {
@@ -1551,7 +1569,7 @@ std::vector<int64_t> runWindow(Ctx* c, const std::vector<Scene*>& ss, const std:
     }
     const size_t oD = b.addv(d), oCells = b.addv(cells), oSV = b.addv(sv);
     const size_t bytes = b.b.size();
-    ensure(c, X.winDev, bytes);
+    ensureGraphArg(c, X.winDev, bytes);
     if (X.winHostCap < bytes) {
         HIPCHK(hipStreamSynchronize(st));
         if (X.winHost) (void)hipHostFree(X.winHost);
@@ -1586,8 +1604,7 @@ std::vector<int64_t> runWindow(Ctx* c, const std::vector<Scene*>& ss, const std:
         lp.sig(key);
         cp.sig(key);
         if (!X.winExec || key != X.winKey) {
-            if (X.winExec) { HIPCHK(hipGraphExecDestroy(X.winExec)); X.winExec = nullptr; }
-            X.winKey.clear();
+            X.dropGraph(c);
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
             try {
@@ -1599,8 +1616,12 @@ std::vector<int64_t> runWindow(Ctx* c, const std::vector<Scene*>& ss, const std:
             }
             HIPCHK(hipStreamEndCapture(st, &g));
             hipError_t e = hipGraphInstantiate(&X.winExec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) { X.winExec = nullptr; throw WdmError(WDM_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e)); }
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                X.winExec = nullptr;
+                throw WdmError(WDM_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+            }
+            X.winGraph = g;
             X.winKey = key;
             X.graphCaptures++;
         }
@@ -2081,6 +2102,7 @@ int wdm_scene_free(wdm_ctx* c, int32_t sid) {
     return guard(c, [&] {
         Scene& S = getScene(c, sid);
         HIPCHK(hipStreamSynchronize(c->stream));
+        c->sh->dropGraph(c);  // no window graph outlives the memory of a scene it stepped
         S.freeAll();
         c->sh->freeSlots.push_back(S.slot);
         c->scenes.erase(sid);
