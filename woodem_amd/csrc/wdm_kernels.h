@@ -287,8 +287,7 @@ struct Map {
 };
 // Every thread searches (the same uniform loads, no barrier); the result is uniform, so
 // the descriptor fields that follow are read with scalar loads.
-__device__ __forceinline__ int mapBlock(const Map& m, int64_t& lblk) {
-    const int64_t b = blockIdx.x;
+__device__ __forceinline__ int mapBlockAt(const Map& m, int64_t b, int64_t& lblk) {
     int lo = 0, hi = m.n - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
@@ -296,6 +295,14 @@ __device__ __forceinline__ int mapBlock(const Map& m, int64_t& lblk) {
     }
     lblk = b - m.bprefix[lo];
     return lo;
+}
+__device__ __forceinline__ int mapBlock(const Map& m, int64_t& lblk) { return mapBlockAt(m, blockIdx.x, lblk); }
+// The hardware hands the blocks of a launch to the 8 XCDs round robin (block b to XCD b % 8).
+// For a launch of a multiple of 8 blocks this gives XCD x the consecutive logical blocks
+// [x n/8, (x+1) n/8), so that work on neighbouring data shares one XCD's L2.
+__device__ __forceinline__ int64_t xcdLogicalBlock() {
+    const int64_t n = gridDim.x, b = blockIdx.x;
+    return (b & 7) * (n >> 3) + (b >> 3);
 }
 
 // ---------------- block reductions (fixed tree: deterministic) ----------------
@@ -1086,158 +1093,214 @@ __device__ __forceinline__ bool pairTest(const SceneDesc& S, const GridArgs& g, 
     return ov && mayCollideD(p, S.n.master, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), S.a.loneMask);
 }
 
-// work items per bound in k_pairs: the 14 cells of the half stencil, then walls and the
-// large bounds listed in the own cell
-#define PAIR_ITEMS 15
-// wavefront queue of candidates that passed the FP32 box test: (owner thread, partner's entry)
+// Candidate pairs, one wavefront per grid cell.  The bounds of a cell (in the (scene, cell)
+// order of the sorted entries) sit one per lane, 64 at a time; the wavefront walks the
+// entries of its own cell (later ones only) and of the 13 "forward" neighbour cells, one
+// partner entry per iteration for all lanes: the partner's FP32 box is one uniform (scalar)
+// load and every lane tests its own box against it, so a partner is read once per cell
+// instead of once per bound.  Lanes whose boxes may overlap queue (own entry, partner entry)
+// in LDS; the exact tests then run over the queue with every lane busy.  Every unordered pair
+// of bounds that fit a cell is visited once.  Each own bound also tests every unbinned
+// particle (walls) and the large bounds listed in its cell (k_large_cells); the wavefront of
+// the large list's own cell (nCells) pairs large bounds with later ones when their masks
+// allow it.  Conservative tests reject most candidates cheaply (FP32 boxes, then an FP64
+// minimum-image test on bound centres); the decision itself is the collider's own
+// closed-interval 3-axis overlap of the stored bounds plus mayCollide, so the set equals the
+// reference's and the grid only prunes.
+// Each wavefront writes its accepted pairs, key (t << 2 idBits | min id << idBits | max id)
+// and packed periods, into its own region of `wcap` entries (no atomics: wavefront w owns
+// entries [w wcap, (w+1) wcap)) and their number to wcnt[w]; a count above wcap (the pairs
+// past it are not written) raises *wmax, and the host runs the pass again with larger
+// regions.  k_compact_pairs then packs the regions in wavefront order.  The order does not
+// matter: the keys are unique and sorted afterwards.
 #define PAIR_QUEUE 512
-template <int MODE>
-__global__ void __launch_bounds__(WDM_BLOCK) k_pairs(const SceneDesc* D, const FRScene* FR, Map m, const uint32_t* ckeys, const int32_t* cvals,
-                                                    const int32_t* cstart, const int32_t* cend, const int32_t* lstart, const int32_t* lend,
-                                                    const int32_t* lvals, const float4* boxes, int32_t* counts, const int64_t* offs,
-                                                    uint64_t* okeys, int32_t* operiods, int idBits) {
+#define PAIR_UNROLL 4
+__device__ __forceinline__ bool stencilCell(const GridArgs& g, uint32_t ck, int q, int64_t& cell) {
+    // half-stencil offset q: (0,0,0), (1,0,0), (-1..1,1,0), (-1..1,-1..1,1)
+    int dx, dy, dz;
+    if (q == 0) { dx = 0; dy = 0; dz = 0; }
+    else if (q == 1) { dx = 1; dy = 0; dz = 0; }
+    else if (q < 5) { dx = q - 3; dy = 1; dz = 0; }
+    else { dx = (q - 5) % 3 - 1; dy = (q - 5) / 3 - 1; dz = 1; }
+    // periodic axes with fewer than 3 cells hold a single cell (no neighbours); aperiodic axes always scan +-1
+    if ((dx && g.periodic && g.n[0] < 3) || (dy && g.periodic && g.n[1] < 3) || (dz && g.periodic && g.n[2] < 3)) return false;
+    const int cx = ck % g.n[0], cy = (ck / g.n[0]) % g.n[1], cz = ck / (g.n[0] * g.n[1]);
+    int z = cz + dz, y = cy + dy, x = cx + dx;
+    if (g.periodic) {
+        z = (z + g.n[2]) % g.n[2]; y = (y + g.n[1]) % g.n[1]; x = (x + g.n[0]) % g.n[0];
+    } else if (z < 0 || z >= g.n[2] || y < 0 || y >= g.n[1] || x < 0 || x >= g.n[0]) return false;
+    cell = (int64_t)(z * g.n[1] + y) * g.n[0] + x;
+    return true;
+}
+__global__ void __launch_bounds__(WDM_BLOCK) k_cell_pairs(const SceneDesc* D, const FRScene* FR, Map m, const int32_t* cvals,
+                                                         const int32_t* cstart, const int32_t* cend, const int32_t* lstart,
+                                                         const int32_t* lend, const int32_t* lvals, const float4* boxes,
+                                                         int32_t* wcnt, int32_t* wmax, int32_t wcap, uint64_t* okeys, int32_t* operiods,
+                                                         int idBits) {
     __shared__ int2 queue[WDM_BLOCK / 64][PAIR_QUEUE];
-    __shared__ int accepted[WDM_BLOCK];  // per thread: pairs found through the queue
-    __shared__ int ownId[WDM_BLOCK];
-    __shared__ int64_t ownOff[WDM_BLOCK];
+    // blocks in XCD-contiguous order (the launch is padded to a multiple of 8 blocks): the
+    // cells of one XCD are a contiguous range, whose neighbours mostly share its L2
+    const int64_t lbk = xcdLogicalBlock();
+    if (lbk >= m.bprefix[m.n]) return;  // padding
     int64_t lb;
-    const int tt = mapBlock(m, lb);
+    const int tt = mapBlockAt(m, lbk, lb);
     const FRScene& F = FR[tt];
     const SceneDesc& S = D[F.desc];
     const PartView p = S.p;
     const GridArgs g = F.g;  // by value: the loops below store to global memory
     uint32_t* err = S.words + 2;
-    const int* unb = S.unb;
-    const int nunb = S.nunb;
-    const int64_t cellBase = F.cellBase;
-    const int nCells = F.nCells;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t item = lb * WDM_BLOCK + threadIdx.x;
-    const int li = (int)(item / PAIR_ITEMS), sItem = (int)(item % PAIR_ITEMS);
-    // every lane stays to the end: the box scan below is a wavefront-wide loop
-    const bool valid = li < S.nbinned;
-    const int64_t i = F.binStart + (valid ? li : 0);
-    const int64_t q = i * PAIR_ITEMS + sItem;  // batch-wide item index
+    const int64_t gw = lbk * (WDM_BLOCK / 64) + wv;  // this wavefront's region and count
+    const int64_t cellL = lb * (WDM_BLOCK / 64) + wv;                 // its cell (nCells: the large list)
+    int cntW = 0;                                                     // pairs found (the same in every lane)
+    GP uint64_t* rk = gpc(okeys) + gw * (int64_t)wcap;
+    GP int32_t* rp = gpc(operiods) + gw * (int64_t)wcap;
+    const int64_t cellBase = F.cellBase;
+    const int s0 = cellL <= F.nCells ? cstart[cellBase + cellL] : -1;  // -1: empty cell or past the scene's cells
+    if (s0 < 0) {
+        if (lane == 0) wcnt[gw] = 0;
+        return;  // the whole wavefront leaves together
+    }
+    const bool largeCell = cellL == F.nCells;
     const uint64_t tkey = (uint64_t)tt << (2 * idBits);
-    const int id = valid ? cvals[i] : 0;
-    const uint32_t ck = valid ? (uint32_t)(ckeys[i] - cellBase) : 0u;
-    const bool large = (int)ck == nCells;
-    const int64_t o = (MODE && valid) ? offs[q] : 0;
-    accepted[threadIdx.x] = 0;
-    ownId[threadIdx.x] = id;
-    ownOff[threadIdx.x] = o;
-    // stencil items of a bound that fits a cell: the entries of one neighbour cell, tested
-    // through their FP32 boxes in a wavefront-wide loop; survivors are queued
-    int s = 0, len = 0;
-    float4 own = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (valid && sItem < 14 && !large) {
-        // half-stencil offset of this item: (0,0,0), (1,0,0), (-1..1,1,0), (-1..1,-1..1,1)
-        int dx, dy, dz;
-        if (sItem == 0) { dx = 0; dy = 0; dz = 0; }
-        else if (sItem == 1) { dx = 1; dy = 0; dz = 0; }
-        else if (sItem < 5) { dx = sItem - 3; dy = 1; dz = 0; }
-        else { dx = (sItem - 5) % 3 - 1; dy = (sItem - 5) / 3 - 1; dz = 1; }
-        // periodic axes with fewer than 3 cells hold a single cell (no neighbours); aperiodic axes always scan +-1
-        const bool okx = dx == 0 || !g.periodic || g.n[0] >= 3, oky = dy == 0 || !g.periodic || g.n[1] >= 3,
-                   okz = dz == 0 || !g.periodic || g.n[2] >= 3;
-        if (okx && oky && okz) {
-            const int cx = ck % g.n[0], cy = (ck / g.n[0]) % g.n[1], cz = ck / (g.n[0] * g.n[1]);
-            int z = cz + dz, y = cy + dy, x = cx + dx;
-            bool in = true;
-            if (g.periodic) {
-                z = (z + g.n[2]) % g.n[2]; y = (y + g.n[1]) % g.n[1]; x = (x + g.n[0]) % g.n[0];
-            } else in = !(z < 0 || z >= g.n[2] || y < 0 || y >= g.n[1] || x < 0 || x >= g.n[0]);
-            if (in) {
-                const int64_t cell = cellBase + (z * g.n[1] + y) * g.n[0] + x;
-                s = cstart[cell];
-                if (s >= 0) {
-                    const int e = cend[cell];
-                    if (sItem == 0) s = (int)i + 1;  // own cell: later entries only
-                    len = e > s ? e - s : 0;
-                    own = boxes[i];
-                } else s = 0;
+    const int e0 = cend[cellBase + cellL];
+    // periodic wrap of the FP32 box test (zero on aperiodic grids: the wrap term vanishes)
+    const float wLx = g.periodic ? g.Lf[0] : 0.f, wLy = g.periodic ? g.Lf[1] : 0.f, wLz = g.periodic ? g.Lf[2] : 0.f;
+    const float wIx = g.periodic ? g.invLf[0] : 0.f, wIy = g.periodic ? g.invLf[1] : 0.f, wIz = g.periodic ? g.invLf[2] : 0.f;
+    // the pairs of the lanes with `ok`, in lane order, into this wavefront's region
+    auto emit = [&](bool ok, int a, int b, const int per[3]) {
+        const uint64_t bal = __ballot(ok);
+        if (!bal) return;
+        if (ok) {
+            const int k = cntW + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            if (k < wcap) {
+                const int lo = a < b ? a : b, hi = a < b ? b : a;
+                rk[k] = tkey | ((uint64_t)(uint32_t)lo << idBits) | (uint32_t)hi;
+                rp[k] = packCd(per[0], per[1], per[2]);
             }
         }
-    }
-    int most = len;
-    for (int w = 32; w > 0; w >>= 1) most = max(most, __shfl_xor(most, w));
-    int queued = 0;  // the same in every lane
-    // exact tests of the queued candidates, one per lane
-    auto flush = [&]() {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        for (int b = 0; b < queued; b += 64) {
-            if (b + lane < queued) {
-                const int2 e = queue[wv][b + lane];
-                const int oid = ownId[e.x];
-                const int j = cvals[e.y];
-                int per[3];
-                if (pairTest(S, g, oid, p.bmin[oid], p.bmax[oid], ptShape(p.ptype[oid]), j, true, err, per)) {
-                    const int slot = atomicAdd(&accepted[e.x], 1);
-                    if (MODE) {
-                        const int lo = oid < j ? oid : j, hi = oid < j ? j : oid;
-                        okeys[ownOff[e.x] + slot] = tkey | ((uint64_t)(uint32_t)lo << idBits) | (uint32_t)hi;
-                        operiods[ownOff[e.x] + slot] = packCd(per[0], per[1], per[2]);
-                    }
-                }
-            }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        queued = 0;
+        cntW += (int)__popcll(bal);
     };
-    // four entries per round, their loads issued together; a round queues at most 256
-    for (int t0 = 0; t0 < most; t0 += 4) {
-        if (queued > PAIR_QUEUE - 256) flush();
-        float4 bx[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-            if (t0 + u < len) bx[u] = boxes[s + t0 + u];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const bool pass = t0 + u < len && boxesMayOverlap(own, bx[u], g);
-            const uint64_t bal = __ballot(pass);
-            if (pass) {
-                const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                queue[wv][queued + rank] = make_int2((int)threadIdx.x, s + t0 + u);  // (owner, entry)
+    for (int k0 = s0; k0 < e0; k0 += 64) {
+        const int i = k0 + lane;  // this lane's own entry
+        const bool valid = i < e0;
+        const float4 own = valid ? boxes[i] : make_float4(0.f, 0.f, 0.f, -1e30f);
+        int queued = 0;  // the same in every lane
+        auto flush = [&]() {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int b = 0; b < queued; b += 64) {
+                bool ok = false;
+                int oid = 0, j = 0;
+                int per[3] = {0, 0, 0};
+                if (b + lane < queued) {
+                    const int2 e = queue[wv][b + lane];
+                    oid = cvals[e.x];
+                    j = cvals[e.y];
+                    ok = pairTest(S, g, oid, p.bmin[oid], p.bmax[oid], ptShape(p.ptype[oid]), j, true, err, per);
+                }
+                emit(ok, oid, j, per);
             }
-            queued += (int)__popcll(bal);
-        }
-    }
-    flush();
-    // the other items test their partners directly (walls, listed large bounds, large-large)
-    int cnt = 0;
-    if (valid && (sItem == 14 || (large && sItem == 0 && F.largePairs))) {
-        const double4 a0 = p.bmin[id], a1 = p.bmax[id];
-        const int sa = ptShape(p.ptype[id]);
-        auto tryPair = [&](int j, bool binned) {
-            int per[3];
-            if (!pairTest(S, g, id, a0, a1, sa, j, binned, err, per)) return;
-            if (MODE) {
-                const int lo = id < j ? id : j, hi = id < j ? j : id;
-                okeys[o + cnt] = tkey | ((uint64_t)(uint32_t)lo << idBits) | (uint32_t)hi;
-                operiods[o + cnt] = packCd(per[0], per[1], per[2]);
-            }
-            cnt++;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            queued = 0;
         };
-        if (sItem == 0) {  // large-large: later entries of the large list
-            const int e = cend[cellBase + nCells];
-            for (int k = (int)i + 1; k < e; k++) tryPair(cvals[k], true);
-        } else {
-            if (!large && lstart) {  // large bounds listed in the own cell
-                const int64_t oc = cellBase + ck;
-                const int ls = lstart[oc];
-                if (ls >= 0) {
-                    const int le = lend[oc];
-                    for (int k = ls; k < le; k++) tryPair(lvals[k], true);
+        // the own cell (entries after each lane's own) and, for cells of the grid, the forward neighbours
+        for (int q = 0; q < (largeCell ? 1 : 14); q++) {
+            int js, je;
+            if (q == 0) {
+                if (largeCell && !F.largePairs) break;
+                js = k0 + 1;
+                je = e0;
+            } else {
+                int64_t nc;
+                if (!stencilCell(g, (uint32_t)cellL, q, nc)) continue;
+                js = cstart[cellBase + nc];
+                if (js < 0) continue;
+                je = cend[cellBase + nc];
+            }
+            // the partners 64 at a time: one coalesced load (lane l holds entry jc + l), then each
+            // partner's box is broadcast to every lane from the register of the lane holding it
+            // (a scalar read) and tested without branches (boxesMayOverlap's arithmetic, with
+            // the periodic wrap multiplied away on aperiodic grids)
+            const int skipBelow = q == 0 ? i : -1;  // own cell: partners after this lane's entry
+            for (int jc = js; jc < je; jc += 64) {
+                const int jn = je - jc < 64 ? je - jc : 64;
+                const float4 pb = lane < jn ? boxes[jc + lane] : make_float4(0.f, 0.f, 0.f, -1e30f);
+                auto test = [&](int jj) {
+                    const float bxx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pb.x), jj));
+                    const float bxy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pb.y), jj));
+                    const float bxz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pb.z), jj));
+                    const float bxw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pb.w), jj));
+                    float dx = bxx - own.x, dy = bxy - own.y, dz = bxz - own.z;
+                    dx -= wLx * rintf(dx * wIx);
+                    dy -= wLy * rintf(dy * wIy);
+                    dz -= wLz * rintf(dz * wIz);
+                    const float lim = (own.w + bxw) + g.pfMargin;
+                    const int j = jc + jj;
+                    const bool pass = (valid & (j > skipBelow)) & !((fabsf(dx) > lim) | (fabsf(dy) > lim) | (fabsf(dz) > lim));
+                    const uint64_t bal = __ballot(pass);
+                    if (pass) {
+                        const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+                        queue[wv][queued + rank] = make_int2(i, j);
+                    }
+                    queued += (int)__popcll(bal);
+                };
+                int jj = 0;
+                for (; jj + PAIR_UNROLL <= jn; jj += PAIR_UNROLL) {
+                    if (queued > PAIR_QUEUE - 64 * PAIR_UNROLL) flush();
+#pragma unroll
+                    for (int u = 0; u < PAIR_UNROLL; u++) test(jj + u);
+                }
+                for (; jj < jn; jj++) {
+                    if (queued > PAIR_QUEUE - 64) flush();
+                    test(jj);
                 }
             }
-            for (int u = 0; u < nunb; u++) tryPair(unb[u], false);
+        }
+        flush();
+        // walls, and the large bounds listed in this cell, tested directly
+        const int id = valid ? cvals[i] : 0;
+        const double4 a0 = valid ? p.bmin[id] : make_double4(0., 0., 0., 0.), a1 = valid ? p.bmax[id] : make_double4(0., 0., 0., 0.);
+        const int sa = valid ? ptShape(p.ptype[id]) : 0;
+        if (!largeCell && lstart) {
+            const int ls = lstart[cellBase + cellL];
+            if (ls >= 0) {
+                const int le = lend[cellBase + cellL];
+                for (int k = ls; k < le; k++) {
+                    int per[3] = {0, 0, 0};
+                    const int j = lvals[k];
+                    const bool ok = valid && pairTest(S, g, id, a0, a1, sa, j, true, err, per);
+                    emit(ok, id, j, per);
+                }
+            }
+        }
+        for (int u = 0; u < S.nunb; u++) {
+            int per[3] = {0, 0, 0};
+            const int j = S.unb[u];
+            const bool ok = valid && pairTest(S, g, id, a0, a1, sa, j, false, err, per);
+            emit(ok, id, j, per);
         }
     }
-    if (!MODE && valid) counts[q] = cnt + accepted[threadIdx.x];
+    if (lane == 0) {
+        wcnt[gw] = cntW;
+        if (cntW > wcap) atomicMax(wmax, cntW);
+    }
+}
+// the wavefront regions of k_cell_pairs packed one after the other (off: exclusive prefix of
+// their counts), one wavefront per region
+__global__ void __launch_bounds__(WDM_BLOCK) k_compact_pairs(const int32_t* wcnt, const int64_t* off, int64_t nw, int32_t wcap,
+                                                            const uint64_t* rk, const int32_t* rp, uint64_t* okeys, int32_t* operiods) {
+    const int64_t w = (int64_t)blockIdx.x * (WDM_BLOCK / 64) + (threadIdx.x >> 6);
+    if (w >= nw) return;
+    const int n = wcnt[w];
+    const int64_t o = off[w];
+    for (int k = threadIdx.x & 63; k < n; k += 64) {
+        okeys[o + k] = rk[w * (int64_t)wcap + k];
+        operiods[o + k] = rp[w * (int64_t)wcap + k];
+    }
 }
 
 __device__ __forceinline__ int64_t lowerBound64(const uint64_t* a, int64_t n, uint64_t k) {

@@ -98,6 +98,7 @@ struct Timing {
     // event completes when the GPU reaches the kernel, not while it idles for the launch
     std::vector<hipEvent_t> ev;
     std::vector<const char*> pend;
+    int open = 0;  // timers whose end event is not recorded yet
 };
 
 struct Shared;
@@ -116,6 +117,7 @@ struct Ctx {
     int window = 16;
     int graphs = 1;
     double cellFactor = 1.;
+    int32_t waveCap = 256;  // pairs a cell's wavefront may find in a collider pass (grows when a pass needs more)
     hipError_t scratchErr = hipSuccess;  // looking up the device address of `scratch` at wdm_create
 };
 
@@ -258,8 +260,13 @@ struct Shared {
     // collider full-run scratch, batch-wide
     DBuf<double> extPart, extOut;
     DBuf<uint32_t> ckeys, ckeys2;
-    DBuf<int32_t> cvals, cvals2, cstart, cend, cnt32, ooFlag, oslot;
-    DBuf<int64_t> counts, offs, ooFlag64, ooRank, rflag, rscan, starts, nrflag, nrscan;
+    DBuf<int32_t> cvals, cvals2, cstart, cend, ooFlag, oslot;
+    // candidate pairs: per wavefront of k_cell_pairs its count, prefix and region of found pairs
+    DBuf<int32_t> wcnt, wmax;
+    DBuf<int64_t> wcnt64, woff;
+    DBuf<uint64_t> regK;
+    DBuf<int32_t> regP;
+    DBuf<int64_t> ooFlag64, ooRank, rflag, rscan, starts, nrflag, nrscan;
     DBuf<uint64_t> candK, candK2, entK, entK2;
     DBuf<int32_t> candP, candP2;
     DBuf<uint32_t> entV, entV2;
@@ -305,7 +312,7 @@ struct Shared {
             s->free(c);
         release(c, epartL); release(c, epartC); release(c, extPart); release(c, extOut);
         release(c, ckeys); release(c, ckeys2); release(c, cvals); release(c, cvals2); release(c, cstart); release(c, cend);
-        release(c, cnt32); release(c, ooFlag); release(c, oslot); release(c, counts); release(c, offs); release(c, ooFlag64);
+        release(c, wcnt); release(c, wmax); release(c, wcnt64); release(c, woff); release(c, regK); release(c, regP); release(c, ooFlag); release(c, oslot); release(c, ooFlag64);
         release(c, ooRank); release(c, rflag); release(c, rscan); release(c, starts); release(c, nrflag); release(c, nrscan); release(c, candK); release(c, candK2);
         release(c, entK); release(c, entK2); release(c, candP); release(c, candP2); release(c, entV); release(c, entV2);
         release(c, temp); release(c, disp); release(c, ddOut);
@@ -440,14 +447,17 @@ void timingFlush(Ctx* c) {  // reads the pending event pairs into the per-kernel
         a.second += 1;
     }
     T.pend.clear();
+    (void)hipGetLastError();  // an event error must not surface at the next kernel launch
 }
-struct KTimer {  // HIP-event timing of one launch (or phase) on the context stream
+struct KTimer {  // HIP-event timing of one launch (or phase, timers may nest) on the context stream
     Ctx* c; const char* name; bool on; size_t k = 0;
     KTimer(Ctx* c_, const char* n) : c(c_), name(n), on(c_->timing.on) {
         if (!on) return;
         Timing& T = c->timing;
-        if (T.pend.size() >= 4096) timingFlush(c);
-        k = T.pend.size();
+        if (T.pend.size() >= 4096 && T.open == 0) timingFlush(c);
+        k = T.pend.size();  // this timer's slot, reserved now so that nested timers take later ones
+        T.pend.push_back(name);
+        T.open++;
         while (T.ev.size() < 2 * k + 2) {
             hipEvent_t e = nullptr;
             HIPCHK(hipEventCreate(&e));
@@ -458,7 +468,7 @@ struct KTimer {  // HIP-event timing of one launch (or phase) on the context str
     ~KTimer() {
         if (!on) return;
         (void)hipEventRecord(c->timing.ev[2 * k + 1], c->stream);
-        c->timing.pend.push_back(name);
+        c->timing.open--;
     }
 };
 
@@ -960,6 +970,7 @@ struct InsertionSortCollider {
         hipStream_t st = c->stream;
         const int nT = (int)L.size();
         if (nT == 0) return;
+        KTimer tAll(c, "full_run");  // the whole pass on the stream, host synchronisations included
         auto S_ = [&](int t) -> Scene& { return *B.ss[L[t]]; };
         int maxNp = 1;
         for (int t = 0; t < nT; t++) maxNp = std::max(maxNp, S_(t).np);
@@ -981,43 +992,60 @@ struct InsertionSortCollider {
             checkLaunch();
         }
         int64_t nbTot = 0;
-        buildGrids(B, L, F, bpB, mB, FRd, nbTot, c->cellFactor);
+        {
+            KTimer tg(c, "grid");
+            buildGrids(B, L, F, bpB, mB, FRd, nbTot, c->cellFactor);
+        }
         bool anyFacets = false;
         for (int t = 0; t < nT; t++) anyFacets |= S_(t).hasFacets;
         const bool large = anyFacets && listLarge(B, bpB, mB, FRd, nbTot, F[nT - 1].cellBase + F[nT - 1].nCells + 1);
 
-        // candidate pairs: count, scan, write, sort by (scene, min id, max id); PAIR_ITEMS
-        // threads per bound
+        // candidate pairs: one wavefront per cell writing into its own region, the regions
+        // packed (and the pass run again with larger regions when a cell found more pairs than
+        // they hold), then sorted by (scene, min id, max id)
         int64_t ncandTot = 0;
         {
-            const int64_t nq = nbTot * PAIR_ITEMS;
-            ensure(c, X.cnt32, nq); ensure(c, X.counts, nq); ensure(c, X.offs, (size_t)nq + 1);
-            std::vector<int64_t> bpQ = blockPrefix(nT, [&](int t) { return (int64_t)S_(t).binnedH.size() * PAIR_ITEMS; });
+            std::vector<int64_t> bpQ(nT + 1, 0);  // one wavefront per cell and per large list: 4 per block
+            for (int t = 0; t < nT; t++) bpQ[t + 1] = bpQ[t] + ((int64_t)F[t].nCells + 1 + 3) / 4;
             Map mQ = uploadMap(c, X.stMapQ, bpQ);
-            if (bpQ[nT] > 0) {
-                KTimer tm(c, "pairs_count");
-                hipLaunchKernelGGL(k_pairs<0>, dim3((unsigned)bpQ[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mQ, (const uint32_t*)X.ckeys2.p,
-                                   (const int32_t*)X.cvals2.p, (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p,
-                                   large ? (const int32_t*)X.lstart.p : nullptr, (const int32_t*)X.lend.p, (const int32_t*)X.lvals2.p,
-                                   (const float4*)X.cbox.p, X.cnt32.p, (const int64_t*)nullptr, (uint64_t*)nullptr, (int32_t*)nullptr, idBits);
-                checkLaunch();
-                hipLaunchKernelGGL(k_i32_to_i64, dim3(nblk(nq)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.cnt32.p, X.counts.p, nq);
-                checkLaunch();
-            }
-            scanInclusiveShift(c, X.counts.p, X.offs.p, nq);
+            const int64_t nW = 4 * bpQ[nT];
+            ensure(c, X.wcnt, std::max<int64_t>(nW, 1)); ensure(c, X.wcnt64, std::max<int64_t>(nW, 1));
+            ensure(c, X.woff, (size_t)nW + 1); ensure(c, X.wmax, 1);
             std::vector<int64_t> first(nT + 1);
-            for (int t = 0; t < nT; t++) first[t] = F[t].binStart * PAIR_ITEMS;
-            first[nT] = nq;
-            std::vector<int64_t> cs = segmentStarts(c, X.offs.p, first);
+            for (int t = 0; t < nT; t++) first[t] = 4 * bpQ[t];
+            first[nT] = nW;
+            std::vector<int64_t> cs;
+            for (int attempt = 0;; attempt++) {
+                const int32_t wcap = c->waveCap;
+                ensure(c, X.regK, std::max<int64_t>(nW * wcap, 1)); ensure(c, X.regP, std::max<int64_t>(nW * wcap, 1));
+                hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(WDM_BLOCK), 0, st, X.wmax.p, (int64_t)1, 0);
+                if (nW > 0) {
+                    KTimer tm(c, "pairs");
+                    hipLaunchKernelGGL(k_cell_pairs, dim3((unsigned)((bpQ[nT] + 7) / 8 * 8)), dim3(WDM_BLOCK), 0, st, B.D, FRd, mQ, (const int32_t*)X.cvals2.p,
+                                       (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p, large ? (const int32_t*)X.lstart.p : nullptr,
+                                       (const int32_t*)X.lend.p, (const int32_t*)X.lvals2.p, (const float4*)X.cbox.p, X.wcnt.p, X.wmax.p, wcap,
+                                       X.regK.p, X.regP.p, idBits);
+                    checkLaunch();
+                    hipLaunchKernelGGL(k_i32_to_i64, dim3(nblk(nW)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.wcnt.p, X.wcnt64.p, nW);
+                    checkLaunch();
+                }
+                scanInclusiveShift(c, X.wcnt64.p, X.woff.p, nW);
+                int32_t wm = 0;
+                HIPCHK(hipMemcpyAsync(&wm, X.wmax.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                cs = segmentStarts(c, X.woff.p, first);  // synchronises (wm is read too)
+                if (wm <= wcap) break;
+                if (attempt > 0) throw WdmError(WDM_E_STATE, "collider: pairs per cell changed between two passes");
+                c->waveCap = (int32_t)std::min<int64_t>(((int64_t)wm + wm / 8 + 63) / 64 * 64, INT32_MAX / 2);
+            }
             for (int t = 0; t < nT; t++) { F[t].candStart = cs[t]; F[t].nCand = cs[t + 1] - cs[t]; }
             ncandTot = cs[nT];
-            ensure(c, X.candK, ncandTot); ensure(c, X.candK2, ncandTot); ensure(c, X.candP, ncandTot); ensure(c, X.candP2, ncandTot);
+            ensure(c, X.candK, std::max<int64_t>(ncandTot, 1)); ensure(c, X.candK2, std::max<int64_t>(ncandTot, 1));
+            ensure(c, X.candP, std::max<int64_t>(ncandTot, 1)); ensure(c, X.candP2, std::max<int64_t>(ncandTot, 1));
             if (ncandTot > 0) {
-                KTimer tm(c, "pairs_write");
-                hipLaunchKernelGGL(k_pairs<1>, dim3((unsigned)bpQ[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mQ, (const uint32_t*)X.ckeys2.p,
-                                   (const int32_t*)X.cvals2.p, (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p,
-                                   large ? (const int32_t*)X.lstart.p : nullptr, (const int32_t*)X.lend.p, (const int32_t*)X.lvals2.p,
-                                   (const float4*)X.cbox.p, (int32_t*)nullptr, (const int64_t*)X.offs.p, X.candK.p, X.candP.p, idBits);
+                KTimer tm(c, "pairs_sort");
+                hipLaunchKernelGGL(k_compact_pairs, dim3((unsigned)((nW + 3) / 4)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.wcnt.p,
+                                   (const int64_t*)X.woff.p, nW, c->waveCap, (const uint64_t*)X.regK.p, (const int32_t*)X.regP.p, X.candK.p,
+                                   X.candP.p);
                 checkLaunch();
                 sortPairs(c, X.candK.p, X.candK2.p, X.candP.p, X.candP2.p, ncandTot, keyBits);
             }
@@ -1046,6 +1074,7 @@ struct InsertionSortCollider {
         ensure(c, X.ooRank, (size_t)nOldTot + 1);
         std::vector<int64_t> noor(nT, 0);
         if (nOldTot > 0) {
+            KTimer to(c, "old_flags");
             hipLaunchKernelGGL(k_flag_old<T>, dim3((unsigned)bpO[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mO, (const uint64_t*)X.candK2.p,
                                X.ooFlag.p, idBits);
             hipLaunchKernelGGL(k_i32_to_i64, dim3(nblk(nOldTot)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.ooFlag.p, X.ooFlag64.p, nOldTot);
