@@ -268,15 +268,29 @@ struct ConDesc {
 };
 // one scene of a batched collider full run: its grid and its segments in the batch-wide
 // scratch arrays (bound entries in cell order, candidate pairs, old and new contacts)
+// The collider's grid of a scene has nLvl levels of cells, level l of cell width lw[l]
+// (twice the width of level l-1), holding the bounds whose largest extent is at most
+// lthr[l] = lw[l] / 1.001 (and above lthr[l-1]); bounds wider than the top level's
+// threshold (facets larger than every sphere) are "large" and kept out of the cells.
+// Cell keys of the scene: lbase[l] + (z n_y + y) n_x + x at level l, and lbase[nLvl] ==
+// nCells for the large bounds.  g holds what the levels share (origin, periodic sizes,
+// FP32 slack) and level 0's cells.
+#define MAXLVL 4
 struct FRScene {
     GridArgs g;
     int64_t cellBase, binStart, candStart, nCand, oldStart, nOld, newStart, nNew;
+    int64_t rowBase;  // first row of the scene in batch-wide per-particle arrays (np + 1 rows per scene)
     GP int64_t* csrOff;
     GP uint32_t* entV;
     double wLarge;  // bounds wider than this (facets larger than every sphere) skip the grid cells
+    double lw[MAXLVL][3];
+    double lthr[MAXLVL];
+    int32_t ln[MAXLVL][3];
+    int32_t lbase[MAXLVL + 1];
+    int32_t nLvl;
     int32_t nCells;      // grid cells of the scene; cell nCells lists its large bounds
     int32_t largePairs;  // large bounds may pair with each other (masks allow it)
-    int32_t desc, pad;
+    int32_t desc;
 };
 
 // blocks of one launch spread over several scenes: scene t owns blocks
@@ -297,13 +311,6 @@ __device__ __forceinline__ int mapBlockAt(const Map& m, int64_t b, int64_t& lblk
     return lo;
 }
 __device__ __forceinline__ int mapBlock(const Map& m, int64_t& lblk) { return mapBlockAt(m, blockIdx.x, lblk); }
-// The hardware hands the blocks of a launch to the 8 XCDs round robin (block b to XCD b % 8).
-// For a launch of a multiple of 8 blocks this gives XCD x the consecutive logical blocks
-// [x n/8, (x+1) n/8), so that work on neighbouring data shares one XCD's L2.
-__device__ __forceinline__ int64_t xcdLogicalBlock() {
-    const int64_t n = gridDim.x, b = blockIdx.x;
-    return (b & 7) * (n >> 3) + (b >> 3);
-}
 
 // ---------------- block reductions (fixed tree: deterministic) ----------------
 // bits: (1 << energy slot) for "slot created" (EnergyTracker::add, core/EnergyTracker.cpp:10-17),
@@ -784,23 +791,40 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_bounds(const SceneDesc* D, const 
     p.bmax[i] = make_double4(mx.x, mx.y, mx.z, 0.);
 }
 
-// extent reduction over binned (finite-bound) particles: min corner, max corner, max extent
-__global__ void __launch_bounds__(WDM_BLOCK) k_extent(const SceneDesc* D, const FRScene* FR, Map m, double* part) {
+// extent reduction over binned (finite-bound) particles: min corner, max corner, max extent;
+// and per scene a histogram of the bounds' largest extents in quarter octaves
+// (bin = floor(4 log2 ext) + 128, clamped to [0, EXT_BINS)), from which the host picks the
+// grid levels
+#define EXT_BINS 256
+__device__ __forceinline__ int extBin(double ext) {
+    if (!(ext > 0)) return 0;
+    const double b = floor(4. * log2(ext)) + 128.;
+    return b < 0. ? 0 : (b > EXT_BINS - 1 ? EXT_BINS - 1 : (int)b);
+}
+__global__ void __launch_bounds__(WDM_BLOCK) k_extent(const SceneDesc* D, const FRScene* FR, Map m, double* part, uint32_t* hist) {
     __shared__ double sh[8][WDM_BLOCK];
+    __shared__ uint32_t hs[EXT_BINS];
     int64_t lb;
-    const SceneDesc& S = D[FR[mapBlock(m, lb)].desc];
+    const int st = mapBlock(m, lb);
+    const SceneDesc& S = D[FR[st].desc];
     const PartView p = S.p;
     int t = threadIdx.x;
     int i = (int)(lb * WDM_BLOCK + t);
     const double inf = __builtin_inf();
     double v[8] = {inf, inf, inf, -inf, -inf, -inf, 0., 0.};
+    for (int k = t; k < EXT_BINS; k += WDM_BLOCK) hs[k] = 0;
+    __syncthreads();
     if (i < S.nbinned) {
         int id = S.binned[i];
         double4 a = p.bmin[id], b = p.bmax[id];
         v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = b.x; v[4] = b.y; v[5] = b.z;
         v[6] = fmax(fmax(b.x - a.x, b.y - a.y), b.z - a.z);
         if (ptShape(p.ptype[id]) == 1) v[7] = v[6];
+        atomicAdd(&hs[extBin(v[6])], 1u);
     }
+    __syncthreads();
+    for (int k = t; k < EXT_BINS; k += WDM_BLOCK)
+        if (hs[k]) atomicAdd(&hist[(size_t)st * EXT_BINS + k], hs[k]);
     for (int k = 0; k < 8; k++) sh[k][t] = v[k];
     __syncthreads();
     for (int s = WDM_BLOCK / 2; s > 0; s >>= 1) {
@@ -834,52 +858,224 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_extent_finalize(Map m, const doub
     if (tid < 8) out[(size_t)t * 8 + tid] = sh[tid][0];
 }
 
-__device__ __forceinline__ int cellCoord(const GridArgs& g, int d, double x) {
+__global__ void k_fill_i32(int32_t* a, int64_t n, int32_t v) {
+    int64_t i = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// ---------------- exclusive prefix sums: per-tile sums, one workgroup over the tile sums,
+// then each tile scanned with its offset (out[0..n], out[n] = the total) ----------------
+#define SCAN_ITEMS 8
+#define SCAN_TILE (WDM_BLOCK * SCAN_ITEMS)
+#define SCAN_TOP 1024
+template <typename V>
+__device__ __forceinline__ V waveInclusiveScan(V x) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const V y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+// exclusive prefix of x over the block's threads in thread order, and the block total;
+// ws: one LDS slot per wavefront
+__device__ __forceinline__ int64_t blockExclusiveScan(int64_t x, int64_t* ws, int64_t& total) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int64_t inc = waveInclusiveScan(x);
+    if (lane == 63) ws[wv] = inc;
+    __syncthreads();
+    int64_t off = 0, tot = 0;
+    for (int k = 0; k < nw; k++) {
+        const int64_t s = ws[k];
+        off += k < wv ? s : 0;
+        tot += s;
+    }
+    __syncthreads();
+    total = tot;
+    return off + inc - x;
+}
+template <typename TIn>
+__global__ void __launch_bounds__(WDM_BLOCK) k_scan_part(const TIn* in, int64_t n, int64_t* tsum) {
+    __shared__ int64_t ws[WDM_BLOCK / 64];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++)
+        if (base + k < n) s += (int64_t)in[base + k];
+    int64_t tot;
+    (void)blockExclusiveScan(s, ws, tot);
+    if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+// one workgroup: tsum[0..nt) to its exclusive prefix, the total to *total
+template <typename TOut>
+__global__ void __launch_bounds__(SCAN_TOP) k_scan_top(int64_t* tsum, int64_t nt, TOut* total) {
+    __shared__ int64_t ws[SCAN_TOP / 64];
+    int64_t carry = 0;
+    for (int64_t b0 = 0; b0 < nt; b0 += SCAN_TOP) {
+        const int64_t i = b0 + threadIdx.x;
+        const int64_t x = i < nt ? tsum[i] : 0;
+        int64_t tot;
+        const int64_t ex = blockExclusiveScan(x, ws, tot);
+        if (i < nt) tsum[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = (TOut)carry;
+}
+template <typename TIn, typename TOut>
+__global__ void __launch_bounds__(WDM_BLOCK) k_scan_fin(const TIn* in, int64_t n, const int64_t* tsum, TOut* out) {
+    __shared__ int64_t ws[WDM_BLOCK / 64];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int64_t v[SCAN_ITEMS];
+    int64_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        v[k] = base + k < n ? (int64_t)in[base + k] : 0;
+        s += v[k];
+    }
+    int64_t tot;
+    int64_t run = tsum[blockIdx.x] + blockExclusiveScan(s, ws, tot);
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        if (base + k < n) out[base + k] = (TOut)run;
+        run += v[k];
+    }
+}
+
+// ---------------- bucket sort: 64-bit items grouped by a bucket index, each bucket in
+// ascending item order (the high 32 bits of the items of one bucket are distinct) ----------------
+// The producer counts its items per bucket with atomics (the count returned is the item's
+// slot) and scatters them to tmp[off[b] + slot] with tmpB = b, off the exclusive prefix of
+// the counts.  k_bucket_rank then places each item of a bucket of at most BKT_SMALL items
+// by counting the smaller items of its bucket; longer buckets are listed and sorted by
+// k_bucket_long, one workgroup each (bitonic sort in LDS, merged through global memory
+// when a bucket exceeds BKT_LDS items).  The writer w(b, position, item) stores the result.
+#define BKT_SMALL 64
+#define BKT_LDS 8192
+#define BKT_THREADS 1024
+// n: items, or (total non-null) at most n items, *total of them
+template <typename OffT, class W>
+__global__ void __launch_bounds__(WDM_BLOCK) k_bucket_rank(const uint64_t* tmp, const uint32_t* tmpB, const OffT* off, int64_t n,
+                                                          const OffT* total, uint32_t* longList, uint32_t* longCnt, W w) {
+    const int64_t q = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
+    if (q >= (total ? (int64_t)*total : n)) return;
+    const uint32_t b = tmpB[q];
+    const int64_t s = (int64_t)off[b], e = (int64_t)off[b + 1];
+    if (e - s > BKT_SMALL) {
+        if (q == s) longList[atomicAdd(longCnt, 1u)] = b;
+        return;
+    }
+    const uint64_t x = tmp[q];
+    int r = 0;
+    for (int64_t k = s; k < e; k++) r += tmp[k] < x ? 1 : 0;
+    w(b, s + r, x);
+}
+__device__ __forceinline__ void bitonicLds(uint64_t* sh, int P) {
+    for (int k = 2; k <= P; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P; i += blockDim.x) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const uint64_t a = sh[i], b = sh[ixj];
+                    if ((a > b) == ((i & k) == 0)) {
+                        sh[i] = b;
+                        sh[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+// aux: scratch of the same size as tmp; tmp's entries of a long bucket are overwritten
+template <typename OffT, class W>
+__global__ void __launch_bounds__(BKT_THREADS) k_bucket_long(uint64_t* tmp, uint64_t* aux, const OffT* off, const uint32_t* longList,
+                                                            const uint32_t* longCnt, W w) {
+    __shared__ uint64_t sh[BKT_LDS];
+    const uint32_t nl = *longCnt;
+    for (uint32_t li = blockIdx.x; li < nl; li += gridDim.x) {
+        const uint32_t b = longList[li];
+        const int64_t s = (int64_t)off[b], len = (int64_t)off[b + 1] - s;
+        for (int64_t c0 = 0; c0 < len; c0 += BKT_LDS) {
+            const int cl = (int)(len - c0 < BKT_LDS ? len - c0 : BKT_LDS);
+            int P = 1;
+            while (P < cl) P <<= 1;
+            for (int k = threadIdx.x; k < P; k += BKT_THREADS) sh[k] = k < cl ? tmp[s + c0 + k] : ~0ull;
+            __syncthreads();
+            bitonicLds(sh, P);
+            for (int k = threadIdx.x; k < cl; k += BKT_THREADS) aux[s + c0 + k] = sh[k];
+            __syncthreads();
+        }
+        // sorted runs of width wd merged pairwise: an item's place is its index in its run
+        // plus the number of smaller items in the other run
+        uint64_t* src = aux + s;
+        uint64_t* dst = tmp + s;
+        for (int64_t wd = BKT_LDS; wd < len; wd <<= 1) {
+            for (int64_t k = threadIdx.x; k < len; k += BKT_THREADS) {
+                const int64_t r0 = (k / wd) * wd, p0 = ((k / wd) ^ 1) * wd;
+                const uint64_t x = src[k];
+                int64_t pos = k;
+                if (p0 < len) {
+                    int64_t lo = p0, hi = p0 + wd < len ? p0 + wd : len;
+                    while (lo < hi) {
+                        const int64_t mid = (lo + hi) >> 1;
+                        if (src[mid] < x) lo = mid + 1; else hi = mid;
+                    }
+                    pos = (r0 < p0 ? r0 : p0) + (k - r0) + (lo - p0);
+                }
+                dst[pos] = x;
+            }
+            __syncthreads();
+            uint64_t* t2 = src;
+            src = dst;
+            dst = t2;
+        }
+        for (int64_t k = threadIdx.x; k < len; k += BKT_THREADS) w(b, s + k, src[k]);
+        __syncthreads();
+    }
+}
+// scene of a batch-wide row (rowBase) or cell key (cellBase): the last scene starting at or before it
+__device__ __forceinline__ int sceneOfRow(const FRScene* FR, int nT, int64_t r) {
+    int lo = 0, hi = nT - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (FR[mid].rowBase <= r) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+__device__ __forceinline__ int sceneOfCell(const FRScene* FR, int nT, int64_t k) {
+    int lo = 0, hi = nT - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (FR[mid].cellBase <= k) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ---------------- collider grid ----------------
+// cell of coordinate x along axis d at level l (clamped to the grid)
+__device__ __forceinline__ int cellCoordL(const FRScene& F, int l, int d, double x) {
+    const int n = F.ln[l][d];
     int c;
-    if (g.periodic) {
-        double u = x / g.L[d];
+    if (F.g.periodic) {
+        double u = x / F.g.L[d];
         double f = u - floor(u);
-        c = (int)(f * g.n[d]);
+        c = (int)(f * n);
     } else {
-        c = (int)floor((x - g.origin[d]) / g.w[d]);
+        c = (int)floor((x - F.g.origin[d]) / F.lw[l][d]);
     }
-    return c < 0 ? 0 : (c >= g.n[d] ? g.n[d] - 1 : c);
+    return c < 0 ? 0 : (c >= n ? n - 1 : c);
 }
-
-// key = scene's first cell + cell of the bound's minimum corner; sorting the keys of a
-// batch orders entries by scene, then by cell
-__global__ void __launch_bounds__(WDM_BLOCK) k_cell_keys(const SceneDesc* D, const FRScene* FR, Map m, uint32_t* keys, int32_t* vals) {
-    int64_t lb;
-    const FRScene& F = FR[mapBlock(m, lb)];
-    const SceneDesc& S = D[F.desc];
-    int i = (int)(lb * WDM_BLOCK + threadIdx.x);
-    if (i >= S.nbinned) return;
-    const GridArgs g = F.g;  // by value: the loops below store to global memory
-    int id = S.binned[i];
-    double4 a = S.p.bmin[id], b = S.p.bmax[id];
-    int64_t cell;
-    if (fmax(fmax(b.x - a.x, b.y - a.y), b.z - a.z) > F.wLarge) cell = F.nCells;  // large: listed apart
-    else {
-        int cx = cellCoord(g, 0, a.x), cy = cellCoord(g, 1, a.y), cz = cellCoord(g, 2, a.z);
-        cell = (cz * g.n[1] + cy) * g.n[0] + cx;
-    }
-    keys[F.binStart + i] = (uint32_t)(F.cellBase + cell);
-    vals[F.binStart + i] = id;
+// level of a bound of largest extent ext (nLvl: large)
+__device__ __forceinline__ int boundLevel(const FRScene& F, double ext) {
+    int l = 0;
+    while (l < F.nLvl && ext > F.lthr[l]) l++;
+    return l;
 }
-
-// FP32 box of each bound entry in (scene, cell) order: centre (wrapped into the periodic
-// cell, or relative to the grid origin) and the largest half extent.  k_pairs scans a
-// neighbour cell's entries contiguously through these 16 bytes and reads a bound's exact
-// FP64 corners only when the FP32 test cannot rule the pair out.
-__global__ void __launch_bounds__(WDM_BLOCK) k_cell_boxes(const SceneDesc* D, const FRScene* FR, Map m, const int32_t* cvals, float4* boxes) {
-    int64_t lb;
-    const FRScene& F = FR[mapBlock(m, lb)];
-    const SceneDesc& S = D[F.desc];
-    const int li = (int)(lb * WDM_BLOCK + threadIdx.x);
-    if (li >= S.nbinned) return;
-    const int64_t i = F.binStart + li;
-    const int id = cvals[i];
-    const double4 a = S.p.bmin[id], b = S.p.bmax[id];
+// FP32 box of a bound: centre (wrapped into the periodic cell, or relative to the grid
+// origin) and the largest half extent.  The pair search scans a neighbour cell's entries
+// contiguously through these 16 bytes and reads a bound's exact FP64 corners only when the
+// FP32 test cannot rule the pair out.
+__device__ __forceinline__ float4 cellBox(const FRScene& F, const double4& a, const double4& b) {
     const double c3[3] = {.5 * (a.x + b.x), .5 * (a.y + b.y), .5 * (a.z + b.z)};
     double r[3];
     for (int d = 0; d < 3; d++) {
@@ -887,32 +1083,59 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_cell_boxes(const SceneDesc* D, co
         else r[d] = c3[d] - F.g.origin[d];
     }
     const double h = fmax(fmax(b.x - a.x, b.y - a.y), b.z - a.z) * .5;
-    boxes[i] = make_float4((float)r[0], (float)r[1], (float)r[2], (float)h);
+    return make_float4((float)r[0], (float)r[1], (float)r[2], (float)h);
 }
-// the FP32 test: false only when the boxes are apart by more than the slack on some axis
-__device__ __forceinline__ bool boxesMayOverlap(const float4& a, const float4& b, const GridArgs& g) {
-    const float lim = a.w + b.w + g.pfMargin;
-    float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
-    if (g.periodic) {
-        dx -= g.Lf[0] * rintf(dx * g.invLf[0]);
-        dy -= g.Lf[1] * rintf(dy * g.invLf[1]);
-        dz -= g.Lf[2] * rintf(dz * g.invLf[2]);
+// Bound entries by cell: key = scene's first cell + cell of the bound's minimum corner at
+// its level (or the scene's large list); the count per key gives each entry its slot.
+__global__ void __launch_bounds__(WDM_BLOCK) k_cell_count(const SceneDesc* D, const FRScene* FR, Map m, uint32_t* ekey, uint32_t* eslot,
+                                                         uint32_t* cnt) {
+    int64_t lb;
+    const FRScene& F = FR[mapBlock(m, lb)];
+    const SceneDesc& S = D[F.desc];
+    const int i = (int)(lb * WDM_BLOCK + threadIdx.x);
+    if (i >= S.nbinned) return;
+    const int id = S.binned[i];
+    const double4 a = S.p.bmin[id], b = S.p.bmax[id];
+    const int l = boundLevel(F, fmax(fmax(b.x - a.x, b.y - a.y), b.z - a.z));
+    int64_t cell = F.nCells;
+    if (l < F.nLvl) {
+        const int cx = cellCoordL(F, l, 0, a.x), cy = cellCoordL(F, l, 1, a.y), cz = cellCoordL(F, l, 2, a.z);
+        cell = F.lbase[l] + ((int64_t)cz * F.ln[l][1] + cy) * F.ln[l][0] + cx;
     }
-    return !(fabsf(dx) > lim || fabsf(dy) > lim || fabsf(dz) > lim);
+    const uint32_t key = (uint32_t)(F.cellBase + cell);
+    ekey[F.binStart + i] = key;
+    eslot[F.binStart + i] = atomicAdd(&cnt[key], 1u);
 }
-
-__global__ void k_fill_i32(int32_t* a, int64_t n, int32_t v) {
-    int64_t i = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
-    if (i < n) a[i] = v;
+// item (id << 32) of each entry at its cell's offset plus its slot
+__global__ void __launch_bounds__(WDM_BLOCK) k_cell_scatter(const SceneDesc* D, const FRScene* FR, Map m, const uint32_t* ekey,
+                                                           const uint32_t* eslot, const int32_t* cbeg, uint64_t* tmp, uint32_t* tmpB) {
+    int64_t lb;
+    const FRScene& F = FR[mapBlock(m, lb)];
+    const SceneDesc& S = D[F.desc];
+    const int i = (int)(lb * WDM_BLOCK + threadIdx.x);
+    if (i >= S.nbinned) return;
+    const uint32_t key = ekey[F.binStart + i];
+    const int64_t pos = (int64_t)cbeg[key] + eslot[F.binStart + i];
+    tmp[pos] = (uint64_t)(uint32_t)S.binned[i] << 32;
+    tmpB[pos] = key;
 }
-
-__global__ void __launch_bounds__(WDM_BLOCK) k_cell_ranges(const uint32_t* keys, int64_t nb, int32_t* cstart, int32_t* cend) {
-    int64_t i = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
-    if (i >= nb) return;
-    uint32_t k = keys[i];
-    if (i == 0 || keys[i - 1] != k) cstart[k] = (int32_t)i;
-    if (i == nb - 1 || keys[i + 1] != k) cend[k] = (int32_t)(i + 1);
-}
+// the sorted entries: particle id, cell key and FP32 box, by (scene, cell, id)
+struct CellOut {
+    const SceneDesc* D;
+    const FRScene* FR;
+    int nT;
+    int32_t* cvals;
+    uint32_t* ckeys;
+    float4* boxes;
+    __device__ void operator()(uint32_t b, int64_t pos, uint64_t x) const {
+        const FRScene& F = FR[sceneOfCell(FR, nT, b)];
+        const PartView& p = D[F.desc].p;
+        const int id = (int)(x >> 32);
+        cvals[pos] = id;
+        ckeys[pos] = b;
+        boxes[pos] = cellBox(F, p.bmin[id], p.bmax[id]);
+    }
+};
 
 // InsertionSortCollider::cellWrap / cellWrapRel (InsertionSortCollider.cpp:650-660)
 __device__ __forceinline__ double cellWrapP(double x, double x0, double x1, int& period) {
@@ -978,24 +1201,7 @@ __device__ __forceinline__ bool mayCollideD(const PartView& p, const GP int32_t*
     return true;
 }
 
-// Candidate pairs.  Thread i takes the i-th bound in (scene, cell) order.  A bound that
-// fits a grid cell visits its own cell (later entries only) and the 13 "forward"
-// neighbour cells, so every unordered pair of such bounds is visited once, and it tests
-// every unbinned particle (walls).  A large bound (wider than a cell: a facet bigger than
-// every sphere) is kept out of the cells; it is listed instead, by k_large_cells, in every
-// cell its bound covers plus margin (a small bound overlapping it starts at most one cell
-// before it), and a small bound tests the large ones listed in its own cell.  Large
-// bounds pair with each other by a direct loop when their masks allow it.  Conservative
-// tests reject most candidates cheaply (FP32 boxes, then an FP64 minimum-image test on
-// bound centres); the decision itself is the collider's own closed-interval 3-axis overlap
-// of the stored bounds plus mayCollide, so the set equals the reference's and the grid
-// only prunes.
-// Each bound's work is split into PAIR_ITEMS threads (one per stencil cell, one for walls
-// and listed large bounds).  A wavefront scans its stencil cells in lockstep and queues
-// the candidates that survive the FP32 test in LDS; the exact tests then run over the
-// queue with every lane busy (run per thread inside the scan, they would serialise the
-// wavefront on its few surviving lanes).  MODE 0 counts per item, MODE 1 writes keys and
-// packed periods at offs[item] (in queue order: the candidates are sorted afterwards).
+// FP64 prefilter of the exact test: minimum-image distance of the bound centres
 __device__ __forceinline__ bool periPrefilter(const double4& a0, const double4& a1, const double4& b0, const double4& b1, const GridArgs& g) {
     // centres and half extents; reject when the minimum-image gap exceeds the half
     // extents by more than 1e-9 L on some axis (rounding here is ~1e-16 L)
@@ -1012,37 +1218,37 @@ __device__ __forceinline__ bool periPrefilter(const double4& a0, const double4& 
     return true;
 }
 
-// Candidate keys of a batch: (t << 2 idBits) | (min id << idBits) | max id, t the scene's
-// index in the full run, so one sort orders them by scene, then like the contact keys.
-// cell index along an axis without wrapping or clamping (large bounds' cell ranges)
-__device__ __forceinline__ int cellCoordRaw(const GridArgs& g, int d, double x) {
-    if (g.periodic) return (int)floor(x / g.L[d] * g.n[d]);
-    return (int)floor((x - g.origin[d]) / g.w[d]);
+// cell index along axis d at level l without wrapping or clamping (large bounds' cell ranges)
+__device__ __forceinline__ int cellCoordRawL(const FRScene& F, int l, int d, double x) {
+    if (F.g.periodic) return (int)floor(x / F.g.L[d] * F.ln[l][d]);
+    return (int)floor((x - F.g.origin[d]) / F.lw[l][d]);
 }
+__device__ __forceinline__ int wrapCell(int x, int n) { return ((x % n) + n) % n; }
 
-// cells listing a large bound: those its bound covers, two more at the low side and one at
-// the high side (a small bound starts at most one cell before it; one cell of rounding
-// margin each way), each cell once in a periodic grid
-__device__ __forceinline__ void largeRange(const GridArgs& g, const double4& a0, const double4& a1, int c0[3], int nc[3]) {
+// cells of level l listing a large bound: those its bound covers, two more at the low side
+// and one at the high side (a bound of the level starts at most one cell before it; one
+// cell of rounding margin each way), each cell once in a periodic grid
+__device__ __forceinline__ void largeRangeL(const FRScene& F, int l, const double4& a0, const double4& a1, int c0[3], int nc[3]) {
     const double lo3[3] = {a0.x, a0.y, a0.z}, hi3[3] = {a1.x, a1.y, a1.z};
     for (int d = 0; d < 3; d++) {
-        int l = cellCoordRaw(g, d, lo3[d]) - 2, h = cellCoordRaw(g, d, hi3[d]) + 1;
-        if (!g.periodic) { l = l < 0 ? 0 : l; h = h > g.n[d] - 1 ? g.n[d] - 1 : h; }
-        c0[d] = l;
-        nc[d] = h >= l ? h - l + 1 : 0;
-        if (g.periodic && nc[d] > g.n[d]) { nc[d] = g.n[d]; c0[d] = 0; }
+        const int n = F.ln[l][d];
+        int lo = cellCoordRawL(F, l, d, lo3[d]) - 2, hi = cellCoordRawL(F, l, d, hi3[d]) + 1;
+        if (!F.g.periodic) { lo = lo < 0 ? 0 : lo; hi = hi > n - 1 ? n - 1 : hi; }
+        c0[d] = lo;
+        nc[d] = hi >= lo ? hi - lo + 1 : 0;
+        if (F.g.periodic && nc[d] > n) { nc[d] = n; c0[d] = 0; }
     }
 }
-// MODE 0: number of cells listing each large bound (0 for the others); MODE 1: the
-// entries (scene's first cell + cell, particle id) at offs[i]
+// Large bounds are listed in the cells of every level.  MODE 0: number of cells listing
+// each sorted entry (0 for the bounds in cells); MODE 1: those listings at offs[i]: cell
+// key, particle id and the slot of the count per cell.
 template <int MODE>
 __global__ void __launch_bounds__(WDM_BLOCK) k_large_cells(const SceneDesc* D, const FRScene* FR, Map m, const uint32_t* ckeys,
-                                                          const int32_t* cvals, int32_t* counts, const int64_t* offs, uint32_t* lkeys,
-                                                          int32_t* lvals) {
+                                                          const int32_t* cvals, uint32_t* counts, const int64_t* offs, uint32_t* lkeys,
+                                                          int32_t* lids, uint32_t* lslot, uint32_t* lcnt) {
     int64_t lb;
     const FRScene& F = FR[mapBlock(m, lb)];
     const SceneDesc& S = D[F.desc];
-    const GridArgs g = F.g;
     const int li = (int)(lb * WDM_BLOCK + threadIdx.x);
     if (li >= S.nbinned) return;
     const int64_t i = F.binStart + li;
@@ -1051,25 +1257,39 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_large_cells(const SceneDesc* D, c
         return;
     }
     const int id = cvals[i];
-    int c0[3], nc[3];
-    largeRange(g, S.p.bmin[id], S.p.bmax[id], c0, nc);
-    if (!MODE) {
-        counts[i] = nc[0] * nc[1] * nc[2];
-        return;
-    }
-    int64_t o = offs[i];
-    for (int zz = 0; zz < nc[2]; zz++) {
-        const int z = g.periodic ? (((c0[2] + zz) % g.n[2]) + g.n[2]) % g.n[2] : c0[2] + zz;
-        for (int yy = 0; yy < nc[1]; yy++) {
-            const int y = g.periodic ? (((c0[1] + yy) % g.n[1]) + g.n[1]) % g.n[1] : c0[1] + yy;
-            for (int xx = 0; xx < nc[0]; xx++) {
-                const int x = g.periodic ? (((c0[0] + xx) % g.n[0]) + g.n[0]) % g.n[0] : c0[0] + xx;
-                lkeys[o] = (uint32_t)(F.cellBase + (z * g.n[1] + y) * g.n[0] + x);
-                lvals[o] = id;
-                o++;
+    const double4 a0 = S.p.bmin[id], a1 = S.p.bmax[id];
+    int64_t o = MODE ? offs[i] : 0;
+    uint32_t tot = 0;
+    for (int l = 0; l < F.nLvl; l++) {
+        int c0[3], nc[3];
+        largeRangeL(F, l, a0, a1, c0, nc);
+        tot += (uint32_t)(nc[0] * nc[1] * nc[2]);
+        if (!MODE) continue;
+        const int nx = F.ln[l][0], ny = F.ln[l][1], nz = F.ln[l][2];
+        for (int zz = 0; zz < nc[2]; zz++) {
+            const int z = F.g.periodic ? wrapCell(c0[2] + zz, nz) : c0[2] + zz;
+            for (int yy = 0; yy < nc[1]; yy++) {
+                const int y = F.g.periodic ? wrapCell(c0[1] + yy, ny) : c0[1] + yy;
+                for (int xx = 0; xx < nc[0]; xx++) {
+                    const int x = F.g.periodic ? wrapCell(c0[0] + xx, nx) : c0[0] + xx;
+                    const uint32_t key = (uint32_t)(F.cellBase + F.lbase[l] + ((int64_t)z * ny + y) * nx + x);
+                    lkeys[o] = key;
+                    lids[o] = id;
+                    lslot[o] = atomicAdd(&lcnt[key], 1u);
+                    o++;
+                }
             }
         }
     }
+    if (!MODE) counts[i] = tot;
+}
+// each listing at its cell's offset plus its slot (the order within a cell does not matter:
+// the candidates are sorted afterwards)
+__global__ void __launch_bounds__(WDM_BLOCK) k_large_scatter(const uint32_t* lkeys, const int32_t* lids, const uint32_t* lslot, int64_t n,
+                                                            const int32_t* lbeg, int32_t* lvals) {
+    const int64_t o = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
+    if (o >= n) return;
+    lvals[lbeg[lkeys[o]] + lslot[o]] = lids[o];
 }
 
 // one candidate of bound `id` (corners a0, a1, shape sa) with particle j: canonical order,
@@ -1093,215 +1313,313 @@ __device__ __forceinline__ bool pairTest(const SceneDesc& S, const GridArgs& g, 
     return ov && mayCollideD(p, S.n.master, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), S.a.loneMask);
 }
 
-// Candidate pairs, one wavefront per grid cell.  The bounds of a cell (in the (scene, cell)
-// order of the sorted entries) sit one per lane, 64 at a time; the wavefront walks the
-// entries of its own cell (later ones only) and of the 13 "forward" neighbour cells, one
-// partner entry per iteration for all lanes: the partner's FP32 box is one uniform (scalar)
-// load and every lane tests its own box against it, so a partner is read once per cell
-// instead of once per bound.  Lanes whose boxes may overlap queue (own entry, partner entry)
-// in LDS; the exact tests then run over the queue with every lane busy.  Every unordered pair
-// of bounds that fit a cell is visited once.  Each own bound also tests every unbinned
-// particle (walls) and the large bounds listed in its cell (k_large_cells); the wavefront of
-// the large list's own cell (nCells) pairs large bounds with later ones when their masks
-// allow it.  Conservative tests reject most candidates cheaply (FP32 boxes, then an FP64
-// minimum-image test on bound centres); the decision itself is the collider's own
+// Candidate pairs, one thread per bound entry (in the (scene, level, cell, id) order of the
+// sorted entries).  An entry visits the entries of its own cell after it and of the 13
+// "forward" neighbour cells of its level, so every unordered pair of entries of one level
+// is visited once (two bounds of a level overlap only if their minimum corners lie in the
+// same or adjacent cells: their extents are below the cell width by 0.1 %); and, for every
+// finer level, the cells in which the minimum corner of an overlapping bound of that level
+// can lie, so every pair of entries of two levels is visited once, by the coarser one.
+// Each entry also tests every unbinned particle (walls) and the large bounds listed in its
+// cell (k_large_cells); an entry of the large list pairs with the later ones when their
+// masks allow it.  Conservative tests reject most candidates cheaply (FP32 boxes, then an
+// FP64 minimum-image test on bound centres); the decision itself is the collider's own
 // closed-interval 3-axis overlap of the stored bounds plus mayCollide, so the set equals the
 // reference's and the grid only prunes.
-// Each wavefront writes its accepted pairs, key (t << 2 idBits | min id << idBits | max id)
-// and packed periods, into its own region of `wcap` entries (no atomics: wavefront w owns
-// entries [w wcap, (w+1) wcap)) and their number to wcnt[w]; a count above wcap (the pairs
-// past it are not written) raises *wmax, and the host runs the pass again with larger
-// regions.  k_compact_pairs then packs the regions in wavefront order.  The order does not
-// matter: the keys are unique and sorted afterwards.
-#define PAIR_QUEUE 512
-#define PAIR_UNROLL 4
-__device__ __forceinline__ bool stencilCell(const GridArgs& g, uint32_t ck, int q, int64_t& cell) {
+// Two passes with the same blocks (wavefront w of both works on the same 64 entries):
+// k_pair_scan runs its lanes' partner lists flattened into one loop, one partner per lane
+// per iteration (a lane whose current cell is used up moves to its next non-empty one), and
+// writes the (entry, partner entry) pairs that pass the FP32 box test to the wavefront's
+// region; it carries no FP64 state, so many wavefronts stay resident to cover the loads.
+// k_pair_exact then runs the exact test over each region with every lane busy, and tests
+// its entries against the walls and the listed large bounds.
+// Output of k_pair_exact: each accepted pair counts in rowCnt[rowBase + min id], whose
+// returned value is the pair's slot in that row, and goes to the wavefront's own region
+// (item max id << 32 | packed periods, row, slot).  A wavefront w owns entries
+// [w cap, (w+1) cap) of a region, with their number in cnt[w]; a count above cap (the
+// pairs past it are not written) raises *wmax, and the host runs the passes again with
+// larger regions.
+__device__ __forceinline__ bool stencilCellL(const FRScene& F, int l, int cx, int cy, int cz, int q, int& cell) {
     // half-stencil offset q: (0,0,0), (1,0,0), (-1..1,1,0), (-1..1,-1..1,1)
     int dx, dy, dz;
     if (q == 0) { dx = 0; dy = 0; dz = 0; }
     else if (q == 1) { dx = 1; dy = 0; dz = 0; }
     else if (q < 5) { dx = q - 3; dy = 1; dz = 0; }
     else { dx = (q - 5) % 3 - 1; dy = (q - 5) / 3 - 1; dz = 1; }
+    const int nx = F.ln[l][0], ny = F.ln[l][1], nz = F.ln[l][2];
+    const bool per = F.g.periodic != 0;
     // periodic axes with fewer than 3 cells hold a single cell (no neighbours); aperiodic axes always scan +-1
-    if ((dx && g.periodic && g.n[0] < 3) || (dy && g.periodic && g.n[1] < 3) || (dz && g.periodic && g.n[2] < 3)) return false;
-    const int cx = ck % g.n[0], cy = (ck / g.n[0]) % g.n[1], cz = ck / (g.n[0] * g.n[1]);
+    if ((dx && per && nx < 3) || (dy && per && ny < 3) || (dz && per && nz < 3)) return false;
     int z = cz + dz, y = cy + dy, x = cx + dx;
-    if (g.periodic) {
-        z = (z + g.n[2]) % g.n[2]; y = (y + g.n[1]) % g.n[1]; x = (x + g.n[0]) % g.n[0];
-    } else if (z < 0 || z >= g.n[2] || y < 0 || y >= g.n[1] || x < 0 || x >= g.n[0]) return false;
-    cell = (int64_t)(z * g.n[1] + y) * g.n[0] + x;
+    if (per) {
+        z = (z + nz) % nz; y = (y + ny) % ny; x = (x + nx) % nx;
+    } else if (z < 0 || z >= nz || y < 0 || y >= ny || x < 0 || x >= nx) return false;
+    cell = F.lbase[l] + (z * ny + y) * nx + x;
     return true;
 }
-__global__ void __launch_bounds__(WDM_BLOCK) k_cell_pairs(const SceneDesc* D, const FRScene* FR, Map m, const int32_t* cvals,
-                                                         const int32_t* cstart, const int32_t* cend, const int32_t* lstart,
-                                                         const int32_t* lend, const int32_t* lvals, const float4* boxes,
-                                                         int32_t* wcnt, int32_t* wmax, int32_t wcap, uint64_t* okeys, int32_t* operiods,
-                                                         int idBits) {
-    __shared__ int2 queue[WDM_BLOCK / 64][PAIR_QUEUE];
-    // blocks in XCD-contiguous order (the launch is padded to a multiple of 8 blocks): the
-    // cells of one XCD are a contiguous range, whose neighbours mostly share its L2
-    const int64_t lbk = xcdLogicalBlock();
-    if (lbk >= m.bprefix[m.n]) return;  // padding
-    int64_t lb;
-    const int tt = mapBlockAt(m, lbk, lb);
+// The hardware hands the blocks of a launch to the 8 XCDs round robin (block b to XCD b % 8).
+// Logical blocks in chunks of PAIR_CHUNK consecutive ones, chunk k on XCD k % 8: the entries
+// of a chunk are a contiguous range of cells, whose neighbours mostly share that XCD's L2,
+// while a stretch of costly entries (the coarser levels come last) is spread over all XCDs.
+// The launch is padded to a multiple of 8 PAIR_CHUNK blocks.
+#define PAIR_CHUNK 16
+__device__ __forceinline__ int64_t xcdChunkBlock() {
+    const int64_t b = blockIdx.x, x = b & 7, r = b >> 3;
+    return ((r / PAIR_CHUNK) * 8 + x) * PAIR_CHUNK + (r % PAIR_CHUNK);
+}
+// this wavefront's index (regions, counts), its scene and its block within the scene; false
+// for padding
+__device__ __forceinline__ bool pairWave(const Map& m, int64_t& gw, int& t, int64_t& lb) {
+    const int64_t lbk = xcdChunkBlock();
+    if (lbk >= m.bprefix[m.n]) return false;
+    t = mapBlockAt(m, lbk, lb);
+    gw = lbk * (WDM_BLOCK / 64) + (threadIdx.x >> 6);
+    return true;
+}
+__global__ void __launch_bounds__(WDM_BLOCK, 6) k_pair_scan(const SceneDesc* D, const FRScene* FR, Map m, const uint32_t* ckeys,
+                                                        const int32_t* cbeg, const float4* boxes, int32_t* cnt, int32_t* wmax, int32_t cap,
+                                                        int2* out) {
+    int64_t gw, lb;
+    int tt;
+    if (!pairWave(m, gw, tt, lb)) return;
+    const FRScene& F = FR[tt];
+    const SceneDesc& S = D[F.desc];
+    const GridArgs& g = F.g;
+    GP int2* ro = gpc(out) + gw * (int64_t)cap;
+    int cntW = 0;  // pairs found (the same in every lane)
+    const int li = (int)(lb * WDM_BLOCK + threadIdx.x);
+    const bool valid = li < S.nbinned;
+    const int64_t e = F.binStart + (valid ? li : 0);  // this lane's entry
+    const int64_t cb = F.cellBase;
+    const float4 own = valid ? boxes[e] : make_float4(0.f, 0.f, 0.f, -1e30f);
+    const int key = valid ? (int)(ckeys[e] - cb) : 0;
+    int lvl = 0;  // level of the entry (nLvl: the large list)
+    while (lvl < F.nLvl && key >= F.lbase[lvl + 1]) lvl++;
+    const bool large = lvl >= F.nLvl;
+    int cx = 0, cy = 0, cz = 0;
+    if (!large) {
+        const int c = key - F.lbase[lvl], nx = F.ln[lvl][0], ny = F.ln[lvl][1];
+        cx = c % nx;
+        cy = (c / nx) % ny;
+        cz = c / (nx * ny);
+    }
+    // periodic wrap of the FP32 box test (zero on aperiodic grids: the wrap term vanishes)
+    const float wLx = g.periodic ? g.Lf[0] : 0.f, wLy = g.periodic ? g.Lf[1] : 0.f, wLz = g.periodic ? g.Lf[2] : 0.f;
+    const float wIx = g.periodic ? g.invLf[0] : 0.f, wIy = g.periodic ? g.invLf[1] : 0.f, wIz = g.periodic ? g.invLf[2] : 0.f;
+    const float margin = g.pfMargin;
+    // partner ranges: phases 0-13 the half stencil of the entry's level (phase 0: its own
+    // cell, entries after it), then one phase per finer level scanning a box of cells
+    const int nPh = large ? (F.largePairs ? 1 : 0) : 14 + lvl;
+    int phase = 1;                               // phase 0 (the own cell) starts below
+    int rl = 0, ri = 0, rn = 0;                  // box scan: level, next cell, cells
+    int r0x = 0, r0y = 0, r0z = 0, rnx = 1, rny = 1;  // box origin (cells) and sizes
+    // the next cell of this lane's list (batch-wide index), -1 past its end
+    auto nextCell = [&]() -> int64_t {
+        for (;;) {
+            if (ri < rn) {
+                const int ix = ri % rnx, iy = (ri / rnx) % rny, iz = ri / (rnx * rny);
+                ri++;
+                const int nx = F.ln[rl][0], ny = F.ln[rl][1], nz = F.ln[rl][2];
+                int x = r0x + ix, y = r0y + iy, z = r0z + iz;
+                if (g.periodic) { x = wrapCell(x, nx); y = wrapCell(y, ny); z = wrapCell(z, nz); }
+                return cb + F.lbase[rl] + ((int64_t)z * ny + y) * nx + x;
+            }
+            if (phase >= nPh) return -1;
+            const int ph = phase++;
+            if (ph < 14) {
+                int c;
+                if (stencilCellL(F, lvl, cx, cy, cz, ph, c)) return cb + c;
+            } else {
+                // finer level rl: a bound there overlapping this one has its minimum corner in
+                // [own min - lthr[rl], own max] on every axis (FP32 box, widened by the FP32 slack)
+                rl = ph - 14;
+                const float oc[3] = {own.x, own.y, own.z};
+                int o3[3], n3[3];
+                for (int d = 0; d < 3; d++) {
+                    const int n = F.ln[rl][d];
+                    const double lo = (double)oc[d] - (double)own.w - F.lthr[rl] - (double)g.pfMargin;
+                    const double hi = (double)oc[d] + (double)own.w + (double)g.pfMargin;
+                    int a, b;
+                    if (g.periodic) {
+                        a = (int)floor(lo * g.invL[d] * n);
+                        b = (int)floor(hi * g.invL[d] * n);
+                        if (b - a + 1 > n) { a = 0; b = n - 1; }
+                    } else {
+                        a = (int)floor(lo / F.lw[rl][d]);
+                        b = (int)floor(hi / F.lw[rl][d]);
+                        a = a < 0 ? 0 : (a > n - 1 ? n - 1 : a);
+                        b = b < 0 ? 0 : (b > n - 1 ? n - 1 : b);
+                    }
+                    o3[d] = a;
+                    n3[d] = b - a + 1;
+                }
+                r0x = o3[0]; r0y = o3[1]; r0z = o3[2];
+                rnx = n3[0]; rny = n3[1];
+                rn = n3[0] * n3[1] * n3[2];
+                ri = 0;
+            }
+        }
+    };
+    // partner entries [j, je); the next cell's range [nj, nje) is loaded one cell ahead
+    int j = 0, je = 0, nj = 0, nje = 0;
+    bool hasNext = false;
+    if (valid && nPh > 0) {
+        j = (int)e + 1;
+        je = cbeg[cb + key + 1];
+        const int64_t c = nextCell();
+        hasNext = c >= 0;
+        if (hasNext) { nj = cbeg[c]; nje = cbeg[c + 1]; }
+    }
+    // the FP32 test of partner box b, into this wavefront's region
+    auto test = [&](bool act, const float4& b, int jj) {
+        float dx = b.x - own.x, dy = b.y - own.y, dz = b.z - own.z;
+        dx -= wLx * rintf(dx * wIx);
+        dy -= wLy * rintf(dy * wIy);
+        dz -= wLz * rintf(dz * wIz);
+        const float lim = (own.w + b.w) + margin;
+        const bool pass = act & !((fabsf(dx) > lim) | (fabsf(dy) > lim) | (fabsf(dz) > lim));
+        const uint64_t bal = __ballot(pass);
+        if (pass) {
+            const int k = cntW + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            if (k < cap) ro[k] = make_int2((int)e, jj);
+        }
+        cntW += (int)__popcll(bal);
+    };
+    for (;;) {
+        // lanes whose range is used up move on to their next non-empty one
+        while (j >= je && hasNext) {
+            j = nj;
+            je = nje;
+            const int64_t c = nextCell();
+            hasNext = c >= 0;
+            if (hasNext) { nj = cbeg[c]; nje = cbeg[c + 1]; }
+        }
+        const bool act = j < je, act2 = j + 1 < je;
+        if (!__any(act)) break;
+        // two partners per iteration, their loads issued together
+        float4 pa = make_float4(0.f, 0.f, 0.f, -1e30f), pb = pa;
+        if (act) pa = boxes[j];
+        if (act2) pb = boxes[j + 1];
+        test(act, pa, j);
+        test(act2, pb, j + 1);
+        j += 2;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        cnt[gw] = cntW;
+        if (cntW > cap) atomicMax(wmax, cntW);
+    }
+}
+__global__ void __launch_bounds__(WDM_BLOCK) k_pair_exact(const SceneDesc* D, const FRScene* FR, Map m, const int32_t* cvals,
+                                                         const uint32_t* ckeys, const int32_t* lbeg, const int32_t* lvals, const int32_t* cnt1,
+                                                         int32_t cap1, const int2* in, int32_t* wcnt, int32_t* wmax, int32_t wcap,
+                                                         uint64_t* rItem, uint32_t* rRow, uint32_t* rSlot, uint32_t* rowCnt) {
+    int64_t gw, lb;
+    int tt;
+    if (!pairWave(m, gw, tt, lb)) return;
     const FRScene& F = FR[tt];
     const SceneDesc& S = D[F.desc];
     const PartView p = S.p;
     const GridArgs g = F.g;  // by value: the loops below store to global memory
     uint32_t* err = S.words + 2;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t gw = lbk * (WDM_BLOCK / 64) + wv;  // this wavefront's region and count
-    const int64_t cellL = lb * (WDM_BLOCK / 64) + wv;                 // its cell (nCells: the large list)
-    int cntW = 0;                                                     // pairs found (the same in every lane)
-    GP uint64_t* rk = gpc(okeys) + gw * (int64_t)wcap;
-    GP int32_t* rp = gpc(operiods) + gw * (int64_t)wcap;
-    const int64_t cellBase = F.cellBase;
-    const int s0 = cellL <= F.nCells ? cstart[cellBase + cellL] : -1;  // -1: empty cell or past the scene's cells
-    if (s0 < 0) {
-        if (lane == 0) wcnt[gw] = 0;
-        return;  // the whole wavefront leaves together
-    }
-    const bool largeCell = cellL == F.nCells;
-    const uint64_t tkey = (uint64_t)tt << (2 * idBits);
-    const int e0 = cend[cellBase + cellL];
-    // periodic wrap of the FP32 box test (zero on aperiodic grids: the wrap term vanishes)
-    const float wLx = g.periodic ? g.Lf[0] : 0.f, wLy = g.periodic ? g.Lf[1] : 0.f, wLz = g.periodic ? g.Lf[2] : 0.f;
-    const float wIx = g.periodic ? g.invLf[0] : 0.f, wIy = g.periodic ? g.invLf[1] : 0.f, wIz = g.periodic ? g.invLf[2] : 0.f;
+    GP uint64_t* rk = gpc(rItem) + gw * (int64_t)wcap;
+    GP uint32_t* rr = gpc(rRow) + gw * (int64_t)wcap;
+    GP uint32_t* rs = gpc(rSlot) + gw * (int64_t)wcap;
+    int cntW = 0;  // pairs found (the same in every lane)
     // the pairs of the lanes with `ok`, in lane order, into this wavefront's region
     auto emit = [&](bool ok, int a, int b, const int per[3]) {
         const uint64_t bal = __ballot(ok);
         if (!bal) return;
         if (ok) {
             const int k = cntW + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            const int lo = a < b ? a : b, hi = a < b ? b : a;
+            const uint32_t row = (uint32_t)(F.rowBase + lo);
+            const uint32_t slot = atomicAdd(&rowCnt[row], 1u);
             if (k < wcap) {
-                const int lo = a < b ? a : b, hi = a < b ? b : a;
-                rk[k] = tkey | ((uint64_t)(uint32_t)lo << idBits) | (uint32_t)hi;
-                rp[k] = packCd(per[0], per[1], per[2]);
+                rk[k] = ((uint64_t)(uint32_t)hi << 32) | (uint32_t)packCd(per[0], per[1], per[2]);
+                rr[k] = row;
+                rs[k] = slot;
             }
         }
         cntW += (int)__popcll(bal);
     };
-    for (int k0 = s0; k0 < e0; k0 += 64) {
-        const int i = k0 + lane;  // this lane's own entry
-        const bool valid = i < e0;
-        const float4 own = valid ? boxes[i] : make_float4(0.f, 0.f, 0.f, -1e30f);
-        int queued = 0;  // the same in every lane
-        auto flush = [&]() {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            for (int b = 0; b < queued; b += 64) {
-                bool ok = false;
-                int oid = 0, j = 0;
-                int per[3] = {0, 0, 0};
-                if (b + lane < queued) {
-                    const int2 e = queue[wv][b + lane];
-                    oid = cvals[e.x];
-                    j = cvals[e.y];
-                    ok = pairTest(S, g, oid, p.bmin[oid], p.bmax[oid], ptShape(p.ptype[oid]), j, true, err, per);
-                }
-                emit(ok, oid, j, per);
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            queued = 0;
-        };
-        // the own cell (entries after each lane's own) and, for cells of the grid, the forward neighbours
-        for (int q = 0; q < (largeCell ? 1 : 14); q++) {
-            int js, je;
-            if (q == 0) {
-                if (largeCell && !F.largePairs) break;
-                js = k0 + 1;
-                je = e0;
-            } else {
-                int64_t nc;
-                if (!stencilCell(g, (uint32_t)cellL, q, nc)) continue;
-                js = cstart[cellBase + nc];
-                if (js < 0) continue;
-                je = cend[cellBase + nc];
-            }
-            // the partners 64 at a time: one coalesced load (lane l holds entry jc + l), then each
-            // partner's box is broadcast to every lane from the register of the lane holding it
-            // (a scalar read) and tested without branches (boxesMayOverlap's arithmetic, with
-            // the periodic wrap multiplied away on aperiodic grids)
-            const int skipBelow = q == 0 ? i : -1;  // own cell: partners after this lane's entry
-            for (int jc = js; jc < je; jc += 64) {
-                const int jn = je - jc < 64 ? je - jc : 64;
-                const float4 pb = lane < jn ? boxes[jc + lane] : make_float4(0.f, 0.f, 0.f, -1e30f);
-                auto test = [&](int jj) {
-                    const float bxx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pb.x), jj));
-                    const float bxy = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pb.y), jj));
-                    const float bxz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pb.z), jj));
-                    const float bxw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pb.w), jj));
-                    float dx = bxx - own.x, dy = bxy - own.y, dz = bxz - own.z;
-                    dx -= wLx * rintf(dx * wIx);
-                    dy -= wLy * rintf(dy * wIy);
-                    dz -= wLz * rintf(dz * wIz);
-                    const float lim = (own.w + bxw) + g.pfMargin;
-                    const int j = jc + jj;
-                    const bool pass = (valid & (j > skipBelow)) & !((fabsf(dx) > lim) | (fabsf(dy) > lim) | (fabsf(dz) > lim));
-                    const uint64_t bal = __ballot(pass);
-                    if (pass) {
-                        const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
-                        queue[wv][queued + rank] = make_int2(i, j);
-                    }
-                    queued += (int)__popcll(bal);
-                };
-                int jj = 0;
-                for (; jj + PAIR_UNROLL <= jn; jj += PAIR_UNROLL) {
-                    if (queued > PAIR_QUEUE - 64 * PAIR_UNROLL) flush();
-#pragma unroll
-                    for (int u = 0; u < PAIR_UNROLL; u++) test(jj + u);
-                }
-                for (; jj < jn; jj++) {
-                    if (queued > PAIR_QUEUE - 64) flush();
-                    test(jj);
-                }
-            }
+    // the pairs that passed the FP32 test, 64 at a time
+    const int lane = threadIdx.x & 63;
+    const int n1 = min(cnt1[gw], cap1);  // a region that overflowed is run again by the host
+    const GP int2* ri = gpc(in) + gw * (int64_t)cap1;
+    for (int b = 0; b < n1; b += 64) {
+        bool ok = false;
+        int oid = 0, j = 0;
+        int per[3] = {0, 0, 0};
+        if (b + lane < n1) {
+            const int2 q = ri[b + lane];
+            oid = cvals[q.x];
+            j = cvals[q.y];
+            ok = pairTest(S, g, oid, p.bmin[oid], p.bmax[oid], ptShape(p.ptype[oid]), j, true, err, per);
         }
-        flush();
-        // walls, and the large bounds listed in this cell, tested directly
-        const int id = valid ? cvals[i] : 0;
-        const double4 a0 = valid ? p.bmin[id] : make_double4(0., 0., 0., 0.), a1 = valid ? p.bmax[id] : make_double4(0., 0., 0., 0.);
-        const int sa = valid ? ptShape(p.ptype[id]) : 0;
-        if (!largeCell && lstart) {
-            const int ls = lstart[cellBase + cellL];
-            if (ls >= 0) {
-                const int le = lend[cellBase + cellL];
-                for (int k = ls; k < le; k++) {
-                    int per[3] = {0, 0, 0};
-                    const int j = lvals[k];
-                    const bool ok = valid && pairTest(S, g, id, a0, a1, sa, j, true, err, per);
-                    emit(ok, id, j, per);
-                }
-            }
+        emit(ok, oid, j, per);
+    }
+    // walls, and the large bounds listed in this entry's cell, tested directly
+    const int li = (int)(lb * WDM_BLOCK + threadIdx.x);
+    const bool valid = li < S.nbinned;
+    const int64_t e = F.binStart + (valid ? li : 0);
+    const int64_t cb = F.cellBase;
+    const int key = valid ? (int)(ckeys[e] - cb) : 0;
+    const bool large = key >= F.nCells;
+    const int id = valid ? cvals[e] : 0;
+    const double4 a0 = valid ? p.bmin[id] : make_double4(0., 0., 0., 0.), a1 = valid ? p.bmax[id] : make_double4(0., 0., 0., 0.);
+    const int sa = valid ? ptShape(p.ptype[id]) : 0;
+    if (lbeg) {
+        int ls = 0, le = 0;
+        if (valid && !large) {
+            ls = lbeg[cb + key];
+            le = lbeg[cb + key + 1];
         }
-        for (int u = 0; u < S.nunb; u++) {
+        for (int k = 0; __any(ls + k < le); k++) {
             int per[3] = {0, 0, 0};
-            const int j = S.unb[u];
-            const bool ok = valid && pairTest(S, g, id, a0, a1, sa, j, false, err, per);
-            emit(ok, id, j, per);
+            const bool has = ls + k < le;
+            const int jj = has ? lvals[ls + k] : 0;
+            const bool ok = has && pairTest(S, g, id, a0, a1, sa, jj, true, err, per);
+            emit(ok, id, jj, per);
         }
+    }
+    for (int u = 0; u < S.nunb; u++) {
+        int per[3] = {0, 0, 0};
+        const int jj = S.unb[u];
+        const bool ok = valid && pairTest(S, g, id, a0, a1, sa, jj, false, err, per);
+        emit(ok, id, jj, per);
     }
     if (lane == 0) {
         wcnt[gw] = cntW;
         if (cntW > wcap) atomicMax(wmax, cntW);
     }
 }
-// the wavefront regions of k_cell_pairs packed one after the other (off: exclusive prefix of
-// their counts), one wavefront per region
-__global__ void __launch_bounds__(WDM_BLOCK) k_compact_pairs(const int32_t* wcnt, const int64_t* off, int64_t nw, int32_t wcap,
-                                                            const uint64_t* rk, const int32_t* rp, uint64_t* okeys, int32_t* operiods) {
+// each pair of the wavefront regions at its row's offset plus its slot (rows by min id)
+__global__ void __launch_bounds__(WDM_BLOCK) k_cand_scatter(const int32_t* wcnt, int64_t nw, int32_t wcap, const uint64_t* rItem,
+                                                           const uint32_t* rRow, const uint32_t* rSlot, const int64_t* rowOff, uint64_t* tmp,
+                                                           uint32_t* tmpB) {
     const int64_t w = (int64_t)blockIdx.x * (WDM_BLOCK / 64) + (threadIdx.x >> 6);
     if (w >= nw) return;
     const int n = wcnt[w];
-    const int64_t o = off[w];
     for (int k = threadIdx.x & 63; k < n; k += 64) {
-        okeys[o + k] = rk[w * (int64_t)wcap + k];
-        operiods[o + k] = rp[w * (int64_t)wcap + k];
+        const int64_t i = w * (int64_t)wcap + k;
+        const uint32_t row = rRow[i];
+        const int64_t pos = rowOff[row] + rSlot[i];
+        tmp[pos] = rItem[i];
+        tmpB[pos] = row;
     }
 }
+// the sorted candidates: contact key (min id << 32 | max id) and packed periods
+struct CandOut {
+    const FRScene* FR;
+    int nT;
+    uint64_t* candK;
+    int32_t* candP;
+    __device__ void operator()(uint32_t b, int64_t pos, uint64_t x) const {
+        const FRScene& F = FR[sceneOfRow(FR, nT, b)];
+        candK[pos] = ((uint64_t)(b - F.rowBase) << 32) | (x >> 32);
+        candP[pos] = (int32_t)(uint32_t)x;
+    }
+};
 
 __device__ __forceinline__ int64_t lowerBound64(const uint64_t* a, int64_t n, uint64_t k) {
     int64_t lo = 0, hi = n;
@@ -1312,25 +1630,32 @@ __device__ __forceinline__ int64_t lowerBound64(const uint64_t* a, int64_t n, ui
     return lo;
 }
 
-// contact key (min id << 32 | max id) of a candidate key of a batch
-__device__ __forceinline__ uint64_t candLocal(uint64_t k, int idBits) {
-    const uint64_t msk = (1ull << idBits) - 1;
-    return (((k >> idBits) & msk) << 32) | (k & msk);
+// Rows: the candidates (batch-wide, by scene then contact key) of min id p of scene t are
+// [rowOff[rowBase + p], rowOff[rowBase + p + 1]); the old contacts of min id p are
+// [oldRow[rowBase + p], oldRow[rowBase + p + 1]) of the scene's old list.
+// oldRow of a scene: first old contact of each min id (binary search in the sorted keys)
+template <typename T>
+__global__ void __launch_bounds__(WDM_BLOCK) k_old_rows(const SceneDesc* D, const ConDesc<T>* C, const FRScene* FR, Map m, int64_t* oldRow) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const FRScene& F = FR[t];
+    const int np = D[F.desc].p.np;
+    const int64_t pi = lb * WDM_BLOCK + threadIdx.x;
+    if (pi > np) return;
+    oldRow[F.rowBase + pi] = lowerBound64(C[t].cv.key, F.nOld, (uint64_t)pi << 32);
 }
-// first candidate of a scene's segment whose contact key is >= key
-__device__ __forceinline__ int64_t lowerBoundCand(const uint64_t* a, int64_t n, uint64_t key, int idBits) {
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-        int64_t mid = (lo + hi) >> 1;
-        if (candLocal(a[mid], idBits) < key) lo = mid + 1; else hi = mid;
-    }
-    return lo;
+// position of contact key k among the candidates of the scene (>= k; its row holds every
+// candidate of that min id)
+__device__ __forceinline__ int64_t candLowerBound(const FRScene& F, const int64_t* rowOff, const uint64_t* candK, uint64_t k) {
+    const int64_t row = F.rowBase + (int64_t)(k >> 32);
+    const int64_t rs = rowOff[row], re = rowOff[row + 1];
+    return rs + lowerBound64(candK + rs, re - rs, k);
 }
 
 // old contacts that stay only because they are real (their bounds no longer overlap)
 template <typename T>
-__global__ void __launch_bounds__(WDM_BLOCK) k_flag_old(const ConDesc<T>* C, const FRScene* FR, Map m, const uint64_t* ckey, int32_t* flag,
-                                                       int idBits) {
+__global__ void __launch_bounds__(WDM_BLOCK) k_flag_old(const ConDesc<T>* C, const FRScene* FR, Map m, const int64_t* rowOff,
+                                                       const uint64_t* candK, int32_t* flag) {
     int64_t lb;
     const int t = mapBlock(m, lb);
     const FRScene& F = FR[t];
@@ -1340,10 +1665,10 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_flag_old(const ConDesc<T>* C, con
     uint32_t inf = oc.info[j];
     int f = 0;
     if ((inf & CI_REAL) && !(inf & CI_DEAD)) {
-        uint64_t k = oc.key[j];
-        const uint64_t* seg = ckey + F.candStart;
-        int64_t l = lowerBoundCand(seg, F.nCand, k, idBits);
-        f = (l < F.nCand && candLocal(seg[l], idBits) == k) ? 0 : 1;
+        const uint64_t k = oc.key[j];
+        const int64_t row = F.rowBase + (int64_t)(k >> 32);
+        const int64_t l = candLowerBound(F, rowOff, candK, k);
+        f = (l < rowOff[row + 1] && candK[l] == k) ? 0 : 1;
     }
     flag[F.oldStart + j] = f;
 }
@@ -1423,8 +1748,8 @@ __device__ __forceinline__ bool nearPair(const StepArgs& a, const NodeView& n, c
 
 template <typename T>
 __global__ void __launch_bounds__(WDM_BLOCK) k_emit_cand(const SceneDesc* D, const ConDesc<T>* C, const FRScene* FR, Map m,
-                                                        const int64_t* ooRank, const uint64_t* ckey, const int32_t* cper, int32_t* oslot,
-                                                        int64_t* rflag, int idBits) {
+                                                        const int64_t* ooRank, const int64_t* oldRow, const uint64_t* ckey,
+                                                        const int32_t* cper, int32_t* oslot, int64_t* rflag) {
     int64_t lb;
     const int t = mapBlock(m, lb);
     const FRScene& F = FR[t];
@@ -1436,8 +1761,10 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_emit_cand(const SceneDesc* D, con
     const double vd = S.a.verletDist, df = S.a.distFactor;
     int64_t i = lb * WDM_BLOCK + threadIdx.x;
     if (i >= F.nCand) return;
-    uint64_t k = candLocal(ckey[F.candStart + i], idBits);
-    int64_t l = lowerBound64(oldc.key, nold, k);
+    const uint64_t k = ckey[F.candStart + i];
+    const int64_t row = F.rowBase + (int64_t)(k >> 32);
+    const int64_t os = oldRow[row];
+    const int64_t l = os + lowerBound64(oldc.key + os, oldRow[row + 1] - os, k);
     int64_t d = i + (ooRank[F.oldStart + l] - ooRank[F.oldStart]);
     int64_t gd = F.newStart + d;
     nc.key[d] = k;
@@ -1475,8 +1802,8 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_emit_cand(const SceneDesc* D, con
 }
 template <typename T>
 __global__ void __launch_bounds__(WDM_BLOCK) k_emit_oor(const ConDesc<T>* C, const FRScene* FR, Map m, const int32_t* flag,
-                                                       const int64_t* ooRank, const uint64_t* ckey, int32_t* oslot, int64_t* rflag,
-                                                       int idBits) {
+                                                       const int64_t* ooRank, const int64_t* rowOff, const uint64_t* ckey, int32_t* oslot,
+                                                       int64_t* rflag) {
     int64_t lb;
     const int t = mapBlock(m, lb);
     const FRScene& F = FR[t];
@@ -1484,7 +1811,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_emit_oor(const ConDesc<T>* C, con
     int64_t j = lb * WDM_BLOCK + threadIdx.x;
     if (j >= F.nOld || !flag[F.oldStart + j]) return;
     uint64_t k = oldc.key[j];
-    int64_t d = lowerBoundCand(ckey + F.candStart, F.nCand, k, idBits) + (ooRank[F.oldStart + j] - ooRank[F.oldStart]);
+    int64_t d = (candLowerBound(F, rowOff, ckey, k) - F.candStart) + (ooRank[F.oldStart + j] - ooRank[F.oldStart]);
     int64_t gd = F.newStart + d;
     nc.key[d] = k;
     copyContact(oldc, j, nc, d);
@@ -1564,54 +1891,66 @@ __global__ void k_set_counters(const ConDesc<T>* C, const FRScene* FR, int n, co
     *C[t].cvNew.nItems = (int32_t)(nRows + NR_ALL(nrscan[F.newStart + F.nNew] - nrscan[F.newStart]));
 }
 
-// CSR entries of the new lists: key (t, particle), value work-list item | isB << 31
-// CSR keys are (t, particle): the contacts come in (min id, max id) order, so the entries
-// of a particle arrive with the other id ascending (first as max id, then as min id) and
-// a stable radix sort on (t, particle) alone leaves them in that order.
-// Dormant contacts cannot become real before the next full run and dead ones never again:
-// their entries get the key (t, np), behind every particle of the scene, so the CSR
-// offsets leave them out of every particle's range.
+// CSR of the new lists (contact -> particle): per particle its contacts in list order (the
+// contacts come in (min id, max id) order, so the other id ascends: first those where the
+// particle is the max id, then those where it is the min id), entry value work-list item |
+// isB << 31.  Dormant contacts cannot become real before the next full run and dead ones
+// never again: they get no entries.  Each entry counts in cnt[rowBase + particle] (the slot
+// returned) and is sorted within its particle by the contact index (bucket item c << 32 |
+// value); bucket b == UINT32_MAX marks an entry that is left out.
 template <typename T>
-__global__ void __launch_bounds__(WDM_BLOCK) k_csr_entries(const SceneDesc* D, const ConDesc<T>* C, const FRScene* FR, Map m, uint64_t* ek,
-                                                          uint32_t* ev, int idBits) {
+__global__ void __launch_bounds__(WDM_BLOCK) k_csr_count(const ConDesc<T>* C, const FRScene* FR, Map m, uint32_t* eB, uint32_t* eS,
+                                                        uint64_t* eI, uint32_t* cnt) {
     int64_t lb;
     const int t = mapBlock(m, lb);
     const FRScene& F = FR[t];
     const ConView<T> nc = C[t].cvNew;
     int64_t c = lb * WDM_BLOCK + threadIdx.x;
     if (c >= F.nNew) return;
-    const uint64_t tkey = (uint64_t)t << idBits;
-    const int2 id = nc.ids[c];
     const int64_t g = F.newStart + c;
     if (nc.info[c] & (CI_DORMANT | CI_DEAD)) {
-        const uint64_t off = tkey | (uint32_t)D[F.desc].p.np;
-        ek[2 * g] = off;
-        ek[2 * g + 1] = off;
-    } else {
-        ek[2 * g] = tkey | (uint32_t)id.x;
-        ek[2 * g + 1] = tkey | (uint32_t)id.y;
+        eB[2 * g] = UINT32_MAX;
+        eB[2 * g + 1] = UINT32_MAX;
+        return;
     }
-    const uint32_t it = (uint32_t)nc.itm[c];  // dormant (-1) entries sit past every range
-    ev[2 * g] = it;
-    ev[2 * g + 1] = it | 0x80000000u;
+    const int2 id = nc.ids[c];
+    const uint32_t it = (uint32_t)nc.itm[c];
+    const uint32_t b0 = (uint32_t)(F.rowBase + id.x), b1 = (uint32_t)(F.rowBase + id.y);
+    eB[2 * g] = b0;
+    eS[2 * g] = atomicAdd(&cnt[b0], 1u);
+    eI[2 * g] = ((uint64_t)c << 32) | it;
+    eB[2 * g + 1] = b1;
+    eS[2 * g + 1] = atomicAdd(&cnt[b1], 1u);
+    eI[2 * g + 1] = ((uint64_t)c << 32) | (it | 0x80000000u);
 }
-// copy each scene's sorted entry values into its own CSR array
-__global__ void __launch_bounds__(WDM_BLOCK) k_csr_scatter(const FRScene* FR, Map m, const uint32_t* ev) {
-    int64_t lb;
-    const FRScene& F = FR[mapBlock(m, lb)];
-    int64_t e = lb * WDM_BLOCK + threadIdx.x;
-    if (e >= 2 * F.nNew) return;
-    F.entV[e] = ev[2 * F.newStart + e];
+__global__ void __launch_bounds__(WDM_BLOCK) k_csr_scatter(const uint32_t* eB, const uint32_t* eS, const uint64_t* eI, int64_t n,
+                                                          const int64_t* off, uint64_t* tmp, uint32_t* tmpB) {
+    const int64_t k = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t b = eB[k];
+    if (b == UINT32_MAX) return;
+    const int64_t pos = off[b] + eS[k];
+    tmp[pos] = eI[k];
+    tmpB[pos] = b;
 }
-__global__ void __launch_bounds__(WDM_BLOCK) k_csr_offsets(const SceneDesc* D, const FRScene* FR, Map m, const uint64_t* ek, int idBits) {
+// the sorted entries into each scene's own CSR array
+struct CsrOut {
+    const FRScene* FR;
+    int nT;
+    const int64_t* off;
+    __device__ void operator()(uint32_t b, int64_t pos, uint64_t x) const {
+        const FRScene& F = FR[sceneOfRow(FR, nT, b)];
+        F.entV[pos - off[F.rowBase]] = (uint32_t)x;
+    }
+};
+__global__ void __launch_bounds__(WDM_BLOCK) k_csr_offsets(const SceneDesc* D, const FRScene* FR, Map m, const int64_t* off) {
     int64_t lb;
     const int t = mapBlock(m, lb);
     const FRScene& F = FR[t];
     const int np = D[F.desc].p.np;
     int i = (int)(lb * WDM_BLOCK + threadIdx.x);
     if (i > np) return;
-    const uint64_t key = ((uint64_t)t << idBits) | (uint32_t)i;
-    F.csrOff[i] = lowerBound64(ek + 2 * F.newStart, 2 * F.nNew, key);
+    F.csrOff[i] = off[F.rowBase + i] - off[F.rowBase];
 }
 
 // ---------------- ContactLoop::run, pkg/dem/ContactLoop.cpp:93-259 ----------------
@@ -2761,8 +3100,9 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_lattice_write(LatticeArgs L, cons
 // Each sphere moves by half of every overlap with its neighbours (found through the
 // same cell grid); all displacements are computed from the old positions, so the
 // result does not depend on thread order.  Walls then push spheres back to their side.
+// The grid has one level here (its cells given by g.n), entries sorted by (cell, id).
 __global__ void __launch_bounds__(WDM_BLOCK) k_relax(const SceneDesc* D, const FRScene* FR, Map m, const uint32_t* ckeys, const int32_t* cvals,
-                                                    const int32_t* cstart, const int32_t* cend, double4* disp) {
+                                                    const int32_t* cbeg, double4* disp) {
     int64_t lb;
     const FRScene& F = FR[mapBlock(m, lb)];
     const SceneDesc& S = D[F.desc];
@@ -2792,9 +3132,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_relax(const SceneDesc* D, const F
                 int x = cx + dx;
                 if (g.periodic) x = (x + g.n[0]) % g.n[0]; else if (x < 0 || x >= g.n[0]) continue;
                 int64_t cell = F.cellBase + (z * g.n[1] + y) * g.n[0] + x;
-                int s = cstart[cell];
-                if (s < 0) continue;
-                int e = cend[cell];
+                const int s = cbeg[cell], e = cbeg[cell + 1];
                 for (int k = s; k < e; k++) {
                     int j = cvals[k];
                     if (j == id) continue;

@@ -113,11 +113,15 @@ struct Ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Shared* sh = nullptr;  // batch scratch, descriptor tables, per-scene words
     // launch-shape knobs (results do not depend on them): steps per window, windows replayed
-    // as captured graphs, collider cell width over the largest sphere bound
+    // as captured graphs, collider cell width over the largest sphere bound, collider grid
+    // levels (0: chosen per scene from its bound sizes; WDM_GRID_LEVELS in the environment)
     int window = 16;
     int graphs = 1;
     double cellFactor = 1.;
-    int32_t waveCap = 256;  // pairs a cell's wavefront may find in a collider pass (grows when a pass needs more)
+    int gridLevels = 0;
+    // pairs a wavefront of the pair search may find: passing the FP32 box test, passing the
+    // exact test (each grows when a pass needs more)
+    int32_t scanCap = 1280, waveCap = 640;
     hipError_t scratchErr = hipSuccess;  // looking up the device address of `scratch` at wdm_create
 };
 
@@ -253,27 +257,40 @@ struct Shared {
     std::vector<int32_t> freeSlots;
     int32_t nslots = 0;
     Staged stDesc, stMapNode, stMapCon, stCon, stFR, stConFR, stIdx, stDyn;
-    Staged stMapP, stMapB, stMapC, stMapO, stMapN, stMapE, stMapP1, stMapQ;
+    Staged stMapP, stMapB, stMapC, stMapO, stMapN, stMapP1;
     hipEvent_t evWords = nullptr;  // recorded after the copy of the words to the host
     // energy partials per block
     DBuf<double> epartL, epartC;
     // collider full-run scratch, batch-wide
     DBuf<double> extPart, extOut;
-    DBuf<uint32_t> ckeys, ckeys2;
-    DBuf<int32_t> cvals, cvals2, cstart, cend, ooFlag, oslot;
-    // candidate pairs: per wavefront of k_cell_pairs its count, prefix and region of found pairs
-    DBuf<int32_t> wcnt, wmax;
-    DBuf<int64_t> wcnt64, woff;
-    DBuf<uint64_t> regK;
-    DBuf<int32_t> regP;
-    DBuf<int64_t> ooFlag64, ooRank, rflag, rscan, starts, nrflag, nrscan;
-    DBuf<uint64_t> candK, candK2, entK, entK2;
-    DBuf<int32_t> candP, candP2;
-    DBuf<uint32_t> entV, entV2;
-    DBuf<uint32_t> lkeys, lkeys2;  // large bounds listed per cell
-    DBuf<int32_t> lvals, lvals2, lstart, lend, lcnt32;
-    DBuf<int64_t> lcnt, loffs;
-    DBuf<float4> cbox;  // FP32 boxes of the bound entries in cell order
+    DBuf<uint32_t> hist;                 // bound extents per scene, quarter octaves
+    DBuf<uint32_t> ekey, eslot, ccnt;    // bound entries: cell key and slot; count per cell
+    DBuf<int32_t> cbeg;                  // first entry of each cell (exclusive prefix of ccnt)
+    DBuf<uint32_t> ckeys2;               // sorted entries: cell key, particle id, FP32 box
+    DBuf<int32_t> cvals2, ooFlag, oslot;
+    DBuf<float4> cbox;
+    // bucket sort scratch: items, their buckets, merge buffer, long buckets
+    DBuf<uint64_t> btmp, baux;
+    DBuf<uint32_t> btmpB, longList, longCnt;
+    DBuf<int64_t> tsum;                  // prefix sums: per-tile sums
+    // candidate pairs: per wavefront of k_pairs its count and region; per row (scene, min id)
+    // the count and offsets
+    DBuf<int32_t> wcnt, wmax, scnt;
+    DBuf<int2> sPair;  // pairs passing the FP32 test: (entry, partner entry)
+    DBuf<uint64_t> rItem;
+    DBuf<uint32_t> rRow, rSlot, rowCnt;
+    DBuf<int64_t> rowOff, oldRow;
+    DBuf<int64_t> ooRank, rflag, rscan, starts, nrflag, nrscan;
+    DBuf<uint64_t> candK;
+    DBuf<int32_t> candP;
+    // CSR entries: bucket (row of the particle), slot, item; count per row and offsets
+    DBuf<uint32_t> eB, eS, csrCnt;
+    DBuf<uint64_t> eI;
+    DBuf<int64_t> csrScan;
+    // large bounds listed per cell: per entry its listing count and offset; the listings
+    DBuf<uint32_t> lcntE, lkeys, lslot, lcnt;
+    DBuf<int64_t> loffE;
+    DBuf<int32_t> lids, lbeg, lvals;
     DBuf<uint8_t> temp;
     DBuf<double4> disp;
     DBuf<unsigned long long> ddOut;  // DynDt: per scene, bits of the minimum squared critical step
@@ -308,16 +325,18 @@ struct Shared {
         hostWords = nullptr;
         hostWordsCap = 0;
         for (Staged* s : {&stDesc, &stMapNode, &stMapCon, &stCon, &stFR, &stConFR, &stIdx, &stDyn, &stMapP, &stMapB, &stMapC, &stMapO, &stMapN,
-                          &stMapE, &stMapP1, &stMapQ})
+                          &stMapP1})
             s->free(c);
-        release(c, epartL); release(c, epartC); release(c, extPart); release(c, extOut);
-        release(c, ckeys); release(c, ckeys2); release(c, cvals); release(c, cvals2); release(c, cstart); release(c, cend);
-        release(c, wcnt); release(c, wmax); release(c, wcnt64); release(c, woff); release(c, regK); release(c, regP); release(c, ooFlag); release(c, oslot); release(c, ooFlag64);
-        release(c, ooRank); release(c, rflag); release(c, rscan); release(c, starts); release(c, nrflag); release(c, nrscan); release(c, candK); release(c, candK2);
-        release(c, entK); release(c, entK2); release(c, candP); release(c, candP2); release(c, entV); release(c, entV2);
+        release(c, epartL); release(c, epartC); release(c, extPart); release(c, extOut); release(c, hist);
+        release(c, ekey); release(c, eslot); release(c, ccnt); release(c, cbeg); release(c, ckeys2); release(c, cvals2); release(c, cbox);
+        release(c, btmp); release(c, baux); release(c, btmpB); release(c, longList); release(c, longCnt); release(c, tsum);
+        release(c, wcnt); release(c, wmax); release(c, scnt); release(c, sPair); release(c, rItem); release(c, rRow); release(c, rSlot); release(c, rowCnt); release(c, rowOff);
+        release(c, oldRow); release(c, ooFlag); release(c, oslot);
+        release(c, ooRank); release(c, rflag); release(c, rscan); release(c, starts); release(c, nrflag); release(c, nrscan); release(c, candK);
+        release(c, candP); release(c, eB); release(c, eS); release(c, csrCnt); release(c, eI); release(c, csrScan);
         release(c, temp); release(c, disp); release(c, ddOut);
-        release(c, lkeys); release(c, lkeys2); release(c, lvals); release(c, lvals2); release(c, lstart); release(c, lend);
-        release(c, lcnt32); release(c, lcnt); release(c, loffs); release(c, cbox);
+        release(c, lcntE); release(c, lkeys); release(c, lslot); release(c, lcnt); release(c, loffE); release(c, lids); release(c, lbeg);
+        release(c, lvals);
         if (evWords) (void)hipEventDestroy(evWords);
         evWords = nullptr;
     }
@@ -480,28 +499,36 @@ void checkLaunch() {
 // temp storage for hipcub calls
 void* tempFor(Ctx* c, size_t bytes) { ensure(c, c->sh->temp, bytes); return c->sh->temp.p; }
 
-template <typename K, typename V>
-void sortPairs(Ctx* c, K* kin, K* kout, V* vin, V* vout, int64_t n, int endBit) {
-    if (n <= 0) return;
-    if (n > INT32_MAX) throw WdmError(WDM_E_ARG, "sort of more than 2^31 entries");
-    size_t bytes = 0;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, (int)n, 0, endBit, c->stream));
-    void* t = tempFor(c, bytes);
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(t, bytes, kin, kout, vin, vout, (int)n, 0, endBit, c->stream));
+// out[i] = sum(in[0..i)) for i in [0, n], on the context stream
+template <typename TIn, typename TOut>
+void scanExclusive(Ctx* c, const TIn* in, TOut* out, int64_t n) {
+    Shared& X = *c->sh;
+    hipStream_t st = c->stream;
+    const int64_t nt = std::max<int64_t>((n + SCAN_TILE - 1) / SCAN_TILE, 1);
+    ensure(c, X.tsum, (size_t)nt);
+    hipLaunchKernelGGL(k_scan_part<TIn>, dim3((unsigned)nt), dim3(WDM_BLOCK), 0, st, in, n, X.tsum.p);
+    hipLaunchKernelGGL(k_scan_top<TOut>, dim3(1), dim3(SCAN_TOP), 0, st, X.tsum.p, nt, out + n);
+    if (n > 0) hipLaunchKernelGGL((k_scan_fin<TIn, TOut>), dim3((unsigned)nt), dim3(WDM_BLOCK), 0, st, in, n, (const int64_t*)X.tsum.p, out);
+    checkLaunch();
 }
-// out[0] = 0, out[i+1] = sum(in[0..i])
-void scanInclusiveShift(Ctx* c, const int64_t* in, int64_t* out, int64_t n) {
-    HIPCHK(hipMemsetAsync(out, 0, sizeof(int64_t), c->stream));
-    if (n <= 0) return;
-    size_t bytes = 0;
-    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, bytes, in, out + 1, (int)n, c->stream));
-    void* t = tempFor(c, bytes);
-    HIPCHK(hipcub::DeviceScan::InclusiveSum(t, bytes, in, out + 1, (int)n, c->stream));
+void fillU32(Ctx* c, uint32_t* a, int64_t n) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n)), dim3(WDM_BLOCK), 0, c->stream, (int32_t*)a, n, 0);
 }
-
-__global__ void k_i32_to_i64(const int32_t* a, int64_t* b, int64_t n) {
-    int64_t i = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
-    if (i < n) b[i] = a[i];
+// the second half of a bucket sort: items scattered into btmp / btmpB (n of them, or at
+// most n with *total of them on the device; off the exclusive prefix of the bucket counts)
+// placed in order through the writer w
+#define BKT_GRID 128
+template <typename OffT, class W>
+void bucketFinish(Ctx* c, int64_t n, const OffT* off, const W& w, const OffT* total = nullptr) {
+    Shared& X = *c->sh;
+    if (n <= 0) return;
+    ensure(c, X.longList, (size_t)(n / (BKT_SMALL + 1) + 1)); ensure(c, X.longCnt, 1); ensure(c, X.baux, (size_t)n);
+    fillU32(c, X.longCnt.p, 1);
+    hipLaunchKernelGGL((k_bucket_rank<OffT, W>), dim3(nblk(n)), dim3(WDM_BLOCK), 0, c->stream, (const uint64_t*)X.btmp.p,
+                       (const uint32_t*)X.btmpB.p, off, n, total, X.longList.p, X.longCnt.p, w);
+    hipLaunchKernelGGL((k_bucket_long<OffT, W>), dim3(BKT_GRID), dim3(BKT_THREADS), 0, c->stream, X.btmp.p, X.baux.p, off,
+                       (const uint32_t*)X.longList.p, (const uint32_t*)X.longCnt.p, w);
+    checkLaunch();
 }
 __global__ void k_gather_i64(const int64_t* src, const int64_t* idx, int n, int64_t* out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -835,10 +862,95 @@ struct Leapfrog {
 
 // ---------------- InsertionSortCollider::run (pkg/dem/InsertionSortCollider.cpp:445-646) ----------------
 struct InsertionSortCollider {
+    // Grid levels of one scene (e: its extent reduction, hs: its histogram of bound extents):
+    // level l of nominal cell width W / 2^(nLvl-1-l), W the largest sphere bound times 1.001
+    // and the cell factor; the number of levels is the one with the smallest estimate of
+    // the pair search's work (box tests and cell visits, from the bound counts per level)
+    // whose cells stay within max(4096, 2 nb) (levels > 0 forces that many).  A single level
+    // that does not fit widens its cells.
+    static void gridLevels(const Scene& S, const double* e, const uint32_t* hs, int nb, double cellFactor, int levels, FRScene& F) {
+        double ext = e[7] > 0 ? e[7] : e[6];
+        if (!(ext > 0) || !std::isfinite(ext)) ext = 1e-12;
+        const double W = ext * 1.001 * cellFactor;
+        const double maxCells = (double)std::max<int64_t>(4096, 2 * (int64_t)nb);
+        // cells per axis and per-axis widths of a level of nominal width w; the total cells
+        auto dims = [&](double w, int n[3], double wd[3]) -> double {
+            double tot = 1;
+            for (int d = 0; d < 3; d++) {
+                double k;
+                if (S.periodic) {
+                    k = std::floor(S.size[d] / w);
+                    if (!(k >= 3)) k = 1;
+                } else {
+                    k = std::floor((e[3 + d] - e[d]) / w) + 1;
+                    if (!(k >= 1)) k = 1;
+                }
+                k = std::min(k, 1e9);
+                n[d] = (int)k;
+                wd[d] = S.periodic ? S.size[d] / k : w;
+                tot *= k;
+            }
+            return tot;
+        };
+        double V = 1;
+        for (int d = 0; d < 3; d++) V *= S.periodic ? S.size[d] : (e[3 + d] - e[d]) + W;
+        V = std::max(V, 1e-300);
+        const double VISIT = 1.5;  // a visited cell, in box tests
+        int best = 0;
+        double bestCost = 0;
+        const int lo = levels > 0 ? std::min(levels, MAXLVL) : 1, hi = levels > 0 ? std::min(levels, MAXLVL) : MAXLVL;
+        for (int nL = lo; nL <= hi; nL++) {
+            double w[MAXLVL], cnt[MAXLVL] = {0, 0, 0, 0}, cells = 0, cvol[MAXLVL];
+            for (int l = 0; l < nL; l++) {
+                w[l] = W / std::ldexp(1., nL - 1 - l);
+                int n[3];
+                double wd[3];
+                cells += dims(w[l], n, wd);
+                cvol[l] = wd[0] * wd[1] * wd[2];
+            }
+            if (cells > maxCells) continue;
+            for (int k = 0; k < EXT_BINS; k++) {
+                if (!hs[k]) continue;
+                const double bl = std::pow(2., (k - 128) / 4.), bh = std::pow(2., (k - 127) / 4.);  // the bin's extents
+                if (bl > W / 1.001) continue;  // large: listed apart
+                int l = 0;
+                while (l < nL - 1 && bh > w[l] / 1.001) l++;
+                cnt[l] += hs[k];
+            }
+            double cost = 0;
+            for (int l = 0; l < nL; l++) {
+                cost += cnt[l] * (13.5 * cnt[l] / V * cvol[l] + 14 * VISIT);
+                for (int l2 = 0; l2 < l; l2++) {
+                    const double s = w[l] + 2 * w[l2], r = w[l] / w[l2] + 2;
+                    cost += cnt[l] * (cnt[l2] / V * s * s * s + VISIT * r * r * r);
+                }
+            }
+            if (!best || cost < bestCost) { best = nL; bestCost = cost; }
+        }
+        const int nL = best ? best : 1;
+        for (int l = 0; l < nL; l++) {
+            double wl = W / std::ldexp(1., nL - 1 - l);
+            if (!best) {  // one level, widened until it fits
+                for (int it = 0; it < 200; it++) {
+                    int n[3];
+                    double wd[3];
+                    if (dims(wl, n, wd) <= maxCells) break;
+                    wl *= 1.26;
+                }
+            }
+            double wd[3];
+            dims(wl, F.ln[l], wd);
+            for (int d = 0; d < 3; d++) F.lw[l][d] = wd[d];
+            F.lthr[l] = wl / 1.001;
+        }
+        F.nLvl = nL;
+    }
+
     // Grids of the scenes L[t] from their bound extents (one synchronisation), then the
-    // bound entries of all of them sorted by (scene, cell) with each cell's entry range.
+    // bound entries of all of them sorted by (scene, cell key, id) with each cell's entry
+    // range [cbeg[k], cbeg[k + 1]) (levels: 0 lets each scene choose).
     static void buildGrids(Batch& B, const std::vector<int>& L, std::vector<FRScene>& F, const std::vector<int64_t>& bpB, const Map& mB,
-                           const FRScene*& FRd, int64_t& nbTot, double cellFactor) {
+                           const FRScene*& FRd, int64_t& nbTot, double cellFactor, int levels) {
         Ctx* c = B.c;
         Shared& X = *c->sh;
         hipStream_t st = c->stream;
@@ -846,116 +958,108 @@ struct InsertionSortCollider {
         const int64_t nblkB = bpB[nT];
         ensure(c, X.extPart, (size_t)nblkB * 8 + 8);
         ensure(c, X.extOut, (size_t)nT * 8);
+        ensure(c, X.hist, (size_t)nT * EXT_BINS);
+        fillU32(c, X.hist.p, (int64_t)nT * EXT_BINS);
         if (nblkB > 0) {
-            hipLaunchKernelGGL(k_extent, dim3((unsigned)nblkB), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, X.extPart.p);
+            hipLaunchKernelGGL(k_extent, dim3((unsigned)nblkB), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, X.extPart.p, X.hist.p);
             checkLaunch();
         }
         hipLaunchKernelGGL(k_extent_finalize, dim3(nT), dim3(WDM_BLOCK), 0, st, mB, (const double*)X.extPart.p, X.extOut.p);
         checkLaunch();
         std::vector<double> h((size_t)nT * 8);
+        std::vector<uint32_t> hh((size_t)nT * EXT_BINS);
         HIPCHK(hipMemcpyAsync(h.data(), X.extOut.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(hh.data(), X.hist.p, hh.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         int64_t cellBase = 0;
         nbTot = 0;
         for (int t = 0; t < nT; t++) {
             Scene& S = *B.ss[L[t]];
-            GridArgs& g = F[t].g;
+            FRScene& Ft = F[t];
+            GridArgs& g = Ft.g;
             std::memset(&g, 0, sizeof(g));
             g.periodic = S.periodic;
-            for (int d = 0; d < 3; d++) { g.n[d] = 1; g.w[d] = 1; g.L[d] = S.size[d]; g.invL[d] = 1. / S.size[d]; }
-            int nb = (int)S.binnedH.size();
-            F[t].wLarge = INFINITY;
-            if (nb > 0) {
-                const double* e = &h[(size_t)t * 8];
-                // cells fit the spheres; larger bounds (facets) are listed apart
-                double ext = e[7] > 0 ? e[7] : e[6];
-                if (!(ext > 0) || !std::isfinite(ext)) ext = 1e-12;
-                double w = ext * 1.001 * cellFactor;
-                const int64_t maxCells = std::max<int64_t>(4096, 2 * (int64_t)nb);
-                for (int it = 0; it < 200; it++) {
-                    int64_t tot = 1;
-                    for (int d = 0; d < 3; d++) {
-                        if (S.periodic) {
-                            int n = (int)std::floor(S.size[d] / w);
-                            if (n < 3) n = 1;
-                            g.n[d] = n;
-                            g.w[d] = S.size[d] / n;
-                            g.origin[d] = 0;
-                        } else {
-                            double ee = e[3 + d] - e[d];
-                            int n = (int)std::floor(ee / w) + 1;
-                            if (n < 1) n = 1;
-                            g.n[d] = n;
-                            g.w[d] = w;
-                            g.origin[d] = e[d];
-                        }
-                        tot *= g.n[d];
-                    }
-                    if (tot <= maxCells) break;
-                    w *= 1.26;
-                }
-                F[t].wLarge = w;
-            }
+            for (int d = 0; d < 3; d++) { g.L[d] = S.size[d]; g.invL[d] = 1. / S.size[d]; }
+            Ft.nLvl = 1;
+            for (int d = 0; d < 3; d++) { Ft.ln[0][d] = 1; Ft.lw[0][d] = 1; }
+            Ft.lthr[0] = INFINITY;
+            const int nb = (int)S.binnedH.size();
+            const double* e = &h[(size_t)t * 8];
+            if (nb > 0) gridLevels(S, e, &hh[(size_t)t * EXT_BINS], nb, cellFactor, levels, Ft);
             double span = 0;
             for (int d = 0; d < 3; d++) {
+                g.n[d] = Ft.ln[0][d];
+                g.w[d] = Ft.lw[0][d];
+                g.origin[d] = (S.periodic || nb == 0) ? 0. : e[d];
                 g.Lf[d] = (float)g.L[d];
                 g.invLf[d] = (float)g.invL[d];
-                span = std::max(span, S.periodic ? g.L[d] : g.n[d] * g.w[d]);
+                span = std::max(span, S.periodic ? g.L[d] : Ft.ln[Ft.nLvl - 1][d] * Ft.lw[Ft.nLvl - 1][d]);
             }
             g.pfMargin = (float)(4e-6 * span);
-            F[t].cellBase = cellBase;
-            F[t].binStart = nbTot;
-            F[t].nCells = g.n[0] * g.n[1] * g.n[2];
-            F[t].largePairs = S.facetMasksPair(S.P.loneMask) ? 1 : 0;
-            cellBase += (int64_t)F[t].nCells + 1;  // + the list of large bounds
+            Ft.lbase[0] = 0;
+            int64_t nc = 0;
+            for (int l = 0; l < Ft.nLvl; l++) {
+                nc += (int64_t)Ft.ln[l][0] * Ft.ln[l][1] * Ft.ln[l][2];
+                if (nc >= INT32_MAX) throw WdmError(WDM_E_ARG, "collider grid too large (2^31 cells)");
+                Ft.lbase[l + 1] = (int32_t)nc;
+            }
+            Ft.cellBase = cellBase;
+            Ft.binStart = nbTot;
+            Ft.nCells = (int32_t)nc;
+            Ft.largePairs = S.facetMasksPair(S.P.loneMask) ? 1 : 0;
+            cellBase += nc + 1;  // + the list of large bounds
             nbTot += nb;
         }
         if (cellBase >= INT32_MAX || nbTot >= INT32_MAX) throw WdmError(WDM_E_ARG, "collider batch too large (2^31 cells or bounds)");
         FRd = (const FRScene*)X.stFR.upload(c, F.data(), F.size() * sizeof(FRScene));
         const int64_t G = cellBase;
-        ensure(c, X.ckeys, nbTot); ensure(c, X.ckeys2, nbTot); ensure(c, X.cvals, nbTot); ensure(c, X.cvals2, nbTot);
-        ensure(c, X.cstart, G); ensure(c, X.cend, G);
+        const size_t nb1 = (size_t)std::max<int64_t>(nbTot, 1);
+        ensure(c, X.ekey, nb1); ensure(c, X.eslot, nb1); ensure(c, X.ckeys2, nb1); ensure(c, X.cvals2, nb1); ensure(c, X.cbox, nb1);
+        ensure(c, X.btmp, nb1); ensure(c, X.btmpB, nb1);
+        ensure(c, X.ccnt, (size_t)G); ensure(c, X.cbeg, (size_t)G + 1);
+        fillU32(c, X.ccnt.p, G);
         if (nblkB > 0) {
-            hipLaunchKernelGGL(k_cell_keys, dim3((unsigned)nblkB), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, X.ckeys.p, X.cvals.p);
+            hipLaunchKernelGGL(k_cell_count, dim3((unsigned)nblkB), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, X.ekey.p, X.eslot.p, X.ccnt.p);
             checkLaunch();
         }
-        sortPairs(c, X.ckeys.p, X.ckeys2.p, X.cvals.p, X.cvals2.p, nbTot, bitsFor((uint64_t)G));
-        hipLaunchKernelGGL(k_fill_i32, dim3(nblk(G)), dim3(WDM_BLOCK), 0, st, X.cstart.p, G, -1);
-        if (nbTot > 0)
-            hipLaunchKernelGGL(k_cell_ranges, dim3(nblk(nbTot)), dim3(WDM_BLOCK), 0, st, (const uint32_t*)X.ckeys2.p, nbTot, X.cstart.p, X.cend.p);
-        ensure(c, X.cbox, std::max<int64_t>(nbTot, 1));
-        if (nblkB > 0)
-            hipLaunchKernelGGL(k_cell_boxes, dim3((unsigned)nblkB), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const int32_t*)X.cvals2.p, X.cbox.p);
-        checkLaunch();
+        scanExclusive(c, (const uint32_t*)X.ccnt.p, X.cbeg.p, G);
+        if (nblkB > 0) {
+            hipLaunchKernelGGL(k_cell_scatter, dim3((unsigned)nblkB), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ekey.p,
+                               (const uint32_t*)X.eslot.p, (const int32_t*)X.cbeg.p, X.btmp.p, X.btmpB.p);
+            checkLaunch();
+        }
+        bucketFinish(c, nbTot, (const int32_t*)X.cbeg.p, CellOut{B.D, FRd, nT, X.cvals2.p, X.ckeys2.p, X.cbox.p});
     }
 
-    // list every large bound (facets wider than a cell) in the cells it covers; false when
-    // the batch has none (one synchronisation for the entry count)
+    // list every large bound (facets wider than the top level's cells) in the cells it
+    // covers at every level; false when the batch has none (one synchronisation for the
+    // listing count)
     static bool listLarge(Batch& B, const std::vector<int64_t>& bpB, const Map& mB, const FRScene* FRd, int64_t nbTot, int64_t G) {
         Ctx* c = B.c;
         Shared& X = *c->sh;
         hipStream_t st = c->stream;
         const int nT = (int)bpB.size() - 1;
         if (bpB[nT] == 0) return false;
-        ensure(c, X.lcnt32, nbTot); ensure(c, X.lcnt, nbTot); ensure(c, X.loffs, (size_t)nbTot + 1);
+        ensure(c, X.lcntE, (size_t)nbTot); ensure(c, X.loffE, (size_t)nbTot + 1);
         hipLaunchKernelGGL(k_large_cells<0>, dim3((unsigned)bpB[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
-                           (const int32_t*)X.cvals2.p, X.lcnt32.p, (const int64_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr);
-        hipLaunchKernelGGL(k_i32_to_i64, dim3(nblk(nbTot)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.lcnt32.p, X.lcnt.p, nbTot);
+                           (const int32_t*)X.cvals2.p, X.lcntE.p, (const int64_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr,
+                           (uint32_t*)nullptr, (uint32_t*)nullptr);
         checkLaunch();
-        scanInclusiveShift(c, X.lcnt.p, X.loffs.p, nbTot);
+        scanExclusive(c, (const uint32_t*)X.lcntE.p, X.loffE.p, nbTot);
         int64_t nl = 0;
-        HIPCHK(hipMemcpyAsync(&nl, X.loffs.p + nbTot, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(&nl, X.loffE.p + nbTot, sizeof(int64_t), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (nl == 0) return false;
         if (nl >= INT32_MAX) throw WdmError(WDM_E_ARG, "too many large-bound cell entries");
-        ensure(c, X.lkeys, nl); ensure(c, X.lkeys2, nl); ensure(c, X.lvals, nl); ensure(c, X.lvals2, nl);
+        ensure(c, X.lkeys, (size_t)nl); ensure(c, X.lids, (size_t)nl); ensure(c, X.lslot, (size_t)nl); ensure(c, X.lvals, (size_t)nl);
+        ensure(c, X.lcnt, (size_t)G); ensure(c, X.lbeg, (size_t)G + 1);
+        fillU32(c, X.lcnt.p, G);
         hipLaunchKernelGGL(k_large_cells<1>, dim3((unsigned)bpB[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
-                           (const int32_t*)X.cvals2.p, (int32_t*)nullptr, (const int64_t*)X.loffs.p, X.lkeys.p, X.lvals.p);
+                           (const int32_t*)X.cvals2.p, (uint32_t*)nullptr, (const int64_t*)X.loffE.p, X.lkeys.p, X.lids.p, X.lslot.p, X.lcnt.p);
         checkLaunch();
-        sortPairs(c, X.lkeys.p, X.lkeys2.p, X.lvals.p, X.lvals2.p, nl, bitsFor((uint64_t)G));
-        ensure(c, X.lstart, G); ensure(c, X.lend, G);
-        hipLaunchKernelGGL(k_fill_i32, dim3(nblk(G)), dim3(WDM_BLOCK), 0, st, X.lstart.p, G, -1);
-        hipLaunchKernelGGL(k_cell_ranges, dim3(nblk(nl)), dim3(WDM_BLOCK), 0, st, (const uint32_t*)X.lkeys2.p, nl, X.lstart.p, X.lend.p);
+        scanExclusive(c, (const uint32_t*)X.lcnt.p, X.lbeg.p, G);
+        hipLaunchKernelGGL(k_large_scatter, dim3(nblk(nl)), dim3(WDM_BLOCK), 0, st, (const uint32_t*)X.lkeys.p, (const int32_t*)X.lids.p,
+                           (const uint32_t*)X.lslot.p, nl, (const int32_t*)X.lbeg.p, X.lvals.p);
         checkLaunch();
         return true;
     }
@@ -972,16 +1076,18 @@ struct InsertionSortCollider {
         if (nT == 0) return;
         KTimer tAll(c, "full_run");  // the whole pass on the stream, host synchronisations included
         auto S_ = [&](int t) -> Scene& { return *B.ss[L[t]]; };
-        int maxNp = 1;
-        for (int t = 0; t < nT; t++) maxNp = std::max(maxNp, S_(t).np);
-        const int idBits = bitsFor((uint64_t)maxNp);
-        const int tBits = bitsFor((uint64_t)(nT - 1));
-        if (2 * idBits + tBits > 64) throw WdmError(WDM_E_ARG, "collider batch too large for 64-bit pair keys");
-        const int keyBits = 2 * idBits + tBits;
 
+        // per-particle rows of the batch: scene t's particles p at rowBase + p (and one row
+        // past its last particle)
         std::vector<FRScene> F(nT);
         std::memset(F.data(), 0, F.size() * sizeof(FRScene));
-        for (int t = 0; t < nT; t++) F[t].desc = L[t];
+        int64_t nRows = 0;
+        for (int t = 0; t < nT; t++) {
+            F[t].desc = L[t];
+            F[t].rowBase = nRows;
+            nRows += (int64_t)S_(t).np + 1;
+        }
+        if (nRows >= INT32_MAX) throw WdmError(WDM_E_ARG, "collider batch too large (2^31 particles)");
         const FRScene* FRd = (const FRScene*)X.stFR.upload(c, F.data(), F.size() * sizeof(FRScene));
         std::vector<int64_t> bpP = blockPrefix(nT, [&](int t) { return S_(t).np; });
         std::vector<int64_t> bpB = blockPrefix(nT, [&](int t) { return (int64_t)S_(t).binnedH.size(); });
@@ -994,60 +1100,67 @@ struct InsertionSortCollider {
         int64_t nbTot = 0;
         {
             KTimer tg(c, "grid");
-            buildGrids(B, L, F, bpB, mB, FRd, nbTot, c->cellFactor);
+            buildGrids(B, L, F, bpB, mB, FRd, nbTot, c->cellFactor, c->gridLevels);
         }
         bool anyFacets = false;
         for (int t = 0; t < nT; t++) anyFacets |= S_(t).hasFacets;
         const bool large = anyFacets && listLarge(B, bpB, mB, FRd, nbTot, F[nT - 1].cellBase + F[nT - 1].nCells + 1);
 
-        // candidate pairs: one wavefront per cell writing into its own region, the regions
-        // packed (and the pass run again with larger regions when a cell found more pairs than
-        // they hold), then sorted by (scene, min id, max id)
+        // candidate pairs: one thread per bound entry, each wavefront writing into its own
+        // region (the pass runs again with larger regions when a wavefront found more pairs
+        // than they hold) and counting the pairs per row (scene, min id); then the pairs
+        // bucket-sorted by row and max id, so that they come in (scene, contact key) order
         int64_t ncandTot = 0;
         {
-            std::vector<int64_t> bpQ(nT + 1, 0);  // one wavefront per cell and per large list: 4 per block
-            for (int t = 0; t < nT; t++) bpQ[t + 1] = bpQ[t] + ((int64_t)F[t].nCells + 1 + 3) / 4;
-            Map mQ = uploadMap(c, X.stMapQ, bpQ);
-            const int64_t nW = 4 * bpQ[nT];
-            ensure(c, X.wcnt, std::max<int64_t>(nW, 1)); ensure(c, X.wcnt64, std::max<int64_t>(nW, 1));
-            ensure(c, X.woff, (size_t)nW + 1); ensure(c, X.wmax, 1);
+            const int64_t nBlk = bpB[nT];  // 4 wavefronts per block
+            const int64_t nW = 4 * nBlk;
+            ensure(c, X.wcnt, (size_t)std::max<int64_t>(nW, 1)); ensure(c, X.scnt, (size_t)std::max<int64_t>(nW, 1));
+            ensure(c, X.wmax, 2);
+            ensure(c, X.rowCnt, (size_t)nRows); ensure(c, X.rowOff, (size_t)nRows + 1);
             std::vector<int64_t> first(nT + 1);
-            for (int t = 0; t < nT; t++) first[t] = 4 * bpQ[t];
-            first[nT] = nW;
+            for (int t = 0; t < nT; t++) first[t] = F[t].rowBase;
+            first[nT] = nRows;
             std::vector<int64_t> cs;
+            auto grow = [](int32_t wm) { return (int32_t)std::min<int64_t>(((int64_t)wm + wm / 8 + 63) / 64 * 64, INT32_MAX / 2); };
             for (int attempt = 0;; attempt++) {
-                const int32_t wcap = c->waveCap;
-                ensure(c, X.regK, std::max<int64_t>(nW * wcap, 1)); ensure(c, X.regP, std::max<int64_t>(nW * wcap, 1));
-                hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(WDM_BLOCK), 0, st, X.wmax.p, (int64_t)1, 0);
+                const int32_t scap = c->scanCap, wcap = c->waveCap;
+                const size_t nreg = (size_t)std::max<int64_t>(nW * wcap, 1);
+                ensure(c, X.sPair, (size_t)std::max<int64_t>(nW * scap, 1));
+                ensure(c, X.rItem, nreg); ensure(c, X.rRow, nreg); ensure(c, X.rSlot, nreg);
+                fillU32(c, (uint32_t*)X.wmax.p, 2);
+                fillU32(c, X.rowCnt.p, nRows);
                 if (nW > 0) {
                     KTimer tm(c, "pairs");
-                    hipLaunchKernelGGL(k_cell_pairs, dim3((unsigned)((bpQ[nT] + 7) / 8 * 8)), dim3(WDM_BLOCK), 0, st, B.D, FRd, mQ, (const int32_t*)X.cvals2.p,
-                                       (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p, large ? (const int32_t*)X.lstart.p : nullptr,
-                                       (const int32_t*)X.lend.p, (const int32_t*)X.lvals2.p, (const float4*)X.cbox.p, X.wcnt.p, X.wmax.p, wcap,
-                                       X.regK.p, X.regP.p, idBits);
-                    checkLaunch();
-                    hipLaunchKernelGGL(k_i32_to_i64, dim3(nblk(nW)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.wcnt.p, X.wcnt64.p, nW);
+                    const int64_t pad = 8 * PAIR_CHUNK;
+                    const dim3 grid((unsigned)((nBlk + pad - 1) / pad * pad));
+                    hipLaunchKernelGGL(k_pair_scan, grid, dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
+                                       (const int32_t*)X.cbeg.p, (const float4*)X.cbox.p, X.scnt.p, X.wmax.p, scap, X.sPair.p);
+                    hipLaunchKernelGGL(k_pair_exact, grid, dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const int32_t*)X.cvals2.p,
+                                       (const uint32_t*)X.ckeys2.p, large ? (const int32_t*)X.lbeg.p : nullptr, (const int32_t*)X.lvals.p,
+                                       (const int32_t*)X.scnt.p, scap, (const int2*)X.sPair.p, X.wcnt.p, X.wmax.p + 1, wcap, X.rItem.p,
+                                       X.rRow.p, X.rSlot.p, X.rowCnt.p);
                     checkLaunch();
                 }
-                scanInclusiveShift(c, X.wcnt64.p, X.woff.p, nW);
-                int32_t wm = 0;
-                HIPCHK(hipMemcpyAsync(&wm, X.wmax.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-                cs = segmentStarts(c, X.woff.p, first);  // synchronises (wm is read too)
-                if (wm <= wcap) break;
-                if (attempt > 0) throw WdmError(WDM_E_STATE, "collider: pairs per cell changed between two passes");
-                c->waveCap = (int32_t)std::min<int64_t>(((int64_t)wm + wm / 8 + 63) / 64 * 64, INT32_MAX / 2);
+                scanExclusive(c, (const uint32_t*)X.rowCnt.p, X.rowOff.p, nRows);
+                int32_t wm[2] = {0, 0};
+                HIPCHK(hipMemcpyAsync(wm, X.wmax.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                cs = segmentStarts(c, X.rowOff.p, first);  // synchronises (wm is read too)
+                if (wm[0] <= scap && wm[1] <= wcap) break;
+                if (attempt > 1) throw WdmError(WDM_E_STATE, "collider: pairs per wavefront changed between passes");
+                if (wm[0] > scap) c->scanCap = grow(wm[0]);
+                if (wm[1] > wcap) c->waveCap = grow(wm[1]);
             }
             for (int t = 0; t < nT; t++) { F[t].candStart = cs[t]; F[t].nCand = cs[t + 1] - cs[t]; }
             ncandTot = cs[nT];
-            ensure(c, X.candK, std::max<int64_t>(ncandTot, 1)); ensure(c, X.candK2, std::max<int64_t>(ncandTot, 1));
-            ensure(c, X.candP, std::max<int64_t>(ncandTot, 1)); ensure(c, X.candP2, std::max<int64_t>(ncandTot, 1));
+            const size_t nc1 = (size_t)std::max<int64_t>(ncandTot, 1);
+            ensure(c, X.candK, nc1); ensure(c, X.candP, nc1); ensure(c, X.btmp, nc1); ensure(c, X.btmpB, nc1);
             if (ncandTot > 0) {
                 KTimer tm(c, "pairs_sort");
-                hipLaunchKernelGGL(k_compact_pairs, dim3((unsigned)((nW + 3) / 4)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.wcnt.p,
-                                   (const int64_t*)X.woff.p, nW, c->waveCap, (const uint64_t*)X.regK.p, (const int32_t*)X.regP.p, X.candK.p,
-                                   X.candP.p);
+                hipLaunchKernelGGL(k_cand_scatter, dim3((unsigned)((nW + 3) / 4)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.wcnt.p, nW,
+                                   c->waveCap, (const uint64_t*)X.rItem.p, (const uint32_t*)X.rRow.p, (const uint32_t*)X.rSlot.p,
+                                   (const int64_t*)X.rowOff.p, X.btmp.p, X.btmpB.p);
                 checkLaunch();
-                sortPairs(c, X.candK.p, X.candK2.p, X.candP.p, X.candP2.p, ncandTot, keyBits);
+                bucketFinish(c, ncandTot, (const int64_t*)X.rowOff.p, CandOut{FRd, nT, X.candK.p, X.candP.p});
             }
         }
 
@@ -1070,23 +1183,29 @@ struct InsertionSortCollider {
         const ConDesc<T>* Cd = (const ConDesc<T>*)X.stConFR.upload(c, C.data(), C.size() * sizeof(ConDesc<T>));
         std::vector<int64_t> bpO = blockPrefix(nT, [&](int t) { return F[t].nOld; });
         Map mO = uploadMap(c, X.stMapO, bpO);
-        ensure(c, X.ooFlag, std::max<int64_t>(nOldTot, 1)); ensure(c, X.ooFlag64, std::max<int64_t>(nOldTot, 1));
+        std::vector<int64_t> bp1 = blockPrefix(nT, [&](int t) { return (int64_t)S_(t).np + 1; });
+        Map m1 = uploadMap(c, X.stMapP1, bp1);  // per scene its rows (np + 1 threads)
+        ensure(c, X.ooFlag, (size_t)std::max<int64_t>(nOldTot, 1));
         ensure(c, X.ooRank, (size_t)nOldTot + 1);
+        ensure(c, X.oldRow, (size_t)nRows + 1);
         std::vector<int64_t> noor(nT, 0);
-        if (nOldTot > 0) {
+        {
             KTimer to(c, "old_flags");
-            hipLaunchKernelGGL(k_flag_old<T>, dim3((unsigned)bpO[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mO, (const uint64_t*)X.candK2.p,
-                               X.ooFlag.p, idBits);
-            hipLaunchKernelGGL(k_i32_to_i64, dim3(nblk(nOldTot)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.ooFlag.p, X.ooFlag64.p, nOldTot);
+            hipLaunchKernelGGL(k_old_rows<T>, dim3((unsigned)bp1[nT]), dim3(WDM_BLOCK), 0, st, B.D, Cd, FRd, m1, X.oldRow.p);
             checkLaunch();
-            scanInclusiveShift(c, X.ooFlag64.p, X.ooRank.p, nOldTot);
-            std::vector<int64_t> first(nT + 1);
-            for (int t = 0; t < nT; t++) first[t] = F[t].oldStart;
-            first[nT] = nOldTot;
-            std::vector<int64_t> os = segmentStarts(c, X.ooRank.p, first);
-            for (int t = 0; t < nT; t++) noor[t] = os[t + 1] - os[t];
-        } else {
-            HIPCHK(hipMemsetAsync(X.ooRank.p, 0, sizeof(int64_t), st));
+            if (nOldTot > 0) {
+                hipLaunchKernelGGL(k_flag_old<T>, dim3((unsigned)bpO[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mO, (const int64_t*)X.rowOff.p,
+                                   (const uint64_t*)X.candK.p, X.ooFlag.p);
+                checkLaunch();
+            }
+            scanExclusive(c, (const int32_t*)X.ooFlag.p, X.ooRank.p, nOldTot);
+            if (nOldTot > 0) {
+                std::vector<int64_t> first(nT + 1);
+                for (int t = 0; t < nT; t++) first[t] = F[t].oldStart;
+                first[nT] = nOldTot;
+                std::vector<int64_t> os = segmentStarts(c, X.ooRank.p, first);
+                for (int t = 0; t < nT; t++) noor[t] = os[t + 1] - os[t];
+            }
         }
 
         // merged lists
@@ -1125,18 +1244,18 @@ struct InsertionSortCollider {
             KTimer tm(c, "merge");
             if (ncandTot > 0)
                 hipLaunchKernelGGL(k_emit_cand<T>, dim3((unsigned)bpC[nT]), dim3(WDM_BLOCK), 0, st, B.D, Cd, FRd, mC, (const int64_t*)X.ooRank.p,
-                                   (const uint64_t*)X.candK2.p, (const int32_t*)X.candP2.p, X.oslot.p, X.rflag.p, idBits);
+                                   (const int64_t*)X.oldRow.p, (const uint64_t*)X.candK.p, (const int32_t*)X.candP.p, X.oslot.p, X.rflag.p);
             if (noorTot > 0)
                 hipLaunchKernelGGL(k_emit_oor<T>, dim3((unsigned)bpO[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mO, (const int32_t*)X.ooFlag.p,
-                                   (const int64_t*)X.ooRank.p, (const uint64_t*)X.candK2.p, X.oslot.p, X.rflag.p, idBits);
+                                   (const int64_t*)X.ooRank.p, (const int64_t*)X.rowOff.p, (const uint64_t*)X.candK.p, X.oslot.p, X.rflag.p);
             checkLaunch();
-            scanInclusiveShift(c, X.rflag.p, X.rscan.p, nNewTot);
+            scanExclusive(c, (const int64_t*)X.rflag.p, X.rscan.p, nNewTot);
             if (nNewTot > 0)
                 hipLaunchKernelGGL(k_copy_state<T>, dim3((unsigned)bpN[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mN, (const int32_t*)X.oslot.p,
                                    (const int64_t*)X.rscan.p, X.nrflag.p);
             checkLaunch();
             // ContactLoop work list: rows first, then the other non-dormant contacts
-            scanInclusiveShift(c, X.nrflag.p, X.nrscan.p, nNewTot);
+            scanExclusive(c, (const int64_t*)X.nrflag.p, X.nrscan.p, nNewTot);
             if (nNewTot > 0)
                 hipLaunchKernelGGL(k_items<T>, dim3((unsigned)bpN[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mN, (const int64_t*)X.rscan.p,
                                    (const int64_t*)X.nrscan.p);
@@ -1151,24 +1270,29 @@ struct InsertionSortCollider {
             S.nFullRuns++;
             S.needFull = false;
         }
-        // CSR (contact -> particle)
+        // CSR (contact -> particle): entries counted per particle row, then bucket-sorted by
+        // contact index within each row
         {
             KTimer tm(c, "csr");
-            int64_t ne = 2 * nNewTot;
-            ensure(c, X.entK, std::max<int64_t>(ne, 1)); ensure(c, X.entK2, std::max<int64_t>(ne, 1));
-            ensure(c, X.entV, std::max<int64_t>(ne, 1)); ensure(c, X.entV2, std::max<int64_t>(ne, 1));
+            const int64_t ne = 2 * nNewTot;
+            const size_t ne1 = (size_t)std::max<int64_t>(ne, 1);
+            ensure(c, X.eB, ne1); ensure(c, X.eS, ne1); ensure(c, X.eI, ne1); ensure(c, X.btmp, ne1); ensure(c, X.btmpB, ne1);
+            ensure(c, X.csrCnt, (size_t)nRows); ensure(c, X.csrScan, (size_t)nRows + 1);
+            fillU32(c, X.csrCnt.p, nRows);
             if (nNewTot > 0) {
-                hipLaunchKernelGGL(k_csr_entries<T>, dim3((unsigned)bpN[nT]), dim3(WDM_BLOCK), 0, st, B.D, Cd, FRd, mN, X.entK.p, X.entV.p, idBits);
-                checkLaunch();
-                sortPairs(c, X.entK.p, X.entK2.p, X.entV.p, X.entV2.p, ne, idBits + tBits);  // stable: other id order kept
-                std::vector<int64_t> bpE = blockPrefix(nT, [&](int t) { return 2 * F[t].nNew; });
-                Map mE = uploadMap(c, X.stMapE, bpE);
-                hipLaunchKernelGGL(k_csr_scatter, dim3((unsigned)bpE[nT]), dim3(WDM_BLOCK), 0, st, FRd, mE, (const uint32_t*)X.entV2.p);
+                hipLaunchKernelGGL(k_csr_count<T>, dim3((unsigned)bpN[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mN, X.eB.p, X.eS.p, X.eI.p,
+                                   X.csrCnt.p);
                 checkLaunch();
             }
-            std::vector<int64_t> bp1 = blockPrefix(nT, [&](int t) { return (int64_t)S_(t).np + 1; });
-            Map m1 = uploadMap(c, X.stMapP1, bp1);
-            hipLaunchKernelGGL(k_csr_offsets, dim3((unsigned)bp1[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, m1, (const uint64_t*)X.entK2.p, idBits);
+            scanExclusive(c, (const uint32_t*)X.csrCnt.p, X.csrScan.p, nRows);
+            if (ne > 0) {
+                hipLaunchKernelGGL(k_csr_scatter, dim3(nblk(ne)), dim3(WDM_BLOCK), 0, st, (const uint32_t*)X.eB.p, (const uint32_t*)X.eS.p,
+                                   (const uint64_t*)X.eI.p, ne, (const int64_t*)X.csrScan.p, X.btmp.p, X.btmpB.p);
+                checkLaunch();
+                bucketFinish(c, ne, (const int64_t*)X.csrScan.p, CsrOut{FRd, nT, (const int64_t*)X.csrScan.p},
+                             (const int64_t*)X.csrScan.p + nRows);
+            }
+            hipLaunchKernelGGL(k_csr_offsets, dim3((unsigned)bp1[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, m1, (const int64_t*)X.csrScan.p);
             checkLaunch();
         }
     }
@@ -1932,9 +2056,9 @@ void relax(Scene& S, int iters) {
         hipLaunchKernelGGL(k_bounds, dim3((unsigned)bpP[1]), dim3(WDM_BLOCK), 0, c->stream, B.D, FRd, mP, 1);
         checkLaunch();
         int64_t nbTot = 0;
-        InsertionSortCollider::buildGrids(B, L, F, bpB, mB, FRd, nbTot, 1.);
+        InsertionSortCollider::buildGrids(B, L, F, bpB, mB, FRd, nbTot, 1., 1);
         hipLaunchKernelGGL(k_relax, dim3((unsigned)bpB[1]), dim3(WDM_BLOCK), 0, c->stream, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
-                           (const int32_t*)X.cvals2.p, (const int32_t*)X.cstart.p, (const int32_t*)X.cend.p, X.disp.p);
+                           (const int32_t*)X.cvals2.p, (const int32_t*)X.cbeg.p, X.disp.p);
         hipLaunchKernelGGL(k_relax_apply, dim3((unsigned)bpB[1]), dim3(WDM_BLOCK), 0, c->stream, B.D, FRd, mB, (const double4*)X.disp.p);
         checkLaunch();
     }
@@ -2054,6 +2178,7 @@ int wdm_create(wdm_ctx** out, int device, size_t budget) {
     // the device address of the no-allocation entry point's memory, once per device, with that
     // device current; a failure is kept and reported by that entry point only
     c->scratchErr = scratchInit(device);
+    if (const char* lv = std::getenv("WDM_GRID_LEVELS")) c->gridLevels = std::max(0, std::min(MAXLVL, std::atoi(lv)));
     *out = c;
     return WDM_OK;
 }
