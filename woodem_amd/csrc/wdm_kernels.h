@@ -792,17 +792,18 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_bounds(const SceneDesc* D, const 
 }
 
 // extent reduction over binned (finite-bound) particles: min corner, max corner, max extent;
-// and per scene a histogram of the bounds' largest extents in quarter octaves
+// and per scene a histogram of the largest extents of a sample of the bounds in quarter octaves
 // (bin = floor(4 log2 ext) + 128, clamped to [0, EXT_BINS)), from which the host picks the
 // grid levels
 #define EXT_BINS 256
+#define EXT_SAMPLE 8  // the bounds of every 8th block of a scene enter the histogram (few same-address atomics)
 __device__ __forceinline__ int extBin(double ext) {
     if (!(ext > 0)) return 0;
     const double b = floor(4. * log2(ext)) + 128.;
     return b < 0. ? 0 : (b > EXT_BINS - 1 ? EXT_BINS - 1 : (int)b);
 }
 __global__ void __launch_bounds__(WDM_BLOCK) k_extent(const SceneDesc* D, const FRScene* FR, Map m, double* part, uint32_t* hist) {
-    __shared__ double sh[8][WDM_BLOCK];
+    __shared__ double sh[8][WDM_BLOCK / 64];
     __shared__ uint32_t hs[EXT_BINS];
     int64_t lb;
     const int st = mapBlock(m, lb);
@@ -820,21 +821,28 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_extent(const SceneDesc* D, const 
         v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = b.x; v[4] = b.y; v[5] = b.z;
         v[6] = fmax(fmax(b.x - a.x, b.y - a.y), b.z - a.z);
         if (ptShape(p.ptype[id]) == 1) v[7] = v[6];
-        atomicAdd(&hs[extBin(v[6])], 1u);
+        if (lb % EXT_SAMPLE == 0) atomicAdd(&hs[extBin(v[6])], 1u);
     }
     __syncthreads();
     for (int k = t; k < EXT_BINS; k += WDM_BLOCK)
         if (hs[k]) atomicAdd(&hist[(size_t)st * EXT_BINS + k], hs[k]);
-    for (int k = 0; k < 8; k++) sh[k][t] = v[k];
-    __syncthreads();
-    for (int s = WDM_BLOCK / 2; s > 0; s >>= 1) {
-        if (t < s) {
-            for (int k = 0; k < 3; k++) sh[k][t] = fmin(sh[k][t], sh[k][t + s]);
-            for (int k = 3; k < 8; k++) sh[k][t] = fmax(sh[k][t], sh[k][t + s]);
+    // minima and maxima over each wavefront, then over the block's wavefronts
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const double y = __shfl_xor(v[k], o, 64);
+            v[k] = k < 3 ? fmin(v[k], y) : fmax(v[k], y);
         }
-        __syncthreads();
     }
-    if (t == 0) for (int k = 0; k < 8; k++) part[(size_t)blockIdx.x * 8 + k] = sh[k][0];
+    if ((t & 63) == 0)
+        for (int k = 0; k < 8; k++) sh[k][t >> 6] = v[k];
+    __syncthreads();
+    if (t < 8) {
+        double r = sh[t][0];
+        for (int w = 1; w < WDM_BLOCK / 64; w++) r = t < 3 ? fmin(r, sh[t][w]) : fmax(r, sh[t][w]);
+        part[(size_t)blockIdx.x * 8 + t] = r;
+    }
 }
 // per scene (one block each): min corner, max corner, max extent, max sphere extent over its blocks
 __global__ void __launch_bounds__(WDM_BLOCK) k_extent_finalize(Map m, const double* part, double* out) {
@@ -945,30 +953,38 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_scan_fin(const TIn* in, int64_t n
 // ---------------- bucket sort: 64-bit items grouped by a bucket index, each bucket in
 // ascending item order (the high 32 bits of the items of one bucket are distinct) ----------------
 // The producer counts its items per bucket with atomics (the count returned is the item's
-// slot) and scatters them to tmp[off[b] + slot] with tmpB = b, off the exclusive prefix of
-// the counts.  k_bucket_rank then places each item of a bucket of at most BKT_SMALL items
+// slot) and scatters them as records (item, b) to tmp[off[b] + slot] (one 16-byte store), off
+// the exclusive prefix of the counts.  k_bucket_rank then places each item of a bucket of at most BKT_SMALL items
 // by counting the smaller items of its bucket; longer buckets are listed and sorted by
 // k_bucket_long, one workgroup each (bitonic sort in LDS, merged through global memory
 // when a bucket exceeds BKT_LDS items).  The writer w(b, position, item) stores the result.
 #define BKT_SMALL 64
 #define BKT_LDS 8192
 #define BKT_THREADS 1024
+struct __align__(16) BItem {
+    uint64_t x;  // the item
+    uint32_t b;  // its bucket
+    uint32_t pad;
+};
+__device__ __forceinline__ void putItem(BItem* tmp, int64_t pos, uint64_t x, uint32_t b) {
+    *(ulonglong2*)&tmp[pos] = make_ulonglong2(x, (unsigned long long)b);
+}
 // n: items, or (total non-null) at most n items, *total of them
 template <typename OffT, class W>
-__global__ void __launch_bounds__(WDM_BLOCK) k_bucket_rank(const uint64_t* tmp, const uint32_t* tmpB, const OffT* off, int64_t n,
-                                                          const OffT* total, uint32_t* longList, uint32_t* longCnt, W w) {
+__global__ void __launch_bounds__(WDM_BLOCK) k_bucket_rank(const BItem* tmp, const OffT* off, int64_t n, const OffT* total,
+                                                          uint32_t* longList, uint32_t* longCnt, W w) {
     const int64_t q = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
     if (q >= (total ? (int64_t)*total : n)) return;
-    const uint32_t b = tmpB[q];
+    const BItem it = tmp[q];
+    const uint32_t b = it.b;
     const int64_t s = (int64_t)off[b], e = (int64_t)off[b + 1];
     if (e - s > BKT_SMALL) {
         if (q == s) longList[atomicAdd(longCnt, 1u)] = b;
         return;
     }
-    const uint64_t x = tmp[q];
     int r = 0;
-    for (int64_t k = s; k < e; k++) r += tmp[k] < x ? 1 : 0;
-    w(b, s + r, x);
+    for (int64_t k = s; k < e; k++) r += tmp[k].x < it.x ? 1 : 0;
+    w(b, s + r, it.x);
 }
 __device__ __forceinline__ void bitonicLds(uint64_t* sh, int P) {
     for (int k = 2; k <= P; k <<= 1)
@@ -986,10 +1002,10 @@ __device__ __forceinline__ void bitonicLds(uint64_t* sh, int P) {
             __syncthreads();
         }
 }
-// aux: scratch of the same size as tmp; tmp's entries of a long bucket are overwritten
+// aux: scratch of twice as many items as tmp holds
 template <typename OffT, class W>
-__global__ void __launch_bounds__(BKT_THREADS) k_bucket_long(uint64_t* tmp, uint64_t* aux, const OffT* off, const uint32_t* longList,
-                                                            const uint32_t* longCnt, W w) {
+__global__ void __launch_bounds__(BKT_THREADS) k_bucket_long(const BItem* tmp, uint64_t* aux, int64_t n, const OffT* off,
+                                                            const uint32_t* longList, const uint32_t* longCnt, W w) {
     __shared__ uint64_t sh[BKT_LDS];
     const uint32_t nl = *longCnt;
     for (uint32_t li = blockIdx.x; li < nl; li += gridDim.x) {
@@ -999,7 +1015,7 @@ __global__ void __launch_bounds__(BKT_THREADS) k_bucket_long(uint64_t* tmp, uint
             const int cl = (int)(len - c0 < BKT_LDS ? len - c0 : BKT_LDS);
             int P = 1;
             while (P < cl) P <<= 1;
-            for (int k = threadIdx.x; k < P; k += BKT_THREADS) sh[k] = k < cl ? tmp[s + c0 + k] : ~0ull;
+            for (int k = threadIdx.x; k < P; k += BKT_THREADS) sh[k] = k < cl ? tmp[s + c0 + k].x : ~0ull;
             __syncthreads();
             bitonicLds(sh, P);
             for (int k = threadIdx.x; k < cl; k += BKT_THREADS) aux[s + c0 + k] = sh[k];
@@ -1008,7 +1024,7 @@ __global__ void __launch_bounds__(BKT_THREADS) k_bucket_long(uint64_t* tmp, uint
         // sorted runs of width wd merged pairwise: an item's place is its index in its run
         // plus the number of smaller items in the other run
         uint64_t* src = aux + s;
-        uint64_t* dst = tmp + s;
+        uint64_t* dst = aux + n + s;
         for (int64_t wd = BKT_LDS; wd < len; wd <<= 1) {
             for (int64_t k = threadIdx.x; k < len; k += BKT_THREADS) {
                 const int64_t r0 = (k / wd) * wd, p0 = ((k / wd) ^ 1) * wd;
@@ -1108,7 +1124,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_cell_count(const SceneDesc* D, co
 }
 // item (id << 32) of each entry at its cell's offset plus its slot
 __global__ void __launch_bounds__(WDM_BLOCK) k_cell_scatter(const SceneDesc* D, const FRScene* FR, Map m, const uint32_t* ekey,
-                                                           const uint32_t* eslot, const int32_t* cbeg, uint64_t* tmp, uint32_t* tmpB) {
+                                                           const uint32_t* eslot, const int32_t* cbeg, BItem* tmp) {
     int64_t lb;
     const FRScene& F = FR[mapBlock(m, lb)];
     const SceneDesc& S = D[F.desc];
@@ -1116,10 +1132,11 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_cell_scatter(const SceneDesc* D, 
     if (i >= S.nbinned) return;
     const uint32_t key = ekey[F.binStart + i];
     const int64_t pos = (int64_t)cbeg[key] + eslot[F.binStart + i];
-    tmp[pos] = (uint64_t)(uint32_t)S.binned[i] << 32;
-    tmpB[pos] = key;
+    putItem(tmp, pos, (uint64_t)(uint32_t)S.binned[i] << 32, key);
 }
-// the sorted entries: particle id, cell key and FP32 box, by (scene, cell, id)
+// the sorted entries, by (scene, cell, id): particle id, cell key, FP32 box, and a copy of
+// the bound's corners, shape and mask (the exact pair test reads a partner by entry, without
+// going through its id)
 struct CellOut {
     const SceneDesc* D;
     const FRScene* FR;
@@ -1127,13 +1144,21 @@ struct CellOut {
     int32_t* cvals;
     uint32_t* ckeys;
     float4* boxes;
+    double4 *eb0, *eb1;
+    int2* einfo;
     __device__ void operator()(uint32_t b, int64_t pos, uint64_t x) const {
         const FRScene& F = FR[sceneOfCell(FR, nT, b)];
         const PartView& p = D[F.desc].p;
         const int id = (int)(x >> 32);
+        const double4 a0 = p.bmin[id], a1 = p.bmax[id];
         cvals[pos] = id;
         ckeys[pos] = b;
-        boxes[pos] = cellBox(F, p.bmin[id], p.bmax[id]);
+        boxes[pos] = cellBox(F, a0, a1);
+        if (eb0) {
+            eb0[pos] = a0;
+            eb1[pos] = a1;
+            einfo[pos] = make_int2(ptShape(p.ptype[id]), (int)p.mask[id]);
+        }
     }
 };
 
@@ -1180,9 +1205,10 @@ WDM_HD void canonical(int i, int j, int si, int sj, int& A, int& B, int& type) {
 
 // Collider::mayCollide, pkg/dem/Collision.cpp:19-41; master: clump master per node (null
 // when the scene has no clumps)
-__device__ __forceinline__ bool mayCollideD(const PartView& p, const GP int32_t* master, int a, int b, int sa, int sb, uint32_t loneMask) {
+// ma, mb: the masks of a and b
+__device__ __forceinline__ bool mayCollideM(const PartView& p, const GP int32_t* master, int a, int b, int sa, int sb, uint32_t ma, uint32_t mb,
+                                            uint32_t loneMask) {
     if (a == b || sa == 0 || sb == 0) return false;
-    uint32_t ma = p.mask[a], mb = p.mask[b];
     if (!(ma & mb)) return false;
     if ((ma & mb & loneMask) != 0) return false;
     if (sa == 3 || sb == 3 || master) {
@@ -1199,6 +1225,9 @@ __device__ __forceinline__ bool mayCollideD(const PartView& p, const GP int32_t*
             }
     }
     return true;
+}
+__device__ __forceinline__ bool mayCollideD(const PartView& p, const GP int32_t* master, int a, int b, int sa, int sb, uint32_t loneMask) {
+    return mayCollideM(p, master, a, b, sa, sb, p.mask[a], p.mask[b], loneMask);
 }
 
 // FP64 prefilter of the exact test: minimum-image distance of the bound centres
@@ -1295,22 +1324,26 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_large_scatter(const uint32_t* lke
 // one candidate of bound `id` (corners a0, a1, shape sa) with particle j: canonical order,
 // the collider's exact overlap (with periods) and mayCollide; a binned j must pass the
 // FP64 prefilter first (it has finite bounds)
-__device__ __forceinline__ bool pairTest(const SceneDesc& S, const GridArgs& g, int id, const double4& a0, const double4& a1, int sa, int j,
-                                         bool binned, uint32_t* err, int per[3]) {
-    const PartView& p = S.p;
-    const double4 b0 = p.bmin[j], b1 = p.bmax[j];
+// the same with j's bounds (b0, b1), shape sb and the masks ma, mb given
+__device__ __forceinline__ bool pairTestB(const SceneDesc& S, const GridArgs& g, int id, const double4& a0, const double4& a1, int sa,
+                                          uint32_t ma, int j, const double4& b0, const double4& b1, int sb, uint32_t mb, bool binned,
+                                          uint32_t* err, int per[3]) {
     if (binned) {
         if (g.periodic) { if (!periPrefilter(a0, a1, b0, b1, g)) return false; }
         else if (!overlapAperiodic(a0, a1, b0, b1)) return false;
     }
-    const int sb = ptShape(p.ptype[j]);
     int A, B, ty;
     canonical(id, j, sa, sb, A, B, ty);
     const bool aIsId = (A == id);
     const double4 &A0 = aIsId ? a0 : b0, &A1 = aIsId ? a1 : b1, &B0 = aIsId ? b0 : a0, &B1 = aIsId ? b1 : a1;
     per[0] = per[1] = per[2] = 0;
     const bool ov = g.periodic ? overlapPeri(A0, A1, B0, B1, g.L, per, err) : (binned || overlapAperiodic(A0, A1, B0, B1));
-    return ov && mayCollideD(p, S.n.master, A, B, ptShape(p.ptype[A]), ptShape(p.ptype[B]), S.a.loneMask);
+    return ov && mayCollideM(S.p, S.n.master, A, B, aIsId ? sa : sb, aIsId ? sb : sa, aIsId ? ma : mb, aIsId ? mb : ma, S.a.loneMask);
+}
+__device__ __forceinline__ bool pairTest(const SceneDesc& S, const GridArgs& g, int id, const double4& a0, const double4& a1, int sa, int j,
+                                         bool binned, uint32_t* err, int per[3]) {
+    const PartView& p = S.p;
+    return pairTestB(S, g, id, a0, a1, sa, p.mask[id], j, p.bmin[j], p.bmax[j], ptShape(p.ptype[j]), p.mask[j], binned, err, per);
 }
 
 // Candidate pairs, one thread per bound entry (in the (scene, level, cell, id) order of the
@@ -1368,9 +1401,10 @@ __device__ __forceinline__ int64_t xcdChunkBlock() {
     return ((r / PAIR_CHUNK) * 8 + x) * PAIR_CHUNK + (r % PAIR_CHUNK);
 }
 // this wavefront's index (regions, counts), its scene and its block within the scene; false
-// for padding
+// for padding.  The logical blocks run backwards, so that the costly entries of the coarser
+// levels (last in each scene) start first and do not trail the launch.
 __device__ __forceinline__ bool pairWave(const Map& m, int64_t& gw, int& t, int64_t& lb) {
-    const int64_t lbk = xcdChunkBlock();
+    const int64_t lbk = (int64_t)gridDim.x - 1 - xcdChunkBlock();
     if (lbk >= m.bprefix[m.n]) return false;
     t = mapBlockAt(m, lbk, lb);
     gw = lbk * (WDM_BLOCK / 64) + (threadIdx.x >> 6);
@@ -1510,9 +1544,10 @@ __global__ void __launch_bounds__(WDM_BLOCK, 6) k_pair_scan(const SceneDesc* D, 
     }
 }
 __global__ void __launch_bounds__(WDM_BLOCK) k_pair_exact(const SceneDesc* D, const FRScene* FR, Map m, const int32_t* cvals,
-                                                         const uint32_t* ckeys, const int32_t* lbeg, const int32_t* lvals, const int32_t* cnt1,
-                                                         int32_t cap1, const int2* in, int32_t* wcnt, int32_t* wmax, int32_t wcap,
-                                                         uint64_t* rItem, uint32_t* rRow, uint32_t* rSlot, uint32_t* rowCnt) {
+                                                         const uint32_t* ckeys, const double4* eb0, const double4* eb1, const int2* einfo,
+                                                         const int32_t* lbeg, const int32_t* lvals, const int32_t* cnt1, int32_t cap1,
+                                                         const int2* in, int32_t* wcnt, int32_t* wmax, int32_t wcap, uint64_t* rItem,
+                                                         uint32_t* rRow, uint32_t* rSlot, uint32_t* rowCnt) {
     int64_t gw, lb;
     int tt;
     if (!pairWave(m, gw, tt, lb)) return;
@@ -1554,7 +1589,8 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pair_exact(const SceneDesc* D, co
             const int2 q = ri[b + lane];
             oid = cvals[q.x];
             j = cvals[q.y];
-            ok = pairTest(S, g, oid, p.bmin[oid], p.bmax[oid], ptShape(p.ptype[oid]), j, true, err, per);
+            const int2 ia = einfo[q.x], ib = einfo[q.y];
+            ok = pairTestB(S, g, oid, eb0[q.x], eb1[q.x], ia.x, (uint32_t)ia.y, j, eb0[q.y], eb1[q.y], ib.x, (uint32_t)ib.y, true, err, per);
         }
         emit(ok, oid, j, per);
     }
@@ -1595,8 +1631,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pair_exact(const SceneDesc* D, co
 }
 // each pair of the wavefront regions at its row's offset plus its slot (rows by min id)
 __global__ void __launch_bounds__(WDM_BLOCK) k_cand_scatter(const int32_t* wcnt, int64_t nw, int32_t wcap, const uint64_t* rItem,
-                                                           const uint32_t* rRow, const uint32_t* rSlot, const int64_t* rowOff, uint64_t* tmp,
-                                                           uint32_t* tmpB) {
+                                                           const uint32_t* rRow, const uint32_t* rSlot, const int64_t* rowOff, BItem* tmp) {
     const int64_t w = (int64_t)blockIdx.x * (WDM_BLOCK / 64) + (threadIdx.x >> 6);
     if (w >= nw) return;
     const int n = wcnt[w];
@@ -1604,8 +1639,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_cand_scatter(const int32_t* wcnt,
         const int64_t i = w * (int64_t)wcap + k;
         const uint32_t row = rRow[i];
         const int64_t pos = rowOff[row] + rSlot[i];
-        tmp[pos] = rItem[i];
-        tmpB[pos] = row;
+        putItem(tmp, pos, rItem[i], row);
     }
 }
 // the sorted candidates: contact key (min id << 32 | max id) and packed periods
@@ -1825,13 +1859,14 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_emit_oor(const ConDesc<T>* C, con
 // nrflag: for a contact that enters the work list without a state row (not dormant), 1 in
 // the low 32 bits plus 1 in the high 32 bits when it is a near pair, so that one scan counts
 // both (NR_ALL, NR_NEAR)
+// srcRow[newStart + row]: the old state row each new row copies (-2: none, a touching pair)
 template <typename T>
 __global__ void __launch_bounds__(WDM_BLOCK) k_copy_state(const ConDesc<T>* C, const FRScene* FR, Map m, const int32_t* oslot,
-                                                         const int64_t* rscan, int64_t* nrflag) {
+                                                         const int64_t* rscan, int64_t* nrflag, int32_t* srcRow) {
     int64_t lb;
     const int t = mapBlock(m, lb);
     const FRScene& F = FR[t];
-    const ConView<T> oldc = C[t].cv, nc = C[t].cvNew;
+    const ConView<T> nc = C[t].cvNew;
     int64_t d = lb * WDM_BLOCK + threadIdx.x;
     if (d >= F.nNew) return;
     int32_t os = oslot[F.newStart + d];
@@ -1844,6 +1879,19 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_copy_state(const ConDesc<T>* C, c
     nrflag[F.newStart + d] = 0;
     int64_t ns = rscan[F.newStart + d] - rscan[F.newStart];
     nc.slot[d] = (int32_t)ns;
+    srcRow[F.newStart + ns] = os;
+}
+// the state of the new rows, one thread per row (the rows of a scene number at most its contacts)
+template <typename T>
+__global__ void __launch_bounds__(WDM_BLOCK) k_copy_rows(const ConDesc<T>* C, const FRScene* FR, Map m, const int64_t* rscan,
+                                                        const int32_t* srcRow) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const FRScene& F = FR[t];
+    const ConView<T> oldc = C[t].cv, nc = C[t].cvNew;
+    const int64_t ns = lb * WDM_BLOCK + threadIdx.x;
+    if (ns >= rscan[F.newStart + F.nNew] - rscan[F.newStart]) return;
+    const int32_t os = srcRow[F.newStart + ns];
     if (os < 0) return;
     const int64_t nROld = *oldc.counter;
     for (int k = 0; k < NSTATE; k++) nc.st[(size_t)k * nc.cap + ns] = *stAt(oldc, os, k, nROld);
@@ -1905,33 +1953,47 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_csr_count(const ConDesc<T>* C, co
     const int t = mapBlock(m, lb);
     const FRScene& F = FR[t];
     const ConView<T> nc = C[t].cvNew;
-    int64_t c = lb * WDM_BLOCK + threadIdx.x;
-    if (c >= F.nNew) return;
+    const int64_t c = lb * WDM_BLOCK + threadIdx.x;
     const int64_t g = F.newStart + c;
-    if (nc.info[c] & (CI_DORMANT | CI_DEAD)) {
+    bool act = c < F.nNew;
+    if (act && (nc.info[c] & (CI_DORMANT | CI_DEAD))) {
         eB[2 * g] = UINT32_MAX;
         eB[2 * g + 1] = UINT32_MAX;
-        return;
+        act = false;
     }
-    const int2 id = nc.ids[c];
+    int2 id = make_int2(0, 0);
+    if (act) id = nc.ids[c];
+    const uint32_t b0 = act ? (uint32_t)(F.rowBase + id.x) : UINT32_MAX, b1 = (uint32_t)(F.rowBase + id.y);
+    // side A: the contacts come sorted by min id, so lanes in a row mostly share b0; one
+    // atomic per run of equal b0 (lanes of a run take consecutive slots)
+    const int lane = threadIdx.x & 63;
+    const uint32_t prev = (uint32_t)__shfl_up((int)b0, 1, 64);
+    const bool head = lane == 0 || prev != b0;
+    const uint64_t heads = __ballot(head);
+    const uint64_t upTo = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    const int hl = 63 - __clzll(heads & upTo);
+    const uint64_t after = heads & ~upTo;
+    const int runEnd = after ? __ffsll((long long)after) - 1 : 64;
+    uint32_t base = 0;
+    if (head && act) base = atomicAdd(&cnt[b0], (uint32_t)(runEnd - hl));
+    base = (uint32_t)__shfl((int)base, hl, 64);
+    if (!act) return;
     const uint32_t it = (uint32_t)nc.itm[c];
-    const uint32_t b0 = (uint32_t)(F.rowBase + id.x), b1 = (uint32_t)(F.rowBase + id.y);
     eB[2 * g] = b0;
-    eS[2 * g] = atomicAdd(&cnt[b0], 1u);
+    eS[2 * g] = base + (uint32_t)(lane - hl);
     eI[2 * g] = ((uint64_t)c << 32) | it;
     eB[2 * g + 1] = b1;
     eS[2 * g + 1] = atomicAdd(&cnt[b1], 1u);
     eI[2 * g + 1] = ((uint64_t)c << 32) | (it | 0x80000000u);
 }
 __global__ void __launch_bounds__(WDM_BLOCK) k_csr_scatter(const uint32_t* eB, const uint32_t* eS, const uint64_t* eI, int64_t n,
-                                                          const int64_t* off, uint64_t* tmp, uint32_t* tmpB) {
+                                                          const int64_t* off, BItem* tmp) {
     const int64_t k = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
     if (k >= n) return;
     const uint32_t b = eB[k];
     if (b == UINT32_MAX) return;
     const int64_t pos = off[b] + eS[k];
-    tmp[pos] = eI[k];
-    tmpB[pos] = b;
+    putItem(tmp, pos, eI[k], b);
 }
 // the sorted entries into each scene's own CSR array
 struct CsrOut {

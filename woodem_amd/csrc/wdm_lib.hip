@@ -256,8 +256,11 @@ struct Shared {
     size_t hostWordsCap = 0;
     std::vector<int32_t> freeSlots;
     int32_t nslots = 0;
-    Staged stDesc, stMapNode, stMapCon, stCon, stFR, stConFR, stIdx, stDyn;
-    Staged stMapP, stMapB, stMapC, stMapO, stMapN, stMapP1;
+    Staged stDesc, stMapNode, stMapCon, stCon, stFR, stDyn;
+    Staged stMapP, stMapB;
+    Staged stRunA, stRunO, stRunN;  // collider full run: the tables of its phases
+    void* pin = nullptr;            // pinned readback memory
+    size_t pinCap = 0;
     hipEvent_t evWords = nullptr;  // recorded after the copy of the words to the host
     // energy partials per block
     DBuf<double> epartL, epartC;
@@ -267,11 +270,14 @@ struct Shared {
     DBuf<uint32_t> ekey, eslot, ccnt;    // bound entries: cell key and slot; count per cell
     DBuf<int32_t> cbeg;                  // first entry of each cell (exclusive prefix of ccnt)
     DBuf<uint32_t> ckeys2;               // sorted entries: cell key, particle id, FP32 box
-    DBuf<int32_t> cvals2, ooFlag, oslot;
+    DBuf<int32_t> cvals2, ooFlag, oslot, srcRow;
     DBuf<float4> cbox;
+    DBuf<double4> eb0, eb1;  // sorted entries: copies of the bound corners, (shape, mask)
+    DBuf<int2> einfo;
     // bucket sort scratch: items, their buckets, merge buffer, long buckets
-    DBuf<uint64_t> btmp, baux;
-    DBuf<uint32_t> btmpB, longList, longCnt;
+    DBuf<BItem> btmp;
+    DBuf<uint64_t> baux;
+    DBuf<uint32_t> longList, longCnt;
     DBuf<int64_t> tsum;                  // prefix sums: per-tile sums
     // candidate pairs: per wavefront of k_pairs its count and region; per row (scene, min id)
     // the count and offsets
@@ -324,19 +330,21 @@ struct Shared {
         if (hostWords) (void)hipHostFree(hostWords);
         hostWords = nullptr;
         hostWordsCap = 0;
-        for (Staged* s : {&stDesc, &stMapNode, &stMapCon, &stCon, &stFR, &stConFR, &stIdx, &stDyn, &stMapP, &stMapB, &stMapC, &stMapO, &stMapN,
-                          &stMapP1})
+        for (Staged* s : {&stDesc, &stMapNode, &stMapCon, &stCon, &stFR, &stDyn, &stMapP, &stMapB, &stRunA, &stRunO, &stRunN})
             s->free(c);
         release(c, epartL); release(c, epartC); release(c, extPart); release(c, extOut); release(c, hist);
-        release(c, ekey); release(c, eslot); release(c, ccnt); release(c, cbeg); release(c, ckeys2); release(c, cvals2); release(c, cbox);
-        release(c, btmp); release(c, baux); release(c, btmpB); release(c, longList); release(c, longCnt); release(c, tsum);
+        release(c, ekey); release(c, eslot); release(c, ccnt); release(c, cbeg); release(c, ckeys2); release(c, cvals2); release(c, cbox); release(c, eb0); release(c, eb1); release(c, einfo);
+        release(c, btmp); release(c, baux); release(c, longList); release(c, longCnt); release(c, tsum);
         release(c, wcnt); release(c, wmax); release(c, scnt); release(c, sPair); release(c, rItem); release(c, rRow); release(c, rSlot); release(c, rowCnt); release(c, rowOff);
-        release(c, oldRow); release(c, ooFlag); release(c, oslot);
+        release(c, oldRow); release(c, ooFlag); release(c, oslot); release(c, srcRow);
         release(c, ooRank); release(c, rflag); release(c, rscan); release(c, starts); release(c, nrflag); release(c, nrscan); release(c, candK);
         release(c, candP); release(c, eB); release(c, eS); release(c, csrCnt); release(c, eI); release(c, csrScan);
         release(c, temp); release(c, disp); release(c, ddOut);
         release(c, lcntE); release(c, lkeys); release(c, lslot); release(c, lcnt); release(c, loffE); release(c, lids); release(c, lbeg);
         release(c, lvals);
+        if (pin) (void)hipHostFree(pin);
+        pin = nullptr;
+        pinCap = 0;
         if (evWords) (void)hipEventDestroy(evWords);
         evWords = nullptr;
     }
@@ -514,7 +522,7 @@ void scanExclusive(Ctx* c, const TIn* in, TOut* out, int64_t n) {
 void fillU32(Ctx* c, uint32_t* a, int64_t n) {
     if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n)), dim3(WDM_BLOCK), 0, c->stream, (int32_t*)a, n, 0);
 }
-// the second half of a bucket sort: items scattered into btmp / btmpB (n of them, or at
+// the second half of a bucket sort: items scattered into btmp (n of them, or at
 // most n with *total of them on the device; off the exclusive prefix of the bucket counts)
 // placed in order through the writer w
 #define BKT_GRID 128
@@ -522,32 +530,65 @@ template <typename OffT, class W>
 void bucketFinish(Ctx* c, int64_t n, const OffT* off, const W& w, const OffT* total = nullptr) {
     Shared& X = *c->sh;
     if (n <= 0) return;
-    ensure(c, X.longList, (size_t)(n / (BKT_SMALL + 1) + 1)); ensure(c, X.longCnt, 1); ensure(c, X.baux, (size_t)n);
+    ensure(c, X.longList, (size_t)(n / (BKT_SMALL + 1) + 1)); ensure(c, X.longCnt, 1); ensure(c, X.baux, 2 * (size_t)n);
     fillU32(c, X.longCnt.p, 1);
-    hipLaunchKernelGGL((k_bucket_rank<OffT, W>), dim3(nblk(n)), dim3(WDM_BLOCK), 0, c->stream, (const uint64_t*)X.btmp.p,
-                       (const uint32_t*)X.btmpB.p, off, n, total, X.longList.p, X.longCnt.p, w);
-    hipLaunchKernelGGL((k_bucket_long<OffT, W>), dim3(BKT_GRID), dim3(BKT_THREADS), 0, c->stream, X.btmp.p, X.baux.p, off,
+    hipLaunchKernelGGL((k_bucket_rank<OffT, W>), dim3(nblk(n)), dim3(WDM_BLOCK), 0, c->stream, (const BItem*)X.btmp.p, off, n, total,
+                       X.longList.p, X.longCnt.p, w);
+    hipLaunchKernelGGL((k_bucket_long<OffT, W>), dim3(BKT_GRID), dim3(BKT_THREADS), 0, c->stream, (const BItem*)X.btmp.p, X.baux.p, n, off,
                        (const uint32_t*)X.longList.p, (const uint32_t*)X.longCnt.p, w);
     checkLaunch();
 }
-__global__ void k_gather_i64(const int64_t* src, const int64_t* idx, int n, int64_t* out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = src[idx[i]];
-}
 
-// values of a batch-wide exclusive prefix at the first entry of each scene's segment (and the total)
-std::vector<int64_t> segmentStarts(Ctx* c, const int64_t* scan, const std::vector<int64_t>& firstEntry) {
+// pinned host memory for readbacks (grows; the stream is drained before it is replaced)
+void* pinnedRead(Ctx* c, size_t bytes) {
     Shared& X = *c->sh;
-    int n = (int)firstEntry.size();
-    const int64_t* idx = (const int64_t*)X.stIdx.upload(c, firstEntry.data(), n * sizeof(int64_t));
-    ensure(c, X.starts, n);
-    hipLaunchKernelGGL(k_gather_i64, dim3((n + 63) / 64), dim3(64), 0, c->stream, scan, idx, n, X.starts.p);
+    if (bytes > X.pinCap) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (X.pin) (void)hipHostFree(X.pin);
+        X.pin = nullptr;
+        X.pinCap = 0;
+        HIPCHK(hipHostMalloc(&X.pin, std::max<size_t>(bytes, 4096), hipHostMallocDefault));
+        X.pinCap = std::max<size_t>(bytes, 4096);
+    }
+    return X.pin;
+}
+// values of a batch-wide exclusive prefix at each scene's first entry (FR[t].rowBase for
+// field 0, FR[t].oldStart for field 1) and at `last`
+__global__ void k_seg_starts(const FRScene* FR, int nT, int field, int64_t last, const int64_t* scan, int64_t* out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > nT) return;
+    const int64_t idx = t == nT ? last : (field == 0 ? FR[t].rowBase : FR[t].oldStart);
+    out[t] = scan[idx];
+}
+// the same read back (one synchronisation); `extraBytes` of device memory at `extra` are read
+// back with them, into *extraOut
+std::vector<int64_t> segmentStarts(Ctx* c, const int64_t* scan, const FRScene* FRd, int nT, int field, int64_t last,
+                                   const void* extra = nullptr, void* extraOut = nullptr, size_t extraBytes = 0) {
+    Shared& X = *c->sh;
+    ensure(c, X.starts, (size_t)nT + 1);
+    hipLaunchKernelGGL(k_seg_starts, dim3((nT + 64) / 64), dim3(64), 0, c->stream, FRd, nT, field, last, scan, X.starts.p);
     checkLaunch();
-    std::vector<int64_t> out(n);
-    HIPCHK(hipMemcpyAsync(out.data(), X.starts.p, n * sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
+    const size_t sb = ((size_t)nT + 1) * sizeof(int64_t);
+    uint8_t* h = (uint8_t*)pinnedRead(c, sb + extraBytes);
+    HIPCHK(hipMemcpyAsync(h, X.starts.p, sb, hipMemcpyDeviceToHost, c->stream));
+    if (extraBytes) HIPCHK(hipMemcpyAsync(h + sb, extra, extraBytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<int64_t> out(nT + 1);
+    std::memcpy(out.data(), h, sb);
+    if (extraBytes) std::memcpy(extraOut, h + sb, extraBytes);
     return out;
 }
+// several host tables in one staged upload
+struct Upload {
+    Blob b;
+    uint8_t* dev = nullptr;
+    template <typename T>
+    size_t add(const std::vector<T>& v) { return b.addv(v); }
+    void send(Ctx* c, Staged& s) { dev = s.upload(c, b.b.data(), b.b.size()); }
+    template <typename T>
+    const T* at(size_t off) const { return (const T*)(dev + off); }
+    Map map(size_t off, int n) const { Map m; m.bprefix = at<int64_t>(off); m.n = n; return m; }
+};
 
 Map uploadMap(Ctx* c, Staged& s, const std::vector<int64_t>& bprefix) {
     Map m;
@@ -896,6 +937,11 @@ struct InsertionSortCollider {
         for (int d = 0; d < 3; d++) V *= S.periodic ? S.size[d] : (e[3 + d] - e[d]) + W;
         V = std::max(V, 1e-300);
         const double VISIT = 1.5;  // a visited cell, in box tests
+        static const std::vector<double> edge = [] {  // bin k holds extents in [edge[k], edge[k + 1])
+            std::vector<double> v(EXT_BINS + 1);
+            for (int k = 0; k <= EXT_BINS; k++) v[k] = std::pow(2., (k - 128) / 4.);
+            return v;
+        }();
         int best = 0;
         double bestCost = 0;
         const int lo = levels > 0 ? std::min(levels, MAXLVL) : 1, hi = levels > 0 ? std::min(levels, MAXLVL) : MAXLVL;
@@ -911,11 +957,11 @@ struct InsertionSortCollider {
             if (cells > maxCells) continue;
             for (int k = 0; k < EXT_BINS; k++) {
                 if (!hs[k]) continue;
-                const double bl = std::pow(2., (k - 128) / 4.), bh = std::pow(2., (k - 127) / 4.);  // the bin's extents
+                const double bl = edge[k], bh = edge[k + 1];  // the bin's extents
                 if (bl > W / 1.001) continue;  // large: listed apart
                 int l = 0;
                 while (l < nL - 1 && bh > w[l] / 1.001) l++;
-                cnt[l] += hs[k];
+                cnt[l] += (double)hs[k] * EXT_SAMPLE;
             }
             double cost = 0;
             for (int l = 0; l < nL; l++) {
@@ -950,7 +996,7 @@ struct InsertionSortCollider {
     // bound entries of all of them sorted by (scene, cell key, id) with each cell's entry
     // range [cbeg[k], cbeg[k + 1]) (levels: 0 lets each scene choose).
     static void buildGrids(Batch& B, const std::vector<int>& L, std::vector<FRScene>& F, const std::vector<int64_t>& bpB, const Map& mB,
-                           const FRScene*& FRd, int64_t& nbTot, double cellFactor, int levels) {
+                           const FRScene*& FRd, int64_t& nbTot, double cellFactor, int levels, bool entryCopies = true) {
         Ctx* c = B.c;
         Shared& X = *c->sh;
         hipStream_t st = c->stream;
@@ -1015,7 +1061,7 @@ struct InsertionSortCollider {
         const int64_t G = cellBase;
         const size_t nb1 = (size_t)std::max<int64_t>(nbTot, 1);
         ensure(c, X.ekey, nb1); ensure(c, X.eslot, nb1); ensure(c, X.ckeys2, nb1); ensure(c, X.cvals2, nb1); ensure(c, X.cbox, nb1);
-        ensure(c, X.btmp, nb1); ensure(c, X.btmpB, nb1);
+        ensure(c, X.btmp, nb1);
         ensure(c, X.ccnt, (size_t)G); ensure(c, X.cbeg, (size_t)G + 1);
         fillU32(c, X.ccnt.p, G);
         if (nblkB > 0) {
@@ -1025,10 +1071,12 @@ struct InsertionSortCollider {
         scanExclusive(c, (const uint32_t*)X.ccnt.p, X.cbeg.p, G);
         if (nblkB > 0) {
             hipLaunchKernelGGL(k_cell_scatter, dim3((unsigned)nblkB), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ekey.p,
-                               (const uint32_t*)X.eslot.p, (const int32_t*)X.cbeg.p, X.btmp.p, X.btmpB.p);
+                               (const uint32_t*)X.eslot.p, (const int32_t*)X.cbeg.p, X.btmp.p);
             checkLaunch();
         }
-        bucketFinish(c, nbTot, (const int32_t*)X.cbeg.p, CellOut{B.D, FRd, nT, X.cvals2.p, X.ckeys2.p, X.cbox.p});
+        if (entryCopies) { ensure(c, X.eb0, nb1); ensure(c, X.eb1, nb1); ensure(c, X.einfo, nb1); }
+        bucketFinish(c, nbTot, (const int32_t*)X.cbeg.p,
+                     CellOut{B.D, FRd, nT, X.cvals2.p, X.ckeys2.p, X.cbox.p, entryCopies ? X.eb0.p : nullptr, X.eb1.p, X.einfo.p});
     }
 
     // list every large bound (facets wider than the top level's cells) in the cells it
@@ -1088,10 +1136,13 @@ struct InsertionSortCollider {
             nRows += (int64_t)S_(t).np + 1;
         }
         if (nRows >= INT32_MAX) throw WdmError(WDM_E_ARG, "collider batch too large (2^31 particles)");
-        const FRScene* FRd = (const FRScene*)X.stFR.upload(c, F.data(), F.size() * sizeof(FRScene));
         std::vector<int64_t> bpP = blockPrefix(nT, [&](int t) { return S_(t).np; });
         std::vector<int64_t> bpB = blockPrefix(nT, [&](int t) { return (int64_t)S_(t).binnedH.size(); });
-        Map mP = uploadMap(c, X.stMapP, bpP), mB = uploadMap(c, X.stMapB, bpB);
+        Upload uA;
+        const size_t oFA = uA.add(F), oP = uA.add(bpP), oB = uA.add(bpB);
+        uA.send(c, X.stRunA);
+        const FRScene* FRd = uA.at<FRScene>(oFA);
+        const Map mP = uA.map(oP, nT), mB = uA.map(oB, nT);
         if (bpP[nT] > 0) {
             KTimer tm(c, "bounds");
             hipLaunchKernelGGL(k_bounds, dim3((unsigned)bpP[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mP, 0);
@@ -1117,9 +1168,6 @@ struct InsertionSortCollider {
             ensure(c, X.wcnt, (size_t)std::max<int64_t>(nW, 1)); ensure(c, X.scnt, (size_t)std::max<int64_t>(nW, 1));
             ensure(c, X.wmax, 2);
             ensure(c, X.rowCnt, (size_t)nRows); ensure(c, X.rowOff, (size_t)nRows + 1);
-            std::vector<int64_t> first(nT + 1);
-            for (int t = 0; t < nT; t++) first[t] = F[t].rowBase;
-            first[nT] = nRows;
             std::vector<int64_t> cs;
             auto grow = [](int32_t wm) { return (int32_t)std::min<int64_t>(((int64_t)wm + wm / 8 + 63) / 64 * 64, INT32_MAX / 2); };
             for (int attempt = 0;; attempt++) {
@@ -1136,15 +1184,15 @@ struct InsertionSortCollider {
                     hipLaunchKernelGGL(k_pair_scan, grid, dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
                                        (const int32_t*)X.cbeg.p, (const float4*)X.cbox.p, X.scnt.p, X.wmax.p, scap, X.sPair.p);
                     hipLaunchKernelGGL(k_pair_exact, grid, dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const int32_t*)X.cvals2.p,
-                                       (const uint32_t*)X.ckeys2.p, large ? (const int32_t*)X.lbeg.p : nullptr, (const int32_t*)X.lvals.p,
+                                       (const uint32_t*)X.ckeys2.p, (const double4*)X.eb0.p, (const double4*)X.eb1.p, (const int2*)X.einfo.p,
+                                       large ? (const int32_t*)X.lbeg.p : nullptr, (const int32_t*)X.lvals.p,
                                        (const int32_t*)X.scnt.p, scap, (const int2*)X.sPair.p, X.wcnt.p, X.wmax.p + 1, wcap, X.rItem.p,
                                        X.rRow.p, X.rSlot.p, X.rowCnt.p);
                     checkLaunch();
                 }
                 scanExclusive(c, (const uint32_t*)X.rowCnt.p, X.rowOff.p, nRows);
                 int32_t wm[2] = {0, 0};
-                HIPCHK(hipMemcpyAsync(wm, X.wmax.p, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-                cs = segmentStarts(c, X.rowOff.p, first);  // synchronises (wm is read too)
+                cs = segmentStarts(c, X.rowOff.p, FRd, nT, 0, nRows, X.wmax.p, wm, sizeof(wm));  // synchronises
                 if (wm[0] <= scap && wm[1] <= wcap) break;
                 if (attempt > 1) throw WdmError(WDM_E_STATE, "collider: pairs per wavefront changed between passes");
                 if (wm[0] > scap) c->scanCap = grow(wm[0]);
@@ -1153,12 +1201,12 @@ struct InsertionSortCollider {
             for (int t = 0; t < nT; t++) { F[t].candStart = cs[t]; F[t].nCand = cs[t + 1] - cs[t]; }
             ncandTot = cs[nT];
             const size_t nc1 = (size_t)std::max<int64_t>(ncandTot, 1);
-            ensure(c, X.candK, nc1); ensure(c, X.candP, nc1); ensure(c, X.btmp, nc1); ensure(c, X.btmpB, nc1);
+            ensure(c, X.candK, nc1); ensure(c, X.candP, nc1); ensure(c, X.btmp, nc1);
             if (ncandTot > 0) {
                 KTimer tm(c, "pairs_sort");
                 hipLaunchKernelGGL(k_cand_scatter, dim3((unsigned)((nW + 3) / 4)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.wcnt.p, nW,
                                    c->waveCap, (const uint64_t*)X.rItem.p, (const uint32_t*)X.rRow.p, (const uint32_t*)X.rSlot.p,
-                                   (const int64_t*)X.rowOff.p, X.btmp.p, X.btmpB.p);
+                                   (const int64_t*)X.rowOff.p, X.btmp.p);
                 checkLaunch();
                 bucketFinish(c, ncandTot, (const int64_t*)X.rowOff.p, CandOut{FRd, nT, X.candK.p, X.candP.p});
             }
@@ -1179,12 +1227,15 @@ struct InsertionSortCollider {
             F[t].nOld = S.nc;
             nOldTot += S.nc;
         }
-        FRd = (const FRScene*)X.stFR.upload(c, F.data(), F.size() * sizeof(FRScene));
-        const ConDesc<T>* Cd = (const ConDesc<T>*)X.stConFR.upload(c, C.data(), C.size() * sizeof(ConDesc<T>));
         std::vector<int64_t> bpO = blockPrefix(nT, [&](int t) { return F[t].nOld; });
-        Map mO = uploadMap(c, X.stMapO, bpO);
         std::vector<int64_t> bp1 = blockPrefix(nT, [&](int t) { return (int64_t)S_(t).np + 1; });
-        Map m1 = uploadMap(c, X.stMapP1, bp1);  // per scene its rows (np + 1 threads)
+        Upload uO;
+        const size_t oFO = uO.add(F), oCO = uO.add(C), oO = uO.add(bpO), o1 = uO.add(bp1);
+        uO.send(c, X.stRunO);
+        FRd = uO.at<FRScene>(oFO);
+        const ConDesc<T>* Cd = uO.at<ConDesc<T>>(oCO);
+        const Map mO = uO.map(oO, nT);
+        const Map m1 = uO.map(o1, nT);  // per scene its rows (np + 1 threads)
         ensure(c, X.ooFlag, (size_t)std::max<int64_t>(nOldTot, 1));
         ensure(c, X.ooRank, (size_t)nOldTot + 1);
         ensure(c, X.oldRow, (size_t)nRows + 1);
@@ -1200,10 +1251,7 @@ struct InsertionSortCollider {
             }
             scanExclusive(c, (const int32_t*)X.ooFlag.p, X.ooRank.p, nOldTot);
             if (nOldTot > 0) {
-                std::vector<int64_t> first(nT + 1);
-                for (int t = 0; t < nT; t++) first[t] = F[t].oldStart;
-                first[nT] = nOldTot;
-                std::vector<int64_t> os = segmentStarts(c, X.ooRank.p, first);
+                std::vector<int64_t> os = segmentStarts(c, X.ooRank.p, FRd, nT, 1, nOldTot);
                 for (int t = 0; t < nT; t++) noor[t] = os[t + 1] - os[t];
             }
         }
@@ -1232,14 +1280,18 @@ struct InsertionSortCollider {
             F[t].entV = gpc(S.entV.p);
             nNewTot += nnew;
         }
-        FRd = (const FRScene*)X.stFR.upload(c, F.data(), F.size() * sizeof(FRScene));
-        Cd = (const ConDesc<T>*)X.stConFR.upload(c, C.data(), C.size() * sizeof(ConDesc<T>));
-        ensure(c, X.oslot, std::max<int64_t>(nNewTot, 1)); ensure(c, X.rflag, std::max<int64_t>(nNewTot, 1));
+        ensure(c, X.oslot, std::max<int64_t>(nNewTot, 1)); ensure(c, X.srcRow, std::max<int64_t>(nNewTot, 1));
+        ensure(c, X.rflag, std::max<int64_t>(nNewTot, 1));
         ensure(c, X.rscan, (size_t)nNewTot + 1);
         ensure(c, X.nrflag, std::max<int64_t>(nNewTot, 1)); ensure(c, X.nrscan, (size_t)nNewTot + 1);
         std::vector<int64_t> bpC = blockPrefix(nT, [&](int t) { return F[t].nCand; });
         std::vector<int64_t> bpN = blockPrefix(nT, [&](int t) { return F[t].nNew; });
-        Map mC = uploadMap(c, X.stMapC, bpC), mN = uploadMap(c, X.stMapN, bpN);
+        Upload uN;
+        const size_t oFN = uN.add(F), oCN = uN.add(C), oC = uN.add(bpC), oN = uN.add(bpN);
+        uN.send(c, X.stRunN);
+        FRd = uN.at<FRScene>(oFN);
+        Cd = uN.at<ConDesc<T>>(oCN);
+        const Map mC = uN.map(oC, nT), mN = uN.map(oN, nT);
         {
             KTimer tm(c, "merge");
             if (ncandTot > 0)
@@ -1252,7 +1304,10 @@ struct InsertionSortCollider {
             scanExclusive(c, (const int64_t*)X.rflag.p, X.rscan.p, nNewTot);
             if (nNewTot > 0)
                 hipLaunchKernelGGL(k_copy_state<T>, dim3((unsigned)bpN[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mN, (const int32_t*)X.oslot.p,
-                                   (const int64_t*)X.rscan.p, X.nrflag.p);
+                                   (const int64_t*)X.rscan.p, X.nrflag.p, X.srcRow.p);
+            if (nNewTot > 0)
+                hipLaunchKernelGGL(k_copy_rows<T>, dim3((unsigned)bpN[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mN, (const int64_t*)X.rscan.p,
+                                   (const int32_t*)X.srcRow.p);
             checkLaunch();
             // ContactLoop work list: rows first, then the other non-dormant contacts
             scanExclusive(c, (const int64_t*)X.nrflag.p, X.nrscan.p, nNewTot);
@@ -1276,7 +1331,7 @@ struct InsertionSortCollider {
             KTimer tm(c, "csr");
             const int64_t ne = 2 * nNewTot;
             const size_t ne1 = (size_t)std::max<int64_t>(ne, 1);
-            ensure(c, X.eB, ne1); ensure(c, X.eS, ne1); ensure(c, X.eI, ne1); ensure(c, X.btmp, ne1); ensure(c, X.btmpB, ne1);
+            ensure(c, X.eB, ne1); ensure(c, X.eS, ne1); ensure(c, X.eI, ne1); ensure(c, X.btmp, ne1);
             ensure(c, X.csrCnt, (size_t)nRows); ensure(c, X.csrScan, (size_t)nRows + 1);
             fillU32(c, X.csrCnt.p, nRows);
             if (nNewTot > 0) {
@@ -1287,7 +1342,7 @@ struct InsertionSortCollider {
             scanExclusive(c, (const uint32_t*)X.csrCnt.p, X.csrScan.p, nRows);
             if (ne > 0) {
                 hipLaunchKernelGGL(k_csr_scatter, dim3(nblk(ne)), dim3(WDM_BLOCK), 0, st, (const uint32_t*)X.eB.p, (const uint32_t*)X.eS.p,
-                                   (const uint64_t*)X.eI.p, ne, (const int64_t*)X.csrScan.p, X.btmp.p, X.btmpB.p);
+                                   (const uint64_t*)X.eI.p, ne, (const int64_t*)X.csrScan.p, X.btmp.p);
                 checkLaunch();
                 bucketFinish(c, ne, (const int64_t*)X.csrScan.p, CsrOut{FRd, nT, (const int64_t*)X.csrScan.p},
                              (const int64_t*)X.csrScan.p + nRows);
@@ -2056,7 +2111,7 @@ void relax(Scene& S, int iters) {
         hipLaunchKernelGGL(k_bounds, dim3((unsigned)bpP[1]), dim3(WDM_BLOCK), 0, c->stream, B.D, FRd, mP, 1);
         checkLaunch();
         int64_t nbTot = 0;
-        InsertionSortCollider::buildGrids(B, L, F, bpB, mB, FRd, nbTot, 1., 1);
+        InsertionSortCollider::buildGrids(B, L, F, bpB, mB, FRd, nbTot, 1., 1, false);
         hipLaunchKernelGGL(k_relax, dim3((unsigned)bpB[1]), dim3(WDM_BLOCK), 0, c->stream, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
                            (const int32_t*)X.cvals2.p, (const int32_t*)X.cbeg.p, X.disp.p);
         hipLaunchKernelGGL(k_relax_apply, dim3((unsigned)bpB[1]), dim3(WDM_BLOCK), 0, c->stream, B.D, FRd, mB, (const double4*)X.disp.p);
