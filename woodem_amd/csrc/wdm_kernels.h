@@ -1384,8 +1384,10 @@ __device__ __forceinline__ bool stencilCellL(const FRScene& F, int l, int cx, in
     // periodic axes with fewer than 3 cells hold a single cell (no neighbours); aperiodic axes always scan +-1
     if ((dx && per && nx < 3) || (dy && per && ny < 3) || (dz && per && nz < 3)) return false;
     int z = cz + dz, y = cy + dy, x = cx + dx;
-    if (per) {
-        z = (z + nz) % nz; y = (y + ny) % ny; x = (x + nx) % nx;
+    if (per) {  // one step across the periodic boundary at most (|d| <= 1, n >= 3)
+        x = x < 0 ? x + nx : (x >= nx ? x - nx : x);
+        y = y < 0 ? y + ny : (y >= ny ? y - ny : y);
+        z = z < 0 ? z + nz : (z >= nz ? z - nz : z);
     } else if (z < 0 || z >= nz || y < 0 || y >= ny || x < 0 || x >= nx) return false;
     cell = F.lbase[l] + (z * ny + y) * nx + x;
     return true;
@@ -1445,17 +1447,22 @@ __global__ void __launch_bounds__(WDM_BLOCK, 6) k_pair_scan(const SceneDesc* D, 
     // cell, entries after it), then one phase per finer level scanning a box of cells
     const int nPh = large ? (F.largePairs ? 1 : 0) : 14 + lvl;
     int phase = 1;                               // phase 0 (the own cell) starts below
-    int rl = 0, ri = 0, rn = 0;                  // box scan: level, next cell, cells
-    int r0x = 0, r0y = 0, r0z = 0, rnx = 1, rny = 1;  // box origin (cells) and sizes
+    int rl = 0;                                  // box scan: level,
+    int r0x = 0, r0y = 0, r0z = 0, rnx = 1, rny = 1, rnz = 0;  // origin (cells, wrapped) and sizes,
+    int ix = 0, iy = 0, iz = 0;                  // next cell within the box
     // the next cell of this lane's list (batch-wide index), -1 past its end
     auto nextCell = [&]() -> int64_t {
         for (;;) {
-            if (ri < rn) {
-                const int ix = ri % rnx, iy = (ri / rnx) % rny, iz = ri / (rnx * rny);
-                ri++;
+            if (iz < rnz) {
                 const int nx = F.ln[rl][0], ny = F.ln[rl][1], nz = F.ln[rl][2];
-                int x = r0x + ix, y = r0y + iy, z = r0z + iz;
-                if (g.periodic) { x = wrapCell(x, nx); y = wrapCell(y, ny); z = wrapCell(z, nz); }
+                int x = r0x + ix, y = r0y + iy, z = r0z + iz;  // below 2 n: the origin is wrapped, the sizes at most n
+                x -= x >= nx ? nx : 0;
+                y -= y >= ny ? ny : 0;
+                z -= z >= nz ? nz : 0;
+                if (++ix == rnx) {
+                    ix = 0;
+                    if (++iy == rny) { iy = 0; ++iz; }
+                }
                 return cb + F.lbase[rl] + ((int64_t)z * ny + y) * nx + x;
             }
             if (phase >= nPh) return -1;
@@ -1477,7 +1484,10 @@ __global__ void __launch_bounds__(WDM_BLOCK, 6) k_pair_scan(const SceneDesc* D, 
                     if (g.periodic) {
                         a = (int)floor(lo * g.invL[d] * n);
                         b = (int)floor(hi * g.invL[d] * n);
-                        if (b - a + 1 > n) { a = 0; b = n - 1; }
+                        if (b - a + 1 > n || b < a) { a = 0; b = n - 1; }
+                        const int a0 = a;
+                        a = wrapCell(a, n);
+                        b = a + (b - a0);
                     } else {
                         a = (int)floor(lo / F.lw[rl][d]);
                         b = (int)floor(hi / F.lw[rl][d]);
@@ -1488,9 +1498,9 @@ __global__ void __launch_bounds__(WDM_BLOCK, 6) k_pair_scan(const SceneDesc* D, 
                     n3[d] = b - a + 1;
                 }
                 r0x = o3[0]; r0y = o3[1]; r0z = o3[2];
-                rnx = n3[0]; rny = n3[1];
-                rn = n3[0] * n3[1] * n3[2];
-                ri = 0;
+                rnx = n3[0]; rny = n3[1]; rnz = n3[2];
+                if (rnx <= 0 || rny <= 0) rnz = 0;  // an empty box (non-finite bounds)
+                ix = iy = iz = 0;
             }
         }
     };

@@ -142,7 +142,8 @@ void devFree(Ctx* c, void* p, size_t bytes) {
 template <typename T>
 void ensure(Ctx* c, DBuf<T>& b, size_t n, bool keep = false) {
     if (b.n >= n && b.p) return;
-    size_t nn = std::max<size_t>(n, b.n + b.n / 2);
+    // headroom: sizes that creep up between collider passes reallocate rarely
+    size_t nn = std::max<size_t>(n + n / 8, b.n + b.n / 2);
     nn = std::max<size_t>(nn, 16);
     T* p = (T*)devAlloc(c, nn * sizeof(T));
     if (keep && b.p && b.n) HIPCHK(hipMemcpyAsync(p, b.p, b.n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
