@@ -562,6 +562,11 @@ def bench_periodic(args, ctx):
                  "full_collider_runs": rm.stats()["nFullRuns"] - f0, "speedup_vs_fp64": mval / sus["value"],
                  "contacts_avg_ms": msm / max(lam, 1), "contacts_real": int(cm["isReal"].sum()),
                  "contacts_processed": rm.ctx.contact_work(rm.sid)[0]}
+        m_bytes = (BYTES_REAL["mixed"] * mixed["contacts_real"]
+                   + BYTES_POT["mixed"] * max(mixed["contacts_processed"] - mixed["contacts_real"], 0))
+        mixed["contacts_roofline"] = {"bound": "hbm", "algorithmic_bytes_per_launch": m_bytes,
+                                      "achieved": m_bytes / (mixed["contacts_avg_ms"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+        mixed["contacts_roofline"]["frac"] = mixed["contacts_roofline"]["achieved"] / HBM_PEAK_GBS
         rm.close()
         from woodem_amd.precision import mixed_error_report
         mixed["errors_vs_fp64"] = mixed_error_report(ctx)

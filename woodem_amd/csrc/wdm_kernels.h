@@ -126,14 +126,14 @@ struct ConView {
     GP T* st2;         // state of the rows from *counter on (items that were not touching at the full run)
     GP int32_t* cd;
     GP T* st;
-    GP double4* cpt;   // per state row: contact point and uN of the last ContactLoop (x, y, z, uN)
+    GP double4* cpt;   // per state row: contact point and uN of the last ContactLoop (x, y, z, uN; mixed: mixedCpt)
     GP int32_t* itm;   // per contact: its item in the ContactLoop's work list, -1 when dormant
     // per item, so that the ContactLoop reads them contiguously: (idA, idB, info, cellDist)
     // and (state row, contact); info and the row are written to both places
     GP int4* irec;
     GP int2* iaux;
     int32_t* nItems;   // items in the work list
-    GP double4* gfc;   // per item: gather record
+    GP double4* gfc;   // per item: gather record (mixed: mixedRecA/B/C)
     int64_t cap;
 };
 
@@ -206,7 +206,19 @@ struct GatherView {
     const GP uint32_t* ent;
     const GP double4* gfc;
     const GP uint64_t* realMask;
+    int64_t gcap;  // items the gather record buffer holds
+    int mixed;     // FP32 gather records (mixed precision)
 };
+// Mixed precision keeps the FP64 records' buffers (contact point: one double4 per state row;
+// gather record: two double4 per item) but fills them in FP32, in fewer bytes:
+//   contact point: float4 (contact point - position of B in A's image, uN) per row;
+//   gather record: 36 bytes per item in three arrays over the buffer: float4 (global force,
+//     branch of A x), float4 (branch of A y z, branch of B x y), float (branch of B z),
+//     a branch being the contact point minus the node position (B's in A's image).
+__host__ __device__ inline GP float4* mixedCpt(GP double4* cpt) { return (GP float4*)cpt; }
+__host__ __device__ inline GP float4* mixedRecA(GP double4* gfc) { return (GP float4*)gfc; }
+__host__ __device__ inline GP float4* mixedRecB(GP double4* gfc, int64_t cap) { return (GP float4*)gfc + cap; }
+__host__ __device__ inline GP float* mixedRecC(GP double4* gfc, int64_t cap) { return (GP float*)((GP float4*)gfc + 2 * cap); }
 
 // One scene of a batch.  Kernels read their views from a table of these, so one launch
 // covers many scenes; blocks are mapped to scenes by a prefix table (mapBlock).
@@ -365,6 +377,32 @@ __device__ __forceinline__ void gatherForces(const StepArgs& a, const GatherView
         for (int u = 0; u < GATHER_CHUNK; u++) {
             const uint32_t c = v[u] & 0x7fffffffu;
             real[u] = eb + u < e1 && ((gv.realMask[c >> 6] >> (c & 63)) & 1ull);  // non-real: no force (ContactLoop.cpp:188)
+        }
+        if (gv.mixed) {  // FP32 records: force and this node's branch
+            const GP float4* RA = (const GP float4*)gv.gfc;
+            const GP float4* RB = RA + gv.gcap;
+            const GP float* RC = (const GP float*)(RA + 2 * gv.gcap);
+            float4 ra[GATHER_CHUNK], rb[GATHER_CHUNK];
+            float rc[GATHER_CHUNK];
+#pragma unroll
+            for (int u = 0; u < GATHER_CHUNK; u++)
+                if (real[u]) {
+                    const size_t c = v[u] & 0x7fffffffu;
+                    ra[u] = RA[c];
+                    rb[u] = RB[c];
+                    rc[u] = (v[u] >> 31) ? RC[c] : 0.f;
+                }
+#pragma unroll
+            for (int u = 0; u < GATHER_CHUNK; u++) {
+                if (!real[u]) continue;
+                const bool isB = (v[u] >> 31) != 0;
+                const double sign = isB ? -1. : 1.;
+                const D3 Fs = mk3((double)ra[u].x * sign, (double)ra[u].y * sign, (double)ra[u].z * sign);
+                const D3 xc = isB ? mk3((double)rb[u].z, (double)rb[u].w, (double)rc[u]) : mk3((double)ra[u].w, (double)rb[u].x, (double)rb[u].y);
+                F = F + Fs;
+                T = T + cross3(xc, Fs);
+            }
+            continue;
         }
 #pragma unroll
         for (int u = 0; u < GATHER_CHUNK; u++)
@@ -1905,7 +1943,8 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_copy_rows(const ConDesc<T>* C, co
     if (os < 0) return;
     const int64_t nROld = *oldc.counter;
     for (int k = 0; k < NSTATE; k++) nc.st[(size_t)k * nc.cap + ns] = *stAt(oldc, os, k, nROld);
-    nc.cpt[ns] = oldc.cpt[os];
+    if constexpr (sizeof(T) == 8) nc.cpt[ns] = oldc.cpt[os];
+    else mixedCpt(nc.cpt)[ns] = mixedCpt(oldc.cpt)[os];
 }
 #define NR_ALL(v) ((v) & 0xffffffffLL)
 #define NR_NEAR(v) ((v) >> 32)
@@ -2232,9 +2271,16 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
             gangVel = mk3<T>(0, 0, 0);
         } else {
             const V3T<T> currNormal = G.normal, prevNormal = trCol(tr, 0);
-            const D3 prevContPt = xyz(cv.cpt[row]);
             D3 shiftVel2 = mk3(0., 0., 0.);
             if (a.periodic) shiftVel2 = mv3(ldM3(a.cell->shiftVelM), mk3((double)unpack10(cdp, 0), (double)unpack10(cdp, 10), (double)unpack10(cdp, 20)));
+            D3 prevContPt;
+            if constexpr (sizeof(T) == 8) prevContPt = xyz(cv.cpt[row]);
+            else {
+                // B's position at the last ContactLoop (Leapfrog moved it by its velocity times
+                // dt since) plus the branch stored then
+                const float4 r = mixedCpt(cv.cpt)[row];
+                prevContPt = (G.pos2 - (G.vel2 + shiftVel2) * dt) + mk3((double)r.x, (double)r.y, (double)r.z);
+            }
             V3T<T> normRotVec = cross3(prevNormal, currNormal);
             V3T<T> midNormal = (prevNormal + currNormal) * (T).5;
             if (wsqrt(sqn3(midNormal)) == 0) midNormal = currNormal;
@@ -2413,9 +2459,17 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
         // global force trsf*F (Contact::getForceTorqueBranch, Contact.cpp:24-34)
         V3T<T> fl = mk3<T>(F[0], F[1], F[2]);
         V3T<T> fg = mk3<T>(dot3(mk3<T>(tr[0], tr[1], tr[2]), fl), dot3(mk3<T>(tr[3], tr[4], tr[5]), fl), dot3(mk3<T>(tr[6], tr[7], tr[8]), fl));
-        cv.gfc[2 * item] = make_double4((double)fg.x, (double)fg.y, (double)fg.z, (double)cdp);
-        cv.gfc[2 * item + 1] = make_double4(G.contPt.x, G.contPt.y, G.contPt.z, (double)uN);
-        cv.cpt[row] = make_double4(G.contPt.x, G.contPt.y, G.contPt.z, (double)uN);
+        if constexpr (sizeof(T) == 8) {
+            cv.gfc[2 * item] = make_double4((double)fg.x, (double)fg.y, (double)fg.z, (double)cdp);
+            cv.gfc[2 * item + 1] = make_double4(G.contPt.x, G.contPt.y, G.contPt.z, (double)uN);
+            cv.cpt[row] = make_double4(G.contPt.x, G.contPt.y, G.contPt.z, (double)uN);
+        } else {
+            const D3 bA = G.contPt - G.pos1, bB = G.contPt - G.pos2;
+            mixedRecA(cv.gfc)[item] = make_float4(fg.x, fg.y, fg.z, (float)bA.x);
+            mixedRecB(cv.gfc, cv.cap)[item] = make_float4((float)bA.y, (float)bA.z, (float)bB.x, (float)bB.y);
+            mixedRecC(cv.gfc, cv.cap)[item] = (float)bB.z;
+            mixedCpt(cv.cpt)[row] = make_float4((float)bB.x, (float)bB.y, (float)bB.z, (float)uN);
+        }
         realOut = true;
     } while (0);
     // one bit per item, one 64-bit word per wavefront: the gather skips non-real contacts
@@ -2423,6 +2477,33 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
     int64_t w0 = lb * WDM_BLOCK + (threadIdx.x & ~63);
     if ((threadIdx.x & 63) == 0 && w0 < nItems) realMask[w0 >> 6] = bal;
     if (a.trackEnergy) blockSumWrite<5>(ev, bits, epart, nullptr, 0u, creOut);
+}
+
+// Mixed precision: the contact points of a list's live real contacts between their stored
+// form (branch of B in A's image, FP32) and absolute FP64 points (DIR 0: to abs[row], DIR 1:
+// from abs[row]), with B's position and the cell's hSize (hs; between steps both as at the
+// last ContactLoop)
+template <int DIR>
+__global__ void __launch_bounds__(WDM_BLOCK) k_cpt_mixed(const double4* posr, ConView<float> cv, int64_t n, int periodic, M3 hs,
+                                                        double4* abs) {
+    const int64_t i = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t info = cv.info[i];
+    const int32_t row = cv.slot[i];
+    if (!(info & CI_REAL) || (info & CI_DEAD) || row < 0 || row >= cv.cap) return;
+    const int2 id = cv.ids[i];
+    D3 p2 = xyz(posr[id.y]);
+    if (periodic) {
+        const int32_t cdp = cv.cd[i];
+        p2 = p2 + mv3(hs, mk3((double)unpack10(cdp, 0), (double)unpack10(cdp, 10), (double)unpack10(cdp, 20)));
+    }
+    if (DIR == 0) {
+        const float4 r = mixedCpt(cv.cpt)[row];
+        abs[row] = make_double4(p2.x + (double)r.x, p2.y + (double)r.y, p2.z + (double)r.z, (double)r.w);
+    } else {
+        const double4 q = abs[row];
+        mixedCpt(cv.cpt)[row] = make_float4((float)(q.x - p2.x), (float)(q.y - p2.y), (float)(q.z - p2.z), (float)q.w);
+    }
 }
 
 // ---------------- DynDt, pkg/dem/DynDt.cpp:16-70 ----------------
@@ -2447,7 +2528,9 @@ __device__ __forceinline__ void stiffAdd(const ConView<T>& cv, int64_t nR, const
             kn = (double)*stAt(cv, c, S_P + 0, nR);
             kT = (double)*stAt(cv, c, S_P + 1, nR);
         } else {
-            const double uN = gv.gfc[2 * (size_t)c + 1].w;
+            double uN;
+            if constexpr (sizeof(T) == 8) uN = gv.gfc[2 * (size_t)c + 1].w;
+            else uN = (double)mixedCpt(cv.cpt)[c].w;
             const double kn0 = (double)*stAt(cv, c, S_P + 0, nR) * sqrt((double)*stAt(cv, c, S_P + 1, nR));
             kn = (3 / 2.) * kn0 * sqrt(-uN);
             kT = (double)*stAt(cv, c, S_P + 3, nR) * sqrt(uN < 0 ? -uN : 0.);

@@ -300,6 +300,7 @@ struct Shared {
     DBuf<int32_t> lids, lbeg, lvals;
     DBuf<uint8_t> temp;
     DBuf<double4> disp;
+    DBuf<double4> cptTmp;  // absolute contact points of a mixed-precision list (download, upload)
     DBuf<unsigned long long> ddOut;  // DynDt: per scene, bits of the minimum squared critical step
     // window of steps: its tables (device and pinned staging) and the last captured graph
     DBuf<uint8_t> winDev;
@@ -340,7 +341,7 @@ struct Shared {
         release(c, oldRow); release(c, ooFlag); release(c, oslot); release(c, srcRow);
         release(c, ooRank); release(c, rflag); release(c, rscan); release(c, starts); release(c, nrflag); release(c, nrscan); release(c, candK);
         release(c, candP); release(c, eB); release(c, eS); release(c, csrCnt); release(c, eI); release(c, csrScan);
-        release(c, temp); release(c, disp); release(c, ddOut);
+        release(c, temp); release(c, disp); release(c, cptTmp); release(c, ddOut);
         release(c, lcntE); release(c, lkeys); release(c, lslot); release(c, lcnt); release(c, loffE); release(c, lids); release(c, lbeg);
         release(c, lvals);
         if (pin) (void)hipHostFree(pin);
@@ -440,6 +441,8 @@ struct Scene {
         GatherView g;
         g.off = gpc(csrOff.p); g.ent = gpc(entV.p); g.realMask = gpc(realMask.p);
         g.gfc = gpc(mixed() ? cb32[cur].gfc.p : cb64[cur].gfc.p);
+        g.mixed = mixed() ? 1 : 0;
+        g.gcap = mixed() ? cb32[cur].cap : cb64[cur].cap;
         return g;
     }
     PartView pview() {
@@ -1963,7 +1966,18 @@ void downloadContacts(Scene& S, ConBuf<T>* cb, int64_t cap, int32_t* idA, int32_
             HIPCHK(hipMemcpy2DAsync(st.get(), (size_t)nR * sizeof(T), b.st.p, (size_t)b.cap * sizeof(T), (size_t)nR * sizeof(T), NSTATE,
                                     hipMemcpyDeviceToHost, c->stream));
         if (nTail > 0) HIPCHK(hipMemcpyAsync(st2.get(), b.st2.p, (size_t)NSTRIDE * nTail * sizeof(T), hipMemcpyDeviceToHost, c->stream));
-        if (nRows > 0) HIPCHK(hipMemcpyAsync(cpt.get(), b.cpt.p, (size_t)nRows * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+        if (nRows > 0) {
+            const double4* src = b.cpt.p;
+            if constexpr (sizeof(T) == 4) {  // stored as branches: absolute points first
+                Shared& X = *c->sh;
+                ensure(c, X.cptTmp, (size_t)b.cap);
+                hipLaunchKernelGGL(k_cpt_mixed<0>, dim3(nblk(n)), dim3(WDM_BLOCK), 0, c->stream, (const double4*)S.posr.p, b.view(), n,
+                                   S.periodic ? 1 : 0, toM3(S.hSize), X.cptTmp.p);
+                checkLaunch();
+                src = X.cptTmp.p;
+            }
+            HIPCHK(hipMemcpyAsync(cpt.get(), src, (size_t)nRows * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+        }
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     // state value q of a row (the layout of the kernels' stAt, compacted to the rows in use)
@@ -2070,10 +2084,22 @@ void uploadContacts(Scene& S, ConBuf<T>* cb, int64_t n, const int32_t* idA, cons
         HIPCHK(hipMemcpyAsync(b.cd.p, cd.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st0));
         HIPCHK(hipMemcpyAsync(b.slot.p, slot.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st0));
         HIPCHK(hipMemcpyAsync(b.st.p, st.data(), st.size() * sizeof(T), hipMemcpyHostToDevice, st0));
-        HIPCHK(hipMemcpyAsync(b.cpt.p, cpt.data(), n * sizeof(double4), hipMemcpyHostToDevice, st0));
+        if constexpr (sizeof(T) == 8) HIPCHK(hipMemcpyAsync(b.cpt.p, cpt.data(), n * sizeof(double4), hipMemcpyHostToDevice, st0));
+        else {  // stored as branches of B
+            Shared& X = *c->sh;
+            ensure(c, X.cptTmp, (size_t)cap);
+            HIPCHK(hipMemcpyAsync(X.cptTmp.p, cpt.data(), n * sizeof(double4), hipMemcpyHostToDevice, st0));
+        }
     }
     HIPCHK(hipMemcpyAsync(b.counter.p, &nR, sizeof(int32_t), hipMemcpyHostToDevice, st0));
     HIPCHK(hipMemcpyAsync(b.nItems.p, &zero, sizeof(int32_t), hipMemcpyHostToDevice, st0));
+    if constexpr (sizeof(T) == 4) {
+        if (n > 0) {
+            hipLaunchKernelGGL(k_cpt_mixed<1>, dim3(nblk(n)), dim3(WDM_BLOCK), 0, st0, (const double4*)S.posr.p, b.view(), n, S.periodic ? 1 : 0,
+                               toM3(S.hSize), c->sh->cptTmp.p);
+            checkLaunch();
+        }
+    }
     HIPCHK(hipStreamSynchronize(st0));
     S.nc = n;
     S.needFull = true;
