@@ -1514,6 +1514,69 @@ int orc_get_nodes(void* h, double* pos3, double* ori4, double* vel3, double* ang
     return 0;
 }
 int64_t orc_num_contacts(void* h) { return (int64_t)O(h)->con.size(); }
+/* The real contacts of a checkpoint become the contact map (Contact with L6Geom and FrictPhys
+   or HertzPhys, as ObjectIO restores them: pkg/dem/L6Geom.hpp, FrictPhys/HertzPhys fields):
+   ids in functor order, cellDist, trsf (row-major), local force, phys (FrictPhys: kn, kt,
+   tanPhi; HertzPhys: K, R, tanPhi, kt0, alpha_sqrtMK), lens, contact point.  uN, the
+   velocities and the Hertz stiffnesses are recomputed by the next ContactLoop; the next step
+   runs a collider full run. */
+int orc_set_contacts(void* h, int64_t n, const int32_t* idA, const int32_t* idB, const int32_t* cellDist3, const double* trsf9,
+                     const double* forceLocal3, const double* phys5, const double* lens2, const double* contPt3) {
+    Oracle* o = O(h);
+    o->con.clear();
+    for (int64_t k = 0; k < n; k++) {
+        const int a = idA[k], b = idB[k];
+        if (a < 0 || b < 0 || a >= o->np || b >= o->np || a == b) { o->err = "contact ids out of range"; return -1; }
+        Contact c{};
+        c.idA = a; c.idB = b;
+        for (int q = 0; q < 3; q++) c.cd[q] = cellDist3[3 * k + q];
+        c.stepCreated = o->step - 1;
+        c.minDist00Sq = -1;
+        c.geom = c.phys = true;
+        for (int r = 0; r < 3; r++)
+            for (int s = 0; s < 3; s++) c.trsf.m[r][s] = trsf9[9 * k + 3 * r + s];
+        for (int q = 0; q < 3; q++) { c.force[q] = forceLocal3[3 * k + q]; c.contPt[q] = contPt3[3 * k + q]; }
+        c.torque = vzero();
+        c.vel = vzero(); c.angVel = vzero();
+        c.lens[0] = lens2[2 * k]; c.lens[1] = lens2[2 * k + 1];
+        c.uN = NAN;
+        const double* ph = phys5 + 5 * k;
+        if (o->P.law == ORC_LAW_HERTZ_DMT) {
+            c.K = ph[0]; c.R = ph[1]; c.tanPhi = ph[2]; c.kt0 = ph[3]; c.alpha_sqrtMK = ph[4];
+            c.gamma = o->P.gamma; c.alpha = o->P.alpha;
+            c.kn = 0; c.kt = 0;
+        } else {
+            c.kn = ph[0]; c.kt = ph[1]; c.tanPhi = ph[2];
+        }
+        if (!o->con.emplace(Oracle::key(a, b), c).second) { o->err = "duplicate contact"; return -1; }
+    }
+    o->contactsDirty = true;
+    return 0;
+}
+/* phys (the layout of orc_set_contacts) and lens of the contacts in orc_get_contacts order
+   (NaN for contacts that are not real) */
+int orc_get_contact_phys(void* h, int64_t cap, double* phys5, double* lens2) {
+    Oracle* o = O(h);
+    int64_t i = 0;
+    for (auto& kv : o->con) {
+        if (i >= cap) break;
+        const Contact& c = kv.second;
+        const bool r = c.isReal();
+        double ph[5] = {c.kn, c.kt, c.tanPhi, 0., 0.};
+        if (o->P.law == ORC_LAW_HERTZ_DMT) { ph[0] = c.K; ph[1] = c.R; ph[2] = c.tanPhi; ph[3] = c.kt0; ph[4] = c.alpha_sqrtMK; }
+        for (int q = 0; q < 5; q++) phys5[5 * i + q] = r ? ph[q] : NAN;
+        for (int q = 0; q < 2; q++) lens2[2 * i + q] = r ? c.lens[q] : NAN;
+        i++;
+    }
+    return 0;
+}
+/* Scene::step and Scene::time of a checkpoint */
+int orc_set_time(void* h, int64_t step, double time) {
+    Oracle* o = O(h);
+    o->step = (long)step;
+    o->time = time;
+    return 0;
+}
 int orc_get_contacts(void* h, int64_t cap, int32_t* idA, int32_t* idB, int32_t* cellDist3, uint8_t* isReal, double* forceLocal3,
                      double* contPt3, double* trsf9, double* uN, int64_t* nOut) {
     Oracle* o = O(h);

@@ -107,6 +107,15 @@ int64_t orc_num_contacts(void* h);
 int orc_get_contacts(void* h, int64_t cap, int32_t* idA, int32_t* idB, int32_t* cellDist3,
                      uint8_t* isReal, double* forceLocal3, double* contPt3, double* trsf9,
                      double* uN, int64_t* nOut);
+/* the real contacts of a checkpoint replace the contact map (phys5: FrictPhys kn, kt, tanPhi,
+   -, -; HertzPhys K, R, tanPhi, kt0, alpha_sqrtMK); the next step runs a full collider run */
+int orc_set_contacts(void* h, int64_t n, const int32_t* idA, const int32_t* idB, const int32_t* cellDist3,
+                     const double* trsf9, const double* forceLocal3, const double* phys5, const double* lens2,
+                     const double* contPt3);
+/* phys5 and lens2 of the contacts in orc_get_contacts order (NaN where not real) */
+int orc_get_contact_phys(void* h, int64_t cap, double* phys5, double* lens2);
+/* step counter and time of a checkpoint */
+int orc_set_time(void* h, int64_t step, double time);
 /* out[ORC_E_N] values, created[ORC_E_N] 0/1 */
 int orc_get_energy(void* h, double* out, int32_t* created);
 /* stats: step, nFullRuns, verletDist(actual) */

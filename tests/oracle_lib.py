@@ -11,7 +11,8 @@ Parity-pinning status of the oracle's functions (details in DESIGN.md):
     work, released shear-spring energy) step by step, collision energy balance,
     ballistic leapfrog, damping, rotation composition about non-parallel axes, GRADV2
     cell, incident velocity under PBC, energy balance, DynDt (single-contact stiffness
-    closed forms, maxRelInc cap, dryRun, PeriodicEngine activation);
+    closed forms, maxRelInc cap, dryRun, PeriodicEngine activation), checkpoint restore
+    (orc_set_contacts: continues like the uninterrupted run; planted mistakes diverge);
   parity unpinned (no reference vector exists; checked only by invariants):
     multi-step L6Geom trsf evolution with renormalisation, IdealElPl Ft history
     over many steps, summation order of forces and energies.
@@ -83,6 +84,9 @@ def lib():
         "orc_get_free_ids": (C.c_int, [P, I32, pi, pi, pi]),
         "orc_make_cloud": (C.c_int, [pd, pd, D, D, I32, I32, D, pd, pd, I32, I32, pu, I64, pd, pd, pl, pd]),
         "orc_regular_pack": (C.c_int, [I32, I32, pd, D, D, I64, pd, pl]),
+        "orc_set_contacts": (C.c_int, [P, I64, pi, pi, pi, pd, pd, pd, pd, pd]),
+        "orc_set_time": (C.c_int, [P, I64, D]),
+        "orc_get_contact_phys": (C.c_int, [P, I64, pd, pd]),
     }
     for n, (r, a) in sig.items():
         f = getattr(L, n)
@@ -169,6 +173,26 @@ class OracleRun:
                                           C.byref(nout)))
         return dict(idA=idA, idB=idB, cellDist=cd, isReal=real.astype(bool), forceLocal=fl, contPt=cp,
                     trsf=tr.reshape(-1, 3, 3), uN=un)
+
+    def set_contacts(self, con):
+        """real contacts of a checkpoint (woodem_amd.snapshot.read's contacts dict)"""
+        D, I32 = C.c_double, C.c_int32
+        g = lambda k, t: np.ascontiguousarray(con[k], t)
+        idA, idB, cd = g("idA", np.int32), g("idB", np.int32), g("cellDist", np.int32)
+        tr, fl, ph, ln, cp = g("trsf", np.float64), g("forceLocal", np.float64), g("phys", np.float64), g("lens", np.float64), \
+            g("contPt", np.float64)
+        self._chk(self.L.orc_set_contacts(self.h, len(idA), _p(idA, I32), _p(idB, I32), _p(cd, I32), _p(tr, D), _p(fl, D), _p(ph, D),
+                                          _p(ln, D), _p(cp, D)))
+
+    def contact_phys(self):
+        """phys and lens of the contacts (order of contacts())"""
+        cap = int(self.L.orc_num_contacts(self.h))
+        ph, ln = np.empty((cap, 5)), np.empty((cap, 2))
+        self._chk(self.L.orc_get_contact_phys(self.h, cap, _p(ph, C.c_double), _p(ln, C.c_double)))
+        return ph, ln
+
+    def set_time(self, step, time):
+        self._chk(self.L.orc_set_time(self.h, int(step), float(time)))
 
     def energy(self):
         v = np.zeros(8)
