@@ -40,6 +40,26 @@ class OracleRunView:
         self.ctx, self.sid, self.a, self.cell0 = OracleCtx(o), 0, a.normalized(), cell
 
 
+def _stress_loop(a, c):
+    """particleStress (pkg/dem/Funcs.cpp:227-240) contact by contact, other id ascending"""
+    sigN, sigT = np.zeros((a.n_par, 3)), np.zeros((a.n_par, 3))
+    r = np.nonzero(c["isReal"])[0]
+    per = {}
+    for k in r:
+        per.setdefault(int(c["idA"][k]), []).append((int(c["idB"][k]), k))
+        per.setdefault(int(c["idB"][k]), []).append((int(c["idA"][k]), k))
+    for p, lst in per.items():
+        for _, k in sorted(lst):
+            A, B = c["idA"][k], c["idB"][k]
+            rA, rB = a.radius[A], a.radius[B]
+            rr = min(rA, rB) if (a.shape[A] == soa.SHAPE_SPHERE and a.shape[B] == soa.SHAPE_SPHERE) else rB
+            contA = np.pi * (rr * rr)
+            f = c["forceLocal"][k]
+            sigN[p] += (1 / contA) * (c["trsf"][k] @ np.array([f[0], 0., 0.]))
+            sigT[p] += (1 / contA) * (c["trsf"][k] @ np.array([0., f[1], f[2]]))
+    return sigN, sigT
+
+
 def test_vtk_export_box(tmp_path):
     a = scenes.box_lattice(nx=5, ny=4, nz=4)
     p, cell = runner.params_from_meta(a), runner.cell_from_meta(a)
@@ -56,20 +76,30 @@ def test_vtk_export_box(tmp_path):
     assert np.array_equal(s["radius"], a.radius[sph]) and np.array_equal(s["mass"], a.mass[sph])
     assert np.array_equal(s["id"], sph) and np.array_equal(s["vel"], nd["vel"][sph])
     assert np.array_equal(s["angVel"], nd["angVel"][sph]) and np.array_equal(s["matId"], a.mat[sph])
+    assert np.array_equal(s["mask"], a.mask[sph]) and np.array_equal(s["color"], np.zeros(len(sph)))
     assert np.array_equal(s["types"], np.ones(len(sph)))  # VTK_VERTEX
     c = o.contacts()
+    sN, sT = _stress_loop(a, c)
+    assert np.abs(sN).max() > 0 and np.abs(sT).max() > 0
+    assert np.allclose(s["sigN"], sN[sph], rtol=1e-12, atol=1e-12 * np.abs(sN).max())
+    assert np.allclose(s["sigT"], sT[sph], rtol=1e-12, atol=1e-12 * np.abs(sT).max())
     r = c["isReal"]
     assert r.sum() > 10
+    # contacts: every particle's node position at its id, then the contact points added for
+    # the wall sides; a line per real contact between the ids (VtkExport.cpp:146-212)
     k = snapshot.read_vtk_arrays(files["con"])
     assert np.array_equal(k["Fn"], c["forceLocal"][r, 0])
     assert np.allclose(k["|Ft|"], np.hypot(c["forceLocal"][r, 1], c["forceLocal"][r, 2]), rtol=1e-15, atol=0)
-    pts = k["Points"].reshape(-1, 2, 3)
-    # sphere ends at the centres, wall ends at the contact point (walls come first in a contact)
-    wall = a.shape[c["idA"][r]] == soa.SHAPE_WALL
+    pts = k["Points"]
+    assert np.array_equal(pts[:a.n_par], nd["pos"][a.nodes[:, 0]])
+    ln = k["connectivity"].reshape(-1, 2)
+    A, B = c["idA"][r], c["idB"][r]
+    wall = a.shape[A] == soa.SHAPE_WALL  # walls come first in a contact
     assert wall.sum() > 0 and (~wall).sum() > 0
-    assert np.array_equal(pts[~wall, 0], nd["pos"][c["idA"][r][~wall]])
-    assert np.array_equal(pts[wall, 0], c["contPt"][r][wall])
-    assert np.array_equal(pts[:, 1], nd["pos"][c["idB"][r]])
+    assert np.array_equal(ln[:, 1], B) and np.array_equal(ln[~wall, 0], A[~wall])
+    assert np.array_equal(ln[wall, 0], a.n_par + np.arange(wall.sum()))  # added points, in contact order
+    assert np.array_equal(pts[ln[wall, 0]], c["contPt"][r][wall])
+    assert len(pts) == a.n_par + wall.sum()
 
 
 def test_vtk_export_periodic_wraps_points(tmp_path):
@@ -85,6 +115,72 @@ def test_vtk_export_periodic_wraps_points(tmp_path):
     h = o.cell()[0]
     want = o.nodes()["pos"] - np.floor(o.nodes()["pos"] / np.diag(h)) * np.diag(h)
     assert np.array_equal(s["Points"], want)
+
+
+def test_vtk_contacts_across_the_periodic_boundary(tmp_path):
+    # a contact whose cellDist differs from the difference of its particles' wrap distances is
+    # drawn twice, each time from one particle to the other's shifted copy (VtkExport.cpp:186-201)
+    a = scenes.periodic_hertz(n=400, seed=3)
+    p, cell = runner.params_from_meta(a), runner.cell_from_meta(a)
+    o = OL.OracleRun(a, p, cell)
+    o.step(5)
+    run = OracleRunView(a, cell, o)
+    files = snapshot.vtk_export(run, str(tmp_path / "p."), what=("con",))
+    k = snapshot.read_vtk_arrays(files["con"])
+    h = o.cell()[0]
+    L = np.diag(h)
+    pos = o.nodes()["pos"][a.nodes[:, 0]]
+    wcd = np.floor(pos / L).astype(np.int64)
+    c = o.contacts()
+    pts, ln = k["Points"], k["connectivity"].reshape(-1, 2)
+    assert np.array_equal(pts[:a.n_par], pos - wcd * L)
+    li, extra, crossing = 0, a.n_par, 0
+    for j in np.nonzero(c["isReal"])[0]:
+        A, B, cd = c["idA"][j], c["idB"][j], c["cellDist"][j]
+        if np.array_equal(cd, wcd[B] - wcd[A]):
+            assert tuple(ln[li]) == (A, B)
+            li += 1
+            continue
+        crossing += 1
+        for me, other in ((A, B), (B, A)):
+            assert tuple(ln[li]) == (me, extra)
+            assert np.array_equal(pts[extra], pos[me] + h @ (wcd[other] - cd).astype(np.float64))
+            li += 1
+            extra += 1
+    assert li == len(ln) and extra == len(pts) and crossing > 0
+    assert len(k["Fn"]) == len(ln)
+
+
+@pytest.mark.parametrize("halfThick", [0., 1e-3])
+def test_vtk_mesh_facets(tmp_path, halfThick):
+    # each facet a triangle with its own three points, or with halfThick its two faces and
+    # closed flat edges (8 triangles on 6 points, thickFacetDiv=1, VtkExport.cpp:296-313)
+    a = scenes.facet_floor(halfThick=halfThick)
+    p, cell = runner.params_from_meta(a), runner.cell_from_meta(a)
+    o = OL.OracleRun(a, p, cell)
+    o.step(3)
+    run = OracleRunView(a, cell, o)
+    files = snapshot.vtk_export(run, str(tmp_path / "f."), what=("mesh",))
+    m = snapshot.read_vtk_arrays(files["mesh"])
+    fac = np.nonzero(a.shape == soa.SHAPE_FACET)[0]
+    per = 1 if halfThick == 0 else 8
+    npt = 3 if halfThick == 0 else 6
+    assert len(m["matId"]) == per * len(fac) and len(m["Points"]) == npt * len(fac)
+    pos = o.nodes()["pos"]
+    tri = m["connectivity"].reshape(-1, 3)
+    for i, f in enumerate(fac):
+        P = m["Points"][npt * i: npt * (i + 1)]
+        A, B, C = pos[a.nodes[f, 0]], pos[a.nodes[f, 1]], pos[a.nodes[f, 2]]
+        if halfThick == 0:
+            assert np.array_equal(P, np.array([A, B, C]))
+        else:
+            nn = np.cross(B - A, C - A)
+            dz = nn / np.linalg.norm(nn) * halfThick
+            assert np.allclose(P, np.array([A + dz, B + dz, C + dz, A - dz, B - dz, C - dz]), rtol=0, atol=1e-15)
+        assert np.all((tri[per * i: per * (i + 1)] >= npt * i) & (tri[per * i: per * (i + 1)] < npt * (i + 1)))
+    assert np.array_equal(m["matId"], np.repeat(a.mat[fac], per))
+    assert np.array_equal(m["vel"], np.repeat(o.nodes()["vel"][a.nodes[fac, 0]], per, axis=0))
+    assert np.array_equal(m["|sigma|"], np.zeros(per * len(fac))) and np.array_equal(m["types"], np.full(per * len(fac), 5))
 
 
 def test_vtk_binary_and_ascii_agree(tmp_path):

@@ -176,10 +176,52 @@ def _wrap(pos, h):
     return pos - np.floor(pos / L) * L
 
 
+def _particle_stress(a, c, n_par):
+    """DemFuncs::particleStress (pkg/dem/Funcs.cpp:227-240) of every particle: over its real
+    contacts, in the order of the other particle's id (Particle::contacts is a map by id),
+    sigN += trsf (Fn, 0, 0) / contA and sigT += trsf (0, Ft1, Ft2) / contA; contA is
+    L6Geom's contact area, pi rr^2 with rr the smaller sphere radius of a sphere pair and the
+    sphere's radius otherwise (L6Geom.cpp:61-63 with the Cg2 functors' r1, r2)."""
+    sigN, sigT = np.zeros((n_par, 3)), np.zeros((n_par, 3))
+    r = c["isReal"]
+    A, B, tr, fl = c["idA"][r], c["idB"][r], c["trsf"][r], c["forceLocal"][r]
+    if len(A) == 0:
+        return sigN, sigT
+    ss = (a.shape[A] == soa.SHAPE_SPHERE) & (a.shape[B] == soa.SHAPE_SPHERE)
+    rr = np.where(ss, np.minimum(a.radius[A], a.radius[B]), a.radius[B])
+    inv = 1. / (np.pi * (rr * rr))
+    fn = np.zeros_like(fl)
+    fn[:, 0] = fl[:, 0]
+    ft = fl.copy()
+    ft[:, 0] = 0.
+    gN = np.einsum("kij,kj->ki", tr, fn) * inv[:, None]
+    gT = np.einsum("kij,kj->ki", tr, ft) * inv[:, None]
+    # each particle's contacts by the other particle's id: entries (particle, other, contact)
+    part = np.concatenate([A, B])
+    other = np.concatenate([B, A])
+    k = np.concatenate([np.arange(len(A)), np.arange(len(A))])
+    order = np.lexsort((other, part))
+    np.add.at(sigN, part[order], gN[k[order]])
+    np.add.at(sigT, part[order], gT[k[order]])
+    return sigN, sigT
+
+
 def vtk_export(run, out, what=("spheres", "mesh", "con"), sphereSphereOnly=False, nanValue=0., binary=True):
-    """Write the scene's current state as VTK XML files named like VtkExport's; returns
-    {'spheres': path, 'mesh': path, 'con': path} for the parts written.  Arrays are inline
-    base64 binary (ascii with binary=False)."""
+    """Write the scene's current state as VTK XML files laid out like VtkExport's
+    (pkg/dem/VtkExport.cpp:123-560); returns {'spheres': path, 'mesh': path, 'con': path} for
+    the parts written.  Arrays are inline base64 binary (ascii with binary=False).
+      spheres: a vertex per sphere (position wrapped into a periodic cell), point data radius,
+        mass, id, mask, color, vel, angVel, matId, cell data sigT, sigN;
+      mesh: facets as triangles, each with its own points (one triangle; with halfThick, the
+        two faces and their closed flat edges as for thickFacetDiv=1), cell data color,
+        matState, matId, vel, angVel (of the facet's first node) and |sigma| (0);
+      con: the node position of every particle id (0 for a removed particle; wrapped into a
+        periodic cell) then added points, a line per real contact between the sphere
+        centres (the contact point for a side that is no sphere), a contact crossing the
+        periodic boundary drawn twice with a shifted end point; cell data Fn and |Ft|.
+    This data model holds no render color and no material state: color is 0, matState
+    nanValue.  Walls carry no display extents (Wall.glAB) and are left out (VtkExport with
+    infError=False)."""
     def arr(*x, **k):  # every array of this export in the chosen form
         return _arr(*x, binary=binary, **k)
 
@@ -187,76 +229,130 @@ def vtk_export(run, out, what=("spheres", "mesh", "con"), sphereSphereOnly=False
     step = ctx.stats(sid)["step"]
     h, _ = ctx.cell(sid)
     periodic = bool(run.cell0["periodic"])
+    L = np.diag(h)
     nd = ctx.download_nodes(sid, a.n_node)
     pos = nd["pos"]
     files = {}
     d = os.path.dirname(out)
     if d:
         os.makedirs(d, exist_ok=True)
-    sph = np.nonzero(a.shape == soa.SHAPE_SPHERE)[0]
-    spos = pos[a.nodes[sph, 0]]
-    if periodic:
-        spos = _wrap(spos, h)
+    c = ctx.contacts(sid) if ("con" in what or "spheres" in what) else None
     if "spheres" in what:
-        n = len(sph)
+        sph = np.nonzero(a.shape == soa.SHAPE_SPHERE)[0]
         nodes = a.nodes[sph, 0]
+        spos = pos[nodes]
+        if periodic:
+            spos = _wrap(spos, h)
+        n = len(sph)
+        sigN, sigT = _particle_stress(a, c, a.n_par)
         xml = ['<?xml version="1.0"?>', '<VTKFile type="UnstructuredGrid" version="0.1" byte_order="LittleEndian">',
                "<UnstructuredGrid>", '<Piece NumberOfPoints="%d" NumberOfCells="%d">' % (n, n),
                "<Points>" + arr("Points", spos, 3) + "</Points>",
                "<Cells>" + arr("connectivity", np.arange(n), 1, "Int32") + arr("offsets", np.arange(1, n + 1), 1, "Int32")
                + arr("types", np.ones(n, np.int32), 1, "UInt8") + "</Cells>",
                "<PointData>", arr("radius", a.radius[sph], 1), arr("mass", a.mass[nodes], 1), arr("id", sph, 1, "Int32"),
-               arr("mask", a.mask[sph], 1, "Int32"), arr("vel", nd["vel"][nodes], 3), arr("angVel", nd["angVel"][nodes], 3),
-               arr("matId", a.mat[sph], 1, "Int32"), "</PointData>", "</Piece>", "</UnstructuredGrid>", "</VTKFile>"]
+               arr("mask", a.mask[sph], 1, "Int32"), arr("color", np.zeros(n), 1), arr("vel", nd["vel"][nodes], 3),
+               arr("angVel", nd["angVel"][nodes], 3), arr("matId", a.mat[sph], 1, "Int32"), "</PointData>",
+               "<CellData>", arr("sigT", sigT[sph], 3), arr("sigN", sigN[sph], 3), "</CellData>",
+               "</Piece>", "</UnstructuredGrid>", "</VTKFile>"]
         fn = "%sspheres.%d.vtu" % (out, step)
         with open(fn, "w") as f:
             f.write("\n".join(xml) + "\n")
         files["spheres"] = fn
     fac = np.nonzero(a.shape == soa.SHAPE_FACET)[0]
     if "mesh" in what and len(fac):
-        vn = np.unique(a.nodes[fac].ravel())
-        local = {int(q): i for i, q in enumerate(vn)}
-        conn = np.array([[local[int(q)] for q in a.nodes[p]] for p in fac])
-        vel = nd["vel"][a.nodes[fac]].mean(axis=1)
-        nf = len(fac)
+        pts, tris, cellOf = [], [], []
+        for p in fac:
+            A, B, C = pos[a.nodes[p, 0]], pos[a.nodes[p, 1]], pos[a.nodes[p, 2]]
+            i0 = len(pts)
+            ht = a.radius[p]
+            if ht == 0.:
+                pts += [A, B, C]
+                t = [(0, 1, 2)]
+            else:  # closed flat edges (thickFacetDiv=1, VtkExport.cpp:300-313)
+                nn = np.cross(B - A, C - A)
+                dz = nn / np.linalg.norm(nn) * ht
+                pts += [A + dz, B + dz, C + dz, A - dz, B - dz, C - dz]
+                t = [(0, 1, 2), (3, 4, 5), (0, 1, 3), (3, 1, 4), (1, 2, 4), (4, 2, 5), (2, 0, 5), (5, 0, 3)]
+            tris += [(i0 + x, i0 + y, i0 + z) for x, y, z in t]
+            cellOf += [p] * len(t)
+        cellOf = np.array(cellOf)
+        nf, nv = len(tris), len(pts)
+        n0 = a.nodes[cellOf, 0]
         xml = ['<?xml version="1.0"?>', '<VTKFile type="UnstructuredGrid" version="0.1" byte_order="LittleEndian">',
-               "<UnstructuredGrid>", '<Piece NumberOfPoints="%d" NumberOfCells="%d">' % (len(vn), nf),
-               "<Points>" + arr("Points", pos[vn], 3) + "</Points>",
-               "<Cells>" + arr("connectivity", conn, 1, "Int32") + arr("offsets", 3 * np.arange(1, nf + 1), 1, "Int32")
+               "<UnstructuredGrid>", '<Piece NumberOfPoints="%d" NumberOfCells="%d">' % (nv, nf),
+               "<Points>" + arr("Points", np.array(pts), 3) + "</Points>",
+               "<Cells>" + arr("connectivity", np.array(tris), 1, "Int32") + arr("offsets", 3 * np.arange(1, nf + 1), 1, "Int32")
                + arr("types", np.full(nf, 5, np.int32), 1, "UInt8") + "</Cells>",
-               "<CellData>", arr("vel", vel, 3), arr("matId", a.mat[fac], 1, "Int32"), "</CellData>",
+               "<CellData>", arr("color", np.zeros(nf), 1), arr("matState", np.full(nf, float(nanValue)), 1),
+               arr("matId", a.mat[cellOf], 1, "Int32"), arr("vel", nd["vel"][n0], 3), arr("angVel", nd["angVel"][n0], 3),
+               arr("|sigma|", np.zeros(nf), 1), "</CellData>",
                "</Piece>", "</UnstructuredGrid>", "</VTKFile>"]
         fn = "%smesh.%d.vtu" % (out, step)
         with open(fn, "w") as f:
             f.write("\n".join(xml) + "\n")
         files["mesh"] = fn
     if "con" in what:
-        c = ctx.contacts(sid)
+        # the node position of every particle id (VtkExport.cpp:146-160), wrapped with its cell distance
+        ppos = np.zeros((a.n_par, 3))
+        live = a.shape != 0  # a removed particle has no shape
+        ppos[live] = pos[a.nodes[live, 0]]
+        wcd = np.zeros((a.n_par, 3), np.int64)
+        if periodic:
+            wcd = np.floor(ppos / L).astype(np.int64)
+            ppos = ppos - wcd * L
+            ppos[~live] = 0.
+            wcd[~live] = 0
         keep = c["isReal"].copy()
         if sphereSphereOnly:
             keep &= (a.shape[c["idA"]] == soa.SHAPE_SPHERE) & (a.shape[c["idB"]] == soa.SHAPE_SPHERE)
-        A, B, cd = c["idA"][keep], c["idB"][keep], c["cellDist"][keep]
-        cp = c["contPt"][keep]
-        # a sphere's end is its centre, any other particle's the contact point (VtkExport.cpp:180-190)
-        sa = (a.shape[A] == soa.SHAPE_SPHERE)[:, None]
-        sb = (a.shape[B] == soa.SHAPE_SPHERE)[:, None]
-        pa = np.where(sa, pos[a.nodes[A, 0]], cp)
-        pb = np.where(sb, pos[a.nodes[B, 0]] + cd @ h.T, cp)
-        if periodic:
-            start = _wrap(pa, h)
-            ends = start + (pb - pa)
-        else:
-            start, ends = pa, pb
-        m = len(A)
-        pts = np.empty((2 * m, 3))
-        pts[0::2], pts[1::2] = start, ends
+        A, B, cd, cp = c["idA"][keep], c["idB"][keep], c["cellDist"][keep].astype(np.int64), c["contPt"][keep]
         fl = c["forceLocal"][keep]
-        Fn = np.nan_to_num(fl[:, 0], nan=nanValue)
-        Ft = np.nan_to_num(np.sqrt(fl[:, 1] ** 2 + fl[:, 2] ** 2), nan=nanValue)
+        nsA, nsB = a.shape[A] != soa.SHAPE_SPHERE, a.shape[B] != soa.SHAPE_SPHERE
+        # a contact crossing the periodic boundary is drawn twice (VtkExport.cpp:186-201)
+        cross = np.zeros(len(A), bool)
+        if periodic:
+            cross = ~np.all(cd == wcd[B] - wcd[A], axis=1)
+        # points added per contact, in the exporter's order: the contact point for a side that
+        # is no sphere (A's, then B's); crossing: A's node shifted next to B's displayed image
+        # [, the contact point shifted likewise], then the same for B
+        nExtra = nsA.astype(np.int64) + nsB + 2 * cross
+        nLines = 1 + cross.astype(np.int64)
+        e0 = a.n_par + np.cumsum(nExtra) - nExtra  # first added point and first line of each contact
+        l0 = np.cumsum(nLines) - nLines
+        extra = np.zeros((int(nExtra.sum()), 3))
+        lines = np.zeros((int(nLines.sum()), 2), np.int64)
+        ppos_raw = np.zeros((a.n_par, 3))
+        ppos_raw[live] = pos[a.nodes[live, 0]]
+        q = ~cross
+        ia = np.where(nsA, e0, A)
+        ib = np.where(nsB, e0 + nsA, B)
+        extra[(e0 - a.n_par)[q & nsA]] = cp[q & nsA]
+        extra[(e0 + nsA - a.n_par)[q & nsB]] = cp[q & nsB]
+        lines[l0[q]] = np.stack([ia[q], ib[q]], 1)
+        if cross.any():
+            x = cross
+            hT = h.T
+            f0 = e0[x]
+            cA = f0 + 1
+            f1 = f0 + 1 + nsA[x]
+            cB = f1 + 1
+            extra[f0 - a.n_par] = ppos_raw[A[x]] + (wcd[B[x]] - cd[x]).astype(np.float64) @ hT
+            extra[f1 - a.n_par] = ppos_raw[B[x]] + (wcd[A[x]] - cd[x]).astype(np.float64) @ hT
+            ma, mb = nsA[x], nsB[x]
+            extra[cA[ma] - a.n_par] = cp[x][ma] + (wcd[A[x]][ma] - cd[x][ma]).astype(np.float64) @ hT
+            extra[cB[mb] - a.n_par] = cp[x][mb] + (wcd[B[x]][mb] - cd[x][mb]).astype(np.float64) @ hT
+            lines[l0[x]] = np.stack([np.where(ma, cA, A[x]), f0], 1)
+            lines[l0[x] + 1] = np.stack([np.where(mb, cB, B[x]), f1], 1)
+        allpts = np.concatenate([ppos, extra])
+        m = len(lines)
+        conn = lines.reshape(-1)
+        Fn = np.nan_to_num(np.repeat(fl[:, 0], nLines), nan=nanValue)
+        Ft = np.nan_to_num(np.repeat(np.sqrt(fl[:, 1] ** 2 + fl[:, 2] ** 2), nLines), nan=nanValue)
         xml = ['<?xml version="1.0"?>', '<VTKFile type="PolyData" version="0.1" byte_order="LittleEndian">', "<PolyData>",
-               '<Piece NumberOfPoints="%d" NumberOfVerts="0" NumberOfLines="%d" NumberOfStrips="0" NumberOfPolys="0">' % (2 * m, m),
-               "<Points>" + arr("Points", pts, 3) + "</Points>",
-               "<Lines>" + arr("connectivity", np.arange(2 * m), 1, "Int32") + arr("offsets", 2 * np.arange(1, m + 1), 1, "Int32")
+               '<Piece NumberOfPoints="%d" NumberOfVerts="0" NumberOfLines="%d" NumberOfStrips="0" NumberOfPolys="0">' % (len(allpts), m),
+               "<Points>" + arr("Points", allpts, 3) + "</Points>",
+               "<Lines>" + arr("connectivity", conn, 1, "Int32") + arr("offsets", 2 * np.arange(1, m + 1), 1, "Int32")
                + "</Lines>",
                "<CellData>", arr("Fn", Fn, 1), arr("|Ft|", Ft, 1), "</CellData>", "</Piece>", "</PolyData>", "</VTKFile>"]
         fn = "%scon.%d.vtp" % (out, step)
