@@ -307,3 +307,138 @@ def test_pack_refusals_before_any_device_call():
     assert b.kind == 0 and b.params == (0., 0., 0., 1., 2., 3.) and b.aabb() == ((0., 0., 0.), (1., 2., 3.))
     assert pack.inSphere((1, 2, 3), .5).params == (1., 2., 3., .5)
     assert pack.inCylinder((0, 0, 0), (0, 0, 1), .2).params == (0., 0., 0., 0., 0., 1., .2)
+
+
+# ---------------- predicates on the host, SpherePack.filtered, randomLoosePsd ----------------
+def _ortho_lattice(mn, mx, r, gap):
+    ax = [np.arange(mn[i] + r, mx[i] - r, 2 * r + gap) for i in range(3)]
+    return np.array([(x, y, z) for x in ax[0] for y in ax[1] for z in ax[2]])
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_host_predicates_select_what_the_oracle_lattice_keeps(kind):
+    # Predicate.contains with pad = r picks out of the full lattice exactly the points the
+    # oracle's regularOrtho keeps (whose predicates the geometric tests above pin), in order
+    from woodem_amd import pack
+    r, gap = .0031, .0007
+    pred = [pack.inAlignedBox((-.02, .01, .1), (.03, .05, .13)), pack.inSphere((.01, -.02, .03), .041),
+            pack.inCylinder((0., .01, -.01), (.02, .03, .07), .022)][kind]
+    mn, mx = (np.asarray(x) for x in pred.aabb())
+    lat = _ortho_lattice(mn, mx, r, gap)
+    keep = pred.contains(lat, r)
+    assert keep.sum() > 50 and (kind == 0 or keep.sum() < len(lat))  # a box keeps its whole lattice
+    assert np.array_equal(lat[keep], OL.regular_pack(0, kind, list(pred.params), r, gap))
+    assert pred(lat[int(np.argmax(keep))], r)
+    if kind:
+        assert not pred(lat[int(np.argmin(keep))], r)
+    else:
+        assert not pred(lat[0] - 1e-9, r) and pred(mx - r, r) and not pred(mx - r + 1e-9, r)
+
+
+def test_predicate_aabbs():
+    from woodem_amd import pack
+    mn, mx = pack.inSphere((1., 2., 3.), .5).aabb()
+    assert mn == (.5, 1.5, 2.5) and mx == (1.5, 2.5, 3.5)
+    # a cylinder's aabb is that of its two base discs: a disc of radius R with unit normal e
+    # reaches R sqrt(1 - e_d^2) along axis d; compare with points on the rims
+    A, B, R = np.array([.1, -.2, .3]), np.array([.4, .2, -.1]), .07
+    mn, mx = (np.asarray(x) for x in pack.inCylinder(A, B, R).aabb())
+    e = (B - A) / np.linalg.norm(B - A)
+    u = np.cross(e, [1., 0., 0.])
+    u /= np.linalg.norm(u)
+    v = np.cross(e, u)
+    t = np.linspace(0, 2 * np.pi, 20001)
+    rim = (R * (np.cos(t)[:, None] * u + np.sin(t)[:, None] * v))
+    pts = np.concatenate([A + rim, B + rim])
+    assert np.allclose(mn, pts.min(0), atol=1e-8) and np.allclose(mx, pts.max(0), atol=1e-8)
+    assert np.all(mn <= pts.min(0) + 1e-15) and np.all(mx >= pts.max(0) - 1e-15)
+    with pytest.raises(ValueError):
+        pack.inCylinder((1, 1, 1), (1, 1, 1), .1)
+
+
+def _pack_of(c, r, cid=None):
+    from woodem_amd import pack
+    sp = pack.SpherePack()
+    sp.c, sp.r = np.asarray(c, np.float64), np.asarray(r, np.float64)
+    sp.clumpId = np.full(len(sp.r), -1, np.int64) if cid is None else np.asarray(cid, np.int64)
+    return sp
+
+
+def test_filtered_recenters_and_pads_by_radius(caplog):
+    from woodem_amd import pack
+    rng = np.random.default_rng(3)
+    c = rng.uniform(0, 1, (400, 3))
+    r = rng.uniform(.01, .05, 400)
+    sp = _pack_of(c, r)
+    sp.cellSize = np.array([1., 1., 1.])
+    box = pack.inAlignedBox((10.2, 10.3, 10.1), (10.8, 10.7, 10.9))
+    f = sp.filtered(box)
+    smn, smx = sp.aabb()
+    off = np.array([10.5, 10.5, 10.5]) - (smn + smx) / 2  # the packing's aabb centre goes to the box's
+    cc = c + off
+    inside = np.all((cc - r[:, None] >= box.params[:3]) & (cc + r[:, None] <= box.params[3:]), axis=1)  # fully inside
+    assert np.array_equal(f.c, cc[inside]) and np.array_equal(f.r, r[inside]) and 20 < len(f) < 400
+    assert np.array_equal(f.cellSize, sp.cellSize) and not f.hasClumps()
+    # without recentering nothing lies in a box that far away
+    assert len(sp.filtered(box, recenter=False)) == 0
+    # a predicate larger than the packing is warned about, the spheres are all kept
+    with caplog.at_level("WARNING"):
+        g = sp.filtered(pack.inAlignedBox((-5, -5, -5), (5, 5, 5)), recenter=False)
+    assert len(g) == 400 and "outside of the predicate aabb" in caplog.text
+
+
+def test_filtered_removes_whole_clumps():
+    from woodem_amd import pack
+    # clump 0 has one sphere outside the sphere predicate, clump 1 is inside, two loose spheres
+    c = [(0, 0, 0), (.9, 0, 0), (1.2, 0, 0), (0, .2, 0), (0, .3, 0), (0, -.4, 0), (0, 0, 1.5)]
+    sp = _pack_of(c, [.1] * 7, [0, 0, 0, 1, 1, -1, -1])
+    f = sp.filtered(pack.inSphere((0, 0, 0), 1.), recenter=False)
+    assert np.array_equal(f.c, np.array(c, float)[[3, 4, 5]]) and list(f.clumpId) == [1, 1, -1]
+    assert len(pack.filterSpherePack(pack.inSphere((0, 0, 0), 1.), sp)) == len(sp.filtered(pack.inSphere((0, 0, 0), 1.)))
+
+
+def test_sanitize_psd():
+    from woodem_amd import pack
+    # leading zero points but the last, trailing top points but the first, passing scaled to 1
+    got = pack.sanitizePsd([(.1, 0), (.2, 0), (.3, 0), (.4, 20), (.5, 50), (.6, 50), (.7, 50)])
+    assert got == [(.3, 0.), (.4, .4), (.5, 1.), ]
+    assert pack.sanitizePsd([(.1, .2), (.2, 1.)]) == [(.1, .2), (.2, 1.)]
+    for bad in ([(.2, 0), (.1, 1)], [(.1, .5), (.2, .4)], [(-.1, 0), (.2, 1)]):
+        with pytest.raises(RuntimeError):
+            pack.sanitizePsd(bad)
+
+
+def test_random_loose_psd_refusals():
+    from woodem_amd import pack
+    box = pack.inAlignedBox((0, 0, 0), (1, 1, 1))
+    with pytest.raises(ValueError):
+        pack.randomLoosePsd(box, [(.1, 0), (.2, 1)])  # returnSpherePack must be True, as in the reference
+    with pytest.raises(NotImplementedError):
+        pack.randomLoosePsd(box, [(.1, 0), (.2, 1)], clumps=[object()], returnSpherePack=True)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", [0, 1, 2])
+def test_gpu_random_loose_psd_is_the_oracle_cloud_filtered(kind):
+    # the device cloud in the predicate's aabb equals the oracle's for the same generator
+    # state; filtered, every sphere lies in the predicate, none overlap, the radii stay in the
+    # PSD's range and the mass below the middle diameter is near the PSD's passing there
+    from woodem_amd import pack
+    from woodem_amd._native import Context
+    ctx = Context(0)
+    pred = [pack.inAlignedBox((0, 0, 0), (.06, .05, .04)), pack.inSphere((.1, .2, .3), .035),
+            pack.inCylinder((0, 0, 0), (.01, .02, .06), .025)][kind]
+    psd = [(.002, 0.), (.003, 30.), (.005, 100.)]  # percent passing, as a sieve curve is often given
+    pack.reset_generator()
+    sp = pack.randomLoosePsd(pred, psd, returnSpherePack=True, seed=777, ctx=ctx)
+    mn, mx = pred.aabb()
+    o_c, o_r, _, _ = OL.make_cloud(mn, mx, psdSizes=[.002, .003, .005], psdCumm=[0., .3, 1.], distributeMass=True, state=777,
+                                   cap=len(sp) * 4 + 20000)
+    want = _pack_of(o_c, o_r).filtered(pred)
+    assert len(sp) == len(want) > 200
+    assert np.array_equal(sp.c, want.c) and np.array_equal(sp.r, want.r)
+    assert np.all(pred.contains(sp.c, sp.r))
+    assert min_gap(sp.c, sp.r) >= -1e-12
+    assert sp.r.min() >= .001 - 1e-15 and sp.r.max() <= .0025 + 1e-15
+    m = sp.r ** 3
+    assert abs(m[sp.r <= .0015].sum() / m.sum() - .3) < .1

@@ -6,6 +6,7 @@ The reference's generator is a function-level static (SpherePack.cpp:136-137): t
 makeCloud call of a process seeds it (seed, or the clock when seed is 0) and later calls go
 on from where it stopped, whatever seed they pass.  This module keeps that state.
 """
+import logging
 import math
 import time
 
@@ -13,6 +14,7 @@ import numpy as np
 
 from . import soa
 
+_log = logging.getLogger(__name__)
 _M = 2147483647
 _state = None
 
@@ -140,6 +142,45 @@ class SpherePack:
 
     loadTxt = load
 
+    def aabb(self):
+        """SpherePack::aabb (SpherePack.hpp:106-110): the box around every sphere"""
+        if not len(self.r):
+            return np.full(3, np.inf), np.full(3, -np.inf)
+        r = self.r[:, None]
+        return (self.c - r).min(0), (self.c + r).max(0)
+
+    def dim(self):
+        mn, mx = self.aabb()
+        return mx - mn
+
+    def filtered(self, predicate, recenter=True):
+        """SpherePack_filtered (py/_packPredicates.cpp:364-405): a new pack of the spheres for
+        which predicate(c, r) holds (the radius is the pad); a clump goes whole when any of its
+        spheres fails.  With recenter (and a finite predicate aabb) the spheres are first moved
+        so that the centre of their aabb is the predicate's."""
+        ret = SpherePack()
+        ret.cellSize = np.array(self.cellSize, np.float64)
+        off = np.zeros(3)
+        smn, smx = self.aabb()
+        pmn, pmx = (np.asarray(x, np.float64) for x in predicate.aabb())
+        if recenter and not np.isinf((pmx - pmn).max()):
+            off = (pmn + pmx) / 2 - (smn + smx) / 2
+            smn, smx = smn + off, smx + off
+        for ax in range(3):
+            if pmn[ax] != -np.inf and pmn[ax] < smn[ax]:
+                _log.warning("SpherePack.filtered: axis=%d, packing aabb (min=%g) outside of the predicate aabb (min=%g)", ax, smn[ax], pmn[ax])
+            if pmx[ax] != np.inf and pmx[ax] > smx[ax]:
+                _log.warning("SpherePack.filtered: axis=%d, packing aabb (max=%g) outside of the predicate aabb (max=%g)", ax, smx[ax], pmx[ax])
+        c = self.c + off
+        keep = np.asarray(predicate.contains(c, self.r), bool)
+        cid = np.asarray(self.clumpId, np.int64) if len(self.clumpId) == len(self.r) else np.full(len(self.r), -1, np.int64)
+        clumped = cid >= 0
+        if np.any(clumped):
+            gone = np.unique(cid[clumped & ~keep])
+            keep = np.where(clumped, ~np.isin(cid, gone), keep)
+        ret.c, ret.r, ret.clumpId = c[keep], np.array(self.r[keep]), cid[keep]
+        return ret
+
     def toArrays(self, density=1e3, young=1e7, ktDivKn=.2, tanPhi=.5, mask=soa.DEFAULT_MOVABLE_MASK):
         """the spheres as scene arrays (free spheres, one FrictMat)"""
         return soa.spheres_arrays(self.c, self.r, density=density, young=young, ktDivKn=ktDivKn, tanPhi=tanPhi, mask=mask)
@@ -162,10 +203,23 @@ class SpherePack:
 
 # ---------------- predicates and regular packings ----------------
 class Predicate:
-    """A solid that regularOrtho/regularHexa clip their lattice to (py/_packPredicates.cpp);
-    `kind` and `params` are what the device evaluates."""
+    """A solid that packings are clipped to (py/_packPredicates.cpp); `kind` and `params` are
+    what the device evaluates.  `pred(pt, pad)` is Predicate::operator() for one point,
+    `pred.contains(c, pad)` the same for an (n, 3) array of points (pad scalar or per point),
+    `pred.aabb()` the (min, max) corners."""
     kind = -1
     params = ()
+
+    def __call__(self, pt, pad=0.):
+        return bool(self.contains(np.asarray(pt, np.float64).reshape(1, 3), pad)[0])
+
+    def center(self):
+        mn, mx = self.aabb()
+        return .5 * (np.asarray(mn) + np.asarray(mx))
+
+    def dim(self):
+        mn, mx = self.aabb()
+        return np.asarray(mx) - np.asarray(mn)
 
 
 class inAlignedBox(Predicate):
@@ -178,21 +232,64 @@ class inAlignedBox(Predicate):
     def aabb(self):
         return self.params[:3], self.params[3:]
 
+    def contains(self, c, pad=0.):
+        p, c = self.params, np.asarray(c, np.float64).reshape(-1, 3)
+        x, y, z = c[:, 0], c[:, 1], c[:, 2]
+        return (p[0] + pad <= x) & (p[3] - pad >= x) & (p[1] + pad <= y) & (p[4] - pad >= y) & (p[2] + pad <= z) & (p[5] - pad >= z)
+
 
 class inSphere(Predicate):
-    """Sphere (py/_packPredicates.cpp:96-104)"""
+    """Sphere (py/_packPredicates.cpp:96-104).  The reference subtracts the pad on both sides
+    of its test, (pt-center).norm()-pad <= radius-pad, so a point is inside when the centre is,
+    whatever the pad; kept as it is."""
     kind = 1
 
     def __init__(self, center, radius):
         self.params = tuple(float(x) for x in center) + (float(radius),)
 
+    def aabb(self):
+        p = self.params
+        return tuple(p[d] - p[3] for d in range(3)), tuple(p[d] + p[3] for d in range(3))
+
+    def contains(self, c, pad=0.):
+        p, c = self.params, np.asarray(c, np.float64).reshape(-1, 3)
+        dx, dy, dz = c[:, 0] - p[0], c[:, 1] - p[1], c[:, 2] - p[2]
+        return np.sqrt((dx * dx + dy * dy) + dz * dz) - pad <= p[3] - pad
+
 
 class inCylinder(Predicate):
-    """Cylinder between the centres of its bases (py/_packPredicates.cpp:189-205)"""
+    """Cylinder between the centres of its bases (py/_packPredicates.cpp:189-213)"""
     kind = 2
 
     def __init__(self, centerBottom, centerTop, radius):
         self.params = tuple(float(x) for x in centerBottom) + tuple(float(x) for x in centerTop) + (float(radius),)
+        c12 = np.subtract(centerTop, centerBottom, dtype=np.float64)
+        if not math.sqrt((c12[0] * c12[0] + c12[1] * c12[1]) + c12[2] * c12[2]) > 0:
+            raise ValueError("inCylinder: the axis has zero length")
+
+    def _ht(self):
+        p = self.params
+        cx, cy, cz = p[3] - p[0], p[4] - p[1], p[5] - p[2]
+        return cx, cy, cz, math.sqrt((cx * cx + cy * cy) + cz * cz)
+
+    def aabb(self):
+        # inCylinder::aabb: the bases' extreme points, radius * sin of the axis' angle per axis
+        A, B, r = self.params[:3], self.params[3:6], self.params[6]
+        ht = self._ht()[3]
+        k = (math.sqrt((A[1] - B[1]) ** 2 + (A[2] - B[2]) ** 2) / ht, math.sqrt((A[0] - B[0]) ** 2 + (A[2] - B[2]) ** 2) / ht,
+             math.sqrt((A[0] - B[0]) ** 2 + (A[1] - B[1]) ** 2) / ht)
+        return tuple(min(A[d], B[d]) - r * k[d] for d in range(3)), tuple(max(A[d], B[d]) + r * k[d] for d in range(3))
+
+    def contains(self, c, pad=0.):
+        p, c = self.params, np.asarray(c, np.float64).reshape(-1, 3)
+        cx, cy, cz, ht = self._ht()
+        x, y, z = c[:, 0], c[:, 1], c[:, 2]
+        u = (((x * cx + y * cy) + z * cz) - ((p[0] * cx + p[1] * cy) + p[2] * cz)) / (ht * ht)
+        axial = ~((u * ht < 0 + pad) | (u * ht > ht - pad))
+        ax, ay, az, bx, by, bz = x - p[0], y - p[1], z - p[2], x - p[3], y - p[4], z - p[5]
+        qx, qy, qz = ay * bz - az * by, az * bx - ax * bz, ax * by - ay * bx
+        axisDist = np.sqrt((qx * qx + qy * qy) + qz * qz) / ht
+        return axial & ~(axisDist > p[6] - pad)
 
 
 def _regular(hexa, predicate, radius, gap, ctx, cap, kw):
@@ -215,3 +312,74 @@ def regularHexa(predicate, radius, gap, ctx=None, cap=None, **kw):
     """Spheres on a hexagonal grid clipped to the predicate, separated by `gap`
     (py/pack.py:334-350), computed on the device; kw go to utils.sphere."""
     return _regular(True, predicate, radius, gap, ctx, cap, kw)
+
+
+def filterSpherePack(predicate, spherePack, **kw):
+    """py/pack.py:387-391: spherePack.filtered(predicate)"""
+    return spherePack.filtered(predicate)
+
+
+def sanitizePsd(psdPts, src="psdPts"):
+    """PsdSphereGenerator::sanitizePsd (pkg/dem/Psd.cpp:19-40): (diameter, passing) points,
+    both non-decreasing and non-negative; leading zero-passing points but the last of them and
+    trailing points at the top passing value but the first of them dropped; passing scaled so
+    that the top is 1."""
+    pts = [(float(d), float(p)) for d, p in psdPts]
+    if not pts:
+        return pts
+    for i in range(len(pts) - 1):
+        if pts[i][0] < 0 or pts[i][1] < 0:
+            raise RuntimeError(src + ": negative values not allowed.")
+        if pts[i][0] > pts[i + 1][0]:
+            raise RuntimeError(src + ": diameters (the x-component) must be increasing (%g>=%g)" % (pts[i][0], pts[i + 1][0]))
+        if pts[i][1] > pts[i + 1][1]:
+            raise RuntimeError(src + ": passing values (the y-component) must be increasing (%g>%g)" % (pts[i][1], pts[i + 1][1]))
+    maxPass = pts[-1][1]
+    ii = 0
+    while ii < len(pts) and not pts[ii][1] > 0:
+        ii += 1
+    if ii > 1:
+        pts = pts[ii - 1:]
+    ii = len(pts) - 2
+    while ii >= 0 and not pts[ii][1] < maxPass:
+        ii -= 1
+    if ii < len(pts) - 2:
+        pts = pts[:ii + 2]
+    if not pts:
+        raise RuntimeError(src + ": empty after removing spurious values?")
+    if maxPass != 1.:
+        pts = [(d, p / maxPass) for d, p in pts]
+    return pts
+
+
+def randomLoosePsd(predicate, psd, mass=True, discrete=False, maxAttempts=5000, clumps=[], returnSpherePack=False, seed=0, ctx=None,
+                   cap=None, **kw):
+    """A loose random packing with the particle size distribution `psd` ((diameter, passing)
+    points) filling the predicate (py/pack.py:353-380), as a SpherePack.
+
+    The reference fills the predicate's aabb with a BoxInlet of a PsdSphereGenerator (always
+    by mass and continuous, whatever `mass` and `discrete` say) until `maxAttempts` placements
+    in a row fail, then keeps the spheres inside the predicate (SpherePack.filtered).  Here the
+    aabb is filled on the device by makeCloud with the sanitized PSD distributed by mass (the
+    random sequential placement of SpherePack.cpp:135-248, which gives up on a sphere after its
+    own 1000 attempts and stops there), then filtered alike.  Both are random loose packings of
+    the same PSD in the same region; the generators differ, so the spheres are not the same.
+    As in the reference, returnSpherePack must be True; clumps are not supported."""
+    if clumps:
+        raise NotImplementedError("randomLoosePsd with clumps is not supported")
+    if not returnSpherePack:
+        raise ValueError("returnSpherePack must be True.")
+    pts = sanitizePsd(psd, "PsdSphereGenerator.psdPts")
+    if not pts:
+        raise RuntimeError("PsdSphereGenerator.psdPts is empty.")
+    mn, mx = predicate.aabb()
+    if not (np.all(np.isfinite(mn)) and np.all(np.isfinite(mx))):
+        raise ValueError("randomLoosePsd: the predicate's aabb must be finite")
+    sizes, cumm = [p[0] for p in pts], [p[1] for p in pts]
+    if cumm[0] != 0.:
+        # the generator makes the passing of the first point spheres of exactly that diameter
+        # (Psd.cpp:96); makeCloud gets it as a bin one millionth of the diameter wide
+        sizes, cumm = [sizes[0] * (1 - 1e-6)] + sizes, [0.] + cumm
+    sp = SpherePack()
+    sp.makeCloud(mn, mx, psdSizes=sizes, psdCumm=cumm, distributeMass=True, seed=seed, ctx=ctx, cap=cap)
+    return sp.filtered(predicate)
