@@ -9,7 +9,9 @@ the solid volume V (1-p) that a mass-weighted PSD with its scaling fills, the ge
 x <- 48271 x mod 2^31-1: after 10000 draws from 1 the state is 399268537, the known value of
 the minimal standard's 10000th output from seed 1), a PSD's radii within its range and the
 retry at a higher porosity when the box cannot hold num spheres.  The device packing is
-compared with the oracle's sphere for sphere, bit for bit.
+compared with the oracle's sphere for sphere, bit for bit; radii drawn by mass as the packing
+goes (num < 0) to an ulp, since the C library's pow, which the oracle calls like the reference,
+is not correctly rounded and the device's t^(-1/2) is.
 """
 import math
 
@@ -418,11 +420,35 @@ def test_random_loose_psd_refusals():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("case", ["rmean_mass", "psd_mass"])
+def test_gpu_stochastic_mass_cloud_within_an_ulp_of_the_oracle(case):
+    # with num < 0 the device draws each radius as it goes; by mass that is pow(t, -1/2), which
+    # the device rounds correctly and the C library's pow (the oracle's, as the reference's)
+    # misrounds by an ulp now and then: every sphere is placed alike, the radii agree to an
+    # ulp, and the centres to the few ulps that such a radius moves them
+    from woodem_amd._native import Context
+    ctx = Context(0)
+    kw = dict(rmean_mass=dict(mn=(0, 0, 0), mx=(.06, .06, .06), rMean=.003, rRelFuzz=.4, distributeMass=True),
+              psd_mass=dict(mn=(0, 0, 0), mx=(.06, .05, .07), psdSizes=[.002, .003, .006], psdCumm=[0, .3, 1], distributeMass=True))[case]
+    mn, mx = kw.pop("mn"), kw.pop("mx")
+    o_c, o_r, o_st, _ = OL.make_cloud(mn, mx, num=-1, state=4242, cap=60000, **kw)
+    st = np.array([4242], np.uint32)
+    args = dict(rMean=-1, rRelFuzz=0, psdSizes=[], psdCumm=[])
+    args.update(kw)
+    g_c, g_r, _ = ctx.make_cloud(mn, mx, args["rMean"], args["rRelFuzz"], -1, False, .5, args["psdSizes"], args["psdCumm"], True, st, 60000)
+    assert len(g_r) == len(o_r) > 500 and int(st[0]) == o_st
+    dr = np.abs(g_r - o_r)
+    assert np.all(dr <= np.spacing(o_r)) and np.count_nonzero(dr) <= .01 * len(o_r)
+    assert np.max(np.abs(g_c - o_c)) <= 1e-16
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("kind", [0, 1, 2])
 def test_gpu_random_loose_psd_is_the_oracle_cloud_filtered(kind):
-    # the device cloud in the predicate's aabb equals the oracle's for the same generator
-    # state; filtered, every sphere lies in the predicate, none overlap, the radii stay in the
-    # PSD's range and the mass below the middle diameter is near the PSD's passing there
+    # randomLoosePsd is the device cloud of the PSD by mass in the predicate's aabb, filtered;
+    # that cloud is the oracle's to an ulp (previous test).  Filtered, every sphere lies in the
+    # predicate, none overlap, the radii stay in the PSD's range and the mass below the middle
+    # diameter is near the PSD's passing there.
     from woodem_amd import pack
     from woodem_amd._native import Context
     ctx = Context(0)
@@ -432,11 +458,17 @@ def test_gpu_random_loose_psd_is_the_oracle_cloud_filtered(kind):
     pack.reset_generator()
     sp = pack.randomLoosePsd(pred, psd, returnSpherePack=True, seed=777, ctx=ctx)
     mn, mx = pred.aabb()
+    pack.reset_generator()
+    cl = pack.SpherePack()
+    cl.makeCloud(mn, mx, psdSizes=[.002, .003, .005], psdCumm=[0., .3, 1.], distributeMass=True, seed=777, ctx=ctx)
+    f = cl.filtered(pred)
+    assert np.array_equal(sp.c, f.c) and np.array_equal(sp.r, f.r)
     o_c, o_r, _, _ = OL.make_cloud(mn, mx, psdSizes=[.002, .003, .005], psdCumm=[0., .3, 1.], distributeMass=True, state=777,
-                                   cap=len(sp) * 4 + 20000)
+                                   cap=len(cl) + 20000)
+    assert len(o_r) == len(cl) and np.all(np.abs(cl.r - o_r) <= np.spacing(o_r))
     want = _pack_of(o_c, o_r).filtered(pred)
     assert len(sp) == len(want) > 200
-    assert np.array_equal(sp.c, want.c) and np.array_equal(sp.r, want.r)
+    assert np.max(np.abs(sp.c - want.c)) <= 1e-15 and np.all(np.abs(sp.r - want.r) <= np.spacing(want.r))
     assert np.all(pred.contains(sp.c, sp.r))
     assert min_gap(sp.c, sp.r) >= -1e-12
     assert sp.r.min() >= .001 - 1e-15 and sp.r.max() <= .0025 + 1e-15

@@ -2757,6 +2757,10 @@ struct CloudArgs {
     double rMean, rRelFuzz;
     const double* psdR;   // PSD radii
     const double* psdC;   // cumulative (psdCumm, or psdCumm2 with distributeMass)
+    // pow(r, -2) of the interpolation ends, from the host's pow: mode 0 rMean(1 -+ rRelFuzz),
+    // mode 1 one per PSD radius
+    double m2lo, m2hi;
+    const double* psdRm2;
     int64_t cap;      // spheres the output holds
     int maxTry;
 };
@@ -2779,12 +2783,48 @@ WDM_HD int psdGetPiece(double x, const double* cumm, int sz, double& norm) {
     norm = (x - cumm[i]) / (cumm[i + 1] - cumm[i]);
     return i;
 }
+// Sign of 1 - (y + h)^2 t, exact: the products split into exact pairs with fma, the terms
+// summed with error-free additions (the leading 1 - y*y*t is exact by Sterbenz' lemma, y*y*t
+// being near 1); the remaining rounding lies far below how close t^(-1/2) can come to a
+// midpoint of two doubles, so the sign is right.
+__device__ __forceinline__ int rsqrtMidSign(double y, double h, double t) {
+    const double p = y * y, pe = fma(y, y, -p);
+    const double s1 = p * t, s1e = fma(p, t, -s1);
+    const double s2 = pe * t, s2e = fma(pe, t, -s2);
+    const double v = y * t, ve = fma(y, t, -v);
+    double hi = 1. - s1, lo = 0.;
+    const double terms[6] = {-s1e, -s2, -2. * h * v, -s2e, -2. * h * ve, -(h * h) * t};
+    for (int k = 0; k < 6; k++) {
+        const double x = terms[k], s = hi + x, bp = s - hi;
+        lo += (hi - (s - bp)) + (x - bp);
+        hi = s;
+    }
+    const double r = hi + lo;
+    return r > 0. ? 1 : (r < 0. ? -1 : 0);
+}
+// t^(-1/2) correctly rounded (t positive, finite, normal): 1/sqrt(t) is within two ulps; each
+// step moves to the neighbour while t^(-1/2) lies beyond the midpoint toward it.
+__device__ double rsqrtRN(double t) {
+    double y = 1. / sqrt(t);
+    for (int k = 0; k < 4; k++) {
+        const double yu = __longlong_as_double(__double_as_longlong(y) + 1);
+        if (rsqrtMidSign(y, (yu - y) * .5, t) > 0) { y = yu; continue; }
+        const double yd = __longlong_as_double(__double_as_longlong(y) - 1);
+        if (rsqrtMidSign(yd, (y - yd) * .5, t) < 0) { y = yd; continue; }
+        break;
+    }
+    return y;
+}
+// pow3Interp on the device: pow(x * (b^-2 - a^-2) + a^-2, -1/2) with a^-2, b^-2 from the host's
+// pow and the outer power correctly rounded; the reference's libm pow misrounds it now and then
+// (glibc: about 4 in 10^4 draws, by one ulp), so draws agree with it to an ulp.
+__device__ __forceinline__ double pow3InterpM2(double x, double am2, double bm2) { return rsqrtRN(x * (bm2 - am2) + am2); }
 __device__ double cloudRadius(const CloudArgs& A, double rnd) {
-    if (A.mode == 0) return A.distributeMass ? pow3Interp(rnd, A.rMean * (1 - A.rRelFuzz), A.rMean * (1 + A.rRelFuzz))
-                                             : A.rMean * (2 * (rnd - .5) * A.rRelFuzz + 1);
+    if (A.mode == 0) return A.distributeMass ? pow3InterpM2(rnd, A.m2lo, A.m2hi) : A.rMean * (2 * (rnd - .5) * A.rRelFuzz + 1);
     double norm;
     const int piece = psdGetPiece(rnd, A.psdC, A.nPsd, norm);
-    return A.distributeMass ? pow3Interp(norm, A.psdR[piece], A.psdR[piece + 1]) : A.psdR[piece] + norm * (A.psdR[piece + 1] - A.psdR[piece]);
+    return A.distributeMass ? pow3InterpM2(norm, A.psdRm2[piece], A.psdRm2[piece + 1])
+                            : A.psdR[piece] + norm * (A.psdR[piece + 1] - A.psdR[piece]);
 }
 __device__ __forceinline__ double cloudWrapRel(double x, double x0, double x1) {
     const double xNorm = (x - x0) / (x1 - x0);

@@ -3028,8 +3028,12 @@ int wdm_make_cloud(wdm_ctx* c, const double* mn3, const double* mx3, double rMea
             A.rRelFuzz = rRelFuzz;
             A.cap = num > 0 ? num : cap;
             A.maxTry = 1000;
+            A.m2lo = std::pow(rM * (1 - rRelFuzz), -2.);
+            A.m2hi = std::pow(rM * (1 + rRelFuzz), -2.);
+            std::vector<double> psdRm2(psdR.size());
+            for (size_t q = 0; q < psdR.size(); q++) psdRm2[q] = std::pow(psdR[q], -2.);
             Blob b;
-            const size_t oR = b.addv(radii), oPR = b.addv(psdR);
+            const size_t oR = b.addv(radii), oPR = b.addv(psdR), oPM = b.addv(psdRm2);
             std::vector<double> cumv(cum ? cum : psdR.data(), (cum ? cum : psdR.data()) + (nPsd > 0 ? nPsd : 0));
             const size_t oPC = b.addv(cumv);
             const size_t oS = b.add(&state, sizeof(state));
@@ -3043,6 +3047,7 @@ int wdm_make_cloud(wdm_ctx* c, const double* mn3, const double* mx3, double rMea
             HIPCHK(hipMemcpyAsync(tb.p, b.b.data(), b.b.size(), hipMemcpyHostToDevice, st));
             A.radii = (const double*)(tb.p + oR);
             A.psdR = (const double*)(tb.p + oPR);
+            A.psdRm2 = (const double*)(tb.p + oPM);
             A.psdC = (const double*)(tb.p + oPC);
             hipLaunchKernelGGL(k_fill_i32, dim3(nblk(cells)), dim3(WDM_BLOCK), 0, st, (int32_t*)cellsB.p, cells, -1);
             hipLaunchKernelGGL(k_make_cloud, dim3(1), dim3(1024), 0, st, A, (uint32_t*)(tb.p + oS), (int32_t*)cellsB.p, (int32_t*)nextB.p,
