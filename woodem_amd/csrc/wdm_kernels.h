@@ -909,11 +909,7 @@ __global__ void k_fill_i32(int32_t* a, int64_t n, int32_t v) {
     if (i < n) a[i] = v;
 }
 
-// ---------------- exclusive prefix sums: per-tile sums, one workgroup over the tile sums,
-// then each tile scanned with its offset (out[0..n], out[n] = the total) ----------------
-#define SCAN_ITEMS 8
-#define SCAN_TILE (WDM_BLOCK * SCAN_ITEMS)
-#define SCAN_TOP 1024
+// ---------------- scans within a wavefront and a workgroup ----------------
 template <typename V>
 __device__ __forceinline__ V waveInclusiveScan(V x) {
     const int lane = threadIdx.x & 63;
@@ -941,21 +937,47 @@ __device__ __forceinline__ int64_t blockExclusiveScan(int64_t x, int64_t* ws, in
     total = tot;
     return off + inc - x;
 }
-template <typename TIn>
-__global__ void __launch_bounds__(WDM_BLOCK) k_scan_part(const TIn* in, int64_t n, int64_t* tsum) {
+// ---------------- exclusive prefix sums: tile sums, then each tile scanned from the sum of
+// the tiles before it (read by the tile's workgroup from the tile sums, or, for many tiles,
+// from their prefix computed by one workgroup in between) ----------------
+#define SCAN_ITEMS 8
+#define SCAN_TILE (WDM_BLOCK * SCAN_ITEMS)
+#define SCAN_DIRECT_TILES 4096
+#define SCAN_TOP 1024
+struct ScanId {
+    __device__ int64_t operator()(int64_t x) const { return x; }
+};
+// a work-list flag: bit 0 a listed contact, bit 1 a near one; counted as 1 in the low 32
+// bits and, for a near one, 1 in the high 32 bits
+struct ScanNearSplit {
+    __device__ int64_t operator()(int64_t x) const { return (x & 1) + ((x >> 1) << 32); }
+};
+// the SCAN_ITEMS items of this thread (zero past n) through op; one 8-byte load for bytes
+template <typename TIn, class Op>
+__device__ __forceinline__ void scanLoad(const TIn* in, int64_t n, int64_t base, Op op, int64_t v[SCAN_ITEMS]) {
+    if (sizeof(TIn) == 1 && base + SCAN_ITEMS <= n && !((uintptr_t)(in + base) & 7)) {
+        const uint64_t w = *(const uint64_t*)(in + base);
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; k++) v[k] = op((int64_t)((w >> (8 * k)) & 0xff));
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) v[k] = base + k < n ? op((int64_t)in[base + k]) : 0;
+}
+template <typename TIn, class Op>
+__global__ void __launch_bounds__(WDM_BLOCK) k_scan_sums(const TIn* in, int64_t n, int64_t* tsum, Op op) {
     __shared__ int64_t ws[WDM_BLOCK / 64];
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int64_t v[SCAN_ITEMS];
+    scanLoad(in, n, (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS, op, v);
     int64_t s = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; k++)
-        if (base + k < n) s += (int64_t)in[base + k];
+    for (int k = 0; k < SCAN_ITEMS; k++) s += v[k];
     int64_t tot;
     (void)blockExclusiveScan(s, ws, tot);
     if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
 }
-// one workgroup: tsum[0..nt) to its exclusive prefix, the total to *total
-template <typename TOut>
-__global__ void __launch_bounds__(SCAN_TOP) k_scan_top(int64_t* tsum, int64_t nt, TOut* total) {
+// one workgroup: tsum[0..nt) to its exclusive prefix
+__global__ void __launch_bounds__(SCAN_TOP) k_scan_top(int64_t* tsum, int64_t nt) {
     __shared__ int64_t ws[SCAN_TOP / 64];
     int64_t carry = 0;
     for (int64_t b0 = 0; b0 < nt; b0 += SCAN_TOP) {
@@ -966,25 +988,44 @@ __global__ void __launch_bounds__(SCAN_TOP) k_scan_top(int64_t* tsum, int64_t nt
         if (i < nt) tsum[i] = carry + ex;
         carry += tot;
     }
-    if (threadIdx.x == 0) *total = (TOut)carry;
 }
-template <typename TIn, typename TOut>
-__global__ void __launch_bounds__(WDM_BLOCK) k_scan_fin(const TIn* in, int64_t n, const int64_t* tsum, TOut* out) {
+// prefixed: tsum holds the exclusive prefix of the tile sums; otherwise the tile sums, and
+// the workgroup adds those before its tile
+template <typename TIn, typename TOut, class Op>
+__global__ void __launch_bounds__(WDM_BLOCK) k_scan_tiles(const TIn* in, int64_t n, const int64_t* tsum, int prefixed, TOut* out, Op op) {
     __shared__ int64_t ws[WDM_BLOCK / 64];
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    const int64_t tile = blockIdx.x, base = tile * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int64_t off;
+    if (prefixed) off = tsum[tile];
+    else {
+        int64_t s = 0;
+        for (int64_t j = threadIdx.x; j < tile; j += WDM_BLOCK) s += tsum[j];
+        int64_t tot;
+        (void)blockExclusiveScan(s, ws, tot);
+        off = tot;
+    }
     int64_t v[SCAN_ITEMS];
+    scanLoad(in, n, base, op, v);
     int64_t s = 0;
 #pragma unroll
-    for (int k = 0; k < SCAN_ITEMS; k++) {
-        v[k] = base + k < n ? (int64_t)in[base + k] : 0;
-        s += v[k];
-    }
+    for (int k = 0; k < SCAN_ITEMS; k++) s += v[k];
     int64_t tot;
-    int64_t run = tsum[blockIdx.x] + blockExclusiveScan(s, ws, tot);
+    int64_t run = off + blockExclusiveScan(s, ws, tot);
+    if (tile == (int64_t)gridDim.x - 1 && threadIdx.x == 0) out[n] = (TOut)(off + tot);
+    // through LDS, so that each store instruction writes consecutive items (rows padded to
+    // SCAN_ITEMS + 1 against bank conflicts)
+    __shared__ TOut lo[WDM_BLOCK * (SCAN_ITEMS + 1)];
 #pragma unroll
     for (int k = 0; k < SCAN_ITEMS; k++) {
-        if (base + k < n) out[base + k] = (TOut)run;
+        lo[threadIdx.x * (SCAN_ITEMS + 1) + k] = (TOut)run;
         run += v[k];
+    }
+    __syncthreads();
+    const int64_t t0 = tile * SCAN_TILE;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) {
+        const int e = k * WDM_BLOCK + threadIdx.x;
+        if (t0 + e < n) out[t0 + e] = lo[(e / SCAN_ITEMS) * (SCAN_ITEMS + 1) + e % SCAN_ITEMS];
     }
 }
 
@@ -1737,7 +1778,7 @@ __device__ __forceinline__ int64_t candLowerBound(const FRScene& F, const int64_
 // old contacts that stay only because they are real (their bounds no longer overlap)
 template <typename T>
 __global__ void __launch_bounds__(WDM_BLOCK) k_flag_old(const ConDesc<T>* C, const FRScene* FR, Map m, const int64_t* rowOff,
-                                                       const uint64_t* candK, int32_t* flag) {
+                                                       const uint64_t* candK, uint8_t* flag) {
     int64_t lb;
     const int t = mapBlock(m, lb);
     const FRScene& F = FR[t];
@@ -1831,7 +1872,7 @@ __device__ __forceinline__ bool nearPair(const StepArgs& a, const NodeView& n, c
 template <typename T>
 __global__ void __launch_bounds__(WDM_BLOCK) k_emit_cand(const SceneDesc* D, const ConDesc<T>* C, const FRScene* FR, Map m,
                                                         const int64_t* ooRank, const int64_t* oldRow, const uint64_t* ckey,
-                                                        const int32_t* cper, int32_t* oslot, int64_t* rflag) {
+                                                        const int32_t* cper, int32_t* oslot, uint8_t* rflag) {
     int64_t lb;
     const int t = mapBlock(m, lb);
     const FRScene& F = FR[t];
@@ -1883,9 +1924,9 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_emit_cand(const SceneDesc* D, con
     }
 }
 template <typename T>
-__global__ void __launch_bounds__(WDM_BLOCK) k_emit_oor(const ConDesc<T>* C, const FRScene* FR, Map m, const int32_t* flag,
+__global__ void __launch_bounds__(WDM_BLOCK) k_emit_oor(const ConDesc<T>* C, const FRScene* FR, Map m, const uint8_t* flag,
                                                        const int64_t* ooRank, const int64_t* rowOff, const uint64_t* ckey, int32_t* oslot,
-                                                       int64_t* rflag) {
+                                                       uint8_t* rflag) {
     int64_t lb;
     const int t = mapBlock(m, lb);
     const FRScene& F = FR[t];
@@ -1904,13 +1945,12 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_emit_oor(const ConDesc<T>* C, con
 
 // dense state rows for the real (and touching) contacts of the merged list, in key order
 // (rscan: exclusive batch-wide prefix of rflag); oslot -2: row without state to copy
-// nrflag: for a contact that enters the work list without a state row (not dormant), 1 in
-// the low 32 bits plus 1 in the high 32 bits when it is a near pair, so that one scan counts
-// both (NR_ALL, NR_NEAR)
+// nrflag: for a contact that enters the work list without a state row (not dormant), bit 0,
+// and bit 1 when it is a near pair; its scan (ScanNearSplit) counts both (NR_ALL, NR_NEAR)
 // srcRow[newStart + row]: the old state row each new row copies (-2: none, a touching pair)
 template <typename T>
 __global__ void __launch_bounds__(WDM_BLOCK) k_copy_state(const ConDesc<T>* C, const FRScene* FR, Map m, const int32_t* oslot,
-                                                         const int64_t* rscan, int64_t* nrflag, int32_t* srcRow) {
+                                                         const int64_t* rscan, uint8_t* nrflag, int32_t* srcRow) {
     int64_t lb;
     const int t = mapBlock(m, lb);
     const FRScene& F = FR[t];
@@ -1921,7 +1961,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_copy_state(const ConDesc<T>* C, c
     if (os == -1) {
         nc.slot[d] = -1;
         const uint32_t inf = nc.info[d];
-        nrflag[F.newStart + d] = (inf & CI_DORMANT) ? 0 : (1 + ((inf & CI_NEAR) ? ((int64_t)1 << 32) : 0));
+        nrflag[F.newStart + d] = (inf & CI_DORMANT) ? 0 : ((inf & CI_NEAR) ? 3 : 1);
         return;
     }
     nrflag[F.newStart + d] = 0;

@@ -271,7 +271,8 @@ struct Shared {
     DBuf<uint32_t> ekey, eslot, ccnt;    // bound entries: cell key and slot; count per cell
     DBuf<int32_t> cbeg;                  // first entry of each cell (exclusive prefix of ccnt)
     DBuf<uint32_t> ckeys2;               // sorted entries: cell key, particle id, FP32 box
-    DBuf<int32_t> cvals2, ooFlag, oslot, srcRow;
+    DBuf<int32_t> cvals2, oslot, srcRow;
+    DBuf<uint8_t> ooFlag, rflag, nrflag;  // 0/1 flags (nrflag: bit 1 a near pair) of the merge's prefix sums
     DBuf<float4> cbox;
     DBuf<double4> eb0, eb1;  // sorted entries: copies of the bound corners, (shape, mask)
     DBuf<int2> einfo;
@@ -279,7 +280,7 @@ struct Shared {
     DBuf<BItem> btmp;
     DBuf<uint64_t> baux;
     DBuf<uint32_t> longList, longCnt;
-    DBuf<int64_t> tsum;                  // prefix sums: per-tile sums
+    DBuf<int64_t> tsum;  // prefix sums: per-tile sums
     // candidate pairs: per wavefront of k_pairs its count and region; per row (scene, min id)
     // the count and offsets
     DBuf<int32_t> wcnt, wmax, scnt;
@@ -287,7 +288,7 @@ struct Shared {
     DBuf<uint64_t> rItem;
     DBuf<uint32_t> rRow, rSlot, rowCnt;
     DBuf<int64_t> rowOff, oldRow;
-    DBuf<int64_t> ooRank, rflag, rscan, starts, nrflag, nrscan;
+    DBuf<int64_t> ooRank, rscan, starts, nrscan;
     DBuf<uint64_t> candK;
     DBuf<int32_t> candP;
     // CSR entries: bucket (row of the particle), slot, item; count per row and offsets
@@ -511,20 +512,23 @@ void checkLaunch() {
 // temp storage for hipcub calls
 void* tempFor(Ctx* c, size_t bytes) { ensure(c, c->sh->temp, bytes); return c->sh->temp.p; }
 
-// out[i] = sum(in[0..i)) for i in [0, n], on the context stream
-template <typename TIn, typename TOut>
-void scanExclusive(Ctx* c, const TIn* in, TOut* out, int64_t n) {
+void fillU32(Ctx* c, uint32_t* a, int64_t n) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n)), dim3(WDM_BLOCK), 0, c->stream, (int32_t*)a, n, 0);
+}
+// out[i] = sum(op(in[0..i))) for i in [0, n], on the context stream
+template <typename TIn, typename TOut, class Op = ScanId>
+void scanExclusive(Ctx* c, const TIn* in, TOut* out, int64_t n, Op op = Op()) {
     Shared& X = *c->sh;
     hipStream_t st = c->stream;
     const int64_t nt = std::max<int64_t>((n + SCAN_TILE - 1) / SCAN_TILE, 1);
+    if (nt >= INT32_MAX) throw WdmError(WDM_E_ARG, "prefix sum too long");
     ensure(c, X.tsum, (size_t)nt);
-    hipLaunchKernelGGL(k_scan_part<TIn>, dim3((unsigned)nt), dim3(WDM_BLOCK), 0, st, in, n, X.tsum.p);
-    hipLaunchKernelGGL(k_scan_top<TOut>, dim3(1), dim3(SCAN_TOP), 0, st, X.tsum.p, nt, out + n);
-    if (n > 0) hipLaunchKernelGGL((k_scan_fin<TIn, TOut>), dim3((unsigned)nt), dim3(WDM_BLOCK), 0, st, in, n, (const int64_t*)X.tsum.p, out);
+    hipLaunchKernelGGL((k_scan_sums<TIn, Op>), dim3((unsigned)nt), dim3(WDM_BLOCK), 0, st, in, n, X.tsum.p, op);
+    const int prefixed = nt > SCAN_DIRECT_TILES;
+    if (prefixed) hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_TOP), 0, st, X.tsum.p, nt);
+    hipLaunchKernelGGL((k_scan_tiles<TIn, TOut, Op>), dim3((unsigned)nt), dim3(WDM_BLOCK), 0, st, in, n, (const int64_t*)X.tsum.p, prefixed,
+                       out, op);
     checkLaunch();
-}
-void fillU32(Ctx* c, uint32_t* a, int64_t n) {
-    if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3(nblk(n)), dim3(WDM_BLOCK), 0, c->stream, (int32_t*)a, n, 0);
 }
 // the second half of a bucket sort: items scattered into btmp (n of them, or at
 // most n with *total of them on the device; off the exclusive prefix of the bucket counts)
@@ -1253,7 +1257,7 @@ struct InsertionSortCollider {
                                    (const uint64_t*)X.candK.p, X.ooFlag.p);
                 checkLaunch();
             }
-            scanExclusive(c, (const int32_t*)X.ooFlag.p, X.ooRank.p, nOldTot);
+            scanExclusive(c, (const uint8_t*)X.ooFlag.p, X.ooRank.p, nOldTot);
             if (nOldTot > 0) {
                 std::vector<int64_t> os = segmentStarts(c, X.ooRank.p, FRd, nT, 1, nOldTot);
                 for (int t = 0; t < nT; t++) noor[t] = os[t + 1] - os[t];
@@ -1302,10 +1306,10 @@ struct InsertionSortCollider {
                 hipLaunchKernelGGL(k_emit_cand<T>, dim3((unsigned)bpC[nT]), dim3(WDM_BLOCK), 0, st, B.D, Cd, FRd, mC, (const int64_t*)X.ooRank.p,
                                    (const int64_t*)X.oldRow.p, (const uint64_t*)X.candK.p, (const int32_t*)X.candP.p, X.oslot.p, X.rflag.p);
             if (noorTot > 0)
-                hipLaunchKernelGGL(k_emit_oor<T>, dim3((unsigned)bpO[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mO, (const int32_t*)X.ooFlag.p,
+                hipLaunchKernelGGL(k_emit_oor<T>, dim3((unsigned)bpO[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mO, (const uint8_t*)X.ooFlag.p,
                                    (const int64_t*)X.ooRank.p, (const int64_t*)X.rowOff.p, (const uint64_t*)X.candK.p, X.oslot.p, X.rflag.p);
             checkLaunch();
-            scanExclusive(c, (const int64_t*)X.rflag.p, X.rscan.p, nNewTot);
+            scanExclusive(c, (const uint8_t*)X.rflag.p, X.rscan.p, nNewTot);
             if (nNewTot > 0)
                 hipLaunchKernelGGL(k_copy_state<T>, dim3((unsigned)bpN[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mN, (const int32_t*)X.oslot.p,
                                    (const int64_t*)X.rscan.p, X.nrflag.p, X.srcRow.p);
@@ -1314,7 +1318,7 @@ struct InsertionSortCollider {
                                    (const int32_t*)X.srcRow.p);
             checkLaunch();
             // ContactLoop work list: rows first, then the other non-dormant contacts
-            scanExclusive(c, (const int64_t*)X.nrflag.p, X.nrscan.p, nNewTot);
+            scanExclusive(c, (const uint8_t*)X.nrflag.p, X.nrscan.p, nNewTot, ScanNearSplit());
             if (nNewTot > 0)
                 hipLaunchKernelGGL(k_items<T>, dim3((unsigned)bpN[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mN, (const int64_t*)X.rscan.p,
                                    (const int64_t*)X.nrscan.p);
