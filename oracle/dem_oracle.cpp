@@ -213,14 +213,31 @@ struct Oracle {
     double verletDistActual = NAN;
     bool verletResolved = false;
     long nFullRuns = 0;
+    long sortChunks = -1, sortPasses = 0;  // the last parallel insertion sort (sortChunks, InsertionSortCollider.hpp:264)
     VecBounds BB[3];
     std::vector<double> minima, maxima;
     std::vector<Aabb> aabb;
     // contacts: key (min<<32|max) -> contact
     std::map<uint64_t, Contact> con;
     std::vector<uint64_t> pending;
-    std::vector<Contact> makeLater;
-    std::vector<uint64_t> removeLater;
+    // contacts to make / remove after the sort, one list per OpenMP thread
+    // (InsertionSortCollider.hpp:94-113: rremoveContacts / rmakeContacts per thread)
+    std::vector<std::vector<Contact>> makeLater;
+    std::vector<std::vector<uint64_t>> removeLater;
+    int tid() const {
+#ifdef _OPENMP
+        return omp_get_thread_num();
+#else
+        return 0;
+#endif
+    }
+    static int maxThreads() {
+#ifdef _OPENMP
+        return omp_get_max_threads();
+#else
+        return 1;
+#endif
+    }
     // deterministic force application: per-particle list of (other id, contact*) sorted by other id
     bool adjDirty = true;
     std::vector<Contact*> lin;
@@ -447,6 +464,8 @@ struct Oracle {
             }
         }
         if (!recompute) return false;
+        // :374, serial at the first full run
+        #pragma omp parallel for num_threads(nFullRuns > 0 ? maxThreads() : 1)
         for (int p = 0; p < np; p++) {
             if (shape[p] == ORC_SHAPE_NONE) continue;
             computeBound(p);
@@ -524,7 +543,7 @@ struct Oracle {
         if (sa == sb) swap = id1 > id2;
         else swap = (sb == ORC_SHAPE_WALL || sb == ORC_SHAPE_FACET) && sa == ORC_SHAPE_SPHERE;
         if (swap) { std::swap(c.idA, c.idB); for (int k = 0; k < 3; k++) c.cd[k] = -c.cd[k]; }
-        makeLater.push_back(c);
+        makeLater[tid()].push_back(c);
     }
     /* handleBoundInversion, :63-98 */
     void handleBoundInversion(int id1, int id2, bool separating) {
@@ -534,7 +553,7 @@ struct Oracle {
         if (!overlap && !hasCon) return;
         if (overlap && hasCon) return;
         if (overlap) { if (!mayCollide(id1, id2)) return; int z[3] = {0, 0, 0}; makeContactLater(id1, id2, z); return; }
-        if (!C->isReal()) removeLater.push_back(key(id1, id2));
+        if (!C->isReal()) removeLater[tid()].push_back(key(id1, id2));
     }
     /* handleBoundInversionPeri, :836-874 */
     void handleBoundInversionPeri(int id1, int id2, bool separating) {
@@ -545,20 +564,60 @@ struct Oracle {
         if (!overlap && !hasCon) return;
         if (overlap && hasCon) return;
         if (overlap) { if (!mayCollide(id1, id2)) return; makeContactLater(id1, id2, periods); return; }
-        if (!C->isReal()) removeLater.push_back(key(id1, id2));
+        if (!C->isReal()) removeLater[tid()].push_back(key(id1, id2));
     }
-    /* insertionSort_part (serial), :162-196 */
-    void insertionSort(VecBounds& v, bool doCollide) {
-        for (long i = 1; i < v.size; i++) {
-            if (!(v[i - 1] > v[i])) continue;
+    /* insertionSort_part, :162-196: bounds [iBegin, iEnd) sorted from iStart on; a part that
+       starts inside its range (iStart != iBegin) stops at the first ordered couple */
+    void insertionSortPart(VecBounds& v, bool doCollide, long iBegin, long iEnd, long iStart) {
+        const bool earlyStop = iBegin != iStart;
+        for (long i = std::max(iStart, iBegin + 1); i < iEnd; i++) {
+            if (!(v[i - 1] > v[i])) {
+                if (earlyStop) return;
+                continue;
+            }
             const Bounds viInit = v[i]; long j = i - 1;
-            while (j >= 0 && v[j] > viInit) {
+            while (j >= iBegin && v[j] > viInit) {
                 v[j + 1] = v[j];
                 if (viInit.isMin != v[j].isMin && doCollide && viInit.hasBB && v[j].hasBB && viInit.id != v[j].id)
                     handleBoundInversion(viInit.id, v[j].id, !viInit.isMin);
                 j--;
             }
             v[j + 1] = viInit;
+        }
+    }
+    /* InsertionSortCollider::insertionSort, :100-158, with the default ompTuneSort (1, 1000, 0)
+       (InsertionSortCollider.hpp:263): one chunk per thread, at least 2000 bounds per chunk;
+       even passes sort the chunks between splits0, odd passes those between splits1 (half a
+       chunk further), each part from its split on, until every inner split is ordered.  Each
+       inverted couple is met once, whichever part meets it, so the contacts made and removed
+       are those of the serial sort. */
+    void insertionSort(VecBounds& v, bool doCollide) {
+        const int TH = maxThreads();
+        const long minPerCore = 2 * 1000;
+        size_t chunks = (size_t)TH;
+        if ((long)(v.size / (long)chunks) < minPerCore)
+            chunks = (size_t)TH * (size_t)std::ceil(std::max(1L, v.size / minPerCore) * 1. / TH);
+        const size_t chunkSize = v.size / chunks;
+        if (chunks == 1 || chunkSize < 100) { insertionSortPart(v, doCollide, 0, v.size, 0); return; }
+        sortChunks = (long)chunks;
+        std::vector<size_t> splits0(chunks + 1), splits1(chunks);
+        for (size_t i = 0; i < chunks; i++) { splits0[i] = i * chunkSize; splits1[i] = i * chunkSize + chunkSize / 2; }
+        splits0[chunks] = v.size;
+        bool isOk = false;
+        for (size_t pass = 0; !isOk; pass++) {
+            const bool even = (pass % 2) == 0;
+            const std::vector<size_t>& s = even ? splits0 : splits1;
+            const long nPart = (long)(even ? chunks : chunks - 1);
+            #pragma omp parallel for schedule(static)
+            for (long chunk = 0; chunk < nPart; chunk++) {
+                const size_t start = pass == 0 ? s[chunk] : (even ? splits1[chunk] : splits0[chunk + 1]);
+                insertionSortPart(v, doCollide, (long)s[chunk], (long)s[chunk + 1], (long)start);
+            }
+            isOk = true;
+            for (size_t chunk = even ? 1 : 0; chunk < chunks; chunk++) {
+                if (v[(long)s[chunk] - 1] > v[(long)s[chunk]]) { isOk = false; break; }
+            }
+            sortPasses = std::max(sortPasses, (long)pass + 1);
         }
     }
     /* insertionSortPeri_part (serial, partial=false), :718-791 */
@@ -597,14 +656,18 @@ struct Oracle {
     }
     /* makeRemoveContactLater_process, :20-48 */
     void makeRemoveLaterProcess() {
-        for (uint64_t k : removeLater) { if (con.erase(k)) adjDirty = true; }
-        removeLater.clear();
-        for (const Contact& c : makeLater) {
-            uint64_t k = key(c.idA, c.idB);
-            if (con.count(k)) continue;
-            con.emplace(k, c); adjDirty = true;
+        for (auto& l : removeLater) {
+            for (uint64_t k : l) { if (con.erase(k)) adjDirty = true; }
+            l.clear();
         }
-        makeLater.clear();
+        for (auto& l : makeLater) {
+            for (const Contact& c : l) {
+                uint64_t k = key(c.idA, c.idB);
+                if (con.count(k)) continue;
+                con.emplace(k, c); adjDirty = true;
+            }
+            l.clear();
+        }
     }
 
     /* InsertionSortCollider::run, :445-646 */
@@ -617,6 +680,7 @@ struct Oracle {
         bool runBboxes = updateBboxes_doFullRun();
         if (runBboxes) fullRun = true;
         if (!fullRun) { removePending(); return; }
+        if ((int)makeLater.size() < maxThreads()) { makeLater.resize(maxThreads()); removeLater.resize(maxThreads()); }
         nFullRuns++;
         long nPar = np;
         bool doInitSort = false;
@@ -635,7 +699,8 @@ struct Oracle {
         }
         minima.assign(3 * nPar, 0.); maxima.assign(3 * nPar, 0.);
         if (periodic) for (int i = 0; i < 3; i++) BB[i].cellDim = cellSize[i];
-        // copy bounds, :516-548
+        // copy bounds, :514-548
+        #pragma omp parallel for schedule(static)
         for (long i = 0; i < 2 * nPar; i++) {
             for (int j = 0; j < 3; j++) {
                 VecBounds& BBj = BB[j];
@@ -1489,6 +1554,11 @@ int orc_set_params(void* h, const orc_params* p) {
 }
 int orc_step(void* h, int32_t n) {
     Oracle* o = O(h);
+#ifdef _OPENMP
+    // the thread count is the OpenMP runtime's, shared by every oracle of the process: each
+    // oracle sets its own before it steps
+    if (o->P.nThreads > 0) omp_set_num_threads(o->P.nThreads);
+#endif
     for (int i = 0; i < o->nn; i++) {
         if (((o->flags[i] & CLUMP_CLUMP) && o->nodeClump[i] < 0) || ((o->flags[i] & CLUMP_CLUMPED) && o->nodeMaster[i] < 0)) {
             o->err = "clump flags set but clumps not given (orc_set_clumps)";
@@ -1609,6 +1679,12 @@ int orc_get_stats(void* h, int64_t* step, int64_t* nFullRuns, double* verletDist
     if (step) *step = o->step;
     if (nFullRuns) *nFullRuns = o->nFullRuns;
     if (verletDist) *verletDist = o->verletDistActual;
+    return 0;
+}
+int orc_get_sort_stats(void* h, int64_t* sortChunks, int64_t* sortPasses) {
+    Oracle* o = O(h);
+    if (sortChunks) *sortChunks = o->sortChunks;
+    if (sortPasses) *sortPasses = o->sortPasses;
     return 0;
 }
 int orc_get_cell(void* h, double* hSize9, double* gradV9) {

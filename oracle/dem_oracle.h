@@ -120,6 +120,9 @@ int orc_set_time(void* h, int64_t step, double time);
 int orc_get_energy(void* h, double* out, int32_t* created);
 /* stats: step, nFullRuns, verletDist(actual) */
 int orc_get_stats(void* h, int64_t* step, int64_t* nFullRuns, double* verletDist);
+/* chunks of the last parallel (chunked) insertion sort, -1 if every sort so far was serial,
+   and the most passes one sort needed */
+int orc_get_sort_stats(void* h, int64_t* sortChunks, int64_t* sortPasses);
 /* current cell hSize (9, row-major) */
 int orc_get_cell(void* h, double* hSize9, double* gradV9);
 

@@ -71,6 +71,7 @@ def lib():
         "orc_get_contacts": (C.c_int, [P, I64, pi, pi, pi, C.POINTER(C.c_uint8), pd, pd, pd, pd, pl]),
         "orc_get_energy": (C.c_int, [P, pd, pi]),
         "orc_get_stats": (C.c_int, [P, pl, pl, pd]),
+        "orc_get_sort_stats": (C.c_int, [P, pl, pl]),
         "orc_get_cell": (C.c_int, [P, pd, pd]),
         "orc_bruteforce_pairs": (I64, [P, I64, pi, pi]),
         "orc_pwave_dt": (D, [P]),
@@ -204,6 +205,11 @@ class OracleRun:
         s, f, vd = C.c_int64(), C.c_int64(), C.c_double()
         self._chk(self.L.orc_get_stats(self.h, C.byref(s), C.byref(f), C.byref(vd)))
         return dict(step=s.value, nFullRuns=f.value, verletDist=vd.value)
+
+    def sort_stats(self):
+        ch, pa = C.c_int64(), C.c_int64()
+        self._chk(self.L.orc_get_sort_stats(self.h, C.byref(ch), C.byref(pa)))
+        return dict(sortChunks=ch.value, sortPasses=pa.value)
 
     def cell(self):
         h, g = np.zeros(9), np.zeros(9)

@@ -382,3 +382,38 @@ def test_sliding_sphere_rolls_at_five_sevenths():
     assert v == pytest.approx(5 / 7. * v0, rel=.01)
     assert w * .01 == pytest.approx(v, rel=.01)
     assert o.contacts()["isReal"][0]
+
+
+def test_chunked_parallel_insertion_sort_equals_serial():
+    """InsertionSortCollider::insertionSort (InsertionSortCollider.cpp:100-158) sorts an
+    aperiodic scene's bounds in chunks on every OpenMP thread, each inverted couple being met
+    by one part; the contacts it makes and removes, hence the whole trajectory, equal the
+    serial sort's.  A 4,000-sphere box settling under gravity (8,000 bounds per axis, 8 chunks
+    of 1,000) runs on 8 threads and on 1, with full runs by insertion sort in between."""
+    a = scenes.box_lattice(nx=20, ny=20, nz=10, jitter=1e-3)
+    p = runner.params_from_meta(a)
+    cell = runner.cell_from_meta(a)
+    runs = {nt: OL.OracleRun(a, p, cell, nThreads=nt) for nt in (1, 8)}
+    for _ in range(4):
+        for o in runs.values():
+            o.step(400)
+        s1, s8 = runs[1].stats(), runs[8].stats()
+        assert s1 == s8
+        k1, k8 = OL.pair_keys(runs[1].contacts()), OL.pair_keys(runs[8].contacts())
+        assert np.array_equal(k1, k8)
+        n1, n8 = runs[1].nodes(), runs[8].nodes()
+        for q in ("pos", "vel", "angVel", "force", "torque"):
+            assert np.array_equal(n1[q], n8[q]), q
+    assert runs[8].stats()["nFullRuns"] >= 3
+    assert runs[8].sort_stats()["sortChunks"] == 8
+    assert runs[1].sort_stats()["sortChunks"] == -1
+    # the chunked sort's contact set: every pair with overlapping bounds, and real contacts
+    lo, hi = runs[8].bruteforce_pairs()
+    bf = np.sort((lo.astype(np.int64) << 32) | hi)
+    c = runs[8].contacts()
+    k = OL.pair_keys(c)
+    assert np.isin(bf, k).all()
+    real = OL.pair_keys({"idA": c["idA"][c["isReal"]], "idB": c["idB"][c["isReal"]]})
+    assert np.isin(k[~np.isin(k, bf)], real).all()
+    for o in runs.values():
+        o.close()
