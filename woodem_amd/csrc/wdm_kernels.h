@@ -302,6 +302,7 @@ struct FRScene {
     int32_t nLvl;
     int32_t nCells;      // grid cells of the scene; cell nCells lists its large bounds
     int32_t largePairs;  // large bounds may pair with each other (masks allow it)
+    int32_t largeListed; // the pass lists large bounds in the cells they cover (k_large_cells)
     int32_t desc;
 };
 
@@ -1181,9 +1182,11 @@ __device__ __forceinline__ float4 cellBox(const FRScene& F, const double4& a, co
     return make_float4((float)r[0], (float)r[1], (float)r[2], (float)h);
 }
 // Bound entries by cell: key = scene's first cell + cell of the bound's minimum corner at
-// its level (or the scene's large list); the count per key gives each entry its slot.
+// its level (or the scene's large list); the count per key gives each entry its slot.  A
+// large bound in a pass that does not list large bounds raises *planFlag (its grid was laid
+// out for smaller bounds).
 __global__ void __launch_bounds__(WDM_BLOCK) k_cell_count(const SceneDesc* D, const FRScene* FR, Map m, uint32_t* ekey, uint32_t* eslot,
-                                                         uint32_t* cnt) {
+                                                         uint32_t* cnt, uint32_t* planFlag) {
     int64_t lb;
     const FRScene& F = FR[mapBlock(m, lb)];
     const SceneDesc& S = D[F.desc];
@@ -1196,6 +1199,8 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_cell_count(const SceneDesc* D, co
     if (l < F.nLvl) {
         const int cx = cellCoordL(F, l, 0, a.x), cy = cellCoordL(F, l, 1, a.y), cz = cellCoordL(F, l, 2, a.z);
         cell = F.lbase[l] + ((int64_t)cz * F.ln[l][1] + cy) * F.ln[l][0] + cx;
+    } else if (planFlag && !F.largeListed) {
+        atomicOr(planFlag, 1u);
     }
     const uint32_t key = (uint32_t)(F.cellBase + cell);
     ekey[F.binStart + i] = key;
@@ -1445,12 +1450,12 @@ __device__ __forceinline__ bool pairTest(const SceneDesc& S, const GridArgs& g, 
 // region; it carries no FP64 state, so many wavefronts stay resident to cover the loads.
 // k_pair_exact then runs the exact test over each region with every lane busy, and tests
 // its entries against the walls and the listed large bounds.
-// Output of k_pair_exact: each accepted pair counts in rowCnt[rowBase + min id], whose
-// returned value is the pair's slot in that row, and goes to the wavefront's own region
-// (item max id << 32 | packed periods, row, slot).  A wavefront w owns entries
-// [w cap, (w+1) cap) of a region, with their number in cnt[w]; a count above cap (the
-// pairs past it are not written) raises *wmax, and the host runs the passes again with
-// larger regions.
+// Output of k_pair_exact: each accepted pair goes to the wavefront's own region (item max
+// id << 32 | packed periods, row, slot) and counts in rowCnt[rowBase + min id], whose
+// returned value is its slot in that row.  A wavefront w owns entries [w cap, (w+1) cap)
+// of a region, with their number in cnt[w]; a count above cap (the pairs past it are
+// neither written nor counted in their rows, so the rows hold at most nw cap pairs)
+// raises *wmax, and the host runs the passes again with larger regions.
 __device__ __forceinline__ bool stencilCellL(const FRScene& F, int l, int cx, int cy, int cz, int q, int& cell) {
     // half-stencil offset q: (0,0,0), (1,0,0), (-1..1,1,0), (-1..1,-1..1,1)
     int dx, dy, dz;
@@ -1657,8 +1662,8 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pair_exact(const SceneDesc* D, co
             const int k = cntW + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
             const int lo = a < b ? a : b, hi = a < b ? b : a;
             const uint32_t row = (uint32_t)(F.rowBase + lo);
-            const uint32_t slot = atomicAdd(&rowCnt[row], 1u);
-            if (k < wcap) {
+            if (k < wcap) {  // only stored pairs are counted: the rows never outgrow the regions
+                const uint32_t slot = atomicAdd(&rowCnt[row], 1u);
                 rk[k] = ((uint64_t)(uint32_t)hi << 32) | (uint32_t)packCd(per[0], per[1], per[2]);
                 rr[k] = row;
                 rs[k] = slot;
@@ -1718,12 +1723,13 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pair_exact(const SceneDesc* D, co
         if (cntW > wcap) atomicMax(wmax, cntW);
     }
 }
-// each pair of the wavefront regions at its row's offset plus its slot (rows by min id)
+// each pair of the wavefront regions at its row's offset plus its slot (rows by min id); a
+// region that overflowed holds its first wcap pairs (the host runs the pass again)
 __global__ void __launch_bounds__(WDM_BLOCK) k_cand_scatter(const int32_t* wcnt, int64_t nw, int32_t wcap, const uint64_t* rItem,
                                                            const uint32_t* rRow, const uint32_t* rSlot, const int64_t* rowOff, BItem* tmp) {
     const int64_t w = (int64_t)blockIdx.x * (WDM_BLOCK / 64) + (threadIdx.x >> 6);
     if (w >= nw) return;
-    const int n = wcnt[w];
+    const int n = min(wcnt[w], wcap);
     for (int k = threadIdx.x & 63; k < n; k += 64) {
         const int64_t i = w * (int64_t)wcap + k;
         const uint32_t row = rRow[i];

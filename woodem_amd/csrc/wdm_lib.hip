@@ -262,6 +262,10 @@ struct Shared {
     Staged stRunA, stRunO, stRunN;  // collider full run: the tables of its phases
     void* pin = nullptr;            // pinned readback memory
     size_t pinCap = 0;
+    // pinned copy of a full run's bound extents and histograms, read by the host after the
+    // run's pair-count synchronisation: the grids of the scenes' next full run
+    uint8_t* extMail = nullptr;
+    size_t extMailCap = 0;
     hipEvent_t evWords = nullptr;  // recorded after the copy of the words to the host
     // energy partials per block
     DBuf<double> epartL, epartC;
@@ -348,6 +352,9 @@ struct Shared {
         if (pin) (void)hipHostFree(pin);
         pin = nullptr;
         pinCap = 0;
+        if (extMail) (void)hipHostFree(extMail);
+        extMail = nullptr;
+        extMailCap = 0;
         if (evWords) (void)hipEventDestroy(evWords);
         evWords = nullptr;
     }
@@ -395,6 +402,12 @@ struct Scene {
     double verletDist = 0;
     bool verletResolved = false;
     bool needFull = true;
+    // bound extents of the last full run (min corner, max corner, max extent, max sphere
+    // extent) and their histogram: the next full run lays its grid out from them without
+    // waiting for its own (extValid: set by a full run, cleared by anything that needs one)
+    bool extValid = false;
+    double extLast[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<uint32_t> histLast;
     bool lastLeapfrogVerletChecked = false;
     bool forcesFromContacts = false;  // node forces are the sums of the last ContactLoop (not materialised)
     // a Leapfrog ran after the last ContactLoop: the node forces are its reset values (m g, no
@@ -585,6 +598,26 @@ std::vector<int64_t> segmentStarts(Ctx* c, const int64_t* scan, const FRScene* F
     std::memcpy(out.data(), h, sb);
     if (extraBytes) std::memcpy(extraOut, h + sb, extraBytes);
     return out;
+}
+// segmentStarts of two prefixes (scan0 at each scene's rowBase, scan1 at its oldStart; last0,
+// last1 their ends) and `extraBytes` at `extra`, with one synchronisation
+void segmentStarts2(Ctx* c, const FRScene* FRd, int nT, const int64_t* scan0, int64_t last0, const int64_t* scan1, int64_t last1,
+                    std::vector<int64_t>& out0, std::vector<int64_t>& out1, const void* extra, void* extraOut, size_t extraBytes) {
+    Shared& X = *c->sh;
+    ensure(c, X.starts, 2 * ((size_t)nT + 1));
+    hipLaunchKernelGGL(k_seg_starts, dim3((nT + 64) / 64), dim3(64), 0, c->stream, FRd, nT, 0, last0, scan0, X.starts.p);
+    if (scan1) hipLaunchKernelGGL(k_seg_starts, dim3((nT + 64) / 64), dim3(64), 0, c->stream, FRd, nT, 1, last1, scan1, X.starts.p + nT + 1);
+    checkLaunch();
+    const size_t sb = ((size_t)nT + 1) * sizeof(int64_t);
+    uint8_t* h = (uint8_t*)pinnedRead(c, 2 * sb + extraBytes);
+    HIPCHK(hipMemcpyAsync(h, X.starts.p, scan1 ? 2 * sb : sb, hipMemcpyDeviceToHost, c->stream));
+    if (extraBytes) HIPCHK(hipMemcpyAsync(h + 2 * sb, extra, extraBytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    out0.assign(nT + 1, 0);
+    out1.assign(nT + 1, 0);
+    std::memcpy(out0.data(), h, sb);
+    if (scan1) std::memcpy(out1.data(), h + sb, sb);
+    if (extraBytes) std::memcpy(extraOut, h + 2 * sb, extraBytes);
 }
 // several host tables in one staged upload
 struct Upload {
@@ -1000,11 +1033,19 @@ struct InsertionSortCollider {
         F.nLvl = nL;
     }
 
-    // Grids of the scenes L[t] from their bound extents (one synchronisation), then the
-    // bound entries of all of them sorted by (scene, cell key, id) with each cell's entry
-    // range [cbeg[k], cbeg[k + 1]) (levels: 0 lets each scene choose).
+    // Grids of the scenes L[t] from their bound extents, then the bound entries of all of
+    // them sorted by (scene, cell key, id) with each cell's entry range [cbeg[k], cbeg[k + 1])
+    // (levels: 0 lets each scene choose).  The extents of this run are reduced on the device
+    // and copied to the pinned mailbox X.extMail.  lagged: the grids come from the extents of
+    // the scenes' previous full run (Scene::extLast) and nothing waits for this run's; a bound
+    // wider than its grid's top level in a scene whose large bounds are not listed
+    // (largeListed 0) would lose its pairs, so k_cell_count raises *planFlag for it and the
+    // caller lays the grids out again from this run's extents.  Otherwise the host waits for
+    // this run's extents (one synchronisation).  A bound outside a lagged aperiodic grid falls
+    // into its edge cells (cell coordinates are clamped), which keeps every pair.
     static void buildGrids(Batch& B, const std::vector<int>& L, std::vector<FRScene>& F, const std::vector<int64_t>& bpB, const Map& mB,
-                           const FRScene*& FRd, int64_t& nbTot, double cellFactor, int levels, bool entryCopies = true) {
+                           const FRScene*& FRd, int64_t& nbTot, double cellFactor, int levels, bool entryCopies = true,
+                           bool lagged = false, uint32_t* planFlag = nullptr, int largeListed = 0) {
         Ctx* c = B.c;
         Shared& X = *c->sh;
         hipStream_t st = c->stream;
@@ -1013,6 +1054,16 @@ struct InsertionSortCollider {
         ensure(c, X.extPart, (size_t)nblkB * 8 + 8);
         ensure(c, X.extOut, (size_t)nT * 8);
         ensure(c, X.hist, (size_t)nT * EXT_BINS);
+        const size_t eb = (size_t)nT * 8 * sizeof(double), hb = (size_t)nT * EXT_BINS * sizeof(uint32_t);
+        if (X.extMailCap < eb + hb) {  // grown before this run's copy (the stream is drained first)
+            HIPCHK(hipStreamSynchronize(st));
+            if (X.extMail) (void)hipHostFree(X.extMail);
+            X.extMail = nullptr;
+            X.extMailCap = 0;
+            const size_t cap = std::max<size_t>(2 * (eb + hb), 4096);
+            HIPCHK(hipHostMalloc((void**)&X.extMail, cap, hipHostMallocDefault));
+            X.extMailCap = cap;
+        }
         fillU32(c, X.hist.p, (int64_t)nT * EXT_BINS);
         if (nblkB > 0) {
             hipLaunchKernelGGL(k_extent, dim3((unsigned)nblkB), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, X.extPart.p, X.hist.p);
@@ -1020,11 +1071,21 @@ struct InsertionSortCollider {
         }
         hipLaunchKernelGGL(k_extent_finalize, dim3(nT), dim3(WDM_BLOCK), 0, st, mB, (const double*)X.extPart.p, X.extOut.p);
         checkLaunch();
+        HIPCHK(hipMemcpyAsync(X.extMail, X.extOut.p, eb, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(X.extMail + eb, X.hist.p, hb, hipMemcpyDeviceToHost, st));
         std::vector<double> h((size_t)nT * 8);
         std::vector<uint32_t> hh((size_t)nT * EXT_BINS);
-        HIPCHK(hipMemcpyAsync(h.data(), X.extOut.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(hh.data(), X.hist.p, hh.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-        HIPCHK(hipStreamSynchronize(st));
+        if (lagged) {
+            for (int t = 0; t < nT; t++) {
+                const Scene& S = *B.ss[L[t]];
+                std::memcpy(&h[(size_t)t * 8], S.extLast, sizeof(S.extLast));
+                std::memcpy(&hh[(size_t)t * EXT_BINS], S.histLast.data(), EXT_BINS * sizeof(uint32_t));
+            }
+        } else {
+            HIPCHK(hipStreamSynchronize(st));
+            std::memcpy(h.data(), X.extMail, eb);
+            std::memcpy(hh.data(), X.extMail + eb, hb);
+        }
         int64_t cellBase = 0;
         nbTot = 0;
         for (int t = 0; t < nT; t++) {
@@ -1061,6 +1122,7 @@ struct InsertionSortCollider {
             Ft.binStart = nbTot;
             Ft.nCells = (int32_t)nc;
             Ft.largePairs = S.facetMasksPair(S.P.loneMask) ? 1 : 0;
+            Ft.largeListed = largeListed;
             cellBase += nc + 1;  // + the list of large bounds
             nbTot += nb;
         }
@@ -1073,7 +1135,8 @@ struct InsertionSortCollider {
         ensure(c, X.ccnt, (size_t)G); ensure(c, X.cbeg, (size_t)G + 1);
         fillU32(c, X.ccnt.p, G);
         if (nblkB > 0) {
-            hipLaunchKernelGGL(k_cell_count, dim3((unsigned)nblkB), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, X.ekey.p, X.eslot.p, X.ccnt.p);
+            hipLaunchKernelGGL(k_cell_count, dim3((unsigned)nblkB), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, X.ekey.p, X.eslot.p, X.ccnt.p,
+                               planFlag);
             checkLaunch();
         }
         scanExclusive(c, (const uint32_t*)X.ccnt.p, X.cbeg.p, G);
@@ -1103,8 +1166,10 @@ struct InsertionSortCollider {
         checkLaunch();
         scanExclusive(c, (const uint32_t*)X.lcntE.p, X.loffE.p, nbTot);
         int64_t nl = 0;
-        HIPCHK(hipMemcpyAsync(&nl, X.loffE.p + nbTot, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        int64_t* pin = (int64_t*)pinnedRead(c, sizeof(int64_t));
+        HIPCHK(hipMemcpyAsync(pin, X.loffE.p + nbTot, sizeof(int64_t), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
+        nl = *pin;
         if (nl == 0) return false;
         if (nl >= INT32_MAX) throw WdmError(WDM_E_ARG, "too many large-bound cell entries");
         ensure(c, X.lkeys, (size_t)nl); ensure(c, X.lids, (size_t)nl); ensure(c, X.lslot, (size_t)nl); ensure(c, X.lvals, (size_t)nl);
@@ -1156,71 +1221,7 @@ struct InsertionSortCollider {
             hipLaunchKernelGGL(k_bounds, dim3((unsigned)bpP[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mP, 0);
             checkLaunch();
         }
-        int64_t nbTot = 0;
-        {
-            KTimer tg(c, "grid");
-            buildGrids(B, L, F, bpB, mB, FRd, nbTot, c->cellFactor, c->gridLevels);
-        }
-        bool anyFacets = false;
-        for (int t = 0; t < nT; t++) anyFacets |= S_(t).hasFacets;
-        const bool large = anyFacets && listLarge(B, bpB, mB, FRd, nbTot, F[nT - 1].cellBase + F[nT - 1].nCells + 1);
-
-        // candidate pairs: one thread per bound entry, each wavefront writing into its own
-        // region (the pass runs again with larger regions when a wavefront found more pairs
-        // than they hold) and counting the pairs per row (scene, min id); then the pairs
-        // bucket-sorted by row and max id, so that they come in (scene, contact key) order
-        int64_t ncandTot = 0;
-        {
-            const int64_t nBlk = bpB[nT];  // 4 wavefronts per block
-            const int64_t nW = 4 * nBlk;
-            ensure(c, X.wcnt, (size_t)std::max<int64_t>(nW, 1)); ensure(c, X.scnt, (size_t)std::max<int64_t>(nW, 1));
-            ensure(c, X.wmax, 2);
-            ensure(c, X.rowCnt, (size_t)nRows); ensure(c, X.rowOff, (size_t)nRows + 1);
-            std::vector<int64_t> cs;
-            auto grow = [](int32_t wm) { return (int32_t)std::min<int64_t>(((int64_t)wm + wm / 8 + 63) / 64 * 64, INT32_MAX / 2); };
-            for (int attempt = 0;; attempt++) {
-                const int32_t scap = c->scanCap, wcap = c->waveCap;
-                const size_t nreg = (size_t)std::max<int64_t>(nW * wcap, 1);
-                ensure(c, X.sPair, (size_t)std::max<int64_t>(nW * scap, 1));
-                ensure(c, X.rItem, nreg); ensure(c, X.rRow, nreg); ensure(c, X.rSlot, nreg);
-                fillU32(c, (uint32_t*)X.wmax.p, 2);
-                fillU32(c, X.rowCnt.p, nRows);
-                if (nW > 0) {
-                    KTimer tm(c, "pairs");
-                    const int64_t pad = 8 * PAIR_CHUNK;
-                    const dim3 grid((unsigned)((nBlk + pad - 1) / pad * pad));
-                    hipLaunchKernelGGL(k_pair_scan, grid, dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
-                                       (const int32_t*)X.cbeg.p, (const float4*)X.cbox.p, X.scnt.p, X.wmax.p, scap, X.sPair.p);
-                    hipLaunchKernelGGL(k_pair_exact, grid, dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const int32_t*)X.cvals2.p,
-                                       (const uint32_t*)X.ckeys2.p, (const double4*)X.eb0.p, (const double4*)X.eb1.p, (const int2*)X.einfo.p,
-                                       large ? (const int32_t*)X.lbeg.p : nullptr, (const int32_t*)X.lvals.p,
-                                       (const int32_t*)X.scnt.p, scap, (const int2*)X.sPair.p, X.wcnt.p, X.wmax.p + 1, wcap, X.rItem.p,
-                                       X.rRow.p, X.rSlot.p, X.rowCnt.p);
-                    checkLaunch();
-                }
-                scanExclusive(c, (const uint32_t*)X.rowCnt.p, X.rowOff.p, nRows);
-                int32_t wm[2] = {0, 0};
-                cs = segmentStarts(c, X.rowOff.p, FRd, nT, 0, nRows, X.wmax.p, wm, sizeof(wm));  // synchronises
-                if (wm[0] <= scap && wm[1] <= wcap) break;
-                if (attempt > 1) throw WdmError(WDM_E_STATE, "collider: pairs per wavefront changed between passes");
-                if (wm[0] > scap) c->scanCap = grow(wm[0]);
-                if (wm[1] > wcap) c->waveCap = grow(wm[1]);
-            }
-            for (int t = 0; t < nT; t++) { F[t].candStart = cs[t]; F[t].nCand = cs[t + 1] - cs[t]; }
-            ncandTot = cs[nT];
-            const size_t nc1 = (size_t)std::max<int64_t>(ncandTot, 1);
-            ensure(c, X.candK, nc1); ensure(c, X.candP, nc1); ensure(c, X.btmp, nc1);
-            if (ncandTot > 0) {
-                KTimer tm(c, "pairs_sort");
-                hipLaunchKernelGGL(k_cand_scatter, dim3((unsigned)((nW + 3) / 4)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.wcnt.p, nW,
-                                   c->waveCap, (const uint64_t*)X.rItem.p, (const uint32_t*)X.rRow.p, (const uint32_t*)X.rSlot.p,
-                                   (const int64_t*)X.rowOff.p, X.btmp.p);
-                checkLaunch();
-                bucketFinish(c, ncandTot, (const int64_t*)X.rowOff.p, CandOut{FRd, nT, X.candK.p, X.candP.p});
-            }
-        }
-
-        // old contacts kept only because they are real
+        // old contact lists (kept only because they are real, or matched by a candidate)
         std::vector<ConDesc<T>> C(nT);
         std::memset(C.data(), 0, C.size() * sizeof(ConDesc<T>));
         int64_t nOldTot = 0;
@@ -1237,31 +1238,123 @@ struct InsertionSortCollider {
         }
         std::vector<int64_t> bpO = blockPrefix(nT, [&](int t) { return F[t].nOld; });
         std::vector<int64_t> bp1 = blockPrefix(nT, [&](int t) { return (int64_t)S_(t).np + 1; });
-        Upload uO;
-        const size_t oFO = uO.add(F), oCO = uO.add(C), oO = uO.add(bpO), o1 = uO.add(bp1);
-        uO.send(c, X.stRunO);
-        FRd = uO.at<FRScene>(oFO);
-        const ConDesc<T>* Cd = uO.at<ConDesc<T>>(oCO);
-        const Map mO = uO.map(oO, nT);
-        const Map m1 = uO.map(o1, nT);  // per scene its rows (np + 1 threads)
         ensure(c, X.ooFlag, (size_t)std::max<int64_t>(nOldTot, 1));
         ensure(c, X.ooRank, (size_t)nOldTot + 1);
         ensure(c, X.oldRow, (size_t)nRows + 1);
+        bool anyFacets = false;
+        for (int t = 0; t < nT; t++) anyFacets |= S_(t).hasFacets;
+        // the grids come from the extents of the scenes' last full run when every scene has
+        // them (no synchronisation); wmax[2] reports a grid that was laid out too small
+        bool lagged = true;
+        for (int t = 0; t < nT; t++) lagged &= S_(t).extValid && !S_(t).needFull;
+        ensure(c, X.wmax, 4);
+        fillU32(c, (uint32_t*)X.wmax.p, 3);
+
+        // candidate pairs: one thread per bound entry, each wavefront writing into its own
+        // region and counting its stored pairs per row (scene, min id); the pairs are then
+        // bucket-sorted by row and max id, so that they come in (scene, contact key) order,
+        // and the old contacts that no candidate matches are flagged, before the host reads
+        // the counts (one synchronisation); a wavefront that found more pairs than its region
+        // holds makes the host run it all again with larger regions, and a lagged grid laid
+        // out for smaller bounds makes it lay the grids out again from this run's extents
+        int64_t ncandTot = 0;
         std::vector<int64_t> noor(nT, 0);
-        {
-            KTimer to(c, "old_flags");
-            hipLaunchKernelGGL(k_old_rows<T>, dim3((unsigned)bp1[nT]), dim3(WDM_BLOCK), 0, st, B.D, Cd, FRd, m1, X.oldRow.p);
-            checkLaunch();
-            if (nOldTot > 0) {
-                hipLaunchKernelGGL(k_flag_old<T>, dim3((unsigned)bpO[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mO, (const int64_t*)X.rowOff.p,
-                                   (const uint64_t*)X.candK.p, X.ooFlag.p);
+        const ConDesc<T>* Cd = nullptr;
+        Map mO, m1;
+        int64_t nbTot = 0;
+        for (int attempt = 0;; attempt++) {
+            {
+                KTimer tg(c, "grid");
+                buildGrids(B, L, F, bpB, mB, FRd, nbTot, c->cellFactor, c->gridLevels, true, lagged, (uint32_t*)X.wmax.p + 2,
+                           anyFacets ? 1 : 0);
+            }
+            const bool large = anyFacets && listLarge(B, bpB, mB, FRd, nbTot, F[nT - 1].cellBase + F[nT - 1].nCells + 1);
+            Upload uO;
+            const size_t oFO = uO.add(F), oCO = uO.add(C), oO = uO.add(bpO), o1 = uO.add(bp1);
+            uO.send(c, X.stRunO);
+            FRd = uO.at<FRScene>(oFO);
+            Cd = uO.at<ConDesc<T>>(oCO);
+            mO = uO.map(oO, nT);
+            m1 = uO.map(o1, nT);  // per scene its rows (np + 1 threads)
+            {
+                KTimer to(c, "old_rows");
+                hipLaunchKernelGGL(k_old_rows<T>, dim3((unsigned)bp1[nT]), dim3(WDM_BLOCK), 0, st, B.D, Cd, FRd, m1, X.oldRow.p);
                 checkLaunch();
             }
-            scanExclusive(c, (const uint8_t*)X.ooFlag.p, X.ooRank.p, nOldTot);
-            if (nOldTot > 0) {
-                std::vector<int64_t> os = segmentStarts(c, X.ooRank.p, FRd, nT, 1, nOldTot);
-                for (int t = 0; t < nT; t++) noor[t] = os[t + 1] - os[t];
+            const int64_t nBlk = bpB[nT];  // 4 wavefronts per block
+            const int64_t nW = 4 * nBlk;
+            ensure(c, X.wcnt, (size_t)std::max<int64_t>(nW, 1)); ensure(c, X.scnt, (size_t)std::max<int64_t>(nW, 1));
+            ensure(c, X.rowCnt, (size_t)nRows); ensure(c, X.rowOff, (size_t)nRows + 1);
+            std::vector<int64_t> cs, os;
+            auto grow = [](int32_t wm) { return (int32_t)std::min<int64_t>(((int64_t)wm + wm / 8 + 63) / 64 * 64, INT32_MAX / 2); };
+            const int32_t scap = c->scanCap, wcap = c->waveCap;
+            const size_t nreg = (size_t)std::max<int64_t>(nW * wcap, 1);
+            ensure(c, X.sPair, (size_t)std::max<int64_t>(nW * scap, 1));
+            ensure(c, X.rItem, nreg); ensure(c, X.rRow, nreg); ensure(c, X.rSlot, nreg);
+            ensure(c, X.candK, nreg); ensure(c, X.candP, nreg); ensure(c, X.btmp, nreg);
+            if (attempt > 0) fillU32(c, (uint32_t*)X.wmax.p, 2);
+            fillU32(c, X.rowCnt.p, nRows);
+            if (nW > 0) {
+                KTimer tm(c, "pairs");
+                const int64_t pad = 8 * PAIR_CHUNK;
+                const dim3 grid((unsigned)((nBlk + pad - 1) / pad * pad));
+                hipLaunchKernelGGL(k_pair_scan, grid, dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
+                                   (const int32_t*)X.cbeg.p, (const float4*)X.cbox.p, X.scnt.p, X.wmax.p, scap, X.sPair.p);
+                hipLaunchKernelGGL(k_pair_exact, grid, dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const int32_t*)X.cvals2.p,
+                                   (const uint32_t*)X.ckeys2.p, (const double4*)X.eb0.p, (const double4*)X.eb1.p, (const int2*)X.einfo.p,
+                                   large ? (const int32_t*)X.lbeg.p : nullptr, (const int32_t*)X.lvals.p,
+                                   (const int32_t*)X.scnt.p, scap, (const int2*)X.sPair.p, X.wcnt.p, X.wmax.p + 1, wcap, X.rItem.p,
+                                   X.rRow.p, X.rSlot.p, X.rowCnt.p);
+                checkLaunch();
             }
+            scanExclusive(c, (const uint32_t*)X.rowCnt.p, X.rowOff.p, nRows);
+            if (nW > 0) {
+                KTimer tm(c, "pairs_sort");
+                hipLaunchKernelGGL(k_cand_scatter, dim3((unsigned)((nW + 3) / 4)), dim3(WDM_BLOCK), 0, st, (const int32_t*)X.wcnt.p, nW,
+                                   wcap, (const uint64_t*)X.rItem.p, (const uint32_t*)X.rRow.p, (const uint32_t*)X.rSlot.p,
+                                   (const int64_t*)X.rowOff.p, X.btmp.p);
+                checkLaunch();
+                bucketFinish(c, (int64_t)nreg, (const int64_t*)X.rowOff.p, CandOut{FRd, nT, X.candK.p, X.candP.p},
+                             (const int64_t*)X.rowOff.p + nRows);
+            }
+            {
+                KTimer to(c, "old_flags");
+                if (nOldTot > 0) {
+                    hipLaunchKernelGGL(k_flag_old<T>, dim3((unsigned)bpO[nT]), dim3(WDM_BLOCK), 0, st, Cd, FRd, mO,
+                                       (const int64_t*)X.rowOff.p, (const uint64_t*)X.candK.p, X.ooFlag.p);
+                    checkLaunch();
+                }
+                scanExclusive(c, (const uint8_t*)X.ooFlag.p, X.ooRank.p, nOldTot);
+            }
+            int32_t wm[3] = {0, 0, 0};
+            segmentStarts2(c, FRd, nT, X.rowOff.p, nRows, nOldTot > 0 ? X.ooRank.p : nullptr, nOldTot, cs, os, X.wmax.p, wm,
+                           sizeof(wm));  // synchronises
+            if (wm[2] != 0) {  // a lagged grid too small for a bound: lay the grids out from this run's extents
+                if (!lagged) throw WdmError(WDM_E_STATE, "collider: bound wider than its grid");
+                lagged = false;
+                fillU32(c, (uint32_t*)X.wmax.p, 3);
+                continue;
+            }
+            if (wm[0] > scap || wm[1] > wcap) {
+                if (attempt > 2) throw WdmError(WDM_E_STATE, "collider: pairs per wavefront changed between passes");
+                if (wm[0] > scap) c->scanCap = grow(wm[0]);
+                if (wm[1] > wcap) c->waveCap = grow(wm[1]);
+                continue;
+            }
+            for (int t = 0; t < nT; t++) { F[t].candStart = cs[t]; F[t].nCand = cs[t + 1] - cs[t]; }
+            ncandTot = cs[nT];
+            if (nOldTot > 0)
+                for (int t = 0; t < nT; t++) noor[t] = os[t + 1] - os[t];
+            // this run's extents (in the mailbox since the synchronisation) lay out the next run's grids
+            const size_t eb = (size_t)nT * 8 * sizeof(double);
+            for (int t = 0; t < nT; t++) {
+                Scene& S = S_(t);
+                std::memcpy(S.extLast, X.extMail + (size_t)t * 8 * sizeof(double), sizeof(S.extLast));
+                S.histLast.resize(EXT_BINS);
+                std::memcpy(S.histLast.data(), X.extMail + eb + (size_t)t * EXT_BINS * sizeof(uint32_t), EXT_BINS * sizeof(uint32_t));
+                S.extValid = true;
+            }
+            break;
         }
 
         // merged lists
@@ -1621,8 +1714,10 @@ struct DynDtEngine {
                 else { if (lw) launchGroup<double, 1>(B, L, o); else launchGroup<double, 0>(B, L, o); }
             }
         std::vector<unsigned long long> h(n);
-        HIPCHK(hipMemcpyAsync(h.data(), X.ddOut.p, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
+        unsigned long long* pin = (unsigned long long*)pinnedRead(c, n * sizeof(unsigned long long));
+        HIPCHK(hipMemcpyAsync(pin, X.ddOut.p, n * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
+        std::memcpy(h.data(), pin, n * sizeof(unsigned long long));
         for (int i = 0; i < n; i++) {
             if (slotOf[i] < 0) continue;
             double v;
