@@ -225,6 +225,27 @@ def measured_traffic(args, n_real, n_active):
     return t["hbm_read_bytes_per_launch"] + t["hbm_write_bytes_per_launch"], "profiles/" + t.get("source", "")
 
 
+def hbm_copy_rate(nbytes=1 << 30, reps=10):
+    """A plain device-to-device copy of nbytes (torch, on its own stream), timed with events:
+    the HBM rate a streaming kernel reaches on this device, beside the nominal peak."""
+    import torch
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    gbps = 2 * nbytes / (ms * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return {"bytes_moved_per_copy": 2 * nbytes, "ms_per_copy": ms, "GBps": gbps, "frac_of_peak": gbps / HBM_PEAK_GBS}
+
+
 def timed_kernels(ctx, sids, steps):
     """HIP-event kernel times over `steps` steps launched one at a time (window 1), so that
     every launch of a step is counted once."""
@@ -544,24 +565,34 @@ def bench_periodic(args, ctx):
 
     mixed = None
     if not args.no_mixed and args.precision == "fp64" and sus is not None:
+        # the mixed-precision scene over the same steps as each FP64 figure: the timed steps
+        # of the headline, the kernel-by-kernel steps of the roofline and the sustained steps
         pm = runner.params_from_meta(b, precision=PREC_MIXED)
         rm = runner.HipRun(b, pm, runner.cell_from_meta(b), ctx=ctx)
-        # to the step where the FP64 sustained leg started (on its own trajectory), then the same steps timed
-        rm.step(args.warmup + args.steps + args.latency_steps + args.timing_steps)
+        rm.step(args.warmup)
+        ctx.sync()
+        tm = time.perf_counter()
+        rm.step(args.steps)
+        ctx.sync()
+        elt = time.perf_counter() - tm
+        rm.step(args.latency_steps)
+        ktm = timed_kernels(ctx, [rm.sid], args.timing_steps)
+        cm = rm.contacts()
+        m_real, m_proc = int(cm["isReal"].sum()), rm.ctx.contact_work(rm.sid)[0]
         ctx.sync()
         f0 = rm.stats()["nFullRuns"]
         tm = time.perf_counter()
         rm.step(args.sustained_steps)
         ctx.sync()
         elm = time.perf_counter() - tm
-        ktm = timed_kernels(ctx, [rm.sid], args.timing_steps)
-        cm = rm.contacts()
         msm, lam = ktm.get("contacts", (float("nan"), 0))
+        mtimed = b.n_par * args.steps / elt
         mval = b.n_par * args.sustained_steps / elm
-        mixed = {"value": mval, "ms_per_step": elm / args.sustained_steps * 1e3, "steps": args.sustained_steps,
-                 "full_collider_runs": rm.stats()["nFullRuns"] - f0, "speedup_vs_fp64": mval / sus["value"],
-                 "contacts_avg_ms": msm / max(lam, 1), "contacts_real": int(cm["isReal"].sum()),
-                 "contacts_processed": rm.ctx.contact_work(rm.sid)[0]}
+        mixed = {"value": mtimed, "ms_per_step": elt / args.steps * 1e3, "steps": args.steps, "speedup_vs_fp64": mtimed / value,
+                 "sustained": {"value": mval, "ms_per_step": elm / args.sustained_steps * 1e3, "steps": args.sustained_steps,
+                               "full_collider_runs": rm.stats()["nFullRuns"] - f0, "speedup_vs_fp64": mval / sus["value"]},
+                 "contacts_avg_ms": msm / max(lam, 1), "contacts_real": m_real, "contacts_processed": m_proc,
+                 "kernels": {k: {"avg_ms": v[0] / max(v[1], 1), "launches": v[1]} for k, v in ktm.items()}}
         m_bytes = (BYTES_REAL["mixed"] * mixed["contacts_real"]
                    + BYTES_POT["mixed"] * max(mixed["contacts_processed"] - mixed["contacts_real"], 0))
         mixed["contacts_roofline"] = {"bound": "hbm", "algorithmic_bytes_per_launch": m_bytes,
@@ -571,6 +602,14 @@ def bench_periodic(args, ctx):
         from woodem_amd.precision import mixed_error_report
         mixed["errors_vs_fp64"] = mixed_error_report(ctx)
         log("mixed precision: %s" % json.dumps(mixed))
+
+    copy = None
+    try:
+        if not args.no_copy:  # torch's first kernel launch in this process (segfaults under rocprofv3's kernel trace)
+            copy = hbm_copy_rate()
+            log("hbm copy: %s" % json.dumps(copy))
+    except Exception as e:  # reported, not fatal
+        log("hbm copy failed: %r" % (e,))
 
     configs = {}
     if not args.no_legs:
@@ -621,7 +660,7 @@ def bench_periodic(args, ctx):
         "data": "synthetic (seeded generator, relaxed on device)",
         "config": {"workload": "cfg2: periodic Hertz, %d spheres, phi=0.60, lognormal r (median 1 mm, sigma .3), %d timed steps "
                                "after %d warm-up steps" % (b.n_par, args.steps, args.warmup),
-                   "n_particles": b.n_par, "precision": args.precision, "dt": b.meta["dt"],
+                   "n_particles": b.n_par, "precision": args.precision, "dt": b.meta["dt"], "broadphase": args.broadphase,
                    "contacts_potential": n_con, "contacts_real": n_real, "contacts_processed": n_active,
                    "full_collider_runs_in_timed_region": st1["nFullRuns"] - st0["nFullRuns"], "launch": launch_config(ctx, None)},
         "sustained": sus,
@@ -629,7 +668,8 @@ def bench_periodic(args, ctx):
         "roofline": {"bound": "hbm", "kernel": "k_contacts", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_src,
                      "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": avg_ms, "launches": launches,
-                     "contacts_real": n_real, "contacts_processed": n_active},
+                     "contacts_real": n_real, "contacts_processed": n_active,
+                     "traffic_GBps": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None, "device_copy": copy},
         "kernels": kernels,
         "mixed_precision": mixed,
         "configs": configs,
@@ -671,6 +711,9 @@ def main():
     ap.add_argument("--window", type=int, default=16, help="steps launched together as one graph (1 = step by step)")
     ap.add_argument("--no-graphs", action="store_true", help="launch windows without capturing graphs")
     ap.add_argument("--cell-factor", type=float, default=1.0, help="collider cell width over the largest sphere bound")
+    ap.add_argument("--no-copy", action="store_true", help="skip the plain device copy that measures the HBM rate (torch)")
+    ap.add_argument("--broadphase", choices=["grid", "sweep"], default="grid",
+                    help="collider broadphase: multi-level grid (default) or sort and sweep (the same contact set)")
     ap.add_argument("--autotune", type=int, default=None,
                     help="batch workload: pick window and cell factor by the autotuner (default 1 for batch, 0 otherwise)")
     ap.add_argument("--tune-steps", type=int, default=100, help="autotuner: timed steps per candidate and round")
@@ -716,6 +759,7 @@ def main():
 
     ctx = Context(0)
     ctx.set_tuning(window=args.window, graphs=not args.no_graphs, cell_factor=args.cell_factor)
+    ctx.set_broadphase(1 if args.broadphase == "sweep" else 0)
     if args.workload == "batch":
         print(json.dumps(bench_batch(args, ctx)), flush=True)
         return

@@ -202,6 +202,14 @@ int wdm_get_time(wdm_ctx* ctx, int32_t scene, double* time, double* dt);
 int wdm_set_tuning(wdm_ctx* ctx, int32_t window, int32_t graphs, double cell_factor);
 int wdm_get_tuning(wdm_ctx* ctx, int32_t* window, int32_t* graphs, double* cell_factor, int64_t* windows,
                    int64_t* graph_captures, int64_t* graph_launches);
+/* Broadphase of the collider's full runs: 0 (default) the multi-level uniform grid, 1 sort and
+   sweep (bound entries radix-sorted by minimum x, swept in LDS tiles, the reference's
+   per-axis principle applied at once rather than incrementally).  Both feed the same exact
+   test (InsertionSortCollider.cpp:52-98, 903-957 and Collider::mayCollide), so the contact
+   set does not depend on the choice; the sweep is kept as the measured alternative to the
+   grid.  No counterpart in the reference. */
+int wdm_set_broadphase(wdm_ctx* ctx, int32_t mode);
+int wdm_get_broadphase(wdm_ctx* ctx, int32_t* mode);
 
 /* Readback (synchronous). */
 int wdm_download_nodes(wdm_ctx* ctx, int32_t scene, double* pos3, double* ori4, double* vel3, double* angVel3,

@@ -108,6 +108,8 @@ def lib():
         "wdm_kernel_times": (C.c_int, [P, I32, C.c_char_p, pd, pl, pi]),
         "wdm_enable_timing": (C.c_int, [P, I32]),
         "wdm_set_tuning": (C.c_int, [P, I32, I32, D]),
+        "wdm_set_broadphase": (C.c_int, [P, I32]),
+        "wdm_get_broadphase": (C.c_int, [P, pi]),
         "wdm_get_tuning": (C.c_int, [P, pi, pi, pd, pl, pl, pl]),
         "wdm_relax_overlaps": (C.c_int, [P, I32, I32]),
         "wdm_set_dyndt": (C.c_int, [P, I32, C.POINTER(WdmDynDt)]),
@@ -308,6 +310,16 @@ class Context:
         g = t["graphs"] if graphs is None else int(bool(graphs))
         f = t["cell_factor"] if cell_factor is None else float(cell_factor)
         self.check(self.L.wdm_set_tuning(self.h, w, g, f))
+
+    def set_broadphase(self, mode):
+        """Collider broadphase: 0 the multi-level grid (default), 1 sort and sweep; the contact
+        set is the same."""
+        self.check(self.L.wdm_set_broadphase(self.h, int(mode)))
+
+    def broadphase(self):
+        m = C.c_int32()
+        self.check(self.L.wdm_get_broadphase(self.h, C.byref(m)))
+        return m.value
 
     def tuning(self):
         w, g = C.c_int32(), C.c_int32()

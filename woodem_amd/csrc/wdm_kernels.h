@@ -1723,6 +1723,129 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_pair_exact(const SceneDesc* D, co
         if (cntW > wcap) atomicMax(wmax, cntW);
     }
 }
+// ---------------- sort-and-sweep broadphase (the alternative to the grid scan) ----------------
+// The bound entries of each scene sorted by the minimum x of their FP32 boxes (radix sort of
+// (scene << 32 | order-preserving bits of x - half extent)), then swept: the entry at sorted
+// position i tests the entries after it whose minimum x does not exceed its maximum x, on y
+// and z with the FP32 box test of k_pair_scan (x directly), and, in a periodic cell, the
+// entries from the start of the scene shifted by one period (a pair that overlaps directly is
+// left to the direct sweep, so each pair is emitted once).  A workgroup stages the sorted
+// boxes in LDS tiles shared by its 256 consecutive positions.  Output: the pairs that pass,
+// (entry, partner entry), into the same wavefront regions as k_pair_scan, which k_pair_exact
+// then tests exactly, so the contact set does not depend on the broadphase.
+__device__ __forceinline__ uint32_t sortableF32(float x) {
+    const uint32_t u = __float_as_uint(x);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__global__ void __launch_bounds__(WDM_BLOCK) k_sweep_keys(const SceneDesc* D, const FRScene* FR, Map m, const float4* boxes,
+                                                         uint64_t* keys, uint32_t* vals) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const FRScene& F = FR[t];
+    const int li = (int)(lb * WDM_BLOCK + threadIdx.x);
+    if (li >= D[F.desc].nbinned) return;
+    const int64_t e = F.binStart + li;
+    const float4 b = boxes[e];
+    keys[e] = ((uint64_t)(uint32_t)t << 32) | sortableF32(b.x - b.w);
+    vals[e] = (uint32_t)e;
+}
+#define SWEEP_TILE 1024
+__global__ void __launch_bounds__(WDM_BLOCK) k_pair_sweep(const SceneDesc* D, const FRScene* FR, Map m, const uint32_t* svals,
+                                                         const float4* boxes, int32_t* cnt, int32_t* wmax, int32_t cap, int2* out) {
+    __shared__ float4 tb[SWEEP_TILE];
+    __shared__ int32_t te[SWEEP_TILE];
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const FRScene& F = FR[t];
+    const GridArgs& g = F.g;
+    const int nb = D[F.desc].nbinned;
+    const int64_t gw = (int64_t)blockIdx.x * (WDM_BLOCK / 64) + (threadIdx.x >> 6);
+    GP int2* ro = gpc(out) + gw * (int64_t)cap;
+    int cntW = 0;
+    const int li = (int)(lb * WDM_BLOCK + threadIdx.x);
+    const bool valid = li < nb;
+    const int64_t s0 = F.binStart, end = F.binStart + nb, si = s0 + li;
+    const int e = valid ? (int)svals[si] : 0;
+    const float4 own = valid ? boxes[e] : make_float4(0.f, 0.f, 0.f, -1e30f);
+    const float margin = g.pfMargin;
+    const float xmax = own.x + own.w + margin;
+    const float wLy = g.periodic ? g.Lf[1] : 0.f, wLz = g.periodic ? g.Lf[2] : 0.f;
+    const float wIy = g.periodic ? g.invLf[1] : 0.f, wIz = g.periodic ? g.invLf[2] : 0.f;
+    const float Lx = g.periodic ? g.Lf[0] : 0.f;
+    auto emit = [&](bool pass, int jj) {
+        const uint64_t bal = __ballot(pass);
+        if (pass) {
+            const int k = cntW + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            if (k < cap) ro[k] = make_int2(e, jj);
+        }
+        cntW += (int)__popcll(bal);
+    };
+    // y and z with the periodic minimum image; x by the caller
+    auto yz = [&](const float4& b) {
+        float dy = b.y - own.y, dz = b.z - own.z;
+        dy -= wLy * rintf(dy * wIy);
+        dz -= wLz * rintf(dz * wIz);
+        const float lim = (own.w + b.w) + margin;
+        return !((fabsf(dy) > lim) | (fabsf(dz) > lim));
+    };
+    // direct sweep over LDS tiles of the sorted boxes after the block's first position
+    bool act = valid;
+    for (int64_t ts = s0 + (int64_t)lb * WDM_BLOCK + 1;; ts += SWEEP_TILE) {
+        if (!__syncthreads_or(act && ts < end)) break;
+        for (int k = threadIdx.x; k < SWEEP_TILE; k += WDM_BLOCK) {
+            const int64_t q = ts + k;
+            if (q < end) {
+                const int e2 = (int)svals[q];
+                tb[k] = boxes[e2];
+                te[k] = e2;
+            }
+        }
+        __syncthreads();
+        if (__any(act)) {
+            for (int k = 0; k < SWEEP_TILE; k++) {
+                const int64_t q = ts + k;
+                bool my = act && q > si;
+                if (act && q >= end) { act = false; my = false; }
+                const float4 b = my ? tb[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+                if (my && b.x - b.w > xmax) { act = false; my = false; }  // sorted: none after it overlaps
+                bool pass = false;
+                if (my) pass = fabsf(b.x - own.x) <= (own.w + b.w) + margin && yz(b);
+                emit(pass, my ? te[k] : 0);
+                if (!__any(act)) break;
+            }
+        }
+        __syncthreads();
+    }
+    // periodic x: the entries from the scene's start, one period up, up to this one's maximum x
+    // (every position looks: a box whose minimum x is below 0 reaches boxes whose maximum x
+    // is below L; the first sorted entry ends the walk at once for most)
+    if (g.periodic) {
+        bool wr = valid;
+        for (int64_t q = s0; __any(wr); q++) {
+            bool my = wr && q < end && q != si;
+            if (wr && q >= end) wr = false;
+            float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
+            int e2 = 0;
+            if (my) {
+                e2 = (int)svals[q];
+                b = boxes[e2];
+                if (b.x - b.w + Lx > xmax) { wr = false; my = false; }
+            }
+            bool pass = false;
+            if (my) {
+                const float lim = (own.w + b.w) + margin;
+                const bool direct = fabsf(b.x - own.x) <= lim;
+                pass = !direct && fabsf(b.x + Lx - own.x) <= lim && yz(b);
+            }
+            emit(pass, e2);
+        }
+    }
+    if ((threadIdx.x & 63) == 0) {
+        cnt[gw] = cntW;
+        if (cntW > cap) atomicMax(wmax, cntW);
+    }
+}
+
 // each pair of the wavefront regions at its row's offset plus its slot (rows by min id); a
 // region that overflowed holds its first wcap pairs (the host runs the pass again)
 __global__ void __launch_bounds__(WDM_BLOCK) k_cand_scatter(const int32_t* wcnt, int64_t nw, int32_t wcap, const uint64_t* rItem,

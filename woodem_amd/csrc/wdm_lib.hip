@@ -122,6 +122,9 @@ struct Ctx {
     // pairs a wavefront of the pair search may find: passing the FP32 box test, passing the
     // exact test (each grows when a pass needs more)
     int32_t scanCap = 1280, waveCap = 640;
+    // broadphase of the collider's full run: 0 the multi-level grid scan, 1 sort and sweep
+    // (radix sort of the bound entries by minimum x, LDS-tiled sweep); the contact set is the same
+    int broadphase = 0;
     hipError_t scratchErr = hipSuccess;  // looking up the device address of `scratch` at wdm_create
 };
 
@@ -273,6 +276,8 @@ struct Shared {
     DBuf<double> extPart, extOut;
     DBuf<uint32_t> hist;                 // bound extents per scene, quarter octaves
     DBuf<uint32_t> ekey, eslot, ccnt;    // bound entries: cell key and slot; count per cell
+    DBuf<uint64_t> swK, swK2;            // sort-and-sweep broadphase: keys (scene, minimum x) and their sorted copy
+    DBuf<uint32_t> swV, swV2;            // entries, and in sorted order
     DBuf<int32_t> cbeg;                  // first entry of each cell (exclusive prefix of ccnt)
     DBuf<uint32_t> ckeys2;               // sorted entries: cell key, particle id, FP32 box
     DBuf<int32_t> cvals2, oslot, srcRow;
@@ -348,7 +353,7 @@ struct Shared {
         release(c, candP); release(c, eB); release(c, eS); release(c, csrCnt); release(c, eI); release(c, csrScan);
         release(c, temp); release(c, disp); release(c, cptTmp); release(c, ddOut);
         release(c, lcntE); release(c, lkeys); release(c, lslot); release(c, lcnt); release(c, loffE); release(c, lids); release(c, lbeg);
-        release(c, lvals);
+        release(c, lvals); release(c, swK); release(c, swK2); release(c, swV); release(c, swV2);
         if (pin) (void)hipHostFree(pin);
         pin = nullptr;
         pinCap = 0;
@@ -1268,7 +1273,21 @@ struct InsertionSortCollider {
                 buildGrids(B, L, F, bpB, mB, FRd, nbTot, c->cellFactor, c->gridLevels, true, lagged, (uint32_t*)X.wmax.p + 2,
                            anyFacets ? 1 : 0);
             }
-            const bool large = anyFacets && listLarge(B, bpB, mB, FRd, nbTot, F[nT - 1].cellBase + F[nT - 1].nCells + 1);
+            const bool sweep = c->broadphase == 1;
+            const bool large = !sweep && anyFacets && listLarge(B, bpB, mB, FRd, nbTot, F[nT - 1].cellBase + F[nT - 1].nCells + 1);
+            if (sweep && nbTot > 0) {  // the entries of each scene by the minimum x of their boxes
+                KTimer tm(c, "sweep_sort");
+                const size_t nb1 = (size_t)nbTot;
+                ensure(c, X.swK, nb1); ensure(c, X.swK2, nb1); ensure(c, X.swV, nb1); ensure(c, X.swV2, nb1);
+                hipLaunchKernelGGL(k_sweep_keys, dim3((unsigned)bpB[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const float4*)X.cbox.p,
+                                   X.swK.p, X.swV.p);
+                checkLaunch();
+                size_t tb = 0;
+                const int hiBit = 32 + bitsFor((uint64_t)nT);
+                HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, X.swK.p, X.swK2.p, X.swV.p, X.swV2.p, (int)nbTot, 0, hiBit, st));
+                void* tmp = tempFor(c, tb);
+                HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, X.swK.p, X.swK2.p, X.swV.p, X.swV2.p, (int)nbTot, 0, hiBit, st));
+            }
             Upload uO;
             const size_t oFO = uO.add(F), oCO = uO.add(C), oO = uO.add(bpO), o1 = uO.add(bp1);
             uO.send(c, X.stRunO);
@@ -1298,8 +1317,12 @@ struct InsertionSortCollider {
                 KTimer tm(c, "pairs");
                 const int64_t pad = 8 * PAIR_CHUNK;
                 const dim3 grid((unsigned)((nBlk + pad - 1) / pad * pad));
-                hipLaunchKernelGGL(k_pair_scan, grid, dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
-                                   (const int32_t*)X.cbeg.p, (const float4*)X.cbox.p, X.scnt.p, X.wmax.p, scap, X.sPair.p);
+                if (sweep)
+                    hipLaunchKernelGGL(k_pair_sweep, dim3((unsigned)nBlk), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.swV2.p,
+                                       (const float4*)X.cbox.p, X.scnt.p, X.wmax.p, scap, X.sPair.p);
+                else
+                    hipLaunchKernelGGL(k_pair_scan, grid, dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
+                                       (const int32_t*)X.cbeg.p, (const float4*)X.cbox.p, X.scnt.p, X.wmax.p, scap, X.sPair.p);
                 hipLaunchKernelGGL(k_pair_exact, grid, dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const int32_t*)X.cvals2.p,
                                    (const uint32_t*)X.ckeys2.p, (const double4*)X.eb0.p, (const double4*)X.eb1.p, (const int2*)X.einfo.p,
                                    large ? (const int32_t*)X.lbeg.p : nullptr, (const int32_t*)X.lvals.p,
@@ -3344,6 +3367,17 @@ int wdm_set_tuning(wdm_ctx* c, int32_t window, int32_t graphs, double cellFactor
         c->window = window;
         c->graphs = graphs != 0;
         c->cellFactor = cellFactor;
+    });
+}
+int wdm_set_broadphase(wdm_ctx* c, int32_t mode) {
+    return guard(c, [&] {
+        if (mode != 0 && mode != 1) throw WdmError(WDM_E_ARG, "broadphase must be 0 (grid) or 1 (sort and sweep)");
+        c->broadphase = mode;
+    });
+}
+int wdm_get_broadphase(wdm_ctx* c, int32_t* mode) {
+    return guard(c, [&] {
+        if (mode) *mode = c->broadphase;
     });
 }
 int wdm_get_tuning(wdm_ctx* c, int32_t* window, int32_t* graphs, double* cellFactor, int64_t* windows, int64_t* captures,
