@@ -225,25 +225,35 @@ def measured_traffic(args, n_real, n_active):
     return t["hbm_read_bytes_per_launch"] + t["hbm_write_bytes_per_launch"], "profiles/" + t.get("source", "")
 
 
-def hbm_copy_rate(nbytes=1 << 30, reps=10):
-    """A plain device-to-device copy of nbytes (torch, on its own stream), timed with events:
-    the HBM rate a streaming kernel reaches on this device, beside the nominal peak."""
-    import torch
-    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
+COPY_PROBE = r"""
+import json, sys, torch
+nbytes, reps = int(sys.argv[1]), int(sys.argv[2])
+a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+b = torch.empty_like(a)
+b.copy_(a)
+torch.cuda.synchronize()
+e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+e0.record()
+for _ in range(reps):
     b.copy_(a)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        b.copy_(a)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
+e1.record()
+torch.cuda.synchronize()
+print(json.dumps({"ms_per_copy": e0.elapsed_time(e1) / reps}))
+"""
+
+
+def hbm_copy_rate(nbytes=1 << 30, reps=10):
+    """A plain device-to-device copy of nbytes (torch, in a child process of its own: torch's HIP
+    runtime does not start in a process that has loaded the library's), timed with events:
+    the HBM rate a streaming kernel reaches on this device, beside the nominal peak."""
+    import subprocess
+    r = subprocess.run([sys.executable, "-c", COPY_PROBE, str(nbytes), str(reps)], capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError("copy probe failed: " + r.stderr[-300:])
+    ms = json.loads(r.stdout.strip().splitlines()[-1])["ms_per_copy"]
     gbps = 2 * nbytes / (ms * 1e-3) / 1e9
-    del a, b
-    torch.cuda.empty_cache()
-    return {"bytes_moved_per_copy": 2 * nbytes, "ms_per_copy": ms, "GBps": gbps, "frac_of_peak": gbps / HBM_PEAK_GBS}
+    return {"bytes_moved_per_copy": 2 * nbytes, "ms_per_copy": ms, "GBps": gbps, "frac_of_peak": gbps / HBM_PEAK_GBS,
+            "how": "torch copy_ of 1 GiB float64, 10 repetitions between two events, in a child process"}
 
 
 def timed_kernels(ctx, sids, steps):
@@ -605,7 +615,7 @@ def bench_periodic(args, ctx):
 
     copy = None
     try:
-        if not args.no_copy:  # torch's first kernel launch in this process (segfaults under rocprofv3's kernel trace)
+        if not args.no_copy:
             copy = hbm_copy_rate()
             log("hbm copy: %s" % json.dumps(copy))
     except Exception as e:  # reported, not fatal
