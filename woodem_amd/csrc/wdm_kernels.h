@@ -324,13 +324,40 @@ __device__ __forceinline__ int mapBlockAt(const Map& m, int64_t b, int64_t& lblk
     return lo;
 }
 __device__ __forceinline__ int mapBlock(const Map& m, int64_t& lblk) { return mapBlockAt(m, blockIdx.x, lblk); }
+// XCD-chunked block order: the hardware hands the blocks of a launch to the 8 XCDs round
+// robin (block b to XCD b % 8), so consecutive blocks, which work on neighbouring particles
+// and contacts, land in different L2 caches.  With a chunk of k blocks, XCD x takes logical
+// blocks [(8 g + x) k, (8 g + x + 1) k) from its g-th group of k launched blocks, so k
+// neighbouring blocks share one L2 while the 8 XCDs still advance through the work together
+// (k = 0: plain order; k < 0: one contiguous eighth per XCD).  Returns -1 for padding blocks.
+#ifndef XCD_CHUNK_LEAP
+#define XCD_CHUNK_LEAP (-1)
+#endif
+#ifndef XCD_CHUNK_CON
+#define XCD_CHUNK_CON 0
+#endif
+__host__ __device__ inline int64_t xcdChunkOf(int64_t n, int64_t k) { return k < 0 ? (n + 7) >> 3 : k; }
+__device__ __forceinline__ int64_t xcdChunkBlock(int64_t n, int64_t k) {
+    const int64_t b = blockIdx.x;
+    if (k == 0) return b < n ? b : -1;
+    k = xcdChunkOf(n, k);
+    const int64_t r = b >> 3;
+    const int64_t l = ((r / k) * 8 + (b & 7)) * k + r % k;
+    return l < n ? l : -1;
+}
+__host__ __device__ inline int64_t xcdChunkGrid(int64_t n, int64_t k) {
+    if (k == 0) return n;
+    k = xcdChunkOf(n, k);
+    return (n + 8 * k - 1) / (8 * k) * (8 * k);
+}
 
 // ---------------- block reductions (fixed tree: deterministic) ----------------
 // bits: (1 << energy slot) for "slot created" (EnergyTracker::add, core/EnergyTracker.cpp:10-17),
 // VERLET_BIT for the Verlet trigger (the trigger word then receives the step's stamp).
 #define VERLET_BIT 0x80000000u
 template <int NV>
-__device__ void blockSumWrite(double (&v)[NV], uint32_t orbits, double* partial, uint32_t* trigOut, uint32_t stamp, uint32_t* creOut) {
+__device__ void blockSumWrite(double (&v)[NV], uint32_t orbits, double* partial, uint32_t* trigOut, uint32_t stamp, uint32_t* creOut,
+                              int64_t blk = -1) {
     __shared__ double sh[NV][WDM_BLOCK];
     __shared__ uint32_t shb[WDM_BLOCK];
     int t = threadIdx.x;
@@ -345,7 +372,8 @@ __device__ void blockSumWrite(double (&v)[NV], uint32_t orbits, double* partial,
         __syncthreads();
     }
     if (t == 0) {
-        if (partial) for (int k = 0; k < NV; k++) partial[(size_t)blockIdx.x * NV + k] = sh[k][0];
+        const size_t pb = blk >= 0 ? (size_t)blk : (size_t)blockIdx.x;
+        if (partial) for (int k = 0; k < NV; k++) partial[pb * NV + k] = sh[k][0];
         uint32_t b = shb[0];
         if ((b & VERLET_BIT) && trigOut) atomicMax(trigOut, stamp);
         if ((b & ~VERLET_BIT) && creOut) atomicOr(creOut, b & ~VERLET_BIT);
@@ -475,8 +503,10 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_gather(const SceneDesc* D, const 
 // from the contacts of that ContactLoop here instead of being read from the arrays.
 template <int GATHER>
 __global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(const SceneDesc* D, const int32_t* idx, Map m, double* epart, const StepVar* sv) {
+    const int64_t blk = xcdChunkBlock(m.bprefix[m.n], XCD_CHUNK_LEAP);
+    if (blk < 0) return;
     int64_t lb;
-    const int t = mapBlock(m, lb);
+    const int t = mapBlockAt(m, blk, lb);
     const int di = idx ? idx[t] : t;
     const SceneDesc& S = D[di];
     StepArgs a = S.a;
@@ -628,7 +658,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_leapfrog(const SceneDesc* D, cons
         if (__any((bits & VERLET_BIT) != 0) && (threadIdx.x & 63) == 0) atomicMax(S.words, a.stamp);
         return;
     }
-    blockSumWrite<3>(ev, bits, epart, S.words, a.stamp, S.words + 1);
+    blockSumWrite<3>(ev, bits, epart, S.words, a.stamp, S.words + 1, blk);
 }
 
 // ClumpData::forceTorqueFromMembers, pkg/dem/Clump.cpp:265-273, one thread per clump before
@@ -1476,21 +1506,16 @@ __device__ __forceinline__ bool stencilCellL(const FRScene& F, int l, int cx, in
     cell = F.lbase[l] + (z * ny + y) * nx + x;
     return true;
 }
-// The hardware hands the blocks of a launch to the 8 XCDs round robin (block b to XCD b % 8).
-// Logical blocks in chunks of PAIR_CHUNK consecutive ones, chunk k on XCD k % 8: the entries
-// of a chunk are a contiguous range of cells, whose neighbours mostly share that XCD's L2,
-// while a stretch of costly entries (the coarser levels come last) is spread over all XCDs.
-// The launch is padded to a multiple of 8 PAIR_CHUNK blocks.
+// Pair scan in XCD chunks of PAIR_CHUNK blocks (xcdChunkBlock): the entries of a chunk are a
+// contiguous range of cells, whose neighbours mostly share that XCD's L2, while a stretch of
+// costly entries (the coarser levels come last) is spread over all XCDs.  The launch is
+// padded to a multiple of 8 PAIR_CHUNK blocks.
 #define PAIR_CHUNK 16
-__device__ __forceinline__ int64_t xcdChunkBlock() {
-    const int64_t b = blockIdx.x, x = b & 7, r = b >> 3;
-    return ((r / PAIR_CHUNK) * 8 + x) * PAIR_CHUNK + (r % PAIR_CHUNK);
-}
 // this wavefront's index (regions, counts), its scene and its block within the scene; false
 // for padding.  The logical blocks run backwards, so that the costly entries of the coarser
 // levels (last in each scene) start first and do not trail the launch.
 __device__ __forceinline__ bool pairWave(const Map& m, int64_t& gw, int& t, int64_t& lb) {
-    const int64_t lbk = (int64_t)gridDim.x - 1 - xcdChunkBlock();
+    const int64_t lbk = (int64_t)gridDim.x - 1 - xcdChunkBlock(gridDim.x, PAIR_CHUNK);
     if (lbk >= m.bprefix[m.n]) return false;
     t = mapBlockAt(m, lbk, lb);
     gw = lbk * (WDM_BLOCK / 64) + (threadIdx.x >> 6);
@@ -2269,8 +2294,10 @@ template <typename T> __device__ __forceinline__ void trSetCol(T* tr, int j, con
 template <typename T, int LAW, int GEOM>
 __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, const ConDesc<T>* C, Map m, double* epart, int spec,
                                                        const StepVar* sv) {
+    const int64_t blk = xcdChunkBlock(m.bprefix[m.n], XCD_CHUNK_CON);
+    if (blk < 0) return;
     int64_t lb;
-    const int tt = mapBlock(m, lb);
+    const int tt = mapBlockAt(m, blk, lb);
     const ConDesc<T>& CD = C[tt];
     const SceneDesc& S = D[CD.desc];
     StepArgs a = S.a;
@@ -2645,7 +2672,7 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_contacts(const SceneDesc* D, cons
     uint64_t bal = __ballot(realOut);
     int64_t w0 = lb * WDM_BLOCK + (threadIdx.x & ~63);
     if ((threadIdx.x & 63) == 0 && w0 < nItems) realMask[w0 >> 6] = bal;
-    if (a.trackEnergy) blockSumWrite<5>(ev, bits, epart, nullptr, 0u, creOut);
+    if (a.trackEnergy) blockSumWrite<5>(ev, bits, epart, nullptr, 0u, creOut, blk);
 }
 
 // Mixed precision: the contact points of a list's live real contacts between their stored

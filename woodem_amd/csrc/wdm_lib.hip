@@ -905,8 +905,9 @@ struct LeapPlan {
             const int64_t nbk = bp[g].back();
             if (nbk > 0) {
                 KTimer t(c, "leapfrog");
-                if (g) hipLaunchKernelGGL(k_leapfrog<1>, dim3((unsigned)nbk), dim3(WDM_BLOCK), 0, st, D, id, m, ep, sv);
-                else hipLaunchKernelGGL(k_leapfrog<0>, dim3((unsigned)nbk), dim3(WDM_BLOCK), 0, st, D, id, m, ep, sv);
+                const dim3 grid((unsigned)xcdChunkGrid(nbk, XCD_CHUNK_LEAP));
+                if (g) hipLaunchKernelGGL(k_leapfrog<1>, grid, dim3(WDM_BLOCK), 0, st, D, id, m, ep, sv);
+                else hipLaunchKernelGGL(k_leapfrog<0>, grid, dim3(WDM_BLOCK), 0, st, D, id, m, ep, sv);
                 checkLaunch();
             }
             if (anyEnergy) {
@@ -1581,7 +1582,7 @@ struct ContactPlan {
         const ConDesc<T>* Cd = (const ConDesc<T>*)(base + g.oC);
         if (g.blocks > 0) {
             KTimer tm(c, "contacts");
-            hipLaunchKernelGGL((k_contacts<T, LAW, GEOM>), dim3((unsigned)g.blocks), dim3(WDM_BLOCK), 0, c->stream, D, Cd, m,
+            hipLaunchKernelGGL((k_contacts<T, LAW, GEOM>), dim3((unsigned)xcdChunkGrid(g.blocks, XCD_CHUNK_CON)), dim3(WDM_BLOCK), 0, c->stream, D, Cd, m,
                                c->sh->epartC.p, spec, sv);
             checkLaunch();
         }
