@@ -1382,13 +1382,10 @@ __device__ __forceinline__ void largeRangeL(const FRScene& F, int l, const doubl
         if (F.g.periodic && nc[d] > n) { nc[d] = n; c0[d] = 0; }
     }
 }
-// Large bounds are listed in the cells of every level.  MODE 0: number of cells listing
-// each sorted entry (0 for the bounds in cells); MODE 1: those listings at offs[i]: cell
-// key, particle id and the slot of the count per cell.
-template <int MODE>
+// Large bounds are listed in the cells of every level: the number of cells listing each sorted
+// entry (0 for the bounds in cells)
 __global__ void __launch_bounds__(WDM_BLOCK) k_large_cells(const SceneDesc* D, const FRScene* FR, Map m, const uint32_t* ckeys,
-                                                          const int32_t* cvals, uint32_t* counts, const int64_t* offs, uint32_t* lkeys,
-                                                          int32_t* lids, uint32_t* lslot, uint32_t* lcnt) {
+                                                          const int32_t* cvals, uint32_t* counts) {
     int64_t lb;
     const FRScene& F = FR[mapBlock(m, lb)];
     const SceneDesc& S = D[F.desc];
@@ -1396,35 +1393,66 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_large_cells(const SceneDesc* D, c
     if (li >= S.nbinned) return;
     const int64_t i = F.binStart + li;
     if ((int)(ckeys[i] - F.cellBase) != F.nCells) {
-        if (!MODE) counts[i] = 0;
+        counts[i] = 0;
         return;
     }
     const int id = cvals[i];
     const double4 a0 = S.p.bmin[id], a1 = S.p.bmax[id];
-    int64_t o = MODE ? offs[i] : 0;
     uint32_t tot = 0;
     for (int l = 0; l < F.nLvl; l++) {
         int c0[3], nc[3];
         largeRangeL(F, l, a0, a1, c0, nc);
         tot += (uint32_t)(nc[0] * nc[1] * nc[2]);
-        if (!MODE) continue;
-        const int nx = F.ln[l][0], ny = F.ln[l][1], nz = F.ln[l][2];
-        for (int zz = 0; zz < nc[2]; zz++) {
-            const int z = F.g.periodic ? wrapCell(c0[2] + zz, nz) : c0[2] + zz;
-            for (int yy = 0; yy < nc[1]; yy++) {
-                const int y = F.g.periodic ? wrapCell(c0[1] + yy, ny) : c0[1] + yy;
-                for (int xx = 0; xx < nc[0]; xx++) {
-                    const int x = F.g.periodic ? wrapCell(c0[0] + xx, nx) : c0[0] + xx;
-                    const uint32_t key = (uint32_t)(F.cellBase + F.lbase[l] + ((int64_t)z * ny + y) * nx + x);
-                    lkeys[o] = key;
-                    lids[o] = id;
-                    lslot[o] = atomicAdd(&lcnt[key], 1u);
-                    o++;
-                }
-            }
-        }
     }
-    if (!MODE) counts[i] = tot;
+    counts[i] = tot;
+}
+// The listings themselves, one thread each (a facet covers hundreds of cells over the levels):
+// listing o belongs to the entry i with offs[i] <= o < offs[i + 1] (offs: the exclusive prefix
+// sum of the counts over the nb sorted entries of the nT scenes); its level and cell follow
+// from o - offs[i] in the order level, z, y, x.  Writes the cell key, the particle id and the
+// slot of the count per cell.
+__global__ void __launch_bounds__(WDM_BLOCK) k_large_list(const SceneDesc* D, const FRScene* FR, int nT, const int32_t* cvals, int64_t nb,
+                                                         const int64_t* offs, int64_t nl, uint32_t* lkeys, int32_t* lids, uint32_t* lslot,
+                                                         uint32_t* lcnt) {
+    const int64_t o = blockIdx.x * (int64_t)WDM_BLOCK + threadIdx.x;
+    if (o >= nl) return;
+    int64_t lo = 0, hi = nb;  // offs[lo] <= o < offs[hi]
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (offs[mid] <= o) lo = mid;
+        else hi = mid;
+    }
+    const int64_t i = lo;
+    int tl = 0, th = nT;  // FR[tl].binStart <= i < FR[th].binStart
+    while (th - tl > 1) {
+        const int mid = (tl + th) >> 1;
+        if (FR[mid].binStart <= i) tl = mid;
+        else th = mid;
+    }
+    const FRScene& F = FR[tl];
+    const SceneDesc& S = D[F.desc];
+    const int id = cvals[i];
+    const double4 a0 = S.p.bmin[id], a1 = S.p.bmax[id];
+    int64_t r = o - offs[i];
+    for (int l = 0; l < F.nLvl; l++) {
+        int c0[3], nc[3];
+        largeRangeL(F, l, a0, a1, c0, nc);
+        const int64_t cnt = (int64_t)nc[0] * nc[1] * nc[2];
+        if (r >= cnt) {
+            r -= cnt;
+            continue;
+        }
+        const int nx = F.ln[l][0], ny = F.ln[l][1], nz = F.ln[l][2];
+        const int xx = (int)(r % nc[0]), yy = (int)((r / nc[0]) % nc[1]), zz = (int)(r / ((int64_t)nc[0] * nc[1]));
+        const int z = F.g.periodic ? wrapCell(c0[2] + zz, nz) : c0[2] + zz;
+        const int y = F.g.periodic ? wrapCell(c0[1] + yy, ny) : c0[1] + yy;
+        const int x = F.g.periodic ? wrapCell(c0[0] + xx, nx) : c0[0] + xx;
+        const uint32_t key = (uint32_t)(F.cellBase + F.lbase[l] + ((int64_t)z * ny + y) * nx + x);
+        lkeys[o] = key;
+        lids[o] = id;
+        lslot[o] = atomicAdd(&lcnt[key], 1u);
+        return;
+    }
 }
 // each listing at its cell's offset plus its slot (the order within a cell does not matter:
 // the candidates are sorted afterwards)

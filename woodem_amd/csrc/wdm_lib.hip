@@ -1166,9 +1166,8 @@ struct InsertionSortCollider {
         const int nT = (int)bpB.size() - 1;
         if (bpB[nT] == 0) return false;
         ensure(c, X.lcntE, (size_t)nbTot); ensure(c, X.loffE, (size_t)nbTot + 1);
-        hipLaunchKernelGGL(k_large_cells<0>, dim3((unsigned)bpB[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
-                           (const int32_t*)X.cvals2.p, X.lcntE.p, (const int64_t*)nullptr, (uint32_t*)nullptr, (int32_t*)nullptr,
-                           (uint32_t*)nullptr, (uint32_t*)nullptr);
+        hipLaunchKernelGGL(k_large_cells, dim3((unsigned)bpB[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
+                           (const int32_t*)X.cvals2.p, X.lcntE.p);
         checkLaunch();
         scanExclusive(c, (const uint32_t*)X.lcntE.p, X.loffE.p, nbTot);
         int64_t nl = 0;
@@ -1181,8 +1180,8 @@ struct InsertionSortCollider {
         ensure(c, X.lkeys, (size_t)nl); ensure(c, X.lids, (size_t)nl); ensure(c, X.lslot, (size_t)nl); ensure(c, X.lvals, (size_t)nl);
         ensure(c, X.lcnt, (size_t)G); ensure(c, X.lbeg, (size_t)G + 1);
         fillU32(c, X.lcnt.p, G);
-        hipLaunchKernelGGL(k_large_cells<1>, dim3((unsigned)bpB[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, mB, (const uint32_t*)X.ckeys2.p,
-                           (const int32_t*)X.cvals2.p, (uint32_t*)nullptr, (const int64_t*)X.loffE.p, X.lkeys.p, X.lids.p, X.lslot.p, X.lcnt.p);
+        hipLaunchKernelGGL(k_large_list, dim3(nblk(nl)), dim3(WDM_BLOCK), 0, st, B.D, FRd, nT, (const int32_t*)X.cvals2.p, nbTot,
+                           (const int64_t*)X.loffE.p, nl, X.lkeys.p, X.lids.p, X.lslot.p, X.lcnt.p);
         checkLaunch();
         scanExclusive(c, (const uint32_t*)X.lcnt.p, X.lbeg.p, G);
         hipLaunchKernelGGL(k_large_scatter, dim3(nblk(nl)), dim3(WDM_BLOCK), 0, st, (const uint32_t*)X.lkeys.p, (const int32_t*)X.lids.p,
