@@ -210,6 +210,16 @@ int wdm_get_tuning(wdm_ctx* ctx, int32_t* window, int32_t* graphs, double* cell_
    grid.  No counterpart in the reference. */
 int wdm_set_broadphase(wdm_ctx* ctx, int32_t mode);
 int wdm_get_broadphase(wdm_ctx* ctx, int32_t* mode);
+/* Work-list relayout between collider full runs: every `steps` steps (default 64; 0 never)
+   after a scene's last full run or relayout, the ContactLoop work list of a scene with at
+   least `min_contacts` contacts (default 2^20) is rewritten with the real contacts first and
+   their state rows contiguous again (contacts formed since the full run otherwise sit
+   scattered among the pairs that stay apart).  Memory order only: the contact list, every
+   value and every result stay the same (energy sums, which follow the work-list order, as
+   well: every way of stepping relayouts at the same steps).  The getter also returns the
+   relayouts of scene sid so far (sid < 0: none).  No counterpart in the reference. */
+int wdm_set_relayout(wdm_ctx* ctx, int32_t steps, int64_t min_contacts);
+int wdm_get_relayout(wdm_ctx* ctx, int32_t sid, int32_t* steps, int64_t* min_contacts, int64_t* relayouts);
 
 /* Readback (synchronous). */
 int wdm_download_nodes(wdm_ctx* ctx, int32_t scene, double* pos3, double* ori4, double* vel3, double* angVel3,

@@ -1,0 +1,94 @@
+"""Work-list relayout between collider full runs (wdm_set_relayout): real contacts moved to the
+front of the ContactLoop's work list with their state rows, contact points and gather records,
+CSR entries remapped.  Memory order only: stepped with a relayout every 8 steps (2 in the
+drum, whose full runs come every few steps), every scene
+kind gives the same nodes, contact list and contact state bit for bit as without relayouts
+(energies sum their terms in work-list order: equal to rounding), and stepping engine by engine
+relayouts at the same steps as wdm_step, energies included."""
+import numpy as np
+import pytest
+
+from woodem_amd import runner, scenes
+from woodem_amd._native import PREC_MIXED, make_dyndt
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(kind, ctx):
+    if kind == "box":
+        a = scenes.relax_box(scenes.deposition_box(seed=1003), ctx, iters=200)
+        return a, runner.params_from_meta(a, trackEnergy=True)
+    if kind == "drum":  # spheres thrown at the spinning wall: facet, wall and sphere contacts early
+        a = scenes.drum(seed=3, n=2000, omega=3., gap=1e-4)
+        ns = a.meta["n_spheres"]
+        rng = np.random.default_rng(5)
+        a.vel[:ns] = rng.normal(0, .3, (ns, 3))
+        a.vel[:ns, 2] -= .8
+        return a, runner.params_from_meta(a, trackEnergy=True)
+    a = scenes.relax_and_order(scenes.periodic_hertz(n=4000, seed=6), ctx, iters=150)
+    if kind == "mixed":
+        return a, runner.params_from_meta(a, trackEnergy=True, precision=PREC_MIXED)
+    return a, runner.params_from_meta(a, trackEnergy=True)
+
+
+def _run(ctx, a, p, steps, relayout, dyndt=False):
+    ctx.set_relayout(relayout, 0)
+    r = runner.HipRun(a, p, runner.cell_from_meta(a), ctx=ctx)
+    if dyndt:
+        ctx.set_dyndt(r.sid, make_dyndt(stepPeriod=50, maxRelInc=1e-3))
+    r.step(steps)
+    out = dict(nodes=r.nodes(), contacts=r.contacts(), energy=r.energy(), stats=r.stats(), relayouts=ctx.relayout(r.sid)[2],
+               time=ctx.time(r.sid))
+    r.close()
+    return out
+
+
+@pytest.mark.parametrize("kind", ["periodic", "box", "drum", "mixed"])
+def test_relayout_changes_no_result(hip_ctx, kind):
+    ctx = hip_ctx
+    a, p = _scene(kind, ctx)
+    steps = 1500 if kind == "drum" else 400
+    try:
+        # the drum's full runs come every few steps: relayouts every 2 steps fall between them
+        x = _run(ctx, a, p, steps, 2 if kind == "drum" else 8, dyndt=kind == "box")
+        y = _run(ctx, a, p, steps, 0, dyndt=kind == "box")
+    finally:
+        ctx.set_relayout(64)
+    assert x["relayouts"] > 10 and y["relayouts"] == 0
+    for q in ("pos", "ori", "vel", "angVel", "force", "torque"):
+        assert np.array_equal(x["nodes"][q], y["nodes"][q]), q
+    for q in ("idA", "idB", "cellDist", "isReal", "forceLocal", "contPt", "trsf", "uN"):
+        assert np.array_equal(x["contacts"][q], y["contacts"][q], equal_nan=True), q
+    assert x["contacts"]["isReal"].sum() > (20 if kind == "drum" else 100)
+    assert x["stats"] == y["stats"] and x["time"] == y["time"]
+    (ex, cx), (ey, cy) = x["energy"], y["energy"]
+    assert np.array_equal(cx, cy)
+    assert np.allclose(ex, ey, rtol=1e-11, atol=1e-11 * np.abs(ey).max())
+
+
+def test_engine_by_engine_relayouts_like_wdm_step(hip_ctx):
+    ctx = hip_ctx
+    a, p = _scene("periodic", ctx)
+    cell = runner.cell_from_meta(a)
+    ctx.set_relayout(8, 0)
+    try:
+        eng = runner.HipRun(a, p, cell, ctx=ctx)
+        whole = runner.HipRun(a, p, cell, ctx=ctx)
+        for s in range(150):
+            sid = eng.sid
+            ctx.step_begin(sid)
+            ctx.leapfrog_run(sid)
+            ctx.collider_run(sid)
+            ctx.contact_loop_run(sid)
+            ctx.step_end(sid)
+        whole.step(150)
+        n1, n2 = eng.nodes(), whole.nodes()
+        for q in ("pos", "vel", "angVel", "force", "torque"):
+            assert np.array_equal(n1[q], n2[q]), q
+        e1, e2 = eng.energy(), whole.energy()
+        assert np.array_equal(e1[0], e2[0]) and np.array_equal(e1[1], e2[1])
+        assert ctx.relayout(eng.sid)[2] == ctx.relayout(whole.sid)[2] > 5
+        eng.close()
+        whole.close()
+    finally:
+        ctx.set_relayout(64)

@@ -110,6 +110,8 @@ def lib():
         "wdm_set_tuning": (C.c_int, [P, I32, I32, D]),
         "wdm_set_broadphase": (C.c_int, [P, I32]),
         "wdm_get_broadphase": (C.c_int, [P, pi]),
+        "wdm_set_relayout": (C.c_int, [P, I32, I64]),
+        "wdm_get_relayout": (C.c_int, [P, I32, pi, pl, pl]),
         "wdm_get_tuning": (C.c_int, [P, pi, pi, pd, pl, pl, pl]),
         "wdm_relax_overlaps": (C.c_int, [P, I32, I32]),
         "wdm_set_dyndt": (C.c_int, [P, I32, C.POINTER(WdmDynDt)]),
@@ -320,6 +322,18 @@ class Context:
         m = C.c_int32()
         self.check(self.L.wdm_get_broadphase(self.h, C.byref(m)))
         return m.value
+
+    def set_relayout(self, steps, min_contacts=1 << 20):
+        """Steps between relayouts of the ContactLoop work list of a scene with at least
+        min_contacts contacts, after its last full run or relayout (0: never); memory order
+        only, results do not change."""
+        self.check(self.L.wdm_set_relayout(self.h, int(steps), int(min_contacts)))
+
+    def relayout(self, sid=-1):
+        """(steps between relayouts, min_contacts, relayouts of scene sid so far)"""
+        st, mn, n = C.c_int32(), C.c_int64(), C.c_int64()
+        self.check(self.L.wdm_get_relayout(self.h, int(sid), C.byref(st), C.byref(mn), C.byref(n)))
+        return st.value, mn.value, n.value
 
     def tuning(self):
         w, g = C.c_int32(), C.c_int32()

@@ -125,6 +125,12 @@ struct Ctx {
     // broadphase of the collider's full run: 0 the multi-level grid scan, 1 sort and sweep
     // (radix sort of the bound entries by minimum x, LDS-tiled sweep); the contact set is the same
     int broadphase = 0;
+    // steps between relayouts of a scene's work list (its real items moved to the front) after
+    // its last full run or relayout, 0: never; only lists of at least relayoutMin contacts
+    // (smaller ones stay in cache, and scenes of a batch would cut each other's windows).
+    // Memory order only: results do not change
+    int relayout = 64;
+    int64_t relayoutMin = 1 << 20;
     hipError_t scratchErr = hipSuccess;  // looking up the device address of `scratch` at wdm_create
 };
 
@@ -263,6 +269,10 @@ struct Shared {
     Staged stDesc, stMapNode, stMapCon, stCon, stFR, stDyn;
     Staged stMapP, stMapB;
     Staged stRunA, stRunO, stRunN;  // collider full run: the tables of its phases
+    Staged stRl;                    // work-list relayout: its tables
+    DBuf<uint8_t> rlFlag;
+    DBuf<int64_t> rlScan;
+    DBuf<int32_t> rlMap;
     void* pin = nullptr;            // pinned readback memory
     size_t pinCap = 0;
     // pinned copy of a full run's bound extents and histograms, read by the host after the
@@ -342,7 +352,8 @@ struct Shared {
         if (hostWords) (void)hipHostFree(hostWords);
         hostWords = nullptr;
         hostWordsCap = 0;
-        for (Staged* s : {&stDesc, &stMapNode, &stMapCon, &stCon, &stFR, &stDyn, &stMapP, &stMapB, &stRunA, &stRunO, &stRunN})
+        release(c, rlFlag); release(c, rlScan); release(c, rlMap);
+        for (Staged* s : {&stDesc, &stMapNode, &stMapCon, &stCon, &stFR, &stDyn, &stMapP, &stMapB, &stRunA, &stRunO, &stRunN, &stRl})
             s->free(c);
         release(c, epartL); release(c, epartC); release(c, extPart); release(c, extOut); release(c, hist);
         release(c, ekey); release(c, eslot); release(c, ccnt); release(c, cbeg); release(c, ckeys2); release(c, cvals2); release(c, cbox); release(c, eb0); release(c, eb1); release(c, einfo);
@@ -399,6 +410,8 @@ struct Scene {
     double time = 0, nextDt = NAN;  // Scene.time, Scene.nextDt (core/Scene.hpp)
     wdm_dyndt dd{};                 // DynDt after ContactLoop (enabled 0: not in the engine list)
     bool contactsRun = false;       // a ContactLoop ran on the current contact list
+    int64_t layoutStep = 0;         // step of the last full run or work-list relayout
+    int64_t relayouts = 0;
     std::vector<int32_t> facetNodesH, nodeMasterH;  // nodes of facets; clump master of each node (-1)
     // ParticleContainer ids (pkg/dem/ParticleContainer.cpp:23-72): particles in use end before
     // parEnd; holes below it are reused first, in the order they were freed
@@ -1447,6 +1460,7 @@ struct InsertionSortCollider {
             S.cur = 1 - S.cur;
             S.nc = F[t].nNew;
             S.nFullRuns++;
+            S.layoutStep = S.step + 1;  // the work list is laid out for the steps from the next one
             S.needFull = false;
         }
         // CSR (contact -> particle): entries counted per particle row, then bucket-sorted by
@@ -1474,6 +1488,70 @@ struct InsertionSortCollider {
             hipLaunchKernelGGL(k_csr_offsets, dim3((unsigned)bp1[nT]), dim3(WDM_BLOCK), 0, st, B.D, FRd, m1, (const int64_t*)X.csrScan.p);
             checkLaunch();
         }
+    }
+
+    // work-list relayout of the scenes L (their lists into the other buffer, which becomes the
+    // current one): real items first, component-major state rows, CSR entries remapped
+    template <typename T>
+    static void relayout(Batch& B, const std::vector<int>& L) {
+        Ctx* c = B.c;
+        Shared& X = *c->sh;
+        hipStream_t st = c->stream;
+        const int n = (int)L.size();
+        if (n == 0) return;
+        std::vector<ConDesc<T>> C(n);
+        std::memset(C.data(), 0, C.size() * sizeof(ConDesc<T>));
+        std::vector<RlScene> R(n);
+        for (int t = 0; t < n; t++) {
+            Scene& S = *B.ss[L[t]];
+            ConBuf<T>* cb = bufs<T>(S);
+            cb[1 - S.cur].reserve(c, S.nc);
+            C[t].cv = cb[S.cur].view();
+            C[t].cvNew = cb[1 - S.cur].view();
+            C[t].nc = S.nc;
+            C[t].realMask = gpc(S.realMask.p);
+            C[t].desc = L[t];
+            R[t].entV = gpc(S.entV.p);
+            R[t].csrOff = gpc(S.csrOff.p);
+            R[t].np = S.np;
+        }
+        const std::vector<int64_t> bp = blockPrefix(n, [&](int t) { return C[t].nc; });
+        const std::vector<int64_t> bpE = blockPrefix(n, [&](int t) { return 2 * C[t].nc; });
+        const int64_t nf = bp[n] * WDM_BLOCK;
+        ensure(c, X.rlFlag, (size_t)std::max<int64_t>(nf, 1));
+        ensure(c, X.rlScan, (size_t)nf + 1);
+        ensure(c, X.rlMap, (size_t)std::max<int64_t>(nf, 1));
+        Upload u;
+        const size_t oC = u.add(C), oR = u.add(R), oB = u.add(bp), oE = u.add(bpE);
+        u.send(c, X.stRl);
+        const ConDesc<T>* Cd = u.at<ConDesc<T>>(oC);
+        const Map m = u.map(oB, n), mE = u.map(oE, n);
+        if (bp[n] > 0) {
+            KTimer tm(c, "relayout");
+            hipLaunchKernelGGL(k_rl_flags<T>, dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, st, Cd, m, X.rlFlag.p);
+            checkLaunch();
+            scanExclusive(c, (const uint8_t*)X.rlFlag.p, X.rlScan.p, nf, ScanNearSplit());
+            hipLaunchKernelGGL(k_rl_contacts<T>, dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, st, Cd, m);
+            hipLaunchKernelGGL(k_rl_move<T>, dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, st, Cd, m, (const int64_t*)X.rlScan.p, X.rlMap.p);
+            hipLaunchKernelGGL(k_rl_finish<T>, dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, st, Cd, m, (const int64_t*)X.rlScan.p);
+            checkLaunch();
+            if (bpE[n] > 0)
+                hipLaunchKernelGGL(k_rl_csr, dim3((unsigned)bpE[n]), dim3(WDM_BLOCK), 0, st, u.at<RlScene>(oR), mE, m,
+                                   (const int32_t*)X.rlMap.p);
+            checkLaunch();
+        }
+        for (int t = 0; t < n; t++) {
+            Scene& S = *B.ss[L[t]];
+            S.cur = 1 - S.cur;
+            S.layoutStep = S.step;
+            S.relayouts++;
+        }
+    }
+    static void relayoutList(Batch& B) {
+        std::vector<int> L64, L32;
+        for (int i = 0; i < B.n(); i++) (B.ss[i]->mixed() ? L32 : L64).push_back(i);
+        relayout<double>(B, L64);
+        relayout<float>(B, L32);
     }
 
     // scenes that need a full run this step: after an upload, with verletDist 0, or when
@@ -1634,6 +1712,26 @@ struct ContactLoop {
     }
 };
 
+// work-list relayout (memory order only), due at the start of a step c->relayout steps after
+// the scene's last full run or relayout, when the current list has a ContactLoop's state
+bool relayoutable(const Ctx* c, const Scene& S) {
+    return S.contactsRun && !S.needFull && !S.failed && S.nc > 0 && S.nc >= c->relayoutMin;
+}
+bool relayoutDue(const Ctx* c, const Scene& S) { return c->relayout > 0 && relayoutable(c, S) && S.step - S.layoutStep >= c->relayout; }
+void relayoutDueScenes(Ctx* c, const std::vector<Scene*>& ss) {
+    std::vector<Scene*> due;
+    for (Scene* S : ss)
+        if (relayoutDue(c, *S)) due.push_back(S);
+    if (due.empty()) return;
+    Batch B(c, due);
+    InsertionSortCollider::relayoutList(B);
+}
+// steps a window may take before a scene's next relayout is due (windows end there, so that
+// every way of stepping a scene relayouts it at the same steps)
+int64_t stepsToRelayout(const Ctx* c, const Scene& S) {
+    if (c->relayout <= 0 || !relayoutable(c, S)) return INT64_MAX;
+    return std::max<int64_t>(1, c->relayout - (S.step - S.layoutStep));
+}
 void checkReady(Scene& S) {
     if (S.failed)
         throw WdmError(WDM_E_STATE, "scene " + std::to_string(S.id) + " was stopped by a device error (" + S.failText + "): free it");
@@ -2696,6 +2794,8 @@ int wdm_set_params(wdm_ctx* c, int32_t sid, const wdm_params* p) {
 int wdm_step_begin(wdm_ctx* c, int32_t sid) {
     return guard(c, [&] {
         Scene& S = getScene(c, sid);
+        checkReady(S);
+        relayoutDueScenes(c, {&S});
         stepBeginCell(S);
         if (S.P.trackEnergy)  // EnergyTracker::resetResettables: kinetic and elast
             HIPCHK(hipMemsetAsync(S.E.p + WDM_E_KINETIC, 0, sizeof(double) * 2, c->stream));
@@ -2789,6 +2889,8 @@ int wdm_step(wdm_ctx* c, const int32_t* scenes, int32_t ns, int32_t nsteps) {
                     act.push_back(ss[i]); la.push_back(left[i]); ai.push_back(i); most = std::max(most, left[i]);
                 }
             if (act.empty()) break;
+            relayoutDueScenes(c, act);
+            for (Scene* S : act) most = std::min(most, std::max<int64_t>(1, stepsToRelayout(c, *S)));
             const int K = (int)std::min<int64_t>(c->window, most);
             bool win = K > 1;
             for (Scene* S : act) win = win && windowable(*S, K);
@@ -3378,6 +3480,20 @@ int wdm_set_broadphase(wdm_ctx* c, int32_t mode) {
 int wdm_get_broadphase(wdm_ctx* c, int32_t* mode) {
     return guard(c, [&] {
         if (mode) *mode = c->broadphase;
+    });
+}
+int wdm_set_relayout(wdm_ctx* c, int32_t steps, int64_t min_contacts) {
+    return guard(c, [&] {
+        if (steps < 0 || min_contacts < 0) throw WdmError(WDM_E_ARG, "relayout steps and min_contacts must be >= 0 (steps 0: never)");
+        c->relayout = steps;
+        c->relayoutMin = min_contacts;
+    });
+}
+int wdm_get_relayout(wdm_ctx* c, int32_t sid, int32_t* steps, int64_t* min_contacts, int64_t* relayouts) {
+    return guard(c, [&] {
+        if (steps) *steps = c->relayout;
+        if (min_contacts) *min_contacts = c->relayoutMin;
+        if (relayouts) *relayouts = sid >= 0 ? getScene(c, sid).relayouts : 0;
     });
 }
 int wdm_get_tuning(wdm_ctx* c, int32_t* window, int32_t* graphs, double* cellFactor, int64_t* windows, int64_t* captures,
