@@ -210,11 +210,11 @@ int wdm_get_tuning(wdm_ctx* ctx, int32_t* window, int32_t* graphs, double* cell_
    grid.  No counterpart in the reference. */
 int wdm_set_broadphase(wdm_ctx* ctx, int32_t mode);
 int wdm_get_broadphase(wdm_ctx* ctx, int32_t* mode);
-/* Work-list relayout between collider full runs: every `steps` steps (default 64; 0 never)
-   after a scene's last full run or relayout, the ContactLoop work list of a scene with at
-   least `min_contacts` contacts (default 2^20) is rewritten with the real contacts first and
-   their state rows contiguous again (contacts formed since the full run otherwise sit
-   scattered among the pairs that stay apart).  Memory order only: the contact list, every
+/* Work-list relayout between collider full runs: every `steps` steps (default 32; 0 never)
+   after a scene's last full run or relayout, in the ContactLoop work list of a scene with at
+   least `min_contacts` contacts (default 2^20) the contacts that became real since the full
+   run are moved right after the state rows of that run, with state rows of their own in the
+   same contiguous block (otherwise they sit scattered among the pairs that stay apart).  Memory order only: the contact list, every
    value and every result stay the same (energy sums, which follow the work-list order, as
    well: every way of stepping relayouts at the same steps).  The getter also returns the
    relayouts of scene sid so far (sid < 0: none).  No counterpart in the reference. */

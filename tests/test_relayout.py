@@ -1,6 +1,6 @@
-"""Work-list relayout between collider full runs (wdm_set_relayout): real contacts moved to the
-front of the ContactLoop's work list with their state rows, contact points and gather records,
-CSR entries remapped.  Memory order only: stepped with a relayout every 8 steps (2 in the
+"""Work-list relayout between collider full runs (wdm_set_relayout): contacts that became real
+after the full run moved right after its state rows in the ContactLoop's work list, with their
+state rows, contact points and gather records, CSR entries remapped.  Memory order only: stepped with a relayout every 8 steps (2 in the
 drum, whose full runs come every few steps), every scene
 kind gives the same nodes, contact list and contact state bit for bit as without relayouts
 (energies sum their terms in work-list order: equal to rounding), and stepping engine by engine
@@ -53,7 +53,7 @@ def test_relayout_changes_no_result(hip_ctx, kind):
         x = _run(ctx, a, p, steps, 2 if kind == "drum" else 8, dyndt=kind == "box")
         y = _run(ctx, a, p, steps, 0, dyndt=kind == "box")
     finally:
-        ctx.set_relayout(64)
+        ctx.set_relayout(32)
     assert x["relayouts"] > 10 and y["relayouts"] == 0
     for q in ("pos", "ori", "vel", "angVel", "force", "torque"):
         assert np.array_equal(x["nodes"][q], y["nodes"][q]), q
@@ -91,4 +91,4 @@ def test_engine_by_engine_relayouts_like_wdm_step(hip_ctx):
         eng.close()
         whole.close()
     finally:
-        ctx.set_relayout(64)
+        ctx.set_relayout(32)

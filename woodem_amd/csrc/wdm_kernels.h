@@ -2287,19 +2287,22 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_csr_offsets(const SceneDesc* D, c
 }
 
 // ---------------- work-list relayout between full runs ----------------
-// Contacts that become real after a full run sit among the items that stay apart, each on
-// its own lines (work-list record, contact-major state row, gather record), and the
-// ContactLoop slows down as they accumulate.  The relayout moves the real items to the front
-// (in their order) and the others after them (in their order), real state rows component-major
-// again, into the list's other buffer; the contacts, their order and every stored value stay
-// as they are, so no result changes.  Map m: per scene the blocks of its nc contacts; the
-// flat index of item j of scene t is m.bprefix[t] * WDM_BLOCK + j.
+// Contacts that become real after a full run sit among the items after the rows (the items
+// that were apart at the full run), each on its own lines (work-list record, contact-major
+// state row, gather record), and the ContactLoop slows down as they accumulate.  The relayout
+// moves the real ones among those items to the front of that part of the list (in their
+// order), right after the rows, with component-major state rows (the row block grows by
+// them), and the others after them (in their order); the rows stay in place.  The list's
+// other buffer, idle between full runs, stages the moved items.  Only memory order changes:
+// the contacts, their order and every stored value stay as they are.  Map m: per scene the
+// blocks of its nc contacts; the flat index of item j of scene t is m.bprefix[t] * WDM_BLOCK + j.
 struct RlScene {
     GP uint32_t* entV;        // CSR entries: work-list item | isB << 31
     const GP int64_t* csrOff; // csrOff[np]: entries in use
     int32_t np;
 };
-// 1 real item, 2 other item, 0 past the work list (ScanNearSplit counts them apart)
+// items after the rows: 1 real, 2 not; 0 for the rows and past the work list (ScanNearSplit
+// counts 1 and 2 apart)
 template <typename T>
 __global__ void __launch_bounds__(WDM_BLOCK) k_rl_flags(const ConDesc<T>* C, Map m, uint8_t* flag) {
     int64_t lb;
@@ -2307,87 +2310,113 @@ __global__ void __launch_bounds__(WDM_BLOCK) k_rl_flags(const ConDesc<T>* C, Map
     const ConView<T> cv = C[t].cv;
     const int64_t j = lb * WDM_BLOCK + threadIdx.x;
     uint8_t f = 0;
-    if (j < *cv.nItems) f = ((uint32_t)cv.irec[j].z & CI_REAL) ? 1 : 2;
+    if (j >= *cv.counter && j < *cv.nItems) f = ((uint32_t)cv.irec[j].z & CI_REAL) ? 1 : 2;
     flag[(int64_t)blockIdx.x * WDM_BLOCK + threadIdx.x] = f;
 }
-// the per-contact arrays into the other buffer
-template <typename T>
-__global__ void __launch_bounds__(WDM_BLOCK) k_rl_contacts(const ConDesc<T>* C, Map m) {
-    int64_t lb;
-    const int t = mapBlock(m, lb);
-    const ConView<T> a = C[t].cv, b = C[t].cvNew;
-    const int64_t c = lb * WDM_BLOCK + threadIdx.x;
-    if (c >= C[t].nc) return;
-    b.key[c] = a.key[c];
-    b.ids[c] = a.ids[c];
-    b.info[c] = a.info[c];
-    b.cd[c] = a.cd[c];
-    b.slot[c] = a.slot[c];
-    b.itm[c] = a.itm[c];
+// the new place of item j >= nR (the rows): nR + its rank among the real ones, or nR + the
+// number of real ones + its rank among the others
+__device__ __forceinline__ int64_t rlPlace(const Map& m, int t, const int64_t* scan, int64_t j, int64_t nR, bool real, int64_t& nNew) {
+    const int64_t g0 = m.bprefix[t] * WDM_BLOCK;
+    nNew = (scan[m.bprefix[t + 1] * WDM_BLOCK] - scan[g0]) & 0xffffffffll;
+    const int64_t sj = scan[g0 + j] - scan[g0];
+    return nR + (real ? (sj & 0xffffffffll) : nNew + (sj >> 32));
 }
-// item j to its new place (real ones: scan's low half; others: after them, scan's high half);
-// a real item takes its state row (item == row), contact point and gather record along
+// stage item j (>= nR) at k = its new place - nR in the other buffer b: work-list record, and
+// for a real one its state row (contact-major, as it is), contact point and gather record;
+// the contact's item and row point to the new place
 template <typename T>
-__global__ void __launch_bounds__(WDM_BLOCK) k_rl_move(const ConDesc<T>* C, Map m, const int64_t* scan, int32_t* map) {
+__global__ void __launch_bounds__(WDM_BLOCK) k_rl_out(const ConDesc<T>* C, Map m, const int64_t* scan, int32_t* map) {
     int64_t lb;
     const int t = mapBlock(m, lb);
     const ConView<T> a = C[t].cv, b = C[t].cvNew;
     const int64_t j = lb * WDM_BLOCK + threadIdx.x;
-    if (j >= *a.nItems) return;
-    const int64_t g0 = m.bprefix[t] * WDM_BLOCK;
-    const int64_t nReal = (scan[m.bprefix[t + 1] * WDM_BLOCK] - scan[g0]) & 0xffffffffll;
-    const int64_t sj = scan[g0 + j] - scan[g0];
+    const int64_t nR = *a.counter;
+    if (j < nR || j >= *a.nItems) return;
     const int4 rec = a.irec[j];
     const int c = a.iaux[j].y;
     const bool real = ((uint32_t)rec.z & CI_REAL) != 0;
-    const int64_t nj = real ? (sj & 0xffffffffll) : nReal + (sj >> 32);
-    b.irec[nj] = rec;
-    b.iaux[nj] = make_int2((int32_t)nj, c);
-    b.itm[c] = (int32_t)nj;
-    b.slot[c] = (int32_t)nj;
-    map[g0 + j] = (int32_t)nj;
+    int64_t nNew;
+    const int64_t nj = rlPlace(m, t, scan, j, nR, real, nNew), k = nj - nR;
+    b.irec[k] = rec;
+    b.iaux[k] = make_int2((int32_t)nj, c);
+    a.itm[c] = (int32_t)nj;
+    a.slot[c] = (int32_t)nj;
+    map[m.bprefix[t] * WDM_BLOCK + j] = (int32_t)nj;
     if (!real) return;
-    const int64_t nRo = *a.counter;
-    for (int k = 0; k < NSTATE; k++) *stAt(b, nj, k, nReal) = *stAt(a, j, k, nRo);
+    for (int q = 0; q < NSTATE; q++) b.st2[(size_t)k * NSTRIDE + q] = *stAt(a, j, q, nR);
     if constexpr (sizeof(T) == 8) {
-        b.cpt[nj] = a.cpt[j];
-        b.gfc[2 * nj] = a.gfc[2 * j];
-        b.gfc[2 * nj + 1] = a.gfc[2 * j + 1];
+        b.cpt[k] = a.cpt[j];
+        b.gfc[2 * k] = a.gfc[2 * j];
+        b.gfc[2 * k + 1] = a.gfc[2 * j + 1];
     } else {
-        mixedCpt(b.cpt)[nj] = mixedCpt(a.cpt)[j];
-        mixedRecA(b.gfc)[nj] = mixedRecA(a.gfc)[j];
-        mixedRecB(b.gfc, b.cap)[nj] = mixedRecB(a.gfc, a.cap)[j];
-        mixedRecC(b.gfc, b.cap)[nj] = mixedRecC(a.gfc, a.cap)[j];
+        mixedCpt(b.cpt)[k] = mixedCpt(a.cpt)[j];
+        mixedRecA(b.gfc)[k] = mixedRecA(a.gfc)[j];
+        mixedRecB(b.gfc, b.cap)[k] = mixedRecB(a.gfc, a.cap)[j];
+        mixedRecC(b.gfc, b.cap)[k] = mixedRecC(a.gfc, a.cap)[j];
     }
 }
-// counters and realness words of the new layout: the first nReal items are the real ones
+// the staged items back at nR + k; the real ones (k < nNew) into the grown component-major block
 template <typename T>
-__global__ void __launch_bounds__(WDM_BLOCK) k_rl_finish(const ConDesc<T>* C, Map m, const int64_t* scan) {
+__global__ void __launch_bounds__(WDM_BLOCK) k_rl_in(const ConDesc<T>* C, Map m, const int64_t* scan) {
     int64_t lb;
     const int t = mapBlock(m, lb);
     const ConView<T> a = C[t].cv, b = C[t].cvNew;
+    const int64_t k = lb * WDM_BLOCK + threadIdx.x;
+    const int64_t nR = *a.counter;
+    if (k >= *a.nItems - nR) return;
     const int64_t g0 = m.bprefix[t] * WDM_BLOCK;
-    const int64_t nReal = (scan[m.bprefix[t + 1] * WDM_BLOCK] - scan[g0]) & 0xffffffffll;
-    const int64_t w = lb * WDM_BLOCK + threadIdx.x;  // realness word: items [64 w, 64 w + 64)
-    const int64_t nWords = (m.bprefix[t + 1] - m.bprefix[t]) * (WDM_BLOCK / 64);
-    if (w < nWords) {
-        const int64_t r = nReal - 64 * w;
-        C[t].realMask[w] = r >= 64 ? ~0ull : (r <= 0 ? 0ull : ((1ull << r) - 1));
-    }
-    if (lb == 0 && threadIdx.x == 0) {
-        *b.counter = (int32_t)nReal;
-        *b.nItems = *a.nItems;
+    const int64_t nNew = (scan[m.bprefix[t + 1] * WDM_BLOCK] - scan[g0]) & 0xffffffffll;
+    const int64_t nj = nR + k;
+    a.irec[nj] = b.irec[k];
+    a.iaux[nj] = b.iaux[k];
+    if (k >= nNew) return;
+    for (int q = 0; q < NSTATE; q++) *stAt(a, nj, q, nR + nNew) = b.st2[(size_t)k * NSTRIDE + q];
+    if constexpr (sizeof(T) == 8) {
+        a.cpt[nj] = b.cpt[k];
+        a.gfc[2 * nj] = b.gfc[2 * k];
+        a.gfc[2 * nj + 1] = b.gfc[2 * k + 1];
+    } else {
+        mixedCpt(a.cpt)[nj] = mixedCpt(b.cpt)[k];
+        mixedRecA(a.gfc)[nj] = mixedRecA(b.gfc)[k];
+        mixedRecB(a.gfc, a.cap)[nj] = mixedRecB(b.gfc, b.cap)[k];
+        mixedRecC(a.gfc, a.cap)[nj] = mixedRecC(b.gfc, b.cap)[k];
     }
 }
-// CSR entries through the item map (their order, by the other particle's id, stays)
-__global__ void __launch_bounds__(WDM_BLOCK) k_rl_csr(const RlScene* R, Map mE, Map m, const int32_t* map) {
+// realness words: the rows' bits stay, the moved real items' bits set, the others' cleared
+template <typename T>
+__global__ void __launch_bounds__(WDM_BLOCK) k_rl_mask(const ConDesc<T>* C, Map m, const int64_t* scan) {
+    int64_t lb;
+    const int t = mapBlock(m, lb);
+    const ConView<T> a = C[t].cv;
+    const int64_t w = lb * WDM_BLOCK + threadIdx.x;  // items [64 w, 64 w + 64)
+    if (w >= (m.bprefix[t + 1] - m.bprefix[t]) * (WDM_BLOCK / 64)) return;
+    const int64_t g0 = m.bprefix[t] * WDM_BLOCK;
+    const int64_t nR = *a.counter;
+    const int64_t nNew = (scan[m.bprefix[t + 1] * WDM_BLOCK] - scan[g0]) & 0xffffffffll;
+    auto below = [](int64_t n) { return n >= 64 ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1)); };
+    const uint64_t keep = below(nR - 64 * w), upto = below(nR + nNew - 64 * w);
+    C[t].realMask[w] = (C[t].realMask[w] & keep) | (upto & ~keep);
+}
+// CSR entries of moved items through the item map (their order, by the other particle's id, stays)
+template <typename T>
+__global__ void __launch_bounds__(WDM_BLOCK) k_rl_csr(const ConDesc<T>* C, const RlScene* R, Map mE, Map m, const int32_t* map) {
     int64_t lb;
     const int t = mapBlock(mE, lb);
     const RlScene& r = R[t];
     const int64_t e = lb * WDM_BLOCK + threadIdx.x;
     if (e >= r.csrOff[r.np]) return;
-    const uint32_t v = r.entV[e];
-    r.entV[e] = (uint32_t)map[m.bprefix[t] * WDM_BLOCK + (v & 0x7fffffffu)] | (v & 0x80000000u);
+    const uint32_t v = r.entV[e], item = v & 0x7fffffffu;
+    if ((int64_t)item < *C[t].cv.counter) return;
+    r.entV[e] = (uint32_t)map[m.bprefix[t] * WDM_BLOCK + item] | (v & 0x80000000u);
+}
+// the row block grown by the moved real items
+template <typename T>
+__global__ void k_rl_counter(const ConDesc<T>* C, Map m, const int64_t* scan, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t g0 = m.bprefix[t] * WDM_BLOCK;
+    const int64_t nNew = (scan[m.bprefix[t + 1] * WDM_BLOCK] - scan[g0]) & 0xffffffffll;
+    *C[t].cv.counter = (int32_t)(*C[t].cv.counter + nNew);
 }
 
 // ---------------- ContactLoop::run, pkg/dem/ContactLoop.cpp:93-259 ----------------

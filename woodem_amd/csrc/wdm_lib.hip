@@ -129,7 +129,7 @@ struct Ctx {
     // its last full run or relayout, 0: never; only lists of at least relayoutMin contacts
     // (smaller ones stay in cache, and scenes of a batch would cut each other's windows).
     // Memory order only: results do not change
-    int relayout = 64;
+    int relayout = 32;
     int64_t relayoutMin = 1 << 20;
     hipError_t scratchErr = hipSuccess;  // looking up the device address of `scratch` at wdm_create
 };
@@ -1490,8 +1490,9 @@ struct InsertionSortCollider {
         }
     }
 
-    // work-list relayout of the scenes L (their lists into the other buffer, which becomes the
-    // current one): real items first, component-major state rows, CSR entries remapped
+    // work-list relayout of the scenes L: the real items after the rows moved right after
+    // them (component-major state rows, the row block grows), the others after those, CSR
+    // entries remapped; the list's other buffer stages the moved items
     template <typename T>
     static void relayout(Batch& B, const std::vector<int>& L) {
         Ctx* c = B.c;
@@ -1528,21 +1529,22 @@ struct InsertionSortCollider {
         const Map m = u.map(oB, n), mE = u.map(oE, n);
         if (bp[n] > 0) {
             KTimer tm(c, "relayout");
+            const int64_t* scan = (const int64_t*)X.rlScan.p;
             hipLaunchKernelGGL(k_rl_flags<T>, dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, st, Cd, m, X.rlFlag.p);
             checkLaunch();
             scanExclusive(c, (const uint8_t*)X.rlFlag.p, X.rlScan.p, nf, ScanNearSplit());
-            hipLaunchKernelGGL(k_rl_contacts<T>, dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, st, Cd, m);
-            hipLaunchKernelGGL(k_rl_move<T>, dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, st, Cd, m, (const int64_t*)X.rlScan.p, X.rlMap.p);
-            hipLaunchKernelGGL(k_rl_finish<T>, dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, st, Cd, m, (const int64_t*)X.rlScan.p);
+            hipLaunchKernelGGL(k_rl_out<T>, dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, st, Cd, m, scan, X.rlMap.p);
+            hipLaunchKernelGGL(k_rl_in<T>, dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, st, Cd, m, scan);
+            hipLaunchKernelGGL(k_rl_mask<T>, dim3((unsigned)bp[n]), dim3(WDM_BLOCK), 0, st, Cd, m, scan);
             checkLaunch();
             if (bpE[n] > 0)
-                hipLaunchKernelGGL(k_rl_csr, dim3((unsigned)bpE[n]), dim3(WDM_BLOCK), 0, st, u.at<RlScene>(oR), mE, m,
+                hipLaunchKernelGGL(k_rl_csr<T>, dim3((unsigned)bpE[n]), dim3(WDM_BLOCK), 0, st, Cd, u.at<RlScene>(oR), mE, m,
                                    (const int32_t*)X.rlMap.p);
+            hipLaunchKernelGGL(k_rl_counter<T>, dim3((n + 63) / 64), dim3(64), 0, st, Cd, m, scan, n);
             checkLaunch();
         }
         for (int t = 0; t < n; t++) {
             Scene& S = *B.ss[L[t]];
-            S.cur = 1 - S.cur;
             S.layoutStep = S.step;
             S.relayouts++;
         }
