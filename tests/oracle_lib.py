@@ -13,9 +13,12 @@ Parity-pinning status of the oracle's functions (details in DESIGN.md):
     cell, incident velocity under PBC, energy balance, DynDt (single-contact stiffness
     closed forms, maxRelInc cap, dryRun, PeriodicEngine activation), checkpoint restore
     (orc_set_contacts: continues like the uninterrupted run; planted mistakes diverge);
-  parity unpinned (no reference vector exists; checked only by invariants):
-    multi-step L6Geom trsf evolution with renormalisation, IdealElPl Ft history
-    over many steps, summation order of forces and energies.
+  multi-step contact history by closed forms: L6Geom trsf of a spinning wall contact
+    over 20,000 steps through the renormalisation, and the IdealElPl shear history of a
+    sliding sphere's transition to rolling (tests/test_oracle.py);
+  parity unpinned (no reference vector exists and no closed form fixes it): the summation
+    order of forces and energies (Woo's OpenMP order is not deterministic; the oracle and
+    the kernels sum in (min, max) key order, Woo's deterministic mode).
 """
 import ctypes as C
 import os
