@@ -4,11 +4,13 @@ state rows, contact points and gather records, CSR entries remapped.  Memory ord
 drum, whose full runs come every few steps), every scene
 kind gives the same nodes, contact list and contact state bit for bit as without relayouts
 (energies sum their terms in work-list order: equal to rounding), and stepping engine by engine
-relayouts at the same steps as wdm_step, energies included."""
+relayouts at the same steps as wdm_step, energies included; with particles removed and
+inserted between relayouts the run still equals the oracle bit for bit."""
 import numpy as np
 import pytest
 
-from woodem_amd import runner, scenes
+import oracle_lib as OL
+from woodem_amd import runner, scenes, soa
 from woodem_amd._native import PREC_MIXED, make_dyndt
 
 pytestmark = pytest.mark.gpu
@@ -90,5 +92,44 @@ def test_engine_by_engine_relayouts_like_wdm_step(hip_ctx):
         assert ctx.relayout(eng.sid)[2] == ctx.relayout(whole.sid)[2] > 5
         eng.close()
         whole.close()
+    finally:
+        ctx.set_relayout(32)
+
+
+def test_relayouts_with_removal_and_insertion_equal_the_oracle(hip_ctx):
+    """Relayouts every 2 steps around DemField::removeParticle / particle insertion
+    (pkg/dem/Particle.cpp:445-493): positions and contact sets equal the oracle's bit for bit."""
+    ctx = hip_ctx
+    r, rho = 5e-3, 3000.
+    ctx.set_relayout(2, 0)
+    try:
+        a = soa.with_spare(scenes.box_lattice(nx=8, ny=6, nz=5), 40)
+        p = runner.params_from_meta(a, trackEnergy=True)
+        cell = runner.cell_from_meta(a)
+        g = runner.HipRun(a, p, cell, ctx=ctx)
+        o = OL.OracleRun(a, p, cell)
+        g.step(1500)
+        o.step(1500)
+        z = o.nodes()["pos"][:, 2]
+        sph = np.nonzero(a.shape == soa.SHAPE_SPHERE)[0]
+        rm = sph[np.argsort(-z[sph], kind="stable")[:25]]
+        ctx.remove_particles(g.sid, rm)
+        o.remove_particles(rm)
+        g.step(200)
+        o.step(200)
+        m, inertia = soa.sphere_mass_inertia(r, rho)
+        newpos = np.array([[.05 + .012 * k, .12, .2] for k in range(30)])
+        assert np.array_equal(ctx.insert_spheres(g.sid, newpos, r, m, inertia, 0, soa.DEFAULT_MOVABLE_MASK),
+                              o.insert_spheres(newpos, r, m, inertia, 0, soa.DEFAULT_MOVABLE_MASK))
+        for n in (1, 50, 600):
+            g.step(n)
+            o.step(n)
+            ng, no = g.nodes(), o.nodes()
+            for q in ("pos", "vel", "angVel", "force", "torque"):
+                assert np.array_equal(ng[q], no[q]), (n, q)
+            assert np.array_equal(OL.pair_keys(g.contacts()), OL.pair_keys(o.contacts()))
+        assert ctx.relayout(g.sid)[2] > 100
+        g.close()
+        o.close()
     finally:
         ctx.set_relayout(32)
