@@ -8,6 +8,8 @@ size; the workload is compared with the oracle at 100,000 spheres in tests/test_
   periodic cell, every contact with spatialOverlapPeri's periods as cellDist
   (pkg/dem/InsertionSortCollider.cpp:903-957; tests/bound_pairs.py, pinned against the oracle's
   collider in tests/test_bound_pairs.py);
+* with the FP64 and the mixed-precision contact law (the bounds are FP64 in both), and with
+  the work list relaid out between the full runs;
 * after each of those steps the contact forces cancel: without gravity the node forces sum to
   zero up to rounding (ContactLoop applies each contact's force to both particles,
   pkg/dem/ContactLoop.cpp:188-190, with sign +1 on A and -1 on B, pkg/dem/Contact.cpp:24-35)."""
@@ -21,10 +23,12 @@ from woodem_amd import runner, scenes
 pytestmark = pytest.mark.gpu
 
 
-def test_full_size_contact_sets_and_force_balance(hip_ctx):
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_full_size_contact_sets_and_force_balance(hip_ctx, prec):
+    from woodem_amd._native import PREC_FP64, PREC_MIXED
     ctx = hip_ctx
     b = scenes.relax_and_order(scenes.periodic_hertz(n=1_000_000), ctx, iters=500)
-    p = runner.params_from_meta(b)
+    p = runner.params_from_meta(b, precision=PREC_MIXED if prec == "mixed" else PREC_FP64)
     run = runner.HipRun(b, p, runner.cell_from_meta(b), ctx=ctx)
     sid = run.sid
     r = run.a.radius
@@ -58,4 +62,6 @@ def test_full_size_contact_sets_and_force_balance(hip_ctx):
         if len(checked) == 2:
             break
     assert checked[0] == 1 and len(checked) == 2, checked
+    # the work list was relaid out between the two full runs (every 32 steps by default)
+    assert checked[1] - checked[0] < 64 or ctx.relayout(sid)[2] >= 1
     run.close()
