@@ -133,3 +133,21 @@ def test_relayouts_with_removal_and_insertion_equal_the_oracle(hip_ctx):
         o.close()
     finally:
         ctx.set_relayout(32)
+
+
+def test_small_lists_keep_their_layout_and_windows(hip_ctx):
+    """With the default settings (relayouts every 32 steps for lists of 2^20 contacts or more)
+    a small scene is never relaid out and its windows are not cut short: 160 steps take 10
+    windows of 16."""
+    ctx = hip_ctx
+    ctx.set_relayout(32)
+    a, p = _scene("periodic", ctx)
+    r = runner.HipRun(a, p, runner.cell_from_meta(a), ctx=ctx)
+    r.step(40)  # the first full runs of the relaxed packing
+    full0 = r.stats()["nFullRuns"]
+    w0 = ctx.tuning()["windows"]
+    r.step(160)
+    if r.stats()["nFullRuns"] == full0:  # a Verlet trigger ends a window early
+        assert ctx.tuning()["windows"] - w0 == 10
+    assert ctx.relayout(r.sid)[1] == 1 << 20 and ctx.relayout(r.sid)[2] == 0
+    r.close()
